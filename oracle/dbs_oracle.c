@@ -1,0 +1,1264 @@
+/*
+ * dbs_oracle.c -- CPU restatement of DBSR's B-spline Slater-integral path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP library and the CPU
+ * baseline timed by bench.py.  The product never links or calls this file.
+ *
+ * Restates, loop for loop, the reference algorithms in
+ *   DBSR_HF/dbsr_lib_dbs.f90   (grid, Gauss tables, moments, diagonal cells,
+ *                               mrk_ and msk_ arrays, density, convol, SUM_AmB,
+ *                               quadr, rk, UPDATE_HS)
+ *   DBSR_HF/dbsr_lib_zcom.f90  (gauleg)
+ *   DBSR_MAT3/MOD_c_data.f90   (blocked sorted coefficient lists)
+ *   DBSR_CI3/rk_evaluate.f90   (list evaluation into HM/DM)
+ *   DBSR_MAT3/R_data.f90, updates.f90, idef_type.f90 (channel blocks)
+ * Loop orders follow the reference so that summation orders agree; arrays
+ * are column-major with the reference's dimensions, indices 0-based here.
+ */
+#include "dbs_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define MAXK 16  /* largest B-spline order handled by the fixed-size scratch */
+
+/* x**n for integer n, by binary powering (the lowering of Fortran real**int) */
+static double ipow(double a, int b) {
+  int recip = b < 0;
+  unsigned int n = recip ? (unsigned int)(-b) : (unsigned int)b;
+  double r = 1.0;
+  for (;;) {
+    if (n & 1u) r *= a;
+    n >>= 1;
+    if (n == 0) break;
+    a *= a;
+  }
+  return recip ? 1.0 / r : r;
+}
+
+/* ===================================================================== */
+/* gauleg: DBSR_HF/dbsr_lib_zcom.f90:1990-2039                           */
+/* ===================================================================== */
+void orc_gauleg(double x1, double x2, double *x, double *w, int n) {
+  const double EPS = 1.0e-14;
+  int m = (n + 1) / 2;
+  double xm = 0.5 * (x2 + x1), xl = 0.5 * (x2 - x1);
+  for (int i = 1; i <= m; ++i) {
+    double z = cos(3.141592654 * (i - 0.25) / (n + 0.5));
+    double z1, pp, p1, p2, p3;
+    do {
+      p1 = 1.0;
+      p2 = 0.0;
+      for (int j = 1; j <= n; ++j) {
+        p3 = p2;
+        p2 = p1;
+        p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / j;
+      }
+      pp = n * (z * p1 - p2) / (z * z - 1.0);
+      z1 = z;
+      z = z1 - p1 / pp;
+    } while (fabs(z - z1) > EPS);
+    x[i - 1] = xm - xl * z;
+    x[n - i] = xm + xl * z;
+    w[i - 1] = 2.0 * xl / ((1.0 - z * z) * pp * pp);
+    w[n - i] = w[i - 1];
+  }
+}
+
+/* ===================================================================== */
+/* zbsplvd / zbsplvb: DBSR_HF/dbsr_lib_dbs.f90:4197-4468                 */
+/* de Boor's bsplvd, one point at a time.  t is 1-based in the reference; */
+/* here T(i) = t[i-1].  dbiatx is (nv,ks,ks) column-major: entry          */
+/* (ip,j,m) = value of the (m-1)st derivative of B-spline left-ks+j at    */
+/* point ip, left = kg+ip-1 (1-based).                                    */
+/* ===================================================================== */
+#define T1(i) t[(i) - 1]
+void orc_zbsplvd(int ns, int ks, int nv, const double *t, int kg, int ni,
+                 const double *x, int nderiv, double *dbiatx) {
+  (void)ns;
+#define DB(ip, j, m) dbiatx[(ip) + (size_t)nv * (((j) - 1) + (size_t)ks * ((m) - 1))]
+  for (int ip = 0; ip < ni; ++ip) {
+    const double xx = x[ip];
+    const int left = kg + ip;  /* 1-based knot index of the interval start */
+    double deltar[MAXK + 1], deltal[MAXK + 1];
+    int jstate = 1;
+    /* bsplvb(jhigh, index): raises the order held in DB(ip,1..j,1) */
+#define BSPLVB(JHIGH, INDEX)                                                   \
+    do {                                                                       \
+      if ((INDEX) == 1) {                                                      \
+        jstate = 1;                                                            \
+        DB(ip, 1, 1) = 1.0;                                                    \
+        if (jstate >= (JHIGH)) break;                                          \
+      }                                                                        \
+      for (;;) {                                                               \
+        int jp1 = jstate + 1;                                                  \
+        double saved = 0.0;                                                    \
+        deltar[jstate] = T1(left + jstate) - xx;                               \
+        deltal[jstate] = xx - T1(left + 1 - jstate);                           \
+        for (int m_ = 1; m_ <= jstate; ++m_) {                                 \
+          double term = DB(ip, m_, 1) / (deltar[m_] + deltal[jp1 - m_]);       \
+          DB(ip, m_, 1) = saved + deltar[m_] * term;                           \
+          saved = deltal[jp1 - m_] * term;                                     \
+        }                                                                      \
+        DB(ip, jp1, 1) = saved;                                                \
+        jstate = jp1;                                                          \
+        if (jstate >= (JHIGH)) break;                                          \
+      }                                                                        \
+    } while (0)
+
+    int mhigh = nderiv < ks ? nderiv : ks;
+    if (mhigh < 1) mhigh = 1;
+    int kp1 = ks + 1;
+    int jhigh = kp1 - mhigh;
+    BSPLVB(jhigh, 1);
+    if (mhigh == 1) continue;
+
+    int ideriv = mhigh;
+    for (int m = 2; m <= mhigh; ++m) {
+      int jpimid = 1;
+      for (int j = ideriv; j <= ks; ++j) {
+        DB(ip, j, ideriv) = DB(ip, jpimid, 1);
+        jpimid++;
+      }
+      ideriv--;
+      jhigh = kp1 - ideriv;
+      BSPLVB(jhigh, 2);
+    }
+    /* w31(i,j): B-coefficients of the derivatives */
+    double w31[MAXK + 1][MAXK + 1];
+    int jlow = 1;
+    for (int i = 1; i <= ks; ++i) {
+      for (int j = jlow; j <= ks; ++j) w31[j][i] = 0.0;
+      jlow = i;
+      w31[i][i] = 1.0;
+    }
+    for (int m = 2; m <= mhigh; ++m) {
+      int kpimm = kp1 - m;
+      double fkpimm = kpimm;
+      int i = ks, il = 0;
+      for (int ld = 1; ld <= kpimm; ++ld) {
+        int n = left - il;
+        double w1 = fkpimm / (T1(n + kpimm) - T1(n));
+        for (int j = 1; j <= i; ++j) w31[i][j] = (w31[i][j] - w31[i - 1][j]) * w1;
+        il++;
+        i--;
+      }
+      for (int ii = 1; ii <= ks; ++ii) {
+        double w2 = 0.0;
+        int jl = ii > m ? ii : m;
+        for (int j = jl; j <= ks; ++j) w2 += w31[j][ii] * DB(ip, j, m);
+        DB(ip, ii, m) = w2;
+      }
+    }
+#undef BSPLVB
+  }
+#undef DB
+}
+
+/* ===================================================================== */
+/* mkgrid_01 (point nucleus): DBSR_HF/dbsr_lib_dbs.f90:724-783            */
+/* returns ns; t must hold ns+ks entries (tcap checked)                   */
+/* ===================================================================== */
+int orc_mkgrid_01(int ks, int ml, double hi, double he, double hmax, double rmax,
+                  double z, double *t, int tcap) {
+  double hs = hi / z;
+  double ti = hs * ml, tj, ht = 1.0 + he;
+  int ns = ks + ml, ne = ns;
+  for (;;) {
+    tj = ti * ht;
+    if (tj - ti > hmax) break;
+    ns = ns + 1; ti = tj; ne = ns;
+    if (ti > rmax) { ns = ns - 1; break; }
+  }
+  if (ti < rmax) {
+    for (;;) {
+      ti = ti + hmax;
+      if (ti >= rmax) break;
+      ns = ns + 1;
+    }
+  }
+  if (t == NULL || tcap < ns + ks) return ns;
+  for (int i = 1; i <= ns + ks; ++i) T1(i) = 0.0;
+  for (int i = ks + 1; i <= ks + ml; ++i) T1(i) = hs * (i - ks);
+  for (int i = ks + ml + 1; i <= ne; ++i) T1(i) = T1(i - 1) * ht;
+  if (T1(ne) < rmax)
+    for (int i = ne + 1; i <= ns + 1; ++i) T1(i) = T1(i - 1) + hmax;
+  for (int i = ns + 2; i <= ns + ks; ++i) T1(i) = T1(ns + 1);
+  return ns;
+}
+
+/* mkgrid_02 (point nucleus): DBSR_HF/dbsr_lib_dbs.f90:787-823 */
+void orc_mkgrid_02(int ns, int ks, double hi, double rmax, double z, double *t) {
+  double ti = hi / z, tmax = rmax;
+  double he = exp((log(tmax) - log(ti)) / (ns - ks));
+  for (int i = 1; i <= ks; ++i) T1(i) = 0.0;
+  T1(ks + 1) = ti;
+  for (int i = ks + 2; i <= ns + 1; ++i) T1(i) = T1(i - 1) * he;
+  for (int i = ns + 1; i <= ns + ks; ++i) T1(i) = tmax;
+}
+#undef T1
+
+/* ===================================================================== */
+/* Gauss tables: alloc_DBS_gauss, Def_pbsp, Def_qbsp                      */
+/* DBSR_HF/dbsr_lib_dbs.f90:911-1074, 565-589                             */
+/* ===================================================================== */
+static void def_xbsp(const orc_grid *g, int kx, int nsx, const double *tx, double *xbsp) {
+  const int nv = g->nv, ks = g->ks;
+  const int nw = nv + 1;
+  double *dbi = (double *)calloc((size_t)nv * kx * kx, sizeof(double));
+  double *xm = (double *)malloc(sizeof(double) * nv);
+#define XB(iv, m, j) xbsp[(iv) + (size_t)nw * ((m) + (size_t)ks * (j))]
+  for (size_t q = 0; q < (size_t)nw * ks * kx; ++q) xbsp[q] = 0.0;
+  for (int m = 0; m < ks; ++m) {
+    for (int iv = 0; iv < nv; ++iv) xm[iv] = g->gr[iv + (size_t)nv * m];
+    orc_zbsplvd(nsx, kx, nv, tx, kx, nv, xm, 1, dbi);
+    for (int j = 0; j < kx; ++j)
+      for (int iv = 0; iv < nv; ++iv) XB(iv, m, j) = dbi[iv + (size_t)nv * j];
+  }
+  /* values at the last knot, row nv+1 (1-based) */
+  double xe = tx[nsx];  /* tx(nsx+1) */
+  orc_zbsplvd(nsx, kx, nv, tx, nsx, 1, &xe, 1, dbi);
+  for (int j = 0; j < kx; ++j) XB(nv, 0, j) = dbi[0 + (size_t)nv * j];
+#undef XB
+  free(dbi);
+  free(xm);
+}
+
+int orc_grid_init(orc_grid *g, int ns, int ks, int ksp, int ksq, const double *t) {
+  memset(g, 0, sizeof(*g));
+  if (ks > MAXK || ks < 2) return -1;
+  if (ksp <= 0 || ksp > ks) ksp = ks - 1;
+  if (ksq <= 0 || ksq > ks) ksq = ks;
+  g->ns = ns; g->ks = ks; g->ksp = ksp; g->ksq = ksq;
+  g->ms = ns + ns; g->nv = ns - ks + 1;
+  g->nsp = g->nv + ksp - 1; g->nsq = g->nv + ksq - 1;
+  const int nv = g->nv, nsp = g->nsp, nsq = g->nsq;
+  g->t = (double *)malloc(sizeof(double) * (ns + ks));
+  memcpy(g->t, t, sizeof(double) * (ns + ks));
+  g->gx = (double *)malloc(sizeof(double) * ks);
+  g->gw = (double *)malloc(sizeof(double) * ks);
+  g->gr = (double *)malloc(sizeof(double) * nv * ks);
+  g->grm = (double *)malloc(sizeof(double) * nv * ks);
+  g->grw = (double *)malloc(sizeof(double) * nv * ks);
+  orc_gauleg(0.0, 1.0, g->gx, g->gw, ks);
+  for (int m = 0; m < ks; ++m)
+    for (int iv = 0; iv < nv; ++iv) {
+      /* t(iv+ks) - t(iv+ks-1) with iv 1-based => t[iv+ks] - t[iv+ks-1] 0-based iv */
+      double h = g->t[iv + ks] - g->t[iv + ks - 1];
+      g->gr[iv + (size_t)nv * m] = h * g->gx[m] + g->t[iv + ks - 1];
+      g->grm[iv + (size_t)nv * m] = 1.0 / g->gr[iv + (size_t)nv * m];
+      g->grw[iv + (size_t)nv * m] = h * g->gw[m];
+    }
+  /* tp(1:ksp)=t(1); tp(ksp+1:nsp)=t(ks+1:ns); tp(nsp+1:nsp+ksp)=t(ns+1:ns+ksp) */
+  g->tp = (double *)malloc(sizeof(double) * (nsp + ksp));
+  g->tq = (double *)malloc(sizeof(double) * (nsq + ksq));
+  for (int i = 0; i < ksp; ++i) g->tp[i] = g->t[0];
+  for (int i = ksp; i < nsp; ++i) g->tp[i] = g->t[ks + (i - ksp)];
+  for (int i = nsp; i < nsp + ksp; ++i) g->tp[i] = g->t[ns + (i - nsp)];
+  for (int i = 0; i < ksq; ++i) g->tq[i] = g->t[0];
+  for (int i = ksq; i < nsq; ++i) g->tq[i] = g->t[ks + (i - ksq)];
+  for (int i = nsq; i < nsq + ksq; ++i) g->tq[i] = g->t[ns + (i - nsq)];
+  g->pbsp = (double *)malloc(sizeof(double) * (nv + 1) * ks * ksp);
+  g->qbsp = (double *)malloc(sizeof(double) * (nv + 1) * ks * ksq);
+  def_xbsp(g, ksp, nsp, g->tp, g->pbsp);
+  def_xbsp(g, ksq, nsq, g->tq, g->qbsp);
+  return 0;
+}
+
+void orc_grid_free(orc_grid *g) {
+  free(g->t); free(g->tp); free(g->tq); free(g->gx); free(g->gw);
+  free(g->gr); free(g->grm); free(g->grw); free(g->pbsp); free(g->qbsp);
+  memset(g, 0, sizeof(*g));
+}
+
+orc_grid *orc_grid_new(int ns, int ks, int ksp, int ksq, const double *t) {
+  orc_grid *g = (orc_grid *)malloc(sizeof(orc_grid));
+  if (orc_grid_init(g, ns, ks, ksp, ksq, t) != 0) { free(g); return NULL; }
+  return g;
+}
+void orc_grid_delete(orc_grid *g) { if (g) { orc_grid_free(g); free(g); } }
+void orc_grid_dims(const orc_grid *g, int *d) {
+  d[0] = g->ns; d[1] = g->ks; d[2] = g->ksp; d[3] = g->ksq;
+  d[4] = g->nv; d[5] = g->nsp; d[6] = g->nsq; d[7] = g->ms;
+}
+void orc_grid_tables(const orc_grid *g, double *gr, double *grw, double *pbsp, double *qbsp) {
+  size_t nvk = (size_t)g->nv * g->ks;
+  if (gr) memcpy(gr, g->gr, sizeof(double) * nvk);
+  if (grw) memcpy(grw, g->grw, sizeof(double) * nvk);
+  if (pbsp) memcpy(pbsp, g->pbsp, sizeof(double) * (g->nv + 1) * g->ks * g->ksp);
+  if (qbsp) memcpy(qbsp, g->qbsp, sizeof(double) * (g->nv + 1) * g->ks * g->ksq);
+}
+
+/* ===================================================================== */
+/* moments_pp/qq/pq/qp + moment_*: DBSR_HF/dbsr_lib_dbs.f90:1384-1564     */
+/* rkm is (ks*ks, nv); only the packed leading part is written.           */
+/* ===================================================================== */
+void orc_moments(const orc_grid *g, int which, int k, double *rkm) {
+  const int nv = g->nv, ks = g->ks, kk = ks * ks;
+  const int nw = nv + 1;
+  memset(rkm, 0, sizeof(double) * (size_t)kk * nv);
+  const double *b1t, *b2t;
+  int k1, k2, sym;
+  switch (which) {
+    case ORC_MOM_PP: b1t = g->pbsp; b2t = g->pbsp; k1 = g->ksp; k2 = g->ksp; sym = 1; break;
+    case ORC_MOM_QQ: b1t = g->qbsp; b2t = g->qbsp; k1 = g->ksq; k2 = g->ksq; sym = 1; break;
+    case ORC_MOM_PQ: b1t = g->pbsp; b2t = g->qbsp; k1 = g->ksp; k2 = g->ksq; sym = 0; break;
+    default:         b1t = g->qbsp; b2t = g->pbsp; k1 = g->ksq; k2 = g->ksp; sym = 0; break;
+  }
+  double gw[MAXK], b2[MAXK][MAXK];
+  for (int iv = 0; iv < nv; ++iv) {
+    for (int m = 0; m < ks; ++m) {
+      double w = g->grw[iv + (size_t)nv * m];
+      if (k > 0) w = w * ipow(g->gr[iv + (size_t)nv * m], k);
+      if (k < 0) w = w * ipow(g->grm[iv + (size_t)nv * m], -k);
+      gw[m] = w;
+    }
+    for (int j = 0; j < k2; ++j)
+      for (int m = 0; m < ks; ++m)
+        b2[j][m] = b2t[iv + (size_t)nw * (m + (size_t)ks * j)] * gw[m];
+    double *rv = rkm + (size_t)kk * iv;
+    int ii = 0;
+    for (int i = 0; i < k1; ++i)
+      for (int j = (sym ? i : 0); j < k2; ++j) {
+        double s = 0.0;
+        for (int m = 0; m < ks; ++m) s += b1t[iv + (size_t)nw * (m + (size_t)ks * i)] * b2[j][m];
+        rv[ii++] = s;
+      }
+  }
+}
+
+/* ===================================================================== */
+/* triang_*: DBSR_HF/dbsr_lib_dbs.f90:2979-3063 (pppp), 3151-3245 (pqpq), */
+/* 3333-3427 (qpqp), 3512-3597 (qqqq), 3682-3767 (ppqq), 3852-3936 (pqqp) */
+/* rkd is (kk,kk) for one interval iv (0-based), zero-filled.             */
+/* ===================================================================== */
+/* inner-rule B-spline values: zbsplvd(nsx,kx,nv,tx,left,1,x,1,...) */
+static void bvals(const double *tx, int nsx, int kx, int left, double x, double *out) {
+  double dbi[MAXK * MAXK];
+  orc_zbsplvd(nsx, kx, 1, tx, left, 1, &x, 1, dbi);
+  for (int j = 0; j < kx; ++j) out[j] = dbi[j];  /* dbiatx(1,j,1) with nv=1 */
+}
+
+void orc_triang(const orc_grid *g, int type, int k, int iv, double *rkd) {
+  const int nv = g->nv, ks = g->ks, kk = ks * ks, ksp = g->ksp, ksq = g->ksq;
+  const int nw = nv + 1;
+  const int ivf = iv + 1;  /* 1-based interval */
+  memset(rkd, 0, sizeof(double) * (size_t)kk * kk);
+#define RKD(a, b) rkd[(a) + (size_t)kk * (b)]
+#define GR(iv_, m) g->gr[(iv_) + (size_t)nv * (m)]
+#define GRW(iv_, m) g->grw[(iv_) + (size_t)nv * (m)]
+#define GRM(iv_, m) g->grm[(iv_) + (size_t)nv * (m)]
+#define PB(iv_, m, j) g->pbsp[(iv_) + (size_t)nw * ((m) + (size_t)ks * (j))]
+#define QB(iv_, m, j) g->qbsp[(iv_) + (size_t)nw * ((m) + (size_t)ks * (j))]
+  double x[MAXK], w[MAXK], gxl[MAXK], gwl[MAXK], gx2[MAXK], bi[MAXK];
+  double INTP[MAXK][MAXK][MAXK], INTQ[MAXK][MAXK][MAXK];  /* (j,jp,m) */
+  double bTp[MAXK][MAXK], bTq[MAXK][MAXK];                        /* (i,j)    */
+  orc_gauleg(0.0, 1.0, x, w, ks);
+
+  int needP = (type == ORC_PPPP || type == ORC_PQPQ || type == ORC_QPQP ||
+               type == ORC_PPQQ || type == ORC_PQQP);
+  int needQ = (type != ORC_PPPP);
+  int mixed = (type == ORC_PPQQ || type == ORC_PQQP);
+  double xbase;
+  if (type == ORC_PPPP) xbase = g->tp[ivf + ksp - 1 - 1];
+  else if (type == ORC_QQQQ) xbase = g->tq[ivf + ksq - 1 - 1];
+  else xbase = g->t[ivf + ks - 1 - 1];
+  int leftp = ivf + ksp - 1, leftq = ivf + ksq - 1;
+
+  for (int m = 0; m < ks; ++m) {
+    for (int i = 0; i < ks; ++i) gxl[i] = (GR(iv, m) - xbase) * x[i] + xbase;
+    if (needP) for (int i = 0; i < ks; ++i) bvals(g->tp, g->nsp, ksp, leftp, gxl[i], bTp[i]);
+    if (needQ) for (int i = 0; i < ks; ++i) bvals(g->tq, g->nsq, ksq, leftq, gxl[i], bTq[i]);
+    for (int i = 0; i < ks; ++i) gwl[i] = (GR(iv, m) - xbase) * w[i];
+    if (type == ORC_PPPP || type == ORC_QQQQ) {
+      if (k > 1) for (int i = 0; i < ks; ++i) gxl[i] = gwl[i] * ipow(gxl[i], k);
+      else if (k == 1) for (int i = 0; i < ks; ++i) gxl[i] = gwl[i] * gxl[i];
+      else if (k == 0) for (int i = 0; i < ks; ++i) gxl[i] = gwl[i];
+    } else {
+      for (int i = 0; i < ks; ++i) gxl[i] = gwl[i] * ipow(gxl[i], k);
+    }
+    if (!mixed) {
+      if (needP)
+        for (int j = 0; j < ksp; ++j) {
+          double gwj[MAXK];
+          for (int i = 0; i < ks; ++i) gwj[i] = gxl[i] * bTp[i][j];
+          for (int jp = j; jp < ksp; ++jp) {
+            double s = 0.0;
+            for (int i = 0; i < ks; ++i) s += gwj[i] * bTp[i][jp];
+            INTP[j][jp][m] = s;
+          }
+        }
+      if (needQ)
+        for (int j = 0; j < ksq; ++j) {
+          double gwj[MAXK];
+          for (int i = 0; i < ks; ++i) gwj[i] = gxl[i] * bTq[i][j];
+          for (int jp = j; jp < ksq; ++jp) {
+            double s = 0.0;
+            for (int i = 0; i < ks; ++i) s += gwj[i] * bTq[i][jp];
+            INTQ[j][jp][m] = s;
+          }
+        }
+    } else {
+      /* INTPQ(j,jp,m): j over P, jp over Q (stored in INTP) */
+      for (int j = 0; j < ksp; ++j) {
+        double gwj[MAXK];
+        for (int i = 0; i < ks; ++i) gwj[i] = gxl[i] * bTp[i][j];
+        for (int jp = 0; jp < ksq; ++jp) {
+          double s = 0.0;
+          for (int i = 0; i < ks; ++i) s += gwj[i] * bTq[i][jp];
+          INTP[j][jp][m] = s;
+        }
+      }
+    }
+  }
+  /* second integration weights */
+  for (int m = 0; m < ks; ++m) {
+    if ((type == ORC_PPPP || type == ORC_QQQQ) && k == 0) gx2[m] = GRW(iv, m) * GRM(iv, m);
+    else gx2[m] = GRW(iv, m) * ipow(GRM(iv, m), k + 1);
+  }
+
+  if (type == ORC_PPPP || type == ORC_QQQQ) {
+    int kx = (type == ORC_PPPP) ? ksp : ksq;
+    double (*IN)[MAXK][MAXK] = (type == ORC_PPPP) ? INTP : INTQ;
+    int ik = kx * (kx + 1) / 2;
+    double *a = (double *)malloc(sizeof(double) * ik * ik);
+    int ii = 0;
+    for (int i = 0; i < kx; ++i)
+      for (int ip = i; ip < kx; ++ip) {
+        for (int m = 0; m < ks; ++m) {
+          double bv = (type == ORC_PPPP) ? PB(iv, m, i) * PB(iv, m, ip) : QB(iv, m, i) * QB(iv, m, ip);
+          bi[m] = bv * gx2[m];
+        }
+        int jj = 0;
+        for (int j = 0; j < kx; ++j)
+          for (int jp = j; jp < kx; ++jp) {
+            double s = 0.0;
+            for (int m = 0; m < ks; ++m) s += bi[m] * IN[j][jp][m];
+            a[ii + (size_t)ik * jj] = s;
+            jj++;
+          }
+        ii++;
+      }
+    for (int b = 0; b < ik; ++b)
+      for (int c = 0; c < ik; ++c) RKD(c, b) = a[c + (size_t)ik * b] + a[b + (size_t)ik * c];
+    free(a);
+  } else if (type == ORC_PQPQ || type == ORC_QPQP) {
+    /* pass 1: P pairs outer, Q pairs inner; pass 2: Q outer, P inner */
+    int ii = 0;
+    for (int i = 0; i < ksp; ++i)
+      for (int ip = i; ip < ksp; ++ip) {
+        for (int m = 0; m < ks; ++m) bi[m] = PB(iv, m, i) * PB(iv, m, ip) * gx2[m];
+        int jj = 0;
+        for (int j = 0; j < ksq; ++j)
+          for (int jp = j; jp < ksq; ++jp) {
+            double s = 0.0;
+            for (int m = 0; m < ks; ++m) s += bi[m] * INTQ[j][jp][m];
+            if (type == ORC_PQPQ) RKD(ii, jj) += s; else RKD(jj, ii) += s;
+            jj++;
+          }
+        ii++;
+      }
+    ii = 0;
+    for (int i = 0; i < ksq; ++i)
+      for (int ip = i; ip < ksq; ++ip) {
+        for (int m = 0; m < ks; ++m) bi[m] = QB(iv, m, i) * QB(iv, m, ip) * gx2[m];
+        int jj = 0;
+        for (int j = 0; j < ksp; ++j)
+          for (int jp = j; jp < ksp; ++jp) {
+            double s = 0.0;
+            for (int m = 0; m < ks; ++m) s += bi[m] * INTP[j][jp][m];
+            if (type == ORC_PQPQ) RKD(jj, ii) += s; else RKD(ii, jj) += s;
+            jj++;
+          }
+        ii++;
+      }
+  } else {
+    /* ppqq: outer (i in P, ip in Q); pqqp: outer (i in Q, ip in P);
+       inner INTPQ(j in P, jp in Q); rkd(jj,ii) */
+    int ii = 0;
+    int ko1 = (type == ORC_PPQQ) ? ksp : ksq, ko2 = (type == ORC_PPQQ) ? ksq : ksp;
+    for (int i = 0; i < ko1; ++i)
+      for (int ip = 0; ip < ko2; ++ip) {
+        for (int m = 0; m < ks; ++m) {
+          double bv = (type == ORC_PPQQ) ? PB(iv, m, i) * QB(iv, m, ip) : QB(iv, m, i) * PB(iv, m, ip);
+          bi[m] = bv * gx2[m];
+        }
+        int jj = 0;
+        for (int j = 0; j < ksp; ++j)
+          for (int jp = 0; jp < ksq; ++jp) {
+            double s = 0.0;
+            for (int m = 0; m < ks; ++m) s += bi[m] * INTP[j][jp][m];
+            RKD(jj, ii) = s;
+            jj++;
+          }
+        ii++;
+      }
+  }
+#undef RKD
+#undef GR
+#undef GRW
+#undef GRM
+#undef PB
+#undef QB
+}
+
+/* ===================================================================== */
+/* mrk_pppp/qqqq/pqpq/qpqp: DBSR_HF/dbsr_lib_dbs.f90:2898-2959, 3431-3492,*/
+/* 3067-3131, 3249-3313;  msk_ppqq/pqqp: 3601-3662, 3771-3833             */
+/* ===================================================================== */
+size_t orc_rkb_size(const orc_grid *g, int type) {
+  size_t ns = (size_t)g->ns;
+  size_t kb = (type >= ORC_PPQQ) ? (size_t)(2 * g->ks - 1) : (size_t)g->ks;
+  return ns * ns * kb * kb;
+}
+
+void orc_mrk(const orc_grid *g, int type, int k, double *rkb) {
+  const int nv = g->nv, ns = g->ns, ks = g->ks, kk = ks * ks;
+  const int ksp = g->ksp, ksq = g->ksq;
+  double *r1 = (double *)malloc(sizeof(double) * kk * nv);
+  double *r2 = (double *)malloc(sizeof(double) * kk * nv);
+  double *r3 = (double *)malloc(sizeof(double) * kk * nv);
+  double *r4 = (double *)malloc(sizeof(double) * kk * nv);
+  double *rkd = (double *)malloc(sizeof(double) * (size_t)kk * kk * nv);
+  memset(rkb, 0, sizeof(double) * orc_rkb_size(g, type));
+  switch (type) {
+    case ORC_PPPP: orc_moments(g, ORC_MOM_PP, k, r1); orc_moments(g, ORC_MOM_PP, -(k + 1), r2); break;
+    case ORC_QQQQ: orc_moments(g, ORC_MOM_QQ, k, r1); orc_moments(g, ORC_MOM_QQ, -(k + 1), r2); break;
+    case ORC_PQPQ:
+    case ORC_QPQP:
+      orc_moments(g, ORC_MOM_PP, k, r1); orc_moments(g, ORC_MOM_QQ, -(k + 1), r2);
+      orc_moments(g, ORC_MOM_QQ, k, r3); orc_moments(g, ORC_MOM_PP, -(k + 1), r4); break;
+    case ORC_PPQQ: orc_moments(g, ORC_MOM_PQ, k, r1); orc_moments(g, ORC_MOM_PQ, -(k + 1), r2); break;
+    default:       orc_moments(g, ORC_MOM_PQ, k, r1); orc_moments(g, ORC_MOM_QP, -(k + 1), r2); break;
+  }
+  for (int iv = 0; iv < nv; ++iv) orc_triang(g, type, k, iv, rkd + (size_t)kk * kk * iv);
+#define M(a, ii, iv) a[(ii) + (size_t)kk * (iv)]
+#define RKDV(ii, jj, iv) rkd[(ii) + (size_t)kk * ((jj) + (size_t)kk * (iv))]
+  if (type <= ORC_QPQP) {
+    int kl = (type == ORC_PPPP || type == ORC_PQPQ) ? ksp : ksq;  /* left (r1) order */
+    int kr = (type == ORC_PPPP || type == ORC_QPQP) ? ksp : ksq;  /* right (r2) order */
+#define RKB(i, j, ip, jp) rkb[(i) + (size_t)ns * ((j) + (size_t)ns * ((ip) + (size_t)ks * (jp)))]
+    for (int jv = 0; jv < nv; ++jv) {
+      int jj = 0;
+      for (int jh = 0; jh < kr; ++jh)
+        for (int jhp = jh; jhp < kr; ++jhp) {
+          int j = jv + jh, jp = jhp - jh;
+          for (int iv = 0; iv < nv; ++iv) {
+            int ii = 0;
+            for (int ih = 0; ih < kl; ++ih)
+              for (int ihp = ih; ihp < kl; ++ihp) {
+                int i = iv + ih, ip = ihp - ih;
+                double c;
+                if (iv < jv) {
+                  if (type == ORC_QPQP) c = M(r3, ii, iv) * M(r4, jj, jv);
+                  else c = M(r1, ii, iv) * M(r2, jj, jv);
+                } else if (iv > jv) {
+                  if (type == ORC_PQPQ) c = M(r3, jj, jv) * M(r4, ii, iv);
+                  else c = M(r1, jj, jv) * M(r2, ii, iv);
+                } else {
+                  c = RKDV(ii, jj, iv);
+                }
+                RKB(i, j, ip, jp) += c;
+                ii++;
+              }
+          }
+          jj++;
+        }
+    }
+#undef RKB
+  } else {
+    const int kb = 2 * ks - 1;
+    int kr1 = (type == ORC_PPQQ) ? ksp : ksq, kr2 = (type == ORC_PPQQ) ? ksq : ksp;
+#define RKB(i, j, ip, jp) rkb[(i) + (size_t)ns * ((j) + (size_t)ns * ((ip) + (size_t)kb * (jp)))]
+    for (int jv = 0; jv < nv; ++jv) {
+      int jj = 0;
+      for (int jh = 0; jh < kr1; ++jh)
+        for (int jhp = 0; jhp < kr2; ++jhp) {
+          int j = jv + jh, jp = jhp - jh + ks - 1;
+          for (int iv = 0; iv <= jv; ++iv) {
+            int ii = 0;
+            for (int ih = 0; ih < ksp; ++ih)
+              for (int ihp = 0; ihp < ksq; ++ihp) {
+                int i = iv + ih, ip = ihp - ih + ks - 1;
+                double c = (iv == jv) ? RKDV(ii, jj, iv) : M(r1, ii, iv) * M(r2, jj, jv);
+                RKB(i, j, ip, jp) += c;
+                ii++;
+              }
+          }
+          jj++;
+        }
+    }
+#undef RKB
+  }
+#undef M
+#undef RKDV
+  free(r1); free(r2); free(r3); free(r4); free(rkd);
+}
+
+/* ===================================================================== */
+/* density: DBSR_HF/dbsr_lib_dbs.f90:2834-2891                            */
+/* ===================================================================== */
+void orc_density(int ns, int ks, double *d, const double *p1, const double *p2, char type) {
+#define D(i, j) d[(i) + (size_t)ns * (j)]
+  if (type == 's' || type == 'u') {
+    memset(d, 0, sizeof(double) * (size_t)ns * ks);
+    for (int i = 0; i < ns; ++i) D(i, 0) = p1[i] * p2[i];
+    for (int j = 1; j < ks; ++j)
+      for (int i = 0; i < ns - j; ++i) D(i, j) = p1[i] * p2[i + j] + p1[i + j] * p2[i];
+  } else if (type == 'l') {
+    memset(d, 0, sizeof(double) * (size_t)ns * ks);
+    for (int i = 0; i < ns; ++i) D(i, ks - 1) = p1[i] * p2[i];
+    for (int j = 1; j <= ks - 1; ++j)   /* 1-based j */
+      for (int i = ks + 1 - j; i <= ns; ++i)
+        D(i - 1, j - 1) = p1[i - 1] * p2[i + j - ks - 1] + p1[i + j - ks - 1] * p2[i - 1];
+  } else if (type == 'n') {
+    memset(d, 0, sizeof(double) * (size_t)ns * (2 * ks - 1));
+    for (int j = 1; j <= 2 * ks - 1; ++j) {
+      int imin = 1 > 1 + ks - j ? 1 : 1 + ks - j;
+      int imax = ns < ns + ks - j ? ns : ns + ks - j;
+      for (int i = imin; i <= imax; ++i) D(i - 1, j - 1) = p1[i - 1] * p2[i + j - ks - 1];
+    }
+  } else {
+    for (int i = 0; i < ns; ++i)
+      for (int j = 0; j < ns; ++j) D(i, j) = p1[i] * p2[j];
+  }
+#undef D
+}
+
+/* ===================================================================== */
+/* convol / convol_sk: DBSR_HF/dbsr_lib_dbs.f90:2208-2478, 2482-2752      */
+/* kb = third/fourth dimension of rkb (ks for R^k, 2ks-1 for S^k)         */
+/* ===================================================================== */
+void orc_convol(int ns, int ks, int kb, const double *rkb, double *a, const double *d,
+                int icase, char si, char sj) {
+#define R(i, j, ip, jp) rkb[((i) - 1) + (size_t)ns * (((j) - 1) + (size_t)ns * (((ip) - 1) + (size_t)kb * ((jp) - 1)))]
+#define A(i, j) a[((i) - 1) + (size_t)ns * ((j) - 1)]
+#define DD(i, j) d[((i) - 1) + (size_t)ns * ((j) - 1)]
+#define IMIN(ip) ((1 > 1 + ks - (ip)) ? 1 : 1 + ks - (ip))
+#define IMAX(ip) ((ns < ns + ks - (ip)) ? ns : ns + ks - (ip))
+  int i, j, ip, jp, ii, jj;
+  double c;
+  if (icase <= 2) {
+    int ncol = (si == 'n' || sj == 'n') ? 2 * ks - 1 : ks;
+    (void)ncol;
+    for (j = 1; j <= ks; ++j) for (i = 1; i <= ns; ++i) A(i, j) = 0.0;
+  } else {
+    memset(a, 0, sizeof(double) * (size_t)ns * ns);
+  }
+  if (icase == 1) {
+    if (si == 's' && sj == 's') {
+      for (ip = 1; ip <= ks; ++ip) for (i = 1; i <= ns - ip + 1; ++i) {
+        c = 0.0;
+        for (jp = 1; jp <= ks; ++jp) for (j = 1; j <= ns - jp + 1; ++j) c += DD(j, jp) * R(i, j, ip, jp);
+        A(i, ip) = c;
+      }
+    } else if (si == 'n' && sj == 'n') {
+      for (ip = 1; ip <= 2 * ks - 1; ++ip) for (i = IMIN(ip); i <= IMAX(ip); ++i) {
+        c = 0.0;
+        for (jp = 1; jp <= 2 * ks - 1; ++jp) for (j = IMIN(jp); j <= IMAX(jp); ++j) c += DD(j, jp) * R(i, j, ip, jp);
+        A(i, ip) = c;
+      }
+    } else if (si == 'l' && sj == 'l') {
+      for (ip = 1; ip <= ks; ++ip) for (i = ks + 1 - ip; i <= ns; ++i) {
+        c = 0.0;
+        for (jp = 1; jp <= ks; ++jp) for (j = ks + 1 - jp; j <= ns; ++j) c += DD(j, jp) * R(i, j, ip, jp);
+        A(i, ip) = c;
+      }
+    } else if (si == 'l' && sj == 'n') {
+      for (ip = 1; ip <= ks; ++ip) for (i = ks + 1 - ip; i <= ns; ++i) {
+        c = 0.0;
+        for (jp = 1; jp <= 2 * ks - 1; ++jp) for (j = IMIN(jp); j <= IMAX(jp); ++j) c += DD(j, jp) * R(i, j, ip, jp);
+        A(i, ip) = c;
+      }
+    } else if (si == 'n' && sj == 'l') {
+      for (ip = 1; ip <= 2 * ks - 1; ++ip) for (i = IMIN(ip); i <= IMAX(ip); ++i) {
+        c = 0.0;
+        for (jp = 1; jp <= ks; ++jp) for (j = ks + 1 - jp; j <= ns; ++j) c += DD(j, jp) * R(i, j, ip, jp);
+        A(i, ip) = c;
+      }
+    }
+  } else if (icase == 2) {
+    if (si == 's' && sj == 's') {
+      for (jp = 1; jp <= ks; ++jp) for (j = 1; j <= ns - jp + 1; ++j) {
+        c = 0.0;
+        for (ip = 1; ip <= ks; ++ip) for (i = 1; i <= ns - ip + 1; ++i) c += DD(i, ip) * R(i, j, ip, jp);
+        A(j, jp) = c;
+      }
+    } else if (si == 'n' && sj == 'n') {
+      for (jp = 1; jp <= 2 * ks - 1; ++jp) for (j = IMIN(jp); j <= IMAX(jp); ++j) {
+        c = 0.0;
+        for (ip = 1; ip <= 2 * ks - 1; ++ip) for (i = IMIN(ip); i <= IMAX(ip); ++i) c += DD(i, ip) * R(i, j, ip, jp);
+        A(j, jp) = c;
+      }
+    } else if (si == 'l' && sj == 'l') {
+      for (jp = 1; jp <= ks; ++jp) for (j = ks + 1 - jp; j <= ns; ++j) {
+        c = 0.0;
+        for (ip = 1; ip <= ks; ++ip) for (i = ks + 1 - ip; i <= ns; ++i) c += DD(i, ip) * R(i, j, ip, jp);
+        A(j, jp) = c;
+      }
+    } else if (si == 'l' && sj == 'n') {
+      for (jp = 1; jp <= 2 * ks - 1; ++jp) for (j = IMIN(jp); j <= IMAX(jp); ++j) {
+        c = 0.0;
+        for (ip = 1; ip <= ks; ++ip) for (i = ks + 1 - ip; i <= ns; ++i) c += DD(i, ip) * R(i, j, ip, jp);
+        A(j, jp) = c;
+      }
+    } else if (si == 'n' && sj == 'l') {
+      for (jp = 1; jp <= ks; ++jp) for (j = ks + 1 - jp; j <= ns; ++j) {
+        c = 0.0;
+        for (ip = 1; ip <= 2 * ks - 1; ++ip) for (i = IMIN(ip); i <= IMAX(ip); ++i) c += DD(i, ip) * R(i, j, ip, jp);
+        A(j, jp) = c;
+      }
+    }
+  } else if (icase == 3) {
+    if (si == 's' && sj == 's') {
+      for (jp = 1; jp <= ks; ++jp) for (j = 1; j <= ns - jp + 1; ++j) { jj = j + jp - 1;
+        for (ip = 1; ip <= ks; ++ip) for (i = 1; i <= ns - ip + 1; ++i) { ii = i + ip - 1;
+          c = R(i, j, ip, jp);               A(i, jj) += c * DD(ii, j);
+          if (ip > 1)                        A(ii, jj) += c * DD(i, j);
+          if (jp > 1)                        A(i, j) += c * DD(ii, jj);
+          if (ip > 1 && jp > 1)              A(ii, j) += c * DD(i, jj);
+        } }
+    } else if (si == 'l' && sj == 'l') {
+      for (jp = 1; jp <= ks; ++jp) for (j = ks + 1 - jp; j <= ns; ++j) { jj = j + jp - ks;
+        for (ip = 1; ip <= ks; ++ip) for (i = ks + 1 - ip; i <= ns; ++i) { ii = i + ip - ks;
+          c = R(i, j, ip, jp);               A(i, jj) += c * DD(ii, j);
+          if (ip != ks)                      A(ii, jj) += c * DD(i, j);
+          if (jp != ks)                      A(i, j) += c * DD(ii, jj);
+          if (ip != ks && jp != ks)          A(ii, j) += c * DD(i, jj);
+        } }
+    } else if (si == 'n' && sj == 'n') {
+      for (jp = 1; jp <= 2 * ks - 1; ++jp) for (j = IMIN(jp); j <= IMAX(jp); ++j) { jj = j + jp - ks;
+        for (ip = 1; ip <= 2 * ks - 1; ++ip) for (i = IMIN(ip); i <= IMAX(ip); ++i) { ii = i + ip - ks;
+          c = R(i, j, ip, jp);               A(i, jj) += c * DD(ii, j);
+        } }
+    } else if (si == 'n' && sj == 'l') {
+      for (jp = 1; jp <= ks; ++jp) for (j = ks + 1 - jp; j <= ns; ++j) { jj = j + jp - ks;
+        for (ip = 1; ip <= 2 * ks - 1; ++ip) for (i = IMIN(ip); i <= IMAX(ip); ++i) { ii = i + ip - ks;
+          c = R(i, j, ip, jp);               A(i, jj) += c * DD(ii, j);
+          if (jp != ks)                      A(i, j) += c * DD(ii, jj);
+        } }
+    } else if (si == 'l' && sj == 'n') {
+      for (jp = 1; jp <= 2 * ks - 1; ++jp) for (j = IMIN(jp); j <= IMAX(jp); ++j) { jj = j + jp - ks;
+        for (ip = 1; ip <= ks; ++ip) for (i = ks + 1 - ip; i <= ns; ++i) { ii = i + ip - ks;
+          c = R(i, j, ip, jp);               A(i, jj) += c * DD(ii, j);
+          if (ip != ks)                      A(ii, jj) += c * DD(i, j);
+        } }
+    }
+  } else if (icase == 4) {
+    if (si == 's' && sj == 's') {
+      for (jp = 1; jp <= ks; ++jp) for (j = 1; j <= ns - jp + 1; ++j) { jj = j + jp - 1;
+        for (ip = 1; ip <= ks; ++ip) for (i = 1; i <= ns - ip + 1; ++i) { ii = i + ip - 1;
+          c = R(i, j, ip, jp);               A(ii, j) += c * DD(i, jj);
+          if (ip > 1)                        A(i, j) += c * DD(ii, jj);
+          if (jp > 1)                        A(ii, jj) += c * DD(i, j);
+          if (ip > 1 && jp > 1)              A(i, jj) += c * DD(ii, j);
+        } }
+    } else if (si == 'n' && sj == 'n') {
+      for (jp = 1; jp <= 2 * ks - 1; ++jp) for (j = IMIN(jp); j <= IMAX(jp); ++j) { jj = j + jp - ks;
+        for (ip = 1; ip <= 2 * ks - 1; ++ip) for (i = IMIN(ip); i <= IMAX(ip); ++i) { ii = i + ip - ks;
+          c = R(i, j, ip, jp);               A(ii, j) += c * DD(i, jj);
+        } }
+    } else if (si == 'l' && sj == 'l') {
+      for (jp = 1; jp <= ks; ++jp) for (j = ks + 1 - jp; j <= ns; ++j) { jj = j + jp - ks;
+        for (ip = 1; ip <= ks; ++ip) for (i = ks + 1 - ip; i <= ns; ++i) { ii = i + ip - ks;
+          c = R(i, j, ip, jp);               A(ii, j) += c * DD(i, jj);
+          if (ip != ks)                      A(i, j) += c * DD(ii, jj);
+          if (jp != ks)                      A(ii, jj) += c * DD(i, j);
+          if (ip != ks && jp != ks)          A(i, jj) += c * DD(ii, j);
+        } }
+    } else if (si == 'n' && sj == 'l') {
+      for (jp = 1; jp <= ks; ++jp) for (j = ks + 1 - jp; j <= ns; ++j) { jj = j + jp - ks;
+        for (ip = 1; ip <= 2 * ks - 1; ++ip) for (i = IMIN(ip); i <= IMAX(ip); ++i) { ii = i + ip - ks;
+          c = R(i, j, ip, jp);               A(ii, j) += c * DD(i, jj);
+          if (jp != ks)                      A(ii, jj) += c * DD(i, j);
+        } }
+    } else if (si == 'l' && sj == 'n') {
+      for (jp = 1; jp <= 2 * ks - 1; ++jp) for (j = IMIN(jp); j <= IMAX(jp); ++j) { jj = j + jp - ks;
+        for (ip = 1; ip <= ks; ++ip) for (i = ks + 1 - ip; i <= ns; ++i) { ii = i + ip - ks;
+          c = R(i, j, ip, jp);               A(ii, j) += c * DD(i, jj);
+          if (ip != ks)                      A(i, j) += c * DD(ii, jj);
+        } }
+    }
+  }
+#undef R
+#undef A
+#undef DD
+#undef IMIN
+#undef IMAX
+}
+
+/* SUM_AmB: DBSR_HF/dbsr_lib_dbs.f90:4027-4080 */
+double orc_sum_amb(int ns, int ks, const double *a, const double *b, char sym) {
+  double x = 0.0;
+#define AB(i, j) (a[((i) - 1) + (size_t)ns * ((j) - 1)] * b[((i) - 1) + (size_t)ns * ((j) - 1)])
+  if (sym == 'x') {
+    for (int i = 1; i <= ns; ++i) for (int j = 1; j <= ns; ++j) x += AB(i, j);
+  } else if (sym == 's') {
+    for (int j = 1; j <= ks; ++j) for (int i = 1; i <= ns - j + 1; ++i) x += AB(i, j);
+  } else if (sym == 'l') {
+    for (int j = 1; j <= ks; ++j) for (int i = ks + 1 - j; i <= ns; ++i) x += AB(i, j);
+  } else if (sym == 'n') {
+    for (int j = 1; j <= 2 * ks - 1; ++j) {
+      int imin = 1 > 1 + ks - j ? 1 : 1 + ks - j, imax = ns < ns + ks - j ? ns : ns + ks - j;
+      for (int i = imin; i <= imax; ++i) x += AB(i, j);
+    }
+  }
+#undef AB
+  return x;
+}
+
+/* quadr: DBSR_HF/dbsr_lib_dbs.f90:3940-3976;  fi, fj are (ns,2) */
+double orc_quadr(const orc_grid *g, const double *fi, const double *fj, int m) {
+  const int nv = g->nv, ks = g->ks, ns = g->ns, nw = nv + 1;
+  double q = 0.0, gx[MAXK], gw[MAXK];
+  for (int comp = 0; comp < 2; ++comp) {
+    int kx = comp == 0 ? g->ksp : g->ksq;
+    const double *xb = comp == 0 ? g->pbsp : g->qbsp;
+    for (int iv = 0; iv < nv; ++iv) {
+      for (int mm = 0; mm < ks; ++mm) gx[mm] = ipow(g->gr[iv + (size_t)nv * mm], m) * g->grw[iv + (size_t)nv * mm];
+      for (int ith = 0; ith < kx; ++ith) {
+        double pi = fi[iv + ith + (size_t)ns * comp];
+        for (int mm = 0; mm < ks; ++mm) gw[mm] = gx[mm] * xb[iv + (size_t)nw * (mm + (size_t)ks * ith)];
+        for (int jth = 0; jth < kx; ++jth) {
+          double pj = fj[iv + jth + (size_t)ns * comp];
+          double s = 0.0;
+          for (int mm = 0; mm < ks; ++mm) s += xb[iv + (size_t)nw * (mm + (size_t)ks * jth)] * gw[mm];
+          q = q + s * pi * pj;
+        }
+      }
+    }
+  }
+  return q;
+}
+
+/* unfold band/full storage into y(ns,ns) (UPDATE_HS, Full_mat_sym) */
+static void unfold(int ns, int ks, const double *d, char sym, double *y) {
+  memset(y, 0, sizeof(double) * (size_t)ns * ns);
+#define Y(i, j) y[((i) - 1) + (size_t)ns * ((j) - 1)]
+#define DV(i, j) d[((i) - 1) + (size_t)ns * ((j) - 1)]
+  if (sym == 's') {
+    for (int jp = 1; jp <= ks; ++jp) for (int i = 1; i <= ns - jp + 1; ++i) {
+      int j = i + jp - 1; Y(i, j) = DV(i, jp); Y(j, i) = DV(i, jp);
+    }
+  } else if (sym == 'n') {
+    for (int jp = 1; jp <= 2 * ks - 1; ++jp) {
+      int imin = 1 > 1 + ks - jp ? 1 : 1 + ks - jp, imax = ns < ns + ks - jp ? ns : ns + ks - jp;
+      for (int i = imin; i <= imax; ++i) { int j = i + jp - ks; Y(i, j) = DV(i, jp); }
+    }
+  } else {
+    memcpy(y, d, sizeof(double) * (size_t)ns * ns);
+  }
+#undef Y
+#undef DV
+}
+
+/* UPDATE_HS (library version): DBSR_HF/dbsr_lib_dbs.f90:4137-4193 */
+void orc_update_hs_lib(int ms, double *hm, int ii, int jj, int ns, int ks, const double *d, char sym) {
+  double *y = (double *)malloc(sizeof(double) * (size_t)ns * ns);
+  unfold(ns, ks, d, sym, y);
+  int i1 = (ii - 1) * ns, j1 = (jj - 1) * ns;
+  for (size_t q = 0; q < (size_t)ns * ns; ++q) y[q] = y[q] / 2.0;
+  for (int b = 0; b < ns; ++b) for (int a = 0; a < ns; ++a)
+    hm[(i1 + a) + (size_t)ms * (j1 + b)] += y[a + (size_t)ns * b];
+  for (int b = 0; b < ns; ++b) for (int a = 0; a < ns; ++a)
+    hm[(j1 + a) + (size_t)ms * (i1 + b)] += y[b + (size_t)ns * a];
+  free(y);
+}
+
+/* UPDATE_HS (MAT3 channel-block version): DBSR_MAT3/updates.f90:110-179.
+ * blk is the (ms,ms) block icc(ich,jch); ii,jj select the P/Q sub-block. */
+void orc_update_hs_mat(int ns, int ks, double *blk, int ich, int jch, const double *d,
+                       char sym, int ii, int jj) {
+  const int ms = 2 * ns;
+  double *y = (double *)malloc(sizeof(double) * (size_t)ns * ns);
+  unfold(ns, ks, d, sym, y);
+  int i1 = (ii - 1) * ns, j1 = (jj - 1) * ns;
+  if (ich > jch) {
+    for (int b = 0; b < ns; ++b) for (int a = 0; a < ns; ++a)
+      blk[(i1 + a) + (size_t)ms * (j1 + b)] += y[a + (size_t)ns * b];
+  } else if (jch > ich) {
+    for (int b = 0; b < ns; ++b) for (int a = 0; a < ns; ++a)
+      blk[(i1 + a) + (size_t)ms * (j1 + b)] += y[b + (size_t)ns * a];
+  } else {
+    for (size_t q = 0; q < (size_t)ns * ns; ++q) y[q] = y[q] / 2.0;
+    for (int b = 0; b < ns; ++b) for (int a = 0; a < ns; ++a)
+      blk[(i1 + a) + (size_t)ms * (j1 + b)] += y[a + (size_t)ns * b];
+    for (int b = 0; b < ns; ++b) for (int a = 0; a < ns; ++a)
+      blk[(j1 + a) + (size_t)ms * (i1 + b)] += y[b + (size_t)ns * a];
+  }
+  free(y);
+}
+
+/* rk: DBSR_HF/dbsr_lib_dbs.f90:3981-4022;  f* are (ns,2) */
+double orc_rk(const orc_grid *g, const double *rpppp, const double *rqqqq,
+              const double *rpqpq, const double *rqpqp,
+              const double *f1, const double *f2, const double *f3, const double *f4) {
+  const int ns = g->ns, ks = g->ks;
+  size_t n = (size_t)ns * ks;
+  double *d1 = (double *)malloc(sizeof(double) * n * 5);
+  double *d2 = d1 + n, *d3 = d2 + n, *d4 = d3 + n, *cv = d4 + n;
+  orc_density(ns, ks, d1, f1, f3, 's');
+  orc_density(ns, ks, d2, f2, f4, 's');
+  orc_density(ns, ks, d3, f1 + ns, f3 + ns, 's');
+  orc_density(ns, ks, d4, f2 + ns, f4 + ns, 's');
+  double r = 0.0;
+  orc_convol(ns, ks, ks, rpppp, cv, d1, 2, 's', 's'); r += orc_sum_amb(ns, ks, cv, d2, 's');
+  orc_convol(ns, ks, ks, rqqqq, cv, d3, 2, 's', 's'); r += orc_sum_amb(ns, ks, cv, d4, 's');
+  orc_convol(ns, ks, ks, rqpqp, cv, d3, 2, 's', 's'); r += orc_sum_amb(ns, ks, cv, d2, 's');
+  orc_convol(ns, ks, ks, rpqpq, cv, d1, 2, 's', 's'); r += orc_sum_amb(ns, ks, cv, d4, 's');
+  free(d1);
+  return r;
+}
+
+/* ===================================================================== */
+/* c_data: DBSR_MAT3/MOD_c_data.f90:2-368.  Arrays are 1-based inside.    */
+/* ===================================================================== */
+struct orc_cdata {
+  int ntype, kpol1, kpol2, mb, nb, kb, ncdata;
+  double eps_cdata;
+  double *CDATA;
+  int *K1, *K2, *K3, *K4, *IPT;
+  int *ipblk, *jpblk, *ipi, *ipj;
+  int *iblk, *nblk, *kblk;  /* iblk(k,type), nblk(k,type), kblk(k,type,nb) */
+  orc_gen_matrix_fn gen;
+  void *user;
+  int error;
+};
+#define IBLK(cd, k, t) (cd)->iblk[((k) - (cd)->kpol1) + ((cd)->kpol2 - (cd)->kpol1 + 1) * ((t) - 1)]
+#define NBLK(cd, k, t) (cd)->nblk[((k) - (cd)->kpol1) + ((cd)->kpol2 - (cd)->kpol1 + 1) * ((t) - 1)]
+#define KBLK(cd, k, t, n) (cd)->kblk[((k) - (cd)->kpol1) + ((cd)->kpol2 - (cd)->kpol1 + 1) * (((t) - 1) + (cd)->ntype * ((n) - 1))]
+
+orc_cdata *orc_cdata_alloc(int nt, int kp1, int kp2, int mblock, int nblock, int kblock,
+                           double eps_c, orc_gen_matrix_fn gen, void *user) {
+  orc_cdata *cd = (orc_cdata *)calloc(1, sizeof(orc_cdata));
+  cd->ntype = nt; cd->kpol1 = kp1; cd->kpol2 = kp2;
+  cd->mb = mblock; cd->nb = nblock; cd->kb = kblock;
+  int i = nt * (kp2 - kp1 + 1) * 2 + 1;
+  if (cd->nb < i) cd->nb = i;
+  cd->eps_cdata = eps_c;
+  cd->gen = gen; cd->user = user;
+  size_t m = (size_t)cd->mb * cd->nb;
+  cd->CDATA = (double *)calloc(m + 1, sizeof(double));
+  cd->K1 = (int *)calloc(m + 1, sizeof(int)); cd->K2 = (int *)calloc(m + 1, sizeof(int));
+  cd->K3 = (int *)calloc(m + 1, sizeof(int)); cd->K4 = (int *)calloc(m + 1, sizeof(int));
+  cd->IPT = (int *)calloc(m + 1, sizeof(int));
+  cd->ipblk = (int *)calloc(cd->nb + 1, sizeof(int)); cd->jpblk = (int *)calloc(cd->nb + 1, sizeof(int));
+  cd->ipi = (int *)calloc(cd->nb + 1, sizeof(int)); cd->ipj = (int *)calloc(cd->nb + 1, sizeof(int));
+  int nk = kp2 - kp1 + 1;
+  cd->iblk = (int *)calloc((size_t)nk * nt, sizeof(int));
+  cd->nblk = (int *)calloc((size_t)nk * nt, sizeof(int));
+  cd->kblk = (int *)calloc((size_t)nk * nt * cd->nb, sizeof(int));
+  for (i = 1; i <= cd->nb; ++i) { cd->ipblk[i] = (i - 1) * cd->mb + 1; cd->jpblk[i] = -1; }
+  i = 0;
+  for (int k = kp1; k <= kp2; ++k)
+    for (int j = 1; j <= nt; ++j) {
+      i++;
+      IBLK(cd, k, j) = i; NBLK(cd, k, j) = 1; KBLK(cd, k, j, 1) = i; cd->jpblk[i] = 0;
+    }
+  return cd;
+}
+
+void orc_cdata_free(orc_cdata *cd) {
+  if (!cd) return;
+  free(cd->CDATA); free(cd->K1); free(cd->K2); free(cd->K3); free(cd->K4); free(cd->IPT);
+  free(cd->ipblk); free(cd->jpblk); free(cd->ipi); free(cd->ipj);
+  free(cd->iblk); free(cd->nblk); free(cd->kblk); free(cd);
+}
+
+/* Add_cdata: binary-search insert into the sorted sub-list ip..jp */
+static int add_cdata(orc_cdata *cd, int ip, int *jp, double C, int j1, int j2, int j3, int j4) {
+  if (*jp < ip) {
+    cd->CDATA[ip] = C; cd->K1[ip] = j1; cd->K2[ip] = j2; cd->K3[ip] = j3; cd->K4[ip] = j4;
+    *jp = ip;
+    return 0;
+  }
+  int k = ip, l = *jp, m;
+  while (k <= l) {
+    m = (k + l) / 2;
+    if (j1 < cd->K1[m]) l = m - 1;
+    else if (j1 > cd->K1[m]) k = m + 1;
+    else if (j2 < cd->K2[m]) l = m - 1;
+    else if (j2 > cd->K2[m]) k = m + 1;
+    else if (j3 < cd->K3[m]) l = m - 1;
+    else if (j3 > cd->K3[m]) k = m + 1;
+    else if (j4 < cd->K4[m]) l = m - 1;
+    else if (j4 > cd->K4[m]) k = m + 1;
+    else { cd->CDATA[m] = cd->CDATA[m] + C; return 0; }
+  }
+  if (*jp - ip + 1 >= cd->mb) return -1;
+  for (int i = *jp; i >= k; --i) {
+    int ii = i + 1;
+    cd->CDATA[ii] = cd->CDATA[i];
+    cd->K1[ii] = cd->K1[i]; cd->K2[ii] = cd->K2[i]; cd->K3[ii] = cd->K3[i]; cd->K4[ii] = cd->K4[i];
+  }
+  cd->CDATA[k] = C; cd->K1[k] = j1; cd->K2[k] = j2; cd->K3[k] = j3; cd->K4[k] = j4;
+  *jp = *jp + 1;
+  return 0;
+}
+
+/* Merge_cdata: k-way merge of sorted blocks, summing equal keys in block order */
+static int merge_cdata(orc_cdata *cd, int nn, int *IP, int *JP) {
+  int nc = 0, mm = 0, i, j, m;
+  for (m = 1; m <= nn; ++m) { if (JP[m] <= 0) continue; mm = m; break; }
+  if (mm == 0) return 0;
+  for (;;) {
+    int merged = 0;
+    for (int ii = 1; ii <= nn - 1 && !merged; ++ii) {
+      if (JP[ii] <= 0) continue;
+      i = IP[ii];
+      for (int jj = ii + 1; jj <= nn; ++jj) {
+        if (JP[jj] <= 0) continue;
+        j = IP[jj];
+        if (cd->K1[i] != cd->K1[j] || cd->K2[i] != cd->K2[j] || cd->K3[i] != cd->K3[j] ||
+            cd->K4[i] != cd->K4[j]) continue;
+        cd->CDATA[i] = cd->CDATA[i] + cd->CDATA[j];
+        cd->CDATA[j] = 0.0;
+        mm = jj; merged = 1;
+        break;
+      }
+    }
+    if (!merged) {
+      j = IP[mm];
+      for (m = 1; m <= nn; ++m) {
+        if (JP[m] == 0) continue;
+        i = IP[m];
+        if (cd->K1[i] < cd->K1[j]) { mm = m; j = i; }
+        else if (cd->K1[i] > cd->K1[j]) continue;
+        else if (cd->K2[i] < cd->K2[j]) { mm = m; j = i; }
+        else if (cd->K2[i] > cd->K2[j]) continue;
+        else if (cd->K3[i] < cd->K3[j]) { mm = m; j = i; }
+        else if (cd->K3[i] > cd->K3[j]) continue;
+        else if (cd->K4[i] < cd->K4[j]) { mm = m; j = i; }
+      }
+      if (fabs(cd->CDATA[j]) > cd->eps_cdata) { nc++; cd->IPT[nc] = IP[mm]; }
+    }
+    IP[mm] = IP[mm] + 1;
+    if (IP[mm] <= JP[mm]) continue;
+    JP[mm] = 0;
+    int found = 0;
+    for (m = 1; m <= nn; ++m) if (JP[m] > 0) { mm = m; found = 1; break; }
+    if (!found) break;
+  }
+  return nc;
+}
+
+int orc_cdata_add_matrix(orc_cdata *cd, int itype, int kpol) {
+  int n = NBLK(cd, kpol, itype);
+  int i = KBLK(cd, kpol, itype, 1), j, nc;
+  if (n == 1 && cd->jpblk[i] <= 0) n = 0;
+  if (n <= 0) {
+    nc = 0;
+    cd->ncdata = 0;
+    return 0;  /* reference returns without calling Gen_matrix */
+  } else if (n == 1) {
+    nc = cd->jpblk[i] - cd->ipblk[i] + 1;
+    j = cd->ipblk[i] - 1;
+    for (i = 1; i <= nc; ++i) cd->IPT[i] = j + i;
+  } else {
+    for (i = 1; i <= n; ++i) {
+      j = KBLK(cd, kpol, itype, i);
+      cd->ipi[i] = cd->ipblk[j];
+      cd->ipj[i] = cd->jpblk[j];
+    }
+    nc = merge_cdata(cd, n, cd->ipi, cd->ipj);
+    for (i = 1; i <= n; ++i) cd->jpblk[KBLK(cd, kpol, itype, i)] = -1;
+  }
+  cd->ncdata = nc;
+  i = KBLK(cd, kpol, itype, 1);
+  IBLK(cd, kpol, itype) = i; NBLK(cd, kpol, itype) = 1; cd->jpblk[i] = 0;
+  if (cd->gen && nc > 0) {
+    double *c = (double *)malloc(sizeof(double) * nc);
+    int *k1 = (int *)malloc(sizeof(int) * nc * 4);
+    int *k2 = k1 + nc, *k3 = k2 + nc, *k4 = k3 + nc;
+    for (int q = 0; q < nc; ++q) {
+      int p = cd->IPT[q + 1];
+      c[q] = cd->CDATA[p]; k1[q] = cd->K1[p]; k2[q] = cd->K2[p]; k3[q] = cd->K3[p]; k4[q] = cd->K4[p];
+    }
+    cd->gen(cd->user, itype, kpol, nc, c, k1, k2, k3, k4);
+    free(c); free(k1);
+  } else if (cd->gen) {
+    cd->gen(cd->user, itype, kpol, 0, NULL, NULL, NULL, NULL, NULL);
+  }
+  return nc;
+}
+
+int orc_cdata_add_coef(orc_cdata *cd, double C, int kpol, int j1, int j2, int j3, int j4, int itype) {
+  int i = IBLK(cd, kpol, itype), ip = cd->ipblk[i], jp = cd->jpblk[i];
+  if (add_cdata(cd, ip, &jp, C, j1, j2, j3, j4) != 0) return -1;
+  cd->jpblk[i] = jp;
+  if (jp - ip + 1 < cd->mb) return 0;
+  int m = 0;
+  if (NBLK(cd, kpol, itype) < cd->kb) {
+    for (i = 1; i <= cd->nb; ++i) {
+      if (cd->jpblk[i] >= 0) continue;
+      IBLK(cd, kpol, itype) = i; cd->jpblk[i] = 0;
+      NBLK(cd, kpol, itype) = NBLK(cd, kpol, itype) + 1;
+      KBLK(cd, kpol, itype, NBLK(cd, kpol, itype)) = i;
+      m = 1; break;
+    }
+    if (m != 0) return 0;
+  }
+  int jtype = itype, jpol = kpol, n = NBLK(cd, kpol, itype);
+  for (i = 1; i <= cd->ntype; ++i)
+    for (int k = cd->kpol1; k <= cd->kpol2; ++k) {
+      if (NBLK(cd, k, i) <= n) continue;
+      jtype = i; jpol = k; n = NBLK(cd, jpol, jtype);
+    }
+  orc_cdata_add_matrix(cd, jtype, jpol);
+  if (jtype == itype && jpol == kpol) return 1;
+  m = 0;
+  for (i = 1; i <= cd->nb; ++i) {
+    if (cd->jpblk[i] >= 0) continue;
+    IBLK(cd, kpol, itype) = i; cd->jpblk[i] = 0;
+    NBLK(cd, kpol, itype) = NBLK(cd, kpol, itype) + 1;
+    KBLK(cd, kpol, itype, NBLK(cd, kpol, itype)) = i;
+    m = 1; break;
+  }
+  if (m == 0) return -2;  /* 'Add_coef: problems with new block' */
+  return 1;
+}
+
+/* ===================================================================== */
+/* Rk_evaluate / Sk_evaluate: DBSR_CI3/rk_evaluate.f90:2-63,              */
+/* sk_evaluate.f90:2-63.  List in IPT order; pq is (ns,2,norb) with       */
+/* 1-based orbital indices inside K1/K2.  R^k types 1..4 use 's' band,    */
+/* S^k types 5..6 use 'n' band.  HM(ldhm,*) and DM are 1-based by ic/jc.  */
+/* ===================================================================== */
+void orc_rk_evaluate(int ns, int ks, int kb, const double *rkb, int itype,
+                     int nc, const double *cdata, const int *K1, const int *K2,
+                     const int *K3, const int *K4, double eps_c, const double *pq, int ibi,
+                     double *HM, int ldhm, int nzero, double *DM) {
+  int ip1, ip2, jp1, jp2;
+  char sym;
+  switch (itype) {
+    case 1: ip1 = 1; ip2 = 1; jp1 = 1; jp2 = 1; sym = 's'; break;
+    case 2: ip1 = 2; ip2 = 2; jp1 = 2; jp2 = 2; sym = 's'; break;
+    case 3: ip1 = 1; ip2 = 1; jp1 = 2; jp2 = 2; sym = 's'; break;
+    case 4: ip1 = 2; ip2 = 2; jp1 = 1; jp2 = 1; sym = 's'; break;
+    case 5: ip1 = 1; ip2 = 2; jp1 = 1; jp2 = 2; sym = 'n'; break;
+    default: ip1 = 1; ip2 = 2; jp1 = 2; jp2 = 1; sym = 'n'; break;
+  }
+  int nb = (sym == 'n') ? 2 * ks - 1 : ks;
+  double *deni = (double *)malloc(sizeof(double) * (size_t)ns * nb * 3);
+  double *denj = deni + (size_t)ns * nb, *conv = denj + (size_t)ns * nb;
+#define PQ(ip, orb) (pq + (size_t)ns * (((ip) - 1) + 2 * (size_t)((orb) - 1)))
+  int ii = 0, jj = 0;
+  double S = 0.0;
+  for (int j = 0; j < nc; ++j) {
+    if (fabs(cdata[j]) < eps_c) continue;
+    if (K1[j] != ii) {
+      int i1 = K1[j] / ibi, i2 = K1[j] % ibi;
+      orc_density(ns, ks, deni, PQ(ip1, i1), PQ(ip2, i2), sym);
+      orc_convol(ns, ks, kb, rkb, conv, deni, 2, sym, sym);
+    }
+    if (K2[j] != jj) {
+      int j1 = K2[j] / ibi, j2 = K2[j] % ibi;
+      orc_density(ns, ks, denj, PQ(jp1, j1), PQ(jp2, j2), sym);
+    }
+    if (K1[j] != ii || K2[j] != jj) S = orc_sum_amb(ns, ks, conv, denj, sym);
+    ii = K1[j]; jj = K2[j];
+    double SS = S * cdata[j];
+    int ic = K3[j], jc = K4[j];
+    if (jc <= nzero) HM[(ic - 1) + (size_t)ldhm * (jc - 1)] += SS;
+    else if (ic == jc) DM[jc - 1] += SS;
+  }
+#undef PQ
+  free(deni);
+}
+
+/* ===================================================================== */
+/* R_data itype 3 (direct) and 4 (exchange): DBSR_MAT3/R_data.f90:214-277 */
+/* atype 0..3 selects mrk_* and the P/Q components (R_data.f90:76-86).    */
+/* hch is (ms,ms,nblocks); icc(ich,jch) (nch x nch, column-major, 1-based */
+/* values, 0 = absent) gives the block.                                    */
+/* ===================================================================== */
+void orc_rdata_34(int ns, int ks, const double *rkb, int itype, int atype,
+                  int nc, const double *cdata, const int *K1, const int *K2,
+                  const int *K3, const double *pq, int ibo, const int *icc,
+                  int nch, double *hch) {
+  int ip1, ip2, jp1, jp2, icase;
+  char sym_d, sym_r;
+  if (itype == 3) { icase = 1; sym_d = 's'; sym_r = 's'; }
+  else { icase = 3; sym_d = 'x'; sym_r = 'x'; }
+  switch (atype) {
+    case 0: ip1 = 1; ip2 = 1; jp1 = 1; jp2 = 1; break;
+    case 1: ip1 = 2; ip2 = 2; jp1 = 2; jp2 = 2; break;
+    case 2: ip1 = 1; ip2 = 1; jp1 = 2; jp2 = 2;
+            if (itype == 4) { ip1 = 1; ip2 = 2; jp1 = 1; jp2 = 2; } break;
+    default: ip1 = 2; ip2 = 2; jp1 = 1; jp2 = 1;
+             if (itype == 4) { ip1 = 2; ip2 = 1; jp1 = 2; jp2 = 1; } break;
+  }
+  const int ms = 2 * ns;
+  size_t nn = (size_t)ns * ns;
+  double *xx = (double *)calloc(nn * 3, sizeof(double));
+  double *dd = xx + nn, *cv = dd + nn;
+#define PQ(ip, orb) (pq + (size_t)ns * (((ip) - 1) + 2 * (size_t)((orb) - 1)))
+  for (int j = 0; j < nc; ++j) {
+    int j1 = K2[j], j2 = K3[j];
+    orc_density(ns, ks, dd, PQ(jp1, j1), PQ(jp2, j2), sym_d);
+    size_t cols = (sym_d == 's') ? (size_t)ns * ks : nn;
+    for (size_t q = 0; q < cols; ++q) xx[q] = xx[q] + cdata[j] * dd[q];
+    int last = 1;
+    if (j < nc - 1 && K1[j] == K1[j + 1]) last = 0;
+    if (!last) continue;
+    orc_convol(ns, ks, ks, rkb, cv, xx, icase, 's', 's');
+    int ich = K1[j] / ibo, jch = K1[j] % ibo;
+    int b = icc[(ich - 1) + (size_t)nch * (jch - 1)];
+    if (b > 0) orc_update_hs_mat(ns, ks, hch + (size_t)ms * ms * (b - 1), ich, jch, cv, sym_r, ip1, ip2);
+    memset(xx, 0, sizeof(double) * nn);
+  }
+#undef PQ
+  free(xx);
+}
+
+/* Idef_Rtype for the channel-channel structures (ic=jc=io=jo=0):
+ * DBSR_MAT3/idef_type.f90:142-302.  ipbs(orbital) > 0 marks a channel
+ * orbital (1-based orbital index; ipbs[0] unused).  Returns itype (3 or 4),
+ * or 0 when the reference would stop with 'Idef_Rtype: itype=0'. */
+int orc_idef_rtype(int j1, int j2, int j3, int j4, const int *ipbs, int ibo,
+                   int *k1, int *k2, int *k3, int *k4) {
+  int i1, i2, i3, i4, t, itype = 0;
+  if (ipbs[j1] + ipbs[j2] == 0 || ipbs[j3] + ipbs[j4] == 0) {
+    i1 = j1; i3 = j3; if (j1 > j3) { i1 = j3; i3 = j1; }
+    i2 = j2; i4 = j4; if (j2 > j4) { i2 = j4; i4 = j2; }
+    if (i1 > i2) { t = i1; i1 = i2; i2 = t; t = i3; i3 = i4; i4 = t; }
+  } else {
+    i1 = j1; i2 = j2; i3 = j3; i4 = j4;
+  }
+  if (ipbs[i1] > 0 && ipbs[i3] > 0) {
+    if (ipbs[i1] >= ipbs[i3]) {
+      *k1 = ipbs[i1] * ibo + ipbs[i3]; *k2 = i2; *k3 = i4; *k4 = 0; itype = 3;
+      if (*k2 > *k3) { *k2 = i4; *k3 = i2; }
+    } else {
+      *k1 = ipbs[i3] * ibo + ipbs[i1]; *k2 = i4; *k3 = i2; *k4 = 0; itype = 3;
+      if (*k2 > *k3) { *k2 = i2; *k3 = i4; }
+    }
+  } else if (ipbs[i2] > 0 && ipbs[i4] > 0) {
+    if (ipbs[i2] >= ipbs[i4]) {
+      *k1 = ipbs[i2] * ibo + ipbs[i4]; *k2 = i1; *k3 = i3; *k4 = 0; itype = 3;
+      if (*k2 > *k3) { *k2 = i3; *k3 = i1; }
+    } else {
+      *k1 = ipbs[i4] * ibo + ipbs[i2]; *k2 = i3; *k3 = i1; *k4 = 0; itype = 3;
+      if (*k2 > *k3) { *k2 = i1; *k3 = i3; }
+    }
+  } else if (ipbs[i1] > 0 && ipbs[i4] > 0) {
+    if (ipbs[i1] >= ipbs[i4]) { *k1 = ipbs[i1] * ibo + ipbs[i4]; *k2 = i3; *k3 = i2; *k4 = 0; itype = 4; }
+    else { *k1 = ipbs[i4] * ibo + ipbs[i1]; *k2 = i2; *k3 = i3; *k4 = 0; itype = 4; }
+  } else if (ipbs[i2] > 0 && ipbs[i3] > 0) {
+    if (ipbs[i3] >= ipbs[i2]) { *k1 = ipbs[i3] * ibo + ipbs[i2]; *k2 = i1; *k3 = i4; *k4 = 0; itype = 4; }
+    else { *k1 = ipbs[i2] * ibo + ipbs[i3]; *k2 = i4; *k3 = i1; *k4 = 0; itype = 4; }
+  }
+  return itype;
+}
+
+/* Decode_int: DBSR_CI3/setup_matrix.f90:206-223 */
+void orc_decode_int(int code, int *met, int *k, int *i1, int *i2, int *i3, int *i4) {
+  int ii = code;
+  *met = ii % 4; ii /= 4;
+  *k = ii % 1024; ii /= 1024;
+  *i4 = ii % 32; ii /= 32;
+  *i3 = ii % 32; ii /= 32;
+  *i2 = ii % 32; *i1 = ii / 32;
+}

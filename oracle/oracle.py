@@ -1,0 +1,330 @@
+"""ctypes front end to the oracle (TEST INFRASTRUCTURE ONLY).
+
+Loads ``oracle/liboracle.so`` -- the C restatement of the reference's
+B-spline Slater-integral algorithms (``oracle/dbs_oracle.c``) -- and, where it
+was built, ``oracle/_ref/libdbsref.so`` -- the reference's own Fortran
+(``DBSR_HF/dbsr_lib_dbs.f90``, ``dbsr_lib_zcom.f90``, ``DBSR_MAT3/MOD_c_data.f90``)
+compiled from its sources.  Only tests, ``__graft_entry__.smoke()`` and
+``bench.py``'s CPU-baseline leg import this module, and only as the checker.
+
+Arrays are numpy float64 in Fortran (column-major) order with the reference's
+dimensions.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libdbsref.so")
+
+PPPP, QQQQ, PQPQ, QPQP, PPQQ, PQQP = 1, 2, 3, 4, 5, 6
+MOM_PP, MOM_QQ, MOM_PQ, MOM_QP = 0, 1, 2, 3
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+def f64(a):
+    return np.asfortranarray(np.asarray(a, dtype=np.float64))
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle not built: {LIB_PATH} (run make -C oracle)")
+        L = C.CDLL(LIB_PATH)
+        L.orc_grid_new.restype = C.c_void_p
+        L.orc_grid_new.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _dp]
+        L.orc_grid_delete.argtypes = [C.c_void_p]
+        L.orc_grid_dims.argtypes = [C.c_void_p, _ip]
+        L.orc_grid_tables.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+        L.orc_moments.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp]
+        L.orc_triang.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, _dp]
+        L.orc_mrk.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp]
+        L.orc_rkb_size.restype = C.c_size_t
+        L.orc_rkb_size.argtypes = [C.c_void_p, C.c_int]
+        L.orc_density.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, C.c_char]
+        L.orc_convol.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, C.c_int, C.c_char, C.c_char]
+        L.orc_sum_amb.restype = C.c_double
+        L.orc_sum_amb.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_char]
+        L.orc_quadr.restype = C.c_double
+        L.orc_quadr.argtypes = [C.c_void_p, _dp, _dp, C.c_int]
+        L.orc_update_hs_lib.argtypes = [C.c_int, _dp, C.c_int, C.c_int, C.c_int, C.c_int, _dp, C.c_char]
+        L.orc_update_hs_mat.argtypes = [C.c_int, C.c_int, _dp, C.c_int, C.c_int, _dp, C.c_char, C.c_int, C.c_int]
+        L.orc_rk.restype = C.c_double
+        L.orc_rk.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_gauleg.argtypes = [C.c_double, C.c_double, _dp, _dp, C.c_int]
+        L.orc_mkgrid_01.restype = C.c_int
+        L.orc_mkgrid_01.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                    C.c_double, C.c_double, _dp, C.c_int]
+        L.orc_mkgrid_02.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, _dp]
+        L.orc_rk_evaluate.argtypes = [C.c_int, C.c_int, C.c_int, _dp, C.c_int, C.c_int, _dp,
+                                      _ip, _ip, _ip, _ip, C.c_double, _dp, C.c_int,
+                                      _dp, C.c_int, C.c_int, _dp]
+        L.orc_rdata_34.argtypes = [C.c_int, C.c_int, _dp, C.c_int, C.c_int, C.c_int, _dp,
+                                   _ip, _ip, _ip, _dp, C.c_int, _ip, C.c_int, _dp]
+        L.orc_idef_rtype.restype = C.c_int
+        L.orc_idef_rtype.argtypes = [C.c_int] * 4 + [_ip, C.c_int, _ip, _ip, _ip, _ip]
+        L.orc_decode_int.argtypes = [C.c_int] + [_ip] * 6
+        _lib = L
+    return _lib
+
+
+# --------------------------------------------------------------------------
+# grids
+# --------------------------------------------------------------------------
+def gauleg(n):
+    x = np.zeros(n)
+    w = np.zeros(n)
+    lib().orc_gauleg(0.0, 1.0, _d(x), _d(w), n)
+    return x, w
+
+
+def mkgrid_01(ks, ml, hi, he, hmax, rmax, z):
+    ns = lib().orc_mkgrid_01(ks, ml, hi, he, hmax, rmax, z, None, 0)
+    t = np.zeros(ns + ks)
+    lib().orc_mkgrid_01(ks, ml, hi, he, hmax, rmax, z, _d(t), ns + ks)
+    return t, ns
+
+
+def mkgrid_02(ns, ks, hi, rmax, z):
+    t = np.zeros(ns + ks)
+    lib().orc_mkgrid_02(ns, ks, hi, rmax, z, _d(t))
+    return t
+
+
+class Grid:
+    """B-spline grid + Gauss tables (alloc_DBS_gauss restated)."""
+
+    def __init__(self, t, ns, ks, ksp=0, ksq=0):
+        self.t = f64(t)
+        h = lib().orc_grid_new(ns, ks, ksp, ksq, _d(self.t))
+        if not h:
+            raise ValueError("bad grid parameters")
+        self.h = C.c_void_p(h)
+        dims = np.zeros(8, dtype=np.int32)
+        lib().orc_grid_dims(self.h, _i(dims))
+        (self.ns, self.ks, self.ksp, self.ksq, self.nv, self.nsp, self.nsq, self.ms) = map(int, dims)
+        self._cache = {}
+
+    def __del__(self):
+        try:
+            lib().orc_grid_delete(self.h)
+        except Exception:
+            pass
+
+    def tables(self):
+        nv, ks = self.nv, self.ks
+        gr = np.zeros((nv, ks), order="F")
+        grw = np.zeros((nv, ks), order="F")
+        pb = np.zeros((nv + 1, ks, self.ksp), order="F")
+        qb = np.zeros((nv + 1, ks, self.ksq), order="F")
+        lib().orc_grid_tables(self.h, _d(gr), _d(grw), _d(pb), _d(qb))
+        return gr, grw, pb, qb
+
+    def moments(self, which, k):
+        r = np.zeros((self.ks * self.ks, self.nv), order="F")
+        lib().orc_moments(self.h, which, k, _d(r))
+        return r
+
+    def triang(self, itype, k, iv):
+        kk = self.ks * self.ks
+        r = np.zeros((kk, kk), order="F")
+        lib().orc_triang(self.h, itype, k, iv, _d(r))
+        return r
+
+    def kb(self, itype):
+        return self.ks if itype <= 4 else 2 * self.ks - 1
+
+    def mrk(self, itype, k, cache=True):
+        key = (itype, k)
+        if cache and key in self._cache:
+            return self._cache[key]
+        kb = self.kb(itype)
+        r = np.zeros((self.ns, self.ns, kb, kb), order="F")
+        lib().orc_mrk(self.h, itype, k, _d(r))
+        if cache:
+            self._cache[key] = r
+        return r
+
+    def quadr(self, fi, fj, m):
+        return lib().orc_quadr(self.h, _d(f64(fi)), _d(f64(fj)), m)
+
+    def rk(self, f1, f2, f3, f4, k):
+        a = [self.mrk(t, k) for t in (PPPP, QQQQ, PQPQ, QPQP)]
+        return lib().orc_rk(self.h, _d(a[0]), _d(a[1]), _d(a[2]), _d(a[3]),
+                            _d(f64(f1)), _d(f64(f2)), _d(f64(f3)), _d(f64(f4)))
+
+
+def density(ns, ks, p1, p2, typ):
+    ncol = {"s": ks, "u": ks, "l": ks, "n": 2 * ks - 1}.get(typ, ns)
+    d = np.zeros((ns, ncol), order="F")
+    lib().orc_density(ns, ks, _d(d), _d(f64(p1)), _d(f64(p2)), typ.encode())
+    return d
+
+
+def convol(ns, ks, rkb, d, icase, si, sj):
+    kb = rkb.shape[2]
+    if icase <= 2:
+        ncol = 2 * ks - 1 if "n" in (si, sj) else ks
+        a = np.zeros((ns, max(ncol, ks)), order="F")
+    else:
+        a = np.zeros((ns, ns), order="F")
+    dd = f64(d)
+    if dd.shape[1] < a.shape[1] and icase <= 2:
+        pad = np.zeros((ns, a.shape[1]), order="F")
+        pad[:, : dd.shape[1]] = dd
+        dd = pad
+    lib().orc_convol(ns, ks, kb, _d(rkb), _d(a), _d(dd), icase, si.encode(), sj.encode())
+    return a
+
+
+def sum_amb(ns, ks, a, b, sym):
+    return lib().orc_sum_amb(ns, ks, _d(f64(a)), _d(f64(b)), sym.encode())
+
+
+def update_hs_lib(hm, ii, jj, ns, ks, d, sym):
+    ms = hm.shape[0]
+    lib().orc_update_hs_lib(ms, _d(hm), ii, jj, ns, ks, _d(f64(d)), sym.encode())
+
+
+def update_hs_mat(blk, ns, ks, ich, jch, d, sym, ii, jj):
+    lib().orc_update_hs_mat(ns, ks, _d(blk), ich, jch, _d(f64(d)), sym.encode(), ii, jj)
+
+
+def rk_evaluate(g, itype, k, lst, pq, ibi, HM, nzero, DM, eps_c=1e-10):
+    """lst = (c, K1, K2, K3, K4) in IPT order; pq (ns,2,norb); HM/DM updated in place."""
+    c, K1, K2, K3, K4 = lst
+    rkb = g.mrk(itype, k)
+    ci = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+    K1, K2, K3, K4 = ci(K1), ci(K2), ci(K3), ci(K4)
+    cc = np.ascontiguousarray(c, dtype=np.float64)
+    lib().orc_rk_evaluate(g.ns, g.ks, rkb.shape[2], _d(rkb), itype, len(cc), _d(cc),
+                          _i(K1), _i(K2), _i(K3), _i(K4), eps_c, _d(f64(pq)), ibi,
+                          _d(HM), HM.shape[0], nzero, _d(DM))
+
+
+def rdata_34(g, itype, atype, k, lst, pq, ibo, icc, hch):
+    c, K1, K2, K3 = lst
+    rkb = g.mrk({0: PPPP, 1: QQQQ, 2: PQPQ, 3: QPQP}[atype], k)
+    ci = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+    icc_f = np.asfortranarray(icc, dtype=np.int32)
+    cc = np.ascontiguousarray(c, dtype=np.float64)
+    lib().orc_rdata_34(g.ns, g.ks, _d(rkb), itype, atype, len(cc), _d(cc), _i(ci(K1)),
+                       _i(ci(K2)), _i(ci(K3)), _d(f64(pq)), ibo, _i(icc_f), icc.shape[0], _d(hch))
+
+
+def idef_rtype(j1, j2, j3, j4, ipbs, ibo):
+    ip = np.ascontiguousarray(ipbs, dtype=np.int32)
+    out = [np.zeros(1, dtype=np.int32) for _ in range(4)]
+    it = lib().orc_idef_rtype(j1, j2, j3, j4, _i(ip), ibo, *[_i(o) for o in out])
+    return it, tuple(int(o[0]) for o in out)
+
+
+def decode_int(code):
+    out = [np.zeros(1, dtype=np.int32) for _ in range(6)]
+    lib().orc_decode_int(int(code), *[_i(o) for o in out])
+    return tuple(int(o[0]) for o in out)
+
+
+# --------------------------------------------------------------------------
+# c_data (DBSR_MAT3/MOD_c_data.f90 restated in dbs_oracle.c)
+# --------------------------------------------------------------------------
+GEN_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_int, C.c_int, _dp, _ip, _ip, _ip, _ip)
+
+
+class CData:
+    """Blocked sorted coefficient lists.  ``gen(itype, kpol, (c,K1,K2,K3,K4))``
+    is called by Add_matrix with the merged list in IPT order."""
+
+    def __init__(self, ntype, kp1, kp2, mblock, nblock, kblock, eps_c, gen):
+        L = lib()
+        L.orc_cdata_alloc.restype = C.c_void_p
+        L.orc_cdata_alloc.argtypes = [C.c_int] * 6 + [C.c_double, GEN_FN, C.c_void_p]
+        L.orc_cdata_free.argtypes = [C.c_void_p]
+        L.orc_cdata_add_coef.restype = C.c_int
+        L.orc_cdata_add_coef.argtypes = [C.c_void_p, C.c_double] + [C.c_int] * 6
+        L.orc_cdata_add_matrix.restype = C.c_int
+        L.orc_cdata_add_matrix.argtypes = [C.c_void_p, C.c_int, C.c_int]
+
+        def _cb(user, itype, kpol, nc, c, k1, k2, k3, k4):
+            if nc > 0:
+                arrs = (np.ctypeslib.as_array(c, (nc,)).copy(),) + tuple(
+                    np.ctypeslib.as_array(p, (nc,)).copy() for p in (k1, k2, k3, k4))
+            else:
+                arrs = (np.zeros(0),) + tuple(np.zeros(0, dtype=np.int32) for _ in range(4))
+            gen(itype, kpol, arrs)
+
+        self._cb = GEN_FN(_cb)
+        self.h = C.c_void_p(L.orc_cdata_alloc(ntype, kp1, kp2, mblock, nblock, kblock, eps_c,
+                                                self._cb, None))
+
+    def add_coef(self, c, kpol, j1, j2, j3, j4, itype):
+        r = lib().orc_cdata_add_coef(self.h, c, kpol, j1, j2, j3, j4, itype)
+        if r < 0:
+            raise RuntimeError(" Add_coef: problems with new block ")
+
+    def add_matrix(self, itype, kpol):
+        lib().orc_cdata_add_matrix(self.h, itype, kpol)
+
+    def __del__(self):
+        try:
+            lib().orc_cdata_free(self.h)
+        except Exception:
+            pass
+
+
+# --------------------------------------------------------------------------
+# the compiled reference (oracle/_ref), when present
+# --------------------------------------------------------------------------
+_ref = None
+
+
+def ref_available():
+    return os.path.exists(REF_PATH)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        if not ref_available():
+            raise RuntimeError("reference build missing: " + REF_PATH)
+        L = C.CDLL(REF_PATH)
+        L.ref_setup.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _dp]
+        L.ref_tables.argtypes = [_dp, _dp, _dp, _dp]
+        L.ref_gauleg.argtypes = [C.c_int, _dp, _dp]
+        L.ref_moments.argtypes = [C.c_int, C.c_int, _dp]
+        L.ref_triang.argtypes = [C.c_int, C.c_int, C.c_int, _dp]
+        L.ref_mrk.argtypes = [C.c_int, C.c_int, _dp]
+        L.ref_convol.argtypes = [C.c_int, C.c_int, C.c_int, C.c_char, C.c_char, _dp, _dp]
+        L.ref_density.argtypes = [_dp, _dp, C.c_char, _dp]
+        L.ref_sum_amb.restype = C.c_double
+        L.ref_sum_amb.argtypes = [_dp, _dp, C.c_char]
+        L.ref_rk.restype = C.c_double
+        L.ref_rk.argtypes = [_dp, _dp, _dp, _dp, C.c_int]
+        L.ref_quadr.restype = C.c_double
+        L.ref_quadr.argtypes = [_dp, _dp, C.c_int]
+        L.ref_update_hs.argtypes = [_dp, C.c_int, C.c_int, _dp, C.c_char]
+        L.ref_cdata_alloc.argtypes = [C.c_int] * 6 + [C.c_double, C.c_int]
+        L.ref_cdata_add.argtypes = [C.c_double] + [C.c_int] * 6
+        L.ref_cdata_flush.argtypes = [C.c_int, C.c_int]
+        L.ref_cdata_nlog.restype = C.c_int
+        L.ref_cdata_get.argtypes = [C.c_int] + [_ip] * 6 + [_dp, _ip]
+        _ref = L
+    return _ref

@@ -1,0 +1,195 @@
+!======================================================================
+! ref_driver.f90 -- C-callable wrappers around the reference's own DBS
+! library (DBSR_HF/dbsr_lib_dbs.f90 + dbsr_lib_zcom.f90) and its c_data
+! module (DBSR_MAT3/MOD_c_data.f90), compiled from where those sources lie
+! by oracle/Makefile into oracle/_ref/libdbsref.so.
+!
+! TEST INFRASTRUCTURE ONLY: used to pin the C restatement (dbs_oracle.c)
+! against the reference's compiled code.  Never linked into the product.
+!
+! The wrappers set the reference's module state directly (DBS_grid) instead
+! of going through def_grid, which reads knot files.
+!======================================================================
+      Subroutine ref_setup(ns_, ks_, ksp_, ksq_, t_) bind(C, name='ref_setup')
+      Use iso_c_binding
+      Use DBS_grid
+      Implicit none
+      Integer(c_int), value :: ns_, ks_, ksp_, ksq_
+      Real(c_double), intent(in) :: t_(ns_ + ks_)
+      ns = ns_; ks = ks_; ksp = ksp_; ksq = ksq_
+      ms = ns + ns; nv = ns - ks + 1; nsp = nv + ksp - 1; nsq = nv + ksq - 1
+      me = 0
+      if (allocated(t)) Deallocate(t)
+      Allocate(t(ns + ks)); t = t_
+      tmax = t(ns + 1)
+      Call alloc_DBS_gauss
+      Call dealloc_DBS_integrals
+      Call dealloc_DBS_sk_integrals
+      Call alloc_DBS_moments
+      End Subroutine ref_setup
+
+      Subroutine ref_tables(gr_, grw_, pbsp_, qbsp_) bind(C, name='ref_tables')
+      Use iso_c_binding
+      Use DBS_grid
+      Use DBS_gauss
+      Implicit none
+      Real(c_double) :: gr_(nv, ks), grw_(nv, ks), pbsp_(nv + 1, ks, ksp), qbsp_(nv + 1, ks, ksq)
+      gr_ = gr; grw_ = grw; pbsp_ = pbsp; qbsp_ = qbsp
+      End Subroutine ref_tables
+
+      Subroutine ref_gauleg(n, x, w) bind(C, name='ref_gauleg')
+      Use iso_c_binding
+      Implicit none
+      Integer(c_int), value :: n
+      Real(c_double) :: x(n), w(n)
+      Call gauleg(0.d0, 1.d0, x, w, n)
+      End Subroutine ref_gauleg
+
+      Subroutine ref_moments(which, k, rkm) bind(C, name='ref_moments')
+      Use iso_c_binding
+      Use DBS_grid
+      Implicit none
+      Integer(c_int), value :: which, k
+      Real(c_double) :: rkm(ks * ks, nv)
+      Select case (which)
+       Case (0); Call moments_pp(k, ks * ks, nv, rkm)
+       Case (1); Call moments_qq(k, ks * ks, nv, rkm)
+       Case (2); Call moments_pq(k, ks * ks, nv, rkm)
+       Case default; Call moments_qp(k, ks * ks, nv, rkm)
+      End Select
+      End Subroutine ref_moments
+
+      Subroutine ref_triang(itype, k, iv, rkd_out) bind(C, name='ref_triang')
+      Use iso_c_binding
+      Use DBS_grid
+      Use DBS_moments
+      Implicit none
+      Integer(c_int), value :: itype, k, iv
+      Real(c_double) :: rkd_out(ks * ks, ks * ks)
+      rkd(:, :, iv) = 0.d0
+      Select case (itype)
+       Case (1); Call triang_pppp(k, iv)
+       Case (2); Call triang_qqqq(k, iv)
+       Case (3); Call triang_pqpq(k, iv)
+       Case (4); Call triang_qpqp(k, iv)
+       Case (5); Call triang_ppqq(k, iv)
+       Case default; Call triang_pqqp(k, iv)
+      End Select
+      rkd_out = rkd(:, :, iv)
+      End Subroutine ref_triang
+
+      Subroutine ref_select(itype, k) bind(C, name='ref_select')
+      Use iso_c_binding
+      Implicit none
+      Integer(c_int), value :: itype, k
+      Integer :: it, kk
+      it = itype; kk = k
+      Call ref_sel(it, kk)
+      End Subroutine ref_select
+
+      Subroutine ref_sel(itype, k)
+      Implicit none
+      Integer :: itype, k
+      Select case (itype)
+       Case (1); Call mrk_pppp(k)
+       Case (2); Call mrk_qqqq(k)
+       Case (3); Call mrk_pqpq(k)
+       Case (4); Call mrk_qpqp(k)
+       Case (5); Call msk_ppqq(k)
+       Case default; Call msk_pqqp(k)
+      End Select
+      End Subroutine ref_sel
+
+      Subroutine ref_mrk(itype, k, rkb_out) bind(C, name='ref_mrk')
+      Use iso_c_binding
+      Use DBS_grid
+      Use DBS_integrals, only: rkb_r => rkb
+      Use DBS_sk_integrals, only: rkb_s => rkb
+      Implicit none
+      Integer(c_int), value :: itype, k
+      Real(c_double) :: rkb_out(*)
+      Integer :: n
+      Call ref_sel(int(itype), int(k))
+      if (itype <= 4) then
+       n = ns * ns * ks * ks
+       rkb_out(1:n) = reshape(rkb_r, (/n/))
+      else
+       n = ns * ns * (2 * ks - 1) * (2 * ks - 1)
+       rkb_out(1:n) = reshape(rkb_s, (/n/))
+      end if
+      End Subroutine ref_mrk
+
+      Subroutine ref_convol(itype, k, icase, si, sj, d, a) bind(C, name='ref_convol')
+      Use iso_c_binding
+      Use DBS_grid
+      Implicit none
+      Integer(c_int), value :: itype, k, icase
+      Character(kind=c_char), value :: si, sj
+      Real(c_double) :: d(ns, *), a(ns, *)
+      Character :: ci, cj
+      ci = si; cj = sj
+      Call ref_sel(int(itype), int(k))
+      if (itype <= 4) then
+       Call convol(ns, ks, a, d, icase, ci, cj)
+      else
+       Call convol_sk(ns, ks, a, d, icase, ci, cj)
+      end if
+      End Subroutine ref_convol
+
+      Subroutine ref_density(p1, p2, typ, d) bind(C, name='ref_density')
+      Use iso_c_binding
+      Use DBS_grid
+      Implicit none
+      Real(c_double) :: p1(ns), p2(ns), d(ns, *)
+      Character(kind=c_char), value :: typ
+      Character :: c
+      c = typ
+      Call density(ns, ks, d, p1, p2, c)
+      End Subroutine ref_density
+
+      Function ref_sum_amb(a, b, sym) result(s) bind(C, name='ref_sum_amb')
+      Use iso_c_binding
+      Use DBS_grid
+      Implicit none
+      Real(c_double) :: a(ns, *), b(ns, *)
+      Character(kind=c_char), value :: sym
+      Real(c_double) :: s
+      Character(1) :: c
+      Real(8), external :: SUM_AmB
+      c = sym
+      s = SUM_AmB(ns, ks, a, b, c)
+      End Function ref_sum_amb
+
+      Function ref_rk(f1, f2, f3, f4, k) result(s) bind(C, name='ref_rk')
+      Use iso_c_binding
+      Use DBS_grid
+      Implicit none
+      Real(c_double) :: f1(ns, 2), f2(ns, 2), f3(ns, 2), f4(ns, 2)
+      Integer(c_int), value :: k
+      Real(c_double) :: s
+      Real(8), external :: rk
+      s = rk(f1, f2, f3, f4, k)
+      End Function ref_rk
+
+      Function ref_quadr(fi, fj, m) result(s) bind(C, name='ref_quadr')
+      Use iso_c_binding
+      Use DBS_grid
+      Implicit none
+      Real(c_double) :: fi(ns, 2), fj(ns, 2)
+      Integer(c_int), value :: m
+      Real(c_double) :: s
+      Real(8), external :: quadr
+      s = quadr(fi, fj, m)
+      End Function ref_quadr
+
+      Subroutine ref_update_hs(hm, ii, jj, d, sym) bind(C, name='ref_update_hs')
+      Use iso_c_binding
+      Use DBS_grid
+      Implicit none
+      Real(c_double) :: hm(ms, ms), d(ns, *)
+      Integer(c_int), value :: ii, jj
+      Character(kind=c_char), value :: sym
+      Character(1) :: c
+      c = sym
+      Call UPDATE_HS(ms, hm, ii, jj, ns, ks, d, c)
+      End Subroutine ref_update_hs
