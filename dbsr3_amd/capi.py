@@ -1,0 +1,114 @@
+"""ctypes bindings of libdbsr3_amd.so (the C-ABI declared in include/dbsr3_amd.h).
+
+The library is built in-tree by ``dbsr3_amd.build``.  Loading fails loudly
+when it is missing: there is no CPU fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdbsr3_amd.so")
+
+vp = C.c_void_p
+dp = C.POINTER(C.c_double)
+ip = C.POINTER(C.c_int)
+i32 = C.c_int
+f64 = C.c_double
+ch = C.c_char
+sz = C.c_size_t
+
+GEN_MATRIX_FN = C.CFUNCTYPE(None, vp, i32, i32, i32, dp, ip, ip, ip, ip)
+
+# name -> argtypes (all return int status unless listed in _RESTYPE)
+PROTOTYPES = {
+    "dbs_version": [],
+    "dbs_ctx_create": [C.POINTER(vp), i32, i32, i32, i32, dp, i32],
+    "dbs_ctx_destroy": [vp],
+    "dbs_ctx_dims": [vp, ip],
+    "dbs_ctx_sync": [vp],
+    "dbs_ctx_tables": [vp, dp, dp, dp, dp],
+    "dbs_ctx_set_cache_limit": [vp, sz],
+    "dbs_malloc": [vp, C.POINTER(vp), sz],
+    "dbs_free": [vp, vp],
+    "dbs_memcpy_h2d": [vp, vp, vp, sz],
+    "dbs_memcpy_d2h": [vp, vp, vp, sz],
+    "dbs_moments_dev": [vp, i32, i32, vp],
+    "dbs_diag_cells_dev": [vp, i32, i32, vp],
+    "dbs_mrk": [vp, i32, i32, C.POINTER(vp)],
+    "dbs_mrk_host": [vp, i32, i32, dp],
+    "dbs_density_dev": [vp, vp, vp, ch, vp],
+    "dbs_convol_dev": [vp, i32, i32, i32, ch, ch, vp, vp],
+    "dbs_sum_amb_dev": [vp, vp, vp, ch, vp],
+    "dbs_rk_dev": [vp, vp, vp, vp, vp, i32, vp],
+    "dbs_quadr_dev": [vp, vp, vp, i32, vp],
+    "dbs_update_hs_dev": [vp, vp, i32, i32, vp, ch],
+    "dbs_density": [vp, dp, dp, ch, dp],
+    "dbs_convol": [vp, i32, i32, i32, ch, ch, dp, dp],
+    "dbs_sum_amb": [vp, dp, dp, ch, dp],
+    "dbs_rk": [vp, dp, dp, dp, dp, i32, dp],
+    "dbs_quadr": [vp, dp, dp, i32, dp],
+    "dbs_cdata_create": [C.POINTER(vp), i32, i32, i32, i32, i32, i32, f64, GEN_MATRIX_FN, vp],
+    "dbs_cdata_destroy": [vp],
+    "dbs_cdata_add_coef": [vp, f64, i32, i32, i32, i32, i32, i32],
+    "dbs_cdata_add_matrix": [vp, i32, i32],
+    "dbs_add_integral_ci": [vp, i32, i32, i32, i32, i32, i32, i32, f64, ip, i32],
+    "dbs_add_integral_mat": [vp, i32, i32, i32, i32, i32, f64, ip, ip, i32, i32, f64],
+    "dbs_decode_int": [i32, ip, ip, ip, ip, ip, ip, ip],
+    "dbs_rk_evaluate_dev": [vp, i32, i32, i32, dp, ip, ip, ip, ip, f64, i32, vp, i32, vp, i32,
+                            i32, vp],
+    "dbs_rdata_34_dev": [vp, i32, i32, i32, i32, dp, ip, ip, ip, i32, vp, i32, ip, i32, vp, i32],
+    "dbs_engine_create": [C.POINTER(vp), i32, C.POINTER(vp), ip, ip, ip, i32],
+    "dbs_engine_destroy": [vp],
+    "dbs_engine_buffers": [vp, i32, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp),
+                           C.POINTER(vp)],
+    "dbs_engine_forward": [vp],
+    "dbs_engine_forward_adjoint": [vp],
+    "dbs_engine_graph_capture": [vp, i32],
+    "dbs_engine_graph_launch": [vp],
+    "dbs_engine_set_tuning": [vp, i32, ip],
+    "dbs_engine_get_tuning": [vp, i32, ip],
+    "dbs_engine_autotune": [vp, i32, i32],
+    "dbs_engine_work": [vp, i32, dp, dp],
+    "dbs_engine_sync": [vp],
+}
+
+_lib = None
+
+
+class DBSError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load the HIP library; raises if it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise DBSError(
+                f"libdbsr3_amd.so not found at {LIB_PATH}; build it with "
+                "`python -m dbsr3_amd.build` (hipcc, gfx950)")
+        L = C.CDLL(LIB_PATH)
+        L.dbs_last_error.restype = C.c_char_p
+        L.dbs_last_error.argtypes = []
+        for name, args in PROTOTYPES.items():
+            f = getattr(L, name, None)
+            if f is None:  # reported by tests/test_capi.py
+                continue
+            f.restype = C.c_int
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        msg = lib().dbs_last_error().decode(errors="replace")
+        raise DBSError(f"status {rc}: {msg}")
+    return rc
+
+
+def exported_symbols():
+    """Every function name the public header declares."""
+    return ["dbs_last_error"] + list(PROTOTYPES)

@@ -1,0 +1,99 @@
+// dbs_internal.h -- shared definitions for the MI355X B-spline Slater-integral
+// library (libdbsr3_amd.so).  Host runtime + HIP kernels for gfx950.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <map>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "dbsr3_amd.h"
+
+#define DBS_MAXK 16  // largest B-spline order supported by fixed-size device scratch
+
+namespace dbs {
+
+// ---------------------------------------------------------------------------
+// error handling: every C entry point returns a status; the last message is
+// kept per thread and returned by dbs_last_error().
+// ---------------------------------------------------------------------------
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+
+#define DBS_HIP(call)                                                          \
+  do {                                                                         \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      return ::dbs::fail(DBS_ERR_HIP, std::string(#call) + ": " +              \
+                                          hipGetErrorString(e_));              \
+  } while (0)
+
+#define DBS_CHECK(cond, code, msg)                                             \
+  do {                                                                         \
+    if (!(cond)) return ::dbs::fail((code), (msg));                            \
+  } while (0)
+
+// device buffer owned by the library
+struct DevBuf {
+  double* p = nullptr;
+  size_t n = 0;
+};
+
+}  // namespace dbs
+
+// ---------------------------------------------------------------------------
+// the context: one B-spline grid, its Gauss tables resident in HBM, a stream,
+// and a cache of materialised R^k/S^k arrays keyed by (type, k).
+// ---------------------------------------------------------------------------
+struct dbs_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  int ns = 0, ks = 0, ksp = 0, ksq = 0, nv = 0, nsp = 0, nsq = 0, ms = 0;
+
+  // knots (1-based in the reference; here t[i-1])
+  double* t = nullptr;   // ns+ks
+  double* tp = nullptr;  // nsp+ksp
+  double* tq = nullptr;  // nsq+ksq
+  double* gx = nullptr;  // ks Gauss-Legendre nodes on [0,1]
+  double* gw = nullptr;  // ks weights
+  // outer Gauss points, column-major (nv,ks) as in the reference
+  double* gr = nullptr;
+  double* grw = nullptr;
+  double* grm = nullptr;
+  // B-spline values at the outer points: (nv+1,ks,ksp) / (nv+1,ks,ksq)
+  double* pbsp = nullptr;
+  double* qbsp = nullptr;
+  // nested (inner) rule of the diagonal cells: for interval iv, outer point m,
+  // inner point i: x = t_iv + (gr(iv,m)-t_iv)*gx(i), weight (gr-t_iv)*gw(i).
+  // Layout [iv][m][i] (row-major), B values [iv][m][i][j].
+  double* xin = nullptr;
+  double* win = nullptr;
+  double* bpin = nullptr;
+  double* bqin = nullptr;
+
+  // host copies of small tables
+  std::vector<double> h_t, h_gx, h_gw;
+
+  // materialised arrays (plain path), key = type*1000 + k
+  std::map<int, double*> rkb_cache;
+  std::vector<int> rkb_lru;
+  size_t rkb_cache_limit = (size_t)8 << 30;  // bytes
+  size_t rkb_cache_bytes = 0;
+  // scratch for the array build
+  double* mom = nullptr;  // 4 x (ks*ks, nv)
+  double* rkd = nullptr;  // (Tmax, Tmax, nv)
+  // general scratch for contractions
+  double* scratch = nullptr;
+  size_t scratch_n = 0;
+};
+
+namespace dbs {
+int ensure_scratch(dbs_ctx* c, size_t n);
+int kband(int type, int ks);  // third/fourth dimension of rkb for the type
+int build_rkb(dbs_ctx* c, int type, int k, double** out);
+}  // namespace dbs

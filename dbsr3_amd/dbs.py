@@ -1,0 +1,199 @@
+"""Host-side mirror of the reference's DBS library interface.
+
+The reference keeps its state in Fortran modules (DBS_grid, DBS_gauss,
+DBS_integrals: DBSR_HF/dbsr_lib_dbs.f90:442-477, 830-891, 1569-1591): the
+grid, the Gauss tables and a module pointer ``rkb`` that mrk_*/msk_* point at
+the array of (type,k); convol then works on that array.  ``DBS`` mirrors that
+interface -- same routine names, same argument meaning and errors -- on top
+of the C-ABI of libdbsr3_amd.so.  Device arrays are torch float64 tensors
+that hold Fortran (column-major) arrays: a Fortran d(ns,ks) is a tensor of
+shape (ks, ns).  torch is used only for device memory.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from .capi import check, lib
+
+PPPP, QQQQ, PQPQ, QPQP, PPQQ, PQQP = 1, 2, 3, 4, 5, 6
+_TYPE_NAMES = {PPPP: "pppp", QQQQ: "qqqq", PQPQ: "pqpq", QPQP: "qpqp", PPQQ: "ppqq", PQQP: "pqqp"}
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def fortran_to_device(a, device="cuda"):
+    """numpy array (any order) -> device tensor holding its Fortran layout."""
+    torch = _torch()
+    a = np.asfortranarray(np.asarray(a, dtype=np.float64))
+    return torch.from_numpy(np.ascontiguousarray(a.T)).to(device)
+
+
+def device_to_fortran(t):
+    """device tensor holding a Fortran array -> numpy Fortran-order array."""
+    return np.asfortranarray(t.detach().cpu().numpy().T)
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr())
+
+
+class DBS:
+    """One B-spline grid on one GPU (def_grid + alloc_DBS_gauss)."""
+
+    def __init__(self, t, ns, ks, ksp=0, ksq=0, device=0):
+        torch = _torch()
+        if not torch.cuda.is_available():
+            raise RuntimeError("dbsr3_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+        self.device = device
+        self.t = np.ascontiguousarray(np.asarray(t, dtype=np.float64))
+        h = C.c_void_p()
+        check(lib().dbs_ctx_create(C.byref(h), ns, ks, ksp, ksq,
+                                   self.t.ctypes.data_as(C.POINTER(C.c_double)), device))
+        self.ctx = h
+        d = np.zeros(8, dtype=np.int32)
+        check(lib().dbs_ctx_dims(self.ctx, d.ctypes.data_as(C.POINTER(C.c_int))))
+        (self.ns, self.ks, self.ksp, self.ksq, self.nv, self.nsp, self.nsq, self.ms) = map(int, d)
+        self.itype = "aaaa"   # module DBS_integrals: itype / krk of the current rkb
+        self.krk = -100
+        self._type = None
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            lib().dbs_ctx_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- plumbing --------------------------------------------------------
+    def _pre(self):
+        _torch().cuda.synchronize()
+
+    def _post(self):
+        check(lib().dbs_ctx_sync(self.ctx))
+
+    def _empty(self, *fshape):
+        torch = _torch()
+        return torch.zeros(tuple(reversed(fshape)), dtype=torch.float64, device="cuda")
+
+    def kb(self, itype):
+        return self.ks if itype <= 4 else 2 * self.ks - 1
+
+    # -- tables ------------------------------------------------------------
+    def tables(self):
+        nv, ks = self.nv, self.ks
+        gr = np.zeros((nv, ks), order="F")
+        grw = np.zeros((nv, ks), order="F")
+        pb = np.zeros((nv + 1, ks, self.ksp), order="F")
+        qb = np.zeros((nv + 1, ks, self.ksq), order="F")
+        dp = C.POINTER(C.c_double)
+        check(lib().dbs_ctx_tables(self.ctx, gr.ctypes.data_as(dp), grw.ctypes.data_as(dp),
+                                   pb.ctypes.data_as(dp), qb.ctypes.data_as(dp)))
+        return gr, grw, pb, qb
+
+    def moments(self, which, k):
+        """moments_pp/qq/pq/qp(k): which 0 pp, 1 qq, 2 pq, 3 qp -> (ks*ks, nv)"""
+        out = self._empty(self.ks * self.ks, self.nv)
+        self._pre()
+        check(lib().dbs_moments_dev(self.ctx, which, k, _ptr(out)))
+        self._post()
+        return device_to_fortran(out)
+
+    def diag_cells(self, itype, k):
+        """triang_<type>(k, iv) for iv=1..nv -> rkd(ks*ks, ks*ks, nv)"""
+        kk = self.ks * self.ks
+        out = self._empty(kk, kk, self.nv)
+        self._pre()
+        check(lib().dbs_diag_cells_dev(self.ctx, itype, k, _ptr(out)))
+        self._post()
+        return device_to_fortran(out)
+
+    # -- arrays: mrk_* / msk_* select the current rkb ------------------------
+    def _select(self, itype, k):
+        p = C.c_void_p()
+        check(lib().dbs_mrk(self.ctx, itype, k, C.byref(p)))
+        self._type = itype
+        self.itype = _TYPE_NAMES[itype]
+        self.krk = k
+
+    def mrk_pppp(self, k): self._select(PPPP, k)
+    def mrk_qqqq(self, k): self._select(QQQQ, k)
+    def mrk_pqpq(self, k): self._select(PQPQ, k)
+    def mrk_qpqp(self, k): self._select(QPQP, k)
+    def msk_ppqq(self, k): self._select(PPQQ, k)
+    def msk_pqqp(self, k): self._select(PQQP, k)
+
+    def rkb(self):
+        """the current array rkb(ns,ns,kb,kb) as a numpy Fortran array"""
+        if self._type is None:
+            raise RuntimeError("rkb: no array selected (call mrk_*/msk_* first)")
+        kb = self.kb(self._type)
+        out = np.zeros((self.ns, self.ns, kb, kb), order="F")
+        check(lib().dbs_mrk_host(self.ctx, self._type, self.krk,
+                                 out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
+    # -- contractions ---------------------------------------------------------
+    def density(self, p1, p2, type):
+        """density(ns,ks,d,p1,p2,type); p1,p2 device tensors of length ns"""
+        ncol = {"s": self.ks, "u": self.ks, "l": self.ks, "n": 2 * self.ks - 1}.get(type, self.ns)
+        d = self._empty(self.ns, ncol)
+        self._pre()
+        check(lib().dbs_density_dev(self.ctx, _ptr(p1), _ptr(p2), type.encode(), _ptr(d)))
+        self._post()
+        return d
+
+    def convol(self, d, icase, sym_i, sym_j):
+        """convol(ns,ks,a,d,icase,sym_i,sym_j) on the current rkb -> a"""
+        if self._type is None:
+            raise RuntimeError("convol: no array selected (call mrk_*/msk_* first)")
+        if icase <= 2:
+            ncol = 2 * self.ks - 1 if "n" in (sym_i, sym_j) else self.ks
+            a = self._empty(self.ns, ncol)
+        else:
+            a = self._empty(self.ns, self.ns)
+        self._pre()
+        check(lib().dbs_convol_dev(self.ctx, self._type, self.krk, icase, sym_i.encode(),
+                                   sym_j.encode(), _ptr(d), _ptr(a)))
+        self._post()
+        return a
+
+    def SUM_AmB(self, a, b, sym):
+        torch = _torch()
+        out = torch.zeros(1, dtype=torch.float64, device="cuda")
+        self._pre()
+        check(lib().dbs_sum_amb_dev(self.ctx, _ptr(a), _ptr(b), sym.encode(), _ptr(out)))
+        self._post()
+        return float(out.item())
+
+    def rk(self, f1, f2, f3, f4, k):
+        """rk(f1,f2,f3,f4,k); f* device tensors holding (ns,2)"""
+        torch = _torch()
+        out = torch.zeros(1, dtype=torch.float64, device="cuda")
+        self._pre()
+        check(lib().dbs_rk_dev(self.ctx, _ptr(f1), _ptr(f2), _ptr(f3), _ptr(f4), k, _ptr(out)))
+        self._post()
+        return float(out.item())
+
+    def quadr(self, fi, fj, m):
+        torch = _torch()
+        out = torch.zeros(1, dtype=torch.float64, device="cuda")
+        self._pre()
+        check(lib().dbs_quadr_dev(self.ctx, _ptr(fi), _ptr(fj), m, _ptr(out)))
+        self._post()
+        return float(out.item())
+
+    def UPDATE_HS(self, hm, ii, jj, d, sym):
+        """UPDATE_HS(ms,hm,ii,jj,ns,ks,d,sym): hm device tensor holding (ms,ms)"""
+        self._pre()
+        check(lib().dbs_update_hs_dev(self.ctx, _ptr(hm), ii, jj, _ptr(d), sym.encode()))
+        self._post()
+        return hm

@@ -1,0 +1,206 @@
+/*
+ * dbsr3_amd.h -- C-ABI of libdbsr3_amd.so, the MI355X (gfx950) engine for
+ * DBSR's two-electron B-spline Slater integrals.
+ *
+ * Conventions
+ *  - Every function returns an int status (DBS_OK = 0); dbs_last_error()
+ *    gives the message of the last failure on the calling thread.  The
+ *    messages of the reference's Stop statements are kept where the same
+ *    condition is detected (e.g. "convol: unknown case").
+ *  - Arrays are column-major with the reference's dimensions (Fortran
+ *    layout), e.g. rkb(ns,ns,ks,ks), d(ns,ks), pq(ns,2,norb).  Orbital and
+ *    channel indices inside coefficient keys are 1-based, as in the reference.
+ *  - Pointers named *_dev are device (HBM) pointers; all others are host.
+ *  - Work runs on the context's HIP stream; calls returning results into
+ *    host memory synchronise that stream, *_dev calls do not.
+ *  - Types: 1 pppp, 2 qqqq, 3 pqpq, 4 qpqp (R^k), 5 ppqq, 6 pqqp (S^k).
+ * Reference paths are relative to the zatsaroi/DBSR3 tree.
+ */
+#ifndef DBSR3_AMD_H
+#define DBSR3_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DBS_OK 0
+#define DBS_ERR_ARG 1      /* invalid argument                          */
+#define DBS_ERR_HIP 2      /* HIP runtime failure                       */
+#define DBS_ERR_STATE 3    /* call out of sequence / missing object     */
+#define DBS_ERR_CAPACITY 4 /* a capacity limit was exceeded             */
+
+#define DBS_PPPP 1
+#define DBS_QQQQ 2
+#define DBS_PQPQ 3
+#define DBS_QPQP 4
+#define DBS_PPQQ 5
+#define DBS_PQQP 6
+#define DBS_RK_TOTAL 0 /* engine kind: the rk() sum over pppp+qqqq+qpqp+pqpq */
+
+typedef struct dbs_ctx dbs_ctx;
+typedef struct dbs_cdata dbs_cdata;
+typedef struct dbs_engine dbs_engine;
+
+const char *dbs_last_error(void);
+int dbs_version(void);
+
+/* ---------------------------------------------------------------------
+ * Grid context.  Replaces the module state set by def_grid +
+ * alloc_DBS_gauss + Def_pbsp/Def_qbsp (DBSR_HF/dbsr_lib_dbs.f90:480-596,
+ * 911-1074): knots t(1:ns+ks), orders ks/ksp/ksq (ksp<=0 -> ks-1,
+ * ksq<=0 -> ks, as :565-566).  Gauss tables and B-spline values are built
+ * on the device and stay resident.
+ * ------------------------------------------------------------------- */
+int dbs_ctx_create(dbs_ctx **ctx, int ns, int ks, int ksp, int ksq, const double *t, int device);
+int dbs_ctx_destroy(dbs_ctx *ctx);
+/* dims8 = {ns, ks, ksp, ksq, nv, nsp, nsq, ms} */
+int dbs_ctx_dims(const dbs_ctx *ctx, int *dims8);
+int dbs_ctx_sync(dbs_ctx *ctx);
+/* copies gr(nv,ks), grw(nv,ks), pbsp(nv+1,ks,ksp), qbsp(nv+1,ks,ksq) to host
+ * (module DBS_gauss, dbsr_lib_dbs.f90:830-891); any pointer may be NULL */
+int dbs_ctx_tables(dbs_ctx *ctx, double *gr, double *grw, double *pbsp, double *qbsp);
+/* bytes of materialised arrays the context may cache (default 8 GiB) */
+int dbs_ctx_set_cache_limit(dbs_ctx *ctx, size_t bytes);
+
+/* device memory helpers for callers without HIP headers (Fortran shim) */
+int dbs_malloc(dbs_ctx *ctx, void **dptr, size_t bytes);
+int dbs_free(dbs_ctx *ctx, void *dptr);
+int dbs_memcpy_h2d(dbs_ctx *ctx, void *dst_dev, const void *src, size_t bytes);
+int dbs_memcpy_d2h(dbs_ctx *ctx, void *dst, const void *src_dev, size_t bytes);
+
+/* ---------------------------------------------------------------------
+ * Plain path: the reference's routines one for one.
+ * ------------------------------------------------------------------- */
+/* moments_pp/qq/pq/qp (dbsr_lib_dbs.f90:1384-1564): which 0 pp, 1 qq,
+ * 2 pq, 3 qp; rkm(ks*ks,nv), packed as the reference */
+int dbs_moments_dev(dbs_ctx *ctx, int which, int k, double *rkm_dev);
+/* triang_* for every interval (dbsr_lib_dbs.f90:2979-3063, 3151-3245,
+ * 3333-3427, 3512-3597, 3682-3767, 3852-3936): rkd(ks*ks,ks*ks,nv) */
+int dbs_diag_cells_dev(dbs_ctx *ctx, int type, int k, double *rkd_dev);
+/* mrk_pppp/qqqq/pqpq/qpqp and msk_ppqq/pqqp (dbsr_lib_dbs.f90:2898-2959,
+ * 3431-3492, 3067-3131, 3249-3313, 3601-3662, 3771-3833): returns the
+ * context's cached device array rkb(ns,ns,kb,kb), kb = ks (R) or 2ks-1 (S).
+ * The pointer stays valid until the context evicts or is destroyed. */
+int dbs_mrk(dbs_ctx *ctx, int type, int k, const double **rkb_dev);
+/* same, copied into a host array of ns*ns*kb*kb doubles */
+int dbs_mrk_host(dbs_ctx *ctx, int type, int k, double *rkb);
+/* density (dbsr_lib_dbs.f90:2834-2891): type 's','l','n','x' */
+int dbs_density_dev(dbs_ctx *ctx, const double *p1_dev, const double *p2_dev, char type, double *d_dev);
+/* convol / convol_sk (dbsr_lib_dbs.f90:2208-2478, 2482-2752) applied to the
+ * array of (type,k); icase 1..4, sym 's','l','n' */
+int dbs_convol_dev(dbs_ctx *ctx, int type, int k, int icase, char sym_i, char sym_j,
+                   const double *d_dev, double *a_dev);
+/* SUM_AmB (dbsr_lib_dbs.f90:4027-4080); result written to out_dev[0] */
+int dbs_sum_amb_dev(dbs_ctx *ctx, const double *a_dev, const double *b_dev, char sym, double *out_dev);
+/* rk (dbsr_lib_dbs.f90:3981-4022): f*(ns,2) device; out_dev[0] */
+int dbs_rk_dev(dbs_ctx *ctx, const double *f1, const double *f2, const double *f3,
+               const double *f4, int k, double *out_dev);
+/* quadr (dbsr_lib_dbs.f90:3940-3976): <fi|r^m|fj>, out_dev[0] */
+int dbs_quadr_dev(dbs_ctx *ctx, const double *fi_dev, const double *fj_dev, int m, double *out_dev);
+/* UPDATE_HS, library form (dbsr_lib_dbs.f90:4137-4193): hm(ms,ms) += y/2 and
+ * its transpose at sub-block (ii,jj); sym 's','n','x' */
+int dbs_update_hs_dev(dbs_ctx *ctx, double *hm_dev, int ii, int jj, const double *d_dev, char sym);
+
+/* host-pointer forms of the per-call routines, for drop-in callers that
+ * keep their arrays in host memory (each stages through the device) */
+int dbs_density(dbs_ctx *ctx, const double *p1, const double *p2, char type, double *d);
+int dbs_convol(dbs_ctx *ctx, int type, int k, int icase, char sym_i, char sym_j,
+               const double *d, double *a);
+int dbs_sum_amb(dbs_ctx *ctx, const double *a, const double *b, char sym, double *out);
+int dbs_rk(dbs_ctx *ctx, const double *f1, const double *f2, const double *f3,
+           const double *f4, int k, double *out);
+int dbs_quadr(dbs_ctx *ctx, const double *fi, const double *fj, int m, double *out);
+
+/* ---------------------------------------------------------------------
+ * Coefficient lists: module c_data (DBSR_MAT3/MOD_c_data.f90:2-368).
+ * Blocked per-(kpol,itype) sorted lists with equal-key summation at insert
+ * (Add_cdata :177-236), block overflow handling (Add_coef :119-173), k-way
+ * merge with eps drop (Merge_cdata :240-314) and Add_matrix (:318-368).
+ * Add_matrix calls gen(user, itype, kpol, nc, c, K1..K4) with the merged
+ * list in IPT order -- the reference's Gen_matrix hook.
+ * ------------------------------------------------------------------- */
+typedef void (*dbs_gen_matrix_fn)(void *user, int itype, int kpol, int nc, const double *c,
+                                  const int *K1, const int *K2, const int *K3, const int *K4);
+int dbs_cdata_create(dbs_cdata **cd, int ntype, int kpol1, int kpol2, int mblock, int nblock,
+                     int kblock, double eps_c, dbs_gen_matrix_fn gen, void *user);
+int dbs_cdata_destroy(dbs_cdata *cd);
+int dbs_cdata_add_coef(dbs_cdata *cd, double c, int kpol, int j1, int j2, int j3, int j4, int itype);
+int dbs_cdata_add_matrix(dbs_cdata *cd, int itype, int kpol);
+
+/* Add_integral, CI3 form (DBSR_CI3/add_integral.f90:2-43), Rk part: the
+ * parity rule on l(i1)+l(i3)+k, l(i2)+l(i4)+k, then Add_coef for types
+ * 1..4 with K1=i1*ibi+i3, K2=i2*ibi+i4, K3=ic, K4=jc.  lorb[i] = l of
+ * 1-based orbital i (lorb[0] unused).  Returns DBS_OK whether or not the
+ * parity rule dropped the term. */
+int dbs_add_integral_ci(dbs_cdata *cd, int kpol, int ic, int jc, int i1, int i2, int i3, int i4,
+                        double c, const int *lorb, int ibi);
+/* Add_integral, MAT3 R-branch for channel-channel terms
+ * (DBSR_MAT3/add_integral.f90:28-32 + Idef_Rtype idef_type.f90:142-302,
+ * structures 6.0-9.0 with ic=jc=io=jo=0).  ipbs[i] > 0 marks orbital i as
+ * channel ipbs[i].  Terms with |c| < eps_c or k > mk are dropped as there. */
+int dbs_add_integral_mat(dbs_cdata *cd, int kpol, int i1, int i2, int i3, int i4, double c,
+                         const int *lorb, const int *ipbs, int ibo, int mk, double eps_c);
+/* Decode_int (DBSR_CI3/setup_matrix.f90:206-223) for n packed int_bnk codes */
+int dbs_decode_int(int n, const int *code, int *met, int *k, int *i1, int *i2, int *i3, int *i4);
+
+/* Rk_evaluate / Sk_evaluate (DBSR_CI3/rk_evaluate.f90:2-63,
+ * sk_evaluate.f90:2-63): evaluate every entry of a merged list (IPT order,
+ * host arrays) of type itype (1..6) and multipole k on the device and add
+ * S*c to HM(ic,jc) (jc <= nzero) or DM(jc) (ic == jc > nzero).  Entries
+ * with |c| < eps_c are skipped.  Contributions to one element are added in
+ * list order, as the reference does.  pq_dev(ns,2,norb); HM_dev(ldhm,*). */
+int dbs_rk_evaluate_dev(dbs_ctx *ctx, int itype, int k, int nc, const double *c, const int *K1,
+                        const int *K2, const int *K3, const int *K4, double eps_c, int ibi,
+                        const double *pq_dev, int norb, double *HM_dev, int ldhm, int nzero,
+                        double *DM_dev);
+/* R_data itype 3 (direct) / 4 (exchange) (DBSR_MAT3/R_data.f90:214-277)
+ * for one atype 0..3 (R_data.f90:76-86): per run of equal K1=(ich,jch), the
+ * c-weighted density of orbitals (K2,K3) is convolved (case 1 's' or case
+ * 3 'x') and added by UPDATE_HS (DBSR_MAT3/updates.f90:110-179) to block
+ * icc(ich,jch) of hch_dev(ms,ms,nblocks); icc is (nch,nch) host, 1-based
+ * block numbers, 0 = not stored.  UPDATE_ACF terms are not produced. */
+int dbs_rdata_34_dev(dbs_ctx *ctx, int itype, int atype, int k, int nc, const double *c,
+                     const int *K1, const int *K2, const int *K3, int ibo, const double *pq_dev,
+                     int norb, const int *icc, int nch, double *hch_dev, int nblocks);
+
+/* ---------------------------------------------------------------------
+ * Engine: batched evaluation of integral sets and their adjoint.
+ * An instance = (context, norb orbitals, rows (k,a,b,c,d)).  Row n is
+ *   kind DBS_RK_TOTAL: R^k(a,b;c,d) = rk(f_a,f_b,f_c,f_d,k)
+ *                      (dbsr_lib_dbs.f90:3981-4022; hf_rk DBSR_HF/hf_rk.f90:2-31)
+ *   kind 1..4: the single-type term of that sum (Rk_evaluate types),
+ *   kind 5/6: S^k ppqq / pqqp (Sk_evaluate, DBSR_CI3/sk_evaluate.f90).
+ * The adjoint returns E = sum_n coef_n*value_n and dE/d(pq), i.e. the
+ * direct+exchange potentials applied to the orbitals that hf_matrix builds
+ * (DBSR_HF/hf_matrix.f90:2-71).  Many instances of different ns/ks/kmax
+ * run in one persistent launch per phase; the sequence can be captured in
+ * a hipGraph and replayed.
+ * ------------------------------------------------------------------- */
+int dbs_engine_create(dbs_engine **eng, int ninst, dbs_ctx *const *ctx, const int *norb,
+                      const int *nrows, const int *rows, int kind);
+int dbs_engine_destroy(dbs_engine *eng);
+/* device pointers into the engine's buffers for instance inst:
+ * pq (ns,2,norb) input, coef (nrows) input, values (nrows) output,
+ * grad (ns,2,norb) output, energy (1) output; any may be NULL */
+int dbs_engine_buffers(dbs_engine *eng, int inst, double **pq, double **coef, double **values,
+                       double **grad, double **energy);
+int dbs_engine_forward(dbs_engine *eng);
+int dbs_engine_forward_adjoint(dbs_engine *eng);
+/* capture forward(+adjoint) into a hipGraph; launch replays it */
+int dbs_engine_graph_capture(dbs_engine *eng, int with_adjoint);
+int dbs_engine_graph_launch(dbs_engine *eng);
+/* tuning parameters (see dbsr3_amd/csrc/dbs_engine.hip); autotune times
+ * the candidates on the engine's own data and keeps the fastest */
+int dbs_engine_set_tuning(dbs_engine *eng, int nparam, const int *params);
+int dbs_engine_get_tuning(dbs_engine *eng, int nparam, int *params);
+int dbs_engine_autotune(dbs_engine *eng, int with_adjoint, int reps);
+/* algorithmic flop count and minimum bytes of one forward (+adjoint) */
+int dbs_engine_work(dbs_engine *eng, int with_adjoint, double *flops, double *bytes);
+int dbs_engine_sync(dbs_engine *eng);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
