@@ -53,7 +53,8 @@ PROTOTYPES = {
     "dbs_cdata_destroy": [vp],
     "dbs_cdata_add_coef": [vp, f64, i32, i32, i32, i32, i32, i32],
     "dbs_cdata_add_matrix": [vp, i32, i32],
-    "dbs_add_integral_ci": [vp, i32, i32, i32, i32, i32, i32, i32, f64, ip, i32],
+    "dbs_add_integral_ci": [vp, i32, i32, i32, i32, i32, i32, i32, f64, ip, i32, i32],
+    "dbs_smu": [i32, i32, i32, i32, i32, i32, dp, dp],
     "dbs_add_integral_mat": [vp, i32, i32, i32, i32, i32, f64, ip, ip, i32, i32, f64],
     "dbs_decode_int": [i32, ip, ip, ip, ip, ip, ip, ip],
     "dbs_rk_evaluate_dev": [vp, i32, i32, i32, dp, ip, ip, ip, ip, f64, i32, vp, i32, vp, i32,
@@ -65,6 +66,7 @@ PROTOTYPES = {
                            C.POINTER(vp)],
     "dbs_engine_forward": [vp],
     "dbs_engine_forward_adjoint": [vp],
+    "dbs_engine_forward_plain": [vp],
     "dbs_engine_graph_capture": [vp, i32],
     "dbs_engine_graph_launch": [vp],
     "dbs_engine_set_tuning": [vp, i32, ip],

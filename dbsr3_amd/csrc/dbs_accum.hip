@@ -1,0 +1,724 @@
+// dbs_accum.hip -- coefficient lists and their accumulation into Hamiltonian
+// matrices on MI355X.
+//
+//  * c_data: blocked per-(kpol,itype) sorted lists (DBSR_MAT3/MOD_c_data.f90).
+//    Host integer work with the reference's exact semantics: binary-search
+//    insert with equal-key summation, block overflow into new blocks or an
+//    early Add_matrix of the fullest type, k-way merge summing equal keys in
+//    block order, eps drop, IPT order.
+//  * Add_integral (CI3 and MAT3 channel-channel forms), Idef_Rtype, SMU,
+//    Decode_int: host integer work.
+//  * Rk_evaluate / Sk_evaluate: every list entry's integral on the device
+//    (engine), then deterministic accumulation into HM/DM -- contributions
+//    to one element are added in list order, as the reference does.
+//  * R_data itype 3/4 + UPDATE_HS: per channel pair the c-weighted density
+//    (list order), a batched convolution over all channel pairs of the list,
+//    and the channel-block update, all on the device.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "dbs_device.h"
+#include "dbs_internal.h"
+
+// ===========================================================================
+// c_data
+// ===========================================================================
+struct dbs_cdata {
+  int ntype = 0, kpol1 = 0, kpol2 = 0, mb = 0, nb = 0, kb = 0;
+  double eps = 0.0;
+  // storage, 1-based positions (index 0 unused)
+  std::vector<double> C;
+  std::vector<int> K1, K2, K3, K4;
+  std::vector<int> ipblk, jpblk;
+  std::vector<int> iblk, nblk;            // (k,type)
+  std::vector<std::vector<int>> kblk;     // (k,type) -> block chain
+  dbs_gen_matrix_fn gen = nullptr;
+  void* user = nullptr;
+  int slot(int k, int t) const { return (k - kpol1) + (kpol2 - kpol1 + 1) * (t - 1); }
+};
+
+namespace {
+
+inline int cmp_key(const dbs_cdata* cd, int p, int j1, int j2, int j3, int j4) {
+  if (j1 != cd->K1[p]) return j1 < cd->K1[p] ? -1 : 1;
+  if (j2 != cd->K2[p]) return j2 < cd->K2[p] ? -1 : 1;
+  if (j3 != cd->K3[p]) return j3 < cd->K3[p] ? -1 : 1;
+  if (j4 != cd->K4[p]) return j4 < cd->K4[p] ? -1 : 1;
+  return 0;
+}
+
+// insert into sub-list ip..jp (sorted); returns new jp or -1 on overflow
+int cdata_insert(dbs_cdata* cd, int ip, int jp, double c, int j1, int j2, int j3, int j4) {
+  auto put = [&](int p) {
+    cd->C[p] = c; cd->K1[p] = j1; cd->K2[p] = j2; cd->K3[p] = j3; cd->K4[p] = j4;
+  };
+  if (jp < ip) {
+    put(ip);
+    return ip;
+  }
+  int lo = ip, hi = jp;
+  while (lo <= hi) {
+    const int m = (lo + hi) / 2;
+    const int r = cmp_key(cd, m, j1, j2, j3, j4);
+    if (r < 0) hi = m - 1;
+    else if (r > 0) lo = m + 1;
+    else {
+      cd->C[m] = cd->C[m] + c;  // the same integral
+      return jp;
+    }
+  }
+  if (jp - ip + 1 >= cd->mb) return -1;
+  for (int i = jp; i >= lo; --i) {
+    cd->C[i + 1] = cd->C[i];
+    cd->K1[i + 1] = cd->K1[i]; cd->K2[i + 1] = cd->K2[i];
+    cd->K3[i + 1] = cd->K3[i]; cd->K4[i + 1] = cd->K4[i];
+  }
+  put(lo);
+  return jp + 1;
+}
+
+// k-way merge of the blocks' heads: equal keys are summed in block order
+// (pairs ii<jj), the smallest key is taken next, |C| <= eps dropped.
+void cdata_merge(dbs_cdata* cd, std::vector<int>& IP, std::vector<int>& JP, std::vector<int>& ipt) {
+  const int nn = (int)IP.size() - 1;
+  int mm = 0;
+  for (int m = 1; m <= nn; ++m)
+    if (JP[m] > 0) { mm = m; break; }
+  if (mm == 0) return;
+  for (;;) {
+    bool merged = false;
+    for (int ii = 1; ii <= nn - 1 && !merged; ++ii) {
+      if (JP[ii] <= 0) continue;
+      const int i = IP[ii];
+      for (int jj = ii + 1; jj <= nn; ++jj) {
+        if (JP[jj] <= 0) continue;
+        const int j = IP[jj];
+        if (cmp_key(cd, i, cd->K1[j], cd->K2[j], cd->K3[j], cd->K4[j]) != 0) continue;
+        cd->C[i] = cd->C[i] + cd->C[j];
+        cd->C[j] = 0.0;
+        mm = jj;
+        merged = true;
+        break;
+      }
+    }
+    if (!merged) {
+      int j = IP[mm];
+      for (int m = 1; m <= nn; ++m) {
+        if (JP[m] == 0) continue;
+        const int i = IP[m];
+        if (cmp_key(cd, j, cd->K1[i], cd->K2[i], cd->K3[i], cd->K4[i]) < 0) { mm = m; j = i; }
+      }
+      if (std::fabs(cd->C[j]) > cd->eps) ipt.push_back(IP[mm]);
+    }
+    IP[mm] += 1;
+    if (IP[mm] <= JP[mm]) continue;
+    JP[mm] = 0;
+    int nxt = 0;
+    for (int m = 1; m <= nn; ++m)
+      if (JP[m] > 0) { nxt = m; break; }
+    if (nxt == 0) break;
+    mm = nxt;
+  }
+}
+
+int cdata_add_matrix(dbs_cdata* cd, int itype, int kpol) {
+  const int s = cd->slot(kpol, itype);
+  int n = cd->nblk[s];
+  const int first = cd->kblk[s][0];
+  if (n == 1 && cd->jpblk[first] <= 0) n = 0;
+  if (n <= 0) return DBS_OK;
+  std::vector<int> ipt;
+  if (n == 1) {
+    for (int p = cd->ipblk[first]; p <= cd->jpblk[first]; ++p) ipt.push_back(p);
+  } else {
+    std::vector<int> IP(n + 1, 0), JP(n + 1, 0);
+    for (int i = 1; i <= n; ++i) {
+      const int b = cd->kblk[s][i - 1];
+      IP[i] = cd->ipblk[b];
+      JP[i] = cd->jpblk[b];
+    }
+    cdata_merge(cd, IP, JP, ipt);
+    for (int b : cd->kblk[s]) cd->jpblk[b] = -1;
+  }
+  // re-arm the first block of the type
+  cd->iblk[s] = first;
+  cd->nblk[s] = 1;
+  cd->kblk[s].assign(1, first);
+  cd->jpblk[first] = 0;
+  const int nc = (int)ipt.size();
+  std::vector<double> c(nc);
+  std::vector<int> k1(nc), k2(nc), k3(nc), k4(nc);
+  for (int q = 0; q < nc; ++q) {
+    const int p = ipt[q];
+    c[q] = cd->C[p]; k1[q] = cd->K1[p]; k2[q] = cd->K2[p]; k3[q] = cd->K3[p]; k4[q] = cd->K4[p];
+  }
+  if (cd->gen) cd->gen(cd->user, itype, kpol, nc, c.data(), k1.data(), k2.data(), k3.data(), k4.data());
+  return DBS_OK;
+}
+
+int cdata_new_block(dbs_cdata* cd, int s) {
+  for (int i = 1; i <= cd->nb; ++i) {
+    if (cd->jpblk[i] >= 0) continue;
+    cd->iblk[s] = i;
+    cd->jpblk[i] = 0;
+    cd->nblk[s] += 1;
+    cd->kblk[s].push_back(i);
+    return 1;
+  }
+  return 0;
+}
+
+inline int l_kappa(int kappa) { return kappa > 0 ? kappa : -kappa - 1; }
+
+// Breit factors (Grant and Pyper), the arithmetic of SMU
+double smu_factors(int ka, int kb, int kc, int kd, int L, int v, double* S) {
+  double smu = 0.0;
+  for (int i = 0; i < 8; ++i) S[i] = 0.0;
+  if (L < 0 || v < 0) return 0.0;
+  if ((l_kappa(ka) + l_kappa(kc) + v) % 2 == 0) return 0.0;
+  if ((l_kappa(kb) + l_kappa(kd) + v) % 2 == 0) return 0.0;
+  if (v == L) {
+    if (L == 0) return 0.0;
+    smu = (double)(-(ka + kc) * (kb + kd));
+    smu = smu / L / (L + 1);
+    for (int i = 0; i < 8; ++i) S[i] = smu;
+    return smu;
+  }
+  const int K = kc - ka, Kp = kd - kb;
+  if (v == L - 1) {
+    double bp = L + 1;
+    bp = bp / 2 / (L + L - 1);
+    double cp = -(L - 2);
+    cp = cp / 2 / L / (L + L - 1);
+    S[0] = (L + K) * (bp + cp * Kp);
+    S[1] = (L + Kp) * (bp + cp * K);
+    S[2] = (L - K) * (bp - cp * Kp);
+    S[3] = (L - Kp) * (bp - cp * K);
+    S[4] = -(L + K) * (bp - cp * Kp);
+    S[5] = -(L - Kp) * (bp + cp * K);
+    S[6] = -(L - K) * (bp + cp * Kp);
+    S[7] = -(L + Kp) * (bp - cp * K);
+  } else if (v == L + 1) {
+    const int i = -L - 1;
+    double b = L;
+    b = b / 2 / (L + L + 3);
+    double c = (L + 3);
+    c = c / (L + L + 2) / (L + L + 3);
+    S[0] = (i + Kp) * (b + c * K);
+    S[1] = (i + K) * (b + c * Kp);
+    S[2] = (i - Kp) * (b - c * K);
+    S[3] = (i - K) * (b - c * Kp);
+    S[4] = -(i - Kp) * (b + c * K);
+    S[5] = -(i + K) * (b - c * Kp);
+    S[6] = -(i + Kp) * (b - c * K);
+    S[7] = -(i - K) * (b + c * Kp);
+  }
+  for (int q = 0; q < 8; ++q) smu = smu + std::fabs(S[q]);
+  return smu;
+}
+
+}  // namespace
+
+// ===========================================================================
+// device kernels for the accumulation
+// ===========================================================================
+namespace dbs {
+
+// HM/DM += S*c per target, entries of one target in list order
+__global__ void k_accumulate_targets(int ntarget, const long long* __restrict__ tpos,
+                                     const int* __restrict__ toff, const int* __restrict__ tent,
+                                     const double* __restrict__ S, const double* __restrict__ c,
+                                     double* __restrict__ HM, double* __restrict__ DM) {
+#pragma clang fp contract(off)
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < ntarget; t += gridDim.x * blockDim.x) {
+    const long long p = tpos[t];
+    double* dst = p >= 0 ? HM + p : DM + (-p - 1);
+    double acc = *dst;
+    for (int e = toff[t]; e < toff[t + 1]; ++e) {
+      const int q = tent[e];
+      const double ss = S[q] * c[q];
+      acc = acc + ss;
+    }
+    *dst = acc;
+  }
+}
+
+// xx_g = sum_e c_e * density(pq(jp1,j1), pq(jp2,j2)) in list order;
+// sym 0: band 's' (ns,ks); 1: full 'x' (ns,ns)
+__global__ void k_group_density(int ng, const int* __restrict__ goff, const int* __restrict__ ej1,
+                                const int* __restrict__ ej2, const double* __restrict__ ec,
+                                const double* __restrict__ pq, int ns, int ks, int jp1, int jp2,
+                                int sym, double* __restrict__ xx) {
+#pragma clang fp contract(off)
+  const int ncol = sym == 0 ? ks : ns;
+  const long long per = (long long)ns * ncol;
+  const long long total = per * ng;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+       q += (long long)gridDim.x * blockDim.x) {
+    const int g = (int)(q / per);
+    const int r = (int)(q % per);
+    const int i = r % ns, j = r / ns;
+    double acc = 0.0;
+    for (int e = goff[g]; e < goff[g + 1]; ++e) {
+      const double* p1 = pq + (long long)ns * ((jp1 - 1) + 2LL * (ej1[e] - 1));
+      const double* p2 = pq + (long long)ns * ((jp2 - 1) + 2LL * (ej2[e] - 1));
+      double d;
+      if (sym == 0) {
+        if (j == 0) d = p1[i] * p2[i];
+        else if (i < ns - j) d = p1[i] * p2[i + j] + p1[i + j] * p2[i];
+        else d = 0.0;
+      } else {
+        d = p1[i] * p2[j];
+      }
+      acc = acc + ec[e] * d;
+    }
+    xx[q] = acc;
+  }
+}
+
+// convol case 1 ('s','s') for many groups: a_g(i,ip) = sum_jp sum_j
+// xx_g(j,jp) rkb(i,j,ip,jp), GB groups per thread so each rkb value feeds
+// GB products.
+template <int GB>
+__global__ void k_group_convol1(int ng, int ns, int ks, const double* __restrict__ rkb,
+                                const double* __restrict__ xx, double* __restrict__ out) {
+#pragma clang fp contract(off)
+  const int ngc = (ng + GB - 1) / GB;
+  const long long total = (long long)ns * ks * ngc;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+       q += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(q % ns);
+    const int ip = (int)((q / ns) % ks);
+    const int g0 = (int)(q / ((long long)ns * ks)) * GB;
+    double acc[GB];
+    for (int b = 0; b < GB; ++b) acc[b] = 0.0;
+    if (i < ns - ip) {
+      for (int jp = 0; jp < ks; ++jp)
+        for (int j = 0; j < ns - jp; ++j) {
+          const double r = rkb[i + (long long)ns * (j + (long long)ns * (ip + (long long)ks * jp))];
+          for (int b = 0; b < GB; ++b)
+            if (g0 + b < ng) acc[b] = acc[b] + xx[(long long)(g0 + b) * ns * ks + j + (long long)ns * jp] * r;
+        }
+    }
+    for (int b = 0; b < GB; ++b)
+      if (g0 + b < ng) out[(long long)(g0 + b) * ns * ks + i + (long long)ns * ip] = acc[b];
+  }
+}
+
+// convol case 3 ('s','s') for many groups, gathered per output element:
+// a(x,y) collects rkb(i,j,ip,jp)*d(u,v) over the four symmetric-storage
+// terms of the reference (convol, dbsr_lib_dbs.f90:2344-2357)
+__global__ void k_group_convol3(int ng, int ns, int ks, const double* __restrict__ rkb,
+                                const double* __restrict__ xx, double* __restrict__ out) {
+  const long long per = (long long)ns * ns;
+  const long long total = per * ng;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+       q += (long long)gridDim.x * blockDim.x) {
+    const int g = (int)(q / per);
+    const int r = (int)(q % per);
+    const int x = r % ns, y = r / ns;  // 0-based output a(x,y)
+    const double* d = xx + (long long)g * per;
+    double acc = 0.0;
+    // terms: (xs,ys,cond): (0,1,-),(1,1,ip>0),(0,0,jp>0),(1,0,ip>0&&jp>0)
+    for (int t = 0; t < 4; ++t) {
+      const int xs = (t == 1 || t == 3), ys = (t == 0 || t == 1);
+      for (int jp = 0; jp < ks; ++jp) {
+        if ((t == 2 || t == 3) && jp == 0) continue;
+        const int j = ys == 0 ? y : y - jp;
+        if (j < 0 || j >= ns - jp) continue;
+        const int jj = j + jp;
+        const int v = ys == 0 ? jj : j;
+        for (int ip = 0; ip < ks; ++ip) {
+          if ((t == 1 || t == 3) && ip == 0) continue;
+          const int i = xs == 0 ? x : x - ip;
+          if (i < 0 || i >= ns - ip) continue;
+          const int ii = i + ip;
+          const int u = xs == 0 ? ii : i;
+          acc += rkb[i + (long long)ns * (j + (long long)ns * (ip + (long long)ks * jp))] * d[u + (long long)ns * v];
+        }
+      }
+    }
+    out[q] = acc;
+  }
+}
+
+// UPDATE_HS, MAT3 form (DBSR_MAT3/updates.f90:110-179): y = unfold(d);
+// ich>jch: h += y; jch>ich: h += y^T; ich==jch: pass 0 h(i1..,j1..) += y/2,
+// pass 1 h(j1..,i1..) += (y/2)^T.
+__global__ void k_update_hs_mat(int ng, const int4* __restrict__ gi, int ns, int ks, int sym,
+                                int ii, int jj, int pass, const double* __restrict__ d,
+                                double* __restrict__ hch) {
+#pragma clang fp contract(off)
+  const long long per = (long long)ns * ns;
+  const long long total = per * ng;
+  const int ms = 2 * ns;
+  const int ncol = sym == 0 ? ks : ns;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+       q += (long long)gridDim.x * blockDim.x) {
+    const int g = (int)(q / per);
+    const int r = (int)(q % per);
+    const int a = r % ns, b = r / ns;
+    const int4 G = gi[g];  // (ich, jch, block0, 0)
+    const int ich = G.x, jch = G.y;
+    if (pass == 1 && ich != jch) continue;
+    const double* dg = d + (long long)g * ns * ncol;
+    auto yv = [&](int u, int v) -> double {
+      if (sym == 1) return dg[u + (long long)ns * v];
+      const int lo = u < v ? u : v, hi = u < v ? v : u;
+      return hi - lo < ks ? dg[lo + (long long)ns * (hi - lo)] : 0.0;
+    };
+    double* h = hch + (long long)G.z * ms * ms;
+    const int i1 = (ii - 1) * ns, j1 = (jj - 1) * ns;
+    if (ich > jch) {
+      h[(i1 + a) + (long long)ms * (j1 + b)] += yv(a, b);
+    } else if (jch > ich) {
+      h[(i1 + a) + (long long)ms * (j1 + b)] += yv(b, a);
+    } else if (pass == 0) {
+      h[(i1 + a) + (long long)ms * (j1 + b)] += yv(a, b) / 2.0;
+    } else {
+      h[(j1 + a) + (long long)ms * (i1 + b)] += yv(b, a) / 2.0;
+    }
+  }
+}
+
+}  // namespace dbs
+
+using namespace dbs;
+
+template <class T>
+static int dev_upload(T** p, const std::vector<T>& v, hipStream_t s) {
+  const size_t n = v.empty() ? 1 : v.size();
+  DBS_HIP(hipMalloc(p, n * sizeof(T)));
+  if (!v.empty()) DBS_HIP(hipMemcpyAsync(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+  return DBS_OK;
+}
+
+static unsigned grid_of(long long n, int block) {
+  long long g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > 65535) g = 65535;
+  return (unsigned)g;
+}
+
+extern "C" {
+
+int dbs_cdata_create(dbs_cdata** out, int ntype, int kpol1, int kpol2, int mblock, int nblock,
+                     int kblock, double eps_c, dbs_gen_matrix_fn gen, void* user) {
+  DBS_CHECK(out, DBS_ERR_ARG, "dbs_cdata_create: null argument");
+  DBS_CHECK(ntype >= 1 && kpol2 >= kpol1 && mblock >= 1 && kblock >= 1, DBS_ERR_ARG,
+            "dbs_cdata_create: bad dimensions");
+  dbs_cdata* cd = new dbs_cdata();
+  cd->ntype = ntype; cd->kpol1 = kpol1; cd->kpol2 = kpol2;
+  cd->mb = mblock; cd->nb = nblock; cd->kb = kblock;
+  const int need = ntype * (kpol2 - kpol1 + 1) * 2 + 1;
+  if (cd->nb < need) cd->nb = need;
+  cd->eps = eps_c;
+  cd->gen = gen;
+  cd->user = user;
+  const size_t m = (size_t)cd->mb * cd->nb;
+  cd->C.assign(m + 2, 0.0);
+  cd->K1.assign(m + 2, 0); cd->K2.assign(m + 2, 0); cd->K3.assign(m + 2, 0); cd->K4.assign(m + 2, 0);
+  cd->ipblk.assign(cd->nb + 1, 0);
+  cd->jpblk.assign(cd->nb + 1, -1);
+  for (int i = 1; i <= cd->nb; ++i) cd->ipblk[i] = (i - 1) * cd->mb + 1;
+  const int ns = (kpol2 - kpol1 + 1) * ntype;
+  cd->iblk.assign(ns, 0);
+  cd->nblk.assign(ns, 0);
+  cd->kblk.assign(ns, {});
+  int i = 0;
+  for (int k = kpol1; k <= kpol2; ++k)
+    for (int t = 1; t <= ntype; ++t) {
+      ++i;
+      const int s = cd->slot(k, t);
+      cd->iblk[s] = i;
+      cd->nblk[s] = 1;
+      cd->kblk[s].assign(1, i);
+      cd->jpblk[i] = 0;
+    }
+  *out = cd;
+  return DBS_OK;
+}
+
+int dbs_cdata_destroy(dbs_cdata* cd) {
+  delete cd;
+  return DBS_OK;
+}
+
+int dbs_cdata_add_coef(dbs_cdata* cd, double c, int kpol, int j1, int j2, int j3, int j4, int itype) {
+  DBS_CHECK(cd, DBS_ERR_ARG, "dbs_cdata_add_coef: null list");
+  DBS_CHECK(kpol >= cd->kpol1 && kpol <= cd->kpol2 && itype >= 1 && itype <= cd->ntype, DBS_ERR_ARG,
+            "Add_coef: kpol or itype out of range");
+  const int s = cd->slot(kpol, itype);
+  int i = cd->iblk[s];
+  const int ip = cd->ipblk[i];
+  const int jp = cdata_insert(cd, ip, cd->jpblk[i], c, j1, j2, j3, j4);
+  DBS_CHECK(jp >= 0, DBS_ERR_CAPACITY, "Promlems with jp in Add_cdata");
+  cd->jpblk[i] = jp;
+  if (jp - ip + 1 < cd->mb) return DBS_OK;
+  if (cd->nblk[s] < cd->kb && cdata_new_block(cd, s)) return DBS_OK;
+  // everything is full: generate the matrix of the fullest type
+  int jtype = itype, jpol = kpol, n = cd->nblk[s];
+  for (int t = 1; t <= cd->ntype; ++t)
+    for (int k = cd->kpol1; k <= cd->kpol2; ++k) {
+      const int q = cd->slot(k, t);
+      if (cd->nblk[q] <= n) continue;
+      jtype = t; jpol = k; n = cd->nblk[q];
+    }
+  int rc = cdata_add_matrix(cd, jtype, jpol);
+  if (rc) return rc;
+  if (jtype == itype && jpol == kpol) return DBS_OK;
+  DBS_CHECK(cdata_new_block(cd, s), DBS_ERR_CAPACITY, " Add_coef: problems with new block ");
+  return DBS_OK;
+}
+
+int dbs_cdata_add_matrix(dbs_cdata* cd, int itype, int kpol) {
+  DBS_CHECK(cd, DBS_ERR_ARG, "dbs_cdata_add_matrix: null list");
+  DBS_CHECK(kpol >= cd->kpol1 && kpol <= cd->kpol2 && itype >= 1 && itype <= cd->ntype, DBS_ERR_ARG,
+            "Add_matrix: kpol or itype out of range");
+  return cdata_add_matrix(cd, itype, kpol);
+}
+
+int dbs_smu(int ka, int kb, int kc, int kd, int L, int v, double* S, double* smu) {
+  DBS_CHECK(S && smu, DBS_ERR_ARG, "dbs_smu: null argument");
+  DBS_CHECK(ka != 0 && kb != 0 && kc != 0 && kd != 0, DBS_ERR_ARG, "l_kappa: kappa=0");
+  *smu = smu_factors(ka, kb, kc, kd, L, v, S);
+  return DBS_OK;
+}
+
+int dbs_add_integral_ci(dbs_cdata* cd, int kpol, int ic, int jc, int i1, int i2, int i3, int i4,
+                        double c, const int* kappa, int ibi, int mbreit) {
+  DBS_CHECK(cd && kappa, DBS_ERR_ARG, "Add_integral: null argument");
+  int rc;
+  const int m = (l_kappa(kappa[i1]) + l_kappa(kappa[i3]) + kpol) % 2 +
+                (l_kappa(kappa[i2]) + l_kappa(kappa[i4]) + kpol) % 2;
+  if (m == 0)
+    for (int t = 1; t <= 4; ++t)
+      if ((rc = dbs_cdata_add_coef(cd, c, kpol, i1 * ibi + i3, i2 * ibi + i4, ic, jc, t))) return rc;
+  if (mbreit == 0) return DBS_OK;
+  double S[8];
+  for (int v = kpol - 1; v <= kpol + 1; ++v) {
+    if (smu_factors(kappa[i1], kappa[i2], kappa[i3], kappa[i4], kpol, v, S) == 0.0) continue;
+    const int a = i1 * ibi + i3, b = i2 * ibi + i4, a2 = i3 * ibi + i1, b2 = i4 * ibi + i2;
+    const int keys[8][3] = {{a, b, 5}, {b, a, 5}, {a2, b2, 5}, {b2, a2, 5},
+                            {a, b, 6}, {b2, a2, 6}, {a2, b2, 6}, {b, a, 6}};
+    for (int q = 0; q < 8; ++q)
+      if ((rc = dbs_cdata_add_coef(cd, c * S[q], v, keys[q][0], keys[q][1], ic, jc, keys[q][2]))) return rc;
+  }
+  return DBS_OK;
+}
+
+// Idef_Rtype for channel-channel terms (structures 6.0-9.0)
+int dbs_add_integral_mat(dbs_cdata* cd, int kpol, int j1, int j2, int j3, int j4, double c,
+                         const int* lorb, const int* ipbs, int ibo, int mk, double eps_c) {
+  DBS_CHECK(cd && lorb && ipbs, DBS_ERR_ARG, "Add_integral: null argument");
+  const int m = (lorb[j1] + lorb[j3] + kpol) % 2 + (lorb[j2] + lorb[j4] + kpol) % 2;
+  if (m != 0 || kpol > mk) return DBS_OK;
+  if (std::fabs(c) < eps_c) return DBS_OK;
+  int i1, i2, i3, i4;
+  if (ipbs[j1] + ipbs[j2] == 0 || ipbs[j3] + ipbs[j4] == 0) {
+    i1 = j1; i3 = j3;
+    if (j1 > j3) { i1 = j3; i3 = j1; }
+    i2 = j2; i4 = j4;
+    if (j2 > j4) { i2 = j4; i4 = j2; }
+    if (i1 > i2) { std::swap(i1, i2); std::swap(i3, i4); }
+  } else {
+    i1 = j1; i2 = j2; i3 = j3; i4 = j4;
+  }
+  int k1 = 0, k2 = 0, k3 = 0, itype = 0;
+  if (ipbs[i1] > 0 && ipbs[i3] > 0) {
+    itype = 3;
+    if (ipbs[i1] >= ipbs[i3]) { k1 = ipbs[i1] * ibo + ipbs[i3]; k2 = i2; k3 = i4; if (k2 > k3) { k2 = i4; k3 = i2; } }
+    else { k1 = ipbs[i3] * ibo + ipbs[i1]; k2 = i4; k3 = i2; if (k2 > k3) { k2 = i2; k3 = i4; } }
+  } else if (ipbs[i2] > 0 && ipbs[i4] > 0) {
+    itype = 3;
+    if (ipbs[i2] >= ipbs[i4]) { k1 = ipbs[i2] * ibo + ipbs[i4]; k2 = i1; k3 = i3; if (k2 > k3) { k2 = i3; k3 = i1; } }
+    else { k1 = ipbs[i4] * ibo + ipbs[i2]; k2 = i3; k3 = i1; if (k2 > k3) { k2 = i1; k3 = i3; } }
+  } else if (ipbs[i1] > 0 && ipbs[i4] > 0) {
+    itype = 4;
+    if (ipbs[i1] >= ipbs[i4]) { k1 = ipbs[i1] * ibo + ipbs[i4]; k2 = i3; k3 = i2; }
+    else { k1 = ipbs[i4] * ibo + ipbs[i1]; k2 = i2; k3 = i3; }
+  } else if (ipbs[i2] > 0 && ipbs[i3] > 0) {
+    itype = 4;
+    if (ipbs[i3] >= ipbs[i2]) { k1 = ipbs[i3] * ibo + ipbs[i2]; k2 = i1; k3 = i4; }
+    else { k1 = ipbs[i2] * ibo + ipbs[i3]; k2 = i4; k3 = i1; }
+  }
+  DBS_CHECK(itype != 0, DBS_ERR_ARG, "Idef_Rtype: itype=0");
+  return dbs_cdata_add_coef(cd, c, kpol, k1, k2, k3, 0, itype);
+}
+
+int dbs_decode_int(int n, const int* code, int* met, int* k, int* i1, int* i2, int* i3, int* i4) {
+  DBS_CHECK(n >= 0 && code && met && k && i1 && i2 && i3 && i4, DBS_ERR_ARG, "Decode_int: null argument");
+  for (int q = 0; q < n; ++q) {
+    int ii = code[q];
+    met[q] = ii % 4; ii /= 4;
+    k[q] = ii % 1024; ii /= 1024;
+    i4[q] = ii % 32; ii /= 32;
+    i3[q] = ii % 32; ii /= 32;
+    i2[q] = ii % 32;
+    i1[q] = ii / 32;
+  }
+  return DBS_OK;
+}
+
+int dbs_rk_evaluate_dev(dbs_ctx* ctx, int itype, int k, int nc, const double* c, const int* K1,
+                        const int* K2, const int* K3, const int* K4, double eps_c, int ibi,
+                        const double* pq_dev, int norb, double* HM_dev, int ldhm, int nzero,
+                        double* DM_dev) {
+  DBS_CHECK(ctx && pq_dev && HM_dev, DBS_ERR_ARG, "Rk_evaluate: null argument");
+  DBS_CHECK(itype >= 1 && itype <= 6, DBS_ERR_ARG, "Rk_evaluate: unknown type of intgrals");
+  if (nc <= 0) return DBS_OK;
+  // entries that survive |c| >= eps_c, as rows (k, a, b, c, d)
+  std::vector<int> rows, keep;
+  for (int j = 0; j < nc; ++j) {
+    if (std::fabs(c[j]) < eps_c) continue;
+    const int a = K1[j] / ibi, cc = K1[j] % ibi, b = K2[j] / ibi, d = K2[j] % ibi;
+    DBS_CHECK(a >= 1 && a <= norb && cc >= 1 && cc <= norb && b >= 1 && b <= norb && d >= 1 && d <= norb,
+              DBS_ERR_ARG, "Rk_evaluate: orbital index out of range");
+    rows.insert(rows.end(), {k, a, b, cc, d});
+    keep.push_back(j);
+  }
+  const int n = (int)keep.size();
+  if (n == 0) return DBS_OK;
+  // targets in first-touch order, entries in list order
+  std::map<long long, int> tindex;
+  std::vector<long long> tpos;
+  std::vector<std::vector<int>> tent;
+  for (int q = 0; q < n; ++q) {
+    const int j = keep[q];
+    const int ic = K3[j], jc = K4[j];
+    long long pos;
+    if (jc <= nzero) pos = (long long)(ic - 1) + (long long)ldhm * (jc - 1);
+    else if (ic == jc) {
+      DBS_CHECK(DM_dev, DBS_ERR_ARG, "Rk_evaluate: DM needed");
+      pos = -(long long)jc;  // DM(jc) -> -(jc-1)-1
+    } else continue;
+    auto it = tindex.find(pos);
+    int t;
+    if (it == tindex.end()) {
+      t = (int)tpos.size();
+      tindex[pos] = t;
+      tpos.push_back(pos);
+      tent.emplace_back();
+    } else {
+      t = it->second;
+    }
+    tent[t].push_back(q);
+  }
+  std::vector<int> toff(1, 0), tflat;
+  for (auto& v : tent) {
+    tflat.insert(tflat.end(), v.begin(), v.end());
+    toff.push_back((int)tflat.size());
+  }
+  std::vector<double> ck(n);
+  for (int q = 0; q < n; ++q) ck[q] = c[keep[q]];
+  // integral values by the engine
+  dbs_engine* eng = nullptr;
+  const int nr = n;
+  int rc = dbs_engine_create(&eng, 1, &ctx, &norb, &nr, rows.data(), itype);
+  if (rc) return rc;
+  double *epq = nullptr, *evals = nullptr;
+  dbs_engine_buffers(eng, 0, &epq, nullptr, &evals, nullptr, nullptr);
+  hipStream_t s = ctx->stream;
+  long long *d_tpos = nullptr;
+  int *d_toff = nullptr, *d_tent = nullptr;
+  double* d_c = nullptr;
+  do {
+    if (hipMemcpyAsync(epq, pq_dev, sizeof(double) * (size_t)ctx->ns * 2 * norb, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) { rc = fail(DBS_ERR_HIP, "Rk_evaluate: copy"); break; }
+    if ((rc = dbs_engine_forward(eng))) break;
+    if ((rc = dbs_engine_sync(eng))) break;
+    if ((rc = dev_upload(&d_tpos, tpos, s)) || (rc = dev_upload(&d_toff, toff, s)) ||
+        (rc = dev_upload(&d_tent, tflat, s)) || (rc = dev_upload(&d_c, ck, s))) break;
+    const int nt = (int)tpos.size();
+    k_accumulate_targets<<<grid_of(nt, 256), 256, 0, s>>>(nt, d_tpos, d_toff, d_tent, evals, d_c, HM_dev, DM_dev);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) rc = fail(DBS_ERR_HIP, std::string("Rk_evaluate: ") + hipGetErrorString(e));
+  } while (0);
+  hipFree(d_tpos); hipFree(d_toff); hipFree(d_tent); hipFree(d_c);
+  dbs_engine_destroy(eng);
+  return rc;
+}
+
+int dbs_rdata_34_dev(dbs_ctx* ctx, int itype, int atype, int k, int nc, const double* c,
+                     const int* K1, const int* K2, const int* K3, int ibo, const double* pq_dev,
+                     int norb, const int* icc, int nch, double* hch_dev, int nblocks) {
+  DBS_CHECK(ctx && c && K1 && K2 && K3 && pq_dev && icc && hch_dev, DBS_ERR_ARG, "R_data: null argument");
+  DBS_CHECK(itype == 3 || itype == 4, DBS_ERR_ARG, " R_data: unknown itype ");
+  DBS_CHECK(atype >= 0 && atype <= 3, DBS_ERR_ARG, "R_data: atype out of range");
+  if (nc <= 0) return DBS_OK;
+  const int ns = ctx->ns, ks = ctx->ks;
+  int ip1, ip2, jp1, jp2;
+  switch (atype) {
+    case 0: ip1 = 1; ip2 = 1; jp1 = 1; jp2 = 1; break;
+    case 1: ip1 = 2; ip2 = 2; jp1 = 2; jp2 = 2; break;
+    case 2: ip1 = 1; ip2 = 1; jp1 = 2; jp2 = 2; if (itype == 4) { ip1 = 1; ip2 = 2; jp1 = 1; jp2 = 2; } break;
+    default: ip1 = 2; ip2 = 2; jp1 = 1; jp2 = 1; if (itype == 4) { ip1 = 2; ip2 = 1; jp1 = 2; jp2 = 1; } break;
+  }
+  const int type = atype + 1;  // mrk_pppp, qqqq, pqpq, qpqp
+  // runs of equal K1 -> groups
+  std::vector<int> goff(1, 0), ej1, ej2;
+  std::vector<double> ec;
+  std::vector<int4> ginfo;
+  for (int j = 0; j < nc; ++j) {
+    DBS_CHECK(K2[j] >= 1 && K2[j] <= norb && K3[j] >= 1 && K3[j] <= norb, DBS_ERR_ARG,
+              "R_data: orbital index out of range");
+    ej1.push_back(K2[j]);
+    ej2.push_back(K3[j]);
+    ec.push_back(c[j]);
+    const bool last = (j == nc - 1) || (K1[j] != K1[j + 1]);
+    if (!last) continue;
+    const int ich = K1[j] / ibo, jch = K1[j] % ibo;
+    DBS_CHECK(ich >= 0, DBS_ERR_ARG, "UPDATE_HS: ich < 0");
+    DBS_CHECK(jch >= 0, DBS_ERR_ARG, "UPDATE_HS: jch < 0");
+    DBS_CHECK(ich <= nch, DBS_ERR_ARG, "UPDATE_HS: ich > nch");
+    DBS_CHECK(jch <= nch, DBS_ERR_ARG, "UPDATE_HS: jch > nch");
+    const int blk = (ich >= 1 && jch >= 1) ? icc[(ich - 1) + (size_t)nch * (jch - 1)] : 0;
+    DBS_CHECK(blk >= 0 && blk <= nblocks, DBS_ERR_ARG, "UPDATE_HX: i out of range");
+    if (blk == 0) {  // block not stored: drop the run
+      ej1.resize(goff.back()); ej2.resize(goff.back()); ec.resize(goff.back());
+      continue;
+    }
+    ginfo.push_back(make_int4(ich, jch, blk - 1, 0));
+    goff.push_back((int)ec.size());
+  }
+  const int ng = (int)ginfo.size();
+  if (ng == 0) return DBS_OK;
+  double* rkb = nullptr;
+  int rc = build_rkb(ctx, type, k, &rkb);
+  if (rc) return rc;
+  hipStream_t s = ctx->stream;
+  const int sym = itype == 3 ? 0 : 1;
+  const long long per = (long long)ns * (sym == 0 ? ks : ns);
+  int *d_goff = nullptr, *d_j1 = nullptr, *d_j2 = nullptr;
+  double *d_c = nullptr, *d_xx = nullptr, *d_a = nullptr;
+  int4* d_gi = nullptr;
+  do {
+    if ((rc = dev_upload(&d_goff, goff, s)) || (rc = dev_upload(&d_j1, ej1, s)) ||
+        (rc = dev_upload(&d_j2, ej2, s)) || (rc = dev_upload(&d_c, ec, s)) ||
+        (rc = dev_upload(&d_gi, ginfo, s))) break;
+    if (hipMalloc(&d_xx, sizeof(double) * per * ng) != hipSuccess ||
+        hipMalloc(&d_a, sizeof(double) * per * ng) != hipSuccess) { rc = fail(DBS_ERR_HIP, "R_data: hipMalloc"); break; }
+    k_group_density<<<grid_of(per * ng, 256), 256, 0, s>>>(ng, d_goff, d_j1, d_j2, d_c, pq_dev, ns, ks,
+                                                            jp1, jp2, sym, d_xx);
+    if (sym == 0) {
+      constexpr int GB = 8;
+      const long long nthr = (long long)ns * ks * ((ng + GB - 1) / GB);
+      k_group_convol1<GB><<<grid_of(nthr, 256), 256, 0, s>>>(ng, ns, ks, rkb, d_xx, d_a);
+    } else {
+      k_group_convol3<<<grid_of(per * ng, 256), 256, 0, s>>>(ng, ns, ks, rkb, d_xx, d_a);
+    }
+    const long long nupd = (long long)ns * ns * ng;
+    for (int pass = 0; pass < 2; ++pass)
+      k_update_hs_mat<<<grid_of(nupd, 256), 256, 0, s>>>(ng, d_gi, ns, ks, sym, ip1, ip2, pass, d_a, hch_dev);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) rc = fail(DBS_ERR_HIP, std::string("R_data: ") + hipGetErrorString(e));
+  } while (0);
+  hipFree(d_goff); hipFree(d_j1); hipFree(d_j2); hipFree(d_c); hipFree(d_gi); hipFree(d_xx); hipFree(d_a);
+  return rc;
+}
+
+}  // extern "C"

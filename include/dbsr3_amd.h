@@ -129,13 +129,16 @@ int dbs_cdata_destroy(dbs_cdata *cd);
 int dbs_cdata_add_coef(dbs_cdata *cd, double c, int kpol, int j1, int j2, int j3, int j4, int itype);
 int dbs_cdata_add_matrix(dbs_cdata *cd, int itype, int kpol);
 
-/* Add_integral, CI3 form (DBSR_CI3/add_integral.f90:2-43), Rk part: the
- * parity rule on l(i1)+l(i3)+k, l(i2)+l(i4)+k, then Add_coef for types
- * 1..4 with K1=i1*ibi+i3, K2=i2*ibi+i4, K3=ic, K4=jc.  lorb[i] = l of
- * 1-based orbital i (lorb[0] unused).  Returns DBS_OK whether or not the
- * parity rule dropped the term. */
+/* Add_integral, CI3 form (DBSR_CI3/add_integral.f90:2-43): the parity
+ * rule on l(i1)+l(i3)+k, l(i2)+l(i4)+k, then Add_coef for types 1..4 with
+ * K1=i1*ibi+i3, K2=i2*ibi+i4, K3=ic, K4=jc; with mbreit != 0 also the eight
+ * S^k terms (types 5/6, multipoles k-1..k+1) weighted by the Breit factors
+ * SMU (DBSR_HF/hf_energy_BR.f90:58-121).  kappa[i] = kappa of 1-based
+ * orbital i (kappa[0] unused); l = l_kappa(kappa). */
 int dbs_add_integral_ci(dbs_cdata *cd, int kpol, int ic, int jc, int i1, int i2, int i3, int i4,
-                        double c, const int *lorb, int ibi);
+                        double c, const int *kappa, int ibi, int mbreit);
+/* SMU itself: returns the factor sum and the eight factors in S[8] */
+int dbs_smu(int ka, int kb, int kc, int kd, int L, int v, double *S, double *smu);
 /* Add_integral, MAT3 R-branch for channel-channel terms
  * (DBSR_MAT3/add_integral.f90:28-32 + Idef_Rtype idef_type.f90:142-302,
  * structures 6.0-9.0 with ic=jc=io=jo=0).  ipbs[i] > 0 marks orbital i as
@@ -188,6 +191,12 @@ int dbs_engine_buffers(dbs_engine *eng, int inst, double **pq, double **coef, do
                        double **grad, double **energy);
 int dbs_engine_forward(dbs_engine *eng);
 int dbs_engine_forward_adjoint(dbs_engine *eng);
+/* the same values by the plain path -- materialised arrays (mrk_ and msk_),
+ * one convol case 2 per distinct left density and one SUM_AmB per row, the
+ * organisation of Rk_evaluate (DBSR_CI3/rk_evaluate.f90:31-56) and rk()
+ * (dbsr_lib_dbs.f90:3981-4022).  Runs on the instances' context streams and
+ * returns after they finish. */
+int dbs_engine_forward_plain(dbs_engine *eng);
 /* capture forward(+adjoint) into a hipGraph; launch replays it */
 int dbs_engine_graph_capture(dbs_engine *eng, int with_adjoint);
 int dbs_engine_graph_launch(dbs_engine *eng);

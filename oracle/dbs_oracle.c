@@ -1262,3 +1262,80 @@ void orc_decode_int(int code, int *met, int *k, int *i1, int *i2, int *i3, int *
   *i3 = ii % 32; ii /= 32;
   *i2 = ii % 32; *i1 = ii / 32;
 }
+
+/* l_kappa: DBSR_HF/dbsr_lib_zcom.f90:500-512 */
+int orc_l_kappa(int kappa) { return kappa > 0 ? kappa : -kappa - 1; }
+
+/* SMU: Breit factors of Grant and Pyper, DBSR_HF/hf_energy_BR.f90:58-121 */
+double orc_smu(int ka, int kb, int kc, int kd, int L, int v, double *S) {
+  double smu = 0.0;
+  for (int i = 0; i < 8; ++i) S[i] = 0.0;
+  if (L < 0 || v < 0) return 0.0;
+  int la = orc_l_kappa(ka), lc = orc_l_kappa(kc);
+  if ((la + lc + v) % 2 == 0) return 0.0;
+  int lb = orc_l_kappa(kb), ld = orc_l_kappa(kd);
+  if ((lb + ld + v) % 2 == 0) return 0.0;
+  if (v == L) {
+    if (L == 0) return 0.0;
+    smu = (double)(-(ka + kc) * (kb + kd));
+    smu = smu / L / (L + 1);
+    for (int i = 0; i < 8; ++i) S[i] = smu;
+    return smu;
+  }
+  int K = kc - ka, Kp = kd - kb;
+  if (v == L - 1) {
+    double bp = L + 1; bp = bp / 2 / (L + L - 1);
+    double cp = -(L - 2); cp = cp / 2 / L / (L + L - 1);
+    S[0] = (L + K) * (bp + cp * Kp);
+    S[1] = (L + Kp) * (bp + cp * K);
+    S[2] = (L - K) * (bp - cp * Kp);
+    S[3] = (L - Kp) * (bp - cp * K);
+    S[4] = -(L + K) * (bp - cp * Kp);
+    S[5] = -(L - Kp) * (bp + cp * K);
+    S[6] = -(L - K) * (bp + cp * Kp);
+    S[7] = -(L + Kp) * (bp - cp * K);
+  } else if (v == L + 1) {
+    int i = -L - 1;
+    double b = L; b = b / 2 / (L + L + 3);
+    double c = (L + 3); c = c / (L + L + 2) / (L + L + 3);
+    S[0] = (i + Kp) * (b + c * K);
+    S[1] = (i + K) * (b + c * Kp);
+    S[2] = (i - Kp) * (b - c * K);
+    S[3] = (i - K) * (b - c * Kp);
+    S[4] = -(i - Kp) * (b + c * K);
+    S[5] = -(i + K) * (b - c * Kp);
+    S[6] = -(i + Kp) * (b - c * K);
+    S[7] = -(i - K) * (b + c * Kp);
+  }
+  for (int i = 0; i < 8; ++i) smu = smu + fabs(S[i]);
+  return smu;
+}
+
+/* Add_integral, CI3: DBSR_CI3/add_integral.f90:2-43.  kappa[i] of 1-based
+ * orbital i (kappa[0] unused); returns the reference's Add_coef calls in
+ * order through add_coef. */
+int orc_add_integral_ci(orc_cdata *cd, int kpol, int ic, int jc, int i1, int i2, int i3, int i4,
+                        double C, const int *kappa, int ibi, int mbreit) {
+  int rc = 0;
+  int m = (orc_l_kappa(kappa[i1]) + orc_l_kappa(kappa[i3]) + kpol) % 2 +
+          (orc_l_kappa(kappa[i2]) + orc_l_kappa(kappa[i4]) + kpol) % 2;
+  if (m == 0)
+    for (int t = 1; t <= 4; ++t)
+      if (orc_cdata_add_coef(cd, C, kpol, i1 * ibi + i3, i2 * ibi + i4, ic, jc, t) < 0) rc = -1;
+  if (mbreit == 0) return rc;
+  double S[8];
+  for (int v = kpol - 1; v <= kpol + 1; ++v) {
+    if (orc_smu(kappa[i1], kappa[i2], kappa[i3], kappa[i4], kpol, v, S) == 0.0) continue;
+    int r = 0;
+    r |= orc_cdata_add_coef(cd, C * S[0], v, i1 * ibi + i3, i2 * ibi + i4, ic, jc, 5);
+    r |= orc_cdata_add_coef(cd, C * S[1], v, i2 * ibi + i4, i1 * ibi + i3, ic, jc, 5);
+    r |= orc_cdata_add_coef(cd, C * S[2], v, i3 * ibi + i1, i4 * ibi + i2, ic, jc, 5);
+    r |= orc_cdata_add_coef(cd, C * S[3], v, i4 * ibi + i2, i3 * ibi + i1, ic, jc, 5);
+    r |= orc_cdata_add_coef(cd, C * S[4], v, i1 * ibi + i3, i2 * ibi + i4, ic, jc, 6);
+    r |= orc_cdata_add_coef(cd, C * S[5], v, i4 * ibi + i2, i3 * ibi + i1, ic, jc, 6);
+    r |= orc_cdata_add_coef(cd, C * S[6], v, i3 * ibi + i1, i4 * ibi + i2, ic, jc, 6);
+    r |= orc_cdata_add_coef(cd, C * S[7], v, i2 * ibi + i4, i1 * ibi + i3, ic, jc, 6);
+    if (r < 0) rc = -1;
+  }
+  return rc;
+}

@@ -99,6 +99,10 @@ void orc_rdata_34(int ns, int ks, const double *rkb, int itype, int atype,
 int  orc_idef_rtype(int j1, int j2, int j3, int j4, const int *ipbs, int ibo,
                     int *k1, int *k2, int *k3, int *k4);
 void orc_decode_int(int code, int *met, int *k, int *i1, int *i2, int *i3, int *i4);
+int  orc_l_kappa(int kappa);
+double orc_smu(int ka, int kb, int kc, int kd, int L, int v, double *S);
+int  orc_add_integral_ci(orc_cdata *cd, int kpol, int ic, int jc, int i1, int i2, int i3, int i4,
+                         double C, const int *kappa, int ibi, int mbreit);
 
 #ifdef __cplusplus
 }

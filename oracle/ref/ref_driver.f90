@@ -193,3 +193,14 @@
       c = sym
       Call UPDATE_HS(ms, hm, ii, jj, ns, ks, d, c)
       End Subroutine ref_update_hs
+
+      Function ref_smu(ka, kb, kc, kd, L, v, S) result(r) bind(C, name='ref_smu')
+      Use iso_c_binding
+      Implicit none
+      Integer(c_int), value :: ka, kb, kc, kd, L, v
+      Real(c_double) :: S(8), r
+      Real(8), external :: SMU
+      Integer :: a, b, c, d, ll, vv
+      a = ka; b = kb; c = kc; d = kd; ll = L; vv = v
+      r = SMU(a, b, c, d, ll, vv, S)
+      End Function ref_smu

@@ -1,0 +1,296 @@
+"""GPU parity of the integral engine (dbs_engine_*) against the oracle.
+
+Values: every row R^k(a,b;c,d) (rk() sum, single P/Q types, S^k types) is
+compared with the oracle's materialised-array evaluation (mrk/msk +
+convol case 2 + SUM_AmB, the reference's rk()/Rk_evaluate path).  The
+engine sums the same quadrature in another order; the bound is 1e-12
+relative, with the relative error of values smaller than 1e-4 of the set's
+largest magnitude measured against that 1e-4 floor (cancellation).
+
+Adjoint: dE/d(pq) of E = sum c_n R_n against the oracle's potentials --
+convol case 1 (direct, over r2) and case 2 (over r1) of every row's
+densities, unfolded by Full_mat_sym and applied to the partner orbital, as
+hf_matrix does (DBSR_HF/hf_matrix.f90:41-59) -- and against central finite
+differences of the engine's own energy.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import fe_grid, random_orbitals, small_grid
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+TYPES_OF_KIND = {0: [(O.PPPP, 0, 0), (O.QQQQ, 1, 1), (O.QPQP, 1, 0), (O.PQPQ, 0, 1)],
+                 1: [(O.PPPP, 0, 0)], 2: [(O.QQQQ, 1, 1)], 3: [(O.PQPQ, 0, 1)], 4: [(O.QPQP, 1, 0)]}
+
+
+def relerr(x, ref):
+    ref = np.asarray(ref)
+    floor = 1e-4 * np.max(np.abs(ref))
+    return np.abs(np.asarray(x) - ref) / np.maximum(np.abs(ref), floor)
+
+
+def oracle_values(g, fs, rows, kind):
+    ns, ks = g.ns, g.ks
+    out = np.zeros(len(rows))
+    dcache = {}
+
+    def dens(a, c, ca, cc, typ):
+        key = (a, c, ca, cc, typ)
+        if key not in dcache:
+            dcache[key] = O.density(ns, ks, fs[a][:, ca], fs[c][:, cc], typ)
+        return dcache[key]
+
+    for n, (k, a, b, c, d) in enumerate(rows):
+        a, b, c, d = a - 1, b - 1, c - 1, d - 1
+        if kind <= 4:
+            s = 0.0
+            for t, cl, cr in TYPES_OF_KIND[kind]:
+                rkb = g.mrk(t, k)
+                conv = O.convol(ns, ks, rkb, dens(a, c, cl, cl, "s"), 2, "s", "s")
+                s += O.sum_amb(ns, ks, conv, dens(b, d, cr, cr, "s"), "s")
+            out[n] = s
+        else:
+            t = kind
+            rkb = g.mrk(t, k)
+            dl = dens(a, c, 0, 1, "n")
+            dr = dens(b, d, 0, 1, "n") if t == O.PPQQ else dens(b, d, 1, 0, "n")
+            conv = O.convol(ns, ks, rkb, dl, 2, "n", "n")
+            out[n] = O.sum_amb(ns, ks, conv, dr, "n")
+    return out
+
+
+def unfold_s(a, ns, ks):
+    y = np.zeros((ns, ns))
+    for jp in range(ks):
+        for i in range(ns - jp):
+            y[i, i + jp] = a[i, jp]
+            y[i + jp, i] = a[i, jp]
+    return y
+
+
+def oracle_gradient(g, fs, rows, coef):
+    """dE/d(pq) for E = sum c*rk via convol case 1/2 potentials (kind 0)."""
+    ns, ks = g.ns, g.ks
+    grad = np.zeros((ns, 2, len(fs)), order="F")
+    for (k, a, b, c, d), cf in zip(rows, coef):
+        a, b, c, d = a - 1, b - 1, c - 1, d - 1
+        for t, cl, cr in TYPES_OF_KIND[0]:
+            rkb = g.mrk(t, k)
+            dl = O.density(ns, ks, fs[a][:, cl], fs[c][:, cl], "s")
+            dr = O.density(ns, ks, fs[b][:, cr], fs[d][:, cr], "s")
+            vl = unfold_s(O.convol(ns, ks, rkb, dr, 1, "s", "s"), ns, ks)  # dR/dDL
+            vr = unfold_s(O.convol(ns, ks, rkb, dl, 2, "s", "s"), ns, ks)  # dR/dDR
+            grad[:, cl, a] += cf * vl @ fs[c][:, cl]
+            grad[:, cl, c] += cf * vl @ fs[a][:, cl]
+            grad[:, cr, b] += cf * vr @ fs[d][:, cr]
+            grad[:, cr, d] += cf * vr @ fs[b][:, cr]
+    return grad
+
+
+def make_rows(rng, norb, kmax, n):
+    rows = np.zeros((n, 5), dtype=np.int32)
+    rows[:, 0] = rng.integers(0, kmax + 1, n)
+    rows[:, 1:] = rng.integers(1, norb + 1, (n, 4))
+    return rows
+
+
+@pytest.fixture(scope="module")
+def setups(gpu_available):
+    from dbsr3_amd import DBS
+    out = {}
+    for name, spec in (("small", small_grid()), ("fe", fe_grid())):
+        t, ns, ks, ksp, ksq = spec
+        g = O.Grid(t, ns, ks, ksp, ksq)
+        out[name] = (g, DBS(t, ns, ks, ksp, ksq))
+    return out
+
+
+def pq_of(fs):
+    return np.stack(fs, axis=2)
+
+
+@pytest.mark.parametrize("name", ["small", "fe"])
+@pytest.mark.parametrize("kind", [0, 1, 2, 3, 4, 5, 6])
+def test_values(setups, name, kind):
+    from dbsr3_amd import Engine
+    g, d = setups[name]
+    norb = 5
+    fs = random_orbitals(g, norb, seed=100 + kind)
+    rng = np.random.default_rng(kind)
+    rows = make_rows(rng, norb, 4 if name == "fe" else 3, 60 if name == "fe" else 120)
+    print(f"[engine values {name} kind={kind} rows={len(rows)}]", flush=True)
+    eng = Engine([(d, norb, rows)], kind=kind)
+    eng.set_orbitals(0, pq_of(fs))
+    eng.forward()
+    got = eng.values()
+    ref = oracle_values(g, fs, rows, kind)
+    e = relerr(got, ref)
+    print(f"  max rel {e.max():.3e} median {np.median(e):.3e}")
+    assert e.max() < TOL, (name, kind, e.max())
+    # plain path (materialised arrays) on the device
+    eng.forward_plain()
+    gp = eng.values()
+    ep = relerr(gp, ref)
+    assert ep.max() < TOL, ("plain", name, kind, ep.max())
+
+
+@pytest.mark.parametrize("name", ["small", "fe"])
+def test_dense_path_matches_sparse(setups, name):
+    from dbsr3_amd import Engine
+    g, d = setups[name]
+    norb = 6
+    fs = random_orbitals(g, norb, seed=7)
+    # all pairs a<=c, b<=d at two k: a dense set
+    pairs = [(a, c) for a in range(1, norb + 1) for c in range(a, norb + 1)]
+    rows = np.array([(k, a, b, c, dd) for k in (0, 2) for (a, c) in pairs for (b, dd) in pairs],
+                    dtype=np.int32)
+    eng = Engine([(d, norb, rows)])
+    eng.set_orbitals(0, pq_of(fs))
+    eng.set_tuning([0])
+    eng.forward()
+    sparse = eng.values()
+    eng.set_tuning([1])
+    eng.forward()
+    dense = eng.values()
+    assert np.max(relerr(dense, sparse)) < 1e-13
+    ref = oracle_values(g, fs, rows[::37], 0)
+    assert np.max(relerr(dense[::37], ref)) < TOL
+
+
+@pytest.mark.parametrize("name", ["small", "fe"])
+def test_adjoint_vs_oracle_potentials(setups, name):
+    from dbsr3_amd import Engine
+    g, d = setups[name]
+    norb = 4
+    fs = random_orbitals(g, norb, seed=21)
+    rng = np.random.default_rng(21)
+    rows = make_rows(rng, norb, 3, 40 if name == "fe" else 80)
+    coef = rng.uniform(-1, 1, len(rows))
+    eng = Engine([(d, norb, rows)])
+    eng.set_orbitals(0, pq_of(fs))
+    eng.set_coef(0, coef)
+    eng.forward_adjoint()
+    E = eng.energy()
+    G = eng.grad()
+    vals = oracle_values(g, fs, rows, 0)
+    assert abs(E - vals @ coef) <= TOL * np.sum(np.abs(vals * coef))
+    Gref = oracle_gradient(g, fs, rows, coef)
+    scale = np.max(np.abs(Gref))
+    err = np.max(np.abs(G - Gref)) / scale
+    print(f"[adjoint {name}] max |dG|/max|G| = {err:.3e}")
+    assert err < TOL
+
+
+def test_adjoint_finite_differences(setups):
+    from dbsr3_amd import Engine
+    g, d = setups["fe"]
+    norb = 4
+    fs = random_orbitals(g, norb, seed=5)
+    rng = np.random.default_rng(5)
+    rows = make_rows(rng, norb, 4, 50)
+    coef = rng.uniform(-1, 1, len(rows))
+    eng = Engine([(d, norb, rows)])
+    pq = pq_of(fs)
+    eng.set_orbitals(0, pq)
+    eng.set_coef(0, coef)
+    eng.forward_adjoint()
+    G = eng.grad()
+    h = 1e-6
+    worst = 0.0
+    for _ in range(24):
+        i = int(rng.integers(1, g.nsp - 1))
+        comp = int(rng.integers(0, 2))
+        o = int(rng.integers(0, norb))
+        p2 = pq.copy(order="F")
+        p2[i, comp, o] += h
+        eng.set_orbitals(0, p2)
+        eng.forward()
+        ep = eng.energy()
+        p2[i, comp, o] -= 2 * h
+        eng.set_orbitals(0, p2)
+        eng.forward()
+        em = eng.energy()
+        fd = (ep - em) / (2 * h)
+        worst = max(worst, abs(fd - G[i, comp, o]) / max(abs(G[i, comp, o]), 1e-3 * np.max(np.abs(G))))
+    print(f"[finite differences] worst rel {worst:.3e}")
+    assert worst < 1e-6
+
+
+def test_batched_instances_match_single(setups):
+    """uneven instances (different grids, orders, norb, k ranges) in one engine"""
+    from dbsr3_amd import DBS, Engine
+    specs = []
+    rng = np.random.default_rng(9)
+    for n, (ns, ks) in enumerate(((20, 5), (31, 7), (26, 6), (40, 8))):
+        t = O.mkgrid_02(ns, ks, 0.25, 25.0, 3.0 + n)
+        g = O.Grid(t, ns, ks, ks - 1, ks)
+        d = DBS(t, ns, ks, ks - 1, ks)
+        norb = 3 + n
+        fs = random_orbitals(g, norb, seed=30 + n)
+        rows = make_rows(rng, norb, 2 + n, 30 + 10 * n)
+        coef = rng.uniform(-1, 1, len(rows))
+        specs.append((g, d, norb, fs, rows, coef))
+    eng = Engine([(d, norb, rows) for (g, d, norb, fs, rows, coef) in specs])
+    for i, (g, d, norb, fs, rows, coef) in enumerate(specs):
+        eng.set_orbitals(i, pq_of(fs))
+        eng.set_coef(i, coef)
+    eng.forward_adjoint()
+    for i, (g, d, norb, fs, rows, coef) in enumerate(specs):
+        single = Engine([(d, norb, rows)])
+        single.set_orbitals(0, pq_of(fs))
+        single.set_coef(0, coef)
+        single.forward_adjoint()
+        assert np.array_equal(eng.values(i), single.values()), i
+        assert np.array_equal(eng.grad(i), single.grad()), i
+        ref = oracle_values(g, fs, rows, 0)
+        assert relerr(eng.values(i), ref).max() < TOL
+
+
+def test_graph_replay_matches_eager(setups):
+    import torch
+    from dbsr3_amd import Engine
+    g, d = setups["fe"]
+    norb = 5
+    fs = random_orbitals(g, norb, seed=3)
+    rng = np.random.default_rng(3)
+    rows = make_rows(rng, norb, 4, 80)
+    coef = rng.uniform(-1, 1, len(rows))
+    eng = Engine([(d, norb, rows)])
+    eng.set_orbitals(0, pq_of(fs))
+    eng.set_coef(0, coef)
+    eng.forward_adjoint()
+    v0, g0, e0 = eng.values(), eng.grad(), eng.energy()
+    eng.graph_capture(with_adjoint=True)
+    for rep in range(4):
+        # eager work of the program between replays
+        x = torch.randn(1000, device="cuda").sum()
+        torch.cuda.synchronize()
+        eng.graph_launch()
+        eng.sync()
+        assert np.array_equal(eng.values(), v0), rep
+        assert np.array_equal(eng.grad(), g0), rep
+        assert eng.energy() == e0
+        del x
+
+
+def test_autotune_keeps_results(setups):
+    from dbsr3_amd import Engine
+    g, d = setups["small"]
+    norb = 5
+    fs = random_orbitals(g, norb, seed=8)
+    rng = np.random.default_rng(8)
+    rows = make_rows(rng, norb, 3, 200)
+    coef = rng.uniform(-1, 1, len(rows))
+    eng = Engine([(d, norb, rows)])
+    eng.set_orbitals(0, pq_of(fs))
+    eng.set_coef(0, coef)
+    eng.forward_adjoint()
+    v0, g0 = eng.values(), eng.grad()
+    best = eng.autotune(with_adjoint=True, reps=2)
+    assert len(best) == 5
+    eng.forward_adjoint()
+    assert np.max(relerr(eng.values(), v0)) < 1e-13
+    assert np.max(np.abs(eng.grad() - g0)) <= 1e-13 * np.max(np.abs(g0))
