@@ -74,6 +74,7 @@ PROTOTYPES = {
     "dbs_engine_autotune": [vp, i32, i32],
     "dbs_engine_work": [vp, i32, dp, dp],
     "dbs_engine_sync": [vp],
+    "dbs_engine_profile": [vp, i32, i32, i32, C.c_char_p, dp, dp, dp, ip],
 }
 
 _lib = None

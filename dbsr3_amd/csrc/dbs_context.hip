@@ -27,11 +27,27 @@ int kband(int type, int ks) { return type >= DBS_PPQQ ? 2 * ks - 1 : ks; }
 
 int ensure_scratch(dbs_ctx* c, size_t n) {
   if (c->scratch_n >= n) return DBS_OK;
-  if (c->scratch) DBS_HIP(hipFree(c->scratch));
+  if (c->scratch) {
+    DBS_HIP(hipStreamSynchronize(c->stream));
+    DBS_HIP(hipFree(c->scratch));
+  }
   c->scratch = nullptr;
   c->scratch_n = 0;
   DBS_HIP(hipMalloc(&c->scratch, n * sizeof(double)));
   c->scratch_n = n;
+  return DBS_OK;
+}
+
+int ensure_stage(dbs_ctx* c, size_t n) {
+  if (c->stage_n >= n) return DBS_OK;
+  if (c->stage) {
+    DBS_HIP(hipStreamSynchronize(c->stream));
+    DBS_HIP(hipFree(c->stage));
+  }
+  c->stage = nullptr;
+  c->stage_n = 0;
+  DBS_HIP(hipMalloc(&c->stage, n * sizeof(double)));
+  c->stage_n = n;
   return DBS_OK;
 }
 
@@ -224,7 +240,7 @@ int dbs_ctx_destroy(dbs_ctx* c) {
   hipSetDevice(c->device);
   if (c->stream) hipStreamSynchronize(c->stream);
   double* bufs[] = {c->t, c->tp, c->tq, c->gx, c->gw, c->gr, c->grw, c->grm, c->pbsp, c->qbsp,
-                    c->xin, c->win, c->bpin, c->bqin, c->mom, c->rkd, c->scratch};
+                    c->xin, c->win, c->bpin, c->bqin, c->mom, c->rkd, c->scratch, c->stage};
   for (double* p : bufs)
     if (p) hipFree(p);
   for (auto& kv : c->rkb_cache) hipFree(kv.second);

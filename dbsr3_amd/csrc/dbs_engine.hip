@@ -656,6 +656,13 @@ struct dbs_engine {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   int graph_adj = -1;
+  // per-phase timing (dbs_engine_profile): events recorded after each launch
+  bool profiling = false;
+  std::vector<hipEvent_t> prof_ev;
+  std::vector<std::string> prof_names;
+  size_t prof_pos = 0;
+  // per-phase algorithmic work of one launch: flops, minimum bytes
+  std::map<std::string, std::pair<double, double>> phase_work;
   // plain path scratch
   double* plain_d = nullptr;
   double* plain_c = nullptr;
@@ -765,31 +772,52 @@ static int refresh_sparse(dbs_engine* e) {
   return rc;
 }
 
+static void mark(dbs_engine* e, const char* name) {
+  if (!e->profiling) return;
+  if (e->prof_pos >= e->prof_ev.size()) {
+    hipEvent_t ev;
+    hipEventCreate(&ev);
+    e->prof_ev.push_back(ev);
+    e->prof_names.emplace_back();
+  }
+  hipEventRecord(e->prof_ev[e->prof_pos], e->stream);
+  e->prof_names[e->prof_pos] = name;
+  e->prof_pos++;
+}
+
 static int run_forward(dbs_engine* e) {
   hipStream_t s = e->stream;
+  mark(e, "start");
   const int cap = e->tune[3];
   k_orb_points<<<grid_for(e->total_orbv, 256, cap * 4), 256, 0, s>>>(
       e->ninst, e->d_seg_orb, e->d_inst, e->pq, e->orbv, e->total_orbv);
+  mark(e, "orb_points");
   k_weights<<<grid_for(e->total_w, 256, cap * 4), 256, 0, s>>>(e->nkg, e->d_seg_w, e->d_kg, e->d_inst,
                                                                 e->wt, e->total_w);
+  mark(e, "weights");
   int maxks = 0;
   for (auto& I : e->h_inst) maxks = std::max(maxks, I.ks);
   const size_t shm = sizeof(double) * (2 * maxks + maxks * maxks);
   k_features<<<grid_for(e->nitems, 1, cap), e->tune[1], shm, s>>>(e->nitems, e->d_items, e->d_kg, e->d_inst,
                                                                   e->d_dens, e->orbv, e->wt, e->F);
+  mark(e, "features");
   k_scan<<<grid_for(e->ndens, 4, cap), 256, 0, s>>>(e->ndens, e->d_dens, e->d_kg, e->d_inst, e->F, 0);
+  mark(e, "prefix_scan");
   SparseCache& sc = g_sparse[e];
   if (sc.n > 0) {
     const int bw = e->tune[2] * 64;
     k_contract_rows<<<grid_for(sc.n, e->tune[2], cap * 4), bw, 0, s>>>(
         sc.n, sc.rows, e->d_dens, e->d_kg, e->d_inst, e->F, sc.out, e->onesided, e->vals);
+    mark(e, "contract_rows");
   }
   if (e->ntiles > 0) {
     k_contract_dense<<<grid_for(e->ntiles, 4, cap * 4), 256, 0, s>>>(
         e->ntiles, e->d_tiles, e->d_kg, e->d_inst, e->d_dens, e->F, e->d_pairrow, e->d_pairoff,
         e->d_row_out, e->onesided, e->vals);
+    mark(e, "contract_dense");
   }
   k_energy<<<grid_for(e->ninst, 1, cap), 256, 0, s>>>(e->ninst, e->d_roff, e->coef, e->vals, e->energy);
+  mark(e, "energy");
   DBS_HIP(hipGetLastError());
   return DBS_OK;
 }
@@ -800,14 +828,58 @@ static int run_adjoint(dbs_engine* e) {
   k_adj_gather<<<grid_for(e->ndens, 1, cap * 4), e->tune[4], 0, s>>>(
       e->ndens, e->d_dens, e->d_kg, e->d_inst, e->d_adjoff, e->d_adj, e->coef, e->d_row_out, e->F,
       e->onesided, e->G);
+  mark(e, "adj_gather");
   k_scan<<<grid_for(e->ndens, 4, cap), 256, 0, s>>>(e->ndens, e->d_dens, e->d_kg, e->d_inst, e->G, 1);
+  mark(e, "adj_suffix_scan");
   k_adj_orb_points<<<grid_for(e->total_orbv, 256, cap * 4), 256, 0, s>>>(
       e->ninst, e->d_seg_orb, e->d_inst, e->d_kg, e->d_dens, e->d_oadjoff, e->d_oadj, e->orbv, e->wt,
       e->G, e->gorb, e->total_orbv);
+  mark(e, "adj_orb_points");
   k_adj_coef<<<grid_for(e->total_grad, 256, cap * 4), 256, 0, s>>>(e->ninst, e->d_seg_coef, e->d_inst,
                                                                     e->gorb, e->grad, e->total_grad);
+  mark(e, "adj_coef");
   DBS_HIP(hipGetLastError());
   return DBS_OK;
+}
+
+static void compute_phase_work(dbs_engine* e) {
+  const int ninst = e->ninst;
+  const std::vector<int>& norb = e->norb;
+  // per phase (one launch): flops and minimum HBM bytes
+    double pts = 0, pts_fl = 0, pq_b = 0, feat_fl = 0, feat_b = 0, scan_fl = 0, scan_b = 0;
+    double rows_fl = 0, rows_b = 0, rows_dense_fl = 0, gath_fl = 0, orbp_fl = 0, coef_fl = 0;
+    for (int i = 0; i < ninst; ++i) {
+      const InstDev& I = e->h_inst[i];
+      const double p = (double)norb[i] * 2 * I.nv * I.np;
+      pts += p;
+      pts_fl += p * 2 * I.ks;
+      pq_b += (double)norb[i] * 2 * I.ns * 8;
+      coef_fl += (double)norb[i] * 2 * I.ns * 2.0 * I.ks * I.np;
+    }
+    for (int g = 0; g < e->nkg; ++g) {
+      const InstDev& I = e->h_inst[e->h_kg[g].inst];
+      const double nv = I.nv, ks = I.ks, nd = e->h_kg[g].ndens, nr = (double)e->group_rows[g].size();
+      const double D = feat_dim(I.nv, I.ks);
+      const double rho = e->kind == 0 ? 3.0 : 1.0;  // flops per density point value
+      feat_fl += nd * nv * (I.np * rho + 4 * ks + ks + 2 * ks * ks);
+      feat_b += nd * D * 8;
+      scan_fl += nd * nv;
+      scan_b += nd * nv * 16;
+      const double prow = (e->onesided ? 2.0 : 4.0) * nv * (1 + ks);
+      if (e->group_dense[g]) rows_dense_fl += 2.0 * nd * nd * D;
+      else { rows_fl += nr * prow; rows_b += nd * D * 8 + nr * 8; }
+      gath_fl += 2.0 * nr * 2.0 * D;
+      orbp_fl += 2.0 * nd * nv * I.np * 3.0;
+    }
+    e->phase_work["orb_points"] = {pts_fl, pq_b + pts * 8};
+    e->phase_work["features"] = {feat_fl, feat_b};
+    e->phase_work["prefix_scan"] = {scan_fl, scan_b};
+    e->phase_work["contract_rows"] = {rows_fl, rows_b};
+    e->phase_work["contract_dense"] = {rows_dense_fl, 0.0};
+    e->phase_work["adj_gather"] = {gath_fl, feat_b * 2};
+    e->phase_work["adj_suffix_scan"] = {scan_fl, scan_b};
+    e->phase_work["adj_orb_points"] = {orbp_fl, pts * 8 * 2};
+    e->phase_work["adj_coef"] = {coef_fl, pts * 8 + pq_b};
 }
 
 extern "C" {
@@ -1059,6 +1131,7 @@ int dbs_engine_create(dbs_engine** out, int ninst, dbs_ctx* const* ctxs, const i
   e->group_dense.assign(e->nkg, 0);
   for (int g = 0; g < e->nkg; ++g) e->group_dense[g] = (char)choose_dense(e, e->h_kg[g], e->group_rows[g].size());
   rc = build_dense_tables(e);
+  compute_phase_work(e);
   if (!rc) rc = refresh_sparse(e);
   if (rc) return bail(rc);
   if (hipStreamSynchronize(s) != hipSuccess) return bail(fail(DBS_ERR_HIP, "dbs_engine_create: setup"));
@@ -1072,6 +1145,7 @@ int dbs_engine_destroy(dbs_engine* e) {
   if (e->stream) hipStreamSynchronize(e->stream);
   if (e->gexec) hipGraphExecDestroy(e->gexec);
   if (e->graph) hipGraphDestroy(e->graph);
+  for (hipEvent_t ev : e->prof_ev) hipEventDestroy(ev);
   void* bufs[] = {e->pq, e->coef, e->vals, e->grad, e->energy, e->orbv, e->gorb, e->wt, e->F, e->G,
                   e->d_inst, e->d_kg, e->d_dens, e->d_rows, e->d_row_out, e->d_seg_orb, e->d_seg_coef,
                   e->d_seg_w, e->d_roff, e->d_items, e->d_adjoff, e->d_adj, e->d_oadjoff, e->d_oadj,
@@ -1162,6 +1236,7 @@ int dbs_engine_set_tuning(dbs_engine* e, int n, const int* p) {
     int rc = build_dense_tables(e);
     if (!rc) rc = refresh_sparse(e);
     if (rc) return rc;
+    compute_phase_work(e);
     DBS_HIP(hipStreamSynchronize(e->stream));
   }
   // a captured graph holds the old launch shapes
@@ -1346,6 +1421,45 @@ int dbs_engine_forward_plain(dbs_engine* e) {
       return rc;
     }
   }
+  return DBS_OK;
+}
+
+// Per-phase device time: events recorded on the engine's stream after each
+// launch, averaged over reps.  names is nmax*32 chars.
+int dbs_engine_profile(dbs_engine* e, int with_adjoint, int reps, int nmax, char* names, double* ms,
+                       double* flops, double* bytes, int* nphase) {
+  DBS_CHECK(e && names && ms && nphase && nmax > 0, DBS_ERR_ARG, "dbs_engine_profile: bad arguments");
+  if (reps < 1) reps = 1;
+  std::vector<std::string> order;
+  std::map<std::string, double> acc;
+  for (int r = 0; r < reps; ++r) {
+    e->profiling = true;
+    e->prof_pos = 0;
+    int rc = run_forward(e);
+    if (!rc && with_adjoint) rc = run_adjoint(e);
+    e->profiling = false;
+    if (rc) return rc;
+    DBS_HIP(hipStreamSynchronize(e->stream));
+    for (size_t i = 1; i < e->prof_pos; ++i) {
+      float x = 0.f;
+      DBS_HIP(hipEventElapsedTime(&x, e->prof_ev[i - 1], e->prof_ev[i]));
+      const std::string& n = e->prof_names[i];
+      if (!acc.count(n)) order.push_back(n);
+      acc[n] += x;
+    }
+  }
+  int n = 0;
+  for (const auto& name : order) {
+    if (n >= nmax) break;
+    std::strncpy(names + 32 * n, name.c_str(), 31);
+    names[32 * n + 31] = 0;
+    ms[n] = acc[name] / reps;
+    auto w = e->phase_work.find(name);
+    if (flops) flops[n] = w == e->phase_work.end() ? 0.0 : w->second.first;
+    if (bytes) bytes[n] = w == e->phase_work.end() ? 0.0 : w->second.second;
+    ++n;
+  }
+  *nphase = n;
   return DBS_OK;
 }
 

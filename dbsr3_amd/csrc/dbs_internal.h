@@ -90,10 +90,14 @@ struct dbs_ctx {
   // general scratch for contractions
   double* scratch = nullptr;
   size_t scratch_n = 0;
+  // staging area of the host-pointer entry points
+  double* stage = nullptr;
+  size_t stage_n = 0;
 };
 
 namespace dbs {
 int ensure_scratch(dbs_ctx* c, size_t n);
+int ensure_stage(dbs_ctx* c, size_t n);
 int kband(int type, int ks);  // third/fourth dimension of rkb for the type
 int build_rkb(dbs_ctx* c, int type, int k, double** out);
 }  // namespace dbs

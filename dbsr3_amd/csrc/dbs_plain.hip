@@ -828,4 +828,86 @@ int dbs_rk_dev(dbs_ctx* c, const double* f1, const double* f2, const double* f3,
   return DBS_OK;
 }
 
+// ---------------------------------------------------------------------------
+// host-pointer forms: stage inputs into the context's device area, run the
+// device routine, copy the result back and synchronise.
+// ---------------------------------------------------------------------------
+static int h2d(dbs_ctx* c, double* dst, const double* src, size_t n) {
+  DBS_HIP(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  return DBS_OK;
+}
+static int d2h_sync(dbs_ctx* c, double* dst, const double* src, size_t n) {
+  DBS_HIP(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  DBS_HIP(hipStreamSynchronize(c->stream));
+  return DBS_OK;
+}
+static int band_ncol(char sym, int ns, int ks) {
+  return sym == 'n' ? 2 * ks - 1 : (sym == 'x' ? ns : ks);
+}
+
+int dbs_density(dbs_ctx* c, const double* p1, const double* p2, char type, double* d) {
+  DBS_CHECK(c && p1 && p2 && d, DBS_ERR_ARG, "dbs_density: null argument");
+  const size_t ns = c->ns;
+  const size_t nd = ns * (size_t)((type == 's' || type == 'u' || type == 'l') ? c->ks
+                                  : (type == 'n' ? 2 * c->ks - 1 : c->ns));
+  int rc = ensure_stage(c, 2 * ns + nd);
+  if (rc) return rc;
+  double *a = c->stage, *b = a + ns, *o = b + ns;
+  if ((rc = h2d(c, a, p1, ns)) || (rc = h2d(c, b, p2, ns))) return rc;
+  if ((rc = dbs_density_dev(c, a, b, type, o))) return rc;
+  return d2h_sync(c, d, o, nd);
+}
+
+int dbs_convol(dbs_ctx* c, int type, int k, int icase, char si, char sj, const double* d, double* a) {
+  DBS_CHECK(c && d && a, DBS_ERR_ARG, "dbs_convol: null argument");
+  DBS_CHECK(icase >= 1 && icase <= 4, DBS_ERR_ARG, "convol: unknown case");
+  const size_t ns = c->ns;
+  size_t nin, nout;
+  if (icase == 1) { nin = ns * band_ncol(sj, c->ns, c->ks); nout = ns * band_ncol(si, c->ns, c->ks); }
+  else if (icase == 2) { nin = ns * band_ncol(si, c->ns, c->ks); nout = ns * band_ncol(sj, c->ns, c->ks); }
+  else { nin = ns * ns; nout = ns * ns; }
+  int rc = ensure_stage(c, nin + nout);
+  if (rc) return rc;
+  double *din = c->stage, *dout = din + nin;
+  if ((rc = h2d(c, din, d, nin))) return rc;
+  if ((rc = dbs_convol_dev(c, type, k, icase, si, sj, din, dout))) return rc;
+  return d2h_sync(c, a, dout, nout);
+}
+
+int dbs_sum_amb(dbs_ctx* c, const double* a, const double* b, char sym, double* out) {
+  DBS_CHECK(c && a && b && out, DBS_ERR_ARG, "dbs_sum_amb: null argument");
+  const size_t n = (size_t)c->ns * band_ncol(sym, c->ns, c->ks);
+  int rc = ensure_stage(c, 2 * n + 1);
+  if (rc) return rc;
+  double *x = c->stage, *y = x + n, *o = y + n;
+  if ((rc = h2d(c, x, a, n)) || (rc = h2d(c, y, b, n))) return rc;
+  if ((rc = dbs_sum_amb_dev(c, x, y, sym, o))) return rc;
+  return d2h_sync(c, out, o, 1);
+}
+
+int dbs_rk(dbs_ctx* c, const double* f1, const double* f2, const double* f3, const double* f4,
+           int k, double* out) {
+  DBS_CHECK(c && f1 && f2 && f3 && f4 && out, DBS_ERR_ARG, "dbs_rk: null argument");
+  const size_t n = 2 * (size_t)c->ns;
+  int rc = ensure_stage(c, 4 * n + 1);
+  if (rc) return rc;
+  double* s = c->stage;
+  const double* src[4] = {f1, f2, f3, f4};
+  for (int q = 0; q < 4; ++q)
+    if ((rc = h2d(c, s + q * n, src[q], n))) return rc;
+  if ((rc = dbs_rk_dev(c, s, s + n, s + 2 * n, s + 3 * n, k, s + 4 * n))) return rc;
+  return d2h_sync(c, out, s + 4 * n, 1);
+}
+
+int dbs_quadr(dbs_ctx* c, const double* fi, const double* fj, int m, double* out) {
+  DBS_CHECK(c && fi && fj && out, DBS_ERR_ARG, "dbs_quadr: null argument");
+  const size_t n = 2 * (size_t)c->ns;
+  int rc = ensure_stage(c, 2 * n + 1);
+  if (rc) return rc;
+  double* s = c->stage;
+  if ((rc = h2d(c, s, fi, n)) || (rc = h2d(c, s + n, fj, n))) return rc;
+  if ((rc = dbs_quadr_dev(c, s, s + n, m, s + 2 * n))) return rc;
+  return d2h_sync(c, out, s + 2 * n, 1);
+}
+
 }  // extern "C"

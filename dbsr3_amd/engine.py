@@ -127,3 +127,21 @@ class Engine:
         b = C.c_double()
         check(lib().dbs_engine_work(self.h, int(with_adjoint), C.byref(f), C.byref(b)))
         return f.value, b.value
+
+    def profile(self, with_adjoint=True, reps=5):
+        """per-phase device time (ms) and algorithmic work of one launch:
+        {name: (ms, flops, bytes)} in launch order"""
+        nmax = 32
+        names = C.create_string_buffer(32 * nmax)
+        ms, fl, by = np.zeros(nmax), np.zeros(nmax), np.zeros(nmax)
+        n = C.c_int()
+        dp = C.POINTER(C.c_double)
+        check(lib().dbs_engine_profile(self.h, int(with_adjoint), reps, nmax, names,
+                                       ms.ctypes.data_as(dp), fl.ctypes.data_as(dp),
+                                       by.ctypes.data_as(dp), C.byref(n)))
+        raw = names.raw
+        out = {}
+        for i in range(n.value):
+            nm = raw[32 * i:32 * (i + 1)].split(b"\0")[0].decode()
+            out[nm] = (float(ms[i]), float(fl[i]), float(by[i]))
+        return out

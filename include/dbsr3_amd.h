@@ -208,6 +208,12 @@ int dbs_engine_autotune(dbs_engine *eng, int with_adjoint, int reps);
 /* algorithmic flop count and minimum bytes of one forward (+adjoint) */
 int dbs_engine_work(dbs_engine *eng, int with_adjoint, double *flops, double *bytes);
 int dbs_engine_sync(dbs_engine *eng);
+/* device time of each phase of forward(+adjoint), from events recorded on
+ * the engine's stream after every launch, averaged over reps; names holds
+ * nmax entries of 32 chars; flops/bytes = that phase's algorithmic work
+ * (minimum HBM bytes) per launch; *nphase = number of phases returned */
+int dbs_engine_profile(dbs_engine *eng, int with_adjoint, int reps, int nmax, char *names,
+                       double *ms, double *flops, double *bytes, int *nphase);
 
 #ifdef __cplusplus
 }

@@ -328,3 +328,43 @@ def ref():
         L.ref_cdata_get.argtypes = [C.c_int] + [_ip] * 6 + [_dp, _ip]
         _ref = L
     return _ref
+
+
+# --------------------------------------------------------------------------
+# integral sets the way the reference evaluates them (rk(), Rk_evaluate):
+# materialised arrays, one convol case 2 per distinct left density, one
+# SUM_AmB per row.  Used as the CPU baseline and as the checker of sets.
+# --------------------------------------------------------------------------
+_SET_TYPES = {0: [(PPPP, 0, 0), (QQQQ, 1, 1), (QPQP, 1, 0), (PQPQ, 0, 1)],
+              1: [(PPPP, 0, 0)], 2: [(QQQQ, 1, 1)], 3: [(PQPQ, 0, 1)], 4: [(QPQP, 1, 0)]}
+
+
+def rk_rows(g, fs, rows, kind=0, deadline=None):
+    """values of rows (k,a,b,c,d), 1-based orbitals, for kind 0 (rk sum) or a
+    single R^k type 1..4; fs: list of (ns,2) Fortran arrays.  With a deadline
+    (time.perf_counter() value) stops after the row that passes it and
+    returns only the rows done."""
+    import time
+    ns, ks = g.ns, g.ks
+    out = np.zeros(len(rows))
+    dens = {}
+    conv = {}
+
+    def d_of(a, c, comp):
+        key = (min(a, c), max(a, c), comp)
+        if key not in dens:
+            dens[key] = density(ns, ks, fs[a][:, comp], fs[c][:, comp], "s")
+        return dens[key]
+
+    for n, (k, a, b, c, d) in enumerate(rows):
+        a, b, c, d = int(a) - 1, int(b) - 1, int(c) - 1, int(d) - 1
+        s = 0.0
+        for t, cl, cr in _SET_TYPES[kind]:
+            key = (t, int(k), min(a, c), max(a, c))
+            if key not in conv:
+                conv[key] = convol(ns, ks, g.mrk(t, int(k)), d_of(a, c, cl), 2, "s", "s")
+            s += sum_amb(ns, ks, conv[key], d_of(b, d, cr), "s")
+        out[n] = s
+        if deadline is not None and time.perf_counter() > deadline:
+            return out[:n + 1]
+    return out
