@@ -48,7 +48,41 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_fortran(verbose)
     return LIB
+
+
+FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
+FSRC = os.path.join(HERE, "fortran", "dbs_gpu.f90")
+FLIB = os.path.join(HERE, "libdbsr3_amd_fortran.so")
+DRIVER_SRC = os.path.join(ROOT, "tests", "fortran", "drv_dropin.f90")
+DRIVER = os.path.join(ROOT, "tests", "fortran", "drv_dropin")
+
+
+def build_fortran(verbose=False):
+    """the ISO_C_BINDING drop-in layer (libdbsr3_amd_fortran.so) and the
+    Fortran host program used by the drop-in test; skipped without flang"""
+    if not os.path.exists(FLANG):
+        if verbose:
+            print("flang not found: Fortran drop-in layer not built")
+        return None
+    mod = os.path.join(OBJDIR, "fmod")
+    os.makedirs(mod, exist_ok=True)
+    obj = os.path.join(OBJDIR, "dbs_gpu.o")
+    steps = [
+        [FLANG, "-O2", "-fPIC", "-module-dir", mod, "-c", FSRC, "-o", obj],
+        [FLANG, "-shared", "-fPIC", obj, "-L" + HERE, "-ldbsr3_amd",
+         "-Wl,-rpath,$ORIGIN", "-o", FLIB],
+        [FLANG, "-O2", DRIVER_SRC, "-I" + mod, "-L" + HERE, "-ldbsr3_amd_fortran", "-ldbsr3_amd",
+         "-Wl,-rpath,$ORIGIN/../../dbsr3_amd", "-o", DRIVER],
+    ]
+    for cmd in steps:
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("flang failed: " + " ".join(cmd) + "\n" + r.stderr[-3000:])
+    if verbose:
+        print("built", FLIB, "and", DRIVER)
+    return FLIB
 
 
 if __name__ == "__main__":

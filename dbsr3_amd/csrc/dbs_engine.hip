@@ -310,7 +310,8 @@ __global__ void __launch_bounds__(256) k_contract_dense(
     int ntiles, const int4* __restrict__ tiles, const KGroupDev* __restrict__ kgs,
     const InstDev* __restrict__ inst, const DensDev* __restrict__ dens,
     const double* __restrict__ F, const int* __restrict__ pairrow, const long long* __restrict__ pairoff,
-    const long long* __restrict__ row_out, int onesided, double* __restrict__ vals) {
+    const int* __restrict__ nextrow, const long long* __restrict__ row_out, int onesided,
+    double* __restrict__ vals) {
   // tile = (group, Ltile, Rtile, 0); pairrow[pairoff[g] + l*ndens + r] = row or -1
   const int waves = blockDim.x >> 6;
   const int lane = threadIdx.x & 63;
@@ -332,15 +333,15 @@ __global__ void __launch_bounds__(256) k_contract_dense(
       double b = (rv && x < D) ? fsig(fr, nv, nk, x, onesided) : 0.0;
       acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
     }
-    // output layout of the 16x16 f64 MFMA: lane holds column lane%16,
-    // rows 4*(lane/16) + j, j = 0..3
+    // output layout of the 16x16x4 f64 MFMA (one output group per
+    // register): lane holds column lane%16 and rows 4*j + lane/16, j = 0..3
     const int col = lane & 15;
     for (int j = 0; j < 4; ++j) {
-      const int row = 4 * (lane >> 4) + j;
+      const int row = 4 * j + (lane >> 4);
       const int L = l0 + row, R = r0 + col;
       if (L < G.ndens && R < G.ndens) {
-        int rr = pairrow[pairoff[T.x] + (long long)L * G.ndens + R];
-        if (rr >= 0) vals[row_out[rr]] = acc[j];
+        for (int rr = pairrow[pairoff[T.x] + (long long)L * G.ndens + R]; rr >= 0; rr = nextrow[rr])
+          vals[row_out[rr]] = acc[j];
       }
     }
   }
@@ -634,6 +635,7 @@ struct dbs_engine {
   int4* d_tiles = nullptr;
   int* d_pairrow = nullptr;
   long long* d_pairoff = nullptr;
+  int* d_nextrow = nullptr;  // next row with the same (L,R) pair, -1 at the end
   int ntiles = 0;
   // sizes
   int nkg = 0, ndens = 0, nitems = 0;
@@ -705,34 +707,32 @@ int build_dense_tables(dbs_engine* e) {
   }
   pairoff[e->nkg] = tot;
   std::vector<int> pairrow((size_t)std::max<long long>(tot, 1), -1);
+  std::vector<int> nextrow(std::max<size_t>(e->h_rows.size(), 1), -1);
   for (int g = 0; g < e->nkg; ++g) {
     if (!e->group_dense[g]) continue;
     const int n = e->h_kg[g].ndens, nt = (n + 15) / 16;
     for (int a = 0; a < nt; ++a)
       for (int b = 0; b < nt; ++b) tiles.push_back(make_int4(g, a, b, 0));
-    for (int r : e->group_rows[g]) {
+    // rows requesting the same (L,R) pair form a chain in row order
+    for (auto it = e->group_rows[g].rbegin(); it != e->group_rows[g].rend(); ++it) {
+      const int r = *it;
       const RowDev& R = e->h_rows[r];
-      int l = R.L - e->h_kg[g].dens0, rr = R.R - e->h_kg[g].dens0;
-      long long slot = pairoff[g] + (long long)l * n + rr;
-      if (pairrow[slot] < 0) pairrow[slot] = r;
-      else {
-        // duplicate (L,R) pair: keep sparse path for this group
-        e->group_dense[g] = 0;
-      }
+      const int l = R.L - e->h_kg[g].dens0, rr = R.R - e->h_kg[g].dens0;
+      const long long slot = pairoff[g] + (long long)l * n + rr;
+      nextrow[r] = pairrow[slot];
+      pairrow[slot] = r;
     }
   }
-  // drop tiles of groups that fell back
-  std::vector<int4> keep;
-  for (auto& t : tiles)
-    if (e->group_dense[t.x]) keep.push_back(t);
-  e->ntiles = (int)keep.size();
+  e->ntiles = (int)tiles.size();
   if (e->d_tiles) hipFree(e->d_tiles);
   if (e->d_pairrow) hipFree(e->d_pairrow);
   if (e->d_pairoff) hipFree(e->d_pairoff);
-  e->d_tiles = nullptr; e->d_pairrow = nullptr; e->d_pairoff = nullptr;
-  int rc = upload(e->stream, &e->d_tiles, keep);
+  if (e->d_nextrow) hipFree(e->d_nextrow);
+  e->d_tiles = nullptr; e->d_pairrow = nullptr; e->d_pairoff = nullptr; e->d_nextrow = nullptr;
+  int rc = upload(e->stream, &e->d_tiles, tiles);
   if (!rc) rc = upload(e->stream, &e->d_pairrow, pairrow);
   if (!rc) rc = upload(e->stream, &e->d_pairoff, pairoff);
+  if (!rc) rc = upload(e->stream, &e->d_nextrow, nextrow);
   return rc;
 }
 
@@ -813,7 +813,7 @@ static int run_forward(dbs_engine* e) {
   if (e->ntiles > 0) {
     k_contract_dense<<<grid_for(e->ntiles, 4, cap * 4), 256, 0, s>>>(
         e->ntiles, e->d_tiles, e->d_kg, e->d_inst, e->d_dens, e->F, e->d_pairrow, e->d_pairoff,
-        e->d_row_out, e->onesided, e->vals);
+        e->d_nextrow, e->d_row_out, e->onesided, e->vals);
     mark(e, "contract_dense");
   }
   k_energy<<<grid_for(e->ninst, 1, cap), 256, 0, s>>>(e->ninst, e->d_roff, e->coef, e->vals, e->energy);
@@ -1149,7 +1149,7 @@ int dbs_engine_destroy(dbs_engine* e) {
   void* bufs[] = {e->pq, e->coef, e->vals, e->grad, e->energy, e->orbv, e->gorb, e->wt, e->F, e->G,
                   e->d_inst, e->d_kg, e->d_dens, e->d_rows, e->d_row_out, e->d_seg_orb, e->d_seg_coef,
                   e->d_seg_w, e->d_roff, e->d_items, e->d_adjoff, e->d_adj, e->d_oadjoff, e->d_oadj,
-                  e->d_tiles, e->d_pairrow, e->d_pairoff, e->plain_d, e->plain_c};
+                  e->d_tiles, e->d_pairrow, e->d_pairoff, e->d_nextrow, e->plain_d, e->plain_c};
   for (void* p : bufs)
     if (p) hipFree(p);
   auto it = g_sparse.find(e);

@@ -1,0 +1,66 @@
+!======================================================================
+! drv_dropin.f90 -- a host program written the way the reference's callers
+! are (rk_evaluate.f90, hf_rk.f90): it calls mrk_*, density, Convol,
+! SUM_AmB, rk and quadr by their reference names and gets the GPU versions
+! from dbsr3_amd/fortran/dbs_gpu.f90.  Reads drv_in.bin, writes drv_out.bin
+! (stream files written/read by tests/test_gpu_fortran.py).
+!======================================================================
+      Program drv_dropin
+      Implicit none
+      Integer :: ns, ks, ksp, ksq, norb, k, n, i
+      Real(8), allocatable :: t(:), pq(:,:,:), d1(:,:), d2(:,:), conv(:,:), out(:)
+      Real(8), external :: rk, SUM_AmB, quadr
+
+      Open(11, file='drv_in.bin', access='stream', form='unformatted', status='old')
+      Read(11) ns, ks, ksp, ksq
+      Allocate(t(ns + ks)); Read(11) t
+      Read(11) norb, k
+      Allocate(pq(ns, 2, norb)); Read(11) pq
+      Close(11)
+
+      Call dbs_gpu_init(ns, ks, ksp, ksq, t)
+      Allocate(d1(ns, ks), d2(ns, ks), conv(ns, ks), out(8))
+
+! ... R^k(1,2;3,4) as rk() assembles it
+      out(1) = rk(pq(1,1,1), pq(1,1,2), pq(1,1,3), pq(1,1,4), k)
+
+! ... the same integral the way Rk_evaluate does it, one type at a time
+      out(2) = 0.d0
+      Call mrk_pppp(k)
+      Call Density(ns, ks, d1, pq(1,1,1), pq(1,1,3), 's')
+      Call Convol(ns, ks, conv, d1, 2, 's', 's')
+      Call Density(ns, ks, d2, pq(1,1,2), pq(1,1,4), 's')
+      out(2) = out(2) + SUM_AmB(ns, ks, conv, d2, 's')
+      Call mrk_qqqq(k)
+      Call Density(ns, ks, d1, pq(1,2,1), pq(1,2,3), 's')
+      Call Convol(ns, ks, conv, d1, 2, 's', 's')
+      Call Density(ns, ks, d2, pq(1,2,2), pq(1,2,4), 's')
+      out(2) = out(2) + SUM_AmB(ns, ks, conv, d2, 's')
+      Call mrk_qpqp(k)
+      Call Density(ns, ks, d1, pq(1,2,1), pq(1,2,3), 's')
+      Call Convol(ns, ks, conv, d1, 2, 's', 's')
+      Call Density(ns, ks, d2, pq(1,1,2), pq(1,1,4), 's')
+      out(2) = out(2) + SUM_AmB(ns, ks, conv, d2, 's')
+      Call mrk_pqpq(k)
+      Call Density(ns, ks, d1, pq(1,1,1), pq(1,1,3), 's')
+      Call Convol(ns, ks, conv, d1, 2, 's', 's')
+      Call Density(ns, ks, d2, pq(1,2,2), pq(1,2,4), 's')
+      out(2) = out(2) + SUM_AmB(ns, ks, conv, d2, 's')
+
+! ... <1|r^m|2> for m = 0, 1, -1
+      out(3) = quadr(pq(1,1,1), pq(1,1,2), 0)
+      out(4) = quadr(pq(1,1,1), pq(1,1,2), 1)
+      out(5) = quadr(pq(1,1,1), pq(1,1,2), -1)
+! ... a direct potential (convol case 1) contracted back: <d1|V(d2)>
+      Call mrk_pppp(k)
+      Call Density(ns, ks, d1, pq(1,1,1), pq(1,1,3), 's')
+      Call Density(ns, ks, d2, pq(1,1,2), pq(1,1,4), 's')
+      Call Convol(ns, ks, conv, d2, 1, 's', 's')
+      out(6) = SUM_AmB(ns, ks, conv, d1, 's')
+      out(7) = dble(ns); out(8) = dble(k)
+
+      Open(12, file='drv_out.bin', access='stream', form='unformatted', status='replace')
+      Write(12) out
+      Close(12)
+      Call dbs_gpu_close
+      End Program drv_dropin

@@ -4,8 +4,10 @@ Values: every row R^k(a,b;c,d) (rk() sum, single P/Q types, S^k types) is
 compared with the oracle's materialised-array evaluation (mrk/msk +
 convol case 2 + SUM_AmB, the reference's rk()/Rk_evaluate path).  The
 engine sums the same quadrature in another order; the bound is 1e-12
-relative, with the relative error of values smaller than 1e-4 of the set's
-largest magnitude measured against that 1e-4 floor (cancellation).
+relative, with the error of values smaller than 1e-3 of the set's largest
+magnitude measured against that floor: such values come out of cancelling
+terms of the set's scale, where rounding of any summation order is ~1e-16 of
+that scale (an absolute bound of 1e-15 of the scale).
 
 Adjoint: dE/d(pq) of E = sum c_n R_n against the oracle's potentials --
 convol case 1 (direct, over r2) and case 2 (over r1) of every row's
@@ -28,7 +30,7 @@ TYPES_OF_KIND = {0: [(O.PPPP, 0, 0), (O.QQQQ, 1, 1), (O.QPQP, 1, 0), (O.PQPQ, 0,
 
 def relerr(x, ref):
     ref = np.asarray(ref)
-    floor = 1e-4 * np.max(np.abs(ref))
+    floor = 1e-3 * np.max(np.abs(ref))
     return np.abs(np.asarray(x) - ref) / np.maximum(np.abs(ref), floor)
 
 
@@ -130,6 +132,11 @@ def test_values(setups, name, kind):
     e = relerr(got, ref)
     print(f"  max rel {e.max():.3e} median {np.median(e):.3e}")
     assert e.max() < TOL, (name, kind, e.max())
+    for path in (0, 1):  # every group on the sparse, then on the dense MFMA path
+        eng.set_tuning([path])
+        eng.forward()
+        ep = relerr(eng.values(), ref)
+        assert ep.max() < TOL, (name, kind, "path", path, ep.max())
     # plain path (materialised arrays) on the device
     eng.forward_plain()
     gp = eng.values()
