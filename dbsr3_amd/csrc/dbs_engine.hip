@@ -47,6 +47,7 @@ enum DensKind { DK_TOT = 0, DK_PP = 1, DK_QQ = 2, DK_PQ = 3 };
 struct InstDev {
   int ns, ks, ksp, ksq, nv, norb, np;  // np = ks + ks*ks points per interval
   int orb_base;                        // global index of the instance's first orbital
+  int kg0, nkg;                        // the instance's (instance,k) groups kg0..kg0+nkg-1
   long long off_pq;                    // into pq/grad buffers
   long long off_orbv;                  // orbital point values
   const double* grw;
@@ -178,30 +179,44 @@ __device__ inline double dens_point(const double* __restrict__ va, const double*
 __global__ void __launch_bounds__(256) k_features(
     int nitems, const int2* __restrict__ items, const KGroupDev* __restrict__ kgs,
     const InstDev* __restrict__ inst, const DensDev* __restrict__ dens,
-    const double* __restrict__ orbv, const double* __restrict__ wt, double* __restrict__ F) {
-  extern __shared__ double sw[];  // Wk(ks) Wm(ks) wkin(ks*ks)
+    const double* __restrict__ orbv, const double* __restrict__ wt, double* __restrict__ F,
+    int stage) {
+  // sw: Wk(ks) Wm(ks) wkin(ks*ks), then (stage) the interval's orbital
+  // values [orb][comp][np]
+  extern __shared__ double sw[];
   for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
     const int g = items[it].x, iv = items[it].y;
     const KGroupDev G = kgs[g];
     const InstDev I = inst[G.inst];
     const int ks = I.ks, nv = I.nv, np = I.np;
+    const int nw = 2 * ks + ks * ks;
     __syncthreads();
-    for (int q = threadIdx.x; q < 2 * ks + ks * ks; q += blockDim.x) {
+    for (int q = threadIdx.x; q < nw; q += blockDim.x) {
       double v;
       if (q < ks) v = wt[G.off_w + (long long)iv * ks + q];
       else if (q < 2 * ks) v = wt[G.off_w + (long long)nv * ks + (long long)iv * ks + (q - ks)];
       else v = wt[G.off_w + 2LL * nv * ks + (long long)iv * ks * ks + (q - 2 * ks)];
       sw[q] = v;
     }
+    double* sv = sw + nw;
+    if (stage) {
+      const int per = 2 * I.norb * np;
+      for (int q = threadIdx.x; q < per; q += blockDim.x) {
+        const int p = q % np, oc = q / np;  // oc = 2*orb + comp
+        sv[q] = orbv[I.off_orbv + ((long long)oc * nv + iv) * np + p];
+      }
+    }
     __syncthreads();
     const double* Wk = sw;
     const double* Wm = sw + ks;
     const double* wk = sw + 2 * ks;
-    const long long cstride = (long long)nv * np;
+    const long long cstride = stage ? (long long)np : (long long)nv * np;
     for (int d = threadIdx.x; d < G.ndens; d += blockDim.x) {
       const DensDev D = dens[G.dens0 + d];
-      const double* va = orbv + I.off_orbv + ((long long)(2 * D.a) * nv + iv) * np;
-      const double* vc = orbv + I.off_orbv + ((long long)(2 * D.c) * nv + iv) * np;
+      const double* va = stage ? sv + (2 * D.a) * np
+                               : orbv + I.off_orbv + ((long long)(2 * D.a) * nv + iv) * np;
+      const double* vc = stage ? sv + (2 * D.c) * np
+                               : orbv + I.off_orbv + ((long long)(2 * D.c) * nv + iv) * np;
       double* f = F + D.off_f;
       double mk = 0.0, mm = 0.0;
       for (int m = 0; m < ks; ++m) {
@@ -348,22 +363,38 @@ __global__ void __launch_bounds__(256) k_contract_dense(
 }
 
 // ---------------------------------------------------------------------------
-// energy: E_inst = sum_rows coef*value, fixed-order tree per instance
+// energy: E_inst = sum_rows coef*value.  Blocks reduce fixed chunks of one
+// instance's rows (tree in a fixed shape); one thread per instance then adds
+// the chunk partials in order.
 // ---------------------------------------------------------------------------
-__global__ void k_energy(int ninst, const long long* __restrict__ roff, const double* __restrict__ coef,
-                         const double* __restrict__ vals, double* __restrict__ energy) {
+#define DBS_ECHUNK 2048
+__global__ void k_energy_partial(int nchunks, const int2* __restrict__ chunks,
+                                 const long long* __restrict__ roff, const double* __restrict__ coef,
+                                 const double* __restrict__ vals, double* __restrict__ part) {
   __shared__ double sh[256];
-  for (int s = blockIdx.x; s < ninst; s += gridDim.x) {
+  for (int ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+    const int2 C = chunks[ch];  // (instance, chunk index within it)
+    const long long r0 = roff[C.x] + (long long)C.y * DBS_ECHUNK;
+    const long long r1 = min(r0 + DBS_ECHUNK, roff[C.x + 1]);
     double acc = 0.0;
-    for (long long r = roff[s] + threadIdx.x; r < roff[s + 1]; r += blockDim.x) acc += coef[r] * vals[r];
+    for (long long r = r0 + threadIdx.x; r < r1; r += blockDim.x) acc += coef[r] * vals[r];
     sh[threadIdx.x] = acc;
     __syncthreads();
     for (int w = blockDim.x / 2; w > 0; w >>= 1) {
       if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
       __syncthreads();
     }
-    if (threadIdx.x == 0) energy[s] = sh[0];
+    if (threadIdx.x == 0) part[ch] = sh[0];
     __syncthreads();
+  }
+}
+
+__global__ void k_energy_final(int ninst, const int* __restrict__ choff, const double* __restrict__ part,
+                               double* __restrict__ energy) {
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < ninst; s += gridDim.x * blockDim.x) {
+    double acc = 0.0;
+    for (int c = choff[s]; c < choff[s + 1]; ++c) acc += part[c];
+    energy[s] = acc;
   }
 }
 
@@ -404,6 +435,118 @@ __global__ void __launch_bounds__(256) k_adj_gather(
       }
       g[x] = acc;
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// panel forms.  The feature dimension of a group is cut into panels whose
+// features for all the group's densities fit one XCD's L2; every work item
+// on a panel is dispatched to the same XCD (workgroup i runs on XCD i % 8),
+// so partner features are read from HBM/MALL once per panel and then served
+// from that XCD's L2.  Items with d < 0 are padding.
+// ---------------------------------------------------------------------------
+struct PanelItem {
+  int d;       // density (gather) or first row (contraction)
+  int n;       // rows in this item (contraction)
+  int x0, x1;  // feature range
+  int tile;    // panel index within the group (contraction partials)
+  int nv, nk;  // intervals and nv*ks of the group's grid
+  int pad;
+};
+
+// per adjacency entry: partner feature offset, coefficient index, role
+struct AdjPre {
+  long long off;
+  long long cidx;
+  int role, pad;
+};
+
+__global__ void __launch_bounds__(256) k_adj_gather_panel(
+    int nitems, const PanelItem* __restrict__ items, const long long* __restrict__ doff,
+    const int* __restrict__ adjoff, const AdjPre* __restrict__ adj, const double* __restrict__ coef,
+    const double* __restrict__ F, int onesided, double* __restrict__ Gb) {
+  __shared__ long long s_off[256];
+  __shared__ double s_c[256];
+  __shared__ int s_role[256];
+  const int it = blockIdx.x;
+  if (it >= nitems) return;
+  const PanelItem P = items[it];
+  if (P.d < 0) return;
+  const int nv = P.nv, nk = P.nk;
+  double* g = Gb + doff[P.d];
+  const int a0 = adjoff[P.d], a1 = adjoff[P.d + 1];
+  // thread-owned accumulators for x = x0 + tid + k*blockDim
+  constexpr int XMAX = 8;
+  double acc[XMAX];
+  for (int q = 0; q < XMAX; ++q) acc[q] = 0.0;
+  for (int c0 = a0; c0 < a1; c0 += 256) {
+    const int cn = min(256, a1 - c0);
+    __syncthreads();
+    if ((int)threadIdx.x < cn) {
+      const AdjPre A = adj[c0 + threadIdx.x];
+      s_off[threadIdx.x] = A.off;
+      s_c[threadIdx.x] = coef[A.cidx];
+      s_role[threadIdx.x] = A.role;
+    }
+    __syncthreads();
+    for (int e = 0; e < cn; ++e) {
+      const double* fp = F + s_off[e];
+      const double c = s_c[e];
+      const int role = s_role[e];
+#pragma unroll
+      for (int q = 0; q < XMAX; ++q) {
+        const int x = P.x0 + threadIdx.x + q * blockDim.x;
+        if (x >= P.x1) break;
+        double v;
+        if (!onesided) {
+          v = fsig(fp, nv, nk, x, 0);
+        } else if (role == 0) {
+          v = (x < nv) ? fp[nv + x] : (x >= 2 * nv + nk ? fp[2 * nv + (x - 2 * nv - nk)] : 0.0);
+        } else {
+          v = (x >= nv && x < 2 * nv) ? fp[x - nv] : ((x >= 2 * nv && x < 2 * nv + nk) ? fp[x + nk] : 0.0);
+        }
+        acc[q] += c * v;
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < XMAX; ++q) {
+    const int x = P.x0 + threadIdx.x + q * blockDim.x;
+    if (x < P.x1) g[x] = acc[q];
+  }
+}
+
+// contraction partials: one wavefront per row of the item, lanes over the
+// panel's feature range; part[row * maxtile + tile].  roff[r] = feature
+// offsets of the row's two densities.
+__global__ void __launch_bounds__(256) k_contract_panel(
+    int nitems, const PanelItem* __restrict__ items, const longlong2* __restrict__ roff,
+    const double* __restrict__ F, int onesided, int maxtile, double* __restrict__ part) {
+  const int it = blockIdx.x;
+  if (it >= nitems) return;
+  const PanelItem P = items[it];
+  if (P.d < 0) return;
+  const int nv = P.nv, nk = P.nk;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (int q = wv; q < P.n; q += nw) {
+    const int r = P.d + q;
+    const longlong2 o = roff[r];
+    const double* fl = F + o.x;
+    const double* fr = F + o.y;
+    double acc = 0.0;
+    for (int x = P.x0 + lane; x < P.x1; x += 64) acc += fl[x] * fsig(fr, nv, nk, x, onesided);
+    acc = wave_sum(acc);
+    if (lane == 0) part[(long long)r * maxtile + P.tile] = acc;
+  }
+}
+
+__global__ void k_contract_reduce(int nrows, const int* __restrict__ ntile, int maxtile,
+                                  const double* __restrict__ part, const long long* __restrict__ row_out,
+                                  double* __restrict__ vals) {
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < nrows; r += gridDim.x * blockDim.x) {
+    double acc = 0.0;
+    for (int t = 0; t < ntile[r]; ++t) acc += part[(long long)r * maxtile + t];
+    vals[row_out[r]] = acc;
   }
 }
 
@@ -468,11 +611,13 @@ __global__ void k_adj_orb_points(int ninst, const long long* __restrict__ seg,
   }
 }
 
-// adjoint phase D: dE/dcoef(j) = sum over intervals and points of B_j * g_f
+// adjoint phase D: dE/dcoef(j) = sum over intervals and points of B_j * g_f;
+// one wavefront per coefficient, lanes over (interval, point)
 __global__ void k_adj_coef(int ninst, const long long* __restrict__ seg, const InstDev* __restrict__ inst,
                            const double* __restrict__ gorb, double* __restrict__ grad, long long total) {
-  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
-       q += (long long)gridDim.x * blockDim.x) {
+  const int lane = threadIdx.x & 63;
+  for (long long q = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6; q < total;
+       q += ((long long)gridDim.x * blockDim.x) >> 6) {
     const int s = find_seg(seg, ninst, q);
     const InstDev I = inst[s];
     long long r = q - seg[s];
@@ -487,15 +632,129 @@ __global__ void k_adj_coef(int ninst, const long long* __restrict__ seg, const I
     double acc = 0.0;
     const int nsx = nv + kx - 1;
     if (j < nsx) {
-      for (int iv = max(0, j - kx + 1); iv <= min(j, nv - 1); ++iv) {
+      const int iv0 = max(0, j - kx + 1), iv1 = min(j, nv - 1);
+      const int nterm = (iv1 - iv0 + 1) * np;
+      for (int t = lane; t < nterm; t += 64) {
+        const int iv = iv0 + t / np, p = t % np;
         const int jj = j - iv;
-        const double* gf = gorb + I.off_orbv + ((long long)(2 * orb + comp) * nv + iv) * np;
-        for (int m = 0; m < ks; ++m) acc += xb[iv + (long long)nw * (m + (long long)ks * jj)] * gf[m];
-        const double* b = bin + (long long)iv * ks * ks * kx;
-        for (int mi = 0; mi < ks * ks; ++mi) acc += b[(long long)mi * kx + jj] * gf[ks + mi];
+        const double gf = gorb[I.off_orbv + ((long long)(2 * orb + comp) * nv + iv) * np + p];
+        const double b = p < ks ? xb[iv + (long long)nw * (p + (long long)ks * jj)]
+                                : bin[((long long)iv * ks * ks + (p - ks)) * kx + jj];
+        acc += b * gf;
       }
     }
-    grad[I.off_pq + q - seg[s]] = acc;
+    acc = wave_sum(acc);
+    if (lane == 0) grad[I.off_pq + q - seg[s]] = acc;
+  }
+}
+
+// adjoint phase C, LDS form: one workgroup per (instance, interval iv,
+// outer point m).  For every (instance,k) group the density gradients at
+// the point are scattered into norb x norb matrices in LDS (symmetric for
+// P_aP_c / Q_aQ_c / total densities, the diagonal counted twice; plain for
+// P_aQ_c), then the orbital gradients are matrix-vector products with the
+// orbital values at the outer point m and at its ks nested points.
+// mode 0: total density (one matrix, both components); 1: P_aP_c and
+// Q_aQ_c densities (two matrices); 2: P_aQ_c densities (one matrix).
+__global__ void __launch_bounds__(256) k_adj_orb_lds(
+    int ninst, const long long* __restrict__ seg, const InstDev* __restrict__ inst,
+    const KGroupDev* __restrict__ kgs, const DensDev* __restrict__ dens,
+    const double* __restrict__ orbv, const double* __restrict__ wt, const double* __restrict__ Gb,
+    int mode, double* __restrict__ gorb, long long total) {
+  extern __shared__ double sm[];
+  for (long long item = blockIdx.x; item < total; item += gridDim.x) {
+    const int s = find_seg(seg, ninst, item);
+    const InstDev I = inst[s];
+    const int r = (int)(item - seg[s]);
+    const int iv = r / I.ks, m = r % I.ks;
+    const int norb = I.norb, ks = I.ks, np = I.np, nv = I.nv, nk = nv * ks;
+    const int ld = norb + 1;
+    const int nmat = mode == 1 ? 2 : 1;
+    double* M1 = sm;
+    double* M2 = M1 + norb * ld;
+    double* fo = M1 + nmat * norb * ld;  // [comp][orb]
+    double* fi = fo + 2 * norb;          // [comp][orb][i]
+    double* oo = fi + 2 * norb * ks;     // outputs, outer point
+    double* oi = oo + 2 * norb;          // outputs, nested points
+    double* wv = oi + 2 * norb * ks;     // Wk, Wm, wkin(ks)
+    __syncthreads();
+    for (int q = threadIdx.x; q < 2 * norb; q += blockDim.x) {
+      const int comp = q / norb, a = q % norb;
+      fo[q] = orbv[I.off_orbv + ((long long)(2 * a + comp) * nv + iv) * np + m];
+      oo[q] = 0.0;
+    }
+    for (int q = threadIdx.x; q < 2 * norb * ks; q += blockDim.x) {
+      const int i = q % ks, a = (q / ks) % norb, comp = q / (ks * norb);
+      fi[q] = orbv[I.off_orbv + ((long long)(2 * a + comp) * nv + iv) * np + ks + m * ks + i];
+      oi[q] = 0.0;
+    }
+    for (int gi = 0; gi < I.nkg; ++gi) {
+      const KGroupDev G = kgs[I.kg0 + gi];
+      __syncthreads();
+      if (threadIdx.x < ks + 2) {
+        const int q = threadIdx.x;
+        wv[q] = q == 0 ? wt[G.off_w + (long long)iv * ks + m]
+              : q == 1 ? wt[G.off_w + (long long)nk + (long long)iv * ks + m]
+                       : wt[G.off_w + 2LL * nk + ((long long)iv * ks + m) * ks + (q - 2)];
+      }
+      for (int pass = 0; pass < 2; ++pass) {
+        for (int q = threadIdx.x; q < nmat * norb * ld; q += blockDim.x) M1[q] = 0.0;
+        __syncthreads();
+        for (int d = threadIdx.x; d < G.ndens; d += blockDim.x) {
+          const DensDev D = dens[G.dens0 + d];
+          const double* g = Gb + D.off_f;
+          double v;
+          if (pass == 0) v = g[iv] * wv[0] + (g[nv + iv] + g[2 * nv + iv * ks + m]) * wv[1];
+          else v = g[2 * nv + nk + iv * ks + m];
+          double* M = (mode == 1 && D.kind == DK_QQ) ? M2 : M1;
+          if (mode == 2) {
+            M[D.a * ld + D.c] = v;
+          } else if (D.a == D.c) {
+            M[D.a * ld + D.a] = 2.0 * v;
+          } else {
+            M[D.a * ld + D.c] = v;
+            M[D.c * ld + D.a] = v;
+          }
+        }
+        __syncthreads();
+        if (pass == 0) {
+          for (int q = threadIdx.x; q < 2 * norb; q += blockDim.x) {
+            const int comp = q / norb, a = q % norb;
+            double acc = 0.0;
+            if (mode == 2) {
+              if (comp == 0) for (int c = 0; c < norb; ++c) acc += M1[a * ld + c] * fo[norb + c];
+              else for (int c = 0; c < norb; ++c) acc += M1[c * ld + a] * fo[c];
+            } else {
+              const double* M = (mode == 1 && comp == 1) ? M2 : M1;
+              for (int c = 0; c < norb; ++c) acc += M[a * ld + c] * fo[comp * norb + c];
+            }
+            oo[q] += acc;
+          }
+        } else {
+          for (int q = threadIdx.x; q < 2 * norb * ks; q += blockDim.x) {
+            const int i = q % ks, a = (q / ks) % norb, comp = q / (ks * norb);
+            double acc = 0.0;
+            if (mode == 2) {
+              if (comp == 0) for (int c = 0; c < norb; ++c) acc += M1[a * ld + c] * fi[(norb + c) * ks + i];
+              else for (int c = 0; c < norb; ++c) acc += M1[c * ld + a] * fi[c * ks + i];
+            } else {
+              const double* M = (mode == 1 && comp == 1) ? M2 : M1;
+              for (int c = 0; c < norb; ++c) acc += M[a * ld + c] * fi[(comp * norb + c) * ks + i];
+            }
+            oi[q] += wv[2 + i] * acc;
+          }
+        }
+        __syncthreads();
+      }
+    }
+    for (int q = threadIdx.x; q < 2 * norb; q += blockDim.x) {
+      const int comp = q / norb, a = q % norb;
+      gorb[I.off_orbv + ((long long)(2 * a + comp) * nv + iv) * np + m] = oo[q];
+    }
+    for (int q = threadIdx.x; q < 2 * norb * ks; q += blockDim.x) {
+      const int i = q % ks, a = (q / ks) % norb, comp = q / (ks * norb);
+      gorb[I.off_orbv + ((long long)(2 * a + comp) * nv + iv) * np + ks + m * ks + i] = oi[q];
+    }
   }
 }
 
@@ -636,6 +895,29 @@ struct dbs_engine {
   int* d_pairrow = nullptr;
   long long* d_pairoff = nullptr;
   int* d_nextrow = nullptr;  // next row with the same (L,R) pair, -1 at the end
+  // energy reduction chunks
+  int2* d_echunks = nullptr;
+  int* d_echoff = nullptr;
+  double* d_epart = nullptr;
+  int nechunks = 0;
+  // LDS sizes (doubles) of the staged kernels; 0 = not staged
+  int feat_stage = 0;
+  size_t feat_shm = 0;
+  int adj_mode = 0;
+  size_t adj_shm = 0;
+  long long adj_items = 0;
+  long long* d_seg_adj = nullptr;
+  // panel work lists (XCD-ordered) for gather and contraction
+  PanelItem* d_gitems = nullptr;
+  int ngitems = 0;
+  PanelItem* d_citems = nullptr;
+  int ncitems = 0;
+  int* d_cntile = nullptr;   // panels per sparse row
+  longlong2* d_croff = nullptr;  // feature offsets (L,R) per sparse row
+  long long* d_doff = nullptr;   // feature offset per density
+  AdjPre* d_adjpre = nullptr;
+  double* d_cpart = nullptr;
+  int cmaxtile = 1;
   int ntiles = 0;
   // sizes
   int nkg = 0, ndens = 0, nitems = 0;
@@ -651,7 +933,8 @@ struct dbs_engine {
   // tuning: [0] contraction path (0 sparse, 1 dense MFMA, 2 auto),
   //         [1] features block size, [2] rows per block (waves),
   //         [3] persistent grid cap (workgroups), [4] adjoint gather block
-  int tune[5] = {2, 128, 4, 2048, 256};
+  //         [5] panel budget in KiB (features of one panel, per XCD L2)
+  int tune[6] = {2, 128, 4, 2048, 256, 2048};
   std::vector<char> group_dense;  // per group: use the dense path
   double flops_fwd = 0, flops_adj = 0, bytes_min = 0;
   // graph
@@ -744,7 +1027,12 @@ static std::vector<RowDev> sparse_rows(const dbs_engine* e, std::vector<long lon
   outmap->clear();
   for (int g = 0; g < e->nkg; ++g) {
     if (e->group_dense[g]) continue;
-    for (int r : e->group_rows[g]) {
+    std::vector<int> rs(e->group_rows[g]);
+    std::stable_sort(rs.begin(), rs.end(), [&](int x, int y) {
+      const RowDev &A = e->h_rows[x], &B = e->h_rows[y];
+      return A.L != B.L ? A.L < B.L : A.R < B.R;
+    });
+    for (int r : rs) {
       v.push_back(e->h_rows[r]);
       outmap->push_back(e->h_row_out[r]);
     }
@@ -759,6 +1047,54 @@ struct SparseCache {
 };
 static std::map<const dbs_engine*, SparseCache> g_sparse;
 
+static int panel_width(const dbs_engine* e, int ndens, int D) {
+  const double budget = (double)e->tune[5] * 1024.0;
+  long long w = (long long)(budget / (8.0 * std::max(ndens, 1)));
+  w = (w / 64) * 64;
+  if (w < 64) w = 64;
+  if (w > 2048) w = 2048;  // 8 features per thread of a 256-thread item
+  if (w > D) w = D;
+  return (int)w;
+}
+
+// interleave per-XCD lists so that item i lands on XCD i % 8
+static std::vector<PanelItem> xcd_interleave(const std::vector<std::vector<PanelItem>>& lists) {
+  size_t mx = 0;
+  for (auto& l : lists) mx = std::max(mx, l.size());
+  PanelItem pad;
+  std::memset(&pad, 0, sizeof(pad));
+  pad.d = -1;
+  std::vector<PanelItem> out(mx * lists.size(), pad);
+  for (size_t x = 0; x < lists.size(); ++x)
+    for (size_t j = 0; j < lists[x].size(); ++j) out[j * lists.size() + x] = lists[x][j];
+  return out;
+}
+
+static int build_gather_panels(dbs_engine* e) {
+  std::vector<std::vector<PanelItem>> lists(8);
+  int panel = 0;
+  for (int g = 0; g < e->nkg; ++g) {
+    const KGroupDev& G = e->h_kg[g];
+    const InstDev& I = e->h_inst[G.inst];
+    const int D = feat_dim(I.nv, I.ks);
+    const int W = panel_width(e, G.ndens, D);
+    for (int x0 = 0; x0 < D; x0 += W, ++panel) {
+      for (int d = 0; d < G.ndens; ++d) {
+        PanelItem it;
+        std::memset(&it, 0, sizeof(it));
+        it.d = G.dens0 + d; it.n = 1; it.x0 = x0; it.x1 = std::min(D, x0 + W);
+        it.nv = I.nv; it.nk = I.nv * I.ks;
+        lists[panel % 8].push_back(it);
+      }
+    }
+  }
+  std::vector<PanelItem> items = xcd_interleave(lists);
+  if (e->d_gitems) hipFree(e->d_gitems);
+  e->d_gitems = nullptr;
+  e->ngitems = (int)items.size();
+  return upload(e->stream, &e->d_gitems, items);
+}
+
 static int refresh_sparse(dbs_engine* e) {
   SparseCache& sc = g_sparse[e];
   if (sc.rows) hipFree(sc.rows);
@@ -769,6 +1105,54 @@ static int refresh_sparse(dbs_engine* e) {
   sc.n = (int)rv.size();
   int rc = upload(e->stream, &sc.rows, rv);
   if (!rc) rc = upload(e->stream, &sc.out, om);
+  if (rc) return rc;
+  // contraction panels over the sparse rows (row chunks per group panel)
+  const int RCH = 16;
+  std::vector<std::vector<PanelItem>> lists(8);
+  std::vector<int> ntile(std::max(sc.n, 1), 0);
+  int panel = 0, maxtile = 1;
+  for (int r0 = 0; r0 < sc.n;) {
+    const int g = rv[r0].kg;
+    int r1 = r0;
+    while (r1 < sc.n && rv[r1].kg == g) ++r1;
+    const KGroupDev& G = e->h_kg[g];
+    const InstDev& I = e->h_inst[G.inst];
+    const int D = feat_dim(I.nv, I.ks);
+    const int W = panel_width(e, G.ndens, D);
+    int t = 0;
+    for (int x0 = 0; x0 < D; x0 += W, ++t, ++panel)
+      for (int a = r0; a < r1; a += RCH) {
+        PanelItem it;
+        std::memset(&it, 0, sizeof(it));
+        it.d = a; it.n = std::min(RCH, r1 - a); it.x0 = x0; it.x1 = std::min(D, x0 + W); it.tile = t;
+        it.nv = I.nv; it.nk = I.nv * I.ks;
+        lists[panel % 8].push_back(it);
+      }
+    for (int r = r0; r < r1; ++r) ntile[r] = t;
+    maxtile = std::max(maxtile, t);
+    r0 = r1;
+  }
+  std::vector<PanelItem> items = xcd_interleave(lists);
+  if (e->d_citems) hipFree(e->d_citems);
+  if (e->d_cntile) hipFree(e->d_cntile);
+  if (e->d_cpart) hipFree(e->d_cpart);
+  e->d_citems = nullptr; e->d_cntile = nullptr; e->d_cpart = nullptr;
+  e->ncitems = (int)items.size();
+  e->cmaxtile = maxtile;
+  std::vector<longlong2> croff(std::max(sc.n, 1));
+  for (int r = 0; r < sc.n; ++r) {
+    croff[r].x = e->h_dens[rv[r].L].off_f;
+    croff[r].y = e->h_dens[rv[r].R].off_f;
+  }
+  if (e->d_croff) hipFree(e->d_croff);
+  e->d_croff = nullptr;
+  rc = upload(e->stream, &e->d_citems, items);
+  if (!rc) rc = upload(e->stream, &e->d_croff, croff);
+  if (!rc) rc = upload(e->stream, &e->d_cntile, ntile);
+  if (!rc) {
+    hipError_t er = hipMalloc(&e->d_cpart, sizeof(double) * (size_t)std::max(sc.n, 1) * maxtile);
+    if (er != hipSuccess) rc = fail(DBS_ERR_HIP, "contraction partials: hipMalloc");
+  }
   return rc;
 }
 
@@ -795,19 +1179,18 @@ static int run_forward(dbs_engine* e) {
   k_weights<<<grid_for(e->total_w, 256, cap * 4), 256, 0, s>>>(e->nkg, e->d_seg_w, e->d_kg, e->d_inst,
                                                                 e->wt, e->total_w);
   mark(e, "weights");
-  int maxks = 0;
-  for (auto& I : e->h_inst) maxks = std::max(maxks, I.ks);
-  const size_t shm = sizeof(double) * (2 * maxks + maxks * maxks);
-  k_features<<<grid_for(e->nitems, 1, cap), e->tune[1], shm, s>>>(e->nitems, e->d_items, e->d_kg, e->d_inst,
-                                                                  e->d_dens, e->orbv, e->wt, e->F);
+  k_features<<<grid_for(e->nitems, 1, cap), e->tune[1], e->feat_shm, s>>>(
+      e->nitems, e->d_items, e->d_kg, e->d_inst, e->d_dens, e->orbv, e->wt, e->F, e->feat_stage);
   mark(e, "features");
   k_scan<<<grid_for(e->ndens, 4, cap), 256, 0, s>>>(e->ndens, e->d_dens, e->d_kg, e->d_inst, e->F, 0);
   mark(e, "prefix_scan");
   SparseCache& sc = g_sparse[e];
   if (sc.n > 0) {
     const int bw = e->tune[2] * 64;
-    k_contract_rows<<<grid_for(sc.n, e->tune[2], cap * 4), bw, 0, s>>>(
-        sc.n, sc.rows, e->d_dens, e->d_kg, e->d_inst, e->F, sc.out, e->onesided, e->vals);
+    k_contract_panel<<<(unsigned)std::max(e->ncitems, 1), bw, 0, s>>>(
+        e->ncitems, e->d_citems, e->d_croff, e->F, e->onesided, e->cmaxtile, e->d_cpart);
+    k_contract_reduce<<<grid_for(sc.n, 256, cap), 256, 0, s>>>(sc.n, e->d_cntile, e->cmaxtile, e->d_cpart,
+                                                               sc.out, e->vals);
     mark(e, "contract_rows");
   }
   if (e->ntiles > 0) {
@@ -816,7 +1199,10 @@ static int run_forward(dbs_engine* e) {
         e->d_nextrow, e->d_row_out, e->onesided, e->vals);
     mark(e, "contract_dense");
   }
-  k_energy<<<grid_for(e->ninst, 1, cap), 256, 0, s>>>(e->ninst, e->d_roff, e->coef, e->vals, e->energy);
+  if (e->nechunks > 0)
+    k_energy_partial<<<grid_for(e->nechunks, 1, cap), 256, 0, s>>>(e->nechunks, e->d_echunks, e->d_roff,
+                                                                   e->coef, e->vals, e->d_epart);
+  k_energy_final<<<grid_for(e->ninst, 64, 64), 64, 0, s>>>(e->ninst, e->d_echoff, e->d_epart, e->energy);
   mark(e, "energy");
   DBS_HIP(hipGetLastError());
   return DBS_OK;
@@ -825,18 +1211,23 @@ static int run_forward(dbs_engine* e) {
 static int run_adjoint(dbs_engine* e) {
   hipStream_t s = e->stream;
   const int cap = e->tune[3];
-  k_adj_gather<<<grid_for(e->ndens, 1, cap * 4), e->tune[4], 0, s>>>(
-      e->ndens, e->d_dens, e->d_kg, e->d_inst, e->d_adjoff, e->d_adj, e->coef, e->d_row_out, e->F,
-      e->onesided, e->G);
+  k_adj_gather_panel<<<(unsigned)std::max(e->ngitems, 1), 256, 0, s>>>(
+      e->ngitems, e->d_gitems, e->d_doff, e->d_adjoff, e->d_adjpre, e->coef, e->F, e->onesided, e->G);
   mark(e, "adj_gather");
   k_scan<<<grid_for(e->ndens, 4, cap), 256, 0, s>>>(e->ndens, e->d_dens, e->d_kg, e->d_inst, e->G, 1);
   mark(e, "adj_suffix_scan");
-  k_adj_orb_points<<<grid_for(e->total_orbv, 256, cap * 4), 256, 0, s>>>(
-      e->ninst, e->d_seg_orb, e->d_inst, e->d_kg, e->d_dens, e->d_oadjoff, e->d_oadj, e->orbv, e->wt,
-      e->G, e->gorb, e->total_orbv);
+  if (e->adj_shm > 0) {
+    k_adj_orb_lds<<<grid_for(e->adj_items, 1, cap * 4), 256, e->adj_shm, s>>>(
+        e->ninst, e->d_seg_adj, e->d_inst, e->d_kg, e->d_dens, e->orbv, e->wt, e->G, e->adj_mode, e->gorb,
+        e->adj_items);
+  } else {
+    k_adj_orb_points<<<grid_for(e->total_orbv, 256, cap * 4), 256, 0, s>>>(
+        e->ninst, e->d_seg_orb, e->d_inst, e->d_kg, e->d_dens, e->d_oadjoff, e->d_oadj, e->orbv, e->wt,
+        e->G, e->gorb, e->total_orbv);
+  }
   mark(e, "adj_orb_points");
-  k_adj_coef<<<grid_for(e->total_grad, 256, cap * 4), 256, 0, s>>>(e->ninst, e->d_seg_coef, e->d_inst,
-                                                                    e->gorb, e->grad, e->total_grad);
+  k_adj_coef<<<grid_for(e->total_grad, 4, cap * 4), 256, 0, s>>>(e->ninst, e->d_seg_coef, e->d_inst,
+                                                                  e->gorb, e->grad, e->total_grad);
   mark(e, "adj_coef");
   DBS_HIP(hipGetLastError());
   return DBS_OK;
@@ -1027,6 +1418,12 @@ int dbs_engine_create(dbs_engine** out, int ninst, dbs_ctx* const* ctxs, const i
     rp += 5LL * nrows[i];
   }
   e->ndens = (int)e->h_dens.size();
+  for (int i = 0, g = 0; i < ninst; ++i) {
+    e->h_inst[i].kg0 = g;
+    int n = 0;
+    while (g < e->nkg && e->h_kg[g].inst == i) { ++g; ++n; }
+    e->h_inst[i].nkg = n;
+  }
   e->total_feat = foff;
   e->total_w = woff;
   // work items (group, iv)
@@ -1108,6 +1505,43 @@ int dbs_engine_create(dbs_engine** out, int ninst, dbs_ctx* const* ctxs, const i
   rc |= dalloc(&e->F, e->total_feat);
   rc |= dalloc(&e->G, e->total_feat);
   if (rc) return bail(rc);
+  // energy chunks: (instance, chunk) with DBS_ECHUNK rows each
+  std::vector<int2> echunks;
+  std::vector<int> echoff(ninst + 1, 0);
+  for (int i = 0; i < ninst; ++i) {
+    const int nch = (nrows[i] + DBS_ECHUNK - 1) / DBS_ECHUNK;
+    for (int c = 0; c < nch; ++c) echunks.push_back(make_int2(i, c));
+    echoff[i + 1] = (int)echunks.size();
+  }
+  e->nechunks = (int)echunks.size();
+  rc |= upload(s, &e->d_echunks, echunks);
+  rc |= upload(s, &e->d_echoff, echoff);
+  rc |= dalloc(&e->d_epart, std::max(e->nechunks, 1));
+  // LDS plans: staged features when the interval's orbital values fit,
+  // the LDS form of the adjoint orbital phase when its matrices fit
+  {
+    const size_t lds_max = 64 * 1024 / sizeof(double);
+    size_t fneed = 0, aneed = 0, wneed = 0;
+    for (auto& I : e->h_inst) {
+      wneed = std::max(wneed, (size_t)(2 * I.ks + I.ks * I.ks));
+      fneed = std::max(fneed, (size_t)(2 * I.ks + I.ks * I.ks) + (size_t)2 * I.norb * I.np);
+      const int nmat = (kind >= 1 && kind <= 4) ? 2 : 1;
+      aneed = std::max(aneed, (size_t)nmat * I.norb * (I.norb + 1) + 4 * I.norb + 4 * I.norb * I.ks + I.ks + 2);
+    }
+    e->feat_stage = fneed <= lds_max ? 1 : 0;
+    e->feat_shm = sizeof(double) * (e->feat_stage ? fneed : wneed);
+    e->adj_mode = kind == 0 ? 0 : (kind <= 4 ? 1 : 2);
+    e->adj_shm = aneed <= lds_max ? sizeof(double) * aneed : 0;
+    std::vector<long long> seg_adj(ninst);
+    long long tot = 0;
+    for (int i = 0; i < ninst; ++i) {
+      seg_adj[i] = tot;
+      tot += (long long)e->h_inst[i].nv * e->h_inst[i].ks;
+    }
+    e->adj_items = tot;
+    rc |= upload(s, &e->d_seg_adj, seg_adj);
+  }
+  if (rc) return bail(rc);
   std::vector<long long> row_out(e->h_row_out);
   rc |= upload(s, &e->d_inst, e->h_inst);
   rc |= upload(s, &e->d_kg, e->h_kg);
@@ -1121,6 +1555,19 @@ int dbs_engine_create(dbs_engine** out, int ninst, dbs_ctx* const* ctxs, const i
   rc |= upload(s, &e->d_items, items);
   rc |= upload(s, &e->d_adjoff, adjoff);
   rc |= upload(s, &e->d_adj, adj_all);
+  {
+    std::vector<long long> doff(std::max(e->ndens, 1));
+    for (int d = 0; d < e->ndens; ++d) doff[d] = e->h_dens[d].off_f;
+    std::vector<AdjPre> pre(std::max(adj_all.size(), (size_t)1));
+    for (size_t q = 0; q < adj_all.size(); ++q) {
+      pre[q].off = e->h_dens[adj_all[q].partner].off_f;
+      pre[q].cidx = e->h_row_out[adj_all[q].row];
+      pre[q].role = adj_all[q].role;
+      pre[q].pad = 0;
+    }
+    rc |= upload(s, &e->d_doff, doff);
+    rc |= upload(s, &e->d_adjpre, pre);
+  }
   rc |= upload(s, &e->d_oadjoff, oadjoff);
   rc |= upload(s, &e->d_oadj, oadj_all);
   if (rc) return bail(rc);
@@ -1133,6 +1580,7 @@ int dbs_engine_create(dbs_engine** out, int ninst, dbs_ctx* const* ctxs, const i
   rc = build_dense_tables(e);
   compute_phase_work(e);
   if (!rc) rc = refresh_sparse(e);
+  if (!rc) rc = build_gather_panels(e);
   if (rc) return bail(rc);
   if (hipStreamSynchronize(s) != hipSuccess) return bail(fail(DBS_ERR_HIP, "dbs_engine_create: setup"));
   *out = e;
@@ -1149,7 +1597,9 @@ int dbs_engine_destroy(dbs_engine* e) {
   void* bufs[] = {e->pq, e->coef, e->vals, e->grad, e->energy, e->orbv, e->gorb, e->wt, e->F, e->G,
                   e->d_inst, e->d_kg, e->d_dens, e->d_rows, e->d_row_out, e->d_seg_orb, e->d_seg_coef,
                   e->d_seg_w, e->d_roff, e->d_items, e->d_adjoff, e->d_adj, e->d_oadjoff, e->d_oadj,
-                  e->d_tiles, e->d_pairrow, e->d_pairoff, e->d_nextrow, e->plain_d, e->plain_c};
+                  e->d_tiles, e->d_pairrow, e->d_pairoff, e->d_nextrow, e->plain_d, e->plain_c,
+                  e->d_echunks, e->d_echoff, e->d_epart, e->d_seg_adj, e->d_gitems, e->d_citems,
+                  e->d_cntile, e->d_cpart, e->d_croff, e->d_doff, e->d_adjpre};
   for (void* p : bufs)
     if (p) hipFree(p);
   auto it = g_sparse.find(e);
@@ -1220,8 +1670,8 @@ int dbs_engine_graph_launch(dbs_engine* e) {
 }
 
 int dbs_engine_set_tuning(dbs_engine* e, int n, const int* p) {
-  DBS_CHECK(e && p && n >= 1 && n <= 5, DBS_ERR_ARG, "dbs_engine_set_tuning: bad arguments");
-  int t[5];
+  DBS_CHECK(e && p && n >= 1 && n <= 6, DBS_ERR_ARG, "dbs_engine_set_tuning: bad arguments");
+  int t[6];
   std::memcpy(t, e->tune, sizeof(t));
   for (int i = 0; i < n; ++i) t[i] = p[i];
   DBS_CHECK(t[0] >= 0 && t[0] <= 2, DBS_ERR_ARG, "tuning[0]: path 0 sparse, 1 dense, 2 auto");
@@ -1229,7 +1679,9 @@ int dbs_engine_set_tuning(dbs_engine* e, int n, const int* p) {
   DBS_CHECK(t[2] >= 1 && t[2] <= 16, DBS_ERR_ARG, "tuning[2]: waves per block 1..16");
   DBS_CHECK(t[3] >= 64 && t[3] <= 65536, DBS_ERR_ARG, "tuning[3]: grid cap");
   DBS_CHECK(t[4] >= 64 && t[4] <= 256 && t[4] % 64 == 0, DBS_ERR_ARG, "tuning[4]: block size 64..256");
+  DBS_CHECK(t[5] >= 64 && t[5] <= 65536, DBS_ERR_ARG, "tuning[5]: panel budget 64..65536 KiB");
   const bool path_changed = t[0] != e->tune[0];
+  const bool panels_changed = t[5] != e->tune[5];
   std::memcpy(e->tune, t, sizeof(t));
   if (path_changed) {
     for (int g = 0; g < e->nkg; ++g) e->group_dense[g] = (char)choose_dense(e, e->h_kg[g], e->group_rows[g].size());
@@ -1239,6 +1691,15 @@ int dbs_engine_set_tuning(dbs_engine* e, int n, const int* p) {
     compute_phase_work(e);
     DBS_HIP(hipStreamSynchronize(e->stream));
   }
+  if (panels_changed && !path_changed) {
+    int rc = refresh_sparse(e);
+    if (rc) return rc;
+  }
+  if (panels_changed) {
+    int rc = build_gather_panels(e);
+    if (rc) return rc;
+    DBS_HIP(hipStreamSynchronize(e->stream));
+  }
   // a captured graph holds the old launch shapes
   if (e->gexec) { hipGraphExecDestroy(e->gexec); e->gexec = nullptr; }
   if (e->graph) { hipGraphDestroy(e->graph); e->graph = nullptr; }
@@ -1246,7 +1707,7 @@ int dbs_engine_set_tuning(dbs_engine* e, int n, const int* p) {
 }
 
 int dbs_engine_get_tuning(dbs_engine* e, int n, int* p) {
-  DBS_CHECK(e && p && n >= 1 && n <= 5, DBS_ERR_ARG, "dbs_engine_get_tuning: bad arguments");
+  DBS_CHECK(e && p && n >= 1 && n <= 6, DBS_ERR_ARG, "dbs_engine_get_tuning: bad arguments");
   for (int i = 0; i < n; ++i) p[i] = e->tune[i];
   return DBS_OK;
 }
@@ -1265,10 +1726,11 @@ int dbs_engine_autotune(dbs_engine* e, int with_adjoint, int reps) {
   DBS_CHECK(e, DBS_ERR_ARG, "dbs_engine_autotune: null engine");
   if (reps < 1) reps = 3;
   const int paths[] = {0, 1};
-  const int fblocks[] = {64, 128, 256};
-  const int waves[] = {2, 4, 8};
-  const int caps[] = {1024, 4096};
-  int best[5];
+  const int fblocks[] = {128, 256};
+  const int waves[] = {4};
+  const int caps[] = {4096};
+  const int budgets[] = {1024, 2048, 4096};
+  int best[6];
   std::memcpy(best, e->tune, sizeof(best));
   float best_ms = 1e30f;
   hipEvent_t a, b;
@@ -1277,9 +1739,10 @@ int dbs_engine_autotune(dbs_engine* e, int with_adjoint, int reps) {
   for (int p : paths)
     for (int fb : fblocks)
       for (int w : waves)
-        for (int cap : caps) {
-          int t[5] = {p, fb, w, cap, 256};
-          if (dbs_engine_set_tuning(e, 5, t)) continue;
+        for (int cap : caps)
+          for (int bud : budgets) {
+          int t[6] = {p, fb, w, cap, 256, bud};
+          if (dbs_engine_set_tuning(e, 6, t)) continue;
           // warm
           int rc = with_adjoint ? dbs_engine_forward_adjoint(e) : dbs_engine_forward(e);
           if (rc) continue;
@@ -1302,7 +1765,7 @@ int dbs_engine_autotune(dbs_engine* e, int with_adjoint, int reps) {
         }
   hipEventDestroy(a);
   hipEventDestroy(b);
-  return dbs_engine_set_tuning(e, 5, best);
+  return dbs_engine_set_tuning(e, 6, best);
 }
 
 // ---------------------------------------------------------------------------

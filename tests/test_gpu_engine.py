@@ -297,7 +297,30 @@ def test_autotune_keeps_results(setups):
     eng.forward_adjoint()
     v0, g0 = eng.values(), eng.grad()
     best = eng.autotune(with_adjoint=True, reps=2)
-    assert len(best) == 5
+    assert len(best) == 6
     eng.forward_adjoint()
     assert np.max(relerr(eng.values(), v0)) < 1e-13
     assert np.max(np.abs(eng.grad() - g0)) <= 1e-13 * np.max(np.abs(g0))
+
+
+@pytest.mark.parametrize("kind", [0, 3, 5])
+def test_many_orbitals_fallback_paths(setups, kind):
+    """60 orbitals: the LDS-staged kernels fall back to their global-memory
+    forms where the per-interval tiles exceed LDS; values and adjoint stay
+    on the oracle"""
+    from dbsr3_amd import Engine
+    g, d = setups["small"]
+    norb = 60
+    fs = random_orbitals(g, norb, seed=77)
+    rng = np.random.default_rng(77)
+    rows = make_rows(rng, norb, 3, 150)
+    coef = rng.uniform(-1, 1, len(rows))
+    eng = Engine([(d, norb, rows)], kind=kind)
+    eng.set_orbitals(0, pq_of(fs))
+    eng.set_coef(0, coef)
+    eng.forward_adjoint()
+    ref = oracle_values(g, fs, rows, kind)
+    assert relerr(eng.values(), ref).max() < TOL
+    if kind == 0:
+        Gref = oracle_gradient(g, fs, rows, coef)
+        assert np.max(np.abs(eng.grad() - Gref)) < TOL * np.max(np.abs(Gref))
