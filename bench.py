@@ -32,6 +32,33 @@ PEAK_FP64_TFS = 78.6
 SEED = 20261203 + 4
 
 
+# engine phase -> the kernel that runs it
+PHASE_KERNEL = {
+    "orb_points": "k_orb_points", "weights": "k_weights", "features": "k_features",
+    "prefix_scan": "k_scan", "contract_rows": "k_contract_panel",
+    "contract_dense": "k_contract_dense", "energy": "k_energy_partial",
+    "adj_gather": "k_adj_gather_panel", "adj_suffix_scan": "k_scan",
+    "adj_orb_points": "k_adj_orb_lds", "adj_coef": "k_adj_coef",
+}
+# rocprofv3 FETCH_SIZE/WRITE_SIZE summary of this workload (HBM bytes per
+# dispatch, copy-kernel scaled); rows "| kernel | tuning | ... | bytes |"
+TRAFFIC_FILE = "engine_rk_fwd_adj_ns200_norb30_rows5e4_pmc.md"
+
+
+def measured_traffic(path, kernel, tuning):
+    """HBM bytes per launch of `kernel` from the committed counter summary,
+    if it was measured with the same tuning; else None."""
+    try:
+        with open(path) as f:
+            for line in f:
+                cells = [c.strip() for c in line.strip().strip("|").split("|")]
+                if len(cells) >= 6 and cells[0] == kernel and cells[1] == ",".join(map(str, tuning)):
+                    return float(cells[-1])
+    except OSError:
+        pass
+    return None
+
+
 def make_workload(ns=200, ks=9, ksp=8, ksq=9, norb=30, kmax=8, nrows=50000, seed=SEED):
     from dbsr3_amd.grid import mkgrid_02
     t = mkgrid_02(ns, ks, 26.0, 0.25, 30.0)
@@ -60,19 +87,20 @@ def normalise_on_gpu(d, pq):
 
 
 def cpu_baseline(t, ns, ks, ksp, ksq, pq, rows, budget_s):
-    """The oracle (C restatement of rk()/Rk_evaluate) on the rows of one
-    multipole, single thread."""
+    """The oracle (C restatement of mrk_*/convol/SUM_AmB as rk() and
+    Rk_evaluate use them) on the rows taken in multipole order, single
+    thread, until the time budget is spent: the R^k arrays of each k are
+    built inside the timed region as the reference builds them on first use."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     g = O.Grid(t, ns, ks, ksp, ksq)
     fs = [np.asfortranarray(pq[:, :, o]) for o in range(pq.shape[2])]
-    k0 = int(rows[0, 0])
-    sample = rows[rows[:, 0] == k0]
+    order = np.argsort(rows[:, 0], kind="stable")
     t0 = time.perf_counter()
-    vals = O.rk_rows(g, fs, sample, deadline=t0 + budget_s)
+    vals = O.rk_rows(g, fs, rows[order], deadline=t0 + budget_s)
     t1 = time.perf_counter()
     n = len(vals)
-    return n, t1 - t0, sample[:n], vals, k0
+    return n, t1 - t0, order[:n], vals
 
 
 def main():
@@ -85,6 +113,13 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--rows", type=int, default=50000)
+    ap.add_argument("--tuning", type=str, default="",
+                    help="comma-separated engine tuning (skips the autotuner)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="before the engine, copy MIB MiB device-to-device 3 times "
+                         "(a plain copy kernel of known size for hardware-counter scaling)")
+    ap.add_argument("--traffic-file", type=str,
+                    default=os.path.join(ROOT, "profiles", TRAFFIC_FILE))
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the engine shards by instance/row blocks "
@@ -94,6 +129,15 @@ def main():
     import torch
     from dbsr3_amd import DBS, Engine
 
+    if args.copy_calibration > 0:
+        n = args.copy_calibration * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
     ns, ks, ksp, ksq, norb = 200, 9, 8, 9, 30
     t, pq0, rows, coef = make_workload(nrows=args.rows)
     d = DBS(t, ns, ks, ksp, ksq)
@@ -101,7 +145,13 @@ def main():
     eng = Engine([(d, norb, rows)])
     eng.set_orbitals(0, pq)
     eng.set_coef(0, coef)
-    tuning = eng.autotune(with_adjoint=True, reps=3) if not args.no_autotune else eng.get_tuning()
+    if args.tuning:
+        eng.set_tuning([int(x) for x in args.tuning.split(",")])
+        tuning = eng.get_tuning()
+    elif args.no_autotune:
+        tuning = eng.get_tuning()
+    else:
+        tuning = eng.autotune(with_adjoint=True, reps=3)
     eng.graph_capture(with_adjoint=True)
     for _ in range(args.warmup):
         eng.graph_launch()
@@ -132,8 +182,9 @@ def main():
         roof = {"bound": "hbm", "achieved": dby / (dms * 1e-3) / 1e9, "peak": PEAK_HBM_GBS,
                 "unit": "GB/s"}
     roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["traffic"] = None
-    roof["kernel"] = dom
+    roof["traffic"] = measured_traffic(args.traffic_file, PHASE_KERNEL.get(dom, dom), tuning)
+    roof["kernel"] = PHASE_KERNEL.get(dom, dom)
+    roof["phase"] = dom
     roof["kernel_ms"] = dms
 
     out_extra = {}
@@ -155,11 +206,12 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        n, secs, sample, cvals, k0 = cpu_baseline(t, ns, ks, ksp, ksq, pq, rows, args.cpu_seconds)
-        sel = np.nonzero(rows[:, 0] == k0)[0][:n]
+        n, secs, sel, cvals = cpu_baseline(t, ns, ks, ksp, ksq, pq, rows, args.cpu_seconds)
         rel = np.abs(vals[sel] - cvals) / np.maximum(np.abs(cvals), 1e-3 * np.max(np.abs(cvals)))
+        kk = sorted(set(int(x) for x in rows[sel, 0]))
         cpu = {"value": n / secs, "unit": "R^k integrals/s", "cores": 1, "kind": "port",
-               "sample": f"{n} rows with k={k0} (arrays built inside the timed region), "
+               "sample": f"first {n} of the rows in multipole order (k={kk[0]}..{kk[-1]}; "
+                         f"R^k arrays of each k built inside the timed region), "
                          f"{secs:.1f} s single-thread"}
         out_extra["max_rel_err_vs_cpu_oracle"] = float(rel.max())
         out_extra["batched_vs_cpu"] = value / cpu["value"]
