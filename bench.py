@@ -32,17 +32,32 @@ PEAK_FP64_TFS = 78.6
 SEED = 20261203 + 4
 
 
-# engine phase -> the kernel that runs it
+# engine phase -> the kernel that runs it ({w}: pair-panel width)
 PHASE_KERNEL = {
     "orb_points": "k_orb_points", "weights": "k_weights", "features": "k_features",
-    "prefix_scan": "k_scan", "contract_rows": "k_contract_panel",
+    "prefix_scan": "k_scan_cols", "pair_panel": "{pair}<{w}, {v}, true>",
+    "contract_rows": "{pair}<{w}, {v}, false>", "rows_reduce": "k_rows_reduce",
     "contract_dense": "k_contract_dense", "energy": "k_energy_partial",
-    "adj_gather": "k_adj_gather_panel", "adj_suffix_scan": "k_scan",
-    "adj_orb_points": "k_adj_orb_lds", "adj_coef": "k_adj_coef",
+    "adj_suffix_scan": "k_scan_cols", "adj_orb_points": "k_adj_orb_lds", "adj_coef": "k_adj_coef",
 }
 # rocprofv3 FETCH_SIZE/WRITE_SIZE summary of this workload (HBM bytes per
 # dispatch, copy-kernel scaled); rows "| kernel | tuning | ... | bytes |"
 TRAFFIC_FILE = "engine_rk_fwd_adj_ns200_norb30_rows5e4_pmc.md"
+
+
+def pair_kernel(rows, tuning):
+    """the pair-panel kernel the engine launches for this row set (total
+    densities of each multipole; one density per thread when they fit a
+    workgroup: 512 at panel width 8, 256 at 16)"""
+    nd = 0
+    for k in np.unique(rows[:, 0]):
+        r = rows[rows[:, 0] == k]
+        pairs = {(min(a, c), max(a, c)) for a, c in r[:, [1, 3]]} | {(min(b, d), max(b, d)) for b, d in r[:, [2, 4]]}
+        nd = max(nd, len(pairs))
+    w = tuning[2]
+    if w >= 8 and nd <= (256 if w >= 16 else 512):
+        return "k_pair_rot", "false"
+    return "k_pair_panel", "true"
 
 
 def measured_traffic(path, kernel, tuning):
@@ -182,8 +197,10 @@ def main():
         roof = {"bound": "hbm", "achieved": dby / (dms * 1e-3) / 1e9, "peak": PEAK_HBM_GBS,
                 "unit": "GB/s"}
     roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["traffic"] = measured_traffic(args.traffic_file, PHASE_KERNEL.get(dom, dom), tuning)
-    roof["kernel"] = PHASE_KERNEL.get(dom, dom)
+    pk, pv = pair_kernel(rows, tuning)
+    kname = PHASE_KERNEL.get(dom, dom).format(w=tuning[2], pair=pk, v=pv)
+    roof["traffic"] = measured_traffic(args.traffic_file, kname, tuning)
+    roof["kernel"] = kname
     roof["phase"] = dom
     roof["kernel_ms"] = dms
 

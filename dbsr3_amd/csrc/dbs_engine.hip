@@ -61,29 +61,57 @@ struct InstDev {
   const double* bqin;
 };
 
-// one (instance, k) group of densities
+// one (instance, k) group of densities.  Its features (and their
+// gradients) form a (2J) x ndens row-major matrix at off_F, J = nv(1+ks)
+// pair slots j with rows x = 2j+h:
+//   j < nv (interval iv = j):     h=0 S(iv) (M^k(iv) before the prefix scan), h=1 M^{-(k+1)}(iv)
+//   j = nv + iv*ks + m:           h=0 Ro(iv,m) = Wm(iv,m) rho(iv,m),          h=1 Y(iv,m)
+// so R^k(L,R) = sum_j u_L(j) v_R(j) + v_L(j) u_R(j) with u = row 2j, v = row 2j+1.
 struct KGroupDev {
   int inst, k, ndens, dens0;   // densities dens0 .. dens0+ndens-1
   long long off_w;             // weight tables: Wk(nv*ks), Wm(nv*ks), wkin(nv*ks*ks)
+  long long off_F;             // feature / gradient matrix of the group
+  long long off_part;          // contraction partials: npr x nrows
+  int nrows, row0;             // the group's rows row0 .. row0+nrows-1 (group order)
+  int npr, dense;              // partial slots per row; rows contracted on the matrix cores
+  int chunk0, nchunk;          // adjacency chunks of 64 densities
 };
 
 struct DensDev {
   int kg, a, c, kind;          // orbitals 0-based
-  long long off_f;             // feature vector offset (F and G buffers)
 };
 
 struct RowDev {
   int L, R, kg, pad;           // global density indices
 };
 
-// adjacency for the adjoint: partner density, row, role (0: this is L)
-struct AdjDev {
-  int partner, row, role, pad;
-};
-
 // orbital -> densities adjacency: density, partner orbital, factor, comp-mask
 struct OrbAdjDev {
   int dens, partner, factor, kind;
+};
+
+// adjacency of the densities of a group in 64-wide column blocks (entry i
+// of density 64c+t at base + 64 i + t): partner density (group-local, -1
+// pads the column) and the row's slot in the group when this density is
+// the row's left one and its value is wanted (-1: the density is the
+// row's right one; -2: left one, value produced elsewhere)
+struct AdjChunk {
+  long long base;
+  int len, pad;
+};
+struct AdjEnt {
+  int P, slot;
+};
+
+// pair-panel work: slots [j0, j1) of group g, partial column pr
+struct PairWork {
+  int g, j0, j1, pr;
+};
+
+// sparse-row reduction: value = sum_p part[pbase + p*stride], p < npr
+struct RowRed {
+  long long pbase, out;
+  int stride, npr;
 };
 
 __host__ __device__ inline int feat_dim(int nv, int ks) { return 2 * nv + 2 * nv * ks; }
@@ -217,114 +245,88 @@ __global__ void __launch_bounds__(256) k_features(
                                : orbv + I.off_orbv + ((long long)(2 * D.a) * nv + iv) * np;
       const double* vc = stage ? sv + (2 * D.c) * np
                                : orbv + I.off_orbv + ((long long)(2 * D.c) * nv + iv) * np;
-      double* f = F + D.off_f;
+      // column d of the group's feature matrix
+      double* f = F + G.off_F + d;
+      const long long ld = G.ndens;
       double mk = 0.0, mm = 0.0;
       for (int m = 0; m < ks; ++m) {
         double ro = dens_point(va, vc, cstride, D.kind, m);
         mk += Wk[m] * ro;
         mm += Wm[m] * ro;
-        f[2 * nv + iv * ks + m] = Wm[m] * ro;
         double y = 0.0;
         for (int i = 0; i < ks; ++i) y += wk[m * ks + i] * dens_point(va, vc, cstride, D.kind, ks + m * ks + i);
-        f[2 * nv + nv * ks + iv * ks + m] = y;
+        const long long x = 2LL * (nv + iv * ks + m);
+        f[x * ld] = Wm[m] * ro;
+        f[(x + 1) * ld] = y;
       }
-      f[iv] = mk;
-      f[nv + iv] = mm;
+      f[(2LL * iv) * ld] = mk;
+      f[(2LL * iv + 1) * ld] = mm;
     }
   }
 }
 
-// exclusive prefix (dir=0) or suffix (dir=1) scan of nv values, one wave
-// per density; each lane owns a contiguous chunk.
-__device__ inline void wave_scan_excl(double* x, int nv, int dir) {
-  const int lane = threadIdx.x & 63;
-  const int chunk = (nv + 63) / 64;
-  const int lo = lane * chunk, hi = min(lo + chunk, nv);
-  double sum = 0.0;
-  if (dir == 0) { for (int i = lo; i < hi; ++i) sum += x[i]; }
-  else { for (int i = hi - 1; i >= lo; --i) sum += x[i]; }
-  // exclusive scan of lane sums in lane order (dir 0) or reverse (dir 1)
-  double incl = sum;
-  for (int off = 1; off < 64; off <<= 1) {
-    double v = dir == 0 ? __shfl_up(incl, off, 64) : __shfl_down(incl, off, 64);
-    bool take = dir == 0 ? lane >= off : lane + off < 64;
-    if (take) incl += v;
-  }
-  double run = incl - sum;
-  if (dir == 0) {
-    for (int i = lo; i < hi; ++i) { double v = x[i]; x[i] = run; run += v; }
-  } else {
-    for (int i = hi - 1; i >= lo; --i) { double v = x[i]; x[i] = run; run += v; }
-  }
-}
-
-__global__ void k_scan(int ndens, const DensDev* __restrict__ dens, const KGroupDev* __restrict__ kgs,
-                       const InstDev* __restrict__ inst, double* __restrict__ F, int dir) {
-  const int waves = blockDim.x >> 6;
-  for (int d = blockIdx.x * waves + (threadIdx.x >> 6); d < ndens; d += gridDim.x * waves) {
-    const DensDev D = dens[d];
-    const int nv = inst[kgs[D.kg].inst].nv;
-    wave_scan_excl(F + D.off_f, nv, dir);
-  }
-}
-
-// ---------------------------------------------------------------------------
-// phase 3: contraction, one wavefront per row
-// ---------------------------------------------------------------------------
-__global__ void k_contract_rows(int nrows, const RowDev* __restrict__ rows,
-                                const DensDev* __restrict__ dens, const KGroupDev* __restrict__ kgs,
-                                const InstDev* __restrict__ inst, const double* __restrict__ F,
-                                const long long* __restrict__ row_out, int onesided,
-                                double* __restrict__ vals) {
-  const int waves = blockDim.x >> 6;
-  const int lane = threadIdx.x & 63;
-  for (int r = blockIdx.x * waves + (threadIdx.x >> 6); r < nrows; r += gridDim.x * waves) {
-    const RowDev R = rows[r];
-    const DensDev DL = dens[R.L], DR = dens[R.R];
-    const InstDev I = inst[kgs[R.kg].inst];
-    const int nv = I.nv, nk = nv * I.ks;
-    const double* fl = F + DL.off_f;
-    const double* fr = F + DR.off_f;
-    double acc = 0.0;
-    if (!onesided) {
-      for (int x = lane; x < nv; x += 64) acc += fl[x] * fr[nv + x] + fl[nv + x] * fr[x];
-      const double* rl = fl + 2 * nv;
-      const double* rr = fr + 2 * nv;
-      for (int x = lane; x < nk; x += 64) acc += rl[x] * rr[nk + x] + rl[nk + x] * rr[x];
-    } else {
-      for (int x = lane; x < nv; x += 64) acc += fl[x] * fr[nv + x];
-      const double* rl = fl + 2 * nv;
-      const double* rr = fr + 2 * nv;
-      for (int x = lane; x < nk; x += 64) acc += rl[nk + x] * rr[x];
+// exclusive prefix (dir=0) or suffix (dir=1) scan over the intervals of the
+// S rows (x = 2 iv) of every density column.  Workgroup = 32 columns x 8
+// interval segments; items (group, first column).  Each thread sums its
+// segment, the segment sums are combined in LDS in a fixed order, then the
+// thread rewrites its segment with the exclusive running sums.
+#define DBS_SCAN_COLS 32
+#define DBS_SCAN_SEGS 8
+__global__ void __launch_bounds__(256) k_scan_cols(int nitems, const int2* __restrict__ items,
+                                                   const KGroupDev* __restrict__ kgs,
+                                                   const InstDev* __restrict__ inst, double* __restrict__ F,
+                                                   int dir) {
+  __shared__ double ssum[DBS_SCAN_SEGS][DBS_SCAN_COLS];
+  const int col = threadIdx.x % DBS_SCAN_COLS, seg = threadIdx.x / DBS_SCAN_COLS;
+  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const KGroupDev G = kgs[items[it].x];
+    const int nv = inst[G.inst].nv, nd = G.ndens;
+    const int d = items[it].y + col;
+    const int per = (nv + DBS_SCAN_SEGS - 1) / DBS_SCAN_SEGS;
+    const int lo = min(nv, seg * per), hi = min(nv, lo + per);
+    double* f = F + G.off_F + d;
+    const long long st = 2LL * nd;  // S(iv) at row 2 iv
+    double sum = 0.0;
+    if (d < nd) {
+      if (dir == 0) { for (int iv = lo; iv < hi; ++iv) sum += f[iv * st]; }
+      else { for (int iv = hi - 1; iv >= lo; --iv) sum += f[iv * st]; }
     }
-    acc = wave_sum(acc);
-    if (lane == 0) vals[row_out[r]] = acc;
+    __syncthreads();
+    ssum[seg][col] = sum;
+    __syncthreads();
+    if (d < nd) {
+      double run = 0.0;
+      if (dir == 0) { for (int q = 0; q < seg; ++q) run += ssum[q][col]; }
+      else { for (int q = DBS_SCAN_SEGS - 1; q > seg; --q) run += ssum[q][col]; }
+      if (dir == 0) {
+        for (int iv = lo; iv < hi; ++iv) { const double v = f[iv * st]; f[iv * st] = run; run += v; }
+      } else {
+        for (int iv = hi - 1; iv >= lo; --iv) { const double v = f[iv * st]; f[iv * st] = run; run += v; }
+      }
+    }
   }
 }
 
 // ---------------------------------------------------------------------------
 // dense contraction on the FP64 matrix cores.  For a group whose rows cover
-// a large part of all density pairs, G = F * F_sigma^T over the feature
-// dimension is one GEMM; v_mfma_f64_16x16x4f64 tiles of 16x16 outputs.
-// One wavefront computes a 16x16 tile of G and scatters the requested pairs.
+// a large part of all density pairs, the values are entries of
+// U V^T + V U^T over the pair slots, one GEMM over the feature rows;
+// v_mfma_f64_16x16x4f64 tiles of 16x16 outputs.  One wavefront computes a
+// 16x16 tile and scatters the requested pairs.
 // ---------------------------------------------------------------------------
 typedef double __attribute__((ext_vector_type(4))) dbl4;
 
-__device__ inline double fsig(const double* f, int nv, int nk, int x, int onesided) {
-  // sigma-permuted feature x of F_R (the partner vector of an L feature)
-  // x in [0,nv): S_L partners Mm_R; [nv,2nv): Mm_L partners S_R (0 if onesided)
-  // [2nv, 2nv+nk): Ro_L partners Y_R (0 if onesided); [2nv+nk, ...): Y_L partners Ro_R
-  if (x < nv) return f[nv + x];
-  if (x < 2 * nv) return onesided ? 0.0 : f[x - nv];
-  int y = x - 2 * nv;
-  if (y < nk) return onesided ? 0.0 : f[2 * nv + nk + y];
-  return f[2 * nv + (y - nk)];
+// partner row of feature row x of the left density (two-sided: u<->v;
+// one-sided S^k: S_L with Mm_R for j < nv, Y_L with Ro_R beyond); -1: none
+__device__ inline int partner_row(int x, int nv, int onesided) {
+  if (!onesided) return x ^ 1;
+  if (x < 2 * nv) return (x & 1) ? -1 : x + 1;
+  return (x & 1) ? x - 1 : -1;
 }
 
 __global__ void __launch_bounds__(256) k_contract_dense(
     int ntiles, const int4* __restrict__ tiles, const KGroupDev* __restrict__ kgs,
-    const InstDev* __restrict__ inst, const DensDev* __restrict__ dens,
-    const double* __restrict__ F, const int* __restrict__ pairrow, const long long* __restrict__ pairoff,
+    const InstDev* __restrict__ inst, const double* __restrict__ F, const int* __restrict__ pairrow, const long long* __restrict__ pairoff,
     const int* __restrict__ nextrow, const long long* __restrict__ row_out, int onesided,
     double* __restrict__ vals) {
   // tile = (group, Ltile, Rtile, 0); pairrow[pairoff[g] + l*ndens + r] = row or -1
@@ -334,18 +336,18 @@ __global__ void __launch_bounds__(256) k_contract_dense(
     const int4 T = tiles[t];
     const KGroupDev G = kgs[T.x];
     const InstDev I = inst[G.inst];
-    const int nv = I.nv, nk = nv * I.ks, D = feat_dim(nv, I.ks);
+    const int nv = I.nv, D = feat_dim(nv, I.ks), nd = G.ndens;
+    const double* Fg = F + G.off_F;
     const int l0 = T.y * 16, r0 = T.z * 16;
     // A: 16 x 4 slice (rows l0+lane%16, cols x0 + lane/16); B: 4 x 16 slice
     const int li = l0 + (lane & 15), ri = r0 + (lane & 15), kq = lane >> 4;
-    const bool lv = li < G.ndens, rv = ri < G.ndens;
-    const double* fl = F + dens[G.dens0 + (lv ? li : 0)].off_f;
-    const double* fr = F + dens[G.dens0 + (rv ? ri : 0)].off_f;
+    const bool lv = li < nd, rv = ri < nd;
     dbl4 acc = {0.0, 0.0, 0.0, 0.0};
     for (int x0 = 0; x0 < D; x0 += 4) {
       const int x = x0 + kq;
-      double a = (lv && x < D) ? fl[x] : 0.0;
-      double b = (rv && x < D) ? fsig(fr, nv, nk, x, onesided) : 0.0;
+      const int xp = x < D ? partner_row(x, nv, onesided) : -1;
+      double a = (lv && xp >= 0) ? Fg[(long long)x * nd + li] : 0.0;
+      double b = (rv && xp >= 0) ? Fg[(long long)xp * nd + ri] : 0.0;
       acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
     }
     // output layout of the 16x16x4 f64 MFMA (one output group per
@@ -354,8 +356,8 @@ __global__ void __launch_bounds__(256) k_contract_dense(
     for (int j = 0; j < 4; ++j) {
       const int row = 4 * j + (lane >> 4);
       const int L = l0 + row, R = r0 + col;
-      if (L < G.ndens && R < G.ndens) {
-        for (int rr = pairrow[pairoff[T.x] + (long long)L * G.ndens + R]; rr >= 0; rr = nextrow[rr])
+      if (L < nd && R < nd) {
+        for (int rr = pairrow[pairoff[T.x] + (long long)L * nd + R]; rr >= 0; rr = nextrow[rr])
           vals[row_out[rr]] = acc[j];
       }
     }
@@ -399,154 +401,289 @@ __global__ void k_energy_final(int ninst, const int* __restrict__ choff, const d
 }
 
 // ---------------------------------------------------------------------------
-// adjoint phase A: G_d = sum over rows touching density d of coef * partner
-// vector.  Each workgroup owns one density (all feature elements), walks
-// its adjacency list in row order.
+// pair panels: contraction of the rows and (with the adjoint) the gather
+//   G_X = sum over rows touching X of coef * (partner features, u<->v)
+// in one pass over the features.  A workgroup owns pair slots [j0, j1) of
+// one group and walks them W slots at a time: the W-slot panel of every
+// density of the group is staged in LDS as [density][u(W) v(W)], then each
+// thread takes one density X, keeps its own panel and gradient in
+// registers and streams its adjacency: per entry the partner's 2W values
+// come from LDS, the row's partial value (X = left density) and the
+// gradient are accumulated in a fixed order.  Partial values of a row over
+// the workgroup's panels go to part[off_part + pr*nrows + slot]; the row
+// value is their sum in pr order (k_rows_reduce).  Every feature is read
+// from HBM once per launch and every gradient written once.
+// LDS = false reads the partner panels from global memory (groups whose
+// panel does not fit in LDS).
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_adj_gather(
-    int ndens, const DensDev* __restrict__ dens, const KGroupDev* __restrict__ kgs,
-    const InstDev* __restrict__ inst, const int* __restrict__ adjoff, const AdjDev* __restrict__ adj,
-    const double* __restrict__ coef, const long long* __restrict__ row_out,
-    const double* __restrict__ F, int onesided, double* __restrict__ Gb) {
-  for (int d = blockIdx.x; d < ndens; d += gridDim.x) {
-    const DensDev Dd = dens[d];
-    const InstDev I = inst[kgs[Dd.kg].inst];
-    const int nv = I.nv, nk = nv * I.ks, D = feat_dim(nv, I.ks);
-    double* g = Gb + Dd.off_f;
-    const int a0 = adjoff[d], a1 = adjoff[d + 1];
-    for (int x = threadIdx.x; x < D; x += blockDim.x) {
-      double acc = 0.0;
-      for (int e = a0; e < a1; ++e) {
-        const AdjDev A = adj[e];
-        const double c = coef[row_out[A.row]];
-        const double* fp = F + dens[A.partner].off_f;
-        double v;
-        if (!onesided) {
-          v = fsig(fp, nv, nk, x, 0);
-        } else if (A.role == 0) {
-          // d is L: dS_L <- Mm_R, dY_L <- Ro_R
-          v = (x < nv) ? fp[nv + x] : (x >= 2 * nv + nk ? fp[2 * nv + (x - 2 * nv - nk)] : 0.0);
-        } else {
-          // d is R: dMm_R <- S_L, dRo_R <- Y_L
-          v = (x >= nv && x < 2 * nv) ? fp[x - nv]
-                                      : ((x >= 2 * nv && x < 2 * nv + nk) ? fp[x + nk] : 0.0);
+// the register panel of W = 16 needs the full register file of a lane
+// (workgroups of at most 256 threads); W <= 8 runs 512-thread workgroups
+template <int W>
+constexpr int pair_block_max() { return W >= 16 ? 256 : 512; }
+
+template <int W, bool LDS, bool ADJ>
+__global__ void __launch_bounds__(pair_block_max<W>()) k_pair_panel(
+    int nwork, const PairWork* __restrict__ work, const KGroupDev* __restrict__ kgs,
+    const InstDev* __restrict__ inst, const AdjChunk* __restrict__ chunks,
+    const AdjEnt* __restrict__ ent, const int* __restrict__ alen, const double* __restrict__ cent,
+    const double* __restrict__ F, double* __restrict__ Gout, double* __restrict__ part, int onesided) {
+  extern __shared__ double sp[];
+  constexpr int LD = 2 * W + 2;  // LDS row: u(W) v(W) + pad (16-byte aligned rows)
+  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+    const PairWork Wk = work[w];
+    const KGroupDev Gp = kgs[Wk.g];
+    const int nv = inst[Gp.inst].nv, nd = Gp.ndens;
+    const double* Fg = F + Gp.off_F;
+    bool first = true;
+    for (int j0 = Wk.j0; j0 < Wk.j1; j0 += W) {
+      const int nj = min(W, Wk.j1 - j0);
+      if (LDS) {
+        __syncthreads();
+        const int tot = 2 * W * nd;
+        for (int q = threadIdx.x; q < tot; q += blockDim.x) {
+          const int x = q / nd, d = q - x * nd;  // x = 2 jl + h
+          const int jl = x >> 1, h = x & 1;
+          sp[d * LD + h * W + jl] = jl < nj ? Fg[(2LL * j0 + x) * nd + d] : 0.0;
         }
-        acc += c * v;
+        __syncthreads();
       }
-      g[x] = acc;
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// panel forms.  The feature dimension of a group is cut into panels whose
-// features for all the group's densities fit one XCD's L2; every work item
-// on a panel is dispatched to the same XCD (workgroup i runs on XCD i % 8),
-// so partner features are read from HBM/MALL once per panel and then served
-// from that XCD's L2.  Items with d < 0 are padding.
-// ---------------------------------------------------------------------------
-struct PanelItem {
-  int d;       // density (gather) or first row (contraction)
-  int n;       // rows in this item (contraction)
-  int x0, x1;  // feature range
-  int tile;    // panel index within the group (contraction partials)
-  int nv, nk;  // intervals and nv*ks of the group's grid
-  int pad;
-};
-
-// per adjacency entry: partner feature offset, coefficient index, role
-struct AdjPre {
-  long long off;
-  long long cidx;
-  int role, pad;
-};
-
-__global__ void __launch_bounds__(256) k_adj_gather_panel(
-    int nitems, const PanelItem* __restrict__ items, const long long* __restrict__ doff,
-    const int* __restrict__ adjoff, const AdjPre* __restrict__ adj, const double* __restrict__ coef,
-    const double* __restrict__ F, int onesided, double* __restrict__ Gb) {
-  __shared__ long long s_off[256];
-  __shared__ double s_c[256];
-  __shared__ int s_role[256];
-  const int it = blockIdx.x;
-  if (it >= nitems) return;
-  const PanelItem P = items[it];
-  if (P.d < 0) return;
-  const int nv = P.nv, nk = P.nk;
-  double* g = Gb + doff[P.d];
-  const int a0 = adjoff[P.d], a1 = adjoff[P.d + 1];
-  // thread-owned accumulators for x = x0 + tid + k*blockDim
-  constexpr int XMAX = 8;
-  double acc[XMAX];
-  for (int q = 0; q < XMAX; ++q) acc[q] = 0.0;
-  for (int c0 = a0; c0 < a1; c0 += 256) {
-    const int cn = min(256, a1 - c0);
-    __syncthreads();
-    if ((int)threadIdx.x < cn) {
-      const AdjPre A = adj[c0 + threadIdx.x];
-      s_off[threadIdx.x] = A.off;
-      s_c[threadIdx.x] = coef[A.cidx];
-      s_role[threadIdx.x] = A.role;
-    }
-    __syncthreads();
-    for (int e = 0; e < cn; ++e) {
-      const double* fp = F + s_off[e];
-      const double c = s_c[e];
-      const int role = s_role[e];
+      // panel value of density d at (h, jl)
+      auto pv = [&](int d, int h, int jl) -> double {
+        if (LDS) return sp[d * LD + h * W + jl];
+        return jl < nj ? Fg[(2LL * (j0 + jl) + h) * nd + d] : 0.0;
+      };
+      for (int X = threadIdx.x; X < nd; X += blockDim.x) {
+        double fu[W], fv[W], gu[W], gv[W];
 #pragma unroll
-      for (int q = 0; q < XMAX; ++q) {
-        const int x = P.x0 + threadIdx.x + q * blockDim.x;
-        if (x >= P.x1) break;
-        double v;
-        if (!onesided) {
-          v = fsig(fp, nv, nk, x, 0);
-        } else if (role == 0) {
-          v = (x < nv) ? fp[nv + x] : (x >= 2 * nv + nk ? fp[2 * nv + (x - 2 * nv - nk)] : 0.0);
-        } else {
-          v = (x >= nv && x < 2 * nv) ? fp[x - nv] : ((x >= 2 * nv && x < 2 * nv + nk) ? fp[x + nk] : 0.0);
+        for (int jl = 0; jl < W; ++jl) {
+          fu[jl] = pv(X, 0, jl);
+          fv[jl] = pv(X, 1, jl);
+          gu[jl] = 0.0;
+          gv[jl] = 0.0;
         }
-        acc[q] += c * v;
+        const AdjChunk C = chunks[Gp.chunk0 + (X >> 6)];
+        const long long eb = C.base + (X & 63);
+        const int n = alen[Gp.dens0 + X];
+        // entries are fetched PF at a time so their loads overlap; the tail
+        // of the last group is padded with inert entries (own row, c = 0,
+        // no row value)
+        constexpr int PF = 4;
+        for (int i0 = 0; i0 < n; i0 += PF) {
+          AdjEnt Aq[PF];
+          double cq[PF];
+#pragma unroll
+          for (int q = 0; q < PF; ++q) {
+            const int i = i0 + q;
+            if (i < n) {
+              Aq[q] = ent[eb + 64LL * i];
+              cq[q] = ADJ ? cent[eb + 64LL * i] : 0.0;
+            } else {
+              Aq[q].P = X;
+              Aq[q].slot = -1;
+              cq[q] = 0.0;
+            }
+          }
+#pragma unroll
+          for (int q = 0; q < PF; ++q) {
+            const AdjEnt A = Aq[q];
+            if (!ADJ && A.slot < 0) continue;
+            const double c = cq[q];
+            double dot = 0.0;
+#pragma unroll
+            for (int jl = 0; jl < W; ++jl) {
+              const double pu = pv(A.P, 0, jl), pvv = pv(A.P, 1, jl);
+              if (!onesided) {
+                dot += fu[jl] * pvv + fv[jl] * pu;
+                if (ADJ) {
+                  gu[jl] += c * pvv;
+                  gv[jl] += c * pu;
+                }
+              } else {
+                // S^k: S_L Mm_R (j < nv) and Y_L Ro_R (j >= nv)
+                const bool lower = j0 + jl < nv;
+                const bool left = A.slot != -1;
+                if (lower) {
+                  dot += fu[jl] * pvv;
+                  if (ADJ) {
+                    if (left) gu[jl] += c * pvv;
+                    else gv[jl] += c * pu;
+                  }
+                } else {
+                  dot += fv[jl] * pu;
+                  if (ADJ) {
+                    if (left) gv[jl] += c * pu;
+                    else gu[jl] += c * pvv;
+                  }
+                }
+              }
+            }
+            if (A.slot >= 0) {
+              double* o = part + Gp.off_part + (long long)Wk.pr * Gp.nrows + A.slot;
+              *o = first ? dot : *o + dot;
+            }
+          }
+        }
+        if (ADJ) {
+          double* g = Gout + Gp.off_F + X;
+#pragma unroll
+          for (int jl = 0; jl < W; ++jl) {
+            if (jl < nj) {
+              g[(2LL * (j0 + jl)) * nd] = gu[jl];
+              g[(2LL * (j0 + jl) + 1) * nd] = gv[jl];
+            }
+          }
+        }
       }
+      first = false;
     }
   }
+}
+
+// Bank-conflict-free pair panels for groups with ndens <= blockDim (one
+// density per thread).  The panel of density X is E = 2W doubles
+// [u0 v0 u1 v1 ...] (element e = 2 jl + h), kept in LDS as B[X][e] with
+// rows of E doubles, so the LDS bank of element e does not depend on the
+// row.  Lane l works on the elements in the rotated order
+// e_t = (t + l) mod E: at every step the 16 lanes that share an LDS cycle
+// touch 16 different banks, wherever their partner rows are.  Registers
+// fx[t], gx[t] hold element e_t.  The transposition between the global
+// layout (rows e, columns X) and the rotated registers goes through a
+// per-thread column of A[e][S] (S even: conflict-free in both directions).
+template <int W, bool ONE, bool ADJ>
+__global__ void __launch_bounds__(pair_block_max<W>()) k_pair_rot(
+    int nwork, const PairWork* __restrict__ work, const KGroupDev* __restrict__ kgs,
+    const InstDev* __restrict__ inst, const AdjChunk* __restrict__ chunks,
+    const AdjEnt* __restrict__ ent, const int* __restrict__ alen, const double* __restrict__ cent,
+    const double* __restrict__ F, double* __restrict__ Gout, double* __restrict__ part) {
+  extern __shared__ double sm[];
+  constexpr int E = 2 * W;
+  constexpr int PF = 2;
+  const int X = threadIdx.x;
+  const int rot = X & 15;
+  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+    const PairWork Wk = work[w];
+    const KGroupDev Gp = kgs[Wk.g];
+    const int nv = inst[Gp.inst].nv, nd = Gp.ndens;
+    const int S = nd + (nd & 1);
+    double* A = sm;           // [E][S]
+    double* B = sm + E * S;   // [nd][E]
+    const double* Fg = F + Gp.off_F;
+    const bool act = X < nd;
+    int n = 0;
+    long long eb = 0;
+    if (act) {
+      const AdjChunk C = chunks[Gp.chunk0 + (X >> 6)];
+      eb = C.base + (X & 63);
+      n = alen[Gp.dens0 + X];
+    }
+    bool first = true;
+    for (int j0 = Wk.j0; j0 < Wk.j1; j0 += W) {
+      const int nj = min(W, Wk.j1 - j0);
+      // B of the previous panel is no longer read
+      __syncthreads();
+      double fx[E], gx[E];
+      unsigned mL = 0;  // S^k: elements that pair as the row's left density
+      if (act) {
 #pragma unroll
-  for (int q = 0; q < XMAX; ++q) {
-    const int x = P.x0 + threadIdx.x + q * blockDim.x;
-    if (x < P.x1) g[x] = acc[q];
+        for (int e = 0; e < E; ++e) A[e * S + X] = (e >> 1) < nj ? Fg[(2LL * j0 + e) * nd + X] : 0.0;
+#pragma unroll
+        for (int t = 0; t < E; ++t) {
+          const int e = (t + rot) & (E - 1);
+          fx[t] = A[e * S + X];
+          B[X * E + e] = fx[t];
+          gx[t] = 0.0;
+          if (ONE) {
+            const bool lower = j0 + (e >> 1) < nv;
+            if (lower ? (e & 1) == 0 : (e & 1) == 1) mL |= 1u << t;
+          }
+        }
+      }
+      __syncthreads();
+      for (int i0 = 0; i0 < n; i0 += PF) {
+        AdjEnt Aq[PF];
+        double cq[PF];
+#pragma unroll
+        for (int q = 0; q < PF; ++q) {
+          const int i = i0 + q;
+          if (i < n) {
+            Aq[q] = ent[eb + 64LL * i];
+            cq[q] = ADJ ? cent[eb + 64LL * i] : 0.0;
+          } else {
+            Aq[q].P = X;
+            Aq[q].slot = -1;
+            cq[q] = 0.0;
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < PF; ++q) {
+          const AdjEnt Aen = Aq[q];
+          if (!ADJ && Aen.slot < 0) continue;
+          const double c = cq[q];
+          const double* pr = B + Aen.P * E;
+          double dot = 0.0;
+          if (!ONE) {
+#pragma unroll
+            for (int t = 0; t < E; ++t) {
+              const double p = pr[((t + rot) & (E - 1)) ^ 1];
+              dot += fx[t] * p;
+              if (ADJ) gx[t] += c * p;
+            }
+          } else {
+            const unsigned mg = Aen.slot != -1 ? mL : ~mL;
+#pragma unroll
+            for (int t = 0; t < E; ++t) {
+              const double p = pr[((t + rot) & (E - 1)) ^ 1];
+              if ((mL >> t) & 1u) dot += fx[t] * p;
+              if (ADJ && ((mg >> t) & 1u)) gx[t] += c * p;
+            }
+          }
+          if (Aen.slot >= 0) {
+            double* o = part + Gp.off_part + (long long)Wk.pr * Gp.nrows + Aen.slot;
+            *o = first ? dot : *o + dot;
+          }
+        }
+      }
+      if (ADJ && act) {
+        // own column of A: rotated registers -> rows e -> coalesced rows of G
+#pragma unroll
+        for (int t = 0; t < E; ++t) A[((t + rot) & (E - 1)) * S + X] = gx[t];
+        double* g = Gout + Gp.off_F + X;
+#pragma unroll
+        for (int e = 0; e < E; ++e)
+          if ((e >> 1) < nj) g[(2LL * j0 + e) * nd] = A[e * S + X];
+      }
+      first = false;
+    }
   }
 }
 
-// contraction partials: one wavefront per row of the item, lanes over the
-// panel's feature range; part[row * maxtile + tile].  roff[r] = feature
-// offsets of the row's two densities.
-__global__ void __launch_bounds__(256) k_contract_panel(
-    int nitems, const PanelItem* __restrict__ items, const longlong2* __restrict__ roff,
-    const double* __restrict__ F, int onesided, int maxtile, double* __restrict__ part) {
-  const int it = blockIdx.x;
-  if (it >= nitems) return;
-  const PanelItem P = items[it];
-  if (P.d < 0) return;
-  const int nv = P.nv, nk = P.nk;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  for (int q = wv; q < P.n; q += nw) {
-    const int r = P.d + q;
-    const longlong2 o = roff[r];
-    const double* fl = F + o.x;
-    const double* fr = F + o.y;
+// row values of the pair-panel path: DBS_RED_LANES lanes per row, lane s
+// sums slots p = s, s+L, ... in order, the lane sums are combined by a
+// fixed butterfly
+#define DBS_RED_LANES 8
+__global__ void k_rows_reduce(int n, const RowRed* __restrict__ red, const double* __restrict__ part,
+                              double* __restrict__ vals) {
+  // the DBS_RED_LANES consecutive lanes of a row share r, so they enter and
+  // leave the loop together and all take part in the shuffles
+  const int sub = threadIdx.x % DBS_RED_LANES;
+  const long long step = (long long)gridDim.x * blockDim.x / DBS_RED_LANES;
+  for (long long r = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / DBS_RED_LANES; r < n; r += step) {
+    const RowRed R = red[r];
     double acc = 0.0;
-    for (int x = P.x0 + lane; x < P.x1; x += 64) acc += fl[x] * fsig(fr, nv, nk, x, onesided);
-    acc = wave_sum(acc);
-    if (lane == 0) part[(long long)r * maxtile + P.tile] = acc;
+    for (int p = sub; p < R.npr; p += DBS_RED_LANES) acc += part[R.pbase + (long long)p * R.stride];
+#pragma unroll
+    for (int off = 1; off < DBS_RED_LANES; off <<= 1) acc += __shfl_xor(acc, off, DBS_RED_LANES);
+    if (sub == 0) vals[R.out] = acc;
   }
 }
 
-__global__ void k_contract_reduce(int nrows, const int* __restrict__ ntile, int maxtile,
-                                  const double* __restrict__ part, const long long* __restrict__ row_out,
-                                  double* __restrict__ vals) {
-  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < nrows; r += gridDim.x * blockDim.x) {
-    double acc = 0.0;
-    for (int t = 0; t < ntile[r]; ++t) acc += part[(long long)r * maxtile + t];
-    vals[row_out[r]] = acc;
+// coefficients of the adjacency entries (0 on padding)
+__global__ void k_adj_cgather(long long n, const long long* __restrict__ cidx, const double* __restrict__ coef,
+                              double* __restrict__ cent) {
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+       q += (long long)gridDim.x * blockDim.x) {
+    const long long c = cidx[q];
+    cent[q] = c >= 0 ? coef[c] : 0.0;
   }
 }
 
@@ -595,15 +732,17 @@ __global__ void k_adj_orb_points(int ninst, const long long* __restrict__ seg,
         else { if (comp != 1) continue; partner = vp[p]; }
       }
       const KGroupDev G = kgs[D.kg];
-      const double* g = Gb + D.off_f;
+      const long long nd = G.ndens;
+      const double* g = Gb + G.off_F + (A.dens - G.dens0);
+      const long long xr = 2LL * (nv + iv * ks + m);
       double grho;
       if (i < 0) {
         const double wkv = wt[G.off_w + (long long)iv * ks + m];
         const double wmv = wt[G.off_w + (long long)nk + (long long)iv * ks + m];
-        grho = g[iv] * wkv + (g[nv + iv] + g[2 * nv + iv * ks + m]) * wmv;
+        grho = g[2LL * iv * nd] * wkv + (g[(2LL * iv + 1) * nd] + g[xr * nd]) * wmv;
       } else {
         const double wki = wt[G.off_w + 2LL * nk + ((long long)iv * ks + m) * ks + i];
-        grho = g[2 * nv + nk + iv * ks + m] * wki;
+        grho = g[(xr + 1) * nd] * wki;
       }
       acc += (double)A.factor * grho * partner;
     }
@@ -700,12 +839,15 @@ __global__ void __launch_bounds__(256) k_adj_orb_lds(
       for (int pass = 0; pass < 2; ++pass) {
         for (int q = threadIdx.x; q < nmat * norb * ld; q += blockDim.x) M1[q] = 0.0;
         __syncthreads();
+        const long long nd = G.ndens;
+        const double* gb = Gb + G.off_F;
+        const long long xr = 2LL * (nv + iv * ks + m);
         for (int d = threadIdx.x; d < G.ndens; d += blockDim.x) {
           const DensDev D = dens[G.dens0 + d];
-          const double* g = Gb + D.off_f;
+          const double* g = gb + d;
           double v;
-          if (pass == 0) v = g[iv] * wv[0] + (g[nv + iv] + g[2 * nv + iv * ks + m]) * wv[1];
-          else v = g[2 * nv + nk + iv * ks + m];
+          if (pass == 0) v = g[2LL * iv * nd] * wv[0] + (g[(2LL * iv + 1) * nd] + g[xr * nd]) * wv[1];
+          else v = g[(xr + 1) * nd];
           double* M = (mode == 1 && D.kind == DK_QQ) ? M2 : M1;
           if (mode == 2) {
             M[D.a * ld + D.c] = v;
@@ -879,15 +1021,14 @@ struct dbs_engine {
   InstDev* d_inst = nullptr;
   KGroupDev* d_kg = nullptr;
   DensDev* d_dens = nullptr;
-  RowDev* d_rows = nullptr;
   long long* d_row_out = nullptr;
   long long* d_seg_orb = nullptr;   // per instance start in flat orbital-point index
   long long* d_seg_coef = nullptr;  // per instance start in flat coefficient index
   long long* d_seg_w = nullptr;     // per group start in flat weight index
   long long* d_roff = nullptr;      // per instance row offsets (ninst+1)
-  int2* d_items = nullptr;
-  int* d_adjoff = nullptr;
-  AdjDev* d_adj = nullptr;
+  int2* d_items = nullptr;          // features: (group, interval)
+  int2* d_scan_items = nullptr;     // scans: (group, first column)
+  int nscan_items = 0;
   int* d_oadjoff = nullptr;
   OrbAdjDev* d_oadj = nullptr;
   // dense-contraction tables
@@ -907,17 +1048,25 @@ struct dbs_engine {
   size_t adj_shm = 0;
   long long adj_items = 0;
   long long* d_seg_adj = nullptr;
-  // panel work lists (XCD-ordered) for gather and contraction
-  PanelItem* d_gitems = nullptr;
-  int ngitems = 0;
-  PanelItem* d_citems = nullptr;
-  int ncitems = 0;
-  int* d_cntile = nullptr;   // panels per sparse row
-  longlong2* d_croff = nullptr;  // feature offsets (L,R) per sparse row
-  long long* d_doff = nullptr;   // feature offset per density
-  AdjPre* d_adjpre = nullptr;
-  double* d_cpart = nullptr;
-  int cmaxtile = 1;
+  // pair panels: adjacency (64-wide column blocks), work lists, partials
+  AdjChunk* d_chunks = nullptr;
+  AdjEnt* d_ent = nullptr;
+  long long* d_ecidx = nullptr;
+  double* d_cent = nullptr;
+  long long nent = 0;
+  int* d_alen = nullptr;  // adjacency length per density
+  // work lists [adjoint][variant]: 0 partner panels read from global
+  // memory, 1 LDS panels, 2 LDS panels with the conflict-free rotation
+  PairWork* d_work[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+  int nwork[2][3] = {{0, 0, 0}, {0, 0, 0}};
+  int pair_w = 16;        // panel width (pair slots) of this launch shape
+  size_t pair_shm = 0;    // LDS bytes of the LDS pair-panel kernel
+  size_t rot_shm = 0;     // LDS bytes of the rotated kernel
+  int rot_block = 64;     // its workgroup size (>= ndens of its groups)
+  size_t lds_block = 0;   // device limit of LDS per workgroup (bytes)
+  RowRed* d_red = nullptr;
+  int nred = 0;
+  double* d_part = nullptr;
   int ntiles = 0;
   // sizes
   int nkg = 0, ndens = 0, nitems = 0;
@@ -928,13 +1077,15 @@ struct dbs_engine {
   std::vector<RowDev> h_rows;
   std::vector<long long> h_row_out;
   std::vector<InstDev> h_inst;
-  // rows per group (plain path)
+  // adjacency of each density (host): partner (global), row (global)
+  std::vector<std::vector<int2>> h_adj;
+  // rows per group (group order)
   std::vector<std::vector<int>> group_rows;
-  // tuning: [0] contraction path (0 sparse, 1 dense MFMA, 2 auto),
-  //         [1] features block size, [2] rows per block (waves),
-  //         [3] persistent grid cap (workgroups), [4] adjoint gather block
-  //         [5] panel budget in KiB (features of one panel, per XCD L2)
-  int tune[6] = {2, 128, 4, 2048, 256, 2048};
+  // tuning: [0] contraction path (0 pair panels, 1 dense MFMA, 2 auto),
+  //         [1] features block size, [2] pair-panel width W (4, 8, 16),
+  //         [3] persistent grid cap (workgroups), [4] pair-panel block size,
+  //         [5] W-panels per pair-panel workgroup
+  int tune[6] = {2, 256, 16, 4096, 512, 1};
   std::vector<char> group_dense;  // per group: use the dense path
   double flops_fwd = 0, flops_adj = 0, bytes_min = 0;
   // graph
@@ -962,6 +1113,12 @@ int upload(hipStream_t s, T** dst, const std::vector<T>& v) {
   DBS_HIP(hipMalloc(dst, n * sizeof(T)));
   if (!v.empty()) DBS_HIP(hipMemcpyAsync(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
   return DBS_OK;
+}
+
+template <class T>
+void dfree(T** p) {
+  if (*p) hipFree(*p);
+  *p = nullptr;
 }
 
 unsigned grid_for(long long n, int block, int cap) {
@@ -1007,11 +1164,10 @@ int build_dense_tables(dbs_engine* e) {
     }
   }
   e->ntiles = (int)tiles.size();
-  if (e->d_tiles) hipFree(e->d_tiles);
-  if (e->d_pairrow) hipFree(e->d_pairrow);
-  if (e->d_pairoff) hipFree(e->d_pairoff);
-  if (e->d_nextrow) hipFree(e->d_nextrow);
-  e->d_tiles = nullptr; e->d_pairrow = nullptr; e->d_pairoff = nullptr; e->d_nextrow = nullptr;
+  dfree(&e->d_tiles);
+  dfree(&e->d_pairrow);
+  dfree(&e->d_pairoff);
+  dfree(&e->d_nextrow);
   int rc = upload(e->stream, &e->d_tiles, tiles);
   if (!rc) rc = upload(e->stream, &e->d_pairrow, pairrow);
   if (!rc) rc = upload(e->stream, &e->d_pairoff, pairoff);
@@ -1021,138 +1177,182 @@ int build_dense_tables(dbs_engine* e) {
 
 }  // namespace
 
-// sparse rows of groups on the sparse path, in group order
-static std::vector<RowDev> sparse_rows(const dbs_engine* e, std::vector<long long>* outmap) {
-  std::vector<RowDev> v;
-  outmap->clear();
+template <int W>
+static void rot_lds_attr(size_t bytes) {
+  hipFuncSetAttribute((const void*)k_pair_rot<W, false, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                      (int)bytes);
+  hipFuncSetAttribute((const void*)k_pair_rot<W, false, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                      (int)bytes);
+  hipFuncSetAttribute((const void*)k_pair_rot<W, true, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                      (int)bytes);
+  hipFuncSetAttribute((const void*)k_pair_rot<W, true, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                      (int)bytes);
+}
+
+static size_t rot_lds_bytes(int W, int ndens) {
+  return (size_t)2 * W * (2 * (size_t)ndens + 1) * sizeof(double);
+}
+
+template <int W>
+static void pair_lds_attr(size_t bytes) {
+  hipFuncSetAttribute((const void*)k_pair_panel<W, true, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                      (int)bytes);
+  hipFuncSetAttribute((const void*)k_pair_panel<W, true, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                      (int)bytes);
+}
+
+static size_t pair_lds_bytes(int W, int ndens) { return (size_t)ndens * (2 * W + 2) * sizeof(double); }
+
+// Pair-panel tables for the current tuning: adjacency blocks (the slot
+// code depends on which groups are dense), work lists, partial layout and
+// the sparse-row reduction list.
+static int build_pair_tables(dbs_engine* e) {
+  int W = e->tune[2];
+  const int NP = e->tune[5];
+  if (e->lds_block == 0) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, e->device) != hipSuccess || v <= 0)
+      v = 64 * 1024;
+    e->lds_block = (size_t)v;
+  }
+  const size_t kPairLdsMax = e->lds_block;
+  // largest width <= tuning whose panels fit LDS for the groups (groups that
+  // do not fit even at width 4 use the global-memory variant)
+  int maxnd = 0;
+  for (int g = 0; g < e->nkg; ++g)
+    if (pair_lds_bytes(4, e->h_kg[g].ndens) <= kPairLdsMax) maxnd = std::max(maxnd, e->h_kg[g].ndens);
+  while (W > 4 && pair_lds_bytes(W, maxnd) > kPairLdsMax) W /= 2;
+  e->pair_w = W;
+  e->pair_shm = pair_lds_bytes(W, std::max(maxnd, 1));
+  if (e->pair_shm > 64 * 1024) {
+    if (W == 4) pair_lds_attr<4>(e->pair_shm);
+    else if (W == 8) pair_lds_attr<8>(e->pair_shm);
+    else pair_lds_attr<16>(e->pair_shm);
+  }
+  // adjacency in 64-wide column blocks
+  std::vector<AdjChunk> chunks;
+  std::vector<AdjEnt> ent;
+  std::vector<long long> ecidx;
+  long long base = 0;
   for (int g = 0; g < e->nkg; ++g) {
-    if (e->group_dense[g]) continue;
-    std::vector<int> rs(e->group_rows[g]);
-    std::stable_sort(rs.begin(), rs.end(), [&](int x, int y) {
-      const RowDev &A = e->h_rows[x], &B = e->h_rows[y];
-      return A.L != B.L ? A.L < B.L : A.R < B.R;
-    });
-    for (int r : rs) {
-      v.push_back(e->h_rows[r]);
-      outmap->push_back(e->h_row_out[r]);
+    KGroupDev& G = e->h_kg[g];
+    G.chunk0 = (int)chunks.size();
+    G.nchunk = (G.ndens + 63) / 64;
+    for (int c = 0; c < G.nchunk; ++c) {
+      int len = 0;
+      for (int t = 0; t < 64 && 64 * c + t < G.ndens; ++t)
+        len = std::max(len, (int)e->h_adj[G.dens0 + 64 * c + t].size());
+      AdjChunk C;
+      C.base = base;
+      C.len = len;
+      C.pad = 0;
+      chunks.push_back(C);
+      ent.resize((size_t)(base + 64LL * len), AdjEnt{-1, -1});
+      ecidx.resize((size_t)(base + 64LL * len), -1);
+      for (int t = 0; t < 64 && 64 * c + t < G.ndens; ++t) {
+        const int X = G.dens0 + 64 * c + t;
+        const auto& lst = e->h_adj[X];
+        for (size_t i = 0; i < lst.size(); ++i) {
+          const int r = lst[i].y;
+          const RowDev& R = e->h_rows[r];
+          const bool left = lst[i].x >= 0;
+          const int partner = left ? R.R : R.L;
+          AdjEnt A;
+          A.P = partner - G.dens0;
+          A.slot = left ? (e->group_dense[g] ? -2 : r - G.row0) : -1;
+          ent[(size_t)(base + 64LL * (long long)i + t)] = A;
+          ecidx[(size_t)(base + 64LL * (long long)i + t)] = e->h_row_out[r];
+        }
+      }
+      base += 64LL * len;
     }
   }
-  return v;
-}
-
-struct SparseCache {
-  RowDev* rows = nullptr;
-  long long* out = nullptr;
-  int n = 0;
-};
-static std::map<const dbs_engine*, SparseCache> g_sparse;
-
-static int panel_width(const dbs_engine* e, int ndens, int D) {
-  const double budget = (double)e->tune[5] * 1024.0;
-  long long w = (long long)(budget / (8.0 * std::max(ndens, 1)));
-  w = (w / 64) * 64;
-  if (w < 64) w = 64;
-  if (w > 2048) w = 2048;  // 8 features per thread of a 256-thread item
-  if (w > D) w = D;
-  return (int)w;
-}
-
-// interleave per-XCD lists so that item i lands on XCD i % 8
-static std::vector<PanelItem> xcd_interleave(const std::vector<std::vector<PanelItem>>& lists) {
-  size_t mx = 0;
-  for (auto& l : lists) mx = std::max(mx, l.size());
-  PanelItem pad;
-  std::memset(&pad, 0, sizeof(pad));
-  pad.d = -1;
-  std::vector<PanelItem> out(mx * lists.size(), pad);
-  for (size_t x = 0; x < lists.size(); ++x)
-    for (size_t j = 0; j < lists[x].size(); ++j) out[j * lists.size() + x] = lists[x][j];
-  return out;
-}
-
-static int build_gather_panels(dbs_engine* e) {
-  std::vector<std::vector<PanelItem>> lists(8);
-  int panel = 0;
+  e->nent = base;
+  // work lists and partial layout
+  std::vector<PairWork> work[2][3];
+  std::vector<RowRed> red;
+  long long poff = 0;
+  // groups for the rotated kernel: one density per thread
+  const int rot_max = W >= 16 ? 256 : 512;
+  int rot_nd = 0;
   for (int g = 0; g < e->nkg; ++g) {
-    const KGroupDev& G = e->h_kg[g];
+    const int nd = e->h_kg[g].ndens;
+    if (W >= 8 && nd <= rot_max && rot_lds_bytes(W, nd) <= kPairLdsMax) rot_nd = std::max(rot_nd, nd);
+  }
+  e->rot_block = std::max(64, (rot_nd + 63) / 64 * 64);
+  e->rot_shm = rot_lds_bytes(W, std::max(rot_nd, 1));
+  if (e->rot_shm > 64 * 1024) {
+    if (W == 8) rot_lds_attr<8>(e->rot_shm);
+    else if (W == 16) rot_lds_attr<16>(e->rot_shm);
+  }
+  for (int g = 0; g < e->nkg; ++g) {
+    KGroupDev& G = e->h_kg[g];
     const InstDev& I = e->h_inst[G.inst];
-    const int D = feat_dim(I.nv, I.ks);
-    const int W = panel_width(e, G.ndens, D);
-    for (int x0 = 0; x0 < D; x0 += W, ++panel) {
-      for (int d = 0; d < G.ndens; ++d) {
-        PanelItem it;
-        std::memset(&it, 0, sizeof(it));
-        it.d = G.dens0 + d; it.n = 1; it.x0 = x0; it.x1 = std::min(D, x0 + W);
-        it.nv = I.nv; it.nk = I.nv * I.ks;
-        lists[panel % 8].push_back(it);
-      }
+    const int J = I.nv * (1 + I.ks);
+    const int span = W * NP;
+    const int npr = (J + span - 1) / span;
+    int var = pair_lds_bytes(W, G.ndens) <= kPairLdsMax ? 1 : 0;
+    if (W >= 8 && G.ndens <= rot_nd) var = 2;
+    G.npr = e->group_dense[g] ? 0 : npr;
+    G.dense = e->group_dense[g];
+    G.off_part = poff;
+    if (!G.dense) poff += (long long)npr * G.nrows;
+    for (int p = 0; p < npr; ++p) {
+      PairWork Wk;
+      Wk.g = g;
+      Wk.j0 = p * span;
+      Wk.j1 = std::min(J, (p + 1) * span);
+      Wk.pr = p;
+      if (G.nrows == 0) continue;
+      work[1][var].push_back(Wk);
+      if (!G.dense) work[0][var].push_back(Wk);
     }
-  }
-  std::vector<PanelItem> items = xcd_interleave(lists);
-  if (e->d_gitems) hipFree(e->d_gitems);
-  e->d_gitems = nullptr;
-  e->ngitems = (int)items.size();
-  return upload(e->stream, &e->d_gitems, items);
-}
-
-static int refresh_sparse(dbs_engine* e) {
-  SparseCache& sc = g_sparse[e];
-  if (sc.rows) hipFree(sc.rows);
-  if (sc.out) hipFree(sc.out);
-  sc.rows = nullptr; sc.out = nullptr;
-  std::vector<long long> om;
-  std::vector<RowDev> rv = sparse_rows(e, &om);
-  sc.n = (int)rv.size();
-  int rc = upload(e->stream, &sc.rows, rv);
-  if (!rc) rc = upload(e->stream, &sc.out, om);
-  if (rc) return rc;
-  // contraction panels over the sparse rows (row chunks per group panel)
-  const int RCH = 16;
-  std::vector<std::vector<PanelItem>> lists(8);
-  std::vector<int> ntile(std::max(sc.n, 1), 0);
-  int panel = 0, maxtile = 1;
-  for (int r0 = 0; r0 < sc.n;) {
-    const int g = rv[r0].kg;
-    int r1 = r0;
-    while (r1 < sc.n && rv[r1].kg == g) ++r1;
-    const KGroupDev& G = e->h_kg[g];
-    const InstDev& I = e->h_inst[G.inst];
-    const int D = feat_dim(I.nv, I.ks);
-    const int W = panel_width(e, G.ndens, D);
-    int t = 0;
-    for (int x0 = 0; x0 < D; x0 += W, ++t, ++panel)
-      for (int a = r0; a < r1; a += RCH) {
-        PanelItem it;
-        std::memset(&it, 0, sizeof(it));
-        it.d = a; it.n = std::min(RCH, r1 - a); it.x0 = x0; it.x1 = std::min(D, x0 + W); it.tile = t;
-        it.nv = I.nv; it.nk = I.nv * I.ks;
-        lists[panel % 8].push_back(it);
+    if (!G.dense)
+      for (int q = 0; q < G.nrows; ++q) {
+        RowRed R;
+        R.pbase = G.off_part + q;
+        R.out = e->h_row_out[G.row0 + q];
+        R.stride = G.nrows;
+        R.npr = npr;
+        red.push_back(R);
       }
-    for (int r = r0; r < r1; ++r) ntile[r] = t;
-    maxtile = std::max(maxtile, t);
-    r0 = r1;
   }
-  std::vector<PanelItem> items = xcd_interleave(lists);
-  if (e->d_citems) hipFree(e->d_citems);
-  if (e->d_cntile) hipFree(e->d_cntile);
-  if (e->d_cpart) hipFree(e->d_cpart);
-  e->d_citems = nullptr; e->d_cntile = nullptr; e->d_cpart = nullptr;
-  e->ncitems = (int)items.size();
-  e->cmaxtile = maxtile;
-  std::vector<longlong2> croff(std::max(sc.n, 1));
-  for (int r = 0; r < sc.n; ++r) {
-    croff[r].x = e->h_dens[rv[r].L].off_f;
-    croff[r].y = e->h_dens[rv[r].R].off_f;
+  // longest groups first so the tail of the launch is short
+  for (int a = 0; a < 2; ++a)
+    for (int l = 0; l < 3; ++l)
+      std::stable_sort(work[a][l].begin(), work[a][l].end(), [&](const PairWork& x, const PairWork& y) {
+        return e->h_kg[x.g].nrows > e->h_kg[y.g].nrows;
+      });
+  dfree(&e->d_chunks);
+  dfree(&e->d_ent);
+  dfree(&e->d_ecidx);
+  dfree(&e->d_cent);
+  dfree(&e->d_red);
+  dfree(&e->d_part);
+  dfree(&e->d_kg);
+  for (int a = 0; a < 2; ++a)
+    for (int l = 0; l < 3; ++l) {
+      dfree(&e->d_work[a][l]);
+      e->nwork[a][l] = (int)work[a][l].size();
+    }
+  e->nred = (int)red.size();
+  int rc = upload(e->stream, &e->d_chunks, chunks);
+  if (!rc) rc = upload(e->stream, &e->d_ent, ent);
+  if (!rc) rc = upload(e->stream, &e->d_ecidx, ecidx);
+  if (!rc) rc = upload(e->stream, &e->d_red, red);
+  if (!rc) rc = upload(e->stream, &e->d_kg, e->h_kg);
+  for (int a = 0; a < 2 && !rc; ++a)
+    for (int l = 0; l < 3 && !rc; ++l) rc = upload(e->stream, &e->d_work[a][l], work[a][l]);
+  {
+    std::vector<int> alen(std::max(e->ndens, 1), 0);
+    for (int d = 0; d < e->ndens; ++d) alen[d] = (int)e->h_adj[d].size();
+    dfree(&e->d_alen);
+    if (!rc) rc = upload(e->stream, &e->d_alen, alen);
   }
-  if (e->d_croff) hipFree(e->d_croff);
-  e->d_croff = nullptr;
-  rc = upload(e->stream, &e->d_citems, items);
-  if (!rc) rc = upload(e->stream, &e->d_croff, croff);
-  if (!rc) rc = upload(e->stream, &e->d_cntile, ntile);
-  if (!rc) {
-    hipError_t er = hipMalloc(&e->d_cpart, sizeof(double) * (size_t)std::max(sc.n, 1) * maxtile);
-    if (er != hipSuccess) rc = fail(DBS_ERR_HIP, "contraction partials: hipMalloc");
-  }
+  if (!rc && hipMalloc(&e->d_cent, sizeof(double) * (size_t)std::max<long long>(e->nent, 1)) != hipSuccess)
+    rc = fail(DBS_ERR_HIP, "pair tables: hipMalloc");
+  if (!rc && hipMalloc(&e->d_part, sizeof(double) * (size_t)std::max<long long>(poff, 1)) != hipSuccess)
+    rc = fail(DBS_ERR_HIP, "pair tables: hipMalloc");
   return rc;
 }
 
@@ -1169,7 +1369,69 @@ static void mark(dbs_engine* e, const char* name) {
   e->prof_pos++;
 }
 
-static int run_forward(dbs_engine* e) {
+template <int W>
+static void launch_pair_w(dbs_engine* e, int adj, int var, hipStream_t s) {
+  const int n = e->nwork[adj][var];
+  if (n == 0) return;
+  const unsigned grid = (unsigned)std::min(n, e->tune[3] * 4);
+  const PairWork* wk = e->d_work[adj][var];
+  if (var == 2) {
+    if constexpr (W >= 8) {
+      const int b = e->rot_block;
+      const size_t shm = e->rot_shm;
+      if (e->onesided) {
+        if (adj)
+          k_pair_rot<W, true, true><<<grid, b, shm, s>>>(n, wk, e->d_kg, e->d_inst, e->d_chunks, e->d_ent,
+                                                         e->d_alen, e->d_cent, e->F, e->G, e->d_part);
+        else
+          k_pair_rot<W, true, false><<<grid, b, shm, s>>>(n, wk, e->d_kg, e->d_inst, e->d_chunks, e->d_ent,
+                                                          e->d_alen, e->d_cent, e->F, e->G, e->d_part);
+      } else {
+        if (adj)
+          k_pair_rot<W, false, true><<<grid, b, shm, s>>>(n, wk, e->d_kg, e->d_inst, e->d_chunks, e->d_ent,
+                                                          e->d_alen, e->d_cent, e->F, e->G, e->d_part);
+        else
+          k_pair_rot<W, false, false><<<grid, b, shm, s>>>(n, wk, e->d_kg, e->d_inst, e->d_chunks, e->d_ent,
+                                                           e->d_alen, e->d_cent, e->F, e->G, e->d_part);
+      }
+    }
+    return;
+  }
+  const int block = std::min(e->tune[4], pair_block_max<W>());
+  const size_t shm = var == 1 ? e->pair_shm : 0;
+  if (adj) {
+    if (var == 1)
+      k_pair_panel<W, true, true><<<grid, block, shm, s>>>(n, wk, e->d_kg, e->d_inst, e->d_chunks, e->d_ent,
+                                                           e->d_alen, e->d_cent, e->F, e->G, e->d_part,
+                                                           e->onesided);
+    else
+      k_pair_panel<W, false, true><<<grid, block, 0, s>>>(n, wk, e->d_kg, e->d_inst, e->d_chunks, e->d_ent,
+                                                          e->d_alen, e->d_cent, e->F, e->G, e->d_part,
+                                                          e->onesided);
+  } else {
+    if (var == 1)
+      k_pair_panel<W, true, false><<<grid, block, shm, s>>>(n, wk, e->d_kg, e->d_inst, e->d_chunks, e->d_ent,
+                                                            e->d_alen, e->d_cent, e->F, e->G, e->d_part,
+                                                            e->onesided);
+    else
+      k_pair_panel<W, false, false><<<grid, block, 0, s>>>(n, wk, e->d_kg, e->d_inst, e->d_chunks, e->d_ent,
+                                                           e->d_alen, e->d_cent, e->F, e->G, e->d_part,
+                                                           e->onesided);
+  }
+}
+
+static void launch_pair(dbs_engine* e, int adj, hipStream_t s) {
+  for (int var = 2; var >= 0; --var) {
+    switch (e->pair_w) {
+      case 4: launch_pair_w<4>(e, adj, var, s); break;
+      case 8: launch_pair_w<8>(e, adj, var, s); break;
+      default: launch_pair_w<16>(e, adj, var, s); break;
+    }
+  }
+}
+
+// the forward (with_adj: fused with the gather of the adjoint)
+static int run_forward(dbs_engine* e, int with_adj) {
   hipStream_t s = e->stream;
   mark(e, "start");
   const int cap = e->tune[3];
@@ -1178,25 +1440,29 @@ static int run_forward(dbs_engine* e) {
   mark(e, "orb_points");
   k_weights<<<grid_for(e->total_w, 256, cap * 4), 256, 0, s>>>(e->nkg, e->d_seg_w, e->d_kg, e->d_inst,
                                                                 e->wt, e->total_w);
+  if (with_adj && e->nent > 0)
+    k_adj_cgather<<<grid_for(e->nent, 256, cap * 4), 256, 0, s>>>(e->nent, e->d_ecidx, e->coef, e->d_cent);
   mark(e, "weights");
   k_features<<<grid_for(e->nitems, 1, cap), e->tune[1], e->feat_shm, s>>>(
       e->nitems, e->d_items, e->d_kg, e->d_inst, e->d_dens, e->orbv, e->wt, e->F, e->feat_stage);
   mark(e, "features");
-  k_scan<<<grid_for(e->ndens, 4, cap), 256, 0, s>>>(e->ndens, e->d_dens, e->d_kg, e->d_inst, e->F, 0);
+  k_scan_cols<<<grid_for(e->nscan_items, 1, cap), 256, 0, s>>>(e->nscan_items, e->d_scan_items, e->d_kg,
+                                                               e->d_inst, e->F, 0);
   mark(e, "prefix_scan");
-  SparseCache& sc = g_sparse[e];
-  if (sc.n > 0) {
-    const int bw = e->tune[2] * 64;
-    k_contract_panel<<<(unsigned)std::max(e->ncitems, 1), bw, 0, s>>>(
-        e->ncitems, e->d_citems, e->d_croff, e->F, e->onesided, e->cmaxtile, e->d_cpart);
-    k_contract_reduce<<<grid_for(sc.n, 256, cap), 256, 0, s>>>(sc.n, e->d_cntile, e->cmaxtile, e->d_cpart,
-                                                               sc.out, e->vals);
-    mark(e, "contract_rows");
+  const int wa = with_adj ? 1 : 0;
+  if (e->nwork[wa][0] + e->nwork[wa][1] + e->nwork[wa][2] > 0) {
+    launch_pair(e, with_adj ? 1 : 0, s);
+    mark(e, with_adj ? "pair_panel" : "contract_rows");
+  }
+  if (e->nred > 0) {
+    k_rows_reduce<<<grid_for((long long)e->nred * DBS_RED_LANES, 256, cap * 4), 256, 0, s>>>(
+        e->nred, e->d_red, e->d_part, e->vals);
+    mark(e, "rows_reduce");
   }
   if (e->ntiles > 0) {
     k_contract_dense<<<grid_for(e->ntiles, 4, cap * 4), 256, 0, s>>>(
-        e->ntiles, e->d_tiles, e->d_kg, e->d_inst, e->d_dens, e->F, e->d_pairrow, e->d_pairoff,
-        e->d_nextrow, e->d_row_out, e->onesided, e->vals);
+        e->ntiles, e->d_tiles, e->d_kg, e->d_inst, e->F, e->d_pairrow, e->d_pairoff, e->d_nextrow,
+        e->d_row_out, e->onesided, e->vals);
     mark(e, "contract_dense");
   }
   if (e->nechunks > 0)
@@ -1208,13 +1474,12 @@ static int run_forward(dbs_engine* e) {
   return DBS_OK;
 }
 
+// the rest of the adjoint after run_forward(e, 1)
 static int run_adjoint(dbs_engine* e) {
   hipStream_t s = e->stream;
   const int cap = e->tune[3];
-  k_adj_gather_panel<<<(unsigned)std::max(e->ngitems, 1), 256, 0, s>>>(
-      e->ngitems, e->d_gitems, e->d_doff, e->d_adjoff, e->d_adjpre, e->coef, e->F, e->onesided, e->G);
-  mark(e, "adj_gather");
-  k_scan<<<grid_for(e->ndens, 4, cap), 256, 0, s>>>(e->ndens, e->d_dens, e->d_kg, e->d_inst, e->G, 1);
+  k_scan_cols<<<grid_for(e->nscan_items, 1, cap), 256, 0, s>>>(e->nscan_items, e->d_scan_items, e->d_kg,
+                                                               e->d_inst, e->G, 1);
   mark(e, "adj_suffix_scan");
   if (e->adj_shm > 0) {
     k_adj_orb_lds<<<grid_for(e->adj_items, 1, cap * 4), 256, e->adj_shm, s>>>(
@@ -1233,44 +1498,60 @@ static int run_adjoint(dbs_engine* e) {
   return DBS_OK;
 }
 
+static int run_all(dbs_engine* e, int with_adj) {
+  int rc = run_forward(e, with_adj);
+  if (!rc && with_adj) rc = run_adjoint(e);
+  return rc;
+}
+
 static void compute_phase_work(dbs_engine* e) {
   const int ninst = e->ninst;
   const std::vector<int>& norb = e->norb;
-  // per phase (one launch): flops and minimum HBM bytes
-    double pts = 0, pts_fl = 0, pq_b = 0, feat_fl = 0, feat_b = 0, scan_fl = 0, scan_b = 0;
-    double rows_fl = 0, rows_b = 0, rows_dense_fl = 0, gath_fl = 0, orbp_fl = 0, coef_fl = 0;
-    for (int i = 0; i < ninst; ++i) {
-      const InstDev& I = e->h_inst[i];
-      const double p = (double)norb[i] * 2 * I.nv * I.np;
-      pts += p;
-      pts_fl += p * 2 * I.ks;
-      pq_b += (double)norb[i] * 2 * I.ns * 8;
-      coef_fl += (double)norb[i] * 2 * I.ns * 2.0 * I.ks * I.np;
+  // per phase (one launch): algorithmic flops and bytes (each input read
+  // once, each output written once)
+  double pts = 0, pts_fl = 0, pq_b = 0, feat_fl = 0, feat_b = 0, scan_fl = 0, scan_b = 0;
+  double rows_fl = 0, rows_b = 0, rows_dense_fl = 0, gath_fl = 0, orbp_fl = 0, coef_fl = 0, red_b = 0;
+  for (int i = 0; i < ninst; ++i) {
+    const InstDev& I = e->h_inst[i];
+    const double p = (double)norb[i] * 2 * I.nv * I.np;
+    pts += p;
+    pts_fl += p * 2 * I.ks;
+    pq_b += (double)norb[i] * 2 * I.ns * 8;
+    coef_fl += (double)norb[i] * 2 * I.ns * 2.0 * I.ks * I.np;
+  }
+  for (int g = 0; g < e->nkg; ++g) {
+    const KGroupDev& G = e->h_kg[g];
+    const InstDev& I = e->h_inst[G.inst];
+    const double nv = I.nv, ks = I.ks, nd = G.ndens, nr = (double)e->group_rows[g].size();
+    const double D = feat_dim(I.nv, I.ks);
+    const double rho = e->kind == 0 ? 3.0 : 1.0;  // flops per density point value
+    feat_fl += nd * nv * (I.np * rho + 4 * ks + ks + 2 * ks * ks);
+    feat_b += nd * D * 8;
+    scan_fl += nd * nv;
+    scan_b += nd * nv * 16;
+    const double prow = (e->onesided ? 1.0 : 2.0) * D;  // flops of one row value
+    if (e->group_dense[g]) {
+      rows_dense_fl += 2.0 * nd * nd * D;
+    } else {
+      rows_fl += nr * prow;
+      rows_b += nd * D * 8;
+      red_b += nr * 8 * (1.0 + G.npr);
     }
-    for (int g = 0; g < e->nkg; ++g) {
-      const InstDev& I = e->h_inst[e->h_kg[g].inst];
-      const double nv = I.nv, ks = I.ks, nd = e->h_kg[g].ndens, nr = (double)e->group_rows[g].size();
-      const double D = feat_dim(I.nv, I.ks);
-      const double rho = e->kind == 0 ? 3.0 : 1.0;  // flops per density point value
-      feat_fl += nd * nv * (I.np * rho + 4 * ks + ks + 2 * ks * ks);
-      feat_b += nd * D * 8;
-      scan_fl += nd * nv;
-      scan_b += nd * nv * 16;
-      const double prow = (e->onesided ? 2.0 : 4.0) * nv * (1 + ks);
-      if (e->group_dense[g]) rows_dense_fl += 2.0 * nd * nd * D;
-      else { rows_fl += nr * prow; rows_b += nd * D * 8 + nr * 8; }
-      gath_fl += 2.0 * nr * 2.0 * D;
-      orbp_fl += 2.0 * nd * nv * I.np * 3.0;
-    }
-    e->phase_work["orb_points"] = {pts_fl, pq_b + pts * 8};
-    e->phase_work["features"] = {feat_fl, feat_b};
-    e->phase_work["prefix_scan"] = {scan_fl, scan_b};
-    e->phase_work["contract_rows"] = {rows_fl, rows_b};
-    e->phase_work["contract_dense"] = {rows_dense_fl, 0.0};
-    e->phase_work["adj_gather"] = {gath_fl, feat_b * 2};
-    e->phase_work["adj_suffix_scan"] = {scan_fl, scan_b};
-    e->phase_work["adj_orb_points"] = {orbp_fl, pts * 8 * 2};
-    e->phase_work["adj_coef"] = {coef_fl, pts * 8 + pq_b};
+    gath_fl += 2.0 * nr * (e->onesided ? 1.0 : 2.0) * D;
+    orbp_fl += 2.0 * nd * nv * I.np * 3.0;
+  }
+  const double coef_b = (double)e->total_rows * 8;
+  e->phase_work["orb_points"] = {pts_fl, pq_b + pts * 8};
+  e->phase_work["features"] = {feat_fl, pts * 8 + feat_b};
+  e->phase_work["prefix_scan"] = {scan_fl, scan_b};
+  // pair panels: features read once, (adjoint) gradients written once
+  e->phase_work["contract_rows"] = {rows_fl, rows_b};
+  e->phase_work["pair_panel"] = {rows_fl + gath_fl, feat_b * 2 + coef_b};
+  e->phase_work["rows_reduce"] = {0.0, red_b};
+  e->phase_work["contract_dense"] = {rows_dense_fl, 0.0};
+  e->phase_work["adj_suffix_scan"] = {scan_fl, scan_b};
+  e->phase_work["adj_orb_points"] = {orbp_fl, feat_b + pts * 8 * 2};
+  e->phase_work["adj_coef"] = {coef_fl, pts * 8 + pq_b};
 }
 
 extern "C" {
@@ -1342,8 +1623,6 @@ int dbs_engine_create(dbs_engine** out, int ninst, dbs_ctx* const* ctxs, const i
   const int* rp = rows;
   long long foff = 0, woff = 0;
   std::vector<long long> seg_w;
-  std::vector<AdjDev> adj_all;
-  std::vector<std::vector<AdjDev>> adj_per_dens;
   std::vector<std::vector<OrbAdjDev>> oadj((size_t)orb_base[ninst]);
   e->h_row_out.resize((size_t)pr);
   for (int i = 0; i < ninst; ++i) {
@@ -1376,8 +1655,6 @@ int dbs_engine_create(dbs_engine** out, int ninst, dbs_ctx* const* ctxs, const i
         DensDev dd;
         dd.kg = e->nkg;
         dd.a = a; dd.c = c; dd.kind = dk;
-        dd.off_f = foff;
-        foff += D;
         int id = (int)e->h_dens.size();
         e->h_dens.push_back(dd);
         dmap[key] = id;
@@ -1409,6 +1686,11 @@ int dbs_engine_create(dbs_engine** out, int ninst, dbs_ctx* const* ctxs, const i
         grows.push_back(gid);
       }
       G.ndens = (int)e->h_dens.size() - G.dens0;
+      G.off_F = foff;
+      foff += (long long)D * G.ndens;
+      G.nrows = (int)grows.size();
+      G.row0 = grows.empty() ? (int)e->h_rows.size() : grows[0];
+      G.off_part = 0; G.npr = 0; G.dense = 0; G.chunk0 = 0; G.nchunk = 0;
       seg_w.push_back(woff);
       woff += 2LL * I.nv * I.ks + (long long)I.nv * I.ks * I.ks;
       e->h_kg.push_back(G);
@@ -1431,17 +1713,18 @@ int dbs_engine_create(dbs_engine** out, int ninst, dbs_ctx* const* ctxs, const i
   for (int g = 0; g < e->nkg; ++g)
     for (int iv = 0; iv < e->h_inst[e->h_kg[g].inst].nv; ++iv) items.push_back(make_int2(g, iv));
   e->nitems = (int)items.size();
-  // adjacency per density (row order)
-  adj_per_dens.resize(e->ndens);
+  // scan items (group, first of 32 columns)
+  std::vector<int2> scan_items;
+  for (int g = 0; g < e->nkg; ++g)
+    for (int c0 = 0; c0 < e->h_kg[g].ndens; c0 += DBS_SCAN_COLS) scan_items.push_back(make_int2(g, c0));
+  e->nscan_items = (int)scan_items.size();
+  // adjacency per density in row order: (0: left density of the row, -1:
+  // right density, row)
+  e->h_adj.assign(e->ndens, {});
   for (size_t r = 0; r < e->h_rows.size(); ++r) {
     const RowDev& R = e->h_rows[r];
-    adj_per_dens[R.L].push_back({R.R, (int)r, 0, 0});
-    adj_per_dens[R.R].push_back({R.L, (int)r, 1, 0});
-  }
-  std::vector<int> adjoff(e->ndens + 1, 0);
-  for (int d = 0; d < e->ndens; ++d) {
-    adjoff[d + 1] = adjoff[d] + (int)adj_per_dens[d].size();
-    for (auto& a : adj_per_dens[d]) adj_all.push_back(a);
+    e->h_adj[R.L].push_back(make_int2(0, (int)r));
+    e->h_adj[R.R].push_back(make_int2(-1, (int)r));
   }
   // orbital adjacency (global orbital index = orb_base[inst] + orb)
   for (int d = 0; d < e->ndens; ++d) {
@@ -1544,30 +1827,14 @@ int dbs_engine_create(dbs_engine** out, int ninst, dbs_ctx* const* ctxs, const i
   if (rc) return bail(rc);
   std::vector<long long> row_out(e->h_row_out);
   rc |= upload(s, &e->d_inst, e->h_inst);
-  rc |= upload(s, &e->d_kg, e->h_kg);
   rc |= upload(s, &e->d_dens, e->h_dens);
-  rc |= upload(s, &e->d_rows, e->h_rows);
   rc |= upload(s, &e->d_row_out, row_out);
   rc |= upload(s, &e->d_seg_orb, seg_orb);
   rc |= upload(s, &e->d_seg_coef, seg_coef);
   rc |= upload(s, &e->d_seg_w, seg_w);
   rc |= upload(s, &e->d_roff, roff);
   rc |= upload(s, &e->d_items, items);
-  rc |= upload(s, &e->d_adjoff, adjoff);
-  rc |= upload(s, &e->d_adj, adj_all);
-  {
-    std::vector<long long> doff(std::max(e->ndens, 1));
-    for (int d = 0; d < e->ndens; ++d) doff[d] = e->h_dens[d].off_f;
-    std::vector<AdjPre> pre(std::max(adj_all.size(), (size_t)1));
-    for (size_t q = 0; q < adj_all.size(); ++q) {
-      pre[q].off = e->h_dens[adj_all[q].partner].off_f;
-      pre[q].cidx = e->h_row_out[adj_all[q].row];
-      pre[q].role = adj_all[q].role;
-      pre[q].pad = 0;
-    }
-    rc |= upload(s, &e->d_doff, doff);
-    rc |= upload(s, &e->d_adjpre, pre);
-  }
+  rc |= upload(s, &e->d_scan_items, scan_items);
   rc |= upload(s, &e->d_oadjoff, oadjoff);
   rc |= upload(s, &e->d_oadj, oadj_all);
   if (rc) return bail(rc);
@@ -1578,10 +1845,9 @@ int dbs_engine_create(dbs_engine** out, int ninst, dbs_ctx* const* ctxs, const i
   e->group_dense.assign(e->nkg, 0);
   for (int g = 0; g < e->nkg; ++g) e->group_dense[g] = (char)choose_dense(e, e->h_kg[g], e->group_rows[g].size());
   rc = build_dense_tables(e);
-  compute_phase_work(e);
-  if (!rc) rc = refresh_sparse(e);
-  if (!rc) rc = build_gather_panels(e);
+  if (!rc) rc = build_pair_tables(e);
   if (rc) return bail(rc);
+  compute_phase_work(e);
   if (hipStreamSynchronize(s) != hipSuccess) return bail(fail(DBS_ERR_HIP, "dbs_engine_create: setup"));
   *out = e;
   return DBS_OK;
@@ -1595,19 +1861,15 @@ int dbs_engine_destroy(dbs_engine* e) {
   if (e->graph) hipGraphDestroy(e->graph);
   for (hipEvent_t ev : e->prof_ev) hipEventDestroy(ev);
   void* bufs[] = {e->pq, e->coef, e->vals, e->grad, e->energy, e->orbv, e->gorb, e->wt, e->F, e->G,
-                  e->d_inst, e->d_kg, e->d_dens, e->d_rows, e->d_row_out, e->d_seg_orb, e->d_seg_coef,
-                  e->d_seg_w, e->d_roff, e->d_items, e->d_adjoff, e->d_adj, e->d_oadjoff, e->d_oadj,
+                  e->d_inst, e->d_kg, e->d_dens, e->d_row_out, e->d_seg_orb, e->d_seg_coef,
+                  e->d_seg_w, e->d_roff, e->d_items, e->d_scan_items, e->d_oadjoff, e->d_oadj,
                   e->d_tiles, e->d_pairrow, e->d_pairoff, e->d_nextrow, e->plain_d, e->plain_c,
-                  e->d_echunks, e->d_echoff, e->d_epart, e->d_seg_adj, e->d_gitems, e->d_citems,
-                  e->d_cntile, e->d_cpart, e->d_croff, e->d_doff, e->d_adjpre};
+                  e->d_echunks, e->d_echoff, e->d_epart, e->d_seg_adj, e->d_chunks, e->d_ent,
+                  e->d_ecidx, e->d_cent, e->d_red, e->d_part, e->d_alen,
+                  e->d_work[0][0], e->d_work[0][1], e->d_work[0][2],
+                  e->d_work[1][0], e->d_work[1][1], e->d_work[1][2]};
   for (void* p : bufs)
     if (p) hipFree(p);
-  auto it = g_sparse.find(e);
-  if (it != g_sparse.end()) {
-    if (it->second.rows) hipFree(it->second.rows);
-    if (it->second.out) hipFree(it->second.out);
-    g_sparse.erase(it);
-  }
   if (e->stream) hipStreamDestroy(e->stream);
   delete e;
   return DBS_OK;
@@ -1626,14 +1888,12 @@ int dbs_engine_buffers(dbs_engine* e, int inst, double** pq, double** coef, doub
 
 int dbs_engine_forward(dbs_engine* e) {
   DBS_CHECK(e, DBS_ERR_ARG, "dbs_engine_forward: null engine");
-  return run_forward(e);
+  return run_all(e, 0);
 }
 
 int dbs_engine_forward_adjoint(dbs_engine* e) {
   DBS_CHECK(e, DBS_ERR_ARG, "dbs_engine_forward_adjoint: null engine");
-  int rc = run_forward(e);
-  if (rc) return rc;
-  return run_adjoint(e);
+  return run_all(e, 1);
 }
 
 int dbs_engine_sync(dbs_engine* e) {
@@ -1648,8 +1908,7 @@ int dbs_engine_graph_capture(dbs_engine* e, int with_adjoint) {
   if (e->gexec) { hipGraphExecDestroy(e->gexec); e->gexec = nullptr; }
   if (e->graph) { hipGraphDestroy(e->graph); e->graph = nullptr; }
   DBS_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-  int rc = run_forward(e);
-  if (!rc && with_adjoint) rc = run_adjoint(e);
+  int rc = run_all(e, with_adjoint ? 1 : 0);
   hipGraph_t g = nullptr;
   hipError_t ee = hipStreamEndCapture(e->stream, &g);
   if (rc) {
@@ -1674,30 +1933,25 @@ int dbs_engine_set_tuning(dbs_engine* e, int n, const int* p) {
   int t[6];
   std::memcpy(t, e->tune, sizeof(t));
   for (int i = 0; i < n; ++i) t[i] = p[i];
-  DBS_CHECK(t[0] >= 0 && t[0] <= 2, DBS_ERR_ARG, "tuning[0]: path 0 sparse, 1 dense, 2 auto");
+  DBS_CHECK(t[0] >= 0 && t[0] <= 2, DBS_ERR_ARG, "tuning[0]: path 0 pair panels, 1 dense, 2 auto");
   DBS_CHECK(t[1] >= 64 && t[1] <= 256 && t[1] % 64 == 0, DBS_ERR_ARG, "tuning[1]: block size 64..256");
-  DBS_CHECK(t[2] >= 1 && t[2] <= 16, DBS_ERR_ARG, "tuning[2]: waves per block 1..16");
+  DBS_CHECK(t[2] == 4 || t[2] == 8 || t[2] == 16, DBS_ERR_ARG, "tuning[2]: panel width 4, 8 or 16");
   DBS_CHECK(t[3] >= 64 && t[3] <= 65536, DBS_ERR_ARG, "tuning[3]: grid cap");
-  DBS_CHECK(t[4] >= 64 && t[4] <= 256 && t[4] % 64 == 0, DBS_ERR_ARG, "tuning[4]: block size 64..256");
-  DBS_CHECK(t[5] >= 64 && t[5] <= 65536, DBS_ERR_ARG, "tuning[5]: panel budget 64..65536 KiB");
+  DBS_CHECK(t[4] >= 64 && t[4] <= 512 && t[4] % 64 == 0, DBS_ERR_ARG, "tuning[4]: block size 64..512");
+  DBS_CHECK(t[5] >= 1 && t[5] <= 256, DBS_ERR_ARG, "tuning[5]: panels per workgroup 1..256");
+  DBS_HIP(hipStreamSynchronize(e->stream));
   const bool path_changed = t[0] != e->tune[0];
-  const bool panels_changed = t[5] != e->tune[5];
+  const bool pair_changed = path_changed || t[2] != e->tune[2] || t[5] != e->tune[5];
   std::memcpy(e->tune, t, sizeof(t));
   if (path_changed) {
     for (int g = 0; g < e->nkg; ++g) e->group_dense[g] = (char)choose_dense(e, e->h_kg[g], e->group_rows[g].size());
     int rc = build_dense_tables(e);
-    if (!rc) rc = refresh_sparse(e);
+    if (rc) return rc;
+  }
+  if (pair_changed) {
+    int rc = build_pair_tables(e);
     if (rc) return rc;
     compute_phase_work(e);
-    DBS_HIP(hipStreamSynchronize(e->stream));
-  }
-  if (panels_changed && !path_changed) {
-    int rc = refresh_sparse(e);
-    if (rc) return rc;
-  }
-  if (panels_changed) {
-    int rc = build_gather_panels(e);
-    if (rc) return rc;
     DBS_HIP(hipStreamSynchronize(e->stream));
   }
   // a captured graph holds the old launch shapes
@@ -1725,11 +1979,20 @@ int dbs_engine_work(dbs_engine* e, int with_adjoint, double* flops, double* byte
 int dbs_engine_autotune(dbs_engine* e, int with_adjoint, int reps) {
   DBS_CHECK(e, DBS_ERR_ARG, "dbs_engine_autotune: null engine");
   if (reps < 1) reps = 3;
-  const int paths[] = {0, 1};
+  // the dense path only where some group is dense under "auto"
+  bool any_dense = false;
+  {
+    int save = e->tune[0];
+    e->tune[0] = 2;
+    for (int g = 0; g < e->nkg; ++g) any_dense |= choose_dense(e, e->h_kg[g], e->group_rows[g].size()) != 0;
+    e->tune[0] = save;
+  }
+  std::vector<int> paths = {0};
+  if (any_dense) paths.push_back(2);
   const int fblocks[] = {128, 256};
-  const int waves[] = {4};
-  const int caps[] = {4096};
-  const int budgets[] = {1024, 2048, 4096};
+  const int widths[] = {8, 16};
+  const int pblocks[] = {256, 512};
+  const int nps[] = {1, 2, 4};
   int best[6];
   std::memcpy(best, e->tune, sizeof(best));
   float best_ms = 1e30f;
@@ -1738,31 +2001,31 @@ int dbs_engine_autotune(dbs_engine* e, int with_adjoint, int reps) {
   DBS_HIP(hipEventCreate(&b));
   for (int p : paths)
     for (int fb : fblocks)
-      for (int w : waves)
-        for (int cap : caps)
-          for (int bud : budgets) {
-          int t[6] = {p, fb, w, cap, 256, bud};
-          if (dbs_engine_set_tuning(e, 6, t)) continue;
-          // warm
-          int rc = with_adjoint ? dbs_engine_forward_adjoint(e) : dbs_engine_forward(e);
-          if (rc) continue;
-          std::vector<float> ms;
-          for (int r = 0; r < reps; ++r) {
-            hipEventRecord(a, e->stream);
-            rc = with_adjoint ? dbs_engine_forward_adjoint(e) : dbs_engine_forward(e);
-            hipEventRecord(b, e->stream);
-            hipEventSynchronize(b);
-            float x = 0;
-            hipEventElapsedTime(&x, a, b);
-            ms.push_back(x);
+      for (int w : widths)
+        for (int pb : pblocks)
+          for (int np : nps) {
+            int t[6] = {p, fb, w, e->tune[3], pb, np};
+            if (dbs_engine_set_tuning(e, 6, t)) continue;
+            int rc = run_all(e, with_adjoint ? 1 : 0);  // warm
+            if (rc) continue;
+            std::vector<float> ms;
+            for (int r = 0; r < reps && !rc; ++r) {
+              hipEventRecord(a, e->stream);
+              rc = run_all(e, with_adjoint ? 1 : 0);
+              hipEventRecord(b, e->stream);
+              hipEventSynchronize(b);
+              float x = 0;
+              hipEventElapsedTime(&x, a, b);
+              ms.push_back(x);
+            }
+            if (rc || ms.empty()) continue;
+            std::sort(ms.begin(), ms.end());
+            const float med = ms[ms.size() / 2];
+            if (med < best_ms) {
+              best_ms = med;
+              std::memcpy(best, t, sizeof(best));
+            }
           }
-          std::sort(ms.begin(), ms.end());
-          float med = ms[ms.size() / 2];
-          if (med < best_ms) {
-            best_ms = med;
-            std::memcpy(best, t, sizeof(best));
-          }
-        }
   hipEventDestroy(a);
   hipEventDestroy(b);
   return dbs_engine_set_tuning(e, 6, best);
@@ -1898,8 +2161,7 @@ int dbs_engine_profile(dbs_engine* e, int with_adjoint, int reps, int nmax, char
   for (int r = 0; r < reps; ++r) {
     e->profiling = true;
     e->prof_pos = 0;
-    int rc = run_forward(e);
-    if (!rc && with_adjoint) rc = run_adjoint(e);
+    int rc = run_all(e, with_adjoint ? 1 : 0);
     e->profiling = false;
     if (rc) return rc;
     DBS_HIP(hipStreamSynchronize(e->stream));

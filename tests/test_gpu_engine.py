@@ -303,6 +303,99 @@ def test_autotune_keeps_results(setups):
     assert np.max(np.abs(eng.grad() - g0)) <= 1e-13 * np.max(np.abs(g0))
 
 
+@pytest.mark.parametrize("kind", [0, 5])
+def test_pair_panel_shapes(setups, kind):
+    """every pair-panel launch shape (panel width, workgroup size, panels
+    per workgroup) and the dense path give the same values, energy and
+    gradient"""
+    from dbsr3_amd import Engine
+    g, d = setups["fe"]
+    norb = 8
+    fs = random_orbitals(g, norb, seed=300 + kind)
+    rng = np.random.default_rng(300 + kind)
+    rows = make_rows(rng, norb, 3, 300)
+    coef = rng.uniform(-1, 1, len(rows))
+    print(f"[pair panel shapes kind={kind} rows={len(rows)}]", flush=True)
+    eng = Engine([(d, norb, rows)], kind=kind)
+    eng.set_orbitals(0, pq_of(fs))
+    eng.set_coef(0, coef)
+    eng.set_tuning([0, 256, 16, 4096, 256, 1])
+    eng.forward_adjoint()
+    v0, g0, e0 = eng.values(), eng.grad(), eng.energy()
+    ref = oracle_values(g, fs, rows[::7], kind)
+    assert relerr(v0[::7], ref).max() < TOL
+    gs = np.max(np.abs(g0))
+    assert gs > 0
+    for t in ([0, 128, 4, 4096, 64, 1], [0, 256, 8, 4096, 512, 3], [0, 128, 16, 64, 128, 64],
+              [0, 256, 8, 4096, 192, 2], [1, 256, 16, 4096, 256, 1], [2, 256, 8, 4096, 256, 4]):
+        print(f"  tuning {t}", flush=True)
+        eng.set_tuning(t)
+        eng.forward_adjoint()
+        assert np.max(relerr(eng.values(), v0)) < 1e-13, t
+        assert np.max(np.abs(eng.grad() - g0)) <= 1e-13 * gs, t
+        assert abs(eng.energy() - e0) <= 1e-13 * np.sum(np.abs(coef * v0)), t
+        eng.forward()
+        assert np.max(relerr(eng.values(), v0)) < 1e-13, t
+
+
+@pytest.mark.parametrize("norb,width", [(24, 16), (24, 8), (40, 8)])
+def test_group_panel_variants(setups, norb, width):
+    """one multipole with 300 (24 orbitals) or 820 (40 orbitals) total
+    densities: the LDS pair panels in their padded form (more densities
+    than one per thread) or the conflict-free rotated form; 40 rows carry
+    coefficients and are checked with their gradient against the oracle"""
+    from dbsr3_amd import Engine
+    g, d = setups["small"]
+    fs = random_orbitals(g, norb, seed=norb + width)
+    rng = np.random.default_rng(norb + width)
+    rows = make_rows(rng, norb, 0, 4000)
+    rows[:, 0] = 2
+    sel = rng.choice(len(rows), 40, replace=False)
+    coef = np.zeros(len(rows))
+    coef[sel] = rng.uniform(-1, 1, len(sel))
+    print(f"[panel variants norb={norb} width={width}]", flush=True)
+    eng = Engine([(d, norb, rows)])
+    eng.set_orbitals(0, pq_of(fs))
+    eng.set_coef(0, coef)
+    eng.set_tuning([0, 256, width, 4096, 256, 2])
+    eng.forward_adjoint()
+    ref = oracle_values(g, fs, rows[sel], 0)
+    assert relerr(eng.values()[sel], ref).max() < TOL
+    Gref = oracle_gradient(g, fs, rows[sel], coef[sel])
+    assert np.max(np.abs(eng.grad() - Gref)) < TOL * np.max(np.abs(Gref))
+    v1 = eng.values()
+    eng.forward()
+    assert np.max(relerr(eng.values(), v1)) < 1e-13
+
+
+def test_large_group_global_panels(setups):
+    """one multipole with ~2080 total densities (64 orbitals): the panel of
+    the group exceeds LDS and the pair panels read partners from global
+    memory.  Only 40 rows carry a coefficient, so the gradient is checked
+    against the oracle's potentials of those rows."""
+    from dbsr3_amd import Engine
+    g, d = setups["small"]
+    norb = 64
+    fs = random_orbitals(g, norb, seed=64)
+    rng = np.random.default_rng(64)
+    rows = make_rows(rng, norb, 0, 8000)
+    rows[:, 0] = 1
+    sel = rng.choice(len(rows), 40, replace=False)
+    coef = np.zeros(len(rows))
+    coef[sel] = rng.uniform(-1, 1, len(sel))
+    print(f"[large group rows={len(rows)}]", flush=True)
+    eng = Engine([(d, norb, rows)])
+    eng.set_orbitals(0, pq_of(fs))
+    eng.set_coef(0, coef)
+    eng.set_tuning([0])
+    eng.forward_adjoint()
+    ref = oracle_values(g, fs, rows[sel], 0)
+    assert relerr(eng.values()[sel], ref).max() < TOL
+    Gref = oracle_gradient(g, fs, rows[sel], coef[sel])
+    assert np.max(np.abs(eng.grad() - Gref)) < TOL * np.max(np.abs(Gref))
+    assert abs(eng.energy() - coef[sel] @ ref) <= TOL * np.sum(np.abs(coef[sel] * ref))
+
+
 @pytest.mark.parametrize("kind", [0, 3, 5])
 def test_many_orbitals_fallback_paths(setups, kind):
     """60 orbitals: the LDS-staged kernels fall back to their global-memory
