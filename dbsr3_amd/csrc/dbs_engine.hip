@@ -75,6 +75,8 @@ struct KGroupDev {
   int nrows, row0;             // the group's rows row0 .. row0+nrows-1 (group order)
   int npr, dense;              // partial slots per row; rows contracted on the matrix cores
   int chunk0, nchunk;          // adjacency chunks of 64 densities
+  int gmask, pad;              // density kinds present: bit DK_TOT..DK_PQ
+  long long off_pidx;          // orbital pair -> density tables, one NO x NO per kind present
 };
 
 struct DensDev {
@@ -113,6 +115,8 @@ struct RowRed {
   long long pbase, out;
   int stride, npr;
 };
+
+typedef double __attribute__((ext_vector_type(4))) dbl4;
 
 __host__ __device__ inline int feat_dim(int nv, int ks) { return 2 * nv + 2 * nv * ks; }
 
@@ -265,6 +269,115 @@ __global__ void __launch_bounds__(256) k_features(
   }
 }
 
+// ---------------------------------------------------------------------------
+// phase 2, matrix-core form.  Every feature of an interval iv is an entry
+// of a Gram matrix over the orbitals, one per density kind:
+//   Y(m)[a][c]     = sum_i wkin(m,i) sum_comp f_a(x_mi) f_c(x_mi)  (ks nested points)
+//   M^k[a][c]      = sum_m Wk(m)     sum_comp f_a(x_m) f_c(x_m)    (ks outer points)
+//   M^-(k+1)[a][c] = sum_m Wm(m)     sum_comp f_a(x_m) f_c(x_m)
+//   Ro(m)[a][c]    = Wm(m)           sum_comp f_a(x_m) f_c(x_m)
+// (sum_comp over P and Q for total densities, P or Q alone for P_aP_c /
+// Q_aQ_c, P of a with Q of c for P_aQ_c).  Each wave takes its own items
+// (group, interval, kind, quantity): it stages the orbital values of the
+// quantity's points in its slice of LDS, V[comp][point][orbital] with the
+// orbital index XOR-swizzled by the point (conflict-free staging writes and
+// MFMA operand reads), builds the 16x16 tiles with v_mfma_f64_16x16x4f64
+// and stores every tile entry that is a density of the group straight from
+// the accumulators: the pair -> density table maps (a,c) to the column, and
+// with densities sorted by pair the 16 lanes of a tile row store 16
+// consecutive columns.  No workgroup barrier: waves run independently.
+// item = (group, interval, kind, quantity q): q < ks Y(q) and Ro(q);
+// ks M^k; ks+1 M^-(k+1).
+// ---------------------------------------------------------------------------
+__device__ inline int vsw(int comp, int p, int o, int npt, int NO) {
+  return (comp * npt + p) * NO + (o ^ (p & (NO - 1)));
+}
+
+__device__ inline void wave_lds_sync() {
+  // LDS accesses of one wave are performed in order; keep the compiler from
+  // moving accesses across this point
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+__global__ void __launch_bounds__(256) k_features_mfma(
+    int nitems, const int4* __restrict__ items, const KGroupDev* __restrict__ kgs,
+    const InstDev* __restrict__ inst, const int* __restrict__ pidx,
+    const double* __restrict__ orbv, const double* __restrict__ wt, double* __restrict__ F, int wave_lds) {
+  extern __shared__ double sm[];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  double* V = sm + (long long)wv * wave_lds;  // this wave's slice
+  for (int it = blockIdx.x * nw + wv; it < nitems; it += gridDim.x * nw) {
+    const int4 T = items[it];
+    const int g = T.x, iv = T.y, kind = T.z, q = T.w;
+    const KGroupDev G = kgs[g];
+    const InstDev I = inst[G.inst];
+    const int ks = I.ks, nv = I.nv, np = I.np, norb = I.norb, nd = G.ndens;
+    const int NO = norb <= 16 ? 16 : (norb <= 32 ? 32 : 64);
+    const int npt = q < ks ? ks + 1 : ks;  // Y(q): nested points of q, then the outer point q
+    double* W = V + 2 * npt * NO;          // weight of each point; W[ks] = Wm(q)
+    wave_lds_sync();                        // the previous item's reads are done
+    for (int e = lane; e < 2 * NO * npt; e += 64) {
+      const int lp = e % npt, oc = e / npt;  // oc = 2*orb + comp
+      const int o = oc >> 1, comp = oc & 1;
+      const int p = q < ks ? (lp < ks ? ks + q * ks + lp : q) : lp;
+      V[vsw(comp, lp, o, npt, NO)] = o < norb ? orbv[I.off_orbv + ((long long)oc * nv + iv) * np + p] : 0.0;
+    }
+    if (lane < npt) {
+      const int lp = lane;
+      double w;
+      if (q < ks) w = lp < ks ? wt[G.off_w + 2LL * nv * ks + ((long long)iv * ks + q) * ks + lp]
+                              : wt[G.off_w + (long long)nv * ks + (long long)iv * ks + q];
+      else w = wt[G.off_w + (q == ks ? 0LL : (long long)nv * ks) + (long long)iv * ks + lp];
+      W[lp] = w;
+    }
+    wave_lds_sync();
+    const int ntl = NO / 16;
+    const bool sym = kind != DK_PQ;
+    const int slot = __popc(G.gmask & ((1 << kind) - 1));
+    const int* pt = pidx + G.off_pidx + (long long)slot * NO * NO;
+    double* Fg = F + G.off_F;
+    long long xrow;
+    if (q < ks) xrow = 2LL * (nv + iv * ks + q) + 1;  // Y(iv,q); Ro(iv,q) is the row before
+    else xrow = 2LL * iv + (q - ks);                   // M^k (the S slot), M^-(k+1)
+    // components of the A (row orbital) and B (column orbital) operands
+    for (int tt = 0; tt < ntl * ntl; ++tt) {
+      const int ta = tt / ntl, tc = tt % ntl;
+      if (sym && ta > tc) continue;  // densities of symmetric kinds have a <= c
+      const int ra = ta * 16 + (lane & 15), cb = tc * 16 + (lane & 15);
+      for (int pass = 0; pass < (q < ks ? 2 : 1); ++pass) {
+        // pass 0: the Gram over the ks points; pass 1 (Y items): Ro over the outer point
+        const int K = pass == 0 ? (kind == DK_TOT ? 2 * ks : ks) : (kind == DK_TOT ? 2 : 1);
+        dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+        for (int k0 = 0; k0 < K; k0 += 4) {
+          const int kk = k0 + (lane >> 4);
+          double av = 0.0, bv = 0.0;
+          if (kk < K) {
+            const int per = pass == 0 ? ks : 1;
+            int ca, cbp, lp;
+            if (kind == DK_TOT) { ca = cbp = kk >= per ? 1 : 0; lp = kk - ca * per; }
+            else if (kind == DK_PP) { ca = cbp = 0; lp = kk; }
+            else if (kind == DK_QQ) { ca = cbp = 1; lp = kk; }
+            else { ca = 0; cbp = 1; lp = kk; }
+            if (pass == 1) lp = ks;  // the outer point
+            av = W[lp] * V[vsw(ca, lp, ra, npt, NO)];
+            bv = V[vsw(cbp, lp, cb, npt, NO)];
+          }
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+        }
+        // lane holds column cb and rows ta*16 + 4j + lane/16
+        const long long x = pass == 0 ? xrow : xrow - 1;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int row = ta * 16 + 4 * j + (lane >> 4);
+          const int d = pt[row * NO + cb];
+          if (d >= 0) Fg[x * nd + d] = acc[j];
+        }
+      }
+    }
+  }
+}
+
 // exclusive prefix (dir=0) or suffix (dir=1) scan over the intervals of the
 // S rows (x = 2 iv) of every density column.  Workgroup = 32 columns x 8
 // interval segments; items (group, first column).  Each thread sums its
@@ -314,8 +427,6 @@ __global__ void __launch_bounds__(256) k_scan_cols(int nitems, const int2* __res
 // v_mfma_f64_16x16x4f64 tiles of 16x16 outputs.  One wavefront computes a
 // 16x16 tile and scatters the requested pairs.
 // ---------------------------------------------------------------------------
-typedef double __attribute__((ext_vector_type(4))) dbl4;
-
 // partner row of feature row x of the left density (two-sided: u<->v;
 // one-sided S^k: S_L with Mm_R for j < nv, Y_L with Ro_R beyond); -1: none
 __device__ inline int partner_row(int x, int nv, int onesided) {
@@ -1044,6 +1155,12 @@ struct dbs_engine {
   // LDS sizes (doubles) of the staged kernels; 0 = not staged
   int feat_stage = 0;
   size_t feat_shm = 0;
+  size_t featm_shm = 0;   // LDS of the matrix-core features kernel (0: not usable)
+  int4* d_fitems = nullptr;  // its work items (group, interval, kind, quantity)
+  int nfitems = 0;
+  int* d_pidx = nullptr;     // orbital pair -> density per group and kind
+  int featm_wave = 0;        // LDS doubles per wave of the matrix-core kernel
+  int max_nd = 0;         // largest group
   int adj_mode = 0;
   size_t adj_shm = 0;
   long long adj_items = 0;
@@ -1443,8 +1560,12 @@ static int run_forward(dbs_engine* e, int with_adj) {
   if (with_adj && e->nent > 0)
     k_adj_cgather<<<grid_for(e->nent, 256, cap * 4), 256, 0, s>>>(e->nent, e->d_ecidx, e->coef, e->d_cent);
   mark(e, "weights");
-  k_features<<<grid_for(e->nitems, 1, cap), e->tune[1], e->feat_shm, s>>>(
-      e->nitems, e->d_items, e->d_kg, e->d_inst, e->d_dens, e->orbv, e->wt, e->F, e->feat_stage);
+  if (e->featm_shm > 0)
+    k_features_mfma<<<grid_for(e->nfitems, 4, cap * 4), 256, e->featm_shm, s>>>(
+        e->nfitems, e->d_fitems, e->d_kg, e->d_inst, e->d_pidx, e->orbv, e->wt, e->F, e->featm_wave);
+  else
+    k_features<<<grid_for(e->nitems, 1, cap), e->tune[1], e->feat_shm, s>>>(
+        e->nitems, e->d_items, e->d_kg, e->d_inst, e->d_dens, e->orbv, e->wt, e->F, e->feat_stage);
   mark(e, "features");
   k_scan_cols<<<grid_for(e->nscan_items, 1, cap), 256, 0, s>>>(e->nscan_items, e->d_scan_items, e->d_kg,
                                                                e->d_inst, e->F, 0);
@@ -1686,11 +1807,36 @@ int dbs_engine_create(dbs_engine** out, int ninst, dbs_ctx* const* ctxs, const i
         grows.push_back(gid);
       }
       G.ndens = (int)e->h_dens.size() - G.dens0;
+      // densities of the group in (kind, a, c) order: neighbouring threads
+      // then read neighbouring orbital-pair entries
+      {
+        std::vector<int> perm(G.ndens);
+        std::iota(perm.begin(), perm.end(), 0);
+        auto key = [&](int x) {
+          const DensDev& D = e->h_dens[G.dens0 + x];
+          return std::make_tuple(D.kind, D.a, D.c);
+        };
+        std::stable_sort(perm.begin(), perm.end(), [&](int x, int y) { return key(x) < key(y); });
+        std::vector<int> newpos(G.ndens);
+        std::vector<DensDev> sorted(G.ndens);
+        for (int q = 0; q < G.ndens; ++q) {
+          newpos[perm[q]] = q;
+          sorted[q] = e->h_dens[G.dens0 + perm[q]];
+        }
+        for (int q = 0; q < G.ndens; ++q) e->h_dens[G.dens0 + q] = sorted[q];
+        for (int r : grows) {
+          e->h_rows[r].L = G.dens0 + newpos[e->h_rows[r].L - G.dens0];
+          e->h_rows[r].R = G.dens0 + newpos[e->h_rows[r].R - G.dens0];
+        }
+        G.gmask = 0;
+        G.pad = 0;
+        for (int q = 0; q < G.ndens; ++q) G.gmask |= 1 << e->h_dens[G.dens0 + q].kind;
+      }
       G.off_F = foff;
       foff += (long long)D * G.ndens;
       G.nrows = (int)grows.size();
       G.row0 = grows.empty() ? (int)e->h_rows.size() : grows[0];
-      G.off_part = 0; G.npr = 0; G.dense = 0; G.chunk0 = 0; G.nchunk = 0;
+      G.off_part = 0; G.npr = 0; G.dense = 0; G.chunk0 = 0; G.nchunk = 0; G.off_pidx = 0;
       seg_w.push_back(woff);
       woff += 2LL * I.nv * I.ks + (long long)I.nv * I.ks * I.ks;
       e->h_kg.push_back(G);
@@ -1813,6 +1959,48 @@ int dbs_engine_create(dbs_engine** out, int ninst, dbs_ctx* const* ctxs, const i
     }
     e->feat_stage = fneed <= lds_max ? 1 : 0;
     e->feat_shm = sizeof(double) * (e->feat_stage ? fneed : wneed);
+    // matrix-core features: per wave V[2][ks+1][NO] + weights
+    size_t wneed2 = 0;
+    bool fits = true;
+    for (int g = 0; g < e->nkg; ++g) e->max_nd = std::max(e->max_nd, e->h_kg[g].ndens);
+    for (auto& I : e->h_inst) {
+      if (I.norb > 64) fits = false;
+      const size_t NO = I.norb <= 16 ? 16 : (I.norb <= 32 ? 32 : 64);
+      wneed2 = std::max(wneed2, 2 * (size_t)(I.ks + 1) * NO + (I.ks + 1));
+    }
+    wneed2 = (wneed2 + 1) & ~(size_t)1;  // 16-byte aligned slices
+    e->featm_wave = (int)wneed2;
+    e->featm_shm = fits ? 4 * wneed2 * sizeof(double) : 0;
+    if (e->featm_shm > 64 * 1024)
+      hipFuncSetAttribute((const void*)k_features_mfma, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)e->featm_shm);
+    if (e->featm_shm > 0) {
+      std::vector<int4> fit;
+      std::vector<int> pidx;
+      for (int g = 0; g < e->nkg; ++g) {
+        KGroupDev& G = e->h_kg[g];
+        const InstDev& I = e->h_inst[G.inst];
+        const int NO = I.norb <= 16 ? 16 : (I.norb <= 32 ? 32 : 64);
+        G.off_pidx = (long long)pidx.size();
+        for (int kd = 0; kd < 4; ++kd) {
+          if (!((G.gmask >> kd) & 1)) continue;
+          const size_t base = pidx.size();
+          pidx.resize(base + (size_t)NO * NO, -1);
+          for (int d = 0; d < G.ndens; ++d) {
+            const DensDev& D = e->h_dens[G.dens0 + d];
+            if (D.kind == kd) pidx[base + (size_t)D.a * NO + D.c] = d;
+          }
+        }
+        for (int iv = 0; iv < I.nv; ++iv)
+          for (int kd = 0; kd < 4; ++kd)
+            if ((G.gmask >> kd) & 1)
+              for (int q = 0; q < I.ks + 2; ++q) fit.push_back(make_int4(g, iv, kd, q));
+      }
+      e->nfitems = (int)fit.size();
+      rc |= upload(s, &e->d_fitems, fit);
+      rc |= upload(s, &e->d_pidx, pidx);
+    }
+
     e->adj_mode = kind == 0 ? 0 : (kind <= 4 ? 1 : 2);
     e->adj_shm = aneed <= lds_max ? sizeof(double) * aneed : 0;
     std::vector<long long> seg_adj(ninst);
@@ -1865,7 +2053,7 @@ int dbs_engine_destroy(dbs_engine* e) {
                   e->d_seg_w, e->d_roff, e->d_items, e->d_scan_items, e->d_oadjoff, e->d_oadj,
                   e->d_tiles, e->d_pairrow, e->d_pairoff, e->d_nextrow, e->plain_d, e->plain_c,
                   e->d_echunks, e->d_echoff, e->d_epart, e->d_seg_adj, e->d_chunks, e->d_ent,
-                  e->d_ecidx, e->d_cent, e->d_red, e->d_part, e->d_alen,
+                  e->d_ecidx, e->d_cent, e->d_red, e->d_part, e->d_alen, e->d_fitems, e->d_pidx,
                   e->d_work[0][0], e->d_work[0][1], e->d_work[0][2],
                   e->d_work[1][0], e->d_work[1][1], e->d_work[1][2]};
   for (void* p : bufs)

@@ -338,11 +338,13 @@ def test_pair_panel_shapes(setups, kind):
         assert np.max(relerr(eng.values(), v0)) < 1e-13, t
 
 
-@pytest.mark.parametrize("norb,width", [(24, 16), (24, 8), (40, 8)])
-def test_group_panel_variants(setups, norb, width):
+@pytest.mark.parametrize("norb,width,fblock", [(24, 16, 256), (24, 8, 64), (40, 8, 256)])
+def test_group_panel_variants(setups, norb, width, fblock):
     """one multipole with 300 (24 orbitals) or 820 (40 orbitals) total
     densities: the LDS pair panels in their padded form (more densities
-    than one per thread) or the conflict-free rotated form; 40 rows carry
+    than one per thread) or the conflict-free rotated form; the features
+    by the matrix-core Gram kernel or (64-thread blocks, more than four
+    densities per thread) the per-density kernel; 40 rows carry
     coefficients and are checked with their gradient against the oracle"""
     from dbsr3_amd import Engine
     g, d = setups["small"]
@@ -353,11 +355,11 @@ def test_group_panel_variants(setups, norb, width):
     sel = rng.choice(len(rows), 40, replace=False)
     coef = np.zeros(len(rows))
     coef[sel] = rng.uniform(-1, 1, len(sel))
-    print(f"[panel variants norb={norb} width={width}]", flush=True)
+    print(f"[panel variants norb={norb} width={width} fblock={fblock}]", flush=True)
     eng = Engine([(d, norb, rows)])
     eng.set_orbitals(0, pq_of(fs))
     eng.set_coef(0, coef)
-    eng.set_tuning([0, 256, width, 4096, 256, 2])
+    eng.set_tuning([0, fblock, width, 4096, 256, 2])
     eng.forward_adjoint()
     ref = oracle_values(g, fs, rows[sel], 0)
     assert relerr(eng.values()[sel], ref).max() < TOL
@@ -366,6 +368,29 @@ def test_group_panel_variants(setups, norb, width):
     v1 = eng.values()
     eng.forward()
     assert np.max(relerr(eng.values(), v1)) < 1e-13
+
+
+@pytest.mark.parametrize("kind", [0, 3])
+def test_features_beyond_64_orbitals(setups, kind):
+    """more than 64 orbitals: the features come from the per-density kernel
+    instead of the matrix-core Gram kernel"""
+    from dbsr3_amd import Engine
+    g, d = setups["small"]
+    norb = 70
+    fs = random_orbitals(g, norb, seed=70 + kind)
+    rng = np.random.default_rng(70 + kind)
+    rows = make_rows(rng, norb, 2, 200)
+    coef = rng.uniform(-1, 1, len(rows))
+    print(f"[features fallback kind={kind}]", flush=True)
+    eng = Engine([(d, norb, rows)], kind=kind)
+    eng.set_orbitals(0, pq_of(fs))
+    eng.set_coef(0, coef)
+    eng.forward_adjoint()
+    ref = oracle_values(g, fs, rows, kind)
+    assert relerr(eng.values(), ref).max() < TOL
+    if kind == 0:
+        Gref = oracle_gradient(g, fs, rows, coef)
+        assert np.max(np.abs(eng.grad() - Gref)) < TOL * np.max(np.abs(Gref))
 
 
 def test_large_group_global_panels(setups):
