@@ -75,7 +75,8 @@ struct KGroupDev {
   int nrows, row0;             // the group's rows row0 .. row0+nrows-1 (group order)
   int npr, dense;              // partial slots per row; rows contracted on the matrix cores
   int chunk0, nchunk;          // adjacency chunks of 64 densities
-  int gmask, pad;              // density kinds present: bit DK_TOT..DK_PQ
+  int gmask, full;             // density kinds present (bit DK_TOT..DK_PQ); 1: every
+                               // orbital pair a <= c is a total density of the group
   long long off_pidx;          // orbital pair -> density tables, one NO x NO per kind present
 };
 
@@ -1011,6 +1012,170 @@ __global__ void __launch_bounds__(256) k_adj_orb_lds(
   }
 }
 
+// adjoint phase C, matrix-core form for total densities (engine kind
+// rk sum).  One workgroup per (instance, interval iv, outer point m).  For
+// every (instance,k) group the density gradients at the point are scattered
+// into two symmetric orbital matrices in LDS (diagonal doubled):
+//   MO[a][c] = G_Mk(iv) Wk(m) + (G_Mm(iv) + G_Ro(iv,m)) Wm(m)   (outer point)
+//   MY[a][c] = G_Y(iv,m)                                       (nested points)
+// and the orbital gradients at the eleven points are the products
+//   gf(x_m)  += MO f(x_m),   gf(x_mi) += MY (wkin(m,i) f(x_mi))
+// on v_mfma_f64_16x16x4f64 tiles, accumulated over the groups in the
+// waves' registers (tasks: row tile x {nested P/Q columns 0-15, 16-31,
+// outer columns}).  The matrices alternate between two LDS buffers so one
+// group's scatter overlaps the previous group's products.
+// dpk[d] = a | c << 8 of every density.
+#define DBS_ADJ_PF 2  // densities per thread whose gradients are fetched one group ahead
+__global__ void __launch_bounds__(256) k_adj_orb_mfma(
+    int ninst, const long long* __restrict__ seg, const InstDev* __restrict__ inst,
+    const KGroupDev* __restrict__ kgs, const int* __restrict__ dpk, const double* __restrict__ orbv,
+    const double* __restrict__ wt, const double* __restrict__ Gb, double* __restrict__ gorb, long long total) {
+  extern __shared__ double sm[];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (long long item = blockIdx.x; item < total; item += gridDim.x) {
+    const int s = find_seg(seg, ninst, item);
+    const InstDev I = inst[s];
+    const int r = (int)(item - seg[s]);
+    const int iv = r / I.ks, m = r % I.ks;
+    const int norb = I.norb, ks = I.ks, np = I.np, nv = I.nv, nk = nv * ks;
+    const int NO = norb <= 16 ? 16 : (norb <= 32 ? 32 : 64);
+    const int LM = NO + 1;
+    double* FI = sm;              // [NO][32]: columns comp*ks + i
+    double* FO = FI + NO * 32;    // [NO][16]: columns comp
+    double* WV = FO + NO * 16;    // 2 x 16: wkin(m, i) of the group, per buffer
+    double* MB = WV + 32;         // 2 buffers x (MO, MY) of [NO][LM]
+    __syncthreads();
+    for (int q = threadIdx.x; q < NO * 32; q += blockDim.x) {
+      const int c = q / 32, col = q % 32;
+      double v = 0.0;
+      if (c < norb && col < 2 * ks) {
+        const int comp = col / ks, i = col % ks;
+        v = orbv[I.off_orbv + ((long long)(2 * c + comp) * nv + iv) * np + ks + m * ks + i];
+      }
+      FI[q] = v;
+    }
+    for (int q = threadIdx.x; q < NO * 16; q += blockDim.x) {
+      const int c = q / 16, col = q % 16;
+      FO[q] = (c < norb && col < 2) ? orbv[I.off_orbv + ((long long)(2 * c + col) * nv + iv) * np + m] : 0.0;
+    }
+    // both buffers start at zero (entries that no density writes stay zero;
+    // rows and columns beyond norb meet zero operands)
+    for (int q = threadIdx.x; q < 4 * NO * LM; q += blockDim.x) MB[q] = 0.0;
+    __syncthreads();
+    const int ntl = NO / 16, ntask = ntl * 3;
+    dbl4 accs[3];
+#pragma unroll
+    for (int n = 0; n < 3; ++n) accs[n] = dbl4{0.0, 0.0, 0.0, 0.0};
+    const long long xr = 2LL * (nv + iv * ks + m);
+    // gradients of the first DBS_ADJ_PF densities per thread, one group ahead
+    double pgo[DBS_ADJ_PF], pgy[DBS_ADJ_PF];
+    int ppk[DBS_ADJ_PF];
+    auto fetch = [&](int gi) {
+      const KGroupDev Gn = kgs[I.kg0 + gi];
+      const double wkv = wt[Gn.off_w + (long long)iv * ks + m];
+      const double wmv = wt[Gn.off_w + (long long)nk + (long long)iv * ks + m];
+      const double* gb = Gb + Gn.off_F;
+      const int ndn = Gn.ndens;
+#pragma unroll
+      for (int u = 0; u < DBS_ADJ_PF; ++u) {
+        const int d = threadIdx.x + u * blockDim.x;
+        ppk[u] = -1;
+        pgo[u] = pgy[u] = 0.0;
+        if (d < ndn) {
+          ppk[u] = dpk[Gn.dens0 + d];
+          pgo[u] = gb[2LL * iv * ndn + d] * wkv + (gb[(2LL * iv + 1) * ndn + d] + gb[xr * ndn + d]) * wmv;
+          pgy[u] = gb[(xr + 1) * ndn + d];
+        }
+      }
+    };
+    if (I.nkg > 0) fetch(0);
+    for (int gi = 0; gi < I.nkg; ++gi) {
+      const KGroupDev G = kgs[I.kg0 + gi];
+      const int nd = G.ndens;
+      const int b = gi & 1;
+      double* MO = MB + (long long)b * 2 * NO * LM;
+      double* MY = MO + NO * LM;
+      // this buffer's previous products finished before the last barrier
+      if (!G.full && gi >= 2)
+        for (int q = threadIdx.x; q < 2 * NO * LM; q += blockDim.x) MO[q] = 0.0;
+      if (threadIdx.x < 16) WV[b * 16 + threadIdx.x] =
+          threadIdx.x < ks ? wt[G.off_w + 2LL * nk + ((long long)iv * ks + m) * ks + threadIdx.x] : 0.0;
+      if (!G.full && gi >= 2) __syncthreads();
+      double go[DBS_ADJ_PF], gy[DBS_ADJ_PF];
+      int pk[DBS_ADJ_PF];
+#pragma unroll
+      for (int u = 0; u < DBS_ADJ_PF; ++u) {
+        go[u] = pgo[u];
+        gy[u] = pgy[u];
+        pk[u] = ppk[u];
+      }
+      if (gi + 1 < I.nkg) fetch(gi + 1);
+      auto put = [&](int p, double vo, double vy) {
+        const int a = p & 255, c = (p >> 8) & 255;
+        if (a == c) {
+          MO[a * LM + a] = 2.0 * vo;
+          MY[a * LM + a] = 2.0 * vy;
+        } else {
+          MO[a * LM + c] = vo;
+          MO[c * LM + a] = vo;
+          MY[a * LM + c] = vy;
+          MY[c * LM + a] = vy;
+        }
+      };
+#pragma unroll
+      for (int u = 0; u < DBS_ADJ_PF; ++u)
+        if (pk[u] >= 0) put(pk[u], go[u], gy[u]);
+      if (nd > DBS_ADJ_PF * (int)blockDim.x) {
+        const double wkv = wt[G.off_w + (long long)iv * ks + m];
+        const double wmv = wt[G.off_w + (long long)nk + (long long)iv * ks + m];
+        const double* gb = Gb + G.off_F;
+        for (int d = threadIdx.x + DBS_ADJ_PF * blockDim.x; d < nd; d += blockDim.x)
+          put(dpk[G.dens0 + d], gb[2LL * iv * nd + d] * wkv + (gb[(2LL * iv + 1) * nd + d] + gb[xr * nd + d]) * wmv,
+              gb[(xr + 1) * nd + d]);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int n = 0; n < 3; ++n) {
+        const int t = wv + n * nw;
+        if (t >= ntask) continue;
+        const int rt = t / 3, ct = t % 3;
+        const double* M = ct < 2 ? MY : MO;
+        const int row = rt * 16 + (lane & 15), col = ct * 16 + (lane & 15);
+        dbl4 acc = accs[n];
+        for (int k0 = 0; k0 < NO; k0 += 4) {
+          const int c = k0 + (lane >> 4);
+          const double av = M[row * LM + c];
+          const double bv = ct < 2 ? WV[b * 16 + (col % ks)] * FI[c * 32 + col] : FO[c * 16 + (lane & 15)];
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+        }
+        accs[n] = acc;
+      }
+    }
+    // results: lane holds column (lane & 15) of its tile, rows rt*16 + 4j + lane/16
+#pragma unroll
+    for (int n = 0; n < 3; ++n) {
+      const int t = wv + n * nw;
+      if (t >= ntask) continue;
+      const int rt = t / 3, ct = t % 3;
+      const dbl4 acc = accs[n];
+      const int col = ct * 16 + (lane & 15);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int a = rt * 16 + 4 * j + (lane >> 4);
+        if (a >= norb) continue;
+        if (ct < 2) {
+          if (col < 2 * ks) {
+            const int comp = col / ks, i = col % ks;
+            gorb[I.off_orbv + ((long long)(2 * a + comp) * nv + iv) * np + ks + m * ks + i] = acc[j];
+          }
+        } else if ((lane & 15) < 2) {
+          gorb[I.off_orbv + ((long long)(2 * a + (lane & 15)) * nv + iv) * np + m] = acc[j];
+        }
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // plain path for integral sets: materialised arrays + convol(case 2) per
 // distinct left density + SUM_AmB per row (the reference's Rk_evaluate
@@ -1159,6 +1324,8 @@ struct dbs_engine {
   int4* d_fitems = nullptr;  // its work items (group, interval, kind, quantity)
   int nfitems = 0;
   int* d_pidx = nullptr;     // orbital pair -> density per group and kind
+  int* d_dpk = nullptr;      // a | c << 8 per density
+  size_t adjm_shm = 0;       // LDS of the matrix-core adjoint kernel (0: not usable)
   int featm_wave = 0;        // LDS doubles per wave of the matrix-core kernel
   int max_nd = 0;         // largest group
   int adj_mode = 0;
@@ -1602,7 +1769,10 @@ static int run_adjoint(dbs_engine* e) {
   k_scan_cols<<<grid_for(e->nscan_items, 1, cap), 256, 0, s>>>(e->nscan_items, e->d_scan_items, e->d_kg,
                                                                e->d_inst, e->G, 1);
   mark(e, "adj_suffix_scan");
-  if (e->adj_shm > 0) {
+  if (e->adjm_shm > 0) {
+    k_adj_orb_mfma<<<grid_for(e->adj_items, 1, cap * 4), 256, e->adjm_shm, s>>>(
+        e->ninst, e->d_seg_adj, e->d_inst, e->d_kg, e->d_dpk, e->orbv, e->wt, e->G, e->gorb, e->adj_items);
+  } else if (e->adj_shm > 0) {
     k_adj_orb_lds<<<grid_for(e->adj_items, 1, cap * 4), 256, e->adj_shm, s>>>(
         e->ninst, e->d_seg_adj, e->d_inst, e->d_kg, e->d_dens, e->orbv, e->wt, e->G, e->adj_mode, e->gorb,
         e->adj_items);
@@ -1829,8 +1999,8 @@ int dbs_engine_create(dbs_engine** out, int ninst, dbs_ctx* const* ctxs, const i
           e->h_rows[r].R = G.dens0 + newpos[e->h_rows[r].R - G.dens0];
         }
         G.gmask = 0;
-        G.pad = 0;
         for (int q = 0; q < G.ndens; ++q) G.gmask |= 1 << e->h_dens[G.dens0 + q].kind;
+        G.full = (G.gmask == (1 << DK_TOT) && G.ndens == norb[i] * (norb[i] + 1) / 2) ? 1 : 0;
       }
       G.off_F = foff;
       foff += (long long)D * G.ndens;
@@ -2003,6 +2173,26 @@ int dbs_engine_create(dbs_engine** out, int ninst, dbs_ctx* const* ctxs, const i
 
     e->adj_mode = kind == 0 ? 0 : (kind <= 4 ? 1 : 2);
     e->adj_shm = aneed <= lds_max ? sizeof(double) * aneed : 0;
+    // matrix-core adjoint (total densities, up to 64 orbitals)
+    {
+      size_t need = 0;
+      bool ok = kind == 0;
+      for (auto& I : e->h_inst) {
+        if (I.norb > 64 || 2 * I.ks > 32 || I.ks > 16) ok = false;
+        const size_t NO = I.norb <= 16 ? 16 : (I.norb <= 32 ? 32 : 64);
+        need = std::max(need, NO * 32 + NO * 16 + 32 + 4 * NO * (NO + 1));
+      }
+      int lmx = 0;
+      if (hipDeviceGetAttribute(&lmx, hipDeviceAttributeMaxSharedMemoryPerBlock, e->device) != hipSuccess)
+        lmx = 64 * 1024;
+      e->adjm_shm = ok && need * sizeof(double) <= (size_t)lmx ? need * sizeof(double) : 0;
+      if (e->adjm_shm > 64 * 1024)
+        hipFuncSetAttribute((const void*)k_adj_orb_mfma, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)e->adjm_shm);
+      std::vector<int> pk(std::max(e->ndens, 1), 0);
+      for (int d = 0; d < e->ndens; ++d) pk[d] = (e->h_dens[d].a & 255) | ((e->h_dens[d].c & 255) << 8);
+      rc |= upload(s, &e->d_dpk, pk);
+    }
     std::vector<long long> seg_adj(ninst);
     long long tot = 0;
     for (int i = 0; i < ninst; ++i) {
@@ -2053,7 +2243,7 @@ int dbs_engine_destroy(dbs_engine* e) {
                   e->d_seg_w, e->d_roff, e->d_items, e->d_scan_items, e->d_oadjoff, e->d_oadj,
                   e->d_tiles, e->d_pairrow, e->d_pairoff, e->d_nextrow, e->plain_d, e->plain_c,
                   e->d_echunks, e->d_echoff, e->d_epart, e->d_seg_adj, e->d_chunks, e->d_ent,
-                  e->d_ecidx, e->d_cent, e->d_red, e->d_part, e->d_alen, e->d_fitems, e->d_pidx,
+                  e->d_ecidx, e->d_cent, e->d_red, e->d_part, e->d_alen, e->d_fitems, e->d_pidx, e->d_dpk,
                   e->d_work[0][0], e->d_work[0][1], e->d_work[0][2],
                   e->d_work[1][0], e->d_work[1][1], e->d_work[1][2]};
   for (void* p : bufs)
