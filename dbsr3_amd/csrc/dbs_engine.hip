@@ -104,6 +104,7 @@ struct AdjChunk {
 };
 struct AdjEnt {
   int P, slot;
+  double c;  // the row's coefficient, gathered at every launch
 };
 
 // pair-panel work: slots [j0, j1) of group g, partial column pr
@@ -141,13 +142,14 @@ __global__ void k_orb_points(int ninst, const long long* __restrict__ seg,
        q += (long long)gridDim.x * blockDim.x) {
     const int s = find_seg(seg, ninst, q);
     const InstDev I = inst[s];
-    long long r = q - seg[s];
-    const int p = (int)(r % I.np);
-    r /= I.np;
-    const int iv = (int)(r % I.nv);
-    r /= I.nv;
-    const int comp = (int)(r % 2);
-    const int orb = (int)(r / 2);
+    // index within the instance (fits 32 bits: norb*2*nv*np values)
+    unsigned r = (unsigned)(q - seg[s]);
+    const int p = (int)(r % (unsigned)I.np);
+    r /= (unsigned)I.np;
+    const int iv = (int)(r % (unsigned)I.nv);
+    r /= (unsigned)I.nv;
+    const int comp = (int)(r & 1u);
+    const int orb = (int)(r >> 1);
     const int kx = comp == 0 ? I.ksp : I.ksq;
     const double* coef = pq + I.off_pq + (long long)I.ns * (comp + 2LL * orb) + iv;
     double v = 0.0;
@@ -380,12 +382,12 @@ __global__ void __launch_bounds__(256) k_features_mfma(
 }
 
 // exclusive prefix (dir=0) or suffix (dir=1) scan over the intervals of the
-// S rows (x = 2 iv) of every density column.  Workgroup = 32 columns x 8
+// S rows (x = 2 iv) of every density column.  Workgroup = 16 columns x 16
 // interval segments; items (group, first column).  Each thread sums its
 // segment, the segment sums are combined in LDS in a fixed order, then the
 // thread rewrites its segment with the exclusive running sums.
-#define DBS_SCAN_COLS 32
-#define DBS_SCAN_SEGS 8
+#define DBS_SCAN_COLS 16
+#define DBS_SCAN_SEGS 16
 __global__ void __launch_bounds__(256) k_scan_cols(int nitems, const int2* __restrict__ items,
                                                    const KGroupDev* __restrict__ kgs,
                                                    const InstDev* __restrict__ inst, double* __restrict__ F,
@@ -402,8 +404,13 @@ __global__ void __launch_bounds__(256) k_scan_cols(int nitems, const int2* __res
     const long long st = 2LL * nd;  // S(iv) at row 2 iv
     double sum = 0.0;
     if (d < nd) {
-      if (dir == 0) { for (int iv = lo; iv < hi; ++iv) sum += f[iv * st]; }
-      else { for (int iv = hi - 1; iv >= lo; --iv) sum += f[iv * st]; }
+      if (dir == 0) {
+#pragma unroll 4
+        for (int iv = lo; iv < hi; ++iv) sum += f[iv * st];
+      } else {
+#pragma unroll 4
+        for (int iv = hi - 1; iv >= lo; --iv) sum += f[iv * st];
+      }
     }
     __syncthreads();
     ssum[seg][col] = sum;
@@ -537,7 +544,7 @@ template <int W, bool LDS, bool ADJ>
 __global__ void __launch_bounds__(pair_block_max<W>()) k_pair_panel(
     int nwork, const PairWork* __restrict__ work, const KGroupDev* __restrict__ kgs,
     const InstDev* __restrict__ inst, const AdjChunk* __restrict__ chunks,
-    const AdjEnt* __restrict__ ent, const int* __restrict__ alen, const double* __restrict__ cent,
+    const AdjEnt* __restrict__ ent, const int* __restrict__ alen,
     const double* __restrict__ F, double* __restrict__ Gout, double* __restrict__ part, int onesided) {
   extern __shared__ double sp[];
   constexpr int LD = 2 * W + 2;  // LDS row: u(W) v(W) + pad (16-byte aligned rows)
@@ -588,7 +595,7 @@ __global__ void __launch_bounds__(pair_block_max<W>()) k_pair_panel(
             const int i = i0 + q;
             if (i < n) {
               Aq[q] = ent[eb + 64LL * i];
-              cq[q] = ADJ ? cent[eb + 64LL * i] : 0.0;
+              cq[q] = ADJ ? Aq[q].c : 0.0;
             } else {
               Aq[q].P = X;
               Aq[q].slot = -1;
@@ -665,7 +672,7 @@ template <int W, bool ONE, bool ADJ>
 __global__ void __launch_bounds__(pair_block_max<W>()) k_pair_rot(
     int nwork, const PairWork* __restrict__ work, const KGroupDev* __restrict__ kgs,
     const InstDev* __restrict__ inst, const AdjChunk* __restrict__ chunks,
-    const AdjEnt* __restrict__ ent, const int* __restrict__ alen, const double* __restrict__ cent,
+    const AdjEnt* __restrict__ ent, const int* __restrict__ alen,
     const double* __restrict__ F, double* __restrict__ Gout, double* __restrict__ part) {
   extern __shared__ double sm[];
   constexpr int E = 2 * W;
@@ -719,7 +726,7 @@ __global__ void __launch_bounds__(pair_block_max<W>()) k_pair_rot(
           const int i = i0 + q;
           if (i < n) {
             Aq[q] = ent[eb + 64LL * i];
-            cq[q] = ADJ ? cent[eb + 64LL * i] : 0.0;
+            cq[q] = ADJ ? Aq[q].c : 0.0;
           } else {
             Aq[q].P = X;
             Aq[q].slot = -1;
@@ -791,11 +798,11 @@ __global__ void k_rows_reduce(int n, const RowRed* __restrict__ red, const doubl
 
 // coefficients of the adjacency entries (0 on padding)
 __global__ void k_adj_cgather(long long n, const long long* __restrict__ cidx, const double* __restrict__ coef,
-                              double* __restrict__ cent) {
+                              AdjEnt* __restrict__ ent) {
   for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n;
        q += (long long)gridDim.x * blockDim.x) {
     const long long c = cidx[q];
-    cent[q] = c >= 0 ? coef[c] : 0.0;
+    ent[q].c = c >= 0 ? coef[c] : 0.0;
   }
 }
 
@@ -871,11 +878,11 @@ __global__ void k_adj_coef(int ninst, const long long* __restrict__ seg, const I
        q += ((long long)gridDim.x * blockDim.x) >> 6) {
     const int s = find_seg(seg, ninst, q);
     const InstDev I = inst[s];
-    long long r = q - seg[s];
-    const int j = (int)(r % I.ns);
-    r /= I.ns;
-    const int comp = (int)(r % 2);
-    const int orb = (int)(r / 2);
+    unsigned r = (unsigned)(q - seg[s]);
+    const int j = (int)(r % (unsigned)I.ns);
+    r /= (unsigned)I.ns;
+    const int comp = (int)(r & 1u);
+    const int orb = (int)(r >> 1);
     const int kx = comp == 0 ? I.ksp : I.ksq;
     const int ks = I.ks, nv = I.nv, np = I.np, nw = nv + 1;
     const double* xb = comp == 0 ? I.pbsp : I.qbsp;
@@ -1062,7 +1069,8 @@ __global__ void __launch_bounds__(256) k_adj_orb_mfma(
     // rows and columns beyond norb meet zero operands)
     for (int q = threadIdx.x; q < 4 * NO * LM; q += blockDim.x) MB[q] = 0.0;
     __syncthreads();
-    const int ntl = NO / 16, ntask = ntl * 3;
+    const int ntl = NO / 16, ntask = ntl * 2;  // nested-point tiles; the outer point on VALU
+    double oacc = 0.0;                            // thread t < 2 norb: orbital t/2, component t%2
     dbl4 accs[3];
 #pragma unroll
     for (int n = 0; n < 3; ++n) accs[n] = dbl4{0.0, 0.0, 0.0, 0.0};
@@ -1134,43 +1142,44 @@ __global__ void __launch_bounds__(256) k_adj_orb_mfma(
               gb[(xr + 1) * nd + d]);
       }
       __syncthreads();
+      if ((int)threadIdx.x < 2 * norb) {
+        const int a = threadIdx.x >> 1, comp = threadIdx.x & 1;
+        double x = 0.0;
+        for (int c = 0; c < norb; ++c) x += MO[a * LM + c] * FO[c * 16 + comp];
+        oacc += x;
+      }
 #pragma unroll
       for (int n = 0; n < 3; ++n) {
         const int t = wv + n * nw;
         if (t >= ntask) continue;
-        const int rt = t / 3, ct = t % 3;
-        const double* M = ct < 2 ? MY : MO;
+        const int rt = t / 2, ct = t % 2;
         const int row = rt * 16 + (lane & 15), col = ct * 16 + (lane & 15);
         dbl4 acc = accs[n];
         for (int k0 = 0; k0 < NO; k0 += 4) {
           const int c = k0 + (lane >> 4);
-          const double av = M[row * LM + c];
-          const double bv = ct < 2 ? WV[b * 16 + (col % ks)] * FI[c * 32 + col] : FO[c * 16 + (lane & 15)];
+          const double av = MY[row * LM + c];
+          const double bv = WV[b * 16 + (col % ks)] * FI[c * 32 + col];
           acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
         }
         accs[n] = acc;
       }
     }
     // results: lane holds column (lane & 15) of its tile, rows rt*16 + 4j + lane/16
+    if ((int)threadIdx.x < 2 * norb)
+      gorb[I.off_orbv + ((long long)threadIdx.x * nv + iv) * np + m] = oacc;  // 2a+comp = t
 #pragma unroll
     for (int n = 0; n < 3; ++n) {
       const int t = wv + n * nw;
       if (t >= ntask) continue;
-      const int rt = t / 3, ct = t % 3;
+      const int rt = t / 2, ct = t % 2;
       const dbl4 acc = accs[n];
       const int col = ct * 16 + (lane & 15);
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int a = rt * 16 + 4 * j + (lane >> 4);
-        if (a >= norb) continue;
-        if (ct < 2) {
-          if (col < 2 * ks) {
-            const int comp = col / ks, i = col % ks;
-            gorb[I.off_orbv + ((long long)(2 * a + comp) * nv + iv) * np + ks + m * ks + i] = acc[j];
-          }
-        } else if ((lane & 15) < 2) {
-          gorb[I.off_orbv + ((long long)(2 * a + (lane & 15)) * nv + iv) * np + m] = acc[j];
-        }
+        if (a >= norb || col >= 2 * ks) continue;
+        const int comp = col / ks, i = col % ks;
+        gorb[I.off_orbv + ((long long)(2 * a + comp) * nv + iv) * np + ks + m * ks + i] = acc[j];
       }
     }
   }
@@ -1336,7 +1345,6 @@ struct dbs_engine {
   AdjChunk* d_chunks = nullptr;
   AdjEnt* d_ent = nullptr;
   long long* d_ecidx = nullptr;
-  double* d_cent = nullptr;
   long long nent = 0;
   int* d_alen = nullptr;  // adjacency length per density
   // work lists [adjoint][variant]: 0 partner panels read from global
@@ -1531,11 +1539,15 @@ static int build_pair_tables(dbs_engine* e) {
       C.len = len;
       C.pad = 0;
       chunks.push_back(C);
-      ent.resize((size_t)(base + 64LL * len), AdjEnt{-1, -1});
+      ent.resize((size_t)(base + 64LL * len), AdjEnt{-1, -1, 0.0});
       ecidx.resize((size_t)(base + 64LL * len), -1);
       for (int t = 0; t < 64 && 64 * c + t < G.ndens; ++t) {
         const int X = G.dens0 + 64 * c + t;
-        const auto& lst = e->h_adj[X];
+        // entries where X is the row's left density first (they also form
+        // the row value), each part in row order
+        std::vector<int2> lst;
+        for (const int2& a : e->h_adj[X]) if (a.x >= 0) lst.push_back(a);
+        for (const int2& a : e->h_adj[X]) if (a.x < 0) lst.push_back(a);
         for (size_t i = 0; i < lst.size(); ++i) {
           const int r = lst[i].y;
           const RowDev& R = e->h_rows[r];
@@ -1544,6 +1556,7 @@ static int build_pair_tables(dbs_engine* e) {
           AdjEnt A;
           A.P = partner - G.dens0;
           A.slot = left ? (e->group_dense[g] ? -2 : r - G.row0) : -1;
+          A.c = 0.0;
           ent[(size_t)(base + 64LL * (long long)i + t)] = A;
           ecidx[(size_t)(base + 64LL * (long long)i + t)] = e->h_row_out[r];
         }
@@ -1610,7 +1623,6 @@ static int build_pair_tables(dbs_engine* e) {
   dfree(&e->d_chunks);
   dfree(&e->d_ent);
   dfree(&e->d_ecidx);
-  dfree(&e->d_cent);
   dfree(&e->d_red);
   dfree(&e->d_part);
   dfree(&e->d_kg);
@@ -1633,8 +1645,6 @@ static int build_pair_tables(dbs_engine* e) {
     dfree(&e->d_alen);
     if (!rc) rc = upload(e->stream, &e->d_alen, alen);
   }
-  if (!rc && hipMalloc(&e->d_cent, sizeof(double) * (size_t)std::max<long long>(e->nent, 1)) != hipSuccess)
-    rc = fail(DBS_ERR_HIP, "pair tables: hipMalloc");
   if (!rc && hipMalloc(&e->d_part, sizeof(double) * (size_t)std::max<long long>(poff, 1)) != hipSuccess)
     rc = fail(DBS_ERR_HIP, "pair tables: hipMalloc");
   return rc;
@@ -1666,17 +1676,17 @@ static void launch_pair_w(dbs_engine* e, int adj, int var, hipStream_t s) {
       if (e->onesided) {
         if (adj)
           k_pair_rot<W, true, true><<<grid, b, shm, s>>>(n, wk, e->d_kg, e->d_inst, e->d_chunks, e->d_ent,
-                                                         e->d_alen, e->d_cent, e->F, e->G, e->d_part);
+                                                         e->d_alen, e->F, e->G, e->d_part);
         else
           k_pair_rot<W, true, false><<<grid, b, shm, s>>>(n, wk, e->d_kg, e->d_inst, e->d_chunks, e->d_ent,
-                                                          e->d_alen, e->d_cent, e->F, e->G, e->d_part);
+                                                          e->d_alen, e->F, e->G, e->d_part);
       } else {
         if (adj)
           k_pair_rot<W, false, true><<<grid, b, shm, s>>>(n, wk, e->d_kg, e->d_inst, e->d_chunks, e->d_ent,
-                                                          e->d_alen, e->d_cent, e->F, e->G, e->d_part);
+                                                          e->d_alen, e->F, e->G, e->d_part);
         else
           k_pair_rot<W, false, false><<<grid, b, shm, s>>>(n, wk, e->d_kg, e->d_inst, e->d_chunks, e->d_ent,
-                                                           e->d_alen, e->d_cent, e->F, e->G, e->d_part);
+                                                           e->d_alen, e->F, e->G, e->d_part);
       }
     }
     return;
@@ -1686,20 +1696,20 @@ static void launch_pair_w(dbs_engine* e, int adj, int var, hipStream_t s) {
   if (adj) {
     if (var == 1)
       k_pair_panel<W, true, true><<<grid, block, shm, s>>>(n, wk, e->d_kg, e->d_inst, e->d_chunks, e->d_ent,
-                                                           e->d_alen, e->d_cent, e->F, e->G, e->d_part,
+                                                           e->d_alen, e->F, e->G, e->d_part,
                                                            e->onesided);
     else
       k_pair_panel<W, false, true><<<grid, block, 0, s>>>(n, wk, e->d_kg, e->d_inst, e->d_chunks, e->d_ent,
-                                                          e->d_alen, e->d_cent, e->F, e->G, e->d_part,
+                                                          e->d_alen, e->F, e->G, e->d_part,
                                                           e->onesided);
   } else {
     if (var == 1)
       k_pair_panel<W, true, false><<<grid, block, shm, s>>>(n, wk, e->d_kg, e->d_inst, e->d_chunks, e->d_ent,
-                                                            e->d_alen, e->d_cent, e->F, e->G, e->d_part,
+                                                            e->d_alen, e->F, e->G, e->d_part,
                                                             e->onesided);
     else
       k_pair_panel<W, false, false><<<grid, block, 0, s>>>(n, wk, e->d_kg, e->d_inst, e->d_chunks, e->d_ent,
-                                                           e->d_alen, e->d_cent, e->F, e->G, e->d_part,
+                                                           e->d_alen, e->F, e->G, e->d_part,
                                                            e->onesided);
   }
 }
@@ -1725,7 +1735,7 @@ static int run_forward(dbs_engine* e, int with_adj) {
   k_weights<<<grid_for(e->total_w, 256, cap * 4), 256, 0, s>>>(e->nkg, e->d_seg_w, e->d_kg, e->d_inst,
                                                                 e->wt, e->total_w);
   if (with_adj && e->nent > 0)
-    k_adj_cgather<<<grid_for(e->nent, 256, cap * 4), 256, 0, s>>>(e->nent, e->d_ecidx, e->coef, e->d_cent);
+    k_adj_cgather<<<grid_for(e->nent, 256, cap * 4), 256, 0, s>>>(e->nent, e->d_ecidx, e->coef, e->d_ent);
   mark(e, "weights");
   if (e->featm_shm > 0)
     k_features_mfma<<<grid_for(e->nfitems, 4, cap * 4), 256, e->featm_shm, s>>>(
@@ -2243,7 +2253,7 @@ int dbs_engine_destroy(dbs_engine* e) {
                   e->d_seg_w, e->d_roff, e->d_items, e->d_scan_items, e->d_oadjoff, e->d_oadj,
                   e->d_tiles, e->d_pairrow, e->d_pairoff, e->d_nextrow, e->plain_d, e->plain_c,
                   e->d_echunks, e->d_echoff, e->d_epart, e->d_seg_adj, e->d_chunks, e->d_ent,
-                  e->d_ecidx, e->d_cent, e->d_red, e->d_part, e->d_alen, e->d_fitems, e->d_pidx, e->d_dpk,
+                  e->d_ecidx, e->d_red, e->d_part, e->d_alen, e->d_fitems, e->d_pidx, e->d_dpk,
                   e->d_work[0][0], e->d_work[0][1], e->d_work[0][2],
                   e->d_work[1][0], e->d_work[1][1], e->d_work[1][2]};
   for (void* p : bufs)
