@@ -869,6 +869,112 @@ __global__ void k_adj_orb_points(int ninst, const long long* __restrict__ seg,
   }
 }
 
+// Orbital values at the points of interval iv, one workgroup per
+// (instance, interval, component): the kx B-spline values at the np points
+// and the kx coefficients of every orbital are staged in LDS, then
+// f(o,p) = sum_j B_j(p) c_o(iv+j) with j ascending.
+__global__ void __launch_bounds__(256) k_orb_values(int ninst, const long long* __restrict__ seg,
+                                                   const InstDev* __restrict__ inst, const double* __restrict__ pq,
+                                                   double* __restrict__ orbv, long long total) {
+  extern __shared__ double sm[];
+  for (long long item = blockIdx.x; item < total; item += gridDim.x) {
+    const int s = find_seg(seg, ninst, item);
+    const InstDev I = inst[s];
+    const int r = (int)(item - seg[s]);
+    const int iv = r >> 1, comp = r & 1;
+    const int kx = comp == 0 ? I.ksp : I.ksq;
+    const int ks = I.ks, np = I.np, nv = I.nv, ns = I.ns, norb = I.norb, nw = nv + 1;
+    double* Bv = sm;              // [np][kx]
+    double* Cf = Bv + np * kx;    // [norb][kx]
+    const double* xb = comp == 0 ? I.pbsp : I.qbsp;
+    const double* bin = comp == 0 ? I.bpin : I.bqin;
+    __syncthreads();
+    for (int q = threadIdx.x; q < np * kx; q += blockDim.x) {
+      const int p = q / kx, j = q % kx;
+      Bv[q] = p < ks ? xb[iv + (long long)nw * (p + (long long)ks * j)]
+                     : bin[((long long)iv * ks * ks + (p - ks)) * kx + j];
+    }
+    for (int q = threadIdx.x; q < norb * kx; q += blockDim.x) {
+      const int o = q / kx, j = q % kx;
+      Cf[q] = pq[I.off_pq + (long long)ns * (comp + 2LL * o) + iv + j];
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < norb * np; q += blockDim.x) {
+      const int o = q / np, p = q % np;
+      double v = 0.0;
+      for (int j = 0; j < kx; ++j) v += Bv[p * kx + j] * Cf[o * kx + j];
+      orbv[I.off_orbv + ((long long)(2 * o + comp) * nv + iv) * np + p] = v;
+    }
+  }
+}
+
+// Projection of the point gradients onto the B-spline coefficients, first
+// half: per (instance, interval, component) workgroup the partial sums
+//   part(o, iv, jj) = sum_p B_jj(x_p) g_f(o, iv, p),  jj < kx
+// stored in rows of ks entries per (orbital, component, interval)
+// (point gradients and B-spline values staged in LDS).
+__global__ void __launch_bounds__(256) k_adj_proj(int ninst, const long long* __restrict__ seg,
+                                                 const InstDev* __restrict__ inst, const double* __restrict__ gorb,
+                                                 const long long* __restrict__ poff, double* __restrict__ part,
+                                                 long long total) {
+  extern __shared__ double sm[];
+  for (long long item = blockIdx.x; item < total; item += gridDim.x) {
+    const int s = find_seg(seg, ninst, item);
+    const InstDev I = inst[s];
+    const int r = (int)(item - seg[s]);
+    const int iv = r >> 1, comp = r & 1;
+    const int kx = comp == 0 ? I.ksp : I.ksq;
+    const int ks = I.ks, np = I.np, nv = I.nv, norb = I.norb, nw = nv + 1;
+    double* Bv = sm;              // [np][kx]
+    double* Gs = Bv + np * kx;    // [norb][np]
+    const double* xb = comp == 0 ? I.pbsp : I.qbsp;
+    const double* bin = comp == 0 ? I.bpin : I.bqin;
+    __syncthreads();
+    for (int q = threadIdx.x; q < np * kx; q += blockDim.x) {
+      const int p = q / kx, j = q % kx;
+      Bv[q] = p < ks ? xb[iv + (long long)nw * (p + (long long)ks * j)]
+                     : bin[((long long)iv * ks * ks + (p - ks)) * kx + j];
+    }
+    for (int q = threadIdx.x; q < norb * np; q += blockDim.x) {
+      const int o = q / np, p = q % np;
+      Gs[q] = gorb[I.off_orbv + ((long long)(2 * o + comp) * nv + iv) * np + p];
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < norb * kx; q += blockDim.x) {
+      const int o = q / kx, jj = q % kx;
+      double acc = 0.0;
+      for (int p = 0; p < np; ++p) acc += Bv[p * kx + jj] * Gs[o * np + p];
+      part[poff[s] + ((long long)(2 * o + comp) * nv + iv) * ks + jj] = acc;  // rows of ks >= kx
+    }
+  }
+}
+
+// second half: dE/dc(o, comp, j) = sum_jj part(o, iv = j - jj, jj), jj ascending
+__global__ void k_adj_coef_sum(int ninst, const long long* __restrict__ seg, const InstDev* __restrict__ inst,
+                               const long long* __restrict__ poff, const double* __restrict__ part,
+                               double* __restrict__ grad, long long total) {
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+       q += (long long)gridDim.x * blockDim.x) {
+    const int s = find_seg(seg, ninst, q);
+    const InstDev I = inst[s];
+    unsigned r = (unsigned)(q - seg[s]);
+    const int j = (int)(r % (unsigned)I.ns);
+    r /= (unsigned)I.ns;
+    const int comp = (int)(r & 1u);
+    const int o = (int)(r >> 1);
+    const int kx = comp == 0 ? I.ksp : I.ksq, nv = I.nv;
+    const int nsx = nv + kx - 1;
+    double acc = 0.0;
+    if (j < nsx) {
+      for (int jj = 0; jj < kx; ++jj) {
+        const int iv = j - jj;
+        if (iv >= 0 && iv < nv) acc += part[poff[s] + ((long long)(2 * o + comp) * nv + iv) * I.ks + jj];
+      }
+    }
+    grad[I.off_pq + q - seg[s]] = acc;
+  }
+}
+
 // adjoint phase D: dE/dcoef(j) = sum over intervals and points of B_j * g_f;
 // one wavefront per coefficient, lanes over (interval, point)
 __global__ void k_adj_coef(int ninst, const long long* __restrict__ seg, const InstDev* __restrict__ inst,
@@ -1334,6 +1440,11 @@ struct dbs_engine {
   int nfitems = 0;
   int* d_pidx = nullptr;     // orbital pair -> density per group and kind
   int* d_dpk = nullptr;      // a | c << 8 per density
+  long long* d_seg_ivc = nullptr;  // per instance start of its (interval, component) items
+  long long n_ivc = 0;
+  size_t ov_shm = 0, proj_shm = 0; // LDS of k_orb_values / k_adj_proj
+  long long* d_ppoff = nullptr;    // per instance offset of the projection partials
+  double* d_ppart = nullptr;
   size_t adjm_shm = 0;       // LDS of the matrix-core adjoint kernel (0: not usable)
   int featm_wave = 0;        // LDS doubles per wave of the matrix-core kernel
   int max_nd = 0;         // largest group
@@ -1729,8 +1840,12 @@ static int run_forward(dbs_engine* e, int with_adj) {
   hipStream_t s = e->stream;
   mark(e, "start");
   const int cap = e->tune[3];
-  k_orb_points<<<grid_for(e->total_orbv, 256, cap * 4), 256, 0, s>>>(
-      e->ninst, e->d_seg_orb, e->d_inst, e->pq, e->orbv, e->total_orbv);
+  if (e->ov_shm <= 64 * 1024)
+    k_orb_values<<<grid_for(e->n_ivc, 1, cap * 4), 256, e->ov_shm, s>>>(e->ninst, e->d_seg_ivc, e->d_inst, e->pq,
+                                                                       e->orbv, e->n_ivc);
+  else
+    k_orb_points<<<grid_for(e->total_orbv, 256, cap * 4), 256, 0, s>>>(
+        e->ninst, e->d_seg_orb, e->d_inst, e->pq, e->orbv, e->total_orbv);
   mark(e, "orb_points");
   k_weights<<<grid_for(e->total_w, 256, cap * 4), 256, 0, s>>>(e->nkg, e->d_seg_w, e->d_kg, e->d_inst,
                                                                 e->wt, e->total_w);
@@ -1792,8 +1907,16 @@ static int run_adjoint(dbs_engine* e) {
         e->G, e->gorb, e->total_orbv);
   }
   mark(e, "adj_orb_points");
-  k_adj_coef<<<grid_for(e->total_grad, 4, cap * 4), 256, 0, s>>>(e->ninst, e->d_seg_coef, e->d_inst,
-                                                                  e->gorb, e->grad, e->total_grad);
+  if (e->proj_shm <= 64 * 1024) {
+    k_adj_proj<<<grid_for(e->n_ivc, 1, cap * 4), 256, e->proj_shm, s>>>(e->ninst, e->d_seg_ivc, e->d_inst, e->gorb,
+                                                                       e->d_ppoff, e->d_ppart, e->n_ivc);
+    k_adj_coef_sum<<<grid_for(e->total_grad, 256, cap * 4), 256, 0, s>>>(e->ninst, e->d_seg_coef, e->d_inst,
+                                                                         e->d_ppoff, e->d_ppart, e->grad,
+                                                                         e->total_grad);
+  } else {
+    k_adj_coef<<<grid_for(e->total_grad, 4, cap * 4), 256, 0, s>>>(e->ninst, e->d_seg_coef, e->d_inst,
+                                                                    e->gorb, e->grad, e->total_grad);
+  }
   mark(e, "adj_coef");
   DBS_HIP(hipGetLastError());
   return DBS_OK;
@@ -2211,6 +2334,25 @@ int dbs_engine_create(dbs_engine** out, int ninst, dbs_ctx* const* ctxs, const i
     }
     e->adj_items = tot;
     rc |= upload(s, &e->d_seg_adj, seg_adj);
+    // (interval, component) items of the orbital-value and projection kernels
+    std::vector<long long> seg_ivc(ninst), ppoff(ninst);
+    long long ti = 0, tp = 0;
+    size_t ovn = 0, prn = 0;
+    for (int i = 0; i < ninst; ++i) {
+      const InstDev& I = e->h_inst[i];
+      seg_ivc[i] = ti;
+      ti += 2LL * I.nv;
+      ppoff[i] = tp;
+      tp += (long long)I.norb * 2 * I.nv * I.ks;
+      ovn = std::max(ovn, (size_t)I.np * I.ks + (size_t)I.norb * I.ks);
+      prn = std::max(prn, (size_t)I.np * I.ks + (size_t)I.norb * I.np);
+    }
+    e->n_ivc = ti;
+    e->ov_shm = ovn * sizeof(double);
+    e->proj_shm = prn * sizeof(double);
+    rc |= upload(s, &e->d_seg_ivc, seg_ivc);
+    rc |= upload(s, &e->d_ppoff, ppoff);
+    rc |= dalloc(&e->d_ppart, tp);
   }
   if (rc) return bail(rc);
   std::vector<long long> row_out(e->h_row_out);
@@ -2253,7 +2395,7 @@ int dbs_engine_destroy(dbs_engine* e) {
                   e->d_seg_w, e->d_roff, e->d_items, e->d_scan_items, e->d_oadjoff, e->d_oadj,
                   e->d_tiles, e->d_pairrow, e->d_pairoff, e->d_nextrow, e->plain_d, e->plain_c,
                   e->d_echunks, e->d_echoff, e->d_epart, e->d_seg_adj, e->d_chunks, e->d_ent,
-                  e->d_ecidx, e->d_red, e->d_part, e->d_alen, e->d_fitems, e->d_pidx, e->d_dpk,
+                  e->d_ecidx, e->d_red, e->d_part, e->d_alen, e->d_fitems, e->d_pidx, e->d_dpk, e->d_seg_ivc, e->d_ppoff, e->d_ppart,
                   e->d_work[0][0], e->d_work[0][1], e->d_work[0][2],
                   e->d_work[1][0], e->d_work[1][1], e->d_work[1][2]};
   for (void* p : bufs)
