@@ -1697,7 +1697,16 @@ static int build_pair_tables(dbs_engine* e) {
     KGroupDev& G = e->h_kg[g];
     const InstDev& I = e->h_inst[G.inst];
     const int J = I.nv * (1 + I.ks);
-    const int span = W * NP;
+    // partial values of the group's rows are bounded to kPartMax doubles:
+    // large row sets take more panels per workgroup
+    const long long kPartMax = 32LL << 20;
+    int np_g = NP;
+    if (!e->group_dense[g] && G.nrows > 0) {
+      const long long maxpr = std::max(1LL, kPartMax / G.nrows);
+      const long long need = (J + (long long)W * np_g - 1) / ((long long)W * np_g);
+      if (need > maxpr) np_g = (int)((J + W * maxpr - 1) / (W * maxpr));
+    }
+    const int span = W * np_g;
     const int npr = (J + span - 1) / span;
     int var = pair_lds_bytes(W, G.ndens) <= kPairLdsMax ? 1 : 0;
     if (W >= 8 && G.ndens <= rot_nd) var = 2;

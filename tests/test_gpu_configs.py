@@ -1,0 +1,231 @@
+"""The engine at the shapes of the integral-set configurations (all FP64):
+
+* config 1 -- 12 orbitals, every R^k(a,b;c,d) with a<=c, b<=d, k = 0..4
+  (30 420 rows) and the S^k ppqq / pqqp sets with the same indices, on the
+  reference's Fe.knot grid (ns=113) and on the Z=26 mkgrid_01 recipe grid
+  (ns=208: exponential step e^0.125 from 1e-3/Z, then 0.25 to r=30);
+* config 3 -- 256 uneven instances (ns 60..400, ks 7..9, kmax 2..10) in one
+  engine, one hipGraph, autotuned; orbitals per instance 4..8 (fewer than
+  the 4..24 of the full configuration, to keep the all-pairs row count at
+  about 2e6);
+* config 5 -- ns=600, 60 orbitals, k = 0..12, 1e6 rows, 20 SCF-style
+  iterations with perturbed orbitals replayed through one hipGraph.
+
+Checks are size independent where the oracle would take minutes: every row
+against the plain GPU path (materialised arrays + convol + SUM_AmB, itself
+bit-exact to the oracle), the symmetry R^k(a,b;c,d) = R^k(b,a;d,c),
+batched = single-instance results, E = sum c*R, central differences of E;
+plus a sample of rows against the oracle.  Relative errors use a floor of
+1e-3 of the set's largest magnitude (near-cancelling S^k entries).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import fe_grid
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+SEED = 20261203
+
+
+def relerr(x, ref):
+    ref = np.asarray(ref)
+    floor = 1e-3 * np.max(np.abs(ref))
+    return np.abs(np.asarray(x) - ref) / np.maximum(np.abs(ref), floor)
+
+
+def orbitals(g, n, seed):
+    """P,Q ~ N(0,1) on 2..nsp-1 / 2..nsq-1, normalised to quadr(f,f,0)=1"""
+    rng = np.random.default_rng(seed)
+    fs = []
+    for _ in range(n):
+        f = np.zeros((g.ns, 2), order="F")
+        f[1:g.nsp - 1, 0] = rng.standard_normal(g.nsp - 2)
+        f[1:g.nsq - 1, 1] = rng.standard_normal(g.nsq - 2)
+        f /= math.sqrt(g.quadr(f, f, 0))
+        fs.append(np.asfortranarray(f))
+    return fs
+
+
+def all_pair_rows(norb, kmax):
+    pairs = [(a, c) for a in range(1, norb + 1) for c in range(a, norb + 1)]
+    return np.array([(k, a, b, c, d) for k in range(kmax + 1) for (a, c) in pairs for (b, d) in pairs],
+                    dtype=np.int32)
+
+
+def oracle_rows(g, fs, rows, kind):
+    if kind == 0:
+        return O.rk_rows(g, fs, rows)
+    ns, ks = g.ns, g.ks
+    out = np.zeros(len(rows))
+    for n, (k, a, b, c, d) in enumerate(rows):
+        fa, fb, fc, fd = (fs[x - 1] for x in (a, b, c, d))
+        if kind == 5:  # ppqq: (P_a Q_c ; P_b Q_d)
+            di = O.density(ns, ks, fa[:, 0], fc[:, 1], "n")
+            dj = O.density(ns, ks, fb[:, 0], fd[:, 1], "n")
+        else:          # pqqp: (P_a Q_c ; Q_b P_d)
+            di = O.density(ns, ks, fa[:, 0], fc[:, 1], "n")
+            dj = O.density(ns, ks, fb[:, 1], fd[:, 0], "n")
+        conv = O.convol(ns, ks, g.mrk(kind, int(k)), di, 2, "n", "n")
+        out[n] = O.sum_amb(ns, ks, conv, dj, "n")
+    return out
+
+
+@pytest.fixture(scope="module", params=["fe", "recipe"])
+def config1_grid(request, gpu_available):
+    from dbsr3_amd import DBS
+    from dbsr3_amd.grid import mkgrid_01
+    if request.param == "fe":
+        t, ns, ks, ksp, ksq = fe_grid()
+    else:
+        t, ns = mkgrid_01(9, 26.0, hi=1e-3, he=math.exp(0.125) - 1.0, hmax=0.25, rmax=30.0)
+        ks, ksp, ksq = 9, 8, 9
+    g = O.Grid(t, ns, ks, ksp, ksq)
+    return request.param, g, DBS(t, ns, ks, ksp, ksq)
+
+
+@pytest.mark.parametrize("kind", [0, 5, 6])
+def test_config1_all_pairs(config1_grid, kind):
+    from dbsr3_amd import Engine
+    name, g, d = config1_grid
+    norb = 12
+    fs = orbitals(g, norb, SEED + 1)
+    rows = all_pair_rows(norb, 4)
+    assert len(rows) == 30420
+    print(f"[config1 {name} ns={g.ns} kind={kind} rows={len(rows)}]", flush=True)
+    eng = Engine([(d, norb, rows)], kind=kind)
+    eng.set_orbitals(0, np.stack(fs, axis=2))
+    coef = np.random.default_rng(SEED + 1).uniform(-1, 1, len(rows))
+    eng.set_coef(0, coef)
+    eng.forward_adjoint()
+    vals = eng.values()
+    E = eng.energy()
+    assert abs(E - vals @ coef) <= TOL * np.sum(np.abs(vals * coef))
+    assert np.all(np.isfinite(eng.grad()))
+    eng.forward_plain()
+    plain = eng.values()
+    e = relerr(vals, plain)
+    print(f"  vs plain path: max {e.max():.2e} median {np.median(e):.2e}")
+    assert e.max() < TOL
+    if kind == 0:
+        # R^k(a,b;c,d) = R^k(b,a;d,c): the row with the pairs swapped
+        idx = {tuple(r): n for n, r in enumerate(rows.tolist())}
+        sw = np.array([idx[(k, b, a, d_, c)] for (k, a, b, c, d_) in rows.tolist()])
+        assert np.max(relerr(vals[sw], vals)) < 1e-13
+    sel = np.nonzero(rows[:, 0] == 2)[0][::97][:40]
+    ref = oracle_rows(g, fs, rows[sel], kind)
+    assert relerr(vals[sel], ref).max() < TOL
+
+
+def test_config3_batched_uneven_instances(gpu_available):
+    from dbsr3_amd import DBS, Engine
+    from dbsr3_amd.grid import mkgrid_02
+    rng = np.random.default_rng(SEED + 3)
+    ninst = 256
+    spec = []
+    for _ in range(ninst):
+        ns = int(rng.integers(60, 401))
+        ks = int(rng.choice([7, 8, 9]))
+        kmax = int(rng.integers(2, 11))
+        norb = int(rng.integers(4, 9))
+        spec.append((ns, ks, kmax, norb))
+    insts, grids, pqs, coefs = [], [], [], []
+    for i, (ns, ks, kmax, norb) in enumerate(spec):
+        t = mkgrid_02(ns, ks, 26.0, 0.25, 30.0)
+        g = O.Grid(t, ns, ks, ks - 1, ks)
+        fs = orbitals(g, norb, SEED + 3 + 1000 * i)
+        rows = all_pair_rows(norb, kmax)
+        d = DBS(t, ns, ks, ks - 1, ks)
+        insts.append((d, norb, rows))
+        grids.append((g, fs))
+        pqs.append(np.stack(fs, axis=2))
+        coefs.append(rng.uniform(-1, 1, len(rows)))
+    nrows = sum(len(r) for _, _, r in insts)
+    print(f"[config3 {ninst} instances, {nrows} rows]", flush=True)
+    eng = Engine(insts)
+    for i in range(ninst):
+        eng.set_orbitals(i, pqs[i])
+        eng.set_coef(i, coefs[i])
+    tuning = eng.autotune(with_adjoint=True, reps=2)
+    print(f"  tuning {tuning}", flush=True)
+    eng.graph_capture(with_adjoint=True)
+    eng.graph_launch()
+    eng.sync()
+    vals = [eng.values(i) for i in range(ninst)]
+    grads = [eng.grad(i) for i in range(ninst)]
+    Es = [eng.energy(i) for i in range(ninst)]
+    for i in range(ninst):
+        assert abs(Es[i] - vals[i] @ coefs[i]) <= TOL * np.sum(np.abs(vals[i] * coefs[i]))
+    # batched = single instance, same launch shape
+    for i in rng.choice(ninst, 6, replace=False):
+        single = Engine([insts[i]])
+        single.set_orbitals(0, pqs[i])
+        single.set_coef(0, coefs[i])
+        single.set_tuning(tuning)
+        single.forward_adjoint()
+        assert np.max(relerr(single.values(), vals[i])) < 1e-14, i
+        gs = np.max(np.abs(grads[i]))
+        assert np.max(np.abs(single.grad() - grads[i])) <= 1e-14 * gs, i
+        single.close()
+    # oracle on rows of the three smallest grids, one multipole each
+    for i in sorted(range(ninst), key=lambda j: spec[j][0])[:3]:
+        g, fs = grids[i]
+        rows = insts[i][2]
+        sel = np.nonzero(rows[:, 0] == 1)[0][::11][:30]
+        ref = O.rk_rows(g, fs, rows[sel])
+        assert relerr(vals[i][sel], ref).max() < TOL, i
+
+
+def test_config5_scf_stress(gpu_available):
+    from dbsr3_amd import DBS, Engine
+    from dbsr3_amd.grid import mkgrid_02
+    ns, ks, ksp, ksq, norb, kmax, nrows = 600, 9, 8, 9, 60, 12, 1_000_000
+    t = mkgrid_02(ns, ks, 26.0, 0.25, 30.0)
+    g = O.Grid(t, ns, ks, ksp, ksq)
+    fs0 = orbitals(g, norb, SEED + 5)
+    rng = np.random.default_rng(SEED + 5)
+    rows = np.zeros((nrows, 5), dtype=np.int32)
+    rows[:, 0] = rng.integers(0, kmax + 1, nrows)
+    rows[:, 1:] = rng.integers(1, norb + 1, (nrows, 4))
+    coef = rng.uniform(-1, 1, nrows)
+    d = DBS(t, ns, ks, ksp, ksq)
+    print(f"[config5 ns={ns} norb={norb} rows={nrows}]", flush=True)
+    eng = Engine([(d, norb, rows)])
+    pq0 = np.stack(fs0, axis=2)
+    eng.set_orbitals(0, pq0)
+    eng.set_coef(0, coef)
+    eng.graph_capture(with_adjoint=True)
+    # deterministic perturbation per iteration (stays in the B-spline space,
+    # first and last coefficients zero)
+    bump = np.zeros_like(pq0)
+    bump[1:-2] = np.sin(np.arange(1, ns - 2))[:, None, None] * 1e-3
+    for it in range(20):
+        pq = pq0 + it * bump
+        eng.set_orbitals(0, pq)
+        eng.graph_launch()
+    eng.sync()
+    fs = [np.asfortranarray(pq[:, :, o]) for o in range(norb)]
+    vals = eng.values()
+    E = eng.energy()
+    assert abs(E - vals @ coef) <= TOL * np.sum(np.abs(vals * coef))
+    sel = np.nonzero(rows[:, 0] == 3)[0][::4999][:40]
+    ref = O.rk_rows(g, fs, rows[sel])
+    e = relerr(vals[sel], ref)
+    print(f"  vs oracle: max {e.max():.2e}", flush=True)
+    assert e.max() < TOL
+    # central differences of E on three coefficients
+    grad = eng.grad()
+    h = 1e-5
+    for (j, comp, o) in [(40, 0, 3), (300, 1, 17), (520, 0, 59)]:
+        e_pm = []
+        for sgn in (1, -1):
+            p2 = pq.copy(order="F")
+            p2[j, comp, o] += sgn * h
+            eng.set_orbitals(0, p2)
+            eng.forward()
+            e_pm.append(eng.energy())
+        fd = (e_pm[0] - e_pm[1]) / (2 * h)
+        assert abs(fd - grad[j, comp, o]) <= 1e-6 * max(abs(fd), 1e-3 * np.max(np.abs(grad))), (j, comp, o)
