@@ -42,6 +42,7 @@ PHASE_KERNEL = {
 }
 # rocprofv3 FETCH_SIZE/WRITE_SIZE summary of this workload (HBM bytes per
 # dispatch, copy-kernel scaled); rows "| kernel | tuning | ... | bytes |"
+# (the kernel name carries the panel width, the tuning column is informative)
 TRAFFIC_FILE = "engine_rk_fwd_adj_ns200_norb30_rows5e4_pmc.md"
 
 
@@ -61,15 +62,15 @@ def pair_kernel(rows, tuning):
 
 
 def measured_traffic(path, kernel, tuning):
-    """HBM bytes per launch of `kernel` from the committed counter summary,
-    if it was measured with the same tuning; else None."""
+    """HBM bytes per launch of `kernel` (name with its template arguments)
+    from the committed counter summary of this workload, else None."""
     try:
         with open(path) as f:
             for line in f:
                 cells = [c.strip() for c in line.strip().strip("|").split("|")]
-                if len(cells) >= 6 and cells[0] == kernel and cells[1] == ",".join(map(str, tuning)):
+                if len(cells) >= 6 and cells[0] == kernel:
                     return float(cells[-1])
-    except OSError:
+    except (OSError, ValueError):
         pass
     return None
 
