@@ -9,10 +9,8 @@ interface for Python callers and tests; it has no CPU fallback.
 """
 from .capi import DBSError, check, lib  # noqa: F401
 from .dbs import DBS  # noqa: F401
-try:  # pragma: no cover - filled in as the engine lands
-    from .engine import Engine  # noqa: F401
-    from .cdata import CData, decode_int  # noqa: F401
-except ImportError:
-    pass
+from .engine import Engine  # noqa: F401
+from .cdata import CData, decode_int  # noqa: F401
+from .matrix import MatBlocks  # noqa: F401
 
-__all__ = ["DBS", "Engine", "CData", "decode_int", "DBSError", "lib", "check"]
+__all__ = ["DBS", "Engine", "CData", "decode_int", "MatBlocks", "DBSError", "lib", "check"]

@@ -20,6 +20,18 @@ ch = C.c_char
 sz = C.c_size_t
 
 GEN_MATRIX_FN = C.CFUNCTYPE(None, vp, i32, i32, i32, dp, ip, ip, ip, ip)
+KBORT_FN = C.CFUNCTYPE(i32, vp, i32, i32)
+
+
+class MatBlocksC(C.Structure):
+    """struct dbs_mat_blocks (MAT3 matrix storage descriptor)"""
+    _fields_ = [("nch", i32), ("npert", i32), ("mk", i32),
+                ("icc", ip), ("nblocks", i32), ("hch_dev", vp), ("acf_dev", vp),
+                ("icb", ip), ("ncb", i32), ("hcp_dev", vp),
+                ("ibb", ip), ("nbb", i32), ("hp_dev", vp)]
+
+
+mbp = C.POINTER(MatBlocksC)
 
 # name -> argtypes (all return int status unless listed in _RESTYPE)
 PROTOTYPES = {
@@ -60,6 +72,16 @@ PROTOTYPES = {
     "dbs_rk_evaluate_dev": [vp, i32, i32, i32, dp, ip, ip, ip, ip, f64, i32, vp, i32, vp, i32,
                             i32, vp],
     "dbs_rdata_34_dev": [vp, i32, i32, i32, i32, dp, ip, ip, ip, i32, vp, i32, ip, i32, vp, i32],
+    "dbs_bspline_overlaps_dev": [vp, vp, vp, vp],
+    "dbs_full_mat_sym_dev": [vp, vp, ch, vp],
+    "dbs_update_hx_dev": [vp, mbp, i32, i32, vp, f64],
+    "dbs_update_hw_dev": [vp, mbp, i32, i32, vp, vp, f64],
+    "dbs_update_hv_dev": [vp, mbp, i32, i32, vp, f64],
+    "dbs_update_hb_dev": [vp, mbp, i32, ip, ip, dp],
+    "dbs_update_acf_dev": [vp, mbp, i32, ip, ip, ip, dp],
+    "dbs_o_data_dev": [vp, mbp, i32, i32, ip, ip, dp, f64, i32, ip, KBORT_FN, vp, vp, i32, vp, i32,
+                       ip, dp],
+    "dbs_overlap_ci_dev": [vp, i32, ip, ip, dp, f64, i32, vp, i32],
     "dbs_engine_create": [C.POINTER(vp), i32, C.POINTER(vp), ip, ip, ip, i32],
     "dbs_engine_destroy": [vp],
     "dbs_engine_buffers": [vp, i32, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp),

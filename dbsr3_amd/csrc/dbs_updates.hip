@@ -1,0 +1,550 @@
+// dbs_updates.hip -- B-spline overlap matrices, band unfolding and the
+// remaining matrix updates of the integral path:
+//   ZINTYM overlaps fpbs/fqbs and the block matrix fppqq
+//     (DBSR_HF/dbsr_lib_dbs.f90:4472-4497; Def_pbsp/Def_qbsp :1026, :1072;
+//      Def_pq_arrays :1078-1102)
+//   Full_mat_sym (DBSR_HF/dbsr_lib_zcom.f90:1247-1294)
+//   UPDATE_HX / HW / HV / HB / ACF (DBSR_MAT3/updates.f90:79-106, 183-292)
+//   O_data (DBSR_MAT3/O_data.f90:2-93) and the CI overlap SM(i,j) += CC
+//     (DBSR_CI3/setup_matrix.f90:108-120)
+// Loop and summation orders follow the reference with FMA contraction off,
+// so every result is bit-identical to it.  The updates of one matrix element
+// are applied by one thread in list order: no atomics, no reassociation.
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "dbs_internal.h"
+
+
+namespace dbs {
+
+// ym(i,j) = sum_iv sum_m ygr(iv,m) b1(iv,m,ith) b2(iv,m,jth), i = iv+ith-1,
+// j = iv+jth-1.  One thread per (i,j); intervals ascending, points in order
+// (the reference's SUM of the elementwise product (ygr*b1)*b2).  ym is
+// (mm,mm) column-major, only its leading (ns,ns) part is written.
+__global__ void k_zintym(int nv, int ks, int ks1, int ks2, const double* __restrict__ b1,
+                         const double* __restrict__ b2, const double* __restrict__ ygr, int ns, int mm,
+                         double* __restrict__ ym) {
+#pragma clang fp contract(off)
+  const long long total = (long long)ns * ns;
+  const long long nw = nv + 1;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+       q += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(q % ns), j = (int)(q / ns);
+    double acc = 0.0;
+    const int lo = max(max(i - ks1 + 1, j - ks2 + 1), 0), hi = min(min(i, j), nv - 1);
+    for (int iv = lo; iv <= hi; ++iv) {
+      const int ith = i - iv, jth = j - iv;
+      double s = 0.0;
+      for (int m = 0; m < ks; ++m) {
+        const double y = ygr[iv + (long long)nv * m];
+        s = s + y * b1[iv + nw * (m + (long long)ks * ith)] * b2[iv + nw * (m + (long long)ks * jth)];
+      }
+      acc = acc + s;
+    }
+    ym[i + (long long)mm * j] = acc;
+  }
+}
+
+// zero the off-diagonal (ns,ns) blocks of a (2ns,2ns) matrix
+__global__ void k_zero_offdiag(int ns, int mm, double* __restrict__ y) {
+  const long long total = (long long)mm * mm;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+       q += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(q % mm), j = (int)(q / mm);
+    if ((i >= ns) != (j >= ns)) y[q] = 0.0;
+  }
+}
+
+// y(ns,ns) from storage sym 0 's', 1 'l', 2 'n', 3 'x'
+__global__ void k_full_mat_sym(int ns, int ks, const double* __restrict__ x, int sym, double* __restrict__ y) {
+  const long long total = (long long)ns * ns;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+       q += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(q % ns), j = (int)(q / ns);
+    double v = 0.0;
+    if (sym == 3) {
+      v = x[q];
+    } else if (sym == 2) {
+      const int jp = j - i + ks - 1;  // x(i, jp+1), j = i + (jp+1) - ks
+      if (jp >= 0 && jp < 2 * ks - 1) v = x[i + (long long)ns * jp];
+    } else {
+      const int a = min(i, j), b = max(i, j), d = b - a;
+      if (d < ks) v = sym == 0 ? x[a + (long long)ns * d]             // 's': x(a, d+1)
+                               : x[b + (long long)ns * (ks - 1 - d)];  // 'l': x(b, ks-d)
+    }
+    y[q] = v;
+  }
+}
+
+// block += C*d (trans 0) or C*d^T (trans 1)
+__global__ void k_update_hx(int ms, double* __restrict__ blk, const double* __restrict__ d, int trans, double C) {
+#pragma clang fp contract(off)
+  const long long total = (long long)ms * ms;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+       q += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(q % ms), j = (int)(q / ms);
+    const double v = trans ? d[j + (long long)ms * i] : d[q];
+    blk[q] = blk[q] + C * v;
+  }
+}
+
+__device__ __forceinline__ double hw_term(double vi, double wj, double vj, double wi, int mode, double C) {
+#pragma clang fp contract(off)
+  const double x = vi * wj;
+  if (mode == 0) return C * x;
+  const double xt = vj * wi;  // TRANSPOSE(x)(i,j) = v(j)*w(i)
+  if (mode == 1) return C * xt;
+  return (C / 2.0) * (x + xt);
+}
+
+// block += C v w^T (mode 0), C (v w^T)^T (1), (C/2)(x + x^T) (2)
+__global__ void k_update_hw(int ms, double* __restrict__ blk, const double* __restrict__ v,
+                            const double* __restrict__ w, int mode, double C) {
+#pragma clang fp contract(off)
+  const long long total = (long long)ms * ms;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+       q += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(q % ms), j = (int)(q / ms);
+    blk[q] = blk[q] + hw_term(v[i], w[j], v[j], w[i], mode, C);
+  }
+}
+
+// col += C v
+__global__ void k_update_hv(int ms, double* __restrict__ col, const double* __restrict__ v, double C) {
+#pragma clang fp contract(off)
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ms; i += gridDim.x * blockDim.x)
+    col[i] = col[i] + C * v[i];
+}
+
+// out[tpos[t]] += vals[e] for the entries e of target t, in list order
+__global__ void k_accumulate_list(int ntarget, const long long* __restrict__ tpos,
+                                  const int* __restrict__ toff, const double* __restrict__ vals,
+                                  double* __restrict__ out) {
+#pragma clang fp contract(off)
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < ntarget; t += gridDim.x * blockDim.x) {
+    double acc = out[tpos[t]];
+    for (int e = toff[t]; e < toff[t + 1]; ++e) acc = acc + vals[e];
+    out[tpos[t]] = acc;
+  }
+}
+
+// One channel-block update of a list: HX with the matrix d (mode = trans)
+// or HW with columns v,w of V (mode 0/1/2).
+struct BlkUpd {
+  int hw, mode, v, w;
+  double C;
+};
+
+// Block targets on blockIdx.y (strided), elements on x.  Each element
+// applies its target's entries in list order.
+__global__ void k_block_list(int ms, int ntarget, const int* __restrict__ tblk, const int* __restrict__ toff,
+                             const BlkUpd* __restrict__ ent, const double* __restrict__ d,
+                             const double* __restrict__ V, int ldv, double* __restrict__ hch) {
+#pragma clang fp contract(off)
+  const long long total = (long long)ms * ms;
+  for (int t = blockIdx.y; t < ntarget; t += gridDim.y) {
+    double* blk = hch + total * tblk[t];
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+         q += (long long)gridDim.x * blockDim.x) {
+      const int i = (int)(q % ms), j = (int)(q / ms);
+      double acc = blk[q];
+      for (int e = toff[t]; e < toff[t + 1]; ++e) {
+        const BlkUpd u = ent[e];
+        if (u.hw) {
+          const double* v = V + (long long)ldv * u.v;
+          const double* w = V + (long long)ldv * u.w;
+          acc = acc + hw_term(v[i], w[j], v[j], w[i], u.mode, u.C);
+        } else {
+          const double x = u.mode ? d[j + (long long)ms * i] : d[q];
+          acc = acc + u.C * x;
+        }
+      }
+      blk[q] = acc;
+    }
+  }
+}
+
+// column targets: col(tcol[t]) += C_e V(:,vcol_e) in list order
+__global__ void k_vec_list(int ms, int ntarget, const int* __restrict__ tcol, const int* __restrict__ toff,
+                           const int* __restrict__ vcol, const double* __restrict__ cval,
+                           const double* __restrict__ V, int ldv, double* __restrict__ hcp) {
+#pragma clang fp contract(off)
+  for (int t = blockIdx.y; t < ntarget; t += gridDim.y) {
+    double* col = hcp + (long long)ms * tcol[t];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ms; i += gridDim.x * blockDim.x) {
+      double acc = col[i];
+      for (int e = toff[t]; e < toff[t + 1]; ++e) acc = acc + cval[e] * V[i + (long long)ldv * vcol[e]];
+      col[i] = acc;
+    }
+  }
+}
+
+namespace {
+
+unsigned grid_n(long long n, int block) {
+  long long g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > 65535) g = 65535;
+  return (unsigned)g;
+}
+
+int sym_code(char sym) {
+  switch (sym) {
+    case 's': return 0;
+    case 'l': return 1;
+    case 'n': return 2;
+    case 'x': return 3;
+    default: return -1;
+  }
+}
+
+// icc(ich,jch) with the reference's checks (updates.f90:89-103); *b = 0
+// when the block is not stored here
+int chan_block(const dbs_mat_blocks* m, int ich, int jch, const char* who, int* b) {
+  const std::string w(who);
+  if (ich < 0) return fail(DBS_ERR_ARG, w + ": ich < 0");
+  if (jch < 0) return fail(DBS_ERR_ARG, w + ": jch < 0");
+  if (ich > m->nch) return fail(DBS_ERR_ARG, w + ": ich > nch");
+  if (jch > m->nch) return fail(DBS_ERR_ARG, w + ": jch > nch");
+  if (ich == 0 || jch == 0) return fail(DBS_ERR_ARG, w + ": channel index 0");
+  const int i = m->icc[(ich - 1) + (long long)m->nch * (jch - 1)];
+  if (i < 0 || i > m->nblocks) return fail(DBS_ERR_ARG, "UPDATE_HX: i out of range");
+  *b = i;
+  return DBS_OK;
+}
+
+int check_blocks(const dbs_mat_blocks* m) {
+  DBS_CHECK(m, DBS_ERR_ARG, "matrix storage: null descriptor");
+  DBS_CHECK(m->nch >= 0 && m->npert >= 0 && m->nblocks >= 0 && m->ncb >= 0 && m->nbb >= 0, DBS_ERR_ARG,
+            "matrix storage: negative dimension");
+  return DBS_OK;
+}
+
+// device copies of host lists, freed after the stream has drained
+struct Uploads {
+  std::vector<void*> p;
+  hipStream_t s;
+  explicit Uploads(hipStream_t st) : s(st) {}
+  ~Uploads() {
+    if (!p.empty()) (void)hipStreamSynchronize(s);
+    for (void* x : p) (void)hipFree(x);
+  }
+  template <class T>
+  int put(const std::vector<T>& h, T** d) {
+    *d = nullptr;
+    if (h.empty()) return DBS_OK;
+    void* x = nullptr;
+    if (hipMalloc(&x, sizeof(T) * h.size()) != hipSuccess) return fail(DBS_ERR_HIP, "list upload: hipMalloc");
+    p.push_back(x);
+    if (hipMemcpyAsync(x, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, s) != hipSuccess)
+      return fail(DBS_ERR_HIP, "list upload: hipMemcpyAsync");
+    *d = (T*)x;
+    return DBS_OK;
+  }
+};
+
+int finish(hipStream_t s, const char* who) {
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return fail(DBS_ERR_HIP, std::string(who) + ": " + hipGetErrorString(e));
+  return DBS_OK;
+}
+
+// stable grouping of entries by target: order = permutation, tval = the
+// distinct targets ascending, toff = their entry ranges in the permutation
+void group_by(const std::vector<long long>& tgt, std::vector<int>& order, std::vector<long long>& tval,
+              std::vector<int>& toff) {
+  order.clear();
+  for (size_t e = 0; e < tgt.size(); ++e)
+    if (tgt[e] >= 0) order.push_back((int)e);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return tgt[a] < tgt[b]; });
+  tval.clear();
+  toff.clear();
+  for (size_t q = 0; q < order.size(); ++q) {
+    if (q == 0 || tgt[order[q]] != tgt[order[q - 1]]) {
+      tval.push_back(tgt[order[q]]);
+      toff.push_back((int)q);
+    }
+  }
+  toff.push_back((int)order.size());
+}
+
+// out[target[e]] += val[e] over the entries with target >= 0, list order
+// kept inside each target
+int accumulate_list(hipStream_t s, const std::vector<long long>& target, const double* val, double* out,
+                    const char* who) {
+  std::vector<int> order, toff;
+  std::vector<long long> tpos;
+  group_by(target, order, tpos, toff);
+  if (tpos.empty()) return DBS_OK;
+  std::vector<double> v(order.size());
+  for (size_t q = 0; q < order.size(); ++q) v[q] = val[order[q]];
+  Uploads up(s);
+  long long* d_tpos;
+  int* d_toff;
+  double* d_v;
+  int rc = up.put(tpos, &d_tpos);
+  if (!rc) rc = up.put(toff, &d_toff);
+  if (!rc) rc = up.put(v, &d_v);
+  if (rc) return rc;
+  const int nt = (int)tpos.size();
+  k_accumulate_list<<<grid_n(nt, 256), 256, 0, s>>>(nt, d_tpos, d_toff, d_v, out);
+  return finish(s, who);
+}
+
+int hb_target(const dbs_mat_blocks* m, int ic, int jc, long long* t) {
+  if (ic > m->npert || jc > m->npert || ic <= 0 || jc <= 0)
+    return fail(DBS_ERR_ARG, "UPDATE_HB: index is out of range");
+  const int i = m->ibb[(ic - 1) + (long long)m->npert * (jc - 1)];
+  if (i < 0 || i > m->nbb) return fail(DBS_ERR_ARG, "UPDATE_HB: ic, jc out of range");
+  *t = i - 1;  // -1: not stored
+  return DBS_OK;
+}
+
+int hv_column(const dbs_mat_blocks* m, int ich, int ip, int* col) {
+  if (ich < 1 || ich > m->nch) return fail(DBS_ERR_ARG, "UPDATE_HV: channel index out is of range");
+  if (ip < 1 || ip > m->npert) return fail(DBS_ERR_ARG, "UPDATE_HV: bound index out is of range");
+  const int i = m->icb[(ich - 1) + (long long)m->nch * (ip - 1)];
+  if (i < 0 || i > m->ncb) return fail(DBS_ERR_ARG, "UPDATE_HV: ich, ip out of range");
+  *col = i;
+  return DBS_OK;
+}
+
+}  // namespace
+}  // namespace dbs
+
+using namespace dbs;
+
+extern "C" {
+
+int dbs_bspline_overlaps_dev(dbs_ctx* ctx, double* fpbs_dev, double* fqbs_dev, double* fppqq_dev) {
+  DBS_CHECK(ctx, DBS_ERR_ARG, "dbs_bspline_overlaps_dev: null context");
+  const int ns = ctx->ns, ms = ctx->ms, nv = ctx->nv, ks = ctx->ks;
+  hipStream_t s = ctx->stream;
+  const unsigned g = grid_n((long long)ns * ns, 256);
+  if (fpbs_dev) k_zintym<<<g, 256, 0, s>>>(nv, ks, ctx->ksp, ctx->ksp, ctx->pbsp, ctx->pbsp, ctx->grw, ns, ns, fpbs_dev);
+  if (fqbs_dev) k_zintym<<<g, 256, 0, s>>>(nv, ks, ctx->ksq, ctx->ksq, ctx->qbsp, ctx->qbsp, ctx->grw, ns, ns, fqbs_dev);
+  if (fppqq_dev) {
+    // fpbs in the PP block, fqbs in the QQ block, zeros elsewhere
+    k_zero_offdiag<<<grid_n((long long)ms * ms, 256), 256, 0, s>>>(ns, ms, fppqq_dev);
+    k_zintym<<<g, 256, 0, s>>>(nv, ks, ctx->ksp, ctx->ksp, ctx->pbsp, ctx->pbsp, ctx->grw, ns, ms, fppqq_dev);
+    k_zintym<<<g, 256, 0, s>>>(nv, ks, ctx->ksq, ctx->ksq, ctx->qbsp, ctx->qbsp, ctx->grw, ns, ms,
+                               fppqq_dev + ns + (size_t)ms * ns);
+  }
+  DBS_HIP(hipGetLastError());
+  return DBS_OK;
+}
+
+int dbs_full_mat_sym_dev(dbs_ctx* ctx, const double* x_dev, char sym, double* y_dev) {
+  DBS_CHECK(ctx && x_dev && y_dev, DBS_ERR_ARG, "Full_mat_sym: null argument");
+  const int sc = sym_code(sym);
+  DBS_CHECK(sc >= 0, DBS_ERR_ARG, "Full_mat_sym: unknown storage mode");
+  const int ns = ctx->ns;
+  k_full_mat_sym<<<grid_n((long long)ns * ns, 256), 256, 0, ctx->stream>>>(ns, ctx->ks, x_dev, sc, y_dev);
+  DBS_HIP(hipGetLastError());
+  return DBS_OK;
+}
+
+int dbs_update_hx_dev(dbs_ctx* ctx, const dbs_mat_blocks* m, int ich, int jch, const double* d_dev, double C) {
+  DBS_CHECK(ctx && d_dev, DBS_ERR_ARG, "UPDATE_HX: null argument");
+  int rc = check_blocks(m), b = 0;
+  if (!rc) rc = chan_block(m, ich, jch, "UPDATE_HX", &b);
+  if (rc) return rc;
+  if (b == 0) return DBS_OK;
+  const int ms = ctx->ms;
+  k_update_hx<<<grid_n((long long)ms * ms, 256), 256, 0, ctx->stream>>>(
+      ms, m->hch_dev + (size_t)ms * ms * (b - 1), d_dev, ich >= jch ? 0 : 1, C);
+  DBS_HIP(hipGetLastError());
+  return DBS_OK;
+}
+
+int dbs_update_hw_dev(dbs_ctx* ctx, const dbs_mat_blocks* m, int ich, int jch, const double* v_dev,
+                      const double* w_dev, double C) {
+  DBS_CHECK(ctx && v_dev && w_dev, DBS_ERR_ARG, "UPDATE_HW: null argument");
+  int rc = check_blocks(m), b = 0;
+  if (!rc) rc = chan_block(m, ich, jch, "UPDATE_HW", &b);
+  if (rc) return rc;
+  if (b == 0) return DBS_OK;
+  const int ms = ctx->ms;
+  const int mode = ich > jch ? 0 : (jch > ich ? 1 : 2);
+  k_update_hw<<<grid_n((long long)ms * ms, 256), 256, 0, ctx->stream>>>(
+      ms, m->hch_dev + (size_t)ms * ms * (b - 1), v_dev, w_dev, mode, C);
+  DBS_HIP(hipGetLastError());
+  return DBS_OK;
+}
+
+int dbs_update_hv_dev(dbs_ctx* ctx, const dbs_mat_blocks* m, int ich, int ip, const double* v_dev, double C) {
+  DBS_CHECK(ctx && v_dev, DBS_ERR_ARG, "UPDATE_HV: null argument");
+  int rc = check_blocks(m), col = 0;
+  if (!rc) rc = hv_column(m, ich, ip, &col);
+  if (rc) return rc;
+  if (col == 0) return DBS_OK;
+  const int ms = ctx->ms;
+  k_update_hv<<<grid_n(ms, 256), 256, 0, ctx->stream>>>(ms, m->hcp_dev + (size_t)ms * (col - 1), v_dev, C);
+  DBS_HIP(hipGetLastError());
+  return DBS_OK;
+}
+
+int dbs_update_hb_dev(dbs_ctx* ctx, const dbs_mat_blocks* m, int n, const int* ic, const int* jc,
+                      const double* c) {
+  DBS_CHECK(ctx && (n <= 0 || (ic && jc && c)), DBS_ERR_ARG, "UPDATE_HB: null argument");
+  int rc = check_blocks(m);
+  if (rc || n <= 0) return rc;
+  std::vector<long long> t(n);
+  for (int e = 0; e < n; ++e)
+    if ((rc = hb_target(m, ic[e], jc[e], &t[e]))) return rc;
+  return accumulate_list(ctx->stream, t, c, m->hp_dev, "UPDATE_HB");
+}
+
+int dbs_update_acf_dev(dbs_ctx* ctx, const dbs_mat_blocks* m, int n, const int* k, const int* ich,
+                       const int* jch, const double* C) {
+  DBS_CHECK(ctx && (n <= 0 || (k && ich && jch && C)), DBS_ERR_ARG, "Update_cf: null argument");
+  int rc = check_blocks(m);
+  if (rc || n <= 0) return rc;
+  std::vector<long long> t(n);
+  for (int e = 0; e < n; ++e) {
+    if (k[e] > m->mk) return fail(DBS_ERR_ARG, "Update_cf: k > kmk ");
+    if (k[e] < 0) return fail(DBS_ERR_ARG, "Update_cf: k < 0");
+    int b = 0;
+    if ((rc = chan_block(m, ich[e], jch[e], "Update_cf", &b))) return rc;
+    t[e] = b == 0 ? -1 : (long long)(b - 1) + (long long)m->nblocks * k[e];
+  }
+  return accumulate_list(ctx->stream, t, C, m->acf_dev, "UPDATE_ACF");
+}
+
+int dbs_o_data_dev(dbs_ctx* ctx, const dbs_mat_blocks* m, int itype, int n, const int* k1, const int* k2,
+                   const double* c, double eps_c, int ibo, const int* ipbs, dbs_kbort_fn kbort,
+                   void* kbort_user, const double* vch_dev, int nvch, const double* fppqq_dev, int iitar,
+                   const int* iptar, double* otarg) {
+  DBS_CHECK(ctx && (n <= 0 || (k1 && k2 && c)), DBS_ERR_ARG, "O_data: null argument");
+  int rc = check_blocks(m);
+  if (rc || n <= 0) return rc;
+  DBS_CHECK(itype >= 1 && itype <= 4, DBS_ERR_ARG, "O_data: unknown structure (itype) ");
+  hipStream_t s = ctx->stream;
+  const int ms = ctx->ms;
+  // orbital io = i1*ibo + i2 -> (i1, 0-based column KBORT(i1,i2)-1 of V_ch)
+  auto vcol = [&](int io, int* col, int* i1) -> int {
+    DBS_CHECK(ibo > 0 && ipbs && kbort && vch_dev, DBS_ERR_ARG, "O_data: bound-orbital tables missing");
+    *i1 = io / ibo;
+    DBS_CHECK(*i1 >= 1, DBS_ERR_ARG, "O_data: orbital index < 1");
+    const int k = kbort(kbort_user, io / ibo, io % ibo);
+    if (k == 0) return fail(DBS_ERR_ARG, "O: KBORT=0");
+    DBS_CHECK(k >= 1 && k <= nvch, DBS_ERR_ARG, "O_data: KBORT column out of range");
+    *col = k - 1;
+    return DBS_OK;
+  };
+
+  if (itype == 1) {  // <ic|jc>
+    std::vector<long long> t(n, -1);
+    for (int e = 0; e < n; ++e) {
+      if (std::fabs(c[e]) < eps_c) continue;
+      if ((rc = hb_target(m, k1[e], k2[e], &t[e]))) return rc;
+    }
+    return accumulate_list(s, t, c, m->hp_dev, "O_data");
+  }
+
+  if (itype == 2) {  // <i|.> jc: hcp(:,icb(ipbs(i1),jc)) += C*V_ch(:,KBORT)
+    std::vector<long long> t(n, -1);
+    std::vector<int> vc(n, 0);
+    for (int e = 0; e < n; ++e) {
+      if (std::fabs(c[e]) < eps_c) continue;
+      int i1 = 0, hc = 0;
+      if ((rc = vcol(k1[e], &vc[e], &i1))) return rc;
+      if ((rc = hv_column(m, ipbs[i1 - 1], k2[e], &hc))) return rc;
+      t[e] = hc - 1;
+    }
+    std::vector<int> order, toff;
+    std::vector<long long> tval;
+    group_by(t, order, tval, toff);
+    if (tval.empty()) return DBS_OK;
+    std::vector<int> tcol(tval.begin(), tval.end()), vcs(order.size());
+    std::vector<double> cvs(order.size());
+    for (size_t q = 0; q < order.size(); ++q) {
+      vcs[q] = vc[order[q]];
+      cvs[q] = c[order[q]];
+    }
+    Uploads up(s);
+    int *d_t, *d_o, *d_v;
+    double* d_c;
+    if ((rc = up.put(tcol, &d_t)) || (rc = up.put(toff, &d_o)) || (rc = up.put(vcs, &d_v)) ||
+        (rc = up.put(cvs, &d_c)))
+      return rc;
+    dim3 grid(grid_n(ms, 256), (unsigned)std::min<size_t>(tcol.size(), 65535));
+    k_vec_list<<<grid, 256, 0, s>>>(ms, (int)tcol.size(), d_t, d_o, d_v, d_c, vch_dev, ms, m->hcp_dev);
+    return finish(s, "O_data");
+  }
+
+  // itype 3 (<i|.><.|j>, UPDATE_HW) and 4 (<i|j>, Target_h + UPDATE_HX)
+  std::vector<long long> t(n, -1);
+  std::vector<BlkUpd> ent(n);
+  for (int e = 0; e < n; ++e) {
+    if (std::fabs(c[e]) < eps_c) continue;
+    if (itype == 3) {
+      int vi = 0, i1 = 0, wj = 0, j1 = 0, b = 0;
+      if ((rc = vcol(k1[e], &vi, &i1)) || (rc = vcol(k2[e], &wj, &j1))) return rc;
+      const int ich = ipbs[i1 - 1], jch = ipbs[j1 - 1];
+      if ((rc = chan_block(m, ich, jch, "UPDATE_HW", &b))) return rc;
+      t[e] = b - 1;
+      ent[e] = {1, ich > jch ? 0 : (jch > ich ? 1 : 2), vi, wj, c[e]};
+    } else {
+      DBS_CHECK(ibo > 0 && ipbs, DBS_ERR_ARG, "O_data: bound-orbital tables missing");
+      const int io = k1[e], i1 = io / ibo, i2 = io % ibo;
+      DBS_CHECK(i1 >= 1 && i2 >= 1, DBS_ERR_ARG, "O_data: orbital index < 1");
+      const int ich = ipbs[i1 - 1], jch = ipbs[i2 - 1];
+      // Target_h(ich,jch,0,C): otarg(ij) += C for a stored block
+      if (ich < 0) return fail(DBS_ERR_ARG, "target_h: ich < 0");
+      if (jch < 0) return fail(DBS_ERR_ARG, "target_h: jch < 0");
+      if (ich > m->nch) return fail(DBS_ERR_ARG, "target_h: ich > nch");
+      if (jch > m->nch) return fail(DBS_ERR_ARG, "target_h: jch > nch");
+      DBS_CHECK(ich >= 1 && jch >= 1, DBS_ERR_ARG, "target_h: channel index 0");
+      const int blk = m->icc[(ich - 1) + (long long)m->nch * (jch - 1)];
+      if (otarg && iptar && blk != 0) {
+        const int it = iptar[ich - 1], jt = iptar[jch - 1];
+        const int hi = std::max(it, jt), lo = std::min(it, jt);
+        otarg[(long long)(hi - 1) * hi / 2 + lo - 1] += c[e];
+      }
+      if (iitar > 1 && ich != jch) {
+        DBS_CHECK(fppqq_dev, DBS_ERR_ARG, "O_data: fppqq missing");
+        int b = 0;
+        if ((rc = chan_block(m, ich, jch, "UPDATE_HX", &b))) return rc;
+        t[e] = b - 1;
+        ent[e] = {0, ich >= jch ? 0 : 1, 0, 0, c[e]};
+      }
+    }
+  }
+  std::vector<int> order, toff;
+  std::vector<long long> tval;
+  group_by(t, order, tval, toff);
+  if (tval.empty()) return DBS_OK;
+  std::vector<int> tblk(tval.begin(), tval.end());
+  std::vector<BlkUpd> es(order.size());
+  for (size_t q = 0; q < order.size(); ++q) es[q] = ent[order[q]];
+  Uploads up(s);
+  int *d_t, *d_o;
+  BlkUpd* d_e;
+  if ((rc = up.put(tblk, &d_t)) || (rc = up.put(toff, &d_o)) || (rc = up.put(es, &d_e))) return rc;
+  const unsigned gy = (unsigned)std::min<size_t>(tblk.size(), 65535);
+  const unsigned gx = std::min(grid_n((long long)ms * ms, 256), std::max(1u, 262144u / gy));
+  k_block_list<<<dim3(gx, gy), 256, 0, s>>>(ms, (int)tblk.size(), d_t, d_o, d_e, fppqq_dev, vch_dev, ms,
+                                            m->hch_dev);
+  return finish(s, "O_data");
+}
+
+int dbs_overlap_ci_dev(dbs_ctx* ctx, int n, const int* ic, const int* jc, const double* cc, double eps_c,
+                       int nzero, double* SM_dev, int ldsm) {
+  DBS_CHECK(ctx && (n <= 0 || (ic && jc && cc)) && SM_dev, DBS_ERR_ARG, "overlap: null argument");
+  DBS_CHECK(ldsm >= 1, DBS_ERR_ARG, "overlap: ldsm < 1");
+  if (n <= 0) return DBS_OK;
+  std::vector<long long> t(n, -1);
+  for (int e = 0; e < n; ++e) {
+    if (std::fabs(cc[e]) < eps_c) continue;  // CC below eps_C is skipped
+    DBS_CHECK(ic[e] >= 1 && ic[e] <= ldsm && jc[e] >= 1, DBS_ERR_ARG, "overlap: index out of range");
+    if (jc[e] <= nzero) t[e] = (long long)(ic[e] - 1) + (long long)ldsm * (jc[e] - 1);
+  }
+  return accumulate_list(ctx->stream, t, cc, SM_dev, "overlap");
+}
+
+}  // extern "C"

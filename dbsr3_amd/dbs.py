@@ -191,6 +191,24 @@ class DBS:
         self._post()
         return float(out.item())
 
+    # -- overlaps and unfolding ---------------------------------------------
+    def bspline_overlaps(self):
+        """fpbs(ns,ns), fqbs(ns,ns), fppqq(ms,ms) as device tensors
+        (ZINTYM via Def_pbsp / Def_qbsp / Def_pq_arrays)"""
+        fp, fq, fpq = self._empty(self.ns, self.ns), self._empty(self.ns, self.ns), self._empty(self.ms, self.ms)
+        self._pre()
+        check(lib().dbs_bspline_overlaps_dev(self.ctx, _ptr(fp), _ptr(fq), _ptr(fpq)))
+        self._post()
+        return fp, fq, fpq
+
+    def Full_mat_sym(self, x, sym):
+        """y(ns,ns) from band/full storage x (device tensor) in mode sym"""
+        y = self._empty(self.ns, self.ns)
+        self._pre()
+        check(lib().dbs_full_mat_sym_dev(self.ctx, _ptr(x), sym.encode(), _ptr(y)))
+        self._post()
+        return y
+
     def UPDATE_HS(self, hm, ii, jj, d, sym):
         """UPDATE_HS(ms,hm,ii,jj,ns,ks,d,sym): hm device tensor holding (ms,ms)"""
         self._pre()

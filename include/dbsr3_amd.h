@@ -169,6 +169,75 @@ int dbs_rdata_34_dev(dbs_ctx *ctx, int itype, int atype, int k, int nc, const do
                      int norb, const int *icc, int nch, double *hch_dev, int nblocks);
 
 /* ---------------------------------------------------------------------
+ * B-spline overlaps, unfolding and the remaining matrix updates
+ * (dbs_updates.hip).  Loop and summation orders are the reference's, so the
+ * results are bit-identical to it.
+ * --------------------------------------------------------------------- */
+/* ZINTYM (dbsr_lib_dbs.f90:4472-4497) as called by Def_pbsp / Def_qbsp
+ * (:1026, :1072) and Def_pq_arrays (:1078-1102): fpbs(ns,ns) = <B^P_i|B^P_j>,
+ * fqbs(ns,ns) = <B^Q_i|B^Q_j>, fppqq(ms,ms) = diag(fpbs, fqbs).  Any output
+ * pointer may be NULL. */
+int dbs_bspline_overlaps_dev(dbs_ctx *ctx, double *fpbs_dev, double *fqbs_dev, double *fppqq_dev);
+/* Full_mat_sym (DBSR_HF/dbsr_lib_zcom.f90:1247-1294): y(ns,ns)
+ * from x in storage sym 's' (upper band x(ns,ks)), 'l' (lower band), 'n'
+ * (non-symmetric band x(ns,2ks-1)) or 'x' (full). */
+int dbs_full_mat_sym_dev(dbs_ctx *ctx, const double *x_dev, char sym, double *y_dev);
+
+/* MAT3 matrix storage (DBSR_MAT3/mod_dbsr_mat.f90:134-147, allocated as in
+ * alloc_dbsr_matrix, DBSR_MAT3/updates.f90:2-75).  Index tables are host,
+ * column-major, 1-based block numbers (0 = block not stored: updates to it
+ * are dropped, as for blocks another process owns).  Matrices are device. */
+typedef struct dbs_mat_blocks {
+  int nch, npert, mk;
+  const int *icc; int nblocks; double *hch_dev; /* icc(nch,nch); hch(ms,ms,nblocks) */
+  double *acf_dev;                              /* acf(nblocks,0:mk) */
+  const int *icb; int ncb; double *hcp_dev;     /* icb(nch,npert); hcp(ms,ncb) */
+  const int *ibb; int nbb; double *hp_dev;      /* ibb(npert,npert); hp(nbb) */
+} dbs_mat_blocks;
+
+/* UPDATE_HX (DBSR_MAT3/updates.f90:79-106): block(ich,jch) += C*d
+ * (ich >= jch) or C*d^T; d_dev(ms,ms) */
+int dbs_update_hx_dev(dbs_ctx *ctx, const dbs_mat_blocks *m, int ich, int jch, const double *d_dev,
+                      double C);
+/* UPDATE_HW (updates.f90:183-217): block(ich,jch) += C*v*w^T (ich > jch),
+ * C*(v*w^T)^T (ich < jch), (C/2)*(x + x^T) (ich == jch); v,w (ms) device */
+int dbs_update_hw_dev(dbs_ctx *ctx, const dbs_mat_blocks *m, int ich, int jch, const double *v_dev,
+                      const double *w_dev, double C);
+/* UPDATE_HV (updates.f90:221-244): hcp(:,icb(ich,ip)) += C*v */
+int dbs_update_hv_dev(dbs_ctx *ctx, const dbs_mat_blocks *m, int ich, int ip, const double *v_dev,
+                      double C);
+/* UPDATE_HB (updates.f90:248-266) for n calls in order (host lists):
+ * hp(ibb(ic,jc)) += c */
+int dbs_update_hb_dev(dbs_ctx *ctx, const dbs_mat_blocks *m, int n, const int *ic, const int *jc,
+                      const double *c);
+/* UPDATE_ACF (updates.f90:270-292) for n calls in order (host lists):
+ * acf(icc(ich,jch),k) += C */
+int dbs_update_acf_dev(dbs_ctx *ctx, const dbs_mat_blocks *m, int n, const int *k, const int *ich,
+                       const int *jch, const double *C);
+
+/* KBORT(i1,i2) of the host program: column of V_ch for the bound orbital
+ * (i1,i2), 0 if none (the reference's external KBORT). */
+typedef int (*dbs_kbort_fn)(void *user, int i1, int i2);
+/* O_data (DBSR_MAT3/O_data.f90:2-93): one merged overlap list (IPT order,
+ * host arrays k1,k2,c) of structure itype 1..4 into the MAT3 matrices.
+ * 1 UPDATE_HB(k1,k2); 2 UPDATE_HV(ipbs(io/ibo), k2, V_ch(:,KBORT)); 3
+ * UPDATE_HW(ipbs(i1), ipbs(j1), V_ch(:,KBORT(i)), V_ch(:,KBORT(j))); 4 the
+ * target overlap otarg(ij) += C (Target_h, DBSR_MAT3/target_subs.f90:2-31;
+ * skipped when otarg is NULL) and, for iitar > 1 and ich != jch,
+ * UPDATE_HX(ich,jch,fppqq,C).  Entries with |c| < eps_c are skipped.  The
+ * updates of one matrix element are applied in list order.  vch_dev(ms,nvch),
+ * fppqq_dev(ms,ms) device; ipbs(orbital) and iptar(channel) host, 1-based. */
+int dbs_o_data_dev(dbs_ctx *ctx, const dbs_mat_blocks *m, int itype, int n, const int *k1,
+                   const int *k2, const double *c, double eps_c, int ibo, const int *ipbs,
+                   dbs_kbort_fn kbort, void *kbort_user, const double *vch_dev, int nvch,
+                   const double *fppqq_dev, int iitar, const int *iptar, double *otarg);
+/* CI overlap, met = 0 (DBSR_CI3/setup_matrix.f90:108-120): for each entry
+ * with |cc| >= eps_c and jc <= nzero, SM(ic,jc) += cc, in list order;
+ * SM_dev(ldsm,nzero). */
+int dbs_overlap_ci_dev(dbs_ctx *ctx, int n, const int *ic, const int *jc, const double *cc,
+                       double eps_c, int nzero, double *SM_dev, int ldsm);
+
+/* ---------------------------------------------------------------------
  * Engine: batched evaluation of integral sets and their adjoint.
  * An instance = (context, norb orbitals, rows (k,a,b,c,d)).  Row n is
  *   kind DBS_RK_TOTAL: R^k(a,b;c,d) = rk(f_a,f_b,f_c,f_d,k)

@@ -842,6 +842,10 @@ static void unfold(int ns, int ks, const double *d, char sym, double *y) {
     for (int jp = 1; jp <= ks; ++jp) for (int i = 1; i <= ns - jp + 1; ++i) {
       int j = i + jp - 1; Y(i, j) = DV(i, jp); Y(j, i) = DV(i, jp);
     }
+  } else if (sym == 'l') {
+    for (int jp = 1; jp <= ks; ++jp) for (int i = ks + 1 - jp; i <= ns; ++i) {
+      int j = i + jp - ks; Y(i, j) = DV(i, jp); Y(j, i) = DV(i, jp);
+    }
   } else if (sym == 'n') {
     for (int jp = 1; jp <= 2 * ks - 1; ++jp) {
       int imin = 1 > 1 + ks - jp ? 1 : 1 + ks - jp, imax = ns < ns + ks - jp ? ns : ns + ks - jp;
@@ -1338,4 +1342,140 @@ int orc_add_integral_ci(orc_cdata *cd, int kpol, int ic, int jc, int i1, int i2,
     if (r < 0) rc = -1;
   }
   return rc;
+}
+
+/* ===================================================================== */
+/* Overlaps, unfolding and the MAT3 matrix updates                        */
+/* ===================================================================== */
+
+/* Full_mat_sym: DBSR_HF/dbsr_lib_zcom.f90:1247-1294 ('s','l','n','x') */
+void orc_full_mat_sym(int ns, int ks, const double *x, double *y, char sym) {
+  unfold(ns, ks, x, sym, y);
+}
+
+/* ZINTYM: DBSR_HF/dbsr_lib_dbs.f90:4472-4497.  b1(nv+1,ks,ks1),
+ * b2(nv+1,ks,ks2), ygr(nv,ks), ym(mm,mm). */
+void orc_zintym(int nv, int ks, int ks1, int ks2, const double *b1, const double *b2,
+                const double *ygr, int mm, double *ym) {
+  const size_t nw = (size_t)nv + 1;
+  memset(ym, 0, sizeof(double) * (size_t)mm * mm);
+  for (int iv = 1; iv <= nv; ++iv)
+    for (int ith = 1; ith <= ks1; ++ith) {
+      const int i = iv + ith - 1;
+      for (int jth = 1; jth <= ks2; ++jth) {
+        const int j = iv + jth - 1;
+        double s = 0.0;
+        for (int m = 1; m <= ks; ++m)
+          s += ygr[(iv - 1) + (size_t)nv * (m - 1)] * b1[(iv - 1) + nw * ((m - 1) + (size_t)ks * (ith - 1))] *
+               b2[(iv - 1) + nw * ((m - 1) + (size_t)ks * (jth - 1))];
+        ym[(i - 1) + (size_t)mm * (j - 1)] += s;
+      }
+    }
+}
+
+/* Def_pbsp / Def_qbsp overlaps and Def_pq_arrays' fppqq:
+ * DBSR_HF/dbsr_lib_dbs.f90:1026, 1072, 1098-1101 */
+void orc_overlaps(const orc_grid *g, double *fpbs, double *fqbs, double *fppqq) {
+  const int ns = g->ns, ms = g->ms;
+  double *p = (double *)malloc(sizeof(double) * (size_t)ns * ns * 2), *q = p + (size_t)ns * ns;
+  orc_zintym(g->nv, g->ks, g->ksp, g->ksp, g->pbsp, g->pbsp, g->grw, ns, p);
+  orc_zintym(g->nv, g->ks, g->ksq, g->ksq, g->qbsp, g->qbsp, g->grw, ns, q);
+  if (fpbs) memcpy(fpbs, p, sizeof(double) * (size_t)ns * ns);
+  if (fqbs) memcpy(fqbs, q, sizeof(double) * (size_t)ns * ns);
+  if (fppqq) {
+    memset(fppqq, 0, sizeof(double) * (size_t)ms * ms);
+    for (int j = 0; j < ns; ++j)
+      for (int i = 0; i < ns; ++i) {
+        fppqq[i + (size_t)ms * j] = p[i + (size_t)ns * j];
+        fppqq[(ns + i) + (size_t)ms * (ns + j)] = q[i + (size_t)ns * j];
+      }
+  }
+  free(p);
+}
+
+/* UPDATE_HX: DBSR_MAT3/updates.f90:79-106 on one block (ms,ms) */
+void orc_update_hx(int ms, double *blk, int ich, int jch, const double *d, double C) {
+  for (int j = 0; j < ms; ++j)
+    for (int i = 0; i < ms; ++i) {
+      const double xx = ich >= jch ? C * d[i + (size_t)ms * j] : C * d[j + (size_t)ms * i];
+      blk[i + (size_t)ms * j] = blk[i + (size_t)ms * j] + xx;
+    }
+}
+
+/* UPDATE_HW: DBSR_MAT3/updates.f90:183-217 on one block (ms,ms) */
+void orc_update_hw(int ms, double *blk, int ich, int jch, const double *v, const double *w, double C) {
+  double *x = (double *)malloc(sizeof(double) * (size_t)ms * ms * 2), *xx = x + (size_t)ms * ms;
+  for (int i = 0; i < ms; ++i) for (int j = 0; j < ms; ++j) x[i + (size_t)ms * j] = v[i] * w[j];
+  for (int j = 0; j < ms; ++j)
+    for (int i = 0; i < ms; ++i) {
+      const double xij = x[i + (size_t)ms * j], xji = x[j + (size_t)ms * i];
+      if (ich > jch) xx[i + (size_t)ms * j] = C * xij;
+      else if (jch > ich) xx[i + (size_t)ms * j] = C * xji;
+      else xx[i + (size_t)ms * j] = (C / 2.0) * (xij + xji);
+    }
+  for (size_t q = 0; q < (size_t)ms * ms; ++q) blk[q] = blk[q] + xx[q];
+  free(x);
+}
+
+/* UPDATE_HV: DBSR_MAT3/updates.f90:221-244 on one column */
+void orc_update_hv(int ms, double *col, const double *v, double C) {
+  for (int i = 0; i < ms; ++i) col[i] = col[i] + C * v[i];
+}
+
+/* O_data: DBSR_MAT3/O_data.f90:2-93, one list in IPT order.  Matrix tables
+ * as the MAT3 storage: icc(nch,nch), icb(nch,npert), ibb(npert,npert)
+ * (1-based block numbers, 0 = not stored), hch(ms,ms,*), hcp(ms,*), hp(*).
+ * kbort(i1,i2) gives the 1-based column of vch(ms,*).  ipbs, iptar are
+ * 1-based tables (element 0 unused).  Returns 0, or a nonzero code where
+ * the reference stops. */
+int orc_o_data(int ms, int itype, int n, const int *k1, const int *k2, const double *c, double eps_c,
+               int ibo, const int *ipbs, orc_kbort_fn kbort, void *user, const double *vch,
+               const double *fppqq, int iitar, const int *iptar, double *otarg,
+               int nch, int npert, const int *icc, double *hch, const int *icb, double *hcp,
+               const int *ibb, double *hp) {
+  for (int e = 0; e < n; ++e) {
+    const double C = c[e];
+    if (fabs(C) < eps_c) continue;
+    if (itype == 1) {
+      const int ic = k1[e], jc = k2[e];
+      if (ic > npert || jc > npert || ic <= 0 || jc <= 0) return 1;
+      const int i = ibb[(ic - 1) + (size_t)npert * (jc - 1)];
+      if (i > 0) hp[i - 1] = hp[i - 1] + C;
+    } else if (itype == 2) {
+      const int io = k1[e], jc = k2[e], i1 = io / ibo, i2 = io % ibo;
+      const int k = kbort(user, i1, i2);
+      if (k == 0) return 2;
+      const int ich = ipbs[i1];
+      if (ich < 1 || ich > nch || jc < 1 || jc > npert) return 3;
+      const int i = icb[(ich - 1) + (size_t)nch * (jc - 1)];
+      if (i > 0) orc_update_hv(ms, hcp + (size_t)ms * (i - 1), vch + (size_t)ms * (k - 1), C);
+    } else if (itype == 3) {
+      const int io = k1[e], jo = k2[e];
+      const int i1 = io / ibo, i2 = io % ibo, j1 = jo / ibo, j2 = jo % ibo;
+      const int ki = kbort(user, i1, i2);
+      if (ki == 0) return 2;
+      const int kj = kbort(user, j1, j2);
+      if (kj == 0) return 2;
+      const int ich = ipbs[i1], jch = ipbs[j1];
+      if (ich < 1 || jch < 1 || ich > nch || jch > nch) return 4;
+      const int b = icc[(ich - 1) + (size_t)nch * (jch - 1)];
+      if (b > 0)
+        orc_update_hw(ms, hch + (size_t)ms * ms * (b - 1), ich, jch, vch + (size_t)ms * (ki - 1),
+                      vch + (size_t)ms * (kj - 1), C);
+    } else if (itype == 4) {
+      const int io = k1[e], i1 = io / ibo, i2 = io % ibo;
+      const int ich = ipbs[i1], jch = ipbs[i2];
+      if (ich < 1 || jch < 1 || ich > nch || jch > nch) return 4;
+      const int b = icc[(ich - 1) + (size_t)nch * (jch - 1)];
+      if (b > 0 && otarg) {  /* Target_h(ich,jch,0,C): DBSR_MAT3/target_subs.f90:2-31 */
+        const int it = iptar[ich], jt = iptar[jch];
+        const int hi = it > jt ? it : jt, lo = it > jt ? jt : it;
+        otarg[(size_t)(hi - 1) * hi / 2 + lo - 1] += C;
+      }
+      if (iitar > 1 && ich != jch && b > 0) orc_update_hx(ms, hch + (size_t)ms * ms * (b - 1), ich, jch, fppqq, C);
+    } else {
+      return 5;
+    }
+  }
+  return 0;
 }

@@ -104,6 +104,21 @@ double orc_smu(int ka, int kb, int kc, int kd, int L, int v, double *S);
 int  orc_add_integral_ci(orc_cdata *cd, int kpol, int ic, int jc, int i1, int i2, int i3, int i4,
                          double C, const int *kappa, int ibi, int mbreit);
 
+/* --- overlaps, unfolding, MAT3 updates ------------------------------- */
+void orc_full_mat_sym(int ns, int ks, const double *x, double *y, char sym);
+void orc_zintym(int nv, int ks, int ks1, int ks2, const double *b1, const double *b2,
+                const double *ygr, int mm, double *ym);
+void orc_overlaps(const orc_grid *g, double *fpbs, double *fqbs, double *fppqq);
+void orc_update_hx(int ms, double *blk, int ich, int jch, const double *d, double C);
+void orc_update_hw(int ms, double *blk, int ich, int jch, const double *v, const double *w, double C);
+void orc_update_hv(int ms, double *col, const double *v, double C);
+typedef int (*orc_kbort_fn)(void *user, int i1, int i2);
+int  orc_o_data(int ms, int itype, int n, const int *k1, const int *k2, const double *c, double eps_c,
+                int ibo, const int *ipbs, orc_kbort_fn kbort, void *user, const double *vch,
+                const double *fppqq, int iitar, const int *iptar, double *otarg,
+                int nch, int npert, const int *icc, double *hch, const int *icb, double *hcp,
+                const int *ibb, double *hp);
+
 #ifdef __cplusplus
 }
 #endif

@@ -41,6 +41,7 @@ def f64(a):
 
 
 _lib = None
+KBORT_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.c_int)
 
 
 def lib():
@@ -82,6 +83,16 @@ def lib():
         L.orc_idef_rtype.restype = C.c_int
         L.orc_idef_rtype.argtypes = [C.c_int] * 4 + [_ip, C.c_int, _ip, _ip, _ip, _ip]
         L.orc_decode_int.argtypes = [C.c_int] + [_ip] * 6
+        L.orc_full_mat_sym.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_char]
+        L.orc_zintym.argtypes = [C.c_int] * 4 + [_dp, _dp, _dp, C.c_int, _dp]
+        L.orc_overlaps.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        L.orc_update_hx.argtypes = [C.c_int, _dp, C.c_int, C.c_int, _dp, C.c_double]
+        L.orc_update_hw.argtypes = [C.c_int, _dp, C.c_int, C.c_int, _dp, _dp, C.c_double]
+        L.orc_update_hv.argtypes = [C.c_int, _dp, _dp, C.c_double]
+        L.orc_o_data.restype = C.c_int
+        L.orc_o_data.argtypes = [C.c_int, C.c_int, C.c_int, _ip, _ip, _dp, C.c_double, C.c_int, _ip,
+                                 KBORT_FN, C.c_void_p, _dp, _dp, C.c_int, _ip, _dp,
+                                 C.c_int, C.c_int, _ip, _dp, _ip, _dp, _ip, _dp]
         _lib = L
     return _lib
 
@@ -230,6 +241,58 @@ def rdata_34(g, itype, atype, k, lst, pq, ibo, icc, hch):
                        _i(ci(K2)), _i(ci(K3)), _d(f64(pq)), ibo, _i(icc_f), icc.shape[0], _d(hch))
 
 
+def full_mat_sym(ns, ks, x, sym):
+    """Full_mat_sym (DBSR_HF/dbsr_lib_zcom.f90:1247-1294)"""
+    y = np.zeros((ns, ns), order="F")
+    lib().orc_full_mat_sym(ns, ks, _d(f64(x)), _d(y), sym.encode())
+    return y
+
+
+def zintym(nv, ks, ks1, ks2, b1, b2, ygr, mm):
+    """ZINTYM (DBSR_HF/dbsr_lib_dbs.f90:4472-4497)"""
+    ym = np.zeros((mm, mm), order="F")
+    lib().orc_zintym(nv, ks, ks1, ks2, _d(f64(b1)), _d(f64(b2)), _d(f64(ygr)), mm, _d(ym))
+    return ym
+
+
+def overlaps(g):
+    """fpbs, fqbs (Def_pbsp/Def_qbsp) and fppqq (Def_pq_arrays)"""
+    fp = np.zeros((g.ns, g.ns), order="F")
+    fq = np.zeros((g.ns, g.ns), order="F")
+    fpq = np.zeros((g.ms, g.ms), order="F")
+    lib().orc_overlaps(g.h, _d(fp), _d(fq), _d(fpq))
+    return fp, fq, fpq
+
+
+def update_hx(blk, ich, jch, d, C_):
+    lib().orc_update_hx(blk.shape[0], _d(blk), ich, jch, _d(f64(d)), C_)
+
+
+def update_hw(blk, ich, jch, v, w, C_):
+    lib().orc_update_hw(blk.shape[0], _d(blk), ich, jch, _d(f64(v)), _d(f64(w)), C_)
+
+
+def update_hv(col, v, C_):
+    lib().orc_update_hv(col.shape[0], _d(col), _d(f64(v)), C_)
+
+
+def o_data(ms, itype, lst, eps_c, ibo, ipbs, kbort, vch, fppqq, iitar, iptar, otarg, mats):
+    """O_data (DBSR_MAT3/O_data.f90:2-93).  lst = (c, k1, k2) in IPT order;
+    ipbs/iptar 1-based (element 0 unused); kbort(i1,i2) -> 1-based column of
+    vch; mats = dict(icc, hch, icb, hcp, ibb, hp), updated in place.
+    Returns the restatement's status (0 = ok)."""
+    c, k1, k2 = lst
+    ci = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+    fi = lambda a: np.asfortranarray(a, dtype=np.int32)
+    cb = KBORT_FN(lambda user, i1, i2: int(kbort(i1, i2)))
+    icc, icb, ibb = fi(mats["icc"]), fi(mats["icb"]), fi(mats["ibb"])
+    nch, npert = icc.shape[0], ibb.shape[0]
+    return lib().orc_o_data(ms, itype, len(c), _i(ci(k1)), _i(ci(k2)), _d(np.ascontiguousarray(c, dtype=np.float64)),
+                            eps_c, ibo, _i(ci(ipbs)), cb, None, _d(f64(vch)), _d(f64(fppqq)), iitar,
+                            _i(ci(iptar)), _d(otarg), nch, npert, _i(icc), _d(mats["hch"]), _i(icb),
+                            _d(mats["hcp"]), _i(ibb), _d(mats["hp"]))
+
+
 def idef_rtype(j1, j2, j3, j4, ipbs, ibo):
     ip = np.ascontiguousarray(ipbs, dtype=np.int32)
     out = [np.zeros(1, dtype=np.int32) for _ in range(4)]
@@ -321,6 +384,8 @@ def ref():
         L.ref_quadr.restype = C.c_double
         L.ref_quadr.argtypes = [_dp, _dp, C.c_int]
         L.ref_update_hs.argtypes = [_dp, C.c_int, C.c_int, _dp, C.c_char]
+        L.ref_overlaps.argtypes = [_dp, _dp, _dp]
+        L.ref_full_mat_sym.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_char]
         L.ref_cdata_alloc.argtypes = [C.c_int] * 6 + [C.c_double, C.c_int]
         L.ref_cdata_add.argtypes = [C.c_double] + [C.c_int] * 6
         L.ref_cdata_flush.argtypes = [C.c_int, C.c_int]

@@ -204,3 +204,23 @@
       a = ka; b = kb; c = kc; d = kd; ll = L; vv = v
       r = SMU(a, b, c, d, ll, vv, S)
       End Function ref_smu
+
+      Subroutine ref_overlaps(fpbs_, fqbs_, fppqq_) bind(C, name='ref_overlaps')
+      Use iso_c_binding
+      Use DBS_grid
+      Use DBS_gauss
+      Implicit none
+      Real(c_double) :: fpbs_(ns, ns), fqbs_(ns, ns), fppqq_(ms, ms)
+      fpbs_ = fpbs; fqbs_ = fqbs; fppqq_ = fppqq
+      End Subroutine ref_overlaps
+
+      Subroutine ref_full_mat_sym(ns_, ks_, x, y, sym) bind(C, name='ref_full_mat_sym')
+      Use iso_c_binding
+      Implicit none
+      Integer(c_int), value :: ns_, ks_
+      Real(c_double) :: x(ns_, *), y(ns_, ns_)
+      Character(kind=c_char), value :: sym
+      Character(1) :: c
+      c = sym
+      Call Full_mat_sym(ns_, ks_, x, y, c)
+      End Subroutine ref_full_mat_sym

@@ -187,3 +187,22 @@ def test_smu():
         a = L.orc_smu(ka, kb, kc, kd, Lk, v, O._d(s1))
         b = R.ref_smu(ka, kb, kc, kd, Lk, v, O._d(s2))
         assert a == b and np.array_equal(s1, s2), (ka, kb, kc, kd, Lk, v)
+
+
+@pytest.mark.parametrize("name", ["small", "fe"])
+def test_overlaps_and_full_mat_sym(name):
+    """ZINTYM overlaps (Def_pbsp/Def_qbsp/Def_pq_arrays) and Full_mat_sym"""
+    R, g = setup_both(GRIDS[name]())
+    fp, fq, fpq = O.overlaps(g)
+    rp, rq = np.zeros_like(fp), np.zeros_like(fq)
+    rpq = np.zeros_like(fpq)
+    R.ref_overlaps(O._d(rp), O._d(rq), O._d(rpq))
+    assert np.array_equal(fp, rp) and np.array_equal(fq, rq) and np.array_equal(fpq, rpq)
+    ns, ks = g.ns, g.ks
+    rng = np.random.default_rng(11)
+    for sym in "slnx":
+        nc = ns if sym == "x" else (2 * ks - 1 if sym == "n" else ks)
+        x = O.f64(rng.standard_normal((ns, nc)))
+        y = np.zeros((ns, ns), order="F")
+        R.ref_full_mat_sym(ns, ks, O._d(x), O._d(y), sym.encode())
+        assert np.array_equal(O.full_mat_sym(ns, ks, x, sym), y), sym
