@@ -33,6 +33,16 @@ class MatBlocksC(C.Structure):
 
 mbp = C.POINTER(MatBlocksC)
 
+
+class Mat3AuxC(C.Structure):
+    """struct dbs_mat3_aux (bound-orbital and target data of a MAT3 run)"""
+    _fields_ = [("ibo", i32), ("ipbs", ip), ("kbort", KBORT_FN), ("kbort_user", vp),
+                ("vch_dev", vp), ("nvch", i32), ("fppqq_dev", vp), ("iitar", i32),
+                ("iptar", ip), ("htarg", dp)]
+
+
+auxp = C.POINTER(Mat3AuxC)
+
 # name -> argtypes (all return int status unless listed in _RESTYPE)
 PROTOTYPES = {
     "dbs_version": [],
@@ -82,6 +92,9 @@ PROTOTYPES = {
     "dbs_o_data_dev": [vp, mbp, i32, i32, ip, ip, dp, f64, i32, ip, KBORT_FN, vp, vp, i32, vp, i32,
                        ip, dp],
     "dbs_overlap_ci_dev": [vp, i32, ip, ip, dp, f64, i32, vp, i32],
+    "dbs_rdata_dev": [vp, mbp, auxp, i32, i32, i32, i32, dp, ip, ip, ip, ip, vp, i32],
+    "dbs_sdata_dev": [vp, mbp, auxp, i32, i32, i32, dp, ip, ip, ip, ip, vp, i32],
+    "dbs_bav_dev": [vp, vp, vp, ch, ch, vp],
     "dbs_engine_create": [C.POINTER(vp), i32, C.POINTER(vp), ip, ip, ip, i32],
     "dbs_engine_destroy": [vp],
     "dbs_engine_buffers": [vp, i32, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp),

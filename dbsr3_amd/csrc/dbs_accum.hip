@@ -710,9 +710,23 @@ int dbs_rdata_34_dev(dbs_ctx* ctx, int itype, int atype, int k, int nc, const do
     } else {
       k_group_convol3<<<grid_of(per * ng, 256), 256, 0, s>>>(ng, ns, ks, rkb, d_xx, d_a);
     }
+    // one launch for all runs when their blocks differ; runs sharing a block
+    // (K1 and its transpose, or a K1 repeated out of order) one launch each,
+    // in list order
+    std::vector<int> blks;
+    for (const int4& G : ginfo) blks.push_back(G.z);
+    std::sort(blks.begin(), blks.end());
+    const bool shared = std::adjacent_find(blks.begin(), blks.end()) != blks.end();
     const long long nupd = (long long)ns * ns * ng;
-    for (int pass = 0; pass < 2; ++pass)
-      k_update_hs_mat<<<grid_of(nupd, 256), 256, 0, s>>>(ng, d_gi, ns, ks, sym, ip1, ip2, pass, d_a, hch_dev);
+    if (!shared) {
+      for (int pass = 0; pass < 2; ++pass)
+        k_update_hs_mat<<<grid_of(nupd, 256), 256, 0, s>>>(ng, d_gi, ns, ks, sym, ip1, ip2, pass, d_a, hch_dev);
+    } else {
+      for (int g = 0; g < ng; ++g)
+        for (int pass = 0; pass < 2; ++pass)
+          k_update_hs_mat<<<grid_of((long long)ns * ns, 256), 256, 0, s>>>(1, d_gi + g, ns, ks, sym, ip1, ip2, pass,
+                                                                           d_a + per * g, hch_dev);
+    }
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) rc = fail(DBS_ERR_HIP, std::string("R_data: ") + hipGetErrorString(e));

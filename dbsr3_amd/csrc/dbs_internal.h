@@ -101,3 +101,33 @@ int ensure_stage(dbs_ctx* c, size_t n);
 int kband(int type, int ks);  // third/fourth dimension of rkb for the type
 int build_rkb(dbs_ctx* c, int type, int k, double** out);
 }  // namespace dbs
+
+namespace dbs {
+// convol on the context stream with a materialised array rkb of the type
+// (the kernels of dbs_convol_dev, without its checks)
+int convol_launch(dbs_ctx* c, const double* rkb, int type, int icase, char si, char sj, const double* d,
+                  double* a);
+
+// ---------------------------------------------------------------------------
+// MAT3 matrix updates shared by O_data, R_data and S_data (dbs_updates.hip)
+// ---------------------------------------------------------------------------
+// One update of a list.  target: hp element (HB), hcp column (HV) or hch
+// block (HW, HX), 0-based; -1 = not stored here (dropped).  Vectors: index
+// >= 0 is a column of V (V_ch), < 0 column -idx-1 of W (computed, length ms).
+struct MatUpd {
+  enum { HB = 0, HV = 1, HW = 2, HX = 3 };
+  int kind, mode, v, w;
+  long long target;
+  double C;
+};
+// applies the updates, in list order per target (stable grouping, one
+// thread per matrix element); synchronises the context stream
+int apply_mat3(dbs_ctx* ctx, const dbs_mat_blocks* m, const std::vector<MatUpd>& u, const double* V, int ldv,
+               const double* W, const double* d, const char* who);
+// icc(ich,jch) with the reference's range checks; *b = 0: block not stored
+int chan_block(const dbs_mat_blocks* m, int ich, int jch, const char* who, int* b);
+int hb_target(const dbs_mat_blocks* m, int ic, int jc, long long* t);
+int hv_column(const dbs_mat_blocks* m, int ich, int ip, int* col);
+int check_blocks(const dbs_mat_blocks* m);
+int target_h(const dbs_mat_blocks* m, int ich, int jch, const int* iptar, double* tgt, double C);
+}  // namespace dbs

@@ -1,0 +1,503 @@
+// dbs_mat3.hip -- DBSR_MAT3's structure drivers on the GPU:
+//   R_data itype 1..4 (DBSR_MAT3/R_data.f90:2-290) and S_data itype 1..9,
+//   atype 0/1 (DBSR_MAT3/S_data.f90:2-287), with BAV (called at
+//   R_data.f90:181, S_data.f90:205; its source is in the un-vendored
+//   Libraries_03: restated as c = A b ('r') / c = A^T b ('l') on the
+//   unfolded band matrix).
+//
+// Structure 1 (integral times a bound/channel factor): the list's integrals
+// come from the engine in one batch, C = S*c, and the updates go through the
+// in-order list kernels.  Structures 2-5 (bound-channel vectors): one
+// density and convolution per distinct pair K1 (the plain kernels), one BAV
+// vector per distinct (K1, orbital), then UPDATE_HW / UPDATE_HV in list
+// order.  Structures 6-9 (S, channel-channel): runs of equal K1 accumulate
+// their densities (one batched kernel), one convolution per run, UPDATE_HS
+// per run in list order.
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "dbs_internal.h"
+
+namespace dbs {
+
+// element (i,j) (0-based) of the (ns,ns) matrix held in storage sym
+// (0 's', 1 'l', 2 'n', 3 'x'), as Full_mat_sym unfolds it
+__device__ __forceinline__ double band_elem(const double* __restrict__ x, int ns, int ks, int sym, int i, int j) {
+  if (sym == 3) return x[i + (long long)ns * j];
+  if (sym == 2) {
+    const int jp = j - i + ks - 1;
+    return (jp >= 0 && jp < 2 * ks - 1) ? x[i + (long long)ns * jp] : 0.0;
+  }
+  const int a = min(i, j), b = max(i, j), d = b - a;
+  if (d >= ks) return 0.0;
+  return sym == 0 ? x[a + (long long)ns * d] : x[b + (long long)ns * (ks - 1 - d)];
+}
+
+// BAV for a list of (matrix g, orbital): W(:,q) = 0 except the ns entries at
+// offset woff: sum_j A_g(i,j) b(j) ('r') or A_g(j,i) b(j) ('l'), j ascending
+__global__ void k_bav_list(int nvec, const int2* __restrict__ vg, const double* __restrict__ A, long long astride,
+                           int sym, int left, const double* __restrict__ pq, int comp, int ns, int ks, int woff,
+                           double* __restrict__ W) {
+#pragma clang fp contract(off)
+  const int ms = 2 * ns;
+  const long long total = (long long)nvec * ns;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+       q += (long long)gridDim.x * blockDim.x) {
+    const int vq = (int)(q / ns), i = (int)(q % ns);
+    const int2 e = vg[vq];  // (matrix, 1-based orbital)
+    const double* a = A + astride * e.x;
+    const double* b = pq + (long long)ns * ((comp - 1) + 2LL * (e.y - 1));
+    double s = 0.0;
+    for (int j = 0; j < ns; ++j)
+      s += (left ? band_elem(a, ns, ks, sym, j, i) : band_elem(a, ns, ks, sym, i, j)) * b[j];
+    double* w = W + (long long)ms * vq;
+    w[woff + i] = s;
+    w[(woff == 0 ? ns : 0) + i] = 0.0;
+  }
+}
+
+__global__ void k_bav(int ns, int ks, const double* __restrict__ a, const double* __restrict__ b, int sym, int left,
+                      double* __restrict__ c) {
+#pragma clang fp contract(off)
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ns; i += gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int j = 0; j < ns; ++j)
+      s += (left ? band_elem(a, ns, ks, sym, j, i) : band_elem(a, ns, ks, sym, i, j)) * b[j];
+    c[i] = s;
+  }
+}
+
+// density of one orbital pair in storage type (0 's', 2 'n', 3 'x'), the
+// element rule of density (dbsr_lib_dbs.f90:2834-2891)
+__device__ __forceinline__ double dens_elem(const double* p1, const double* p2, int ns, int ks, int type, int i,
+                                            int j) {
+#pragma clang fp contract(off)
+  if (type == 0) {
+    if (j == 0) return p1[i] * p2[i];
+    if (i < ns - j) return p1[i] * p2[i + j] + p1[i + j] * p2[i];
+    return 0.0;
+  }
+  if (type == 2) {
+    const int jf = j + 1, iff = i + 1;
+    const int imin = max(1, 1 + ks - jf), imax = min(ns, ns + ks - jf);
+    return (iff >= imin && iff <= imax) ? p1[i] * p2[i + jf - ks] : 0.0;
+  }
+  return p1[i] * p2[j];
+}
+
+// xx_g = sum over the run's entries (list order) of c_e * density(pq(jp1,
+// j1_e), pq(jp2, j2_e)), as S_data's xx = xx + cdata*dd
+__global__ void k_run_density(int ng, const int* __restrict__ goff, const int* __restrict__ ej1,
+                              const int* __restrict__ ej2, const double* __restrict__ ec,
+                              const double* __restrict__ pq, int ns, int ks, int jp1, int jp2, int type,
+                              double* __restrict__ xx) {
+#pragma clang fp contract(off)
+  const int ncol = type == 0 ? ks : (type == 2 ? 2 * ks - 1 : ns);
+  const long long per = (long long)ns * ncol;
+  const long long total = per * ng;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+       q += (long long)gridDim.x * blockDim.x) {
+    const int g = (int)(q / per);
+    const int r = (int)(q % per);
+    const int i = r % ns, j = r / ns;
+    double acc = 0.0;
+    for (int e = goff[g]; e < goff[g + 1]; ++e) {
+      const double* p1 = pq + (long long)ns * ((jp1 - 1) + 2LL * (ej1[e] - 1));
+      const double* p2 = pq + (long long)ns * ((jp2 - 1) + 2LL * (ej2[e] - 1));
+      acc = acc + ec[e] * dens_elem(p1, p2, ns, ks, type, i, j);
+    }
+    xx[q] = acc;
+  }
+}
+
+// UPDATE_HS, MAT3 form (DBSR_MAT3/updates.f90:110-179), one block: y =
+// unfold(d, sym); ich>jch: h(ii,jj) += y; jch>ich: += y^T; ich==jch: pass 0
+// += y/2, pass 1 h(jj,ii) += (y/2)^T
+__global__ void k_update_hs_one(int ns, int ks, int sym, int ich, int jch, int ii, int jj, int pass,
+                                const double* __restrict__ d, double* __restrict__ h) {
+#pragma clang fp contract(off)
+  const int ms = 2 * ns;
+  const long long total = (long long)ns * ns;
+  const int i1 = (ii - 1) * ns, j1 = (jj - 1) * ns;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+       q += (long long)gridDim.x * blockDim.x) {
+    const int a = (int)(q % ns), b = (int)(q / ns);
+    if (ich > jch) {
+      h[(i1 + a) + (long long)ms * (j1 + b)] += band_elem(d, ns, ks, sym, a, b);
+    } else if (jch > ich) {
+      h[(i1 + a) + (long long)ms * (j1 + b)] += band_elem(d, ns, ks, sym, b, a);
+    } else if (pass == 0) {
+      h[(i1 + a) + (long long)ms * (j1 + b)] += band_elem(d, ns, ks, sym, a, b) / 2.0;
+    } else {
+      h[(j1 + a) + (long long)ms * (i1 + b)] += band_elem(d, ns, ks, sym, b, a) / 2.0;
+    }
+  }
+}
+
+namespace {
+
+unsigned grid_for(long long n, int block) {
+  long long g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > 65535) g = 65535;
+  return (unsigned)g;
+}
+
+int sym_of(char c) { return c == 's' ? 0 : c == 'l' ? 1 : c == 'n' ? 2 : c == 'x' ? 3 : -1; }
+
+// device buffer freed after the stream drained
+struct DevMem {
+  hipStream_t s;
+  std::vector<void*> p;
+  explicit DevMem(hipStream_t st) : s(st) {}
+  ~DevMem() {
+    if (!p.empty()) (void)hipStreamSynchronize(s);
+    for (void* x : p) (void)hipFree(x);
+  }
+  template <class T>
+  int alloc(T** d, size_t n) {
+    void* x = nullptr;
+    if (hipMalloc(&x, sizeof(T) * std::max<size_t>(n, 1)) != hipSuccess) return fail(DBS_ERR_HIP, "hipMalloc");
+    p.push_back(x);
+    *d = (T*)x;
+    return DBS_OK;
+  }
+  template <class T>
+  int put(const std::vector<T>& h, T** d) {
+    int rc = alloc(d, h.size());
+    if (rc || h.empty()) return rc;
+    if (hipMemcpyAsync(*d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, s) != hipSuccess)
+      return fail(DBS_ERR_HIP, "hipMemcpyAsync");
+    return DBS_OK;
+  }
+};
+
+// the component/case table of one structure
+struct Plan {
+  int type, kb, ip1, ip2, jp1, jp2, icase;
+  char sym_i, sym_j, sym_d, sym_r, sym_v;
+};
+
+// R_data.f90:65-86 (stype 0) and S_data.f90:73-115 (stype 1)
+int make_plan(int stype, int itype, int atype, int ks, Plan* p) {
+  static const int s_tab[2][9][6] = {
+      {{1, 2, 1, 2, 2, 0}, {1, 2, 1, 2, 1, 'r'}, {1, 2, 1, 2, 2, 'r'}, {2, 1, 1, 2, 1, 'l'}, {2, 1, 1, 2, 2, 'l'},
+       {1, 2, 1, 2, 1, 0}, {1, 2, 1, 2, 2, 0}, {1, 2, 2, 1, 3, 0}, {2, 1, 1, 2, 4, 0}},
+      {{1, 2, 2, 1, 2, 0}, {1, 2, 2, 1, 1, 'r'}, {2, 1, 1, 2, 2, 'r'}, {2, 1, 2, 1, 1, 'l'}, {1, 2, 1, 2, 2, 'l'},
+       {1, 2, 2, 1, 1, 0}, {2, 1, 1, 2, 2, 0}, {1, 1, 2, 2, 3, 0}, {2, 2, 1, 1, 4, 0}}};
+  Plan q{};
+  q.sym_v = 'r';
+  if (stype == 0) {
+    DBS_CHECK(itype >= 1 && itype <= 4, DBS_ERR_ARG, " R_data: unknown itype ");
+    DBS_CHECK(atype >= 0 && atype <= 3, DBS_ERR_ARG, "R_data: atype out of range");
+    q.type = atype + 1;
+    q.kb = ks;
+    q.sym_i = q.sym_j = 's';
+    q.icase = itype == 1 ? 2 : (itype == 4 ? 3 : 1);
+    q.sym_d = q.sym_r = itype == 4 ? 'x' : 's';
+    const int tab[4][4] = {{1, 1, 1, 1}, {2, 2, 2, 2}, {1, 1, 2, 2}, {2, 2, 1, 1}};
+    q.ip1 = tab[atype][0]; q.ip2 = tab[atype][1]; q.jp1 = tab[atype][2]; q.jp2 = tab[atype][3];
+    if (itype == 4 && atype == 2) { q.ip1 = 1; q.ip2 = 2; q.jp1 = 1; q.jp2 = 2; }
+    if (itype == 4 && atype == 3) { q.ip1 = 2; q.ip2 = 1; q.jp1 = 2; q.jp2 = 1; }
+  } else {
+    DBS_CHECK(itype >= 1 && itype <= 9, DBS_ERR_ARG, " S_data: unknown itype ");
+    DBS_CHECK(atype == 0 || atype == 1, DBS_ERR_ARG, "S_data: atype out of range");
+    const int* t = s_tab[atype][itype - 1];
+    q.type = DBS_PPQQ + atype;
+    q.kb = 2 * ks - 1;
+    q.ip1 = t[0]; q.ip2 = t[1]; q.jp1 = t[2]; q.jp2 = t[3]; q.icase = t[4];
+    if (t[5]) q.sym_v = (char)t[5];
+    q.sym_i = q.sym_j = 'n';
+    q.sym_d = q.sym_r = q.icase <= 2 ? 'n' : 'x';
+  }
+  *p = q;
+  return DBS_OK;
+}
+
+int band_ncol(char sym, int ns, int ks) { return sym == 'x' ? ns : (sym == 'n' ? 2 * ks - 1 : ks); }
+
+struct Aux {
+  const dbs_mat3_aux* a;
+  const char* kbort_msg;
+  // 0-based V_ch column of the bound orbital io = i1*ibo + i2; *i1 = io/ibo
+  int vcol(int io, int* col, int* i1) const {
+    DBS_CHECK(a->ibo > 0 && a->ipbs && a->kbort && a->vch_dev, DBS_ERR_ARG, "bound-orbital tables missing");
+    *i1 = io / a->ibo;
+    DBS_CHECK(*i1 >= 1, DBS_ERR_ARG, "orbital index < 1");
+    const int k = a->kbort(a->kbort_user, io / a->ibo, io % a->ibo);
+    if (k == 0) return fail(DBS_ERR_ARG, kbort_msg);
+    DBS_CHECK(k >= 1 && k <= a->nvch, DBS_ERR_ARG, "KBORT column out of range");
+    *col = k - 1;
+    return DBS_OK;
+  }
+  int ipbs(int orb) const { return a->ipbs[orb - 1]; }
+};
+
+// structure 1: Rk/Sk( . . ; . . ) times ic,jc / <i|.> / <i|.><.|j> / <i|j>
+int structure1(dbs_ctx* ctx, const dbs_mat_blocks* m, const Aux& X, const Plan& P, int stype, int k, int nc,
+               const double* c, const int* K1, const int* K2, const int* K3, const int* K4,
+               const double* pq_dev, int norb) {
+  const int ibo = X.a->ibo;
+  DBS_CHECK(ibo > 0, DBS_ERR_ARG, "ibo < 1");
+  std::vector<int> rows;
+  rows.reserve(5 * (size_t)nc);
+  for (int j = 0; j < nc; ++j) {
+    const int a = K1[j] / ibo, cc = K1[j] % ibo, b = K2[j] / ibo, d = K2[j] % ibo;
+    DBS_CHECK(a >= 1 && a <= norb && cc >= 1 && cc <= norb && b >= 1 && b <= norb && d >= 1 && d <= norb,
+              DBS_ERR_ARG, "R/S_data: orbital index out of range");
+    rows.insert(rows.end(), {k, a, b, cc, d});
+  }
+  // the integrals S of the entries, in one engine pass
+  dbs_engine* eng = nullptr;
+  int rc = dbs_engine_create(&eng, 1, &ctx, &norb, &nc, rows.data(), P.type);
+  if (rc) return rc;
+  std::vector<double> S(nc);
+  double *epq = nullptr, *evals = nullptr;
+  dbs_engine_buffers(eng, 0, &epq, nullptr, &evals, nullptr, nullptr);
+  hipStream_t s = ctx->stream;
+  do {
+    // the engine runs on its own stream: the orbitals must have landed first
+    if (hipMemcpyAsync(epq, pq_dev, sizeof(double) * (size_t)ctx->ns * 2 * norb, hipMemcpyDeviceToDevice, s) !=
+            hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) { rc = fail(DBS_ERR_HIP, "R/S_data: copy"); break; }
+    if ((rc = dbs_engine_forward(eng))) break;
+    if ((rc = dbs_engine_sync(eng))) break;
+    if (hipMemcpy(S.data(), evals, sizeof(double) * nc, hipMemcpyDeviceToHost) != hipSuccess)
+      rc = fail(DBS_ERR_HIP, "R/S_data: values");
+  } while (0);
+  dbs_engine_destroy(eng);
+  if (rc) return rc;
+  const char* unknown = stype ? " S_data: unknown structure for itype=1" : " R_data: unknown structure for itype=1";
+  std::vector<MatUpd> u;
+  u.reserve(nc);
+  for (int j = 0; j < nc; ++j) {
+    const double C = S[j] * c[j];
+    const int ic = -K3[j], jc = -K4[j], io = -ic, jo = -jc;
+    if (ic > 0 && jc > 0) {
+      long long t = -1;
+      if ((rc = hb_target(m, ic, jc, &t))) return rc;
+      u.push_back({MatUpd::HB, 0, 0, 0, t, C});
+    } else if (ic < 0 && jc > 0) {
+      int vc = 0, i1 = 0, col = 0;
+      if ((rc = X.vcol(io, &vc, &i1)) || (rc = hv_column(m, X.ipbs(i1), jc, &col))) return rc;
+      u.push_back({MatUpd::HV, 0, vc, 0, (long long)col - 1, C});
+    } else if (ic < 0 && jc < 0) {
+      int vi = 0, i1 = 0, wj = 0, j1 = 0, b = 0;
+      if ((rc = X.vcol(io, &vi, &i1)) || (rc = X.vcol(jo, &wj, &j1))) return rc;
+      const int ich = X.ipbs(i1), jch = X.ipbs(j1);
+      if ((rc = chan_block(m, ich, jch, "UPDATE_HW", &b))) return rc;
+      u.push_back({MatUpd::HW, ich > jch ? 0 : (jch > ich ? 1 : 2), vi, wj, (long long)b - 1, C});
+    } else if (ic < 0 && jc == 0) {
+      DBS_CHECK(X.a->ipbs, DBS_ERR_ARG, "bound-orbital tables missing");
+      DBS_CHECK(io / ibo >= 1 && io % ibo >= 1, DBS_ERR_ARG, "orbital index < 1");
+      const int ich = X.ipbs(io / ibo), jch = X.ipbs(io % ibo);
+      if ((rc = target_h(m, ich, jch, X.a->iptar, X.a->htarg, C))) return rc;
+      if (X.a->iitar > 0 && ich != jch) {
+        int b = 0;
+        if ((rc = chan_block(m, ich, jch, "UPDATE_HX", &b))) return rc;
+        u.push_back({MatUpd::HX, ich >= jch ? 0 : 1, 0, 0, (long long)b - 1, C});
+      }
+    } else {
+      return fail(DBS_ERR_ARG, unknown);
+    }
+  }
+  return apply_mat3(ctx, m, u, X.a->vch_dev, ctx->ms, nullptr, X.a->fppqq_dev, stype ? "S_data" : "R_data");
+}
+
+// structures 2-5: Rk/Sk( i . . .) <j|.> or ic -- bound-channel vectors
+int structure2(dbs_ctx* ctx, const dbs_mat_blocks* m, const Aux& X, const Plan& P, int stype, int k, int nc,
+               const double* c, const int* K1, const int* K2, const int* K3, const int* K4,
+               const double* pq_dev, int norb) {
+  const int ns = ctx->ns, ks = ctx->ks, ibo = X.a->ibo;
+  DBS_CHECK(ibo > 0, DBS_ERR_ARG, "ibo < 1");
+  hipStream_t s = ctx->stream;
+  // distinct pairs K1 -> matrices, distinct (K1, orbital) -> vectors
+  std::map<int, int> gidx;
+  std::vector<int> gkey;
+  std::map<std::pair<int, int>, int> vidx;
+  std::vector<int2> vg;
+  std::vector<int> evec(nc);
+  for (int j = 0; j < nc; ++j) {
+    const int jj = K1[j], ii = K2[j];
+    DBS_CHECK(jj / ibo >= 1 && jj / ibo <= norb && jj % ibo >= 1 && jj % ibo <= norb && ii >= 1 && ii <= norb,
+              DBS_ERR_ARG, "R/S_data: orbital index out of range");
+    auto it = gidx.find(jj);
+    int g;
+    if (it == gidx.end()) {
+      g = (int)gkey.size();
+      gidx[jj] = g;
+      gkey.push_back(jj);
+    } else {
+      g = it->second;
+    }
+    auto iv = vidx.find({g, ii});
+    if (iv == vidx.end()) {
+      evec[j] = (int)vg.size();
+      vidx[{g, ii}] = evec[j];
+      vg.push_back(make_int2(g, ii));
+    } else {
+      evec[j] = iv->second;
+    }
+  }
+  // the updates (validated before any launch)
+  const char* unknown = stype ? " S_data: uknown structure for itype 2-5 " : " R_data: uknown structure for itype=2 ";
+  std::vector<MatUpd> u;
+  int rc = DBS_OK;
+  for (int j = 0; j < nc; ++j) {
+    const int ich = K3[j], io = K4[j];
+    const int wv = -evec[j] - 1;
+    if (io > 0) {
+      int wc = 0, j1 = 0, b = 0;
+      if ((rc = X.vcol(io, &wc, &j1))) return rc;
+      const int jch = X.ipbs(j1);
+      if ((rc = chan_block(m, ich, jch, "UPDATE_HW", &b))) return rc;
+      u.push_back({MatUpd::HW, ich > jch ? 0 : (jch > ich ? 1 : 2), wv, wc, (long long)b - 1, c[j]});
+    } else if (io < 0) {
+      int col = 0;
+      if ((rc = hv_column(m, ich, -io, &col))) return rc;
+      u.push_back({MatUpd::HV, 0, wv, 0, (long long)col - 1, c[j]});
+    } else {
+      return fail(DBS_ERR_ARG, unknown);
+    }
+  }
+  double* rkb = nullptr;
+  if ((rc = build_rkb(ctx, P.type, k, &rkb))) return rc;
+  const int ng = (int)gkey.size(), nvec = (int)vg.size();
+  const int ncd = band_ncol(P.sym_d, ns, ks);
+  const int nco = band_ncol(P.icase == 1 ? P.sym_i : P.sym_j, ns, ks);
+  DevMem mem(s);
+  double *d = nullptr, *A = nullptr, *W = nullptr;
+  int2* d_vg = nullptr;
+  if ((rc = mem.alloc(&d, (size_t)ns * ncd)) || (rc = mem.alloc(&A, (size_t)ns * nco * ng)) ||
+      (rc = mem.alloc(&W, (size_t)2 * ns * nvec)) || (rc = mem.put(vg, &d_vg)))
+    return rc;
+  for (int g = 0; g < ng; ++g) {
+    const int jj = gkey[g];
+    const double* p1 = pq_dev + (size_t)ns * ((P.jp1 - 1) + 2 * (size_t)(jj / ibo - 1));
+    const double* p2 = pq_dev + (size_t)ns * ((P.jp2 - 1) + 2 * (size_t)(jj % ibo - 1));
+    if ((rc = dbs_density_dev(ctx, p1, p2, P.sym_d, d))) return rc;
+    if ((rc = convol_launch(ctx, rkb, P.type, P.icase, P.sym_i, P.sym_j, d, A + (size_t)ns * nco * g))) return rc;
+  }
+  // R: BAV(xx, pq(ip1,ii), 'r'); S: BAV(xx, pq(ip2,ii), sym_v); both placed by ip2
+  const int comp = stype ? P.ip2 : P.ip1;
+  const int left = stype ? (P.sym_v == 'l') : 0;
+  k_bav_list<<<grid_for((long long)nvec * ns, 256), 256, 0, s>>>(nvec, d_vg, A, (long long)ns * nco,
+                                                                  sym_of(P.sym_r), left, pq_dev, comp, ns, ks,
+                                                                  P.ip2 == 1 ? 0 : ns, W);
+  DBS_HIP(hipGetLastError());
+  return apply_mat3(ctx, m, u, X.a->vch_dev, ctx->ms, W, nullptr, stype ? "S_data" : "R_data");
+}
+
+// structures 6-9 (S): Sk(i . j .) etc. -- channel-channel blocks
+int structure6(dbs_ctx* ctx, const dbs_mat_blocks* m, const Aux& X, const Plan& P, int k, int nc, const double* c,
+               const int* K1, const int* K2, const int* K3, const double* pq_dev, int norb) {
+  const int ns = ctx->ns, ks = ctx->ks, ibo = X.a->ibo;
+  DBS_CHECK(ibo > 0, DBS_ERR_ARG, "ibo < 1");
+  hipStream_t s = ctx->stream;
+  std::vector<int> goff(1, 0), ej1, ej2, gch, gjch, gblk;
+  std::vector<double> ec;
+  int rc = DBS_OK;
+  for (int j = 0; j < nc; ++j) {
+    DBS_CHECK(K2[j] >= 1 && K2[j] <= norb && K3[j] >= 1 && K3[j] <= norb, DBS_ERR_ARG,
+              "S_data: orbital index out of range");
+    ej1.push_back(K2[j]);
+    ej2.push_back(K3[j]);
+    ec.push_back(c[j]);
+    if (j < nc - 1 && K1[j] == K1[j + 1]) continue;
+    const int ich = K1[j] / ibo, jch = K1[j] % ibo;
+    int b = 0;
+    if ((rc = chan_block(m, ich, jch, "UPDATE_HS", &b))) return rc;
+    if (b == 0) {  // block not stored: drop the run
+      ej1.resize(goff.back()); ej2.resize(goff.back()); ec.resize(goff.back());
+      continue;
+    }
+    gch.push_back(ich);
+    gjch.push_back(jch);
+    gblk.push_back(b - 1);
+    goff.push_back((int)ec.size());
+  }
+  const int ng = (int)gblk.size();
+  if (ng == 0) return DBS_OK;
+  double* rkb = nullptr;
+  if ((rc = build_rkb(ctx, P.type, k, &rkb))) return rc;
+  const int ncd = band_ncol(P.sym_d, ns, ks);
+  const int nco = P.icase >= 3 ? ns : band_ncol(P.icase == 1 ? P.sym_i : P.sym_j, ns, ks);
+  DevMem mem(s);
+  int *d_goff, *d_j1, *d_j2;
+  double *d_c, *XX, *D;
+  if ((rc = mem.put(goff, &d_goff)) || (rc = mem.put(ej1, &d_j1)) || (rc = mem.put(ej2, &d_j2)) ||
+      (rc = mem.put(ec, &d_c)) || (rc = mem.alloc(&XX, (size_t)ns * ncd * ng)) ||
+      (rc = mem.alloc(&D, (size_t)ns * nco * ng)))
+    return rc;
+  k_run_density<<<grid_for((long long)ns * ncd * ng, 256), 256, 0, s>>>(ng, d_goff, d_j1, d_j2, d_c, pq_dev, ns, ks,
+                                                                         P.jp1, P.jp2, sym_of(P.sym_d), XX);
+  DBS_HIP(hipGetLastError());
+  const int ms = ctx->ms;
+  for (int g = 0; g < ng; ++g) {
+    double* dg = D + (size_t)ns * nco * g;
+    if ((rc = convol_launch(ctx, rkb, P.type, P.icase, P.sym_i, P.sym_j, XX + (size_t)ns * ncd * g, dg))) return rc;
+    double* h = m->hch_dev + (size_t)ms * ms * gblk[g];
+    for (int pass = 0; pass < (gch[g] == gjch[g] ? 2 : 1); ++pass)
+      k_update_hs_one<<<grid_for((long long)ns * ns, 256), 256, 0, s>>>(ns, ks, sym_of(P.sym_r), gch[g], gjch[g],
+                                                                         P.ip1, P.ip2, pass, dg, h);
+  }
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return fail(DBS_ERR_HIP, std::string("S_data: ") + hipGetErrorString(e));
+  return DBS_OK;
+}
+
+int rs_data(dbs_ctx* ctx, const dbs_mat_blocks* m, const dbs_mat3_aux* aux, int stype, int itype, int atype, int k,
+            int nc, const double* c, const int* K1, const int* K2, const int* K3, const int* K4,
+            const double* pq_dev, int norb) {
+  const char* who = stype ? "S_data" : "R_data";
+  DBS_CHECK(ctx && aux && pq_dev, DBS_ERR_ARG, std::string(who) + ": null argument");
+  int rc = check_blocks(m);
+  if (rc) return rc;
+  Plan P;
+  if ((rc = make_plan(stype, itype, atype, ctx->ks, &P))) return rc;
+  if (nc <= 0) return DBS_OK;
+  DBS_CHECK(c && K1 && K2 && K3 && K4, DBS_ERR_ARG, std::string(who) + ": null list");
+  Aux X{aux, stype ? "S: KBORT = 0" : "RK: KBORT =0"};
+  if (stype == 0 && (itype == 3 || itype == 4))
+    return dbs_rdata_34_dev(ctx, itype, atype, k, nc, c, K1, K2, K3, aux->ibo, pq_dev, norb, m->icc, m->nch,
+                            m->hch_dev, m->nblocks);
+  if (itype == 1) return structure1(ctx, m, X, P, stype, k, nc, c, K1, K2, K3, K4, pq_dev, norb);
+  if (itype <= 5) return structure2(ctx, m, X, P, stype, k, nc, c, K1, K2, K3, K4, pq_dev, norb);
+  return structure6(ctx, m, X, P, k, nc, c, K1, K2, K3, pq_dev, norb);
+}
+
+}  // namespace
+}  // namespace dbs
+
+using namespace dbs;
+
+extern "C" {
+
+int dbs_rdata_dev(dbs_ctx* ctx, const dbs_mat_blocks* m, const dbs_mat3_aux* aux, int itype, int atype, int k,
+                  int nc, const double* c, const int* K1, const int* K2, const int* K3, const int* K4,
+                  const double* pq_dev, int norb) {
+  return rs_data(ctx, m, aux, 0, itype, atype, k, nc, c, K1, K2, K3, K4, pq_dev, norb);
+}
+
+int dbs_sdata_dev(dbs_ctx* ctx, const dbs_mat_blocks* m, const dbs_mat3_aux* aux, int ktype, int k, int nc,
+                  const double* c, const int* K1, const int* K2, const int* K3, const int* K4, const double* pq_dev,
+                  int norb) {
+  const int ibtype = 10;  // mod_dbsr_mat.f90:103
+  return rs_data(ctx, m, aux, 1, ktype % ibtype, ktype / ibtype, k, nc, c, K1, K2, K3, K4, pq_dev, norb);
+}
+
+int dbs_bav_dev(dbs_ctx* ctx, const double* a_dev, const double* b_dev, char sym, char dir, double* c_dev) {
+  DBS_CHECK(ctx && a_dev && b_dev && c_dev, DBS_ERR_ARG, "BAV: null argument");
+  const int sc = sym_of(sym);
+  DBS_CHECK(sc >= 0, DBS_ERR_ARG, "BAV: unknown symmetry");
+  DBS_CHECK(dir == 'l' || dir == 'r', DBS_ERR_ARG, "BAV: unknown direction");
+  k_bav<<<grid_for(ctx->ns, 256), 256, 0, ctx->stream>>>(ctx->ns, ctx->ks, a_dev, b_dev, sc, dir == 'l', c_dev);
+  DBS_HIP(hipGetLastError());
+  return DBS_OK;
+}
+
+}  // extern "C"

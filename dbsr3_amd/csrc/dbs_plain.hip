@@ -679,6 +679,33 @@ using namespace dbs;
 
 static bool valid_sym(char s) { return s == 's' || s == 'l' || s == 'n'; }
 
+namespace dbs {
+int convol_launch(dbs_ctx* c, const double* rkb, int type, int icase, char si, char sj, const double* d,
+                  double* a) {
+  const int ns = c->ns, ks = c->ks, kb = kband(type, ks);
+  if (icase == 1) {
+    const int jchunk = 32;
+    int nci = si == 'n' ? 2 * ks - 1 : ks, ncj = sj == 'n' ? 2 * ks - 1 : ks;
+    int nch = (ns + jchunk - 1) / jchunk;
+    size_t need = (size_t)ncj * nch * ns * nci;
+    int rc = ensure_scratch(c, need);
+    if (rc) return rc;
+    dim3 grid(nci, ncj, nch);
+    k_convol1_partial<<<grid, 256, 0, c->stream>>>(ns, ks, kb, si, sj, jchunk, rkb, d, c->scratch);
+    k_convol1_reduce<<<(ns * nci + 255) / 256, 256, 0, c->stream>>>(ns, ks, si, ncj * nch, c->scratch, a);
+  } else if (icase == 2) {
+    int ncj = sj == 'n' ? 2 * ks - 1 : ks;
+    size_t threads = (size_t)ns * ncj * 64;
+    k_convol2<<<(unsigned)((threads + 255) / 256), 256, 0, c->stream>>>(ns, ks, kb, si, sj, rkb, d, a);
+  } else {
+    size_t n = (size_t)ns * ns;
+    k_convol34<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(ns, ks, kb, icase, si, sj, rkb, d, a);
+  }
+  DBS_HIP(hipGetLastError());
+  return DBS_OK;
+}
+}  // namespace dbs
+
 extern "C" {
 
 int dbs_moments_dev(dbs_ctx* c, int which, int k, double* rkm) {
@@ -734,28 +761,7 @@ int dbs_convol_dev(dbs_ctx* c, int type, int k, int icase, char si, char sj, con
   double* rkb = nullptr;
   int rc = build_rkb(c, type, k, &rkb);
   if (rc) return rc;
-  const int ns = c->ns, ks = c->ks, kb = kband(type, ks);
-  if (icase == 1) {
-    const int jchunk = 32;
-    int nci = si == 'n' ? 2 * ks - 1 : ks, ncj = sj == 'n' ? 2 * ks - 1 : ks;
-    int nch = (ns + jchunk - 1) / jchunk;
-    size_t need = (size_t)ncj * nch * ns * nci;
-    rc = ensure_scratch(c, need);
-    if (rc) return rc;
-    dim3 grid(nci, ncj, nch);
-    k_convol1_partial<<<grid, 256, 0, c->stream>>>(ns, ks, kb, si, sj, jchunk, rkb, d, c->scratch);
-    k_convol1_reduce<<<(ns * nci + 255) / 256, 256, 0, c->stream>>>(ns, ks, si, ncj * nch,
-                                                                      c->scratch, a);
-  } else if (icase == 2) {
-    int ncj = sj == 'n' ? 2 * ks - 1 : ks;
-    size_t threads = (size_t)ns * ncj * 64;
-    k_convol2<<<(unsigned)((threads + 255) / 256), 256, 0, c->stream>>>(ns, ks, kb, si, sj, rkb, d, a);
-  } else {
-    size_t n = (size_t)ns * ns;
-    k_convol34<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(ns, ks, kb, icase, si, sj, rkb, d, a);
-  }
-  DBS_HIP(hipGetLastError());
-  return DBS_OK;
+  return convol_launch(c, rkb, type, icase, si, sj, d, a);
 }
 
 int dbs_sum_amb_dev(dbs_ctx* c, const double* a, const double* b, char sym, double* out) {

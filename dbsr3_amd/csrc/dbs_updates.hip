@@ -131,18 +131,19 @@ __global__ void k_accumulate_list(int ntarget, const long long* __restrict__ tpo
   }
 }
 
-// One channel-block update of a list: HX with the matrix d (mode = trans)
-// or HW with columns v,w of V (mode 0/1/2).
-struct BlkUpd {
-  int hw, mode, v, w;
-  double C;
-};
+// vector of a list entry: idx >= 0 column idx of V (leading dimension ldv),
+// idx < 0 column -idx-1 of W (leading dimension ms)
+__device__ __forceinline__ const double* entry_vec(int idx, const double* V, int ldv, const double* W, int ms) {
+  return idx >= 0 ? V + (long long)ldv * idx : W + (long long)ms * (-idx - 1);
+}
 
 // Block targets on blockIdx.y (strided), elements on x.  Each element
-// applies its target's entries in list order.
+// applies its target's entries in list order: HX with the matrix d
+// (mode = transpose) or HW with vectors v, w (mode 0/1/2).
 __global__ void k_block_list(int ms, int ntarget, const int* __restrict__ tblk, const int* __restrict__ toff,
-                             const BlkUpd* __restrict__ ent, const double* __restrict__ d,
-                             const double* __restrict__ V, int ldv, double* __restrict__ hch) {
+                             const MatUpd* __restrict__ ent, const double* __restrict__ d,
+                             const double* __restrict__ V, int ldv, const double* __restrict__ W,
+                             double* __restrict__ hch) {
 #pragma clang fp contract(off)
   const long long total = (long long)ms * ms;
   for (int t = blockIdx.y; t < ntarget; t += gridDim.y) {
@@ -152,10 +153,10 @@ __global__ void k_block_list(int ms, int ntarget, const int* __restrict__ tblk, 
       const int i = (int)(q % ms), j = (int)(q / ms);
       double acc = blk[q];
       for (int e = toff[t]; e < toff[t + 1]; ++e) {
-        const BlkUpd u = ent[e];
-        if (u.hw) {
-          const double* v = V + (long long)ldv * u.v;
-          const double* w = V + (long long)ldv * u.w;
+        const MatUpd u = ent[e];
+        if (u.kind == MatUpd::HW) {
+          const double* v = entry_vec(u.v, V, ldv, W, ms);
+          const double* w = entry_vec(u.w, V, ldv, W, ms);
           acc = acc + hw_term(v[i], w[j], v[j], w[i], u.mode, u.C);
         } else {
           const double x = u.mode ? d[j + (long long)ms * i] : d[q];
@@ -167,16 +168,19 @@ __global__ void k_block_list(int ms, int ntarget, const int* __restrict__ tblk, 
   }
 }
 
-// column targets: col(tcol[t]) += C_e V(:,vcol_e) in list order
+// column targets: col(tcol[t]) += C_e v_e in list order
 __global__ void k_vec_list(int ms, int ntarget, const int* __restrict__ tcol, const int* __restrict__ toff,
-                           const int* __restrict__ vcol, const double* __restrict__ cval,
-                           const double* __restrict__ V, int ldv, double* __restrict__ hcp) {
+                           const MatUpd* __restrict__ ent, const double* __restrict__ V, int ldv,
+                           const double* __restrict__ W, double* __restrict__ hcp) {
 #pragma clang fp contract(off)
   for (int t = blockIdx.y; t < ntarget; t += gridDim.y) {
     double* col = hcp + (long long)ms * tcol[t];
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ms; i += gridDim.x * blockDim.x) {
       double acc = col[i];
-      for (int e = toff[t]; e < toff[t + 1]; ++e) acc = acc + cval[e] * V[i + (long long)ldv * vcol[e]];
+      for (int e = toff[t]; e < toff[t + 1]; ++e) {
+        const MatUpd u = ent[e];
+        acc = acc + u.C * entry_vec(u.v, V, ldv, W, ms)[i];
+      }
       col[i] = acc;
     }
   }
@@ -199,28 +203,6 @@ int sym_code(char sym) {
     case 'x': return 3;
     default: return -1;
   }
-}
-
-// icc(ich,jch) with the reference's checks (updates.f90:89-103); *b = 0
-// when the block is not stored here
-int chan_block(const dbs_mat_blocks* m, int ich, int jch, const char* who, int* b) {
-  const std::string w(who);
-  if (ich < 0) return fail(DBS_ERR_ARG, w + ": ich < 0");
-  if (jch < 0) return fail(DBS_ERR_ARG, w + ": jch < 0");
-  if (ich > m->nch) return fail(DBS_ERR_ARG, w + ": ich > nch");
-  if (jch > m->nch) return fail(DBS_ERR_ARG, w + ": jch > nch");
-  if (ich == 0 || jch == 0) return fail(DBS_ERR_ARG, w + ": channel index 0");
-  const int i = m->icc[(ich - 1) + (long long)m->nch * (jch - 1)];
-  if (i < 0 || i > m->nblocks) return fail(DBS_ERR_ARG, "UPDATE_HX: i out of range");
-  *b = i;
-  return DBS_OK;
-}
-
-int check_blocks(const dbs_mat_blocks* m) {
-  DBS_CHECK(m, DBS_ERR_ARG, "matrix storage: null descriptor");
-  DBS_CHECK(m->nch >= 0 && m->npert >= 0 && m->nblocks >= 0 && m->ncb >= 0 && m->nbb >= 0, DBS_ERR_ARG,
-            "matrix storage: negative dimension");
-  return DBS_OK;
 }
 
 // device copies of host lists, freed after the stream has drained
@@ -295,6 +277,30 @@ int accumulate_list(hipStream_t s, const std::vector<long long>& target, const d
   return finish(s, who);
 }
 
+}  // namespace
+
+// icc(ich,jch) with the reference's checks (updates.f90:89-103); *b = 0
+// when the block is not stored here
+int chan_block(const dbs_mat_blocks* m, int ich, int jch, const char* who, int* b) {
+  const std::string w(who);
+  if (ich < 0) return fail(DBS_ERR_ARG, w + ": ich < 0");
+  if (jch < 0) return fail(DBS_ERR_ARG, w + ": jch < 0");
+  if (ich > m->nch) return fail(DBS_ERR_ARG, w + ": ich > nch");
+  if (jch > m->nch) return fail(DBS_ERR_ARG, w + ": jch > nch");
+  if (ich == 0 || jch == 0) return fail(DBS_ERR_ARG, w + ": channel index 0");
+  const int i = m->icc[(ich - 1) + (long long)m->nch * (jch - 1)];
+  if (i < 0 || i > m->nblocks) return fail(DBS_ERR_ARG, "UPDATE_HX: i out of range");
+  *b = i;
+  return DBS_OK;
+}
+
+int check_blocks(const dbs_mat_blocks* m) {
+  DBS_CHECK(m, DBS_ERR_ARG, "matrix storage: null descriptor");
+  DBS_CHECK(m->nch >= 0 && m->npert >= 0 && m->nblocks >= 0 && m->ncb >= 0 && m->nbb >= 0, DBS_ERR_ARG,
+            "matrix storage: negative dimension");
+  return DBS_OK;
+}
+
 int hb_target(const dbs_mat_blocks* m, int ic, int jc, long long* t) {
   if (ic > m->npert || jc > m->npert || ic <= 0 || jc <= 0)
     return fail(DBS_ERR_ARG, "UPDATE_HB: index is out of range");
@@ -313,7 +319,84 @@ int hv_column(const dbs_mat_blocks* m, int ich, int ip, int* col) {
   return DBS_OK;
 }
 
-}  // namespace
+// Target_h (DBSR_MAT3/target_subs.f90:2-31): tgt(ij) += C for a stored
+// block, ij the packed (max,min) pair of the channels' targets; tgt may be
+// NULL (bookkeeping not kept)
+int target_h(const dbs_mat_blocks* m, int ich, int jch, const int* iptar, double* tgt, double C) {
+  if (ich < 0) return fail(DBS_ERR_ARG, "target_h: ich < 0");
+  if (jch < 0) return fail(DBS_ERR_ARG, "target_h: jch < 0");
+  if (ich > m->nch) return fail(DBS_ERR_ARG, "target_h: ich > nch");
+  if (jch > m->nch) return fail(DBS_ERR_ARG, "target_h: jch > nch");
+  DBS_CHECK(ich >= 1 && jch >= 1, DBS_ERR_ARG, "target_h: channel index 0");
+  if (m->icc[(ich - 1) + (long long)m->nch * (jch - 1)] == 0) return DBS_OK;
+  if (!tgt || !iptar) return DBS_OK;
+  const int it = iptar[ich - 1], jt = iptar[jch - 1];
+  const int hi = std::max(it, jt), lo = std::min(it, jt);
+  DBS_CHECK(lo >= 1, DBS_ERR_ARG, "target_h: target index < 1");
+  tgt[(long long)(hi - 1) * hi / 2 + lo - 1] += C;
+  return DBS_OK;
+}
+
+// the updates of one list, grouped by matrix and target (stable), each
+// target's entries applied in list order by one thread per element
+int apply_mat3(dbs_ctx* ctx, const dbs_mat_blocks* m, const std::vector<MatUpd>& u, const double* V, int ldv,
+               const double* W, const double* d, const char* who) {
+  hipStream_t s = ctx->stream;
+  const int ms = ctx->ms;
+  // HB: scalar targets
+  {
+    std::vector<long long> t(u.size(), -1);
+    std::vector<double> c(u.size());
+    bool any = false;
+    for (size_t e = 0; e < u.size(); ++e) {
+      c[e] = u[e].C;
+      if (u[e].kind == MatUpd::HB) { t[e] = u[e].target; any = any || t[e] >= 0; }
+    }
+    if (any) {
+      int rc = accumulate_list(s, t, c.data(), m->hp_dev, who);
+      if (rc) return rc;
+    }
+  }
+  // HV: columns of hcp; HW/HX: blocks of hch
+  for (int pass = 0; pass < 2; ++pass) {
+    std::vector<long long> t(u.size(), -1);
+    for (size_t e = 0; e < u.size(); ++e) {
+      const int k = u[e].kind;
+      if (pass == 0 ? k == MatUpd::HV : (k == MatUpd::HW || k == MatUpd::HX)) t[e] = u[e].target;
+    }
+    std::vector<int> order, toff;
+    std::vector<long long> tval;
+    group_by(t, order, tval, toff);
+    if (tval.empty()) continue;
+    std::vector<int> tt(tval.begin(), tval.end());
+    std::vector<MatUpd> es(order.size());
+    for (size_t q = 0; q < order.size(); ++q) {
+      es[q] = u[order[q]];
+      if (pass == 1 && es[q].kind == MatUpd::HX) DBS_CHECK(d, DBS_ERR_ARG, std::string(who) + ": fppqq missing");
+      if (es[q].kind != MatUpd::HX) {
+        DBS_CHECK((es[q].v >= 0 ? V != nullptr : W != nullptr), DBS_ERR_ARG, std::string(who) + ": vector missing");
+        if (es[q].kind == MatUpd::HW)
+          DBS_CHECK((es[q].w >= 0 ? V != nullptr : W != nullptr), DBS_ERR_ARG, std::string(who) + ": vector missing");
+      }
+    }
+    Uploads up(s);
+    int *d_t, *d_o;
+    MatUpd* d_e;
+    int rc;
+    if ((rc = up.put(tt, &d_t)) || (rc = up.put(toff, &d_o)) || (rc = up.put(es, &d_e))) return rc;
+    const unsigned gy = (unsigned)std::min<size_t>(tt.size(), 65535);
+    if (pass == 0) {
+      k_vec_list<<<dim3(grid_n(ms, 256), gy), 256, 0, s>>>(ms, (int)tt.size(), d_t, d_o, d_e, V, ldv, W,
+                                                          m->hcp_dev);
+    } else {
+      const unsigned gx = std::min(grid_n((long long)ms * ms, 256), std::max(1u, 262144u / gy));
+      k_block_list<<<dim3(gx, gy), 256, 0, s>>>(ms, (int)tt.size(), d_t, d_o, d_e, d, V, ldv, W, m->hch_dev);
+    }
+    if ((rc = finish(s, who))) return rc;
+  }
+  return DBS_OK;
+}
+
 }  // namespace dbs
 
 using namespace dbs;
@@ -423,8 +506,6 @@ int dbs_o_data_dev(dbs_ctx* ctx, const dbs_mat_blocks* m, int itype, int n, cons
   int rc = check_blocks(m);
   if (rc || n <= 0) return rc;
   DBS_CHECK(itype >= 1 && itype <= 4, DBS_ERR_ARG, "O_data: unknown structure (itype) ");
-  hipStream_t s = ctx->stream;
-  const int ms = ctx->ms;
   // orbital io = i1*ibo + i2 -> (i1, 0-based column KBORT(i1,i2)-1 of V_ch)
   auto vcol = [&](int io, int* col, int* i1) -> int {
     DBS_CHECK(ibo > 0 && ipbs && kbort && vch_dev, DBS_ERR_ARG, "O_data: bound-orbital tables missing");
@@ -436,101 +517,40 @@ int dbs_o_data_dev(dbs_ctx* ctx, const dbs_mat_blocks* m, int itype, int n, cons
     *col = k - 1;
     return DBS_OK;
   };
-
-  if (itype == 1) {  // <ic|jc>
-    std::vector<long long> t(n, -1);
-    for (int e = 0; e < n; ++e) {
-      if (std::fabs(c[e]) < eps_c) continue;
-      if ((rc = hb_target(m, k1[e], k2[e], &t[e]))) return rc;
-    }
-    return accumulate_list(s, t, c, m->hp_dev, "O_data");
-  }
-
-  if (itype == 2) {  // <i|.> jc: hcp(:,icb(ipbs(i1),jc)) += C*V_ch(:,KBORT)
-    std::vector<long long> t(n, -1);
-    std::vector<int> vc(n, 0);
-    for (int e = 0; e < n; ++e) {
-      if (std::fabs(c[e]) < eps_c) continue;
-      int i1 = 0, hc = 0;
-      if ((rc = vcol(k1[e], &vc[e], &i1))) return rc;
-      if ((rc = hv_column(m, ipbs[i1 - 1], k2[e], &hc))) return rc;
-      t[e] = hc - 1;
-    }
-    std::vector<int> order, toff;
-    std::vector<long long> tval;
-    group_by(t, order, tval, toff);
-    if (tval.empty()) return DBS_OK;
-    std::vector<int> tcol(tval.begin(), tval.end()), vcs(order.size());
-    std::vector<double> cvs(order.size());
-    for (size_t q = 0; q < order.size(); ++q) {
-      vcs[q] = vc[order[q]];
-      cvs[q] = c[order[q]];
-    }
-    Uploads up(s);
-    int *d_t, *d_o, *d_v;
-    double* d_c;
-    if ((rc = up.put(tcol, &d_t)) || (rc = up.put(toff, &d_o)) || (rc = up.put(vcs, &d_v)) ||
-        (rc = up.put(cvs, &d_c)))
-      return rc;
-    dim3 grid(grid_n(ms, 256), (unsigned)std::min<size_t>(tcol.size(), 65535));
-    k_vec_list<<<grid, 256, 0, s>>>(ms, (int)tcol.size(), d_t, d_o, d_v, d_c, vch_dev, ms, m->hcp_dev);
-    return finish(s, "O_data");
-  }
-
-  // itype 3 (<i|.><.|j>, UPDATE_HW) and 4 (<i|j>, Target_h + UPDATE_HX)
-  std::vector<long long> t(n, -1);
-  std::vector<BlkUpd> ent(n);
+  std::vector<MatUpd> u;
+  u.reserve(n);
   for (int e = 0; e < n; ++e) {
-    if (std::fabs(c[e]) < eps_c) continue;
-    if (itype == 3) {
+    const double C = c[e];
+    if (std::fabs(C) < eps_c) continue;
+    if (itype == 1) {  // <ic|jc>: UPDATE_HB
+      long long t = -1;
+      if ((rc = hb_target(m, k1[e], k2[e], &t))) return rc;
+      u.push_back({MatUpd::HB, 0, 0, 0, t, C});
+    } else if (itype == 2) {  // <i|.> jc: UPDATE_HV(ipbs(i1), jc, V_ch(:,KBORT))
+      int vc = 0, i1 = 0, hc = 0;
+      if ((rc = vcol(k1[e], &vc, &i1))) return rc;
+      if ((rc = hv_column(m, ipbs[i1 - 1], k2[e], &hc))) return rc;
+      u.push_back({MatUpd::HV, 0, vc, 0, (long long)hc - 1, C});
+    } else if (itype == 3) {  // <i|.><.|j>: UPDATE_HW
       int vi = 0, i1 = 0, wj = 0, j1 = 0, b = 0;
       if ((rc = vcol(k1[e], &vi, &i1)) || (rc = vcol(k2[e], &wj, &j1))) return rc;
       const int ich = ipbs[i1 - 1], jch = ipbs[j1 - 1];
       if ((rc = chan_block(m, ich, jch, "UPDATE_HW", &b))) return rc;
-      t[e] = b - 1;
-      ent[e] = {1, ich > jch ? 0 : (jch > ich ? 1 : 2), vi, wj, c[e]};
-    } else {
+      u.push_back({MatUpd::HW, ich > jch ? 0 : (jch > ich ? 1 : 2), vi, wj, (long long)b - 1, C});
+    } else {  // <i|j>: Target_h(ich,jch,0,C) and, for iitar > 1, UPDATE_HX(fppqq)
       DBS_CHECK(ibo > 0 && ipbs, DBS_ERR_ARG, "O_data: bound-orbital tables missing");
       const int io = k1[e], i1 = io / ibo, i2 = io % ibo;
       DBS_CHECK(i1 >= 1 && i2 >= 1, DBS_ERR_ARG, "O_data: orbital index < 1");
       const int ich = ipbs[i1 - 1], jch = ipbs[i2 - 1];
-      // Target_h(ich,jch,0,C): otarg(ij) += C for a stored block
-      if (ich < 0) return fail(DBS_ERR_ARG, "target_h: ich < 0");
-      if (jch < 0) return fail(DBS_ERR_ARG, "target_h: jch < 0");
-      if (ich > m->nch) return fail(DBS_ERR_ARG, "target_h: ich > nch");
-      if (jch > m->nch) return fail(DBS_ERR_ARG, "target_h: jch > nch");
-      DBS_CHECK(ich >= 1 && jch >= 1, DBS_ERR_ARG, "target_h: channel index 0");
-      const int blk = m->icc[(ich - 1) + (long long)m->nch * (jch - 1)];
-      if (otarg && iptar && blk != 0) {
-        const int it = iptar[ich - 1], jt = iptar[jch - 1];
-        const int hi = std::max(it, jt), lo = std::min(it, jt);
-        otarg[(long long)(hi - 1) * hi / 2 + lo - 1] += c[e];
-      }
+      if ((rc = target_h(m, ich, jch, iptar, otarg, C))) return rc;
       if (iitar > 1 && ich != jch) {
-        DBS_CHECK(fppqq_dev, DBS_ERR_ARG, "O_data: fppqq missing");
         int b = 0;
         if ((rc = chan_block(m, ich, jch, "UPDATE_HX", &b))) return rc;
-        t[e] = b - 1;
-        ent[e] = {0, ich >= jch ? 0 : 1, 0, 0, c[e]};
+        u.push_back({MatUpd::HX, ich >= jch ? 0 : 1, 0, 0, (long long)b - 1, C});
       }
     }
   }
-  std::vector<int> order, toff;
-  std::vector<long long> tval;
-  group_by(t, order, tval, toff);
-  if (tval.empty()) return DBS_OK;
-  std::vector<int> tblk(tval.begin(), tval.end());
-  std::vector<BlkUpd> es(order.size());
-  for (size_t q = 0; q < order.size(); ++q) es[q] = ent[order[q]];
-  Uploads up(s);
-  int *d_t, *d_o;
-  BlkUpd* d_e;
-  if ((rc = up.put(tblk, &d_t)) || (rc = up.put(toff, &d_o)) || (rc = up.put(es, &d_e))) return rc;
-  const unsigned gy = (unsigned)std::min<size_t>(tblk.size(), 65535);
-  const unsigned gx = std::min(grid_n((long long)ms * ms, 256), std::max(1u, 262144u / gy));
-  k_block_list<<<dim3(gx, gy), 256, 0, s>>>(ms, (int)tblk.size(), d_t, d_o, d_e, fppqq_dev, vch_dev, ms,
-                                            m->hch_dev);
-  return finish(s, "O_data");
+  return apply_mat3(ctx, m, u, vch_dev, ctx->ms, nullptr, fppqq_dev, "O_data");
 }
 
 int dbs_overlap_ci_dev(dbs_ctx* ctx, int n, const int* ic, const int* jc, const double* cc, double eps_c,

@@ -237,6 +237,45 @@ int dbs_o_data_dev(dbs_ctx *ctx, const dbs_mat_blocks *m, int itype, int n, cons
 int dbs_overlap_ci_dev(dbs_ctx *ctx, int n, const int *ic, const int *jc, const double *cc,
                        double eps_c, int nzero, double *SM_dev, int ldsm);
 
+/* Bound-orbital and target data of a MAT3 run (module dbsr_mat,
+ * DBSR_MAT3/mod_dbsr_mat.f90): pair packing base ibo (:61), ipbs(orbital)
+ * (channel of a channel orbital, 0 for bound; 1-based orbitals), the host
+ * program's KBORT, V_ch(ms,nvch) and fppqq(ms,ms) on the device, iitar,
+ * iptar(channel) and the packed target interactions htarg (host, may be
+ * NULL; Target_h, DBSR_MAT3/target_subs.f90:2-31). */
+typedef struct dbs_mat3_aux {
+  int ibo;
+  const int *ipbs;
+  dbs_kbort_fn kbort;
+  void *kbort_user;
+  const double *vch_dev;
+  int nvch;
+  const double *fppqq_dev;
+  int iitar;
+  const int *iptar;
+  double *htarg;
+} dbs_mat3_aux;
+/* R_data(itype, kpol, btype) (DBSR_MAT3/R_data.f90:2-290): one merged list
+ * (IPT order, host arrays) of structure itype 1..4 and atype 0..3 into the
+ * MAT3 matrices: 1 integral x {UPDATE_HB, UPDATE_HV, UPDATE_HW, Target_h +
+ * UPDATE_HX(fppqq)}; 2 the bound-channel vectors BAV(convol(case 1,
+ * density(K1)), pq(K2)) into UPDATE_HW / UPDATE_HV; 3/4 as dbs_rdata_34_dev.
+ * pq_dev(ns,2,norb). */
+int dbs_rdata_dev(dbs_ctx *ctx, const dbs_mat_blocks *m, const dbs_mat3_aux *aux, int itype, int atype,
+                  int k, int nc, const double *c, const int *K1, const int *K2, const int *K3,
+                  const int *K4, const double *pq_dev, int norb);
+/* S_data(ktype, kpol) (DBSR_MAT3/S_data.f90:2-287): ktype = atype*10 +
+ * itype, atype 0 ppqq / 1 pqqp, itype 1..9 (1 as R_data 1; 2..5 vectors by
+ * BAV; 6..9 runs of equal K1 into UPDATE_HS). */
+int dbs_sdata_dev(dbs_ctx *ctx, const dbs_mat_blocks *m, const dbs_mat3_aux *aux, int ktype, int k,
+                  int nc, const double *c, const int *K1, const int *K2, const int *K3, const int *K4,
+                  const double *pq_dev, int norb);
+/* BAV (called at DBSR_MAT3/R_data.f90:181, S_data.f90:205; defined in the
+ * un-vendored Libraries_03): c(ns) = A b (dir 'r') or A^T b (dir 'l'), A the
+ * (ns,ns) matrix held in storage sym 's','l','n','x'. */
+int dbs_bav_dev(dbs_ctx *ctx, const double *a_dev, const double *b_dev, char sym, char dir,
+                double *c_dev);
+
 /* ---------------------------------------------------------------------
  * Engine: batched evaluation of integral sets and their adjoint.
  * An instance = (context, norb orbitals, rows (k,a,b,c,d)).  Row n is

@@ -1479,3 +1479,176 @@ int orc_o_data(int ms, int itype, int n, const int *k1, const int *k2, const dou
   }
   return 0;
 }
+
+/* BAV: called at DBSR_MAT3/R_data.f90:181 and S_data.f90:205; its source
+ * lives in the un-vendored Libraries_03.  Restated from its name and call
+ * sites: A(ns,ns) unfolded from storage sym as Full_mat_sym does, then
+ * c = A b (dir 'r') or c = A^T b (dir 'l'), each element summed over the
+ * full index range in ascending order.  PARITY UNPINNED for 'n' storage and
+ * the 'l' direction (for symmetric storage both directions coincide and the
+ * contraction property pins it). */
+void orc_bav(int ns, int ks, const double *a, const double *b, double *c, char sym, char dir) {
+  double *y = (double *)malloc(sizeof(double) * (size_t)ns * ns);
+  unfold(ns, ks, a, sym, y);
+  for (int i = 0; i < ns; ++i) {
+    double s = 0.0;
+    for (int j = 0; j < ns; ++j) s += (dir == 'l' ? y[j + (size_t)ns * i] : y[i + (size_t)ns * j]) * b[j];
+    c[i] = s;
+  }
+  free(y);
+}
+
+/* component and case table of S_data (DBSR_MAT3/S_data.f90:73-106):
+ * [atype][itype-1] = {ip1, ip2, jp1, jp2, int, sym_v} */
+static const int s_tab[2][9][6] = {
+  {{1,2,1,2,2,0}, {1,2,1,2,1,'r'}, {1,2,1,2,2,'r'}, {2,1,1,2,1,'l'}, {2,1,1,2,2,'l'},
+   {1,2,1,2,1,0}, {1,2,1,2,2,0}, {1,2,2,1,3,0}, {2,1,1,2,4,0}},
+  {{1,2,2,1,2,0}, {1,2,2,1,1,'r'}, {2,1,1,2,2,'r'}, {2,1,2,1,1,'l'}, {1,2,1,2,2,'l'},
+   {1,2,2,1,1,0}, {2,1,1,2,2,0}, {1,1,2,2,3,0}, {2,2,1,1,4,0}}};
+
+/* R_data (DBSR_MAT3/R_data.f90:2-290, stype 0) and S_data (S_data.f90:2-287,
+ * stype 1) for one merged list in IPT order.  rkb is the array of the
+ * (type,k): R types atype+1 (ks band), S types 5+atype (2ks-1 band).
+ * MAT3 storage as orc_o_data; htarg: packed target interactions
+ * (Target_h(ich,jch,C,0)), may be NULL.  Returns 0, or a nonzero code where
+ * the reference stops: 1 HB range, 2 KBORT=0, 3 HV range, 4 channel range,
+ * 6/7 unknown structure. */
+int orc_rs_data(int ns, int ks, const double *rkb, int stype, int itype, int atype, int nc,
+                const double *cdata, const int *K1, const int *K2, const int *K3, const int *K4,
+                const double *pq, int ibo, const int *ipbs, orc_kbort_fn kbort, void *user,
+                const double *vch, const double *fppqq, int iitar, const int *iptar, double *htarg,
+                int nch, int npert, const int *icc, double *hch, const int *icb, double *hcp,
+                const int *ibb, double *hp) {
+  const int ms = 2 * ns;
+  int ip1 = 1, ip2 = 1, jp1 = 1, jp2 = 1, icase = 1, kb;
+  char sym_i, sym_j, sym_d, sym_r, sym_v = 'r';
+  if (stype == 0) {
+    if (itype == 3 || itype == 4) {
+      orc_rdata_34(ns, ks, rkb, itype, atype, nc, cdata, K1, K2, K3, pq, ibo, icc, nch, hch);
+      return 0;
+    }
+    kb = ks; sym_i = sym_j = 's';
+    if (itype == 1) { icase = 2; sym_d = 's'; sym_r = 's'; }
+    else if (itype == 2) { icase = 1; sym_d = 's'; sym_r = 's'; }
+    else return 7;
+    switch (atype) {
+      case 0: ip1 = 1; ip2 = 1; jp1 = 1; jp2 = 1; break;
+      case 1: ip1 = 2; ip2 = 2; jp1 = 2; jp2 = 2; break;
+      case 2: ip1 = 1; ip2 = 1; jp1 = 2; jp2 = 2; break;
+      default: ip1 = 2; ip2 = 2; jp1 = 1; jp2 = 1; break;
+    }
+  } else {
+    if (itype < 1 || itype > 9 || atype < 0 || atype > 1) return 7;
+    const int *t = s_tab[atype][itype - 1];
+    ip1 = t[0]; ip2 = t[1]; jp1 = t[2]; jp2 = t[3]; icase = t[4];
+    if (t[5]) sym_v = (char)t[5];
+    kb = 2 * ks - 1; sym_i = sym_j = 'n';
+    if (icase <= 2) { sym_d = 'n'; sym_r = 'n'; } else { sym_d = 'x'; sym_r = 'x'; }
+  }
+  size_t nn = (size_t)ns * ns;
+  double *xx = (double *)calloc(nn * 3, sizeof(double));
+  double *dd = xx + nn, *d = dd + nn;
+  double *v = (double *)malloc(sizeof(double) * (size_t)ns);
+  double *vv = (double *)malloc(sizeof(double) * (size_t)ms);
+  int rc = 0;
+#define PQ(ip, orb) (pq + (size_t)ns * (((ip) - 1) + 2 * (size_t)((orb) - 1)))
+#define VCH(k) (vch + (size_t)ms * ((k) - 1))
+  if (itype == 1) {
+    int iii = 0, jjj = 0;
+    double S = 0.0;
+    for (int j = 0; j < nc && !rc; ++j) {
+      if (K1[j] != iii) {
+        orc_density(ns, ks, d, PQ(ip1, K1[j] / ibo), PQ(ip2, K1[j] % ibo), sym_d);
+        orc_convol(ns, ks, kb, rkb, xx, d, icase, sym_i, sym_j);
+      }
+      if (K2[j] != jjj) orc_density(ns, ks, dd, PQ(jp1, K2[j] / ibo), PQ(jp2, K2[j] % ibo), sym_r);
+      if (K1[j] != iii || K2[j] != jjj) S = orc_sum_amb(ns, ks, xx, dd, sym_r);
+      iii = K1[j]; jjj = K2[j];
+      const double C = S * cdata[j];
+      const int ic = -K3[j], jc = -K4[j], io = -ic, jo = -jc;
+      if (ic > 0 && jc > 0) {                                   /* UPDATE_HB */
+        if (ic > npert || jc > npert) { rc = 1; break; }
+        const int i = ibb[(ic - 1) + (size_t)npert * (jc - 1)];
+        if (i > 0) hp[i - 1] = hp[i - 1] + C;
+      } else if (ic < 0 && jc > 0) {                            /* UPDATE_HV */
+        const int k = kbort(user, io / ibo, io % ibo);
+        if (k == 0) { rc = 2; break; }
+        const int ich = ipbs[io / ibo];
+        if (ich < 1 || ich > nch || jc > npert) { rc = 3; break; }
+        const int i = icb[(ich - 1) + (size_t)nch * (jc - 1)];
+        if (i > 0) orc_update_hv(ms, hcp + (size_t)ms * (i - 1), VCH(k), C);
+      } else if (ic < 0 && jc < 0) {                            /* UPDATE_HW */
+        const int ki = kbort(user, io / ibo, io % ibo);
+        if (ki == 0) { rc = 2; break; }
+        const int kj = kbort(user, jo / ibo, jo % ibo);
+        if (kj == 0) { rc = 2; break; }
+        const int ich = ipbs[io / ibo], jch = ipbs[jo / ibo];
+        if (ich < 1 || jch < 1 || ich > nch || jch > nch) { rc = 4; break; }
+        const int b = icc[(ich - 1) + (size_t)nch * (jch - 1)];
+        if (b > 0) orc_update_hw(ms, hch + nn * 4 * (b - 1), ich, jch, VCH(ki), VCH(kj), C);
+      } else if (ic < 0 && jc == 0) {                           /* Target_h + UPDATE_HX */
+        const int ich = ipbs[io / ibo], jch = ipbs[io % ibo];
+        if (ich < 1 || jch < 1 || ich > nch || jch > nch) { rc = 4; break; }
+        const int b = icc[(ich - 1) + (size_t)nch * (jch - 1)];
+        if (b > 0 && htarg) {
+          const int it = iptar[ich], jt = iptar[jch];
+          const int hi = it > jt ? it : jt, lo = it > jt ? jt : it;
+          htarg[(size_t)(hi - 1) * hi / 2 + lo - 1] += C;
+        }
+        if (iitar > 0 && ich != jch && b > 0) orc_update_hx(ms, hch + nn * 4 * (b - 1), ich, jch, fppqq, C);
+      } else {
+        rc = 6;
+      }
+    }
+  } else if (itype <= 5) {
+    int iii = 0, jjj = 0;
+    const int ipb = stype == 0 ? ip1 : ip2;
+    const char dir = stype == 0 ? 'r' : sym_v;
+    for (int j = 0; j < nc && !rc; ++j) {
+      const int jj = K1[j], ii = K2[j], ich = K3[j], io = K4[j];
+      if (jj != jjj || ii != iii) {
+        if (jj != jjj) {
+          orc_density(ns, ks, dd, PQ(jp1, jj / ibo), PQ(jp2, jj % ibo), sym_d);
+          orc_convol(ns, ks, kb, rkb, xx, dd, icase, sym_i, sym_j);
+          jjj = jj;
+        }
+        orc_bav(ns, ks, xx, PQ(ipb, ii), v, sym_r, dir);
+        iii = ii;
+      }
+      memset(vv, 0, sizeof(double) * (size_t)ms);
+      memcpy(vv + (ip2 == 1 ? 0 : ns), v, sizeof(double) * (size_t)ns);
+      if (io > 0) {                                             /* UPDATE_HW */
+        const int k = kbort(user, io / ibo, io % ibo);
+        if (k == 0) { rc = 2; break; }
+        const int jch = ipbs[io / ibo];
+        if (ich < 1 || jch < 1 || ich > nch || jch > nch) { rc = 4; break; }
+        const int b = icc[(ich - 1) + (size_t)nch * (jch - 1)];
+        if (b > 0) orc_update_hw(ms, hch + nn * 4 * (b - 1), ich, jch, vv, VCH(k), cdata[j]);
+      } else if (io < 0) {                                      /* UPDATE_HV */
+        const int jc = -io;
+        if (ich < 1 || ich > nch || jc > npert) { rc = 3; break; }
+        const int i = icb[(ich - 1) + (size_t)nch * (jc - 1)];
+        if (i > 0) orc_update_hv(ms, hcp + (size_t)ms * (i - 1), vv, cdata[j]);
+      } else {
+        rc = 7;
+      }
+    }
+  } else {                                                      /* S 6..9 */
+    const size_t cols = sym_d == 'x' ? nn : (size_t)ns * (2 * ks - 1);
+    for (int j = 0; j < nc; ++j) {
+      orc_density(ns, ks, dd, PQ(jp1, K2[j]), PQ(jp2, K3[j]), sym_d);
+      for (size_t q = 0; q < cols; ++q) xx[q] = xx[q] + cdata[j] * dd[q];
+      if (j < nc - 1 && K1[j] == K1[j + 1]) continue;
+      orc_convol(ns, ks, kb, rkb, dd, xx, icase, sym_i, sym_j);
+      const int ich = K1[j] / ibo, jch = K1[j] % ibo;
+      if (ich < 1 || jch < 1 || ich > nch || jch > nch) { rc = 4; break; }
+      const int b = icc[(ich - 1) + (size_t)nch * (jch - 1)];
+      if (b > 0) orc_update_hs_mat(ns, ks, hch + nn * 4 * (b - 1), ich, jch, dd, sym_r, ip1, ip2);
+      memset(xx, 0, sizeof(double) * nn);
+    }
+  }
+#undef PQ
+#undef VCH
+  free(xx); free(v); free(vv);
+  return rc;
+}

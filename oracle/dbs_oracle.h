@@ -118,6 +118,13 @@ int  orc_o_data(int ms, int itype, int n, const int *k1, const int *k2, const do
                 const double *fppqq, int iitar, const int *iptar, double *otarg,
                 int nch, int npert, const int *icc, double *hch, const int *icb, double *hcp,
                 const int *ibb, double *hp);
+void orc_bav(int ns, int ks, const double *a, const double *b, double *c, char sym, char dir);
+int  orc_rs_data(int ns, int ks, const double *rkb, int stype, int itype, int atype, int nc,
+                 const double *cdata, const int *K1, const int *K2, const int *K3, const int *K4,
+                 const double *pq, int ibo, const int *ipbs, orc_kbort_fn kbort, void *user,
+                 const double *vch, const double *fppqq, int iitar, const int *iptar, double *htarg,
+                 int nch, int npert, const int *icc, double *hch, const int *icb, double *hcp,
+                 const int *ibb, double *hp);
 
 #ifdef __cplusplus
 }

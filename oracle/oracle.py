@@ -93,6 +93,11 @@ def lib():
         L.orc_o_data.argtypes = [C.c_int, C.c_int, C.c_int, _ip, _ip, _dp, C.c_double, C.c_int, _ip,
                                  KBORT_FN, C.c_void_p, _dp, _dp, C.c_int, _ip, _dp,
                                  C.c_int, C.c_int, _ip, _dp, _ip, _dp, _ip, _dp]
+        L.orc_bav.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, C.c_char, C.c_char]
+        L.orc_rs_data.restype = C.c_int
+        L.orc_rs_data.argtypes = ([C.c_int, C.c_int, _dp] + [C.c_int] * 4 + [_dp] + [_ip] * 4 +
+                                  [_dp, C.c_int, _ip, KBORT_FN, C.c_void_p, _dp, _dp, C.c_int, _ip, _dp,
+                                   C.c_int, C.c_int, _ip, _dp, _ip, _dp, _ip, _dp])
         _lib = L
     return _lib
 
@@ -293,6 +298,30 @@ def o_data(ms, itype, lst, eps_c, ibo, ipbs, kbort, vch, fppqq, iitar, iptar, ot
                             _d(mats["hcp"]), _i(ibb), _d(mats["hp"]))
 
 
+def bav(ns, ks, a, b, sym, dir_):
+    """BAV restated: A b ('r') or A^T b ('l') for A held in storage sym"""
+    c = np.zeros(ns)
+    lib().orc_bav(ns, ks, _d(f64(a)), _d(f64(b)), _d(c), sym.encode(), dir_.encode())
+    return c
+
+
+def rs_data(g, stype, itype, atype, k, lst, pq, ibo, ipbs, kbort, vch, fppqq, iitar, iptar, htarg, mats):
+    """R_data (stype 0) / S_data (stype 1) restated; lst = (c, K1, K2, K3, K4)
+    in IPT order; ipbs/iptar 1-based (element 0 unused); mats as o_data."""
+    c, K1, K2, K3, K4 = lst
+    typ = (5 + atype) if stype else (atype + 1)
+    rkb = g.mrk(typ, k)
+    ci = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+    fi = lambda a: np.asfortranarray(a, dtype=np.int32)
+    cb = KBORT_FN(lambda user, i1, i2: int(kbort(i1, i2)))
+    icc, icb, ibb = fi(mats["icc"]), fi(mats["icb"]), fi(mats["ibb"])
+    return lib().orc_rs_data(g.ns, g.ks, _d(rkb), stype, itype, atype, len(c),
+                             _d(np.ascontiguousarray(c, dtype=np.float64)), _i(ci(K1)), _i(ci(K2)), _i(ci(K3)),
+                             _i(ci(K4)), _d(f64(pq)), ibo, _i(ci(ipbs)), cb, None, _d(f64(vch)), _d(f64(fppqq)),
+                             iitar, _i(ci(iptar)), _d(htarg), icc.shape[0], ibb.shape[0], _i(icc),
+                             _d(mats["hch"]), _i(icb), _d(mats["hcp"]), _i(ibb), _d(mats["hp"]))
+
+
 def idef_rtype(j1, j2, j3, j4, ipbs, ibo):
     ip = np.ascontiguousarray(ipbs, dtype=np.int32)
     out = [np.zeros(1, dtype=np.int32) for _ in range(4)]
@@ -385,6 +414,9 @@ def ref():
         L.ref_quadr.argtypes = [_dp, _dp, C.c_int]
         L.ref_update_hs.argtypes = [_dp, C.c_int, C.c_int, _dp, C.c_char]
         L.ref_overlaps.argtypes = [_dp, _dp, _dp]
+        L.ref_set_orbitals.argtypes = [C.c_int, _dp]
+        L.ref_sk.restype = C.c_double
+        L.ref_sk.argtypes = [C.c_int] * 6
         L.ref_full_mat_sym.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_char]
         L.ref_cdata_alloc.argtypes = [C.c_int] * 6 + [C.c_double, C.c_int]
         L.ref_cdata_add.argtypes = [C.c_double] + [C.c_int] * 6

@@ -224,3 +224,27 @@
       c = sym
       Call Full_mat_sym(ns_, ks_, x, y, c)
       End Subroutine ref_full_mat_sym
+
+      Subroutine ref_set_orbitals(n, pq) bind(C, name='ref_set_orbitals')
+      Use iso_c_binding
+      Use DBS_grid
+      Use df_orbitals, only: p
+      Implicit none
+      Integer(c_int), value :: n
+      Real(c_double), intent(in) :: pq(ns, 2, n)
+      if (allocated(p)) Deallocate(p)
+      Allocate(p(ns, 2, n)); p = pq
+      End Subroutine ref_set_orbitals
+
+      Function ref_sk(itype, i1, j1, i2, j2, k) result(s) bind(C, name='ref_sk')
+      Use iso_c_binding
+      Implicit none
+      Integer(c_int), value :: itype, i1, j1, i2, j2, k
+      Real(c_double) :: s
+      Real(8), external :: sk_ppqq, sk_pqqp
+      if (itype == 5) then
+        s = sk_ppqq(i1, j1, i2, j2, k)
+      else
+        s = sk_pqqp(i1, j1, i2, j2, k)
+      end if
+      End Function ref_sk
