@@ -43,6 +43,12 @@ class Mat3AuxC(C.Structure):
 
 auxp = C.POINTER(Mat3AuxC)
 
+
+class Mat3OrbsC(C.Structure):
+    """struct dbs_mat3_orbs (orbital tables of MAT3's Add_integral)"""
+    _fields_ = [("norb", i32), ("lbs", ip), ("kbs", ip), ("ipbs", ip), ("ibo", i32), ("mk", i32),
+                ("eps_c", f64), ("check_target", i32)]
+
 # name -> argtypes (all return int status unless listed in _RESTYPE)
 PROTOTYPES = {
     "dbs_version": [],
@@ -78,6 +84,7 @@ PROTOTYPES = {
     "dbs_add_integral_ci": [vp, i32, i32, i32, i32, i32, i32, i32, f64, ip, i32, i32],
     "dbs_smu": [i32, i32, i32, i32, i32, i32, dp, dp],
     "dbs_add_integral_mat": [vp, i32, i32, i32, i32, i32, f64, ip, ip, i32, i32, f64],
+    "dbs_add_integral_mat3": [vp, C.POINTER(Mat3OrbsC), i32, i32, i32, i32, i32, i32, f64, i32, i32, i32, i32],
     "dbs_decode_int": [i32, ip, ip, ip, ip, ip, ip, ip],
     "dbs_rk_evaluate_dev": [vp, i32, i32, i32, dp, ip, ip, ip, ip, f64, i32, vp, i32, vp, i32,
                             i32, vp],

@@ -12,7 +12,7 @@ import ctypes as C
 
 import numpy as np
 
-from .capi import GEN_MATRIX_FN, check, lib
+from .capi import GEN_MATRIX_FN, Mat3OrbsC, check, lib
 
 _ip = C.POINTER(C.c_int)
 _dp = C.POINTER(C.c_double)
@@ -61,6 +61,15 @@ class CData:
         k = _i32(kappa)
         check(lib().dbs_add_integral_ci(self.h, kpol, ic, jc, i1, i2, i3, i4, c,
                                         k.ctypes.data_as(_ip), ibi, mbreit))
+
+    def add_integral_mat3(self, icase, kpol, i1, i2, i3, i4, c, ic, jc, io, jo, lbs, kbs, ipbs,
+                          ibo=2 ** 15, mk=20, eps_c=1e-10, check_target=0):
+        """MAT3 Add_integral (icase 0 overlaps, 2 R, 3 S); lbs/kbs/ipbs hold
+        orbital o at element o-1"""
+        a, b, d = _i32(lbs), _i32(kbs), _i32(ipbs)
+        o = Mat3OrbsC(len(d), a.ctypes.data_as(_ip), b.ctypes.data_as(_ip), d.ctypes.data_as(_ip), ibo, mk,
+                      eps_c, check_target)
+        check(lib().dbs_add_integral_mat3(self.h, C.byref(o), icase, kpol, i1, i2, i3, i4, c, ic, jc, io, jo))
 
     def add_integral_mat(self, kpol, i1, i2, i3, i4, c, lorb, ipbs, ibo=2 ** 15, mk=20, eps_c=1e-10):
         lo, ipb = _i32(lorb), _i32(ipbs)

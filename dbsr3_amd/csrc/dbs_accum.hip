@@ -509,15 +509,26 @@ int dbs_add_integral_ci(dbs_cdata* cd, int kpol, int ic, int jc, int i1, int i2,
   return DBS_OK;
 }
 
-// Idef_Rtype for channel-channel terms (structures 6.0-9.0)
-int dbs_add_integral_mat(dbs_cdata* cd, int kpol, int j1, int j2, int j3, int j4, double c,
-                         const int* lorb, const int* ipbs, int ibo, int mk, double eps_c) {
-  DBS_CHECK(cd && lorb && ipbs, DBS_ERR_ARG, "Add_integral: null argument");
-  const int m = (lorb[j1] + lorb[j3] + kpol) % 2 + (lorb[j2] + lorb[j4] + kpol) % 2;
-  if (m != 0 || kpol > mk) return DBS_OK;
-  if (std::fabs(c) < eps_c) return DBS_OK;
+// MAT3 coefficient routing (DBSR_MAT3/idef_type.f90).  ipbs(o) is read as
+// ipb(o): the channel of orbital o (1-based), 0 for a bound orbital.
+}  // extern "C"
+
+namespace {
+
+// Idef_Rtype (idef_type.f90:141-302): canonical orbital order, structure
+// 1..4 (integral x bound/channel factor, channel vectors, direct and
+// exchange channel-channel) and the c_data key.  Returns the itype, 0 when
+// the entry is dropped (|C| < eps_C, or a target structure without
+// check_target), or a negative status after fail().
+template <class IPB>
+int idef_rtype(const IPB& ipb, int ibo, double eps_c, int check_target, int j1, int j2, int j3, int j4, double C,
+               int ic, int jc, int io, int jo, int* key) {
+  if (std::fabs(C) < eps_c) return 0;
+  if (io != 0 && std::abs(io) <= ibo) return -fail(DBS_ERR_ARG, "Idef_Rtype: io<ibo");
+  if (jo != 0 && std::abs(jo) <= ibo) return -fail(DBS_ERR_ARG, "Idef_Rtype: jo<ibo");
+  if (jc > 0 && ic <= 0) return -fail(DBS_ERR_ARG, "jc>0, ic<=0");
   int i1, i2, i3, i4;
-  if (ipbs[j1] + ipbs[j2] == 0 || ipbs[j3] + ipbs[j4] == 0) {
+  if (ipb(j1) + ipb(j2) == 0 || ipb(j3) + ipb(j4) == 0) {
     i1 = j1; i3 = j3;
     if (j1 > j3) { i1 = j3; i3 = j1; }
     i2 = j2; i4 = j4;
@@ -526,26 +537,163 @@ int dbs_add_integral_mat(dbs_cdata* cd, int kpol, int j1, int j2, int j3, int j4
   } else {
     i1 = j1; i2 = j2; i3 = j3; i4 = j4;
   }
-  int k1 = 0, k2 = 0, k3 = 0, itype = 0;
-  if (ipbs[i1] > 0 && ipbs[i3] > 0) {
-    itype = 3;
-    if (ipbs[i1] >= ipbs[i3]) { k1 = ipbs[i1] * ibo + ipbs[i3]; k2 = i2; k3 = i4; if (k2 > k3) { k2 = i4; k3 = i2; } }
-    else { k1 = ipbs[i3] * ibo + ipbs[i1]; k2 = i4; k3 = i2; if (k2 > k3) { k2 = i2; k3 = i4; } }
-  } else if (ipbs[i2] > 0 && ipbs[i4] > 0) {
-    itype = 3;
-    if (ipbs[i2] >= ipbs[i4]) { k1 = ipbs[i2] * ibo + ipbs[i4]; k2 = i1; k3 = i3; if (k2 > k3) { k2 = i3; k3 = i1; } }
-    else { k1 = ipbs[i4] * ibo + ipbs[i2]; k2 = i3; k3 = i1; if (k2 > k3) { k2 = i1; k3 = i3; } }
-  } else if (ipbs[i1] > 0 && ipbs[i4] > 0) {
-    itype = 4;
-    if (ipbs[i1] >= ipbs[i4]) { k1 = ipbs[i1] * ibo + ipbs[i4]; k2 = i3; k3 = i2; }
-    else { k1 = ipbs[i4] * ibo + ipbs[i1]; k2 = i2; k3 = i3; }
-  } else if (ipbs[i2] > 0 && ipbs[i3] > 0) {
-    itype = 4;
-    if (ipbs[i3] >= ipbs[i2]) { k1 = ipbs[i3] * ibo + ipbs[i2]; k2 = i1; k3 = i4; }
-    else { k1 = ipbs[i2] * ibo + ipbs[i3]; k2 = i4; k3 = i1; }
+  int k1 = 0, k2 = 0, k3 = 0, k4 = 0, itype = 0;
+  auto chan_vec = [&](int last) {  // Rk with one channel orbital: vectors (2.x)
+    if (ipb(i1) > 0) { k1 = i2 * ibo + i4; k2 = i3; k3 = ipb(i1); k4 = last; itype = 2; }
+    else if (ipb(i2) > 0) { k1 = i1 * ibo + i3; k2 = i4; k3 = ipb(i2); k4 = last; itype = 2; }
+    else if (ipb(i3) > 0) { k1 = i2 * ibo + i4; k2 = i1; k3 = ipb(i3); k4 = last; itype = 2; }
+    else if (ipb(i4) > 0) { k1 = i1 * ibo + i3; k2 = i2; k3 = ipb(i4); k4 = last; itype = 2; }
+  };
+  if (jc > 0) {
+    k1 = i1 * ibo + i3; k2 = i2 * ibo + i4; k3 = -ic; k4 = -jc; itype = 1;
+  } else if (ic > 0) {
+    if (io == 0) chan_vec(-ic);
+    else if (io > 0) { k1 = i1 * ibo + i3; k2 = i2 * ibo + i4; k3 = io; k4 = -ic; itype = 1; }
+  } else if (io == 0) {
+    if (ipb(i1) > 0 && ipb(i3) > 0) {
+      itype = 3;
+      if (ipb(i1) >= ipb(i3)) { k1 = ipb(i1) * ibo + ipb(i3); k2 = i2; k3 = i4; if (k2 > k3) { k2 = i4; k3 = i2; } }
+      else { k1 = ipb(i3) * ibo + ipb(i1); k2 = i4; k3 = i2; if (k2 > k3) { k2 = i2; k3 = i4; } }
+    } else if (ipb(i2) > 0 && ipb(i4) > 0) {
+      itype = 3;
+      if (ipb(i2) >= ipb(i4)) { k1 = ipb(i2) * ibo + ipb(i4); k2 = i1; k3 = i3; if (k2 > k3) { k2 = i3; k3 = i1; } }
+      else { k1 = ipb(i4) * ibo + ipb(i2); k2 = i3; k3 = i1; if (k2 > k3) { k2 = i1; k3 = i3; } }
+    } else if (ipb(i1) > 0 && ipb(i4) > 0) {
+      itype = 4;
+      if (ipb(i1) >= ipb(i4)) { k1 = ipb(i1) * ibo + ipb(i4); k2 = i3; k3 = i2; }
+      else { k1 = ipb(i4) * ibo + ipb(i1); k2 = i2; k3 = i3; }
+    } else if (ipb(i2) > 0 && ipb(i3) > 0) {
+      itype = 4;
+      if (ipb(i3) >= ipb(i2)) { k1 = ipb(i3) * ibo + ipb(i2); k2 = i1; k3 = i4; }
+      else { k1 = ipb(i2) * ibo + ipb(i3); k2 = i4; k3 = i1; }
+    }
+  } else if (io > 0 && jo == 0) {
+    chan_vec(io);
+    if (itype == 0) {
+      k1 = i1 * ibo + i3; k2 = i2 * ibo + i4; k3 = io; k4 = 0; itype = 1;
+      if (check_target == 0) return 0;
+    }
+  } else if (io > 0 && jo > 0) {
+    k1 = i1 * ibo + i3; k2 = i2 * ibo + i4; k3 = io; k4 = jo; itype = 1;
   }
-  DBS_CHECK(itype != 0, DBS_ERR_ARG, "Idef_Rtype: itype=0");
-  return dbs_cdata_add_coef(cd, c, kpol, k1, k2, k3, 0, itype);
+  if (itype == 0) return -fail(DBS_ERR_ARG, "Idef_Rtype: itype=0");
+  key[0] = k1; key[1] = k2; key[2] = k3; key[3] = k4;
+  return itype;
+}
+
+// Idef_itype (idef_type.f90:306-437): the S^k structures 1..9, no orbital
+// canonicalisation; itype + atype*ibtype
+template <class IPB>
+int idef_itype(const IPB& ipb, int ibo, int atype, int i1, int i2, int i3, int i4, double C, int ic, int jc,
+               int io, int jo, int* key) {
+  if (C == 0.0) return 0;
+  if (io != 0 && std::abs(io) <= ibo) return -fail(DBS_ERR_ARG, "Idef_type: io,ibo = " + std::to_string(io) + " " + std::to_string(ibo));
+  if (jo != 0 && std::abs(jo) <= ibo) return -fail(DBS_ERR_ARG, "Idef_type: jo,ibo = " + std::to_string(jo) + " " + std::to_string(ibo));
+  if (jc > 0 && ic <= 0) return -fail(DBS_ERR_ARG, "jc>0,ic<=0");
+  int k1 = 0, k2 = 0, k3 = 0, k4 = 0, itype = 0;
+  auto chan_vec = [&](int last) {
+    if (ipb(i1) > 0) { k1 = i2 * ibo + i4; k2 = i3; k3 = ipb(i1); k4 = last; itype = 2; }
+    else if (ipb(i2) > 0) { k1 = i1 * ibo + i3; k2 = i4; k3 = ipb(i2); k4 = last; itype = 3; }
+    else if (ipb(i3) > 0) { k1 = i2 * ibo + i4; k2 = i1; k3 = ipb(i3); k4 = last; itype = 4; }
+    else if (ipb(i4) > 0) { k1 = i1 * ibo + i3; k2 = i2; k3 = ipb(i4); k4 = last; itype = 5; }
+  };
+  if (jc > 0) {
+    k1 = i1 * ibo + i3; k2 = i2 * ibo + i4; k3 = -ic; k4 = -jc; itype = 1;
+  } else if (ic > 0) {
+    if (io == 0) chan_vec(-ic);
+    else if (io > 0) { k1 = i1 * ibo + i3; k2 = i2 * ibo + i4; k3 = io; k4 = -ic; itype = 1; }
+  } else if (io == 0) {
+    if (ipb(i1) > 0 && ipb(i3) > 0) { k1 = ipb(i1) * ibo + ipb(i3); k2 = i2; k3 = i4; itype = 6; }
+    else if (ipb(i2) > 0 && ipb(i4) > 0) { k1 = ipb(i2) * ibo + ipb(i4); k2 = i1; k3 = i3; itype = 7; }
+    else if (ipb(i1) > 0 && ipb(i4) > 0) { k1 = ipb(i1) * ibo + ipb(i4); k2 = i3; k3 = i2; itype = 8; }
+    else if (ipb(i2) > 0 && ipb(i3) > 0) { k1 = ipb(i3) * ibo + ipb(i2); k2 = i1; k3 = i4; itype = 9; }
+  } else if (io > 0 && jo == 0) {
+    chan_vec(io);
+    if (itype == 0) { k1 = i1 * ibo + i3; k2 = i2 * ibo + i4; k3 = io; k4 = 0; itype = 1; }
+  } else if (io > 0 && jo > 0) {
+    k1 = i1 * ibo + i3; k2 = i2 * ibo + i4; k3 = io; k4 = jo; itype = 1;
+  }
+  if (itype == 0) return -fail(DBS_ERR_ARG, "IDEF_TYPE: itype=0");
+  key[0] = k1; key[1] = k2; key[2] = k3; key[3] = k4;
+  return atype * 10 + itype;  // ibtype = 10 (mod_dbsr_mat.f90:103)
+}
+
+// Idef_Otype (idef_type.f90:2-50): overlap structures 1..4
+int idef_otype(double eps_c, int check_target, double C, int ic, int jc, int io, int jo, int* key) {
+  if (std::fabs(C) < eps_c) return 0;
+  if (jc > 0 && ic <= 0) return -fail(DBS_ERR_ARG, "jc > 0, ic <= 0");
+  int itype = 0;
+  key[2] = key[3] = 0;
+  if (jc > 0) { key[0] = ic; key[1] = jc; itype = 1; }
+  else if (ic > 0 && io > 0) { key[0] = io; key[1] = ic; itype = 2; }
+  else if (io > 0 && jo > 0) { key[0] = io; key[1] = jo; itype = 3; }
+  else if (io > 0 && jo == 0) { key[0] = io; key[1] = 0; itype = 4; }
+  if (itype == 0) return -fail(DBS_ERR_ARG, "Idef_Otype: itype=0");
+  if (check_target == 0 && itype == 4) return 0;
+  return itype;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dbs_add_integral_mat(dbs_cdata* cd, int kpol, int j1, int j2, int j3, int j4, double c,
+                         const int* lorb, const int* ipbs, int ibo, int mk, double eps_c) {
+  DBS_CHECK(cd && lorb && ipbs, DBS_ERR_ARG, "Add_integral: null argument");
+  const int m = (lorb[j1] + lorb[j3] + kpol) % 2 + (lorb[j2] + lorb[j4] + kpol) % 2;
+  if (m != 0 || kpol > mk) return DBS_OK;
+  auto ipb = [&](int o) { return ipbs[o]; };
+  int key[4];
+  const int t = idef_rtype(ipb, ibo, eps_c, 1, j1, j2, j3, j4, c, 0, 0, 0, 0, key);
+  if (t < 0) return -t;
+  if (t == 0) return DBS_OK;
+  return dbs_cdata_add_coef(cd, c, kpol, key[0], key[1], key[2], key[3], t);
+}
+
+int dbs_add_integral_mat3(dbs_cdata* cd, const dbs_mat3_orbs* o, int icase, int kpol, int i1, int i2, int i3,
+                          int i4, double C, int ic, int jc, int io, int jo) {
+  DBS_CHECK(cd && o && o->ipbs, DBS_ERR_ARG, "Add_integral: null argument");
+  auto ipb = [&](int orb) { return (orb >= 1 && orb <= o->norb) ? o->ipbs[orb - 1] : 0; };
+  int key[4], t, rc;
+  switch (icase) {
+    case 0:  // overlaps
+      t = idef_otype(o->eps_c, o->check_target, C, ic, jc, io, jo, key);
+      if (t <= 0) return -t;
+      return dbs_cdata_add_coef(cd, C, 0, key[0], key[1], key[2], key[3], t);
+    case 1:
+      return fail(DBS_ERR_ARG, "Add_integral: L-integrals (one-electron) are not part of this library");
+    case 2: {  // R-integrals: the parity rule, then Idef_Rtype
+      DBS_CHECK(o->lbs, DBS_ERR_ARG, "Add_integral: lbs missing");
+      for (int q : {i1, i2, i3, i4}) DBS_CHECK(q >= 1 && q <= o->norb, DBS_ERR_ARG, "Add_integral: orbital index out of range");
+      const int m = (o->lbs[i1 - 1] + o->lbs[i3 - 1] + kpol) % 2 + (o->lbs[i2 - 1] + o->lbs[i4 - 1] + kpol) % 2;
+      if (m != 0 || kpol > o->mk) return DBS_OK;
+      t = idef_rtype(ipb, o->ibo, o->eps_c, o->check_target, i1, i2, i3, i4, C, ic, jc, io, jo, key);
+      if (t <= 0) return -t;
+      return dbs_cdata_add_coef(cd, C, kpol, key[0], key[1], key[2], key[3], t);
+    }
+    case 3: {  // S-integrals: SMU factors, eight permuted terms per multipole
+      DBS_CHECK(o->kbs, DBS_ERR_ARG, "Add_integral: kbs missing");
+      for (int q : {i1, i2, i3, i4}) DBS_CHECK(q >= 1 && q <= o->norb, DBS_ERR_ARG, "Add_integral: orbital index out of range");
+      double S[8];
+      for (int v = kpol - 1; v <= kpol + 1; ++v) {
+        if (v > o->mk) continue;
+        if (smu_factors(o->kbs[i1 - 1], o->kbs[i2 - 1], o->kbs[i3 - 1], o->kbs[i4 - 1], kpol, v, S) == 0.0) continue;
+        const int perm[8][4] = {{i1, i2, i3, i4}, {i2, i1, i4, i3}, {i3, i4, i1, i2}, {i4, i3, i2, i1},
+                                {i1, i2, i3, i4}, {i4, i3, i2, i1}, {i3, i4, i1, i2}, {i2, i1, i4, i3}};
+        for (int q = 0; q < 8; ++q) {
+          const double Cq = C * S[q];
+          t = idef_itype(ipb, o->ibo, q < 4 ? 0 : 1, perm[q][0], perm[q][1], perm[q][2], perm[q][3], Cq, ic, jc, io,
+                         jo, key);
+          if (t < 0) return -t;
+          if (t == 0) continue;
+          if ((rc = dbs_cdata_add_coef(cd, Cq, v, key[0], key[1], key[2], key[3], t))) return rc;
+        }
+      }
+      return DBS_OK;
+    }
+    default:
+      return fail(DBS_ERR_ARG, "Add_integral: unknown icase");
+  }
 }
 
 int dbs_decode_int(int n, const int* code, int* met, int* k, int* i1, int* i2, int* i3, int* i4) {

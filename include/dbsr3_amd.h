@@ -145,6 +145,24 @@ int dbs_smu(int ka, int kb, int kc, int kd, int L, int v, double *S, double *smu
  * channel ipbs[i].  Terms with |c| < eps_c or k > mk are dropped as there. */
 int dbs_add_integral_mat(dbs_cdata *cd, int kpol, int i1, int i2, int i3, int i4, double c,
                          const int *lorb, const int *ipbs, int ibo, int mk, double eps_c);
+/* MAT3 orbital tables for Add_integral (module dbsr_mat / orb_jj): for
+ * orbital o (1-based) element o-1 of lbs (l), kbs (kappa), ipbs (channel,
+ * 0 = bound); ibo = 2**15 (mod_dbsr_mat.f90:61), mk, eps_C, check_target. */
+typedef struct dbs_mat3_orbs {
+  int norb;
+  const int *lbs, *kbs, *ipbs;
+  int ibo, mk;
+  double eps_c;
+  int check_target;
+} dbs_mat3_orbs;
+/* Add_integral, MAT3 form (DBSR_MAT3/add_integral.f90:2-57): icase 0
+ * overlaps -> Idef_Otype; 2 R-integrals -> parity rule, Idef_Rtype (all 16
+ * structures with ic, jc, io, jo); 3 S-integrals -> SMU and Idef_itype for
+ * the eight permuted terms of multipoles kpol-1..kpol+1 (atype 0 / 1,
+ * itype + 10*atype) (DBSR_MAT3/idef_type.f90:2-437).  icase 1 (one-electron
+ * L-integrals) returns an error. */
+int dbs_add_integral_mat3(dbs_cdata *cd, const dbs_mat3_orbs *o, int icase, int kpol, int i1, int i2,
+                          int i3, int i4, double C, int ic, int jc, int io, int jo);
 /* Decode_int (DBSR_CI3/setup_matrix.f90:206-223) for n packed int_bnk codes */
 int dbs_decode_int(int n, const int *code, int *met, int *k, int *i1, int *i2, int *i3, int *i4);
 

@@ -1652,3 +1652,150 @@ int orc_rs_data(int ns, int ks, const double *rkb, int stype, int itype, int aty
   free(xx); free(v); free(vv);
   return rc;
 }
+
+/* ===================================================================== */
+/* MAT3 coefficient routing: DBSR_MAT3/idef_type.f90 and add_integral.f90 */
+/* ipbs[o] for orbital o (1-based, element 0 unused).  Each function      */
+/* returns the c_data itype (> 0), 0 for a dropped entry, or -1 where the  */
+/* reference stops.                                                        */
+/* ===================================================================== */
+
+/* Idef_Rtype: idef_type.f90:141-302 */
+int orc_idef_rtype_full(int j1, int j2, int j3, int j4, double C, int ic, int jc, int io, int jo,
+                        const int *ipbs, int ibo, double eps_c, int check_target, int *key) {
+  if (fabs(C) < eps_c) return 0;
+  if (io != 0 && abs(io) <= ibo) return -1;
+  if (jo != 0 && abs(jo) <= ibo) return -1;
+  if (jc > 0 && ic <= 0) return -1;
+  int i1, i2, i3, i4, t;
+  if (ipbs[j1] + ipbs[j2] == 0 || ipbs[j3] + ipbs[j4] == 0) {
+    i1 = j1; i3 = j3; if (j1 > j3) { i1 = j3; i3 = j1; }
+    i2 = j2; i4 = j4; if (j2 > j4) { i2 = j4; i4 = j2; }
+    if (i1 > i2) { t = i1; i1 = i2; i2 = t; t = i3; i3 = i4; i4 = t; }
+  } else {
+    i1 = j1; i2 = j2; i3 = j3; i4 = j4;
+  }
+  int k1 = 0, k2 = 0, k3 = 0, k4 = 0, itype = 0;
+  if (jc > 0) {
+    k1 = i1 * ibo + i3; k2 = i2 * ibo + i4; k3 = -ic; k4 = -jc; itype = 1;
+  } else if (ic > 0) {
+    if (io == 0) {
+      if (ipbs[i1] > 0) { k1 = i2 * ibo + i4; k2 = i3; k3 = ipbs[i1]; k4 = -ic; itype = 2; }
+      else if (ipbs[i2] > 0) { k1 = i1 * ibo + i3; k2 = i4; k3 = ipbs[i2]; k4 = -ic; itype = 2; }
+      else if (ipbs[i3] > 0) { k1 = i2 * ibo + i4; k2 = i1; k3 = ipbs[i3]; k4 = -ic; itype = 2; }
+      else if (ipbs[i4] > 0) { k1 = i1 * ibo + i3; k2 = i2; k3 = ipbs[i4]; k4 = -ic; itype = 2; }
+    } else if (io > 0) {
+      k1 = i1 * ibo + i3; k2 = i2 * ibo + i4; k3 = io; k4 = -ic; itype = 1;
+    }
+  } else if (io == 0) {
+    int kk[4];
+    itype = orc_idef_rtype(j1, j2, j3, j4, ipbs, ibo, &kk[0], &kk[1], &kk[2], &kk[3]);
+    k1 = kk[0]; k2 = kk[1]; k3 = kk[2]; k4 = kk[3];
+  } else if (io > 0 && jo == 0) {
+    if (ipbs[i1] > 0) { k1 = i2 * ibo + i4; k2 = i3; k3 = ipbs[i1]; k4 = io; itype = 2; }
+    else if (ipbs[i2] > 0) { k1 = i1 * ibo + i3; k2 = i4; k3 = ipbs[i2]; k4 = io; itype = 2; }
+    else if (ipbs[i3] > 0) { k1 = i2 * ibo + i4; k2 = i1; k3 = ipbs[i3]; k4 = io; itype = 2; }
+    else if (ipbs[i4] > 0) { k1 = i1 * ibo + i3; k2 = i2; k3 = ipbs[i4]; k4 = io; itype = 2; }
+    else {
+      k1 = i1 * ibo + i3; k2 = i2 * ibo + i4; k3 = io; k4 = 0; itype = 1;
+      if (check_target == 0) return 0;
+    }
+  } else if (io > 0 && jo > 0) {
+    k1 = i1 * ibo + i3; k2 = i2 * ibo + i4; k3 = io; k4 = jo; itype = 1;
+  }
+  if (itype == 0) return -1;
+  key[0] = k1; key[1] = k2; key[2] = k3; key[3] = k4;
+  return itype;
+}
+
+/* Idef_itype: idef_type.f90:306-437 (S^k structures; itype + 10*atype) */
+int orc_idef_itype(int atype, int i1, int i2, int i3, int i4, double C, int ic, int jc, int io, int jo,
+                   const int *ipbs, int ibo, int *key) {
+  if (C == 0.0) return 0;
+  if (io != 0 && abs(io) <= ibo) return -1;
+  if (jo != 0 && abs(jo) <= ibo) return -1;
+  if (jc > 0 && ic <= 0) return -1;
+  int k1 = 0, k2 = 0, k3 = 0, k4 = 0, itype = 0;
+  if (jc > 0) {
+    k1 = i1 * ibo + i3; k2 = i2 * ibo + i4; k3 = -ic; k4 = -jc; itype = 1;
+  } else if (ic > 0) {
+    if (io == 0) {
+      if (ipbs[i1] > 0) { k1 = i2 * ibo + i4; k2 = i3; k3 = ipbs[i1]; k4 = -ic; itype = 2; }
+      else if (ipbs[i2] > 0) { k1 = i1 * ibo + i3; k2 = i4; k3 = ipbs[i2]; k4 = -ic; itype = 3; }
+      else if (ipbs[i3] > 0) { k1 = i2 * ibo + i4; k2 = i1; k3 = ipbs[i3]; k4 = -ic; itype = 4; }
+      else if (ipbs[i4] > 0) { k1 = i1 * ibo + i3; k2 = i2; k3 = ipbs[i4]; k4 = -ic; itype = 5; }
+    } else if (io > 0) {
+      k1 = i1 * ibo + i3; k2 = i2 * ibo + i4; k3 = io; k4 = -ic; itype = 1;
+    }
+  } else if (io == 0) {
+    if (ipbs[i1] > 0 && ipbs[i3] > 0) { k1 = ipbs[i1] * ibo + ipbs[i3]; k2 = i2; k3 = i4; itype = 6; }
+    else if (ipbs[i2] > 0 && ipbs[i4] > 0) { k1 = ipbs[i2] * ibo + ipbs[i4]; k2 = i1; k3 = i3; itype = 7; }
+    else if (ipbs[i1] > 0 && ipbs[i4] > 0) { k1 = ipbs[i1] * ibo + ipbs[i4]; k2 = i3; k3 = i2; itype = 8; }
+    else if (ipbs[i2] > 0 && ipbs[i3] > 0) { k1 = ipbs[i3] * ibo + ipbs[i2]; k2 = i1; k3 = i4; itype = 9; }
+  } else if (io > 0 && jo == 0) {
+    if (ipbs[i1] > 0) { k1 = i2 * ibo + i4; k2 = i3; k3 = ipbs[i1]; k4 = io; itype = 2; }
+    else if (ipbs[i2] > 0) { k1 = i1 * ibo + i3; k2 = i4; k3 = ipbs[i2]; k4 = io; itype = 3; }
+    else if (ipbs[i3] > 0) { k1 = i2 * ibo + i4; k2 = i1; k3 = ipbs[i3]; k4 = io; itype = 4; }
+    else if (ipbs[i4] > 0) { k1 = i1 * ibo + i3; k2 = i2; k3 = ipbs[i4]; k4 = io; itype = 5; }
+    else { k1 = i1 * ibo + i3; k2 = i2 * ibo + i4; k3 = io; k4 = 0; itype = 1; }
+  } else if (io > 0 && jo > 0) {
+    k1 = i1 * ibo + i3; k2 = i2 * ibo + i4; k3 = io; k4 = jo; itype = 1;
+  }
+  if (itype == 0) return -1;
+  key[0] = k1; key[1] = k2; key[2] = k3; key[3] = k4;
+  return atype * 10 + itype;
+}
+
+/* Idef_Otype: idef_type.f90:2-50 */
+int orc_idef_otype(double C, int ic, int jc, int io, int jo, double eps_c, int check_target, int *key) {
+  if (fabs(C) < eps_c) return 0;
+  if (jc > 0 && ic <= 0) return -1;
+  int itype = 0;
+  key[2] = 0; key[3] = 0;
+  if (jc > 0) { key[0] = ic; key[1] = jc; itype = 1; }
+  else if (ic > 0 && io > 0) { key[0] = io; key[1] = ic; itype = 2; }
+  else if (io > 0 && jo > 0) { key[0] = io; key[1] = jo; itype = 3; }
+  else if (io > 0 && jo == 0) { key[0] = io; key[1] = 0; itype = 4; }
+  if (itype == 0) return -1;
+  if (check_target == 0 && itype == 4) return 0;
+  return itype;
+}
+
+/* Add_integral, MAT3 form: DBSR_MAT3/add_integral.f90:2-57.  lbs, kbs,
+ * ipbs indexed by orbital (element 0 unused).  Returns 0 or -1. */
+int orc_add_integral_mat3(orc_cdata *cd, int icase, int kpol, int i1, int i2, int i3, int i4, double C,
+                          int ic, int jc, int io, int jo, const int *lbs, const int *kbs, const int *ipbs,
+                          int ibo, int mk, double eps_c, int check_target) {
+  int key[4], t;
+  if (icase == 0) {
+    t = orc_idef_otype(C, ic, jc, io, jo, eps_c, check_target, key);
+    if (t <= 0) return t;
+    return orc_cdata_add_coef(cd, C, 0, key[0], key[1], key[2], key[3], t) < 0 ? -1 : 0;
+  }
+  if (icase == 2) {
+    int m = (lbs[i1] + lbs[i3] + kpol) % 2 + (lbs[i2] + lbs[i4] + kpol) % 2;
+    if (m != 0 || kpol > mk) return 0;
+    t = orc_idef_rtype_full(i1, i2, i3, i4, C, ic, jc, io, jo, ipbs, ibo, eps_c, check_target, key);
+    if (t <= 0) return t;
+    return orc_cdata_add_coef(cd, C, kpol, key[0], key[1], key[2], key[3], t) < 0 ? -1 : 0;
+  }
+  if (icase == 3) {
+    double S[8];
+    for (int v = kpol - 1; v <= kpol + 1; ++v) {
+      if (v > mk) continue;
+      if (orc_smu(kbs[i1], kbs[i2], kbs[i3], kbs[i4], kpol, v, S) == 0.0) continue;
+      const int perm[8][4] = {{i1, i2, i3, i4}, {i2, i1, i4, i3}, {i3, i4, i1, i2}, {i4, i3, i2, i1},
+                              {i1, i2, i3, i4}, {i4, i3, i2, i1}, {i3, i4, i1, i2}, {i2, i1, i4, i3}};
+      for (int q = 0; q < 8; ++q) {
+        const double Cq = C * S[q];
+        t = orc_idef_itype(q < 4 ? 0 : 1, perm[q][0], perm[q][1], perm[q][2], perm[q][3], Cq, ic, jc, io, jo,
+                           ipbs, ibo, key);
+        if (t < 0) return -1;
+        if (t == 0) continue;
+        if (orc_cdata_add_coef(cd, Cq, v, key[0], key[1], key[2], key[3], t) < 0) return -1;
+      }
+    }
+    return 0;
+  }
+  return -1;
+}

@@ -118,6 +118,14 @@ int  orc_o_data(int ms, int itype, int n, const int *k1, const int *k2, const do
                 const double *fppqq, int iitar, const int *iptar, double *otarg,
                 int nch, int npert, const int *icc, double *hch, const int *icb, double *hcp,
                 const int *ibb, double *hp);
+int  orc_idef_rtype_full(int j1, int j2, int j3, int j4, double C, int ic, int jc, int io, int jo,
+                         const int *ipbs, int ibo, double eps_c, int check_target, int *key);
+int  orc_idef_itype(int atype, int i1, int i2, int i3, int i4, double C, int ic, int jc, int io, int jo,
+                    const int *ipbs, int ibo, int *key);
+int  orc_idef_otype(double C, int ic, int jc, int io, int jo, double eps_c, int check_target, int *key);
+int  orc_add_integral_mat3(orc_cdata *cd, int icase, int kpol, int i1, int i2, int i3, int i4, double C,
+                           int ic, int jc, int io, int jo, const int *lbs, const int *kbs, const int *ipbs,
+                           int ibo, int mk, double eps_c, int check_target);
 void orc_bav(int ns, int ks, const double *a, const double *b, double *c, char sym, char dir);
 int  orc_rs_data(int ns, int ks, const double *rkb, int stype, int itype, int atype, int nc,
                  const double *cdata, const int *K1, const int *K2, const int *K3, const int *K4,

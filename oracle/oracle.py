@@ -329,6 +329,18 @@ def idef_rtype(j1, j2, j3, j4, ipbs, ibo):
     return it, tuple(int(o[0]) for o in out)
 
 
+def idef_rtype_full(j1, j2, j3, j4, c, ic, jc, io, jo, ipbs, ibo, eps_c=1e-10, check_target=1):
+    """Idef_Rtype, all structures; returns (itype, key) (itype <= 0: dropped / stop)"""
+    L = lib()
+    L.orc_idef_rtype_full.restype = C.c_int
+    L.orc_idef_rtype_full.argtypes = [C.c_int] * 4 + [C.c_double] + [C.c_int] * 4 + [_ip, C.c_int, C.c_double,
+                                                                                      C.c_int, _ip]
+    key = np.zeros(4, dtype=np.int32)
+    it = L.orc_idef_rtype_full(j1, j2, j3, j4, c, ic, jc, io, jo, _i(np.ascontiguousarray(ipbs, dtype=np.int32)),
+                               ibo, eps_c, check_target, _i(key))
+    return it, tuple(int(x) for x in key)
+
+
 def decode_int(code):
     out = [np.zeros(1, dtype=np.int32) for _ in range(6)]
     lib().orc_decode_int(int(code), *[_i(o) for o in out])
@@ -374,6 +386,18 @@ class CData:
 
     def add_matrix(self, itype, kpol):
         lib().orc_cdata_add_matrix(self.h, itype, kpol)
+
+    def add_integral_mat3(self, icase, kpol, i1, i2, i3, i4, c, ic, jc, io, jo, lbs, kbs, ipbs,
+                          ibo=2 ** 15, mk=20, eps_c=1e-10, check_target=0):
+        """MAT3 Add_integral; lbs/kbs/ipbs indexed by orbital (element 0 unused)"""
+        L = lib()
+        L.orc_add_integral_mat3.restype = C.c_int
+        L.orc_add_integral_mat3.argtypes = ([C.c_void_p] + [C.c_int] * 6 + [C.c_double] + [C.c_int] * 4 +
+                                            [_ip] * 3 + [C.c_int, C.c_int, C.c_double, C.c_int])
+        ci = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+        a, b, d = ci(lbs), ci(kbs), ci(ipbs)
+        return L.orc_add_integral_mat3(self.h, icase, kpol, i1, i2, i3, i4, c, ic, jc, io, jo, _i(a), _i(b), _i(d),
+                                       ibo, mk, eps_c, check_target)
 
     def __del__(self):
         try:
