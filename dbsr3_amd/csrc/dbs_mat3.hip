@@ -137,6 +137,11 @@ __global__ void k_update_hs_one(int ns, int ks, int sym, int ich, int jch, int i
   }
 }
 
+__global__ void k_div(long long n, double* __restrict__ h, double q) {
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x)
+    h[e] = h[e] / q;
+}
+
 namespace {
 
 unsigned grid_for(long long n, int block) {
@@ -450,6 +455,123 @@ int structure6(dbs_ctx* ctx, const dbs_mat_blocks* m, const Aux& X, const Plan& 
   return DBS_OK;
 }
 
+// hf_matrix, two-electron part (DBSR_HF/hf_matrix.f90:2-78, Breit block
+// :97-163): per (ii,jj) the direct and exchange coefficient densities of
+// all k in one launch each, then per k one convolution and one UPDATE_HS,
+// in the reference's order
+int hf_matrix(dbs_ctx* ctx, int i, int norb, const double* pq_dev, const double* A, const double* B, int kmax,
+              double qsum, int mbreit, int nrk, const int* kr1, const int* kr2, const int* kr3, const int* kr4,
+              const double* crk, double* hfm) {
+  const int ns = ctx->ns, ks = ctx->ks, ms = ctx->ms, ng = kmax + 1;
+  hipStream_t s = ctx->stream;
+  DevMem mem(s);
+  double *DD, *XX, *D, *X;
+  int rc;
+  if ((rc = mem.alloc(&DD, (size_t)ns * ks * ng)) || (rc = mem.alloc(&XX, (size_t)ns * ns * ng)) ||
+      (rc = mem.alloc(&D, (size_t)ns * ns)) || (rc = mem.alloc(&X, (size_t)ns * ns)))
+    return rc;
+  for (int ii = 1; ii <= 2; ++ii)
+    for (int jj = 1; jj <= 2; ++jj) {
+      const int type = ii == 1 ? (jj == 1 ? DBS_PPPP : DBS_PQPQ) : (jj == 1 ? DBS_QPQP : DBS_QQQQ);
+      // coefficient lists per k: direct (a, doubled for j == i) and exchange (b)
+      std::vector<int> goff_d(1, 0), goff_x(1, 0), jd, jx;
+      std::vector<double> cd, cx;
+      for (int k = 0; k <= kmax; ++k) {
+        for (int j = 1; j <= norb; ++j) {
+          double c = A[(j - 1) + (size_t)norb * k];
+          if (c != 0.0) {
+            if (i == j) c = c + c;
+            jd.push_back(j);
+            cd.push_back(c);
+          }
+          const double b = B[(j - 1) + (size_t)norb * k];
+          if (b != 0.0) {
+            jx.push_back(j);
+            cx.push_back(b);
+          }
+        }
+        goff_d.push_back((int)cd.size());
+        goff_x.push_back((int)cx.size());
+      }
+      DevMem lists(s);
+      int *d_gd, *d_gx, *d_jd, *d_jx;
+      double *d_cd, *d_cx;
+      if ((rc = lists.put(goff_d, &d_gd)) || (rc = lists.put(goff_x, &d_gx)) || (rc = lists.put(jd, &d_jd)) ||
+          (rc = lists.put(jx, &d_jx)) || (rc = lists.put(cd, &d_cd)) || (rc = lists.put(cx, &d_cx)))
+        return rc;
+      k_run_density<<<grid_for((long long)ns * ks * ng, 256), 256, 0, s>>>(ng, d_gd, d_jd, d_jd, d_cd, pq_dev, ns,
+                                                                            ks, jj, jj, 0, DD);
+      k_run_density<<<grid_for((long long)ns * ns * ng, 256), 256, 0, s>>>(ng, d_gx, d_jx, d_jx, d_cx, pq_dev, ns,
+                                                                            ks, ii, jj, 3, XX);
+      DBS_HIP(hipGetLastError());
+      for (int k = 0; k <= kmax; ++k) {
+        const bool dir = goff_d[k + 1] > goff_d[k], exch = goff_x[k + 1] > goff_x[k];
+        if (!dir && !exch) continue;
+        double* rkb = nullptr;
+        if ((rc = build_rkb(ctx, type, k, &rkb))) return rc;
+        if (dir) {
+          if ((rc = convol_launch(ctx, rkb, type, 1, 's', 's', DD + (size_t)ns * ks * k, D))) return rc;
+          if ((rc = dbs_update_hs_dev(ctx, hfm, ii, ii, D, 's'))) return rc;
+        }
+        if (exch) {
+          if ((rc = convol_launch(ctx, rkb, type, 4, 's', 's', XX + (size_t)ns * ns * k, X))) return rc;
+          if ((rc = dbs_update_hs_dev(ctx, hfm, ii, jj, X, 'x'))) return rc;
+        }
+      }
+      DBS_HIP(hipStreamSynchronize(s));  // the lists are freed next
+    }
+  if (mbreit == 2 && nrk > 0) {
+    static const int tab[2][4][4] = {{{1, 2, 1, 2}, {1, 2, 1, 2}, {1, 2, 2, 1}, {2, 1, 1, 2}},
+                                     {{1, 2, 2, 1}, {2, 1, 1, 2}, {1, 1, 2, 2}, {2, 2, 1, 1}}};
+    static const int kint[4] = {2, 1, 4, 3};
+    for (int itype = 0; itype <= 1; ++itype)
+      for (int k = 0; k <= kmax; ++k) {
+        const int kk = k + 1000 * itype;
+        // int 3/4 call convol_sk with sym_i = sym_j = 'x', for which it has no
+        // branch (dbsr_lib_dbs.f90:2618-2743): the result stays zero and the
+        // update adds nothing, so only int 1/2 contribute
+        for (int in = 1; in <= 2; ++in) {
+          const char sym = 'n';
+          const int* t = tab[itype][in - 1];
+          std::vector<int> goff(1, 0), jl;
+          std::vector<double> cl;
+          for (int ik = 0; ik < nrk; ++ik) {
+            if (kk != kr1[ik]) continue;
+            int jnt, j;
+            if (i == kr3[ik]) { jnt = kr2[ik]; j = kr4[ik]; }
+            else if (i == kr4[ik]) {
+              DBS_CHECK(kr2[ik] >= 1 && kr2[ik] <= 4, DBS_ERR_ARG, "hf_matrix_breit: kr2 out of range");
+              jnt = kint[kr2[ik] - 1]; j = kr3[ik];
+            } else continue;
+            if (jnt != in) continue;
+            DBS_CHECK(j >= 1 && j <= norb, DBS_ERR_ARG, "hf_matrix_breit: orbital index out of range");
+            jl.push_back(j);
+            cl.push_back(crk[ik]);
+          }
+          if (cl.empty()) continue;
+          goff.push_back((int)cl.size());
+          DevMem lists(s);
+          int *d_g, *d_j;
+          double* d_c;
+          if ((rc = lists.put(goff, &d_g)) || (rc = lists.put(jl, &d_j)) || (rc = lists.put(cl, &d_c))) return rc;
+          const int type = DBS_PPQQ + itype;
+          const long long nel = (long long)ns * (2 * ks - 1);
+          k_run_density<<<grid_for(nel, 256), 256, 0, s>>>(1, d_g, d_j, d_j, d_c, pq_dev, ns, ks, t[2], t[3],
+                                                            sym_of(sym), XX);
+          double* rkb = nullptr;
+          if ((rc = build_rkb(ctx, type, k, &rkb))) return rc;
+          if ((rc = convol_launch(ctx, rkb, type, in, sym, sym, XX, X))) return rc;
+          if ((rc = dbs_update_hs_dev(ctx, hfm, t[0], t[1], X, sym))) return rc;
+        }
+      }
+  }
+  k_div<<<grid_for((long long)ms * ms, 256), 256, 0, s>>>((long long)ms * ms, hfm, qsum);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return fail(DBS_ERR_HIP, std::string("hf_matrix: ") + hipGetErrorString(e));
+  return DBS_OK;
+}
+
 int rs_data(dbs_ctx* ctx, const dbs_mat_blocks* m, const dbs_mat3_aux* aux, int stype, int itype, int atype, int k,
             int nc, const double* c, const int* K1, const int* K2, const int* K3, const int* K4,
             const double* pq_dev, int norb) {
@@ -488,6 +610,17 @@ int dbs_sdata_dev(dbs_ctx* ctx, const dbs_mat_blocks* m, const dbs_mat3_aux* aux
                   int norb) {
   const int ibtype = 10;  // mod_dbsr_mat.f90:103
   return rs_data(ctx, m, aux, 1, ktype % ibtype, ktype / ibtype, k, nc, c, K1, K2, K3, K4, pq_dev, norb);
+}
+
+int dbs_hf_matrix_dev(dbs_ctx* ctx, int i, int norb, const double* pq_dev, const double* acoef,
+                      const double* bcoef, int kmax, double qsum, int mbreit, int nrk, const int* kr1, const int* kr2,
+                      const int* kr3, const int* kr4, const double* crk, double* hfm_dev) {
+  DBS_CHECK(ctx && pq_dev && acoef && bcoef && hfm_dev, DBS_ERR_ARG, "hf_matrix: null argument");
+  DBS_CHECK(i >= 1 && i <= norb && kmax >= 0, DBS_ERR_ARG, "hf_matrix: orbital or kmax out of range");
+  DBS_CHECK(qsum != 0.0, DBS_ERR_ARG, "hf_matrix: qsum = 0");
+  DBS_CHECK(mbreit != 2 || nrk <= 0 || (kr1 && kr2 && kr3 && kr4 && crk), DBS_ERR_ARG,
+            "hf_matrix: Breit list missing");
+  return hf_matrix(ctx, i, norb, pq_dev, acoef, bcoef, kmax, qsum, mbreit, nrk, kr1, kr2, kr3, kr4, crk, hfm_dev);
 }
 
 int dbs_bav_dev(dbs_ctx* ctx, const double* a_dev, const double* b_dev, char sym, char dir, double* c_dev) {

@@ -209,6 +209,33 @@ class DBS:
         self._post()
         return y
 
+    def hf_matrix(self, i, pq, acoef, bcoef, kmax, qsum, hfm=None, breit=None):
+        """two-electron part of hf_matrix for orbital i (1-based): pq device
+        tensor holding pq(ns,2,norb); acoef/bcoef (norb, kmax+1) host arrays
+        of a(i,j,k), b(i,j,k); breit = (kr1, kr2, kr3, kr4, crk) or None.
+        hfm: device tensor holding hfm(ms,ms) (zeros if None), updated."""
+        norb = int(pq.shape[0])
+        if hfm is None:
+            hfm = self._empty(self.ms, self.ms)
+        A = np.asfortranarray(acoef, dtype=np.float64)
+        B = np.asfortranarray(bcoef, dtype=np.float64)
+        ci = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+        if breit is None:
+            lst, mb = (0, None, None, None, None, None), 0
+        else:
+            kr1, kr2, kr3, kr4, crk = breit
+            lst = (len(crk), ci(kr1), ci(kr2), ci(kr3), ci(kr4), np.ascontiguousarray(crk, dtype=np.float64))
+            mb = 2
+        ip_ = C.POINTER(C.c_int)
+        dp_ = C.POINTER(C.c_double)
+        arr = lambda a, t: a.ctypes.data_as(t) if a is not None else None
+        self._pre()
+        check(lib().dbs_hf_matrix_dev(self.ctx, i, norb, _ptr(pq), A.ctypes.data_as(dp_), B.ctypes.data_as(dp_),
+                                      kmax, float(qsum), mb, lst[0], arr(lst[1], ip_), arr(lst[2], ip_),
+                                      arr(lst[3], ip_), arr(lst[4], ip_), arr(lst[5], dp_), _ptr(hfm)))
+        self._post()
+        return hfm
+
     def UPDATE_HS(self, hm, ii, jj, d, sym):
         """UPDATE_HS(ms,hm,ii,jj,ns,ks,d,sym): hm device tensor holding (ms,ms)"""
         self._pre()

@@ -288,6 +288,19 @@ int dbs_rdata_dev(dbs_ctx *ctx, const dbs_mat_blocks *m, const dbs_mat3_aux *aux
 int dbs_sdata_dev(dbs_ctx *ctx, const dbs_mat_blocks *m, const dbs_mat3_aux *aux, int ktype, int k,
                   int nc, const double *c, const int *K1, const int *K2, const int *K3, const int *K4,
                   const double *pq_dev, int norb);
+/* hf_matrix, two-electron part (DBSR_HF/hf_matrix.f90:2-78 with
+ * mrk_aaaa :82-92; Breit block hf_matrix_breit :97-163 when mbreit == 2):
+ * for orbital i (1-based) of pq_dev(ns,2,norb), with acoef/bcoef(norb,
+ * kmax+1) the reference's a(i,j,k) / b(i,j,k) (host), adds the direct and
+ * exchange potentials of every (P/Q block, k) to hfm_dev(ms,ms) by UPDATE_HS
+ * in the reference's order, then divides hfm by qsum(i).  The caller puts
+ * qsum(i)*dhl (one-electron operator) or zeros into hfm first.  Breit list
+ * (rk4_data, DBSR_HF/dbsr_lib_zcom.f90:2984-3003): kr1 = k + 1000*itype,
+ * kr2 = int (1..4), kr3, kr4 orbitals, crk coefficients (host). */
+int dbs_hf_matrix_dev(dbs_ctx *ctx, int i, int norb, const double *pq_dev, const double *acoef,
+                      const double *bcoef, int kmax, double qsum, int mbreit, int nrk, const int *kr1,
+                      const int *kr2, const int *kr3, const int *kr4, const double *crk,
+                      double *hfm_dev);
 /* BAV (called at DBSR_MAT3/R_data.f90:181, S_data.f90:205; defined in the
  * un-vendored Libraries_03): c(ns) = A b (dir 'r') or A^T b (dir 'l'), A the
  * (ns,ns) matrix held in storage sym 's','l','n','x'. */

@@ -1799,3 +1799,94 @@ int orc_add_integral_mat3(orc_cdata *cd, int icase, int kpol, int i1, int i2, in
   }
   return -1;
 }
+
+/* ===================================================================== */
+/* hf_matrix, two-electron part: DBSR_HF/hf_matrix.f90:2-78 (direct and   */
+/* exchange per (ii,jj) block and k) and hf_matrix_breit :97-163.         */
+/* a(j,k), b(j,k): the coefficients a(i,j,k), b(i,j,k) of orbital i       */
+/* ((norb, kmax+1) column-major).  hfm(ms,ms) holds on entry what the     */
+/* caller put there (the reference's qsum(i)*dhl); on return it has the    */
+/* two-electron terms added and is divided by qsum.  The Breit list       */
+/* (kr1..kr4, crk) is used when mbreit == 2.                              */
+/* ===================================================================== */
+void orc_hf_matrix(const orc_grid *g, int i, int norb, const double *pq, const double *acoef,
+                   const double *bcoef, int kmax, double qsum, int mbreit, int nrk, const int *kr1,
+                   const int *kr2, const int *kr3, const int *kr4, const double *crk, double *hfm) {
+  const int ns = g->ns, ks = g->ks, ms = g->ms;
+  const size_t nn = (size_t)ns * ns;
+  double *d = (double *)malloc(sizeof(double) * nn * 4);
+  double *dd = d + nn, *x = dd + nn, *xx = x + nn;
+  double *rkb = (double *)malloc(sizeof(double) * orc_rkb_size(g, ORC_PPQQ));
+#define P(comp, orb) (pq + (size_t)ns * (((comp) - 1) + 2 * (size_t)((orb) - 1)))
+  for (int ii = 1; ii <= 2; ++ii)
+    for (int jj = 1; jj <= 2; ++jj)
+      for (int k = 0; k <= kmax; ++k) {
+        const int type = ii == 1 ? (jj == 1 ? ORC_PPPP : ORC_PQPQ) : (jj == 1 ? ORC_QPQP : ORC_QQQQ);
+        orc_mrk(g, type, k, rkb);
+        memset(dd, 0, sizeof(double) * nn);
+        int met = 0;
+        for (int j = 1; j <= norb; ++j) {
+          double c = acoef[(j - 1) + (size_t)norb * k];
+          if (c == 0.0) continue;
+          if (i == j) c = c + c;
+          orc_density(ns, ks, d, P(jj, j), P(jj, j), 's');
+          for (size_t q = 0; q < (size_t)ns * ks; ++q) dd[q] = dd[q] + c * d[q];
+          ++met;
+        }
+        if (met > 0) {
+          orc_convol(ns, ks, ks, rkb, d, dd, 1, 's', 's');
+          orc_update_hs_lib(ms, hfm, ii, ii, ns, ks, d, 's');
+        }
+        memset(xx, 0, sizeof(double) * nn);
+        met = 0;
+        for (int j = 1; j <= norb; ++j) {
+          const double c = bcoef[(j - 1) + (size_t)norb * k];
+          if (c == 0.0) continue;
+          orc_density(ns, ks, x, P(ii, j), P(jj, j), 'x');
+          for (size_t q = 0; q < nn; ++q) xx[q] = xx[q] + x[q] * c;
+          ++met;
+        }
+        if (met > 0) {
+          orc_convol(ns, ks, ks, rkb, x, xx, 4, 's', 's');
+          orc_update_hs_lib(ms, hfm, ii, jj, ns, ks, x, 'x');
+        }
+      }
+  if (mbreit == 2) {
+    /* (itype, int) -> ip1, ip2, jp1, jp2 (hf_matrix.f90:122-137) */
+    static const int tab[2][4][4] = {{{1, 2, 1, 2}, {1, 2, 1, 2}, {1, 2, 2, 1}, {2, 1, 1, 2}},
+                                     {{1, 2, 2, 1}, {2, 1, 1, 2}, {1, 1, 2, 2}, {2, 2, 1, 1}}};
+    static const int kint[4] = {2, 1, 4, 3};
+    const size_t sk = (size_t)ns * (2 * ks - 1);
+    for (int itype = 0; itype <= 1; ++itype)
+      for (int k = 0; k <= kmax; ++k) {
+        const int kk = k + 1000 * itype;
+        orc_mrk(g, itype == 0 ? ORC_PPQQ : ORC_PQQP, k, rkb);
+        for (int in = 1; in <= 4; ++in) {
+          const char sym = in <= 2 ? 'n' : 'x';
+          const int *t = tab[itype][in - 1];
+          memset(xx, 0, sizeof(double) * nn);
+          int met = 0;
+          for (int ik = 0; ik < nrk; ++ik) {
+            if (kk != kr1[ik]) continue;
+            int jnt, j;
+            if (i == kr3[ik]) { jnt = kr2[ik]; j = kr4[ik]; }
+            else if (i == kr4[ik]) { jnt = kint[kr2[ik] - 1]; j = kr3[ik]; }
+            else continue;
+            if (jnt != in) continue;
+            orc_density(ns, ks, x, P(t[2], j), P(t[3], j), sym);
+            const size_t cols = sym == 'x' ? nn : sk;
+            for (size_t q = 0; q < cols; ++q) xx[q] = xx[q] + crk[ik] * x[q];
+            ++met;
+          }
+          if (met > 0) {
+            orc_convol(ns, ks, 2 * ks - 1, rkb, x, xx, in, sym, sym);
+            orc_update_hs_lib(ms, hfm, t[0], t[1], ns, ks, x, sym);
+          }
+        }
+      }
+  }
+#undef P
+  for (size_t q = 0; q < (size_t)ms * ms; ++q) hfm[q] = hfm[q] / qsum;
+  free(d);
+  free(rkb);
+}

@@ -126,6 +126,9 @@ int  orc_idef_otype(double C, int ic, int jc, int io, int jo, double eps_c, int 
 int  orc_add_integral_mat3(orc_cdata *cd, int icase, int kpol, int i1, int i2, int i3, int i4, double C,
                            int ic, int jc, int io, int jo, const int *lbs, const int *kbs, const int *ipbs,
                            int ibo, int mk, double eps_c, int check_target);
+void orc_hf_matrix(const orc_grid *g, int i, int norb, const double *pq, const double *acoef,
+                   const double *bcoef, int kmax, double qsum, int mbreit, int nrk, const int *kr1,
+                   const int *kr2, const int *kr3, const int *kr4, const double *crk, double *hfm);
 void orc_bav(int ns, int ks, const double *a, const double *b, double *c, char sym, char dir);
 int  orc_rs_data(int ns, int ks, const double *rkb, int stype, int itype, int atype, int nc,
                  const double *cdata, const int *K1, const int *K2, const int *K3, const int *K4,

@@ -94,6 +94,8 @@ def lib():
                                  KBORT_FN, C.c_void_p, _dp, _dp, C.c_int, _ip, _dp,
                                  C.c_int, C.c_int, _ip, _dp, _ip, _dp, _ip, _dp]
         L.orc_bav.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, C.c_char, C.c_char]
+        L.orc_hf_matrix.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp, C.c_int, C.c_double, C.c_int,
+                                    C.c_int, _ip, _ip, _ip, _ip, _dp, _dp]
         L.orc_rs_data.restype = C.c_int
         L.orc_rs_data.argtypes = ([C.c_int, C.c_int, _dp] + [C.c_int] * 4 + [_dp] + [_ip] * 4 +
                                   [_dp, C.c_int, _ip, KBORT_FN, C.c_void_p, _dp, _dp, C.c_int, _ip, _dp,
@@ -298,6 +300,26 @@ def o_data(ms, itype, lst, eps_c, ibo, ipbs, kbort, vch, fppqq, iitar, iptar, ot
                             _d(mats["hcp"]), _i(ibb), _d(mats["hp"]))
 
 
+def hf_matrix(g, i, pq, acoef, bcoef, kmax, qsum, hfm=None, breit=None):
+    """hf_matrix two-electron part for orbital i (1-based); acoef/bcoef
+    (norb, kmax+1) = a(i,j,k), b(i,j,k); breit = (kr1, kr2, kr3, kr4, crk)
+    or None.  Returns hfm (ms,ms)."""
+    norb = pq.shape[2]
+    h = np.zeros((g.ms, g.ms), order="F") if hfm is None else hfm
+    ci = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+    if breit is None:
+        z = np.zeros(1, dtype=np.int32)
+        br = (0, z, z, z, z, np.zeros(1))
+        mb = 0
+    else:
+        kr1, kr2, kr3, kr4, crk = breit
+        br = (len(crk), ci(kr1), ci(kr2), ci(kr3), ci(kr4), np.ascontiguousarray(crk, dtype=np.float64))
+        mb = 2
+    lib().orc_hf_matrix(g.h, i, norb, _d(f64(pq)), _d(f64(acoef)), _d(f64(bcoef)), kmax, qsum, mb, br[0],
+                        _i(br[1]), _i(br[2]), _i(br[3]), _i(br[4]), _d(br[5]), _d(h))
+    return h
+
+
 def bav(ns, ks, a, b, sym, dir_):
     """BAV restated: A b ('r') or A^T b ('l') for A held in storage sym"""
     c = np.zeros(ns)
@@ -441,6 +463,8 @@ def ref():
         L.ref_set_orbitals.argtypes = [C.c_int, _dp]
         L.ref_sk.restype = C.c_double
         L.ref_sk.argtypes = [C.c_int] * 6
+        L.ref_hf_setup.argtypes = [C.c_int] * 3 + [_ip] * 3 + [_dp, _dp, _dp, C.c_int] + [_ip] * 4 + [_dp]
+        L.ref_hf_matrix.argtypes = [C.c_int, _dp]
         L.ref_full_mat_sym.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_char]
         L.ref_cdata_alloc.argtypes = [C.c_int] * 6 + [C.c_double, C.c_int]
         L.ref_cdata_add.argtypes = [C.c_double] + [C.c_int] * 6

@@ -248,3 +248,48 @@
         s = sk_pqqp(i1, j1, i2, j2, k)
       end if
       End Function ref_sk
+
+      Subroutine ref_hf_setup(n, kmax_, mbreit_, kbs_, lbs_, jbs_, qsum_, coef_, pq, nrk_, k1, k2, k3, k4, c) &
+                 bind(C, name='ref_hf_setup')
+! ... module state of DBSR_HF for hf_matrix: orbitals, energy coefficients
+! ... coef(i,j,k), kmax, mbreit and the Breit list (rk4_data)
+      Use iso_c_binding
+      Use DBS_grid
+      Use DBS_dhl_pq
+      Use df_orbitals
+      Use hf_energy
+      Use dbsr_hf, only: kmax, mbreit
+      Use rk4_data
+      Implicit none
+      Integer(c_int), value :: n, kmax_, mbreit_, nrk_
+      Integer(c_int), intent(in) :: kbs_(n), lbs_(n), jbs_(n), k1(*), k2(*), k3(*), k4(*)
+      Real(c_double), intent(in) :: qsum_(n), coef_(n, n, 0:kmax_), pq(ns, 2, n), c(*)
+      Call alloc_df_orbitals(n)
+      Call alloc_DF_radial(ns)
+      kbs(1:n) = kbs_; lbs(1:n) = lbs_; jbs(1:n) = jbs_; qsum(1:n) = qsum_
+      p(:, :, 1:n) = pq
+      if (allocated(coef)) Deallocate(coef)
+      Allocate(coef(n, n, 0:kmax_)); coef = coef_
+      kmax = kmax_; mbreit = mbreit_
+      if (allocated(crk)) Deallocate(crk, kr1, kr2, kr3, kr4)
+      mrk = max(nrk_, 1); Allocate(crk(mrk), kr1(mrk), kr2(mrk), kr3(mrk), kr4(mrk))
+      nrk = nrk_
+      if (nrk > 0) then
+        crk(1:nrk) = c(1:nrk); kr1(1:nrk) = k1(1:nrk); kr2(1:nrk) = k2(1:nrk)
+        kr3(1:nrk) = k3(1:nrk); kr4(1:nrk) = k4(1:nrk)
+      end if
+! ... the one-electron operator is outside the two-electron path: dhl = 0
+      Call alloc_dhl_pq(ms)
+      End Subroutine ref_hf_setup
+
+      Subroutine ref_hf_matrix(i, hfm) bind(C, name='ref_hf_matrix')
+      Use iso_c_binding
+      Use DBS_grid
+      Use DBS_dhl_pq
+      Use df_orbitals, only: kbs
+      Implicit none
+      Integer(c_int), value :: i
+      Real(c_double) :: hfm(ms, ms)
+      k_dhl = kbs(i)
+      Call hf_matrix(i, hfm)
+      End Subroutine ref_hf_matrix
