@@ -1142,11 +1142,13 @@ __global__ void __launch_bounds__(256) k_adj_orb_lds(
 __global__ void __launch_bounds__(256) k_adj_orb_mfma(
     int ninst, const long long* __restrict__ seg, const InstDev* __restrict__ inst,
     const KGroupDev* __restrict__ kgs, const int* __restrict__ dpk, const double* __restrict__ orbv,
-    const double* __restrict__ wt, const double* __restrict__ Gb, double* __restrict__ gorb, long long total) {
+    const double* __restrict__ wt, const double* __restrict__ Gb, double* __restrict__ gorb, long long total,
+    const int* __restrict__ accf) {
   extern __shared__ double sm[];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
   for (long long item = blockIdx.x; item < total; item += gridDim.x) {
     const int s = find_seg(seg, ninst, item);
+    if (accf[s]) continue;  // instance done by k_adj_orb_acc
     const InstDev I = inst[s];
     const int r = (int)(item - seg[s]);
     const int iv = r / I.ks, m = r % I.ks;
@@ -1287,6 +1289,101 @@ __global__ void __launch_bounds__(256) k_adj_orb_mfma(
         const int comp = col / ks, i = col % ks;
         gorb[I.off_orbv + ((long long)(2 * a + comp) * nv + iv) * np + ks + m * ks + i] = acc[j];
       }
+    }
+  }
+}
+
+// adjoint phase C when every (instance,k) group of the instance holds all
+// orbital pairs of the total density (the densities are then listed in the
+// same (a,c) order, a <= c, in every group).  One workgroup per (instance,
+// iv, m); thread t owns densities t, t+256, ...: it sums their gradients over
+// the groups in registers -- the outer-point combination G_Mk Wk + (G_Mm +
+// G_Ro) Wm and the nested-point G_Y weighted by each group's wkin(m,i) --
+// with no barrier inside the group loop, and stores the ks+1 sums per density
+// to LDS (V[q][d], q = 0 the outer point, 1..ks the nested points).  After
+// one barrier every (point, orbital, component) output is a row of the
+// symmetric orbital matrix (diagonal doubled) applied to the orbital values,
+// read straight from V through the pair index d(a,c).  Items are laid out
+// XCD-major (workgroup b runs item (b&7)*per + (b>>3)) so the ks items of an
+// interval share one L2 for the interval's S/Mm rows.
+template <int U>
+__global__ void __launch_bounds__(256) k_adj_orb_acc(
+    int ninst, const long long* __restrict__ seg, const InstDev* __restrict__ inst,
+    const KGroupDev* __restrict__ kgs, const double* __restrict__ orbv, const double* __restrict__ wt,
+    const double* __restrict__ Gb, double* __restrict__ gorb, long long total, long long per,
+    const int* __restrict__ accf) {
+  constexpr int KSM = 10;
+  extern __shared__ double sm[];
+  const int xcd = blockIdx.x & 7;
+  for (long long j = blockIdx.x >> 3; j < per; j += gridDim.x >> 3) {
+    const long long item = xcd * per + j;
+    if (item >= total) break;
+    const int s = find_seg(seg, ninst, item);
+    if (!accf[s]) continue;  // instance done by k_adj_orb_mfma
+    const InstDev I = inst[s];
+    const int r = (int)(item - seg[s]);
+    const int iv = r / I.ks, m = r % I.ks;
+    const int norb = I.norb, ks = I.ks, np = I.np, nv = I.nv, nk = nv * ks;
+    const int nd = norb * (norb + 1) / 2, FW = 2 * (ks + 1);
+    double* V = sm;                          // [ks+1][nd]
+    double* FV = V + (size_t)(ks + 1) * nd;  // [norb][FW]: column 2q + comp
+    __syncthreads();                         // the previous item's readers are done
+    for (int q = threadIdx.x; q < norb * FW; q += blockDim.x) {
+      const int c = q / FW, col = q % FW, pt = col >> 1, comp = col & 1;
+      const long long base = I.off_orbv + ((long long)(2 * c + comp) * nv + iv) * np;
+      FV[q] = orbv[base + (pt == 0 ? m : ks + m * ks + pt - 1)];
+    }
+    double vo[U], vy[U][KSM];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      vo[u] = 0.0;
+#pragma unroll
+      for (int i = 0; i < KSM; ++i) vy[u][i] = 0.0;
+    }
+    const long long xr = 2LL * (nv + iv * ks + m);
+    for (int gi = 0; gi < I.nkg; ++gi) {
+      const KGroupDev G = kgs[I.kg0 + gi];
+      const double* w = wt + G.off_w;
+      const double wkv = w[(long long)iv * ks + m], wmv = w[(long long)nk + (long long)iv * ks + m];
+      double wi[KSM];
+#pragma unroll
+      for (int i = 0; i < KSM; ++i) wi[i] = i < ks ? w[2LL * nk + ((long long)iv * ks + m) * ks + i] : 0.0;
+      const double* gb = Gb + G.off_F;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int d = threadIdx.x + u * 256;
+        if (d < nd) {
+          vo[u] += gb[2LL * iv * nd + d] * wkv + (gb[(2LL * iv + 1) * nd + d] + gb[xr * nd + d]) * wmv;
+          const double gy = gb[(xr + 1) * nd + d];
+#pragma unroll
+          for (int i = 0; i < KSM; ++i) vy[u][i] += gy * wi[i];
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int d = threadIdx.x + u * 256;
+      if (d < nd) {
+        V[d] = vo[u];
+#pragma unroll
+        for (int i = 0; i < KSM; ++i)
+          if (i < ks) V[(size_t)(1 + i) * nd + d] = vy[u][i];
+      }
+    }
+    __syncthreads();
+    // outputs (point q, orbital a, component comp), row a of the matrix of q
+    for (int t = threadIdx.x; t < (ks + 1) * 2 * norb; t += blockDim.x) {
+      const int q = t / (2 * norb), o = t % (2 * norb), a = o >> 1, comp = o & 1;
+      const double* Vq = V + (size_t)q * nd;
+      const double* f = FV + 2 * q + comp;
+      double x = 0.0;
+      for (int c = 0; c < norb; ++c) {
+        const int lo = a < c ? a : c, hi = a < c ? c : a;
+        const double v = Vq[lo * norb - lo * (lo - 1) / 2 + (hi - lo)];
+        x += (a == c ? 2.0 * v : v) * f[c * FW];
+      }
+      const long long base = I.off_orbv + ((long long)o * nv + iv) * np;
+      gorb[base + (q == 0 ? m : ks + m * ks + q - 1)] = x;
     }
   }
 }
@@ -1446,6 +1543,10 @@ struct dbs_engine {
   long long* d_ppoff = nullptr;    // per instance offset of the projection partials
   double* d_ppart = nullptr;
   size_t adjm_shm = 0;       // LDS of the matrix-core adjoint kernel (0: not usable)
+  int adjacc_u = 0;          // densities per thread of k_adj_orb_acc (0: no instance uses it)
+  size_t adjacc_shm = 0;
+  bool adjacc_all = false;   // every instance uses k_adj_orb_acc
+  int* d_accf = nullptr;     // per instance: 1 = adjoint by k_adj_orb_acc
   int featm_wave = 0;        // LDS doubles per wave of the matrix-core kernel
   int max_nd = 0;         // largest group
   int adj_mode = 0;
@@ -1903,9 +2004,24 @@ static int run_adjoint(dbs_engine* e) {
   k_scan_cols<<<grid_for(e->nscan_items, 1, cap), 256, 0, s>>>(e->nscan_items, e->d_scan_items, e->d_kg,
                                                                e->d_inst, e->G, 1);
   mark(e, "adj_suffix_scan");
+  if (e->adjacc_u > 0) {
+    const long long per = (e->adj_items + 7) / 8;
+    const unsigned grid = 8u * (unsigned)std::min<long long>(per, std::max(1, cap * 4 / 8));
+    if (e->adjacc_u == 1)
+      k_adj_orb_acc<1><<<grid, 256, e->adjacc_shm, s>>>(e->ninst, e->d_seg_adj, e->d_inst, e->d_kg, e->orbv,
+                                                        e->wt, e->G, e->gorb, e->adj_items, per, e->d_accf);
+    else if (e->adjacc_u == 2)
+      k_adj_orb_acc<2><<<grid, 256, e->adjacc_shm, s>>>(e->ninst, e->d_seg_adj, e->d_inst, e->d_kg, e->orbv,
+                                                        e->wt, e->G, e->gorb, e->adj_items, per, e->d_accf);
+    else
+      k_adj_orb_acc<4><<<grid, 256, e->adjacc_shm, s>>>(e->ninst, e->d_seg_adj, e->d_inst, e->d_kg, e->orbv,
+                                                        e->wt, e->G, e->gorb, e->adj_items, per, e->d_accf);
+  }
   if (e->adjm_shm > 0) {
-    k_adj_orb_mfma<<<grid_for(e->adj_items, 1, cap * 4), 256, e->adjm_shm, s>>>(
-        e->ninst, e->d_seg_adj, e->d_inst, e->d_kg, e->d_dpk, e->orbv, e->wt, e->G, e->gorb, e->adj_items);
+    if (!e->adjacc_all)
+      k_adj_orb_mfma<<<grid_for(e->adj_items, 1, cap * 4), 256, e->adjm_shm, s>>>(
+          e->ninst, e->d_seg_adj, e->d_inst, e->d_kg, e->d_dpk, e->orbv, e->wt, e->G, e->gorb, e->adj_items,
+          e->d_accf);
   } else if (e->adj_shm > 0) {
     k_adj_orb_lds<<<grid_for(e->adj_items, 1, cap * 4), 256, e->adj_shm, s>>>(
         e->ninst, e->d_seg_adj, e->d_inst, e->d_kg, e->d_dens, e->orbv, e->wt, e->G, e->adj_mode, e->gorb,
@@ -2334,6 +2450,44 @@ int dbs_engine_create(dbs_engine** out, int ninst, dbs_ctx* const* ctxs, const i
       std::vector<int> pk(std::max(e->ndens, 1), 0);
       for (int d = 0; d < e->ndens; ++d) pk[d] = (e->h_dens[d].a & 255) | ((e->h_dens[d].c & 255) << 8);
       rc |= upload(s, &e->d_dpk, pk);
+      // register-accumulating form, per instance: every group complete,
+      // ks <= 10, up to 1024 densities (the choice depends on the instance
+      // alone, so batched and single-instance results are identical)
+      std::vector<int> accf(ninst, 0);
+      int maxnd = 0, nacc = 0;
+      size_t aneed2 = 0;
+      for (int i = 0; i < ninst && e->adjm_shm > 0; ++i) {
+        const InstDev& I = e->h_inst[i];
+        bool a = I.ks <= 10 && I.nkg > 0;
+        int nd = 0;
+        for (int gi = 0; gi < I.nkg; ++gi) {
+          const KGroupDev& G = e->h_kg[I.kg0 + gi];
+          if (!G.full) a = false;
+          nd = std::max(nd, G.ndens);
+          // the kernel finds pair (a,c), a <= c, at a*norb - a(a-1)/2 + c - a
+          for (int d = 0; a && d < G.ndens; ++d) {
+            const DensDev& D = e->h_dens[G.dens0 + d];
+            if (D.a > D.c || D.a * I.norb - D.a * (D.a - 1) / 2 + (D.c - D.a) != d) a = false;
+          }
+        }
+        if (nd > 1024) a = false;
+        if (!a) continue;
+        accf[i] = 1;
+        ++nacc;
+        maxnd = std::max(maxnd, nd);
+        aneed2 = std::max(aneed2, (size_t)(I.ks + 1) * nd + (size_t)I.norb * 2 * (I.ks + 1));
+      }
+      const int U = maxnd <= 256 ? 1 : (maxnd <= 512 ? 2 : 4);
+      if (aneed2 * sizeof(double) > (size_t)lmx) { std::fill(accf.begin(), accf.end(), 0); nacc = 0; }
+      e->adjacc_u = nacc > 0 ? U : 0;
+      e->adjacc_all = nacc == ninst;
+      e->adjacc_shm = e->adjacc_u ? aneed2 * sizeof(double) : 0;
+      rc |= upload(s, &e->d_accf, accf);
+      if (e->adjacc_shm > 64 * 1024) {
+        hipFuncSetAttribute((const void*)k_adj_orb_acc<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)e->adjacc_shm);
+        hipFuncSetAttribute((const void*)k_adj_orb_acc<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)e->adjacc_shm);
+        hipFuncSetAttribute((const void*)k_adj_orb_acc<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)e->adjacc_shm);
+      }
     }
     std::vector<long long> seg_adj(ninst);
     long long tot = 0;
@@ -2404,7 +2558,7 @@ int dbs_engine_destroy(dbs_engine* e) {
                   e->d_seg_w, e->d_roff, e->d_items, e->d_scan_items, e->d_oadjoff, e->d_oadj,
                   e->d_tiles, e->d_pairrow, e->d_pairoff, e->d_nextrow, e->plain_d, e->plain_c,
                   e->d_echunks, e->d_echoff, e->d_epart, e->d_seg_adj, e->d_chunks, e->d_ent,
-                  e->d_ecidx, e->d_red, e->d_part, e->d_alen, e->d_fitems, e->d_pidx, e->d_dpk, e->d_seg_ivc, e->d_ppoff, e->d_ppart,
+                  e->d_ecidx, e->d_red, e->d_part, e->d_alen, e->d_fitems, e->d_pidx, e->d_dpk, e->d_accf, e->d_seg_ivc, e->d_ppoff, e->d_ppart,
                   e->d_work[0][0], e->d_work[0][1], e->d_work[0][2],
                   e->d_work[1][0], e->d_work[1][1], e->d_work[1][2]};
   for (void* p : bufs)

@@ -442,3 +442,49 @@ def test_many_orbitals_fallback_paths(setups, kind):
     if kind == 0:
         Gref = oracle_gradient(g, fs, rows, coef)
         assert np.max(np.abs(eng.grad() - Gref)) < TOL * np.max(np.abs(Gref))
+
+
+@pytest.mark.parametrize("name,norb", [("small", 12), ("fe", 12), ("small", 28), ("fe", 40)])
+def test_complete_groups_register_adjoint(setups, name, norb):
+    """every multipole lists every orbital pair (a <= c), so the adjoint
+    sums each pair's gradients over the groups in registers (78 to 820
+    densities: one to four per thread); gradient against the oracle and
+    against central differences of E"""
+    from dbsr3_amd import Engine
+    g, d = setups[name]
+    fs = random_orbitals(g, norb, seed=7 * norb)
+    rng = np.random.default_rng(norb)
+    pairs = [(a, c) for a in range(1, norb + 1) for c in range(a, norb + 1)]
+    rows = []
+    for k in range(4):
+        for (a, c) in pairs:
+            b, dd = pairs[int(rng.integers(len(pairs)))]
+            rows.append((k, a, b, c, dd))
+    rows = np.array(rows, dtype=np.int32)
+    coef = rng.uniform(-1, 1, len(rows))
+    print(f"[complete groups {name} norb={norb} rows={len(rows)}]", flush=True)
+    eng = Engine([(d, norb, rows)])
+    eng.set_orbitals(0, pq_of(fs))
+    eng.set_coef(0, coef)
+    eng.forward_adjoint()
+    G = eng.grad()
+    sel = rng.choice(len(rows), 60, replace=False)
+    c2 = np.zeros(len(rows))
+    c2[sel] = coef[sel]
+    eng.set_coef(0, c2)
+    eng.forward_adjoint()
+    Gref = oracle_gradient(g, fs, rows[sel], coef[sel])
+    assert np.max(np.abs(eng.grad() - Gref)) < TOL * np.max(np.abs(Gref))
+    # central differences of the full E on two coefficients
+    eng.set_coef(0, coef)
+    pq = pq_of(fs)
+    for (j, comp, o) in [(3, 0, 1), (g.ns // 2, 1, norb - 1)]:
+        e_pm = []
+        for sgn in (1, -1):
+            p2 = pq.copy(order="F")
+            p2[j, comp, o] += sgn * 1e-5
+            eng.set_orbitals(0, p2)
+            eng.forward()
+            e_pm.append(eng.energy())
+        fd = (e_pm[0] - e_pm[1]) / 2e-5
+        assert abs(fd - G[j, comp, o]) <= 1e-6 * max(abs(fd), 1e-3 * np.max(np.abs(G)))
