@@ -1325,8 +1325,9 @@ __global__ void __launch_bounds__(256) k_adj_orb_acc(
     const int iv = r / I.ks, m = r % I.ks;
     const int norb = I.norb, ks = I.ks, np = I.np, nv = I.nv, nk = nv * ks;
     const int nd = norb * (norb + 1) / 2, FW = 2 * (ks + 1);
-    double* V = sm;                          // [ks+1][nd]
-    double* FV = V + (size_t)(ks + 1) * nd;  // [norb][FW]: column 2q + comp
+    const int QH = (ks + 2) / 2;             // points per half of the table
+    double* V = sm;                          // [QH][nd]: the points of one half
+    double* FV = V + (size_t)QH * nd;        // [norb][FW]: column 2q + comp
     __syncthreads();                         // the previous item's readers are done
     for (int q = threadIdx.x; q < norb * FW; q += blockDim.x) {
       const int c = q / FW, col = q % FW, pt = col >> 1, comp = col & 1;
@@ -1360,30 +1361,38 @@ __global__ void __launch_bounds__(256) k_adj_orb_acc(
         }
       }
     }
+    // the points in two halves (q0 = 0: points 0..QH-1, q0 = QH: the rest)
+    for (int q0 = 0; q0 <= ks; q0 += QH) {
+      const int nq = min(QH, ks + 1 - q0);
+      if (q0 > 0) __syncthreads();  // the first half's readers are done
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int d = threadIdx.x + u * 256;
-      if (d < nd) {
-        V[d] = vo[u];
+      for (int u = 0; u < U; ++u) {
+        const int d = threadIdx.x + u * 256;
+        if (d < nd) {
+          if (q0 == 0) V[d] = vo[u];
 #pragma unroll
-        for (int i = 0; i < KSM; ++i)
-          if (i < ks) V[(size_t)(1 + i) * nd + d] = vy[u][i];
+          for (int i = 0; i < KSM; ++i) {
+            const int q = 1 + i - q0;  // slot of nested point i in this half
+            if (i < ks && q >= 0 && q < nq) V[(size_t)q * nd + d] = vy[u][i];
+          }
+        }
       }
-    }
-    __syncthreads();
-    // outputs (point q, orbital a, component comp), row a of the matrix of q
-    for (int t = threadIdx.x; t < (ks + 1) * 2 * norb; t += blockDim.x) {
-      const int q = t / (2 * norb), o = t % (2 * norb), a = o >> 1, comp = o & 1;
-      const double* Vq = V + (size_t)q * nd;
-      const double* f = FV + 2 * q + comp;
-      double x = 0.0;
-      for (int c = 0; c < norb; ++c) {
-        const int lo = a < c ? a : c, hi = a < c ? c : a;
-        const double v = Vq[lo * norb - lo * (lo - 1) / 2 + (hi - lo)];
-        x += (a == c ? 2.0 * v : v) * f[c * FW];
+      __syncthreads();
+      // outputs (orbital a, component comp, point q0 + q): consecutive
+      // threads write the consecutive points of one orbital component
+      for (int t = threadIdx.x; t < nq * 2 * norb; t += blockDim.x) {
+        const int o = t / nq, q = t % nq, a = o >> 1, comp = o & 1, qq = q0 + q;
+        const double* Vq = V + (size_t)q * nd;
+        const double* f = FV + 2 * qq + comp;
+        double x = 0.0;
+        for (int c = 0; c < norb; ++c) {
+          const int lo = a < c ? a : c, hi = a < c ? c : a;
+          const double v = Vq[lo * norb - lo * (lo - 1) / 2 + (hi - lo)];
+          x += (a == c ? 2.0 * v : v) * f[c * FW];
+        }
+        const long long base = I.off_orbv + ((long long)o * nv + iv) * np;
+        gorb[base + (qq == 0 ? m : ks + m * ks + qq - 1)] = x;
       }
-      const long long base = I.off_orbv + ((long long)o * nv + iv) * np;
-      gorb[base + (q == 0 ? m : ks + m * ks + q - 1)] = x;
     }
   }
 }
@@ -2475,7 +2484,7 @@ int dbs_engine_create(dbs_engine** out, int ninst, dbs_ctx* const* ctxs, const i
         accf[i] = 1;
         ++nacc;
         maxnd = std::max(maxnd, nd);
-        aneed2 = std::max(aneed2, (size_t)(I.ks + 1) * nd + (size_t)I.norb * 2 * (I.ks + 1));
+        aneed2 = std::max(aneed2, (size_t)((I.ks + 2) / 2) * nd + (size_t)I.norb * 2 * (I.ks + 1));
       }
       const int U = maxnd <= 256 ? 1 : (maxnd <= 512 ? 2 : 4);
       if (aneed2 * sizeof(double) > (size_t)lmx) { std::fill(accf.begin(), accf.end(), 0); nacc = 0; }
