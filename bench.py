@@ -49,14 +49,14 @@ TRAFFIC_FILE = "engine_rk_fwd_adj_ns200_norb30_rows5e4_pmc.md"
 def pair_kernel(rows, tuning):
     """the pair-panel kernel the engine launches for this row set (total
     densities of each multipole; one density per thread when they fit a
-    workgroup: 512 at panel width 8, 256 at 16)"""
+    workgroup of up to 512)"""
     nd = 0
     for k in np.unique(rows[:, 0]):
         r = rows[rows[:, 0] == k]
         pairs = {(min(a, c), max(a, c)) for a, c in r[:, [1, 3]]} | {(min(b, d), max(b, d)) for b, d in r[:, [2, 4]]}
         nd = max(nd, len(pairs))
     w = tuning[2]
-    if w >= 8 and nd <= (256 if w >= 16 else 512):
+    if w >= 8 and nd <= 512:
         return "k_pair_rot", "false"
     return "k_pair_panel", "true"
 

@@ -659,33 +659,34 @@ __global__ void __launch_bounds__(pair_block_max<W>()) k_pair_panel(
 }
 
 // Bank-conflict-free pair panels for groups with ndens <= blockDim (one
-// density per thread).  The panel of density X is E = 2W doubles
+// density per thread, up to 512).  The panel of density X is E = 2W doubles
 // [u0 v0 u1 v1 ...] (element e = 2 jl + h), kept in LDS as B[X][e] with
 // rows of E doubles, so the LDS bank of element e does not depend on the
 // row.  Lane l works on the elements in the rotated order
-// e_t = (t + l) mod E: at every step the 16 lanes that share an LDS cycle
-// touch 16 different banks, wherever their partner rows are.  Registers
-// fx[t], gx[t] hold element e_t.  The transposition between the global
-// layout (rows e, columns X) and the rotated registers goes through a
-// per-thread column of A[e][S] (S even: conflict-free in both directions).
+// e_t = (t + rot) mod E, rot = l mod 16: at every step the 16 lanes that
+// share an LDS cycle touch 16 different banks, wherever their partner rows
+// are.  Registers fx[t], gx[t] hold element e_t: the panel is loaded
+// coalesced in element order and rotated in registers by four conditional
+// stages (rot bits 1, 2, 4, 8), and the gradient rotated back before its
+// coalesced store, so the LDS holds only the panels (E*ndens doubles).
+// W = 16 prefetches one adjacency entry (the registers of two panels of 32
+// elements would not fit), W = 8 two.
 template <int W, bool ONE, bool ADJ>
-__global__ void __launch_bounds__(pair_block_max<W>()) k_pair_rot(
+__global__ void __launch_bounds__(512) k_pair_rot(
     int nwork, const PairWork* __restrict__ work, const KGroupDev* __restrict__ kgs,
     const InstDev* __restrict__ inst, const AdjChunk* __restrict__ chunks,
     const AdjEnt* __restrict__ ent, const int* __restrict__ alen,
     const double* __restrict__ F, double* __restrict__ Gout, double* __restrict__ part) {
   extern __shared__ double sm[];
   constexpr int E = 2 * W;
-  constexpr int PF = 2;
+  constexpr int PF = W >= 16 ? 1 : 2;
   const int X = threadIdx.x;
   const int rot = X & 15;
   for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
     const PairWork Wk = work[w];
     const KGroupDev Gp = kgs[Wk.g];
     const int nv = inst[Gp.inst].nv, nd = Gp.ndens;
-    const int S = nd + (nd & 1);
-    double* A = sm;           // [E][S]
-    double* B = sm + E * S;   // [nd][E]
+    double* B = sm;  // [nd][E]
     const double* Fg = F + Gp.off_F;
     const bool act = X < nd;
     int n = 0;
@@ -704,11 +705,21 @@ __global__ void __launch_bounds__(pair_block_max<W>()) k_pair_rot(
       unsigned mL = 0;  // S^k: elements that pair as the row's left density
       if (act) {
 #pragma unroll
-        for (int e = 0; e < E; ++e) A[e * S + X] = (e >> 1) < nj ? Fg[(2LL * j0 + e) * nd + X] : 0.0;
+        for (int e = 0; e < E; ++e) fx[e] = (e >> 1) < nj ? Fg[(2LL * j0 + e) * nd + X] : 0.0;
+        // fx[t] = element (t + rot) mod E
+#pragma unroll
+        for (int b = 1; b < 16; b <<= 1) {
+          if (rot & b) {
+            double tmp[E];
+#pragma unroll
+            for (int t = 0; t < E; ++t) tmp[t] = fx[(t + b) & (E - 1)];
+#pragma unroll
+            for (int t = 0; t < E; ++t) fx[t] = tmp[t];
+          }
+        }
 #pragma unroll
         for (int t = 0; t < E; ++t) {
           const int e = (t + rot) & (E - 1);
-          fx[t] = A[e * S + X];
           B[X * E + e] = fx[t];
           gx[t] = 0.0;
           if (ONE) {
@@ -763,13 +774,21 @@ __global__ void __launch_bounds__(pair_block_max<W>()) k_pair_rot(
         }
       }
       if (ADJ && act) {
-        // own column of A: rotated registers -> rows e -> coalesced rows of G
+        // back to element order, then coalesced rows of G
 #pragma unroll
-        for (int t = 0; t < E; ++t) A[((t + rot) & (E - 1)) * S + X] = gx[t];
+        for (int b = 1; b < 16; b <<= 1) {
+          if (rot & b) {
+            double tmp[E];
+#pragma unroll
+            for (int t = 0; t < E; ++t) tmp[t] = gx[(t - b) & (E - 1)];
+#pragma unroll
+            for (int t = 0; t < E; ++t) gx[t] = tmp[t];
+          }
+        }
         double* g = Gout + Gp.off_F + X;
 #pragma unroll
         for (int e = 0; e < E; ++e)
-          if ((e >> 1) < nj) g[(2LL * j0 + e) * nd] = A[e * S + X];
+          if ((e >> 1) < nj) g[(2LL * j0 + e) * nd] = gx[e];
       }
       first = false;
     }
@@ -1703,7 +1722,7 @@ static void rot_lds_attr(size_t bytes) {
 }
 
 static size_t rot_lds_bytes(int W, int ndens) {
-  return (size_t)2 * W * (2 * (size_t)ndens + 1) * sizeof(double);
+  return (size_t)2 * W * (size_t)ndens * sizeof(double);
 }
 
 template <int W>
@@ -1791,7 +1810,7 @@ static int build_pair_tables(dbs_engine* e) {
   std::vector<RowRed> red;
   long long poff = 0;
   // groups for the rotated kernel: one density per thread
-  const int rot_max = W >= 16 ? 256 : 512;
+  const int rot_max = 512;
   int rot_nd = 0;
   for (int g = 0; g < e->nkg; ++g) {
     const int nd = e->h_kg[g].ndens;
