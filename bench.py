@@ -32,13 +32,14 @@ PEAK_FP64_TFS = 78.6
 SEED = 20261203 + 4
 
 
-# engine phase -> the kernel that runs it ({w}: pair-panel width)
+# engine phase -> the (main) kernel that runs it for this workload ({w}:
+# pair-panel width)
 PHASE_KERNEL = {
-    "orb_points": "k_orb_points", "weights": "k_weights", "features": "k_features",
+    "orb_points": "k_orb_values", "coef_gather": "k_adj_cgather", "features": "k_features_mfma",
     "prefix_scan": "k_scan_cols", "pair_panel": "{pair}<{w}, {v}, true>",
     "contract_rows": "{pair}<{w}, {v}, false>", "rows_reduce": "k_rows_reduce",
     "contract_dense": "k_contract_dense", "energy": "k_energy_partial",
-    "adj_suffix_scan": "k_scan_cols", "adj_orb_points": "k_adj_orb_lds", "adj_coef": "k_adj_coef",
+    "adj_suffix_scan": "k_scan_cols", "adj_orb_points": "k_adj_orb_acc<2>", "adj_coef": "k_adj_proj",
 }
 # rocprofv3 FETCH_SIZE/WRITE_SIZE summary of this workload (HBM bytes per
 # dispatch, copy-kernel scaled); rows "| kernel | tuning | ... | bytes |"

@@ -289,8 +289,10 @@ __global__ void __launch_bounds__(256) k_features(
 // the accumulators: the pair -> density table maps (a,c) to the column, and
 // with densities sorted by pair the 16 lanes of a tile row store 16
 // consecutive columns.  No workgroup barrier: waves run independently.
-// item = (group, interval, kind, quantity q): q < ks Y(q) and Ro(q);
-// ks M^k; ks+1 M^-(k+1).
+// item = (first group g, interval, kind | ng << 8, quantity q): q < ks Y(q)
+// and Ro(q); ks M^k; ks+1 M^-(k+1).  The item covers the ng consecutive
+// groups g.. of one instance (all with the kind): the orbital values of the
+// points are staged once and each group's tiles use its own weights.
 // ---------------------------------------------------------------------------
 __device__ inline int vsw(int comp, int p, int o, int npt, int NO) {
   return (comp * npt + p) * NO + (o ^ (p & (NO - 1)));
@@ -312,13 +314,12 @@ __global__ void __launch_bounds__(256) k_features_mfma(
   double* V = sm + (long long)wv * wave_lds;  // this wave's slice
   for (int it = blockIdx.x * nw + wv; it < nitems; it += gridDim.x * nw) {
     const int4 T = items[it];
-    const int g = T.x, iv = T.y, kind = T.z, q = T.w;
-    const KGroupDev G = kgs[g];
-    const InstDev I = inst[G.inst];
-    const int ks = I.ks, nv = I.nv, np = I.np, norb = I.norb, nd = G.ndens;
+    const int g0 = T.x, iv = T.y, kind = T.z & 255, ng = T.z >> 8, q = T.w;
+    const InstDev I = inst[kgs[g0].inst];
+    const int ks = I.ks, nv = I.nv, np = I.np, norb = I.norb;
     const int NO = norb <= 16 ? 16 : (norb <= 32 ? 32 : 64);
     const int npt = q < ks ? ks + 1 : ks;  // Y(q): nested points of q, then the outer point q
-    double* W = V + 2 * npt * NO;          // weight of each point; W[ks] = Wm(q)
+    double* Wa = V + 2 * npt * NO;         // [ng][npt] weight of each point; W[ks] = Wm(q)
     wave_lds_sync();                        // the previous item's reads are done
     for (int e = lane; e < 2 * NO * npt; e += 64) {
       const int lp = e % npt, oc = e / npt;  // oc = 2*orb + comp
@@ -326,17 +327,22 @@ __global__ void __launch_bounds__(256) k_features_mfma(
       const int p = q < ks ? (lp < ks ? ks + q * ks + lp : q) : lp;
       V[vsw(comp, lp, o, npt, NO)] = o < norb ? orbv[I.off_orbv + ((long long)oc * nv + iv) * np + p] : 0.0;
     }
-    if (lane < npt) {
-      const int lp = lane;
+    for (int e = lane; e < ng * npt; e += 64) {
+      const int lp = e % npt;
+      const long long ow = kgs[g0 + e / npt].off_w;
       double w;
-      if (q < ks) w = lp < ks ? wt[G.off_w + 2LL * nv * ks + ((long long)iv * ks + q) * ks + lp]
-                              : wt[G.off_w + (long long)nv * ks + (long long)iv * ks + q];
-      else w = wt[G.off_w + (q == ks ? 0LL : (long long)nv * ks) + (long long)iv * ks + lp];
-      W[lp] = w;
+      if (q < ks) w = lp < ks ? wt[ow + 2LL * nv * ks + ((long long)iv * ks + q) * ks + lp]
+                              : wt[ow + (long long)nv * ks + (long long)iv * ks + q];
+      else w = wt[ow + (q == ks ? 0LL : (long long)nv * ks) + (long long)iv * ks + lp];
+      Wa[e] = w;
     }
     wave_lds_sync();
     const int ntl = NO / 16;
     const bool sym = kind != DK_PQ;
+    for (int gi = 0; gi < ng; ++gi) {
+    const KGroupDev G = kgs[g0 + gi];
+    const int nd = G.ndens;
+    const double* W = Wa + gi * npt;
     const int slot = __popc(G.gmask & ((1 << kind) - 1));
     const int* pt = pidx + G.off_pidx + (long long)slot * NO * NO;
     double* Fg = F + G.off_F;
@@ -378,6 +384,7 @@ __global__ void __launch_bounds__(256) k_features_mfma(
         }
       }
     }
+    }  // groups of the item
   }
 }
 
@@ -539,6 +546,7 @@ __global__ void k_energy_final(int ninst, const int* __restrict__ choff, const d
 // (workgroups of at most 256 threads); W <= 8 runs 512-thread workgroups
 template <int W>
 constexpr int pair_block_max() { return W >= 16 ? 256 : 512; }
+constexpr int kFeatGroups = 4;  // groups per item of k_features_mfma
 
 template <int W, bool LDS, bool ADJ>
 __global__ void __launch_bounds__(pair_block_max<W>()) k_pair_panel(
@@ -1985,11 +1993,9 @@ static int run_forward(dbs_engine* e, int with_adj) {
     k_orb_points<<<grid_for(e->total_orbv, 256, cap * 4), 256, 0, s>>>(
         e->ninst, e->d_seg_orb, e->d_inst, e->pq, e->orbv, e->total_orbv);
   mark(e, "orb_points");
-  k_weights<<<grid_for(e->total_w, 256, cap * 4), 256, 0, s>>>(e->nkg, e->d_seg_w, e->d_kg, e->d_inst,
-                                                                e->wt, e->total_w);
   if (with_adj && e->nent > 0)
     k_adj_cgather<<<grid_for(e->nent, 256, cap * 4), 256, 0, s>>>(e->nent, e->d_ecidx, e->coef, e->d_ent);
-  mark(e, "weights");
+  mark(e, "coef_gather");
   if (e->featm_shm > 0)
     k_features_mfma<<<grid_for(e->nfitems, 4, cap * 4), 256, e->featm_shm, s>>>(
         e->nfitems, e->d_fitems, e->d_kg, e->d_inst, e->d_pidx, e->orbv, e->wt, e->F, e->featm_wave);
@@ -2422,7 +2428,7 @@ int dbs_engine_create(dbs_engine** out, int ninst, dbs_ctx* const* ctxs, const i
     for (auto& I : e->h_inst) {
       if (I.norb > 64) fits = false;
       const size_t NO = I.norb <= 16 ? 16 : (I.norb <= 32 ? 32 : 64);
-      wneed2 = std::max(wneed2, 2 * (size_t)(I.ks + 1) * NO + (I.ks + 1));
+      wneed2 = std::max(wneed2, 2 * (size_t)(I.ks + 1) * NO + kFeatGroups * (I.ks + 1));
     }
     wneed2 = (wneed2 + 1) & ~(size_t)1;  // 16-byte aligned slices
     e->featm_wave = (int)wneed2;
@@ -2447,10 +2453,27 @@ int dbs_engine_create(dbs_engine** out, int ninst, dbs_ctx* const* ctxs, const i
             if (D.kind == kd) pidx[base + (size_t)D.a * NO + D.c] = d;
           }
         }
-        for (int iv = 0; iv < I.nv; ++iv)
-          for (int kd = 0; kd < 4; ++kd)
-            if ((G.gmask >> kd) & 1)
-              for (int q = 0; q < I.ks + 2; ++q) fit.push_back(make_int4(g, iv, kd, q));
+      }
+      // runs of consecutive groups of one instance with the kind, at most
+      // kFeatGroups per item, spread evenly
+      for (int i = 0; i < ninst; ++i) {
+        const InstDev& I = e->h_inst[i];
+        for (int kd = 0; kd < 4; ++kd) {
+          int gi = 0;
+          while (gi < I.nkg) {
+            if (!((e->h_kg[I.kg0 + gi].gmask >> kd) & 1)) { ++gi; continue; }
+            int ge = gi;
+            while (ge < I.nkg && ((e->h_kg[I.kg0 + ge].gmask >> kd) & 1)) ++ge;
+            const int run = ge - gi, nch = (run + kFeatGroups - 1) / kFeatGroups;
+            for (int c = 0; c < nch; ++c) {
+              const int a = gi + run * c / nch, b = gi + run * (c + 1) / nch;
+              for (int iv = 0; iv < I.nv; ++iv)
+                for (int q = 0; q < I.ks + 2; ++q)
+                  fit.push_back(make_int4(I.kg0 + a, iv, kd | ((b - a) << 8), q));
+            }
+            gi = ge;
+          }
+        }
       }
       e->nfitems = (int)fit.size();
       rc |= upload(s, &e->d_fitems, fit);
@@ -2568,6 +2591,10 @@ int dbs_engine_create(dbs_engine** out, int ninst, dbs_ctx* const* ctxs, const i
   rc = build_dense_tables(e);
   if (!rc) rc = build_pair_tables(e);
   if (rc) return bail(rc);
+  // the quadrature weights of every (instance,k) group depend on the grid
+  // and k only: computed once here
+  k_weights<<<grid_for(e->total_w, 256, 4096), 256, 0, s>>>(e->nkg, e->d_seg_w, e->d_kg, e->d_inst, e->wt,
+                                                             e->total_w);
   compute_phase_work(e);
   if (hipStreamSynchronize(s) != hipSuccess) return bail(fail(DBS_ERR_HIP, "dbs_engine_create: setup"));
   *out = e;
