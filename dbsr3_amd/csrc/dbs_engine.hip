@@ -677,8 +677,7 @@ __global__ void __launch_bounds__(pair_block_max<W>()) k_pair_panel(
 // coalesced in element order and rotated in registers by four conditional
 // stages (rot bits 1, 2, 4, 8), and the gradient rotated back before its
 // coalesced store, so the LDS holds only the panels (E*ndens doubles).
-// W = 16 prefetches one adjacency entry (the registers of two panels of 32
-// elements would not fit), W = 8 two.
+// Adjacency entries are loaded one iteration ahead of their use.
 template <int W, bool ONE, bool ADJ>
 __global__ void __launch_bounds__(512) k_pair_rot(
     int nwork, const PairWork* __restrict__ work, const KGroupDev* __restrict__ kgs,
@@ -687,7 +686,6 @@ __global__ void __launch_bounds__(512) k_pair_rot(
     const double* __restrict__ F, double* __restrict__ Gout, double* __restrict__ part) {
   extern __shared__ double sm[];
   constexpr int E = 2 * W;
-  constexpr int PF = W >= 16 ? 1 : 2;
   const int X = threadIdx.x;
   const int rot = X & 15;
   for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
@@ -737,48 +735,36 @@ __global__ void __launch_bounds__(512) k_pair_rot(
         }
       }
       __syncthreads();
-      for (int i0 = 0; i0 < n; i0 += PF) {
-        AdjEnt Aq[PF];
-        double cq[PF];
+      // the next entry is loaded while the current one is processed, so its
+      // partner index is known before its LDS reads are issued
+      AdjEnt nxt;
+      if (n > 0) nxt = ent[eb];
+      for (int i = 0; i < n; ++i) {
+        const AdjEnt Aen = nxt;
+        if (i + 1 < n) nxt = ent[eb + 64LL * (i + 1)];
+        if (!ADJ && Aen.slot < 0) continue;
+        const double c = ADJ ? Aen.c : 0.0;
+        const double* pr = B + Aen.P * E;
+        double dot = 0.0;
+        if (!ONE) {
 #pragma unroll
-        for (int q = 0; q < PF; ++q) {
-          const int i = i0 + q;
-          if (i < n) {
-            Aq[q] = ent[eb + 64LL * i];
-            cq[q] = ADJ ? Aq[q].c : 0.0;
-          } else {
-            Aq[q].P = X;
-            Aq[q].slot = -1;
-            cq[q] = 0.0;
+          for (int t = 0; t < E; ++t) {
+            const double p = pr[((t + rot) & (E - 1)) ^ 1];
+            dot += fx[t] * p;
+            if (ADJ) gx[t] += c * p;
+          }
+        } else {
+          const unsigned mg = Aen.slot != -1 ? mL : ~mL;
+#pragma unroll
+          for (int t = 0; t < E; ++t) {
+            const double p = pr[((t + rot) & (E - 1)) ^ 1];
+            if ((mL >> t) & 1u) dot += fx[t] * p;
+            if (ADJ && ((mg >> t) & 1u)) gx[t] += c * p;
           }
         }
-#pragma unroll
-        for (int q = 0; q < PF; ++q) {
-          const AdjEnt Aen = Aq[q];
-          if (!ADJ && Aen.slot < 0) continue;
-          const double c = cq[q];
-          const double* pr = B + Aen.P * E;
-          double dot = 0.0;
-          if (!ONE) {
-#pragma unroll
-            for (int t = 0; t < E; ++t) {
-              const double p = pr[((t + rot) & (E - 1)) ^ 1];
-              dot += fx[t] * p;
-              if (ADJ) gx[t] += c * p;
-            }
-          } else {
-            const unsigned mg = Aen.slot != -1 ? mL : ~mL;
-#pragma unroll
-            for (int t = 0; t < E; ++t) {
-              const double p = pr[((t + rot) & (E - 1)) ^ 1];
-              if ((mL >> t) & 1u) dot += fx[t] * p;
-              if (ADJ && ((mg >> t) & 1u)) gx[t] += c * p;
-            }
-          }
-          if (Aen.slot >= 0) {
-            double* o = part + Gp.off_part + (long long)Wk.pr * Gp.nrows + Aen.slot;
-            *o = first ? dot : *o + dot;
-          }
+        if (Aen.slot >= 0) {
+          double* o = part + Gp.off_part + (long long)Wk.pr * Gp.nrows + Aen.slot;
+          *o = first ? dot : *o + dot;
         }
       }
       if (ADJ && act) {
