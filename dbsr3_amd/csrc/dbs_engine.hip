@@ -671,11 +671,12 @@ __global__ void __launch_bounds__(pair_block_max<W>()) k_pair_panel(
 // [u0 v0 u1 v1 ...] (element e = 2 jl + h), kept in LDS as B[X][e] with
 // rows of E doubles, so the LDS bank of element e does not depend on the
 // row.  Lane l works on the elements in the rotated order
-// e_t = (t + rot) mod E, rot = l mod 16: at every step the 16 lanes that
-// share an LDS cycle touch 16 different banks, wherever their partner rows
+// e_t = (t + rot) mod E, rot = 2 (l mod 8), and reads its partner's element
+// pair (e_t, e_t + 1) with one 16-byte access: the 8 lanes that share an LDS
+// cycle touch 8 different groups of 4 banks, wherever their partner rows
 // are.  Registers fx[t], gx[t] hold element e_t: the panel is loaded
-// coalesced in element order and rotated in registers by four conditional
-// stages (rot bits 1, 2, 4, 8), and the gradient rotated back before its
+// coalesced in element order and rotated in registers by three conditional
+// stages (rot bits 2, 4, 8), and the gradient rotated back before its
 // coalesced store, so the LDS holds only the panels (E*ndens doubles).
 // Adjacency entries are loaded one iteration ahead of their use.
 template <int W, bool ONE, bool ADJ>
@@ -687,7 +688,7 @@ __global__ void __launch_bounds__(512) k_pair_rot(
   extern __shared__ double sm[];
   constexpr int E = 2 * W;
   const int X = threadIdx.x;
-  const int rot = X & 15;
+  const int rot = 2 * (X & 7);  // even: element pairs (u,v) stay 16-byte aligned
   for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
     const PairWork Wk = work[w];
     const KGroupDev Gp = kgs[Wk.g];
@@ -714,7 +715,7 @@ __global__ void __launch_bounds__(512) k_pair_rot(
         for (int e = 0; e < E; ++e) fx[e] = (e >> 1) < nj ? Fg[(2LL * j0 + e) * nd + X] : 0.0;
         // fx[t] = element (t + rot) mod E
 #pragma unroll
-        for (int b = 1; b < 16; b <<= 1) {
+        for (int b = 2; b < 16; b <<= 1) {
           if (rot & b) {
             double tmp[E];
 #pragma unroll
@@ -746,20 +747,28 @@ __global__ void __launch_bounds__(512) k_pair_rot(
         const double c = ADJ ? Aen.c : 0.0;
         const double* pr = B + Aen.P * E;
         double dot = 0.0;
+        // partner pair (u,v) of elements t, t+1 in one 16-byte read; the
+        // pairing swaps u and v
         if (!ONE) {
 #pragma unroll
-          for (int t = 0; t < E; ++t) {
-            const double p = pr[((t + rot) & (E - 1)) ^ 1];
-            dot += fx[t] * p;
-            if (ADJ) gx[t] += c * p;
+          for (int t = 0; t < E; t += 2) {
+            const double2 pp = *reinterpret_cast<const double2*>(pr + ((t + rot) & (E - 1)));
+            dot += fx[t] * pp.y;
+            dot += fx[t + 1] * pp.x;
+            if (ADJ) {
+              gx[t] += c * pp.y;
+              gx[t + 1] += c * pp.x;
+            }
           }
         } else {
           const unsigned mg = Aen.slot != -1 ? mL : ~mL;
 #pragma unroll
-          for (int t = 0; t < E; ++t) {
-            const double p = pr[((t + rot) & (E - 1)) ^ 1];
-            if ((mL >> t) & 1u) dot += fx[t] * p;
-            if (ADJ && ((mg >> t) & 1u)) gx[t] += c * p;
+          for (int t = 0; t < E; t += 2) {
+            const double2 pp = *reinterpret_cast<const double2*>(pr + ((t + rot) & (E - 1)));
+            if ((mL >> t) & 1u) dot += fx[t] * pp.y;
+            if ((mL >> (t + 1)) & 1u) dot += fx[t + 1] * pp.x;
+            if (ADJ && ((mg >> t) & 1u)) gx[t] += c * pp.y;
+            if (ADJ && ((mg >> (t + 1)) & 1u)) gx[t + 1] += c * pp.x;
           }
         }
         if (Aen.slot >= 0) {
@@ -770,7 +779,7 @@ __global__ void __launch_bounds__(512) k_pair_rot(
       if (ADJ && act) {
         // back to element order, then coalesced rows of G
 #pragma unroll
-        for (int b = 1; b < 16; b <<= 1) {
+        for (int b = 2; b < 16; b <<= 1) {
           if (rot & b) {
             double tmp[E];
 #pragma unroll
