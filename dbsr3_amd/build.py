@@ -22,6 +22,9 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # matrix-core accumulators stay in VGPRs: no AGPR copies and result
+    # waits around every v_mfma in the operand loops
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
     "-I" + INCLUDE, "-I" + CSRC, "-Wall", "-Wno-unused-function",
 ]
 

@@ -310,7 +310,9 @@ __global__ void __launch_bounds__(256) k_features_mfma(
     const InstDev* __restrict__ inst, const int* __restrict__ pidx,
     const double* __restrict__ orbv, const double* __restrict__ wt, double* __restrict__ F, int wave_lds) {
   extern __shared__ double sm[];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  // the wave index is uniform across the wave: readfirstlane tells the
+  // compiler, so item fields, loop bounds and branches stay scalar
+  const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), nw = blockDim.x >> 6;
   double* V = sm + (long long)wv * wave_lds;  // this wave's slice
   for (int it = blockIdx.x * nw + wv; it < nitems; it += gridDim.x * nw) {
     const int4 T = items[it];
@@ -359,20 +361,21 @@ __global__ void __launch_bounds__(256) k_features_mfma(
         const int K = pass == 0 ? (kind == DK_TOT ? 2 * ks : ks) : (kind == DK_TOT ? 2 : 1);
         dbl4 acc = {0.0, 0.0, 0.0, 0.0};
         for (int k0 = 0; k0 < K; k0 += 4) {
+          // lanes past K read an in-range entry and contribute zero (no
+          // divergent branch around the matrix-core chain)
           const int kk = k0 + (lane >> 4);
-          double av = 0.0, bv = 0.0;
-          if (kk < K) {
-            const int per = pass == 0 ? ks : 1;
-            int ca, cbp, lp;
-            if (kind == DK_TOT) { ca = cbp = kk >= per ? 1 : 0; lp = kk - ca * per; }
-            else if (kind == DK_PP) { ca = cbp = 0; lp = kk; }
-            else if (kind == DK_QQ) { ca = cbp = 1; lp = kk; }
-            else { ca = 0; cbp = 1; lp = kk; }
-            if (pass == 1) lp = ks;  // the outer point
-            av = W[lp] * V[vsw(ca, lp, ra, npt, NO)];
-            bv = V[vsw(cbp, lp, cb, npt, NO)];
-          }
-          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+          const bool ok = kk < K;
+          const int kc = ok ? kk : 0;
+          const int per = pass == 0 ? ks : 1;
+          int ca, cbp, lp;
+          if (kind == DK_TOT) { ca = cbp = kc >= per ? 1 : 0; lp = kc - ca * per; }
+          else if (kind == DK_PP) { ca = cbp = 0; lp = kc; }
+          else if (kind == DK_QQ) { ca = cbp = 1; lp = kc; }
+          else { ca = 0; cbp = 1; lp = kc; }
+          if (pass == 1) lp = ks;  // the outer point
+          const double a = W[lp] * V[vsw(ca, lp, ra, npt, NO)];
+          const double b = V[vsw(cbp, lp, cb, npt, NO)];
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(ok ? a : 0.0, ok ? b : 0.0, acc, 0, 0, 0);
         }
         // lane holds column cb and rows ta*16 + 4j + lane/16
         const long long x = pass == 0 ? xrow : xrow - 1;
@@ -458,7 +461,9 @@ __global__ void __launch_bounds__(256) k_contract_dense(
   // tile = (group, Ltile, Rtile, 0); pairrow[pairoff[g] + l*ndens + r] = row or -1
   const int waves = blockDim.x >> 6;
   const int lane = threadIdx.x & 63;
-  for (int t = blockIdx.x * waves + (threadIdx.x >> 6); t < ntiles; t += gridDim.x * waves) {
+  // wave index made scalar (uniform across the wave)
+  for (int t = blockIdx.x * waves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); t < ntiles;
+       t += gridDim.x * waves) {
     const int4 T = tiles[t];
     const KGroupDev G = kgs[T.x];
     const InstDev I = inst[G.inst];
@@ -1167,7 +1172,8 @@ __global__ void __launch_bounds__(256) k_adj_orb_mfma(
     const double* __restrict__ wt, const double* __restrict__ Gb, double* __restrict__ gorb, long long total,
     const int* __restrict__ accf) {
   extern __shared__ double sm[];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  // wave index uniform across the wave (scalar for the compiler)
+  const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), nw = blockDim.x >> 6;
   for (long long item = blockIdx.x; item < total; item += gridDim.x) {
     const int s = find_seg(seg, ninst, item);
     if (accf[s]) continue;  // instance done by k_adj_orb_acc
