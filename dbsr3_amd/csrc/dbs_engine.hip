@@ -557,7 +557,7 @@ template <int W, bool LDS, bool ADJ>
 __global__ void __launch_bounds__(pair_block_max<W>()) k_pair_panel(
     int nwork, const PairWork* __restrict__ work, const KGroupDev* __restrict__ kgs,
     const InstDev* __restrict__ inst, const AdjChunk* __restrict__ chunks,
-    const AdjEnt* __restrict__ ent, const int* __restrict__ alen,
+    const AdjEnt* __restrict__ ent, const int2* __restrict__ alen,
     const double* __restrict__ F, double* __restrict__ Gout, double* __restrict__ part, int onesided) {
   extern __shared__ double sp[];
   constexpr int LD = 2 * W + 2;  // LDS row: u(W) v(W) + pad (16-byte aligned rows)
@@ -595,7 +595,8 @@ __global__ void __launch_bounds__(pair_block_max<W>()) k_pair_panel(
         }
         const AdjChunk C = chunks[Gp.chunk0 + (X >> 6)];
         const long long eb = C.base + (X & 63);
-        const int n = alen[Gp.dens0 + X];
+        // the forward stops after the entries where X is the row's left one
+        const int n = ADJ ? alen[Gp.dens0 + X].x : alen[Gp.dens0 + X].y;
         // entries are fetched PF at a time so their loads overlap; the tail
         // of the last group is padded with inert entries (own row, c = 0,
         // no row value)
@@ -688,7 +689,7 @@ template <int W, bool ONE, bool ADJ>
 __global__ void __launch_bounds__(512) k_pair_rot(
     int nwork, const PairWork* __restrict__ work, const KGroupDev* __restrict__ kgs,
     const InstDev* __restrict__ inst, const AdjChunk* __restrict__ chunks,
-    const AdjEnt* __restrict__ ent, const int* __restrict__ alen,
+    const AdjEnt* __restrict__ ent, const int2* __restrict__ alen,
     const double* __restrict__ F, double* __restrict__ Gout, double* __restrict__ part) {
   extern __shared__ double sm[];
   constexpr int E = 2 * W;
@@ -706,7 +707,8 @@ __global__ void __launch_bounds__(512) k_pair_rot(
     if (act) {
       const AdjChunk C = chunks[Gp.chunk0 + (X >> 6)];
       eb = C.base + (X & 63);
-      n = alen[Gp.dens0 + X];
+      // the forward stops after the entries where X is the row's left one
+      n = ADJ ? alen[Gp.dens0 + X].x : alen[Gp.dens0 + X].y;
     }
     bool first = true;
     for (int j0 = Wk.j0; j0 < Wk.j1; j0 += W) {
@@ -1595,7 +1597,7 @@ struct dbs_engine {
   AdjEnt* d_ent = nullptr;
   long long* d_ecidx = nullptr;
   long long nent = 0;
-  int* d_alen = nullptr;  // adjacency length per density
+  int2* d_alen = nullptr;  // per density: adjacency length, entries as the row's left density
   // work lists [adjoint][variant]: 0 partner panels read from global
   // memory, 1 LDS panels, 2 LDS panels with the conflict-free rotation
   PairWork* d_work[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
@@ -1898,8 +1900,12 @@ static int build_pair_tables(dbs_engine* e) {
   for (int a = 0; a < 2 && !rc; ++a)
     for (int l = 0; l < 3 && !rc; ++l) rc = upload(e->stream, &e->d_work[a][l], work[a][l]);
   {
-    std::vector<int> alen(std::max(e->ndens, 1), 0);
-    for (int d = 0; d < e->ndens; ++d) alen[d] = (int)e->h_adj[d].size();
+    std::vector<int2> alen(std::max(e->ndens, 1), make_int2(0, 0));
+    for (int d = 0; d < e->ndens; ++d) {
+      int nl = 0;
+      for (const int2& a : e->h_adj[d]) nl += a.x >= 0 ? 1 : 0;
+      alen[d] = make_int2((int)e->h_adj[d].size(), nl);
+    }
     dfree(&e->d_alen);
     if (!rc) rc = upload(e->stream, &e->d_alen, alen);
   }
