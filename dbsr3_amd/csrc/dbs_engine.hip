@@ -497,14 +497,16 @@ __global__ void __launch_bounds__(256) k_contract_dense(
 
 // ---------------------------------------------------------------------------
 // energy: E_inst = sum_rows coef*value.  Blocks reduce fixed chunks of one
-// instance's rows (tree in a fixed shape); one thread per instance then adds
-// the chunk partials in order.
+// instance's rows (tree in a fixed shape); the last block then adds each
+// instance's chunk partials in order.
 // ---------------------------------------------------------------------------
 #define DBS_ECHUNK 2048
-__global__ void k_energy_partial(int nchunks, const int2* __restrict__ chunks,
-                                 const long long* __restrict__ roff, const double* __restrict__ coef,
-                                 const double* __restrict__ vals, double* __restrict__ part) {
+__global__ void k_energy(int nchunks, const int2* __restrict__ chunks, const long long* __restrict__ roff,
+                         const double* __restrict__ coef, const double* __restrict__ vals,
+                         double* __restrict__ part, int ninst, const int* __restrict__ choff,
+                         double* __restrict__ energy, unsigned* __restrict__ done) {
   __shared__ double sh[256];
+  __shared__ bool last;
   for (int ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
     const int2 C = chunks[ch];  // (instance, chunk index within it)
     const long long r0 = roff[C.x] + (long long)C.y * DBS_ECHUNK;
@@ -520,15 +522,20 @@ __global__ void k_energy_partial(int nchunks, const int2* __restrict__ chunks,
     if (threadIdx.x == 0) part[ch] = sh[0];
     __syncthreads();
   }
-}
-
-__global__ void k_energy_final(int ninst, const int* __restrict__ choff, const double* __restrict__ part,
-                               double* __restrict__ energy) {
-  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < ninst; s += gridDim.x * blockDim.x) {
+  // the last workgroup to finish adds each instance's chunk partials in
+  // order (integer counter; it resets the counter for the next launch)
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) last = atomicAdd(done, 1u) == gridDim.x - 1;
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  for (int s = threadIdx.x; s < ninst; s += blockDim.x) {
     double acc = 0.0;
     for (int c = choff[s]; c < choff[s + 1]; ++c) acc += part[c];
     energy[s] = acc;
   }
+  if (threadIdx.x == 0) *done = 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -901,12 +908,25 @@ __global__ void k_adj_orb_points(int ninst, const long long* __restrict__ seg,
 // Orbital values at the points of interval iv, one workgroup per
 // (instance, interval, component): the kx B-spline values at the np points
 // and the kx coefficients of every orbital are staged in LDS, then
-// f(o,p) = sum_j B_j(p) c_o(iv+j) with j ascending.
+// f(o,p) = sum_j B_j(p) c_o(iv+j) with j ascending.  Workgroups past the
+// first nbo gather the adjacency coefficients (k_adj_cgather's work) in the
+// same launch.
 __global__ void __launch_bounds__(256) k_orb_values(int ninst, const long long* __restrict__ seg,
                                                    const InstDev* __restrict__ inst, const double* __restrict__ pq,
-                                                   double* __restrict__ orbv, long long total) {
+                                                   double* __restrict__ orbv, long long total, int nbo,
+                                                   long long nent, const long long* __restrict__ cidx,
+                                                   const double* __restrict__ coef, AdjEnt* __restrict__ ent) {
   extern __shared__ double sm[];
-  for (long long item = blockIdx.x; item < total; item += gridDim.x) {
+  if ((int)blockIdx.x >= nbo) {
+    // workgroups past the first nbo gather the adjacency coefficients
+    const long long nb = gridDim.x - nbo;
+    for (long long q = (long long)(blockIdx.x - nbo) * blockDim.x + threadIdx.x; q < nent; q += nb * blockDim.x) {
+      const long long c = cidx[q];
+      ent[q].c = c >= 0 ? coef[c] : 0.0;
+    }
+    return;
+  }
+  for (long long item = blockIdx.x; item < total; item += nbo) {
     const int s = find_seg(seg, ninst, item);
     const InstDev I = inst[s];
     const int r = (int)(item - seg[s]);
@@ -1567,6 +1587,7 @@ struct dbs_engine {
   int2* d_echunks = nullptr;
   int* d_echoff = nullptr;
   double* d_epart = nullptr;
+  unsigned* d_edone = nullptr;  // workgroups of k_energy done (zero between launches)
   int nechunks = 0;
   // LDS sizes (doubles) of the staged kernels; 0 = not staged
   int feat_stage = 0;
@@ -1993,16 +2014,21 @@ static int run_forward(dbs_engine* e, int with_adj) {
   hipStream_t s = e->stream;
   mark(e, "start");
   const int cap = e->tune[3];
-  if (e->ov_shm <= 64 * 1024)
-    k_orb_values<<<grid_for(e->n_ivc, 1, cap * 4), 256, e->ov_shm, s>>>(e->ninst, e->d_seg_ivc, e->d_inst, e->pq,
-                                                                       e->orbv, e->n_ivc);
-  else
+  const long long ng = with_adj ? e->nent : 0;
+  if (e->ov_shm <= 64 * 1024) {
+    // the coefficient gather rides in the same launch (extra workgroups)
+    const int nbo = (int)grid_for(e->n_ivc, 1, cap * 4);
+    const int nbg = ng > 0 ? (int)grid_for(ng, 256, cap * 4) : 0;
+    k_orb_values<<<nbo + nbg, 256, e->ov_shm, s>>>(e->ninst, e->d_seg_ivc, e->d_inst, e->pq, e->orbv, e->n_ivc,
+                                                   nbo, ng, e->d_ecidx, e->coef, e->d_ent);
+    mark(e, "orb_points");
+  } else {
     k_orb_points<<<grid_for(e->total_orbv, 256, cap * 4), 256, 0, s>>>(
         e->ninst, e->d_seg_orb, e->d_inst, e->pq, e->orbv, e->total_orbv);
-  mark(e, "orb_points");
-  if (with_adj && e->nent > 0)
-    k_adj_cgather<<<grid_for(e->nent, 256, cap * 4), 256, 0, s>>>(e->nent, e->d_ecidx, e->coef, e->d_ent);
-  mark(e, "coef_gather");
+    mark(e, "orb_points");
+    if (ng > 0) k_adj_cgather<<<grid_for(ng, 256, cap * 4), 256, 0, s>>>(ng, e->d_ecidx, e->coef, e->d_ent);
+    mark(e, "coef_gather");
+  }
   if (e->featm_shm > 0)
     k_features_mfma<<<grid_for(e->nfitems, 4, cap * 4), 256, e->featm_shm, s>>>(
         e->nfitems, e->d_fitems, e->d_kg, e->d_inst, e->d_pidx, e->orbv, e->wt, e->F, e->featm_wave);
@@ -2029,10 +2055,9 @@ static int run_forward(dbs_engine* e, int with_adj) {
         e->d_row_out, e->onesided, e->vals);
     mark(e, "contract_dense");
   }
-  if (e->nechunks > 0)
-    k_energy_partial<<<grid_for(e->nechunks, 1, cap), 256, 0, s>>>(e->nechunks, e->d_echunks, e->d_roff,
-                                                                   e->coef, e->vals, e->d_epart);
-  k_energy_final<<<grid_for(e->ninst, 64, 64), 64, 0, s>>>(e->ninst, e->d_echoff, e->d_epart, e->energy);
+  k_energy<<<grid_for(e->nechunks, 1, cap), 256, 0, s>>>(e->nechunks, e->d_echunks, e->d_roff, e->coef, e->vals,
+                                                         e->d_epart, e->ninst, e->d_echoff, e->energy,
+                                                         e->d_edone);
   mark(e, "energy");
   DBS_HIP(hipGetLastError());
   return DBS_OK;
@@ -2415,6 +2440,9 @@ int dbs_engine_create(dbs_engine** out, int ninst, dbs_ctx* const* ctxs, const i
   rc |= upload(s, &e->d_echunks, echunks);
   rc |= upload(s, &e->d_echoff, echoff);
   rc |= dalloc(&e->d_epart, std::max(e->nechunks, 1));
+  if (hipMalloc(&e->d_edone, sizeof(unsigned)) != hipSuccess ||
+      hipMemsetAsync(e->d_edone, 0, sizeof(unsigned), s) != hipSuccess)
+    rc |= fail(DBS_ERR_HIP, "dbs_engine_create: energy counter");
   // LDS plans: staged features when the interval's orbital values fit,
   // the LDS form of the adjoint orbital phase when its matrices fit
   {
@@ -2620,7 +2648,7 @@ int dbs_engine_destroy(dbs_engine* e) {
                   e->d_seg_w, e->d_roff, e->d_items, e->d_scan_items, e->d_oadjoff, e->d_oadj,
                   e->d_tiles, e->d_pairrow, e->d_pairoff, e->d_nextrow, e->plain_d, e->plain_c,
                   e->d_echunks, e->d_echoff, e->d_epart, e->d_seg_adj, e->d_chunks, e->d_ent,
-                  e->d_ecidx, e->d_red, e->d_part, e->d_alen, e->d_fitems, e->d_pidx, e->d_dpk, e->d_accf, e->d_seg_ivc, e->d_ppoff, e->d_ppart,
+                  e->d_ecidx, e->d_red, e->d_part, e->d_alen, e->d_fitems, e->d_pidx, e->d_dpk, e->d_accf, e->d_seg_ivc, e->d_ppoff, e->d_ppart, e->d_edone,
                   e->d_work[0][0], e->d_work[0][1], e->d_work[0][2],
                   e->d_work[1][0], e->d_work[1][1], e->d_work[1][2]};
   for (void* p : bufs)
