@@ -785,7 +785,7 @@ int dbs_rk_evaluate_dev(dbs_ctx* ctx, int itype, int k, int nc, const double* c,
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) rc = fail(DBS_ERR_HIP, std::string("Rk_evaluate: ") + hipGetErrorString(e));
   } while (0);
-  hipFree(d_tpos); hipFree(d_toff); hipFree(d_tent); hipFree(d_c);
+  (void)hipFree(d_tpos); (void)hipFree(d_toff); (void)hipFree(d_tent); (void)hipFree(d_c);
   dbs_engine_destroy(eng);
   return rc;
 }
@@ -879,7 +879,8 @@ int dbs_rdata_34_dev(dbs_ctx* ctx, int itype, int atype, int k, int nc, const do
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) rc = fail(DBS_ERR_HIP, std::string("R_data: ") + hipGetErrorString(e));
   } while (0);
-  hipFree(d_goff); hipFree(d_j1); hipFree(d_j2); hipFree(d_c); hipFree(d_gi); hipFree(d_xx); hipFree(d_a);
+  (void)hipFree(d_goff); (void)hipFree(d_j1); (void)hipFree(d_j2); (void)hipFree(d_c); (void)hipFree(d_gi);
+  (void)hipFree(d_xx); (void)hipFree(d_a);
   return rc;
 }
 

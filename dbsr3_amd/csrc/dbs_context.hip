@@ -237,14 +237,15 @@ int dbs_ctx_create(dbs_ctx** out, int ns, int ks, int ksp, int ksq, const double
 
 int dbs_ctx_destroy(dbs_ctx* c) {
   if (!c) return DBS_OK;
-  hipSetDevice(c->device);
-  if (c->stream) hipStreamSynchronize(c->stream);
+  // teardown: failures here leave nothing to recover
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
   double* bufs[] = {c->t, c->tp, c->tq, c->gx, c->gw, c->gr, c->grw, c->grm, c->pbsp, c->qbsp,
                     c->xin, c->win, c->bpin, c->bqin, c->mom, c->rkd, c->scratch, c->stage};
   for (double* p : bufs)
-    if (p) hipFree(p);
-  for (auto& kv : c->rkb_cache) hipFree(kv.second);
-  if (c->own_stream && c->stream) hipStreamDestroy(c->stream);
+    if (p) (void)hipFree(p);
+  for (auto& kv : c->rkb_cache) (void)hipFree(kv.second);
+  if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return DBS_OK;
 }

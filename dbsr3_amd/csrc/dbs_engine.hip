@@ -1681,7 +1681,7 @@ int upload(hipStream_t s, T** dst, const std::vector<T>& v) {
 
 template <class T>
 void dfree(T** p) {
-  if (*p) hipFree(*p);
+  if (*p) (void)hipFree(*p);
   *p = nullptr;
 }
 
@@ -1741,16 +1741,16 @@ int build_dense_tables(dbs_engine* e) {
 
 }  // namespace
 
+// dynamic LDS above 64 KB for every variant of a kernel; the first failure
 template <int W>
-static void rot_lds_attr(size_t bytes) {
-  hipFuncSetAttribute((const void*)k_pair_rot<W, false, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                      (int)bytes);
-  hipFuncSetAttribute((const void*)k_pair_rot<W, false, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                      (int)bytes);
-  hipFuncSetAttribute((const void*)k_pair_rot<W, true, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                      (int)bytes);
-  hipFuncSetAttribute((const void*)k_pair_rot<W, true, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                      (int)bytes);
+static hipError_t rot_lds_attr(size_t bytes) {
+  const void* fs[] = {(const void*)k_pair_rot<W, false, true>, (const void*)k_pair_rot<W, false, false>,
+                      (const void*)k_pair_rot<W, true, true>, (const void*)k_pair_rot<W, true, false>};
+  for (const void* f : fs) {
+    const hipError_t err = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (err != hipSuccess) return err;
+  }
+  return hipSuccess;
 }
 
 static size_t rot_lds_bytes(int W, int ndens) {
@@ -1758,11 +1758,13 @@ static size_t rot_lds_bytes(int W, int ndens) {
 }
 
 template <int W>
-static void pair_lds_attr(size_t bytes) {
-  hipFuncSetAttribute((const void*)k_pair_panel<W, true, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                      (int)bytes);
-  hipFuncSetAttribute((const void*)k_pair_panel<W, true, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                      (int)bytes);
+static hipError_t pair_lds_attr(size_t bytes) {
+  const void* fs[] = {(const void*)k_pair_panel<W, true, true>, (const void*)k_pair_panel<W, true, false>};
+  for (const void* f : fs) {
+    const hipError_t err = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (err != hipSuccess) return err;
+  }
+  return hipSuccess;
 }
 
 static size_t pair_lds_bytes(int W, int ndens) { return (size_t)ndens * (2 * W + 2) * sizeof(double); }
@@ -1789,9 +1791,9 @@ static int build_pair_tables(dbs_engine* e) {
   e->pair_w = W;
   e->pair_shm = pair_lds_bytes(W, std::max(maxnd, 1));
   if (e->pair_shm > 64 * 1024) {
-    if (W == 4) pair_lds_attr<4>(e->pair_shm);
-    else if (W == 8) pair_lds_attr<8>(e->pair_shm);
-    else pair_lds_attr<16>(e->pair_shm);
+    const hipError_t err = W == 4 ? pair_lds_attr<4>(e->pair_shm)
+                                  : (W == 8 ? pair_lds_attr<8>(e->pair_shm) : pair_lds_attr<16>(e->pair_shm));
+    if (err != hipSuccess) return fail(DBS_ERR_HIP, std::string("pair tables: LDS size: ") + hipGetErrorString(err));
   }
   // adjacency in 64-wide column blocks
   std::vector<AdjChunk> chunks;
@@ -1850,9 +1852,9 @@ static int build_pair_tables(dbs_engine* e) {
   }
   e->rot_block = std::max(64, (rot_nd + 63) / 64 * 64);
   e->rot_shm = rot_lds_bytes(W, std::max(rot_nd, 1));
-  if (e->rot_shm > 64 * 1024) {
-    if (W == 8) rot_lds_attr<8>(e->rot_shm);
-    else if (W == 16) rot_lds_attr<16>(e->rot_shm);
+  if (e->rot_shm > 64 * 1024 && W >= 8) {
+    const hipError_t err = W == 8 ? rot_lds_attr<8>(e->rot_shm) : rot_lds_attr<16>(e->rot_shm);
+    if (err != hipSuccess) return fail(DBS_ERR_HIP, std::string("pair tables: LDS size: ") + hipGetErrorString(err));
   }
   for (int g = 0; g < e->nkg; ++g) {
     KGroupDev& G = e->h_kg[g];
@@ -1938,12 +1940,12 @@ static int build_pair_tables(dbs_engine* e) {
 static void mark(dbs_engine* e, const char* name) {
   if (!e->profiling) return;
   if (e->prof_pos >= e->prof_ev.size()) {
-    hipEvent_t ev;
-    hipEventCreate(&ev);
+    hipEvent_t ev = nullptr;
+    if (hipEventCreate(&ev) != hipSuccess) return;  // profiling only: the phase is skipped
     e->prof_ev.push_back(ev);
     e->prof_names.emplace_back();
   }
-  hipEventRecord(e->prof_ev[e->prof_pos], e->stream);
+  (void)hipEventRecord(e->prof_ev[e->prof_pos], e->stream);  // profiling only
   e->prof_names[e->prof_pos] = name;
   e->prof_pos++;
 }
@@ -2191,7 +2193,7 @@ int dbs_engine_create(dbs_engine** out, int ninst, dbs_ctx* const* ctxs, const i
       return bail(fail(DBS_ERR_ARG, "dbs_engine_create: contexts must exist and share one device"));
     if (norb[i] < 1 || nrows[i] < 0) return bail(fail(DBS_ERR_ARG, "dbs_engine_create: bad sizes"));
   }
-  hipSetDevice(e->device);
+  if (hipSetDevice(e->device) != hipSuccess) return bail(fail(DBS_ERR_HIP, "dbs_engine_create: device"));
   if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess)
     return bail(fail(DBS_ERR_HIP, "dbs_engine_create: stream"));
   e->ctx.assign(ctxs, ctxs + ninst);
@@ -2468,9 +2470,11 @@ int dbs_engine_create(dbs_engine** out, int ninst, dbs_ctx* const* ctxs, const i
     wneed2 = (wneed2 + 1) & ~(size_t)1;  // 16-byte aligned slices
     e->featm_wave = (int)wneed2;
     e->featm_shm = fits ? 4 * wneed2 * sizeof(double) : 0;
-    if (e->featm_shm > 64 * 1024)
-      hipFuncSetAttribute((const void*)k_features_mfma, hipFuncAttributeMaxDynamicSharedMemorySize,
-                          (int)e->featm_shm);
+    // a kernel whose LDS size cannot be granted falls back to the next form
+    if (e->featm_shm > 64 * 1024 &&
+        hipFuncSetAttribute((const void*)k_features_mfma, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)e->featm_shm) != hipSuccess)
+      e->featm_shm = 0;
     if (e->featm_shm > 0) {
       std::vector<int4> fit;
       std::vector<int> pidx;
@@ -2530,9 +2534,10 @@ int dbs_engine_create(dbs_engine** out, int ninst, dbs_ctx* const* ctxs, const i
       if (hipDeviceGetAttribute(&lmx, hipDeviceAttributeMaxSharedMemoryPerBlock, e->device) != hipSuccess)
         lmx = 64 * 1024;
       e->adjm_shm = ok && need * sizeof(double) <= (size_t)lmx ? need * sizeof(double) : 0;
-      if (e->adjm_shm > 64 * 1024)
-        hipFuncSetAttribute((const void*)k_adj_orb_mfma, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)e->adjm_shm);
+      if (e->adjm_shm > 64 * 1024 &&
+          hipFuncSetAttribute((const void*)k_adj_orb_mfma, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)e->adjm_shm) != hipSuccess)
+        e->adjm_shm = 0;
       std::vector<int> pk(std::max(e->ndens, 1), 0);
       for (int d = 0; d < e->ndens; ++d) pk[d] = (e->h_dens[d].a & 255) | ((e->h_dens[d].c & 255) << 8);
       rc |= upload(s, &e->d_dpk, pk);
@@ -2570,9 +2575,11 @@ int dbs_engine_create(dbs_engine** out, int ninst, dbs_ctx* const* ctxs, const i
       e->adjacc_shm = e->adjacc_u ? aneed2 * sizeof(double) : 0;
       rc |= upload(s, &e->d_accf, accf);
       if (e->adjacc_shm > 64 * 1024) {
-        hipFuncSetAttribute((const void*)k_adj_orb_acc<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)e->adjacc_shm);
-        hipFuncSetAttribute((const void*)k_adj_orb_acc<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)e->adjacc_shm);
-        hipFuncSetAttribute((const void*)k_adj_orb_acc<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)e->adjacc_shm);
+        const void* fs[] = {(const void*)k_adj_orb_acc<1>, (const void*)k_adj_orb_acc<2>,
+                            (const void*)k_adj_orb_acc<4>};
+        for (const void* f : fs)
+          if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)e->adjacc_shm) != hipSuccess)
+            rc |= fail(DBS_ERR_HIP, "dbs_engine_create: LDS size of the register-summed adjoint");
       }
     }
     std::vector<long long> seg_adj(ninst);
@@ -2638,11 +2645,12 @@ int dbs_engine_create(dbs_engine** out, int ninst, dbs_ctx* const* ctxs, const i
 
 int dbs_engine_destroy(dbs_engine* e) {
   if (!e) return DBS_OK;
-  hipSetDevice(e->device);
-  if (e->stream) hipStreamSynchronize(e->stream);
-  if (e->gexec) hipGraphExecDestroy(e->gexec);
-  if (e->graph) hipGraphDestroy(e->graph);
-  for (hipEvent_t ev : e->prof_ev) hipEventDestroy(ev);
+  // teardown: failures here leave nothing to recover
+  (void)hipSetDevice(e->device);
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+  if (e->graph) (void)hipGraphDestroy(e->graph);
+  for (hipEvent_t ev : e->prof_ev) (void)hipEventDestroy(ev);
   void* bufs[] = {e->pq, e->coef, e->vals, e->grad, e->energy, e->orbv, e->gorb, e->wt, e->F, e->G,
                   e->d_inst, e->d_kg, e->d_dens, e->d_row_out, e->d_seg_orb, e->d_seg_coef,
                   e->d_seg_w, e->d_roff, e->d_items, e->d_scan_items, e->d_oadjoff, e->d_oadj,
@@ -2652,8 +2660,8 @@ int dbs_engine_destroy(dbs_engine* e) {
                   e->d_work[0][0], e->d_work[0][1], e->d_work[0][2],
                   e->d_work[1][0], e->d_work[1][1], e->d_work[1][2]};
   for (void* p : bufs)
-    if (p) hipFree(p);
-  if (e->stream) hipStreamDestroy(e->stream);
+    if (p) (void)hipFree(p);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
   return DBS_OK;
 }
@@ -2688,14 +2696,14 @@ int dbs_engine_sync(dbs_engine* e) {
 int dbs_engine_graph_capture(dbs_engine* e, int with_adjoint) {
   DBS_CHECK(e, DBS_ERR_ARG, "dbs_engine_graph_capture: null engine");
   DBS_HIP(hipStreamSynchronize(e->stream));
-  if (e->gexec) { hipGraphExecDestroy(e->gexec); e->gexec = nullptr; }
-  if (e->graph) { hipGraphDestroy(e->graph); e->graph = nullptr; }
+  if (e->gexec) { (void)hipGraphExecDestroy(e->gexec); e->gexec = nullptr; }
+  if (e->graph) { (void)hipGraphDestroy(e->graph); e->graph = nullptr; }
   DBS_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
   int rc = run_all(e, with_adjoint ? 1 : 0);
   hipGraph_t g = nullptr;
   hipError_t ee = hipStreamEndCapture(e->stream, &g);
   if (rc) {
-    if (g) hipGraphDestroy(g);
+    if (g) (void)hipGraphDestroy(g);
     return rc;
   }
   if (ee != hipSuccess) return fail(DBS_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ee));
@@ -2738,8 +2746,8 @@ int dbs_engine_set_tuning(dbs_engine* e, int n, const int* p) {
     DBS_HIP(hipStreamSynchronize(e->stream));
   }
   // a captured graph holds the old launch shapes
-  if (e->gexec) { hipGraphExecDestroy(e->gexec); e->gexec = nullptr; }
-  if (e->graph) { hipGraphDestroy(e->graph); e->graph = nullptr; }
+  if (e->gexec) { (void)hipGraphExecDestroy(e->gexec); e->gexec = nullptr; }
+  if (e->graph) { (void)hipGraphDestroy(e->graph); e->graph = nullptr; }
   return DBS_OK;
 }
 
@@ -2793,13 +2801,13 @@ int dbs_engine_autotune(dbs_engine* e, int with_adjoint, int reps) {
             if (rc) continue;
             std::vector<float> ms;
             for (int r = 0; r < reps && !rc; ++r) {
-              hipEventRecord(a, e->stream);
+              if (hipEventRecord(a, e->stream) != hipSuccess) { rc = DBS_ERR_HIP; break; }
               rc = run_all(e, with_adjoint ? 1 : 0);
-              hipEventRecord(b, e->stream);
-              hipEventSynchronize(b);
               float x = 0;
-              hipEventElapsedTime(&x, a, b);
-              ms.push_back(x);
+              if (!rc && (hipEventRecord(b, e->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
+                          hipEventElapsedTime(&x, a, b) != hipSuccess))
+                rc = DBS_ERR_HIP;
+              if (!rc) ms.push_back(x);
             }
             if (rc || ms.empty()) continue;
             std::sort(ms.begin(), ms.end());
@@ -2809,8 +2817,8 @@ int dbs_engine_autotune(dbs_engine* e, int with_adjoint, int reps) {
               std::memcpy(best, t, sizeof(best));
             }
           }
-  hipEventDestroy(a);
-  hipEventDestroy(b);
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
   return dbs_engine_set_tuning(e, 6, best);
 }
 
@@ -2879,12 +2887,12 @@ static int plain_group(dbs_engine* e, int g, int type, int accumulate) {
   if (rc) return rc;
   size_t need_d = spec.size() * (size_t)ns * nb, need_c = lslots.size() * (size_t)ns * nb;
   if (need_d > e->plain_d_n) {
-    if (e->plain_d) hipFree(e->plain_d);
+    if (e->plain_d) (void)hipFree(e->plain_d);
     DBS_HIP(hipMalloc(&e->plain_d, need_d * sizeof(double)));
     e->plain_d_n = need_d;
   }
   if (need_c > e->plain_c_n) {
-    if (e->plain_c) hipFree(e->plain_c);
+    if (e->plain_c) (void)hipFree(e->plain_c);
     DBS_HIP(hipMalloc(&e->plain_c, need_c * sizeof(double)));
     e->plain_c_n = need_c;
   }
@@ -2910,10 +2918,10 @@ static int plain_group(dbs_engine* e, int g, int type, int accumulate) {
     if (ee == hipSuccess) ee = hipStreamSynchronize(s);
     if (ee != hipSuccess) rc = fail(DBS_ERR_HIP, std::string("plain path: ") + hipGetErrorString(ee));
   }
-  hipFree(d_spec);
-  hipFree(d_ls);
-  hipFree(d_lr);
-  hipFree(d_out);
+  (void)hipFree(d_spec);
+  (void)hipFree(d_ls);
+  (void)hipFree(d_lr);
+  (void)hipFree(d_out);
   return rc;
 }
 

@@ -381,7 +381,7 @@ int build_rkb(dbs_ctx* c, int type, int k, double** out) {
   }
   if (rc == DBS_OK) rc = launch_diag(c, type, k, c->rkd);
   if (rc != DBS_OK) {
-    hipFree(rkb);
+    (void)hipFree(rkb);
     return rc;
   }
   const int B = 256;
@@ -396,7 +396,7 @@ int build_rkb(dbs_ctx* c, int type, int k, double** out) {
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
-    hipFree(rkb);
+    (void)hipFree(rkb);
     return fail(DBS_ERR_HIP, std::string("mrk: ") + hipGetErrorString(e));
   }
   c->rkb_cache[key] = rkb;
