@@ -13,6 +13,19 @@ forward only) on the same rows, the per-phase roofline of the dominant
 kernel (device time from HIP events on the engine's stream), and the CPU
 restatement of the reference algorithm timed on a bounded sample on this
 host (one multipole's rows).
+
+`--config N` measures one of the other integral-set configurations instead
+(same JSON line format, not the headline):
+  1  all R^k(a,b;c,d), a<=c, b<=d, k=0..4 of 12 orbitals (30 420 rows) and the
+     S^k ppqq/pqqp sets, on the Z=26 recipe grid (ns=208) and the Fe knots
+     (ns=113), forward only;
+  2  MAT3 channel blocks: 2e5 seeded entries (40 channels, 20 bound orbitals,
+     ns=150, k=0..6, itype 3 direct / 4 exchange, atype 0..3) accumulated by
+     R_data into 820 symmetric 300x300 blocks;
+  3  256 uneven instances (ns 60..400, ks 7..9, kmax 2..10, 4..8 orbitals,
+     all pairs) forward+adjoint in one hipGraph, autotuned;
+  5  SCF-style: ns=600, 60 orbitals, k=0..12, 1e6 rows, 20 iterations of
+     new orbitals + one forward+adjoint hipGraph replay.
 """
 from __future__ import annotations
 
@@ -120,6 +133,321 @@ def cpu_baseline(t, ns, ks, ksp, ksq, pq, rows, budget_s):
     return n, t1 - t0, order[:n], vals
 
 
+def roofline_of(prof):
+    """the dominant phase of an engine profile and its roofline (bytes and
+    flops from the engine's algorithmic counts, time from HIP events)"""
+    dom = max((k for k in prof if k != "start"), key=lambda k: prof[k][0])
+    dms, dfl, dby = prof[dom]
+    intensity = dfl / dby if dby > 0 else float("inf")
+    if intensity > PEAK_FP64_TFS * 1e12 / (PEAK_HBM_GBS * 1e9):
+        roof = {"bound": "mfma", "achieved": dfl / (dms * 1e-3) / 1e12, "peak": PEAK_FP64_TFS,
+                "unit": "TFLOP/s"}
+    else:
+        roof = {"bound": "hbm", "achieved": dby / (dms * 1e-3) / 1e9, "peak": PEAK_HBM_GBS,
+                "unit": "GB/s"}
+    roof["frac"] = roof["achieved"] / roof["peak"]
+    roof["phase"] = dom
+    roof["kernel_ms"] = dms
+    return roof, dom
+
+
+def time_graph(eng, with_adjoint, steps, warmup, before=None):
+    """ms per replay of the engine's captured step (before(i) runs on the
+    host ahead of replay i, e.g. new orbitals)"""
+    import torch
+    eng.graph_capture(with_adjoint=with_adjoint)
+    for i in range(warmup):
+        if before:
+            before(i)
+        eng.graph_launch()
+    eng.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        if before:
+            before(i)
+        eng.graph_launch()
+    eng.sync()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps
+
+
+def seeded_orbitals(t, ns, ks, ksp, ksq, norb, seed):
+    """P,Q ~ N(0,1) on 2..nsp-1 / 2..nsq-1 (first and last coefficients zero)"""
+    rng = np.random.default_rng(seed)
+    nv = ns - ks + 1
+    nsp, nsq = nv + ksp - 1, nv + ksq - 1
+    pq = np.zeros((ns, 2, norb), order="F")
+    for o in range(norb):
+        pq[1:nsp - 1, 0, o] = rng.standard_normal(nsp - 2)
+        pq[1:nsq - 1, 1, o] = rng.standard_normal(nsq - 2)
+    return pq
+
+
+def all_pair_rows(norb, kmax):
+    pairs = [(a, c) for a in range(1, norb + 1) for c in range(a, norb + 1)]
+    return np.array([(k, a, b, c, d) for k in range(kmax + 1) for (a, c) in pairs for (b, d) in pairs],
+                    dtype=np.int32)
+
+
+def cpu_sample(fn, budget_s):
+    """run fn(deadline) single-thread; returns (units done, seconds)"""
+    t0 = time.perf_counter()
+    n = fn(t0 + budget_s)
+    return n, time.perf_counter() - t0
+
+
+def config_line(cfg, value, unit, ms, steps, warmup, workload, extra):
+    line = {"metric": f"config {cfg}: {unit}", "value": value, "unit": unit, "n_gpus": 1,
+            "steps": steps, "warmup": warmup, "ms_per_step": ms, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic (seeded orbitals and coefficients)", "config": workload}
+    line.update(extra)
+    return line
+
+
+def run_config1(args):
+    import math
+    import torch
+    from dbsr3_amd import DBS, Engine
+    from dbsr3_amd.grid import mkgrid_01
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    norb, kmax = 12, 4
+    rows = all_pair_rows(norb, kmax)
+    grids = {"recipe": None, "fe": None}
+    t, ns = mkgrid_01(9, 26.0, hi=1e-3, he=math.exp(0.125) - 1.0, hmax=0.25, rmax=30.0)
+    grids["recipe"] = (t, ns)
+    grids["fe"] = (np.loadtxt(os.path.join(ROOT, "tests", "golden", "fe_knot_t.txt")), 113)
+    rates, main = {}, None
+    for gname, (t, ns) in grids.items():
+        d = DBS(t, ns, 9, 8, 9)
+        pq = normalise_on_gpu(d, seeded_orbitals(t, ns, 9, 8, 9, norb, 20261203 + 1))
+        for kind, kname in ((0, "R"), (5, "S_ppqq"), (6, "S_pqqp")):
+            eng = Engine([(d, norb, rows)], kind=kind)
+            eng.set_orbitals(0, pq)
+            eng.set_coef(0, np.ones(len(rows)))
+            ms = time_graph(eng, False, args.steps, args.warmup)
+            r = {"ms": ms, "integrals_per_s": len(rows) / (ms * 1e-3)}
+            if kind == 0 and gname == "recipe":
+                roof, _ = roofline_of(eng.profile(with_adjoint=False, reps=3))
+                vals = eng.values()
+                eng.forward_plain()
+                torch.cuda.synchronize()
+                p0 = time.perf_counter()
+                eng.forward_plain()
+                plain_ms = (time.perf_counter() - p0) * 1e3
+                r["plain_gpu_ms"] = plain_ms
+                main = (ms, roof, t, ns, pq, vals)
+            rates[f"{gname}_ns{ns}_{kname}"] = r
+            eng.close()
+    ms, roof, t, ns, pq, vals = main
+    value = len(rows) / (ms * 1e-3)
+    cpu = None
+    if not args.no_cpu:
+        g = O.Grid(t, ns, 9, 8, 9)
+        fs = [np.asfortranarray(pq[:, :, o]) for o in range(norb)]
+        got = {}
+        def work(deadline):
+            got["v"] = O.rk_rows(g, fs, rows, deadline=deadline)
+            return len(got["v"])
+        n, secs = cpu_sample(work, args.cpu_seconds)
+        if n > 0:
+            cpu = {"value": n / secs, "unit": "R^k integrals/s", "cores": 1, "kind": "port",
+                   "sample": f"first {n} rows (k=0 first; arrays built inside the timed region), "
+                             f"{secs:.1f} s single-thread"}
+    workload = {"workload": "all R^k/S^k of 12 orbitals, a<=c, b<=d, k=0..4 (30420 rows per set), "
+                            "forward only, one hipGraph per step; value: R^k on the recipe grid",
+                "rows": int(len(rows)), "norb": norb, "kmax": kmax, "grids": {"recipe": grids["recipe"][1],
+                                                                             "fe": 113}}
+    return config_line(1, value, "R^k integrals/s", ms, args.steps, args.warmup, workload,
+                       {"sets": rates, "roofline": roof, "cpu_baseline": cpu})
+
+
+def run_config2(args):
+    import torch
+    from dbsr3_amd import DBS
+    from dbsr3_amd.cdata import r_data_34
+    from dbsr3_amd.dbs import fortran_to_device
+    from dbsr3_amd.grid import mkgrid_02
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    ns, ks, ksp, ksq, nch, norb, kmax, nent, ibo = 150, 9, 8, 9, 40, 20, 6, 200000, 2 ** 15
+    t = mkgrid_02(ns, ks, 26.0, 0.25, 30.0)
+    d = DBS(t, ns, ks, ksp, ksq)
+    pq = normalise_on_gpu(d, seeded_orbitals(t, ns, ks, ksp, ksq, norb, 20261203 + 2))
+    rng = np.random.default_rng(20261203 + 2)
+    # entries: (c, k, bound orbitals i, j, channel block ich >= jch, itype 3/4, atype)
+    c = rng.uniform(-1, 1, nent)
+    k = rng.integers(0, kmax + 1, nent)
+    i2 = rng.integers(1, norb + 1, nent)
+    i3 = rng.integers(1, norb + 1, nent)
+    blk = rng.integers(0, nch * (nch + 1) // 2, nent)
+    it = rng.integers(3, 5, nent)
+    at = rng.integers(0, 4, nent)
+    ich_of, jch_of = [], []
+    for jch in range(1, nch + 1):
+        for ich in range(jch, nch + 1):
+            ich_of.append(ich)
+            jch_of.append(jch)
+    ich_of, jch_of = np.array(ich_of), np.array(jch_of)
+    K1 = ich_of[blk] * ibo + jch_of[blk]
+    icc = np.zeros((nch, nch), dtype=np.int32, order="F")
+    for b in range(len(ich_of)):
+        icc[ich_of[b] - 1, jch_of[b] - 1] = b + 1
+    # merged lists per (itype, atype, k): sorted by (K1, K2, K3), equal keys summed
+    lists = []
+    for itype in (3, 4):
+        for atype in range(4):
+            for kk in range(kmax + 1):
+                sel = (it == itype) & (at == atype) & (k == kk)
+                keys = np.stack([K1[sel], i2[sel], i3[sel]], axis=1)
+                uk, inv = np.unique(keys, axis=0, return_inverse=True)
+                cs = np.zeros(len(uk))
+                np.add.at(cs, inv.ravel(), c[sel])
+                lists.append((itype, atype, kk, (cs, uk[:, 0].astype(np.int32), uk[:, 1].astype(np.int32),
+                                                 uk[:, 2].astype(np.int32))))
+    ms2 = 2 * ns
+    nblk = len(ich_of)
+    hch = torch.zeros(nblk * ms2 * ms2, dtype=torch.float64, device="cuda")
+    pq_dev = fortran_to_device(pq)
+    def sweep():
+        for itype, atype, kk, lst in lists:
+            r_data_34(d, itype, atype, kk, lst, pq_dev, norb, icc, hch, nblk)
+    sweep()  # warm: builds and caches the R^k arrays
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sweep()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    nl = sum(len(l[3][0]) for l in lists)
+    value = nl / (ms * 1e-3)
+    cpu = None
+    if not args.no_cpu:
+        g = O.Grid(t, ns, ks, ksp, ksq)
+        pqh = pq.copy(order="F")
+        href = np.zeros((ms2, ms2, nblk), order="F")
+        def work(deadline):
+            n = 0
+            for itype, atype, kk, lst in lists:
+                if time.perf_counter() > deadline:
+                    break
+                O.rdata_34(g, itype, atype, kk, lst, pqh, ibo, icc, href)
+                n += len(lst[0])
+            return n
+        n, secs = cpu_sample(work, args.cpu_seconds)
+        if n > 0:
+            cpu = {"value": n / secs, "unit": "list entries/s", "cores": 1, "kind": "port",
+                   "sample": f"first {n} merged entries in list order (arrays built inside the "
+                             f"timed region), {secs:.1f} s single-thread"}
+    workload = {"workload": "MAT3 R_data itype 3/4 into 820 channel blocks (ms=300): 2e5 entries, "
+                            "40 channels, 20 bound orbitals, ns=150, k=0..6, atype 0..3; host lists in, "
+                            "blocks on the device; one step = every list once",
+                "entries": nent, "merged_entries": int(nl), "lists": len(lists), "ns": ns, "channels": nch}
+    roof = {"bound": "hbm", "achieved": None, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": None,
+            "traffic": None, "note": "many small launches per list; see the plain-path kernels"}
+    return config_line(2, value, "list entries/s", ms, args.steps, 1, workload,
+                       {"roofline": roof, "cpu_baseline": cpu})
+
+
+def run_config3(args):
+    from dbsr3_amd import DBS, Engine
+    from dbsr3_amd.grid import mkgrid_02
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    rng = np.random.default_rng(20261203 + 3)
+    insts, host = [], []
+    for i in range(256):
+        ns = int(rng.integers(60, 401))
+        ks = int(rng.choice([7, 8, 9]))
+        kmax = int(rng.integers(2, 11))
+        norb = int(rng.integers(4, 9))
+        t = mkgrid_02(ns, ks, 26.0, 0.25, 30.0)
+        d = DBS(t, ns, ks, ks - 1, ks)
+        pq = normalise_on_gpu(d, seeded_orbitals(t, ns, ks, ks - 1, ks, norb, 20261203 + 3 + 1000 * i))
+        rows = all_pair_rows(norb, kmax)
+        insts.append((d, norb, rows))
+        host.append((t, ns, ks, pq, rows, rng.uniform(-1, 1, len(rows))))
+    nrows = sum(len(h[4]) for h in host)
+    eng = Engine(insts)
+    for i, h in enumerate(host):
+        eng.set_orbitals(i, h[3])
+        eng.set_coef(i, h[5])
+    tuning = eng.autotune(with_adjoint=True, reps=2)
+    ms = time_graph(eng, True, args.steps, args.warmup)
+    value = nrows / (ms * 1e-3)
+    roof, _ = roofline_of(eng.profile(with_adjoint=True, reps=3))
+    cpu = None
+    if not args.no_cpu:
+        order = sorted(range(len(host)), key=lambda j: host[j][1])
+        def work(deadline):
+            n = 0
+            for j in order:
+                t, ns, ks, pq, rows, _ = host[j]
+                g = O.Grid(t, ns, ks, ks - 1, ks)
+                fs = [np.asfortranarray(pq[:, :, o]) for o in range(pq.shape[2])]
+                n += len(O.rk_rows(g, fs, rows, deadline=deadline))
+                if time.perf_counter() > deadline:
+                    break
+            return n
+        n, secs = cpu_sample(work, args.cpu_seconds)
+        if n > 0:
+            cpu = {"value": n / secs, "unit": "R^k integrals/s", "cores": 1, "kind": "port",
+                   "sample": f"{n} rows of the smallest instances first (arrays built inside the timed "
+                             f"region), {secs:.1f} s single-thread; the GPU figure is forward+adjoint"}
+    workload = {"workload": "256 uneven instances (ns 60..400, ks 7..9, kmax 2..10, 4..8 orbitals, all "
+                            "pairs) forward+adjoint, one hipGraph per step, autotuned",
+                "instances": 256, "rows": int(nrows), "tuning": tuning}
+    return config_line(3, value, "R^k integrals/s", ms, args.steps, args.warmup, workload,
+                       {"roofline": roof, "cpu_baseline": cpu, "energy_0": eng.energy(0)})
+
+
+def run_config5(args):
+    from dbsr3_amd import DBS, Engine
+    from dbsr3_amd.grid import mkgrid_02
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    ns, ks, ksp, ksq, norb, kmax, nrows = 600, 9, 8, 9, 60, 12, 1_000_000
+    t = mkgrid_02(ns, ks, 26.0, 0.25, 30.0)
+    d = DBS(t, ns, ks, ksp, ksq)
+    pq0 = normalise_on_gpu(d, seeded_orbitals(t, ns, ks, ksp, ksq, norb, 20261203 + 5))
+    rng = np.random.default_rng(20261203 + 5)
+    rows = np.zeros((nrows, 5), dtype=np.int32)
+    rows[:, 0] = rng.integers(0, kmax + 1, nrows)
+    rows[:, 1:] = rng.integers(1, norb + 1, (nrows, 4))
+    coef = rng.uniform(-1, 1, nrows)
+    eng = Engine([(d, norb, rows)])
+    eng.set_orbitals(0, pq0)
+    eng.set_coef(0, coef)
+    if args.tuning:
+        eng.set_tuning([int(x) for x in args.tuning.split(",")])
+    tuning = eng.get_tuning()
+    bump = np.zeros_like(pq0)
+    bump[1:-2] = np.sin(np.arange(1, ns - 2))[:, None, None] * 1e-3
+    # one SCF iteration = new orbitals (a 576 KB upload) + the captured step
+    ms = time_graph(eng, True, 20, args.warmup, before=lambda i: eng.set_orbitals(0, pq0 + i * bump))
+    value = nrows / (ms * 1e-3)
+    roof, _ = roofline_of(eng.profile(with_adjoint=True, reps=2))
+    cpu = None
+    if not args.no_cpu:
+        g = O.Grid(t, ns, ks, ksp, ksq)
+        fs = [np.asfortranarray(pq0[:, :, o]) for o in range(norb)]
+        order = np.argsort(rows[:, 0], kind="stable")
+        def work(deadline):
+            return len(O.rk_rows(g, fs, rows[order], deadline=deadline))
+        n, secs = cpu_sample(work, args.cpu_seconds)
+        if n > 0:
+            cpu = {"value": n / secs, "unit": "R^k integrals/s", "cores": 1, "kind": "port",
+                   "sample": f"first {n} rows in multipole order (arrays built inside the timed region), "
+                             f"{secs:.1f} s single-thread; the GPU figure is forward+adjoint"}
+    workload = {"workload": "SCF-style: ns=600, ks=9/8/9, 60 orbitals, k=0..12, 1e6 rows; 20 iterations of "
+                            "new orbitals + one forward+adjoint hipGraph replay (default tuning)",
+                "rows": nrows, "ns": ns, "norb": norb, "kmax": kmax, "iterations": 20, "tuning": tuning}
+    return config_line(5, value, "R^k integrals/s", ms, 20, args.warmup, workload,
+                       {"roofline": roof, "cpu_baseline": cpu})
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -137,11 +465,17 @@ def main():
                          "(a plain copy kernel of known size for hardware-counter scaling)")
     ap.add_argument("--traffic-file", type=str,
                     default=os.path.join(ROOT, "profiles", TRAFFIC_FILE))
+    ap.add_argument("--config", type=int, default=4, choices=[1, 2, 3, 4, 5],
+                    help="integral-set configuration (4: the headline workload)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the engine shards by instance/row blocks "
               "but no multi-GPU figure is produced in this build (use --gpus 1)", file=sys.stderr)
         return 2
+    if args.config != 4:
+        run = {1: run_config1, 2: run_config2, 3: run_config3, 5: run_config5}[args.config]
+        print(json.dumps(run(args)), flush=True)
+        return 0
 
     import torch
     from dbsr3_amd import DBS, Engine
