@@ -679,8 +679,10 @@ __global__ void __launch_bounds__(pair_block_max<W>()) k_pair_panel(
   }
 }
 
-// Bank-conflict-free pair panels for groups with ndens <= blockDim (one
-// density per thread, up to 512).  The panel of density X is E = 2W doubles
+// Bank-conflict-free pair panels for groups of up to T*DPT densities
+// (thread t owns densities t + q blockDim, q < DPT; T = 512 threads, 1024
+// at W = 4; DPT = 2 for the narrow panels W = 8, 4 of groups up to 1024 or
+// 2048 densities).  The panel of density X is E = 2W doubles
 // [u0 v0 u1 v1 ...] (element e = 2 jl + h), kept in LDS as B[X][e] with
 // rows of E doubles, so the LDS bank of element e does not depend on the
 // row.  Lane l works on the elements in the rotated order
@@ -692,120 +694,148 @@ __global__ void __launch_bounds__(pair_block_max<W>()) k_pair_panel(
 // stages (rot bits 2, 4, 8), and the gradient rotated back before its
 // coalesced store, so the LDS holds only the panels (E*ndens doubles).
 // Adjacency entries are loaded one iteration ahead of their use.
-template <int W, bool ONE, bool ADJ>
-__global__ void __launch_bounds__(512) k_pair_rot(
+// workgroup size limit of the rotated kernel: W = 4 panels leave room for
+// 1024 threads (more waves to hide the adjacency loads of narrow panels)
+template <int W>
+constexpr int rot_threads_max() { return W == 4 ? 1024 : 512; }
+
+template <int W, bool ONE, bool ADJ, int DPT>
+__global__ void __launch_bounds__(rot_threads_max<W>()) k_pair_rot(
     int nwork, const PairWork* __restrict__ work, const KGroupDev* __restrict__ kgs,
     const InstDev* __restrict__ inst, const AdjChunk* __restrict__ chunks,
     const AdjEnt* __restrict__ ent, const int2* __restrict__ alen,
     const double* __restrict__ F, double* __restrict__ Gout, double* __restrict__ part) {
   extern __shared__ double sm[];
   constexpr int E = 2 * W;
-  const int X = threadIdx.x;
-  const int rot = 2 * (X & 7);  // even: element pairs (u,v) stay 16-byte aligned
+  // even: element pairs (u,v) stay 16-byte aligned
+  const int rot = (2 * (threadIdx.x & 7)) & (E - 1);
   for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
     const PairWork Wk = work[w];
     const KGroupDev Gp = kgs[Wk.g];
     const int nv = inst[Gp.inst].nv, nd = Gp.ndens;
     double* B = sm;  // [nd][E]
     const double* Fg = F + Gp.off_F;
-    const bool act = X < nd;
-    int n = 0;
-    long long eb = 0;
-    if (act) {
-      const AdjChunk C = chunks[Gp.chunk0 + (X >> 6)];
-      eb = C.base + (X & 63);
-      // the forward stops after the entries where X is the row's left one
-      n = ADJ ? alen[Gp.dens0 + X].x : alen[Gp.dens0 + X].y;
+    // the thread's densities X = threadIdx.x + q blockDim.x, q < DPT
+    bool act[DPT];
+    int n[DPT];
+    long long eb[DPT];
+#pragma unroll
+    for (int q = 0; q < DPT; ++q) {
+      const int X = threadIdx.x + q * blockDim.x;
+      act[q] = X < nd;
+      n[q] = 0;
+      eb[q] = 0;
+      if (act[q]) {
+        const AdjChunk C = chunks[Gp.chunk0 + (X >> 6)];
+        eb[q] = C.base + (X & 63);
+        // the forward stops after the entries where X is the row's left one
+        n[q] = ADJ ? alen[Gp.dens0 + X].x : alen[Gp.dens0 + X].y;
+      }
     }
     bool first = true;
     for (int j0 = Wk.j0; j0 < Wk.j1; j0 += W) {
       const int nj = min(W, Wk.j1 - j0);
       // B of the previous panel is no longer read
       __syncthreads();
-      double fx[E], gx[E];
+      double fx[DPT][E], gx[DPT][E];
       unsigned mL = 0;  // S^k: elements that pair as the row's left density
-      if (act) {
-#pragma unroll
-        for (int e = 0; e < E; ++e) fx[e] = (e >> 1) < nj ? Fg[(2LL * j0 + e) * nd + X] : 0.0;
-        // fx[t] = element (t + rot) mod E
-#pragma unroll
-        for (int b = 2; b < 16; b <<= 1) {
-          if (rot & b) {
-            double tmp[E];
-#pragma unroll
-            for (int t = 0; t < E; ++t) tmp[t] = fx[(t + b) & (E - 1)];
-#pragma unroll
-            for (int t = 0; t < E; ++t) fx[t] = tmp[t];
-          }
-        }
+      if (ONE) {
 #pragma unroll
         for (int t = 0; t < E; ++t) {
           const int e = (t + rot) & (E - 1);
-          B[X * E + e] = fx[t];
-          gx[t] = 0.0;
-          if (ONE) {
-            const bool lower = j0 + (e >> 1) < nv;
-            if (lower ? (e & 1) == 0 : (e & 1) == 1) mL |= 1u << t;
+          const bool lower = j0 + (e >> 1) < nv;
+          if (lower ? (e & 1) == 0 : (e & 1) == 1) mL |= 1u << t;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < DPT; ++q) {
+        const int X = threadIdx.x + q * blockDim.x;
+#pragma unroll
+        for (int t = 0; t < E; ++t) gx[q][t] = 0.0;
+        if (act[q]) {
+#pragma unroll
+          for (int e = 0; e < E; ++e) fx[q][e] = (e >> 1) < nj ? Fg[(2LL * j0 + e) * nd + X] : 0.0;
+          // fx[t] = element (t + rot) mod E
+#pragma unroll
+          for (int b = 2; b < E; b <<= 1) {
+            if (rot & b) {
+              double tmp[E];
+#pragma unroll
+              for (int t = 0; t < E; ++t) tmp[t] = fx[q][(t + b) & (E - 1)];
+#pragma unroll
+              for (int t = 0; t < E; ++t) fx[q][t] = tmp[t];
+            }
           }
+#pragma unroll
+          for (int t = 0; t < E; ++t) B[X * E + ((t + rot) & (E - 1))] = fx[q][t];
         }
       }
       __syncthreads();
-      // the next entry is loaded while the current one is processed, so its
-      // partner index is known before its LDS reads are issued
-      AdjEnt nxt;
-      if (n > 0) nxt = ent[eb];
-      for (int i = 0; i < n; ++i) {
-        const AdjEnt Aen = nxt;
-        if (i + 1 < n) nxt = ent[eb + 64LL * (i + 1)];
-        if (!ADJ && Aen.slot < 0) continue;
-        const double c = ADJ ? Aen.c : 0.0;
-        const double* pr = B + Aen.P * E;
-        double dot = 0.0;
-        // partner pair (u,v) of elements t, t+1 in one 16-byte read; the
-        // pairing swaps u and v
-        if (!ONE) {
 #pragma unroll
-          for (int t = 0; t < E; t += 2) {
-            const double2 pp = *reinterpret_cast<const double2*>(pr + ((t + rot) & (E - 1)));
-            dot += fx[t] * pp.y;
-            dot += fx[t + 1] * pp.x;
-            if (ADJ) {
-              gx[t] += c * pp.y;
-              gx[t + 1] += c * pp.x;
+      for (int q = 0; q < DPT; ++q) {
+        // the next entry is loaded while the current one is processed, so
+        // its partner index is known before its LDS reads are issued
+        const int nq = n[q];
+        AdjEnt nxt;
+        if (nq > 0) nxt = ent[eb[q]];
+        for (int i = 0; i < nq; ++i) {
+          const AdjEnt Aen = nxt;
+          if (i + 1 < nq) nxt = ent[eb[q] + 64LL * (i + 1)];
+          if (!ADJ && Aen.slot < 0) continue;
+          const double c = ADJ ? Aen.c : 0.0;
+          const double* pr = B + Aen.P * E;
+          // the row's partial of the previous panels, requested before the
+          // partner reads (workgroups with several panels)
+          double* const o = part + Gp.off_part + (long long)Wk.pr * Gp.nrows + (Aen.slot >= 0 ? Aen.slot : 0);
+          const double old = (!first && Aen.slot >= 0) ? *o : 0.0;
+          double dot = 0.0;
+          // partner pair (u,v) of elements t, t+1 in one 16-byte read; the
+          // pairing swaps u and v
+          if (!ONE) {
+#pragma unroll
+            for (int t = 0; t < E; t += 2) {
+              const double2 pp = *reinterpret_cast<const double2*>(pr + ((t + rot) & (E - 1)));
+              dot += fx[q][t] * pp.y;
+              dot += fx[q][t + 1] * pp.x;
+              if (ADJ) {
+                gx[q][t] += c * pp.y;
+                gx[q][t + 1] += c * pp.x;
+              }
+            }
+          } else {
+            const unsigned mg = Aen.slot != -1 ? mL : ~mL;
+#pragma unroll
+            for (int t = 0; t < E; t += 2) {
+              const double2 pp = *reinterpret_cast<const double2*>(pr + ((t + rot) & (E - 1)));
+              if ((mL >> t) & 1u) dot += fx[q][t] * pp.y;
+              if ((mL >> (t + 1)) & 1u) dot += fx[q][t + 1] * pp.x;
+              if (ADJ && ((mg >> t) & 1u)) gx[q][t] += c * pp.y;
+              if (ADJ && ((mg >> (t + 1)) & 1u)) gx[q][t + 1] += c * pp.x;
             }
           }
-        } else {
-          const unsigned mg = Aen.slot != -1 ? mL : ~mL;
-#pragma unroll
-          for (int t = 0; t < E; t += 2) {
-            const double2 pp = *reinterpret_cast<const double2*>(pr + ((t + rot) & (E - 1)));
-            if ((mL >> t) & 1u) dot += fx[t] * pp.y;
-            if ((mL >> (t + 1)) & 1u) dot += fx[t + 1] * pp.x;
-            if (ADJ && ((mg >> t) & 1u)) gx[t] += c * pp.y;
-            if (ADJ && ((mg >> (t + 1)) & 1u)) gx[t + 1] += c * pp.x;
-          }
-        }
-        if (Aen.slot >= 0) {
-          double* o = part + Gp.off_part + (long long)Wk.pr * Gp.nrows + Aen.slot;
-          *o = first ? dot : *o + dot;
+          if (Aen.slot >= 0) *o = first ? dot : old + dot;
         }
       }
-      if (ADJ && act) {
-        // back to element order, then coalesced rows of G
+      if (ADJ) {
 #pragma unroll
-        for (int b = 2; b < 16; b <<= 1) {
-          if (rot & b) {
-            double tmp[E];
+        for (int q = 0; q < DPT; ++q) {
+          if (!act[q]) continue;
+          // back to element order, then coalesced rows of G
 #pragma unroll
-            for (int t = 0; t < E; ++t) tmp[t] = gx[(t - b) & (E - 1)];
+          for (int b = 2; b < E; b <<= 1) {
+            if (rot & b) {
+              double tmp[E];
 #pragma unroll
-            for (int t = 0; t < E; ++t) gx[t] = tmp[t];
+              for (int t = 0; t < E; ++t) tmp[t] = gx[q][(t - b) & (E - 1)];
+#pragma unroll
+              for (int t = 0; t < E; ++t) gx[q][t] = tmp[t];
+            }
           }
-        }
-        double* g = Gout + Gp.off_F + X;
+          double* g = Gout + Gp.off_F + threadIdx.x + q * blockDim.x;
 #pragma unroll
-        for (int e = 0; e < E; ++e)
-          if ((e >> 1) < nj) g[(2LL * j0 + e) * nd] = gx[e];
+          for (int e = 0; e < E; ++e)
+            if ((e >> 1) < nj) g[(2LL * j0 + e) * nd] = gx[q][e];
+        }
       }
       first = false;
     }
@@ -1626,7 +1656,8 @@ struct dbs_engine {
   int pair_w = 16;        // panel width (pair slots) of this launch shape
   size_t pair_shm = 0;    // LDS bytes of the LDS pair-panel kernel
   size_t rot_shm = 0;     // LDS bytes of the rotated kernel
-  int rot_block = 64;     // its workgroup size (>= ndens of its groups)
+  int rot_block = 64;     // its workgroup size (rot_block * rot_dpt >= ndens of its groups)
+  int rot_dpt = 1;        // densities per thread of the rotated kernel
   size_t lds_block = 0;   // device limit of LDS per workgroup (bytes)
   RowRed* d_red = nullptr;
   int nred = 0;
@@ -1742,10 +1773,10 @@ int build_dense_tables(dbs_engine* e) {
 }  // namespace
 
 // dynamic LDS above 64 KB for every variant of a kernel; the first failure
-template <int W>
+template <int W, int DPT>
 static hipError_t rot_lds_attr(size_t bytes) {
-  const void* fs[] = {(const void*)k_pair_rot<W, false, true>, (const void*)k_pair_rot<W, false, false>,
-                      (const void*)k_pair_rot<W, true, true>, (const void*)k_pair_rot<W, true, false>};
+  const void* fs[] = {(const void*)k_pair_rot<W, false, true, DPT>, (const void*)k_pair_rot<W, false, false, DPT>,
+                      (const void*)k_pair_rot<W, true, true, DPT>, (const void*)k_pair_rot<W, true, false, DPT>};
   for (const void* f : fs) {
     const hipError_t err = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     if (err != hipSuccess) return err;
@@ -1843,17 +1874,23 @@ static int build_pair_tables(dbs_engine* e) {
   std::vector<PairWork> work[2][3];
   std::vector<RowRed> red;
   long long poff = 0;
-  // groups for the rotated kernel: one density per thread
-  const int rot_max = 512;
+  // groups for the rotated kernel: up to T threads (512, 1024 at W = 4)
+  // with DPT densities each (DPT = 1 at W = 16, up to 2 at W = 8 and 4)
+  const int tmax = W == 4 ? 1024 : 512;
+  const int rot_max = W >= 16 ? tmax : 2 * tmax;
   int rot_nd = 0;
   for (int g = 0; g < e->nkg; ++g) {
     const int nd = e->h_kg[g].ndens;
-    if (W >= 8 && nd <= rot_max && rot_lds_bytes(W, nd) <= kPairLdsMax) rot_nd = std::max(rot_nd, nd);
+    if (nd <= rot_max && rot_lds_bytes(W, nd) <= kPairLdsMax) rot_nd = std::max(rot_nd, nd);
   }
-  e->rot_block = std::max(64, (rot_nd + 63) / 64 * 64);
+  e->rot_dpt = rot_nd <= tmax ? 1 : 2;
+  e->rot_block = std::max(64, ((rot_nd + e->rot_dpt - 1) / e->rot_dpt + 63) / 64 * 64);
   e->rot_shm = rot_lds_bytes(W, std::max(rot_nd, 1));
-  if (e->rot_shm > 64 * 1024 && W >= 8) {
-    const hipError_t err = W == 8 ? rot_lds_attr<8>(e->rot_shm) : rot_lds_attr<16>(e->rot_shm);
+  if (e->rot_shm > 64 * 1024) {
+    hipError_t err;
+    if (W == 16) err = rot_lds_attr<16, 1>(e->rot_shm);
+    else if (W == 8) err = e->rot_dpt == 1 ? rot_lds_attr<8, 1>(e->rot_shm) : rot_lds_attr<8, 2>(e->rot_shm);
+    else err = e->rot_dpt == 1 ? rot_lds_attr<4, 1>(e->rot_shm) : rot_lds_attr<4, 2>(e->rot_shm);
     if (err != hipSuccess) return fail(DBS_ERR_HIP, std::string("pair tables: LDS size: ") + hipGetErrorString(err));
   }
   for (int g = 0; g < e->nkg; ++g) {
@@ -1872,7 +1909,7 @@ static int build_pair_tables(dbs_engine* e) {
     const int span = W * np_g;
     const int npr = (J + span - 1) / span;
     int var = pair_lds_bytes(W, G.ndens) <= kPairLdsMax ? 1 : 0;
-    if (W >= 8 && G.ndens <= rot_nd) var = 2;
+    if (G.ndens <= rot_nd) var = 2;
     G.npr = e->group_dense[g] ? 0 : npr;
     G.dense = e->group_dense[g];
     G.off_part = poff;
@@ -1950,6 +1987,27 @@ static void mark(dbs_engine* e, const char* name) {
   e->prof_pos++;
 }
 
+template <int W, int DPT>
+static void launch_rot(dbs_engine* e, int adj, unsigned grid, const PairWork* wk, int n, hipStream_t s) {
+  const int b = e->rot_block;
+  const size_t shm = e->rot_shm;
+  if (e->onesided) {
+    if (adj)
+      k_pair_rot<W, true, true, DPT><<<grid, b, shm, s>>>(n, wk, e->d_kg, e->d_inst, e->d_chunks, e->d_ent,
+                                                          e->d_alen, e->F, e->G, e->d_part);
+    else
+      k_pair_rot<W, true, false, DPT><<<grid, b, shm, s>>>(n, wk, e->d_kg, e->d_inst, e->d_chunks, e->d_ent,
+                                                           e->d_alen, e->F, e->G, e->d_part);
+  } else {
+    if (adj)
+      k_pair_rot<W, false, true, DPT><<<grid, b, shm, s>>>(n, wk, e->d_kg, e->d_inst, e->d_chunks, e->d_ent,
+                                                           e->d_alen, e->F, e->G, e->d_part);
+    else
+      k_pair_rot<W, false, false, DPT><<<grid, b, shm, s>>>(n, wk, e->d_kg, e->d_inst, e->d_chunks, e->d_ent,
+                                                            e->d_alen, e->F, e->G, e->d_part);
+  }
+}
+
 template <int W>
 static void launch_pair_w(dbs_engine* e, int adj, int var, hipStream_t s) {
   const int n = e->nwork[adj][var];
@@ -1957,25 +2015,8 @@ static void launch_pair_w(dbs_engine* e, int adj, int var, hipStream_t s) {
   const unsigned grid = (unsigned)std::min(n, e->tune[3] * 4);
   const PairWork* wk = e->d_work[adj][var];
   if (var == 2) {
-    if constexpr (W >= 8) {
-      const int b = e->rot_block;
-      const size_t shm = e->rot_shm;
-      if (e->onesided) {
-        if (adj)
-          k_pair_rot<W, true, true><<<grid, b, shm, s>>>(n, wk, e->d_kg, e->d_inst, e->d_chunks, e->d_ent,
-                                                         e->d_alen, e->F, e->G, e->d_part);
-        else
-          k_pair_rot<W, true, false><<<grid, b, shm, s>>>(n, wk, e->d_kg, e->d_inst, e->d_chunks, e->d_ent,
-                                                          e->d_alen, e->F, e->G, e->d_part);
-      } else {
-        if (adj)
-          k_pair_rot<W, false, true><<<grid, b, shm, s>>>(n, wk, e->d_kg, e->d_inst, e->d_chunks, e->d_ent,
-                                                          e->d_alen, e->F, e->G, e->d_part);
-        else
-          k_pair_rot<W, false, false><<<grid, b, shm, s>>>(n, wk, e->d_kg, e->d_inst, e->d_chunks, e->d_ent,
-                                                           e->d_alen, e->F, e->G, e->d_part);
-      }
-    }
+    if (e->rot_dpt == 1) launch_rot<W, 1>(e, adj, grid, wk, n, s);
+    else if constexpr (W <= 8) launch_rot<W, 2>(e, adj, grid, wk, n, s);
     return;
   }
   const int block = std::min(e->tune[4], pair_block_max<W>());
