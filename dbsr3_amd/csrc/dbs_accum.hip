@@ -840,47 +840,58 @@ int dbs_rdata_34_dev(dbs_ctx* ctx, int itype, int atype, int k, int nc, const do
   hipStream_t s = ctx->stream;
   const int sym = itype == 3 ? 0 : 1;
   const long long per = (long long)ns * (sym == 0 ? ks : ns);
-  int *d_goff = nullptr, *d_j1 = nullptr, *d_j2 = nullptr;
-  double *d_c = nullptr, *d_xx = nullptr, *d_a = nullptr;
-  int4* d_gi = nullptr;
-  do {
-    if ((rc = dev_upload(&d_goff, goff, s)) || (rc = dev_upload(&d_j1, ej1, s)) ||
-        (rc = dev_upload(&d_j2, ej2, s)) || (rc = dev_upload(&d_c, ec, s)) ||
-        (rc = dev_upload(&d_gi, ginfo, s))) break;
-    if (hipMalloc(&d_xx, sizeof(double) * per * ng) != hipSuccess ||
-        hipMalloc(&d_a, sizeof(double) * per * ng) != hipSuccess) { rc = fail(DBS_ERR_HIP, "R_data: hipMalloc"); break; }
-    k_group_density<<<grid_of(per * ng, 256), 256, 0, s>>>(ng, d_goff, d_j1, d_j2, d_c, pq_dev, ns, ks,
-                                                            jp1, jp2, sym, d_xx);
-    if (sym == 0) {
-      constexpr int GB = 8;
-      const long long nthr = (long long)ns * ks * ((ng + GB - 1) / GB);
-      k_group_convol1<GB><<<grid_of(nthr, 256), 256, 0, s>>>(ng, ns, ks, rkb, d_xx, d_a);
-    } else {
-      k_group_convol3<<<grid_of(per * ng, 256), 256, 0, s>>>(ng, ns, ks, rkb, d_xx, d_a);
-    }
-    // one launch for all runs when their blocks differ; runs sharing a block
-    // (K1 and its transpose, or a K1 repeated out of order) one launch each,
-    // in list order
-    std::vector<int> blks;
-    for (const int4& G : ginfo) blks.push_back(G.z);
-    std::sort(blks.begin(), blks.end());
-    const bool shared = std::adjacent_find(blks.begin(), blks.end()) != blks.end();
-    const long long nupd = (long long)ns * ns * ng;
-    if (!shared) {
+  // every buffer of the call in the context's grow-only scratch (no device
+  // allocation per call): densities, convolutions, coefficients, then the
+  // integer tables (run offsets, orbitals, run info), 16-byte aligned
+  const size_t nxx = (size_t)per * ng, nce = ec.size();
+  const size_t n_int = goff.size() + ej1.size() + ej2.size();
+  const size_t off_c = 2 * nxx, off_i = off_c + nce, off_g = off_i + (n_int + 1) / 2 + 1;
+  const size_t need = (off_g + 2 * (size_t)ng + 2) & ~(size_t)1;
+  if ((rc = ensure_scratch(ctx, need))) return rc;
+  double* d_xx = ctx->scratch;
+  double* d_a = d_xx + nxx;
+  double* d_c = ctx->scratch + off_c;
+  int* d_goff = reinterpret_cast<int*>(ctx->scratch + off_i);
+  int* d_j1 = d_goff + goff.size();
+  int* d_j2 = d_j1 + ej1.size();
+  int4* d_gi = reinterpret_cast<int4*>(ctx->scratch + ((off_g + 1) & ~(size_t)1));
+  auto put = [&](void* dst, const void* src, size_t bytes) {
+    return bytes == 0 || hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s) == hipSuccess;
+  };
+  if (!put(d_c, ec.data(), nce * sizeof(double)) || !put(d_goff, goff.data(), goff.size() * sizeof(int)) ||
+      !put(d_j1, ej1.data(), ej1.size() * sizeof(int)) || !put(d_j2, ej2.data(), ej2.size() * sizeof(int)) ||
+      !put(d_gi, ginfo.data(), ginfo.size() * sizeof(int4)))
+    return fail(DBS_ERR_HIP, "R_data: upload");
+  k_group_density<<<grid_of(per * ng, 256), 256, 0, s>>>(ng, d_goff, d_j1, d_j2, d_c, pq_dev, ns, ks, jp1, jp2,
+                                                          sym, d_xx);
+  if (sym == 0) {
+    constexpr int GB = 8;
+    const long long nthr = (long long)ns * ks * ((ng + GB - 1) / GB);
+    k_group_convol1<GB><<<grid_of(nthr, 256), 256, 0, s>>>(ng, ns, ks, rkb, d_xx, d_a);
+  } else {
+    k_group_convol3<<<grid_of(per * ng, 256), 256, 0, s>>>(ng, ns, ks, rkb, d_xx, d_a);
+  }
+  // one launch for all runs when their blocks differ; runs sharing a block
+  // (K1 and its transpose, or a K1 repeated out of order) one launch each,
+  // in list order
+  std::vector<int> blks;
+  for (const int4& G : ginfo) blks.push_back(G.z);
+  std::sort(blks.begin(), blks.end());
+  const bool shared = std::adjacent_find(blks.begin(), blks.end()) != blks.end();
+  const long long nupd = (long long)ns * ns * ng;
+  if (!shared) {
+    for (int pass = 0; pass < 2; ++pass)
+      k_update_hs_mat<<<grid_of(nupd, 256), 256, 0, s>>>(ng, d_gi, ns, ks, sym, ip1, ip2, pass, d_a, hch_dev);
+  } else {
+    for (int g = 0; g < ng; ++g)
       for (int pass = 0; pass < 2; ++pass)
-        k_update_hs_mat<<<grid_of(nupd, 256), 256, 0, s>>>(ng, d_gi, ns, ks, sym, ip1, ip2, pass, d_a, hch_dev);
-    } else {
-      for (int g = 0; g < ng; ++g)
-        for (int pass = 0; pass < 2; ++pass)
-          k_update_hs_mat<<<grid_of((long long)ns * ns, 256), 256, 0, s>>>(1, d_gi + g, ns, ks, sym, ip1, ip2, pass,
-                                                                           d_a + per * g, hch_dev);
-    }
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    if (e != hipSuccess) rc = fail(DBS_ERR_HIP, std::string("R_data: ") + hipGetErrorString(e));
-  } while (0);
-  (void)hipFree(d_goff); (void)hipFree(d_j1); (void)hipFree(d_j2); (void)hipFree(d_c); (void)hipFree(d_gi);
-  (void)hipFree(d_xx); (void)hipFree(d_a);
+        k_update_hs_mat<<<grid_of((long long)ns * ns, 256), 256, 0, s>>>(1, d_gi + g, ns, ks, sym, ip1, ip2, pass,
+                                                                         d_a + per * g, hch_dev);
+  }
+  // the host tables must outlive their copies
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) rc = fail(DBS_ERR_HIP, std::string("R_data: ") + hipGetErrorString(e));
   return rc;
 }
 
