@@ -309,18 +309,24 @@ __global__ void k_group_convol1(int ng, int ns, int ks, const double* __restrict
 
 // convol case 3 ('s','s') for many groups, gathered per output element:
 // a(x,y) collects rkb(i,j,ip,jp)*d(u,v) over the four symmetric-storage
-// terms of the reference (convol, dbsr_lib_dbs.f90:2344-2357)
+// terms of the reference (convol, dbsr_lib_dbs.f90:2344-2357); GB groups
+// per thread so each rkb value and its index arithmetic feed GB products
+template <int GB>
 __global__ void k_group_convol3(int ng, int ns, int ks, const double* __restrict__ rkb,
                                 const double* __restrict__ xx, double* __restrict__ out) {
   const long long per = (long long)ns * ns;
-  const long long total = per * ng;
+  const int ngc = (ng + GB - 1) / GB;
+  const long long total = per * ngc;
   for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
        q += (long long)gridDim.x * blockDim.x) {
-    const int g = (int)(q / per);
+    const int g0 = (int)(q / per) * GB;
     const int r = (int)(q % per);
     const int x = r % ns, y = r / ns;  // 0-based output a(x,y)
-    const double* d = xx + (long long)g * per;
-    double acc = 0.0;
+    const double* d = xx + (long long)g0 * per;
+    const int nb = min(GB, ng - g0);    // groups of this thread
+    double acc[GB];
+#pragma unroll
+    for (int b = 0; b < GB; ++b) acc[b] = 0.0;
     // terms: (xs,ys,cond): (0,1,-),(1,1,ip>0),(0,0,jp>0),(1,0,ip>0&&jp>0)
     for (int t = 0; t < 4; ++t) {
       const int xs = (t == 1 || t == 3), ys = (t == 0 || t == 1);
@@ -336,11 +342,18 @@ __global__ void k_group_convol3(int ng, int ns, int ks, const double* __restrict
           if (i < 0 || i >= ns - ip) continue;
           const int ii = i + ip;
           const int u = xs == 0 ? ii : i;
-          acc += rkb[i + (long long)ns * (j + (long long)ns * (ip + (long long)ks * jp))] * d[u + (long long)ns * v];
+          // one rkb value feeds the GB groups' products
+          const double rv = rkb[i + (long long)ns * (j + (long long)ns * (ip + (long long)ks * jp))];
+          const double* du = d + u + (long long)ns * v;
+#pragma unroll
+          for (int b = 0; b < GB; ++b)
+            if (b < nb) acc[b] += rv * du[(long long)b * per];
         }
       }
     }
-    out[q] = acc;
+#pragma unroll
+    for (int b = 0; b < GB; ++b)
+      if (b < nb) out[(long long)(g0 + b) * per + r] = acc[b];
   }
 }
 
@@ -869,7 +882,8 @@ int dbs_rdata_34_dev(dbs_ctx* ctx, int itype, int atype, int k, int nc, const do
     const long long nthr = (long long)ns * ks * ((ng + GB - 1) / GB);
     k_group_convol1<GB><<<grid_of(nthr, 256), 256, 0, s>>>(ng, ns, ks, rkb, d_xx, d_a);
   } else {
-    k_group_convol3<<<grid_of(per * ng, 256), 256, 0, s>>>(ng, ns, ks, rkb, d_xx, d_a);
+    constexpr int GB = 8;
+    k_group_convol3<GB><<<grid_of(per * ((ng + GB - 1) / GB), 256), 256, 0, s>>>(ng, ns, ks, rkb, d_xx, d_a);
   }
   // one launch for all runs when their blocks differ; runs sharing a block
   // (K1 and its transpose, or a K1 repeated out of order) one launch each,
