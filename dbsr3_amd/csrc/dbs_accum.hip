@@ -246,11 +246,13 @@ __global__ void k_accumulate_targets(int ntarget, const long long* __restrict__ 
 }
 
 // xx_g = sum_e c_e * density(pq(jp1,j1), pq(jp2,j2)) in list order;
-// sym 0: band 's' (ns,ks); 1: full 'x' (ns,ns)
+// sym 0: band 's' (ns,ks); 1: full 'x' (ns,ns). tgb = 0: group-major
+// xx[g][r]; tgb > 0: chunks of tgb groups with the group index fastest,
+// xx[g/tgb][r][g%tgb] (the last chunk's missing groups are never written)
 __global__ void k_group_density(int ng, const int* __restrict__ goff, const int* __restrict__ ej1,
                                 const int* __restrict__ ej2, const double* __restrict__ ec,
                                 const double* __restrict__ pq, int ns, int ks, int jp1, int jp2,
-                                int sym, double* __restrict__ xx) {
+                                int sym, int tgb, double* __restrict__ xx) {
 #pragma clang fp contract(off)
   const int ncol = sym == 0 ? ks : ns;
   const long long per = (long long)ns * ncol;
@@ -274,37 +276,134 @@ __global__ void k_group_density(int ng, const int* __restrict__ goff, const int*
       }
       acc = acc + ec[e] * d;
     }
-    xx[q] = acc;
+    xx[tgb > 0 ? ((long long)(g / tgb) * per + r) * tgb + g % tgb : q] = acc;
   }
 }
 
-// convol case 1 ('s','s') for many groups: a_g(i,ip) = sum_jp sum_j
-// xx_g(j,jp) rkb(i,j,ip,jp), GB groups per thread so each rkb value feeds
-// GB products.
+// convol case 1 for many groups, one workgroup per 256 outputs (i,ip) and
+// a chunk of GB groups: the chunk is uniform over the workgroup and its
+// densities are stored group-fastest (k_group_density with tgb = GB), so the
+// GB values of one (j,jp) are one contiguous wave-uniform (scalar) read
+// shared by all lanes, and each rkb value feeds GB products. XCD-aware block order as in
+// k_group_convol3_tile (one (i,ip) range's chunks on one XCD, its rkb slice
+// in that XCD's L2). a_g(i,ip) = sum_jp sum_j xx_g(j,jp) rkb(i,j,ip,jp),
+// summed over jp, then j (convol case 1, dbsr_lib_dbs.f90:2249-2253); outputs with
+// i >= ns-ip are 0. Needs a grid that is a multiple of 8.
+constexpr int CV1_GB = 16;
+
 template <int GB>
-__global__ void k_group_convol1(int ng, int ns, int ks, const double* __restrict__ rkb,
-                                const double* __restrict__ xx, double* __restrict__ out) {
+__global__ __launch_bounds__(256) void k_group_convol1_u(int ng, int ns, int ks, int nqb,
+                                                         const double* __restrict__ rkb,
+                                                         const double* __restrict__ xx,
+                                                         double* __restrict__ out) {
 #pragma clang fp contract(off)
-  const int ngc = (ng + GB - 1) / GB;
-  const long long total = (long long)ns * ks * ngc;
-  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
-       q += (long long)gridDim.x * blockDim.x) {
-    const int i = (int)(q % ns);
-    const int ip = (int)((q / ns) % ks);
-    const int g0 = (int)(q / ((long long)ns * ks)) * GB;
-    double acc[GB];
-    for (int b = 0; b < GB; ++b) acc[b] = 0.0;
-    if (i < ns - ip) {
-      for (int jp = 0; jp < ks; ++jp)
-        for (int j = 0; j < ns - jp; ++j) {
-          const double r = rkb[i + (long long)ns * (j + (long long)ns * (ip + (long long)ks * jp))];
-          for (int b = 0; b < GB; ++b)
-            if (g0 + b < ng) acc[b] = acc[b] + xx[(long long)(g0 + b) * ns * ks + j + (long long)ns * jp] * r;
-        }
+  const int nchunk = (ng + GB - 1) / GB;
+  const int lid = (int)(blockIdx.x % 8) * (int)(gridDim.x / 8) + (int)(blockIdx.x / 8);
+  if (lid >= nqb * nchunk) return;
+  const int qb = lid / nchunk, chunk = lid % nchunk;
+  const int q = qb * 256 + (int)threadIdx.x;
+  if (q >= ns * ks) return;
+  const int i = q % ns, ip = q / ns;
+  const int g0 = chunk * GB, nb = min(GB, ng - g0);
+  const long long per = (long long)ns * ks, nn = (long long)ns * ns;
+  const double* xc = xx + (long long)chunk * per * GB;  // this chunk: [r][b]
+  double acc[GB];
+#pragma unroll
+  for (int b = 0; b < GB; ++b) acc[b] = 0.0;
+  // every lane runs the loop (rkb(i,j,ip,jp) is in bounds for all i, j <
+  // ns), so the density loads stay uniform; invalid outputs are zeroed below
+  for (int jp = 0; jp < ks; ++jp) {
+    const double* rr = rkb + i + nn * (ip + (long long)ks * jp);
+    const int nj = ns - jp;
+    const double* xj = xc + (long long)ns * jp * GB;
+    // rkb values in batches of 8 loads issued ahead of their products
+    for (int j = 0; j < nj; j += 8) {
+      double r[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) r[u] = j + u < nj ? rr[(long long)ns * (j + u)] : 0.0;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        if (j + u >= nj) break;
+        const double* xv = xj + (long long)(j + u) * GB;
+#pragma unroll
+        for (int b = 0; b < GB; ++b) acc[b] = acc[b] + xv[b] * r[u];
+      }
     }
-    for (int b = 0; b < GB; ++b)
-      if (g0 + b < ng) out[(long long)(g0 + b) * ns * ks + i + (long long)ns * ip] = acc[b];
   }
+  const bool valid = i < ns - ip;
+#pragma unroll
+  for (int b = 0; b < GB; ++b)
+    if (b < nb) out[(long long)(g0 + b) * per + i + (long long)ns * ip] = valid ? acc[b] : 0.0;
+}
+
+// tiled convol case 3: one workgroup computes a CV_TX x CV_TY block of
+// outputs a(x,y) for GB groups, with those groups' densities over the block
+// and its ks-1 halo staged in LDS, so the 4*ks*ks terms of each output read
+// LDS instead of global memory and each rkb value feeds GB products. Block
+// order is XCD-aware: the dispatcher deals workgroups to the 8 XCDs round
+// robin, so logical block (xcd, w) runs one tile's group chunks back to back
+// on one XCD and that tile's rkb slice stays in the XCD's L2. Terms are
+// summed in the same order as k_group_convol3. Needs ks <= CV_KS and a grid
+// that is a multiple of 8.
+constexpr int CV_TX = 32, CV_TY = 8, CV_KS = 9, CV_H = CV_KS - 1;
+constexpr int CV_SX = CV_TX + 2 * CV_H, CV_SY = CV_TY + 2 * CV_H;
+
+template <int GB>
+__global__ __launch_bounds__(CV_TX* CV_TY) void k_group_convol3_tile(int ng, int ns, int ks, int ntx, int nty,
+                                                                     const double* __restrict__ rkb,
+                                                                     const double* __restrict__ xx,
+                                                                     double* __restrict__ out) {
+  __shared__ double L[GB][CV_SY][CV_SX];
+  const int nchunk = (ng + GB - 1) / GB;
+  const int lid = (int)(blockIdx.x % 8) * (int)(gridDim.x / 8) + (int)(blockIdx.x / 8);
+  if (lid >= ntx * nty * nchunk) return;  // whole workgroup: before any barrier
+  const int tile = lid / nchunk, chunk = lid % nchunk;
+  const int x0 = (tile % ntx) * CV_TX, y0 = (tile / ntx) * CV_TY;
+  const int g0 = chunk * GB, nb = min(GB, ng - g0);
+  const long long per = (long long)ns * ns;
+  for (int e = threadIdx.x; e < GB * CV_SY * CV_SX; e += CV_TX * CV_TY) {
+    const int b = e / (CV_SY * CV_SX), rem = e % (CV_SY * CV_SX);
+    const int u = x0 - CV_H + rem % CV_SX, v = y0 - CV_H + rem / CV_SX;
+    double val = 0.0;
+    if (b < nb && u >= 0 && u < ns && v >= 0 && v < ns) val = xx[(long long)(g0 + b) * per + u + (long long)ns * v];
+    (&L[0][0][0])[e] = val;
+  }
+  __syncthreads();
+  const int tx = (int)threadIdx.x % CV_TX, ty = (int)threadIdx.x / CV_TX;
+  const int x = x0 + tx, y = y0 + ty;
+  if (x >= ns || y >= ns) return;  // no barrier follows
+  double acc[GB];
+#pragma unroll
+  for (int b = 0; b < GB; ++b) acc[b] = 0.0;
+  // terms: (xs,ys,cond): (0,1,-),(1,1,ip>0),(0,0,jp>0),(1,0,ip>0&&jp>0)
+  for (int t = 0; t < 4; ++t) {
+    const int xs = (t == 1 || t == 3), ys = (t == 0 || t == 1);
+    for (int jp = 0; jp < ks; ++jp) {
+      if (t >= 2 && jp == 0) continue;
+      const int j = ys == 0 ? y : y - jp;
+      if (j < 0 || j >= ns - jp) continue;
+      const int vl = CV_H + ty + (ys == 0 ? jp : -jp);  // LDS row of v
+      const double* rr = rkb + (long long)ns * (j + (long long)ns * ks * jp);
+      double rv[CV_KS];
+      bool ok[CV_KS];
+#pragma unroll
+      for (int ip = 0; ip < CV_KS; ++ip) {
+        const int i = xs == 0 ? x : x - ip;
+        ok[ip] = ip < ks && !(xs == 1 && ip == 0) && i >= 0 && i < ns - ip;
+        rv[ip] = ok[ip] ? rr[i + per * ip] : 0.0;
+      }
+#pragma unroll
+      for (int ip = 0; ip < CV_KS; ++ip) {
+        if (!ok[ip]) continue;
+        const int ul = CV_H + tx + (xs == 0 ? ip : -ip);  // LDS column of u
+#pragma unroll
+        for (int b = 0; b < GB; ++b) acc[b] += rv[ip] * L[b][vl][ul];
+      }
+    }
+  }
+#pragma unroll
+  for (int b = 0; b < GB; ++b)
+    if (b < nb) out[(long long)(g0 + b) * per + x + (long long)ns * y] = acc[b];
 }
 
 // convol case 3 ('s','s') for many groups, gathered per output element:
@@ -856,7 +955,9 @@ int dbs_rdata_34_dev(dbs_ctx* ctx, int itype, int atype, int k, int nc, const do
   // every buffer of the call in the context's grow-only scratch (no device
   // allocation per call): densities, convolutions, coefficients, then the
   // integer tables (run offsets, orbitals, run info), 16-byte aligned
-  const size_t nxx = (size_t)per * ng, nce = ec.size();
+  // band densities in chunks of CV1_GB groups (group index fastest), padded
+  // to whole chunks; the full-square case keeps one group after another
+  const size_t nxx = (size_t)per * (sym == 0 ? (ng + CV1_GB - 1) / CV1_GB * CV1_GB : ng), nce = ec.size();
   const size_t n_int = goff.size() + ej1.size() + ej2.size();
   const size_t off_c = 2 * nxx, off_i = off_c + nce, off_g = off_i + (n_int + 1) / 2 + 1;
   const size_t need = (off_g + 2 * (size_t)ng + 2) & ~(size_t)1;
@@ -876,11 +977,21 @@ int dbs_rdata_34_dev(dbs_ctx* ctx, int itype, int atype, int k, int nc, const do
       !put(d_gi, ginfo.data(), ginfo.size() * sizeof(int4)))
     return fail(DBS_ERR_HIP, "R_data: upload");
   k_group_density<<<grid_of(per * ng, 256), 256, 0, s>>>(ng, d_goff, d_j1, d_j2, d_c, pq_dev, ns, ks, jp1, jp2,
-                                                          sym, d_xx);
+                                                          sym, sym == 0 ? CV1_GB : 0, d_xx);
   if (sym == 0) {
+    constexpr int GB = CV1_GB;
+    const int nqb = (ns * ks + 255) / 256;
+    const long long nblk = (long long)nqb * ((ng + GB - 1) / GB);
+    if (nblk > (1ll << 30)) return fail(DBS_ERR_ARG, "R_data: too many convolution blocks");
+    const unsigned grid = (unsigned)((nblk + 7) / 8 * 8);
+    k_group_convol1_u<GB><<<grid, 256, 0, s>>>(ng, ns, ks, nqb, rkb, d_xx, d_a);
+  } else if (ks <= CV_KS) {
     constexpr int GB = 8;
-    const long long nthr = (long long)ns * ks * ((ng + GB - 1) / GB);
-    k_group_convol1<GB><<<grid_of(nthr, 256), 256, 0, s>>>(ng, ns, ks, rkb, d_xx, d_a);
+    const int ntx = (ns + CV_TX - 1) / CV_TX, nty = (ns + CV_TY - 1) / CV_TY;
+    const long long nblk = (long long)ntx * nty * ((ng + GB - 1) / GB);
+    if (nblk > (1ll << 30)) return fail(DBS_ERR_ARG, "R_data: too many convolution tiles");
+    const unsigned grid = (unsigned)((nblk + 7) / 8 * 8);
+    k_group_convol3_tile<GB><<<grid, CV_TX * CV_TY, 0, s>>>(ng, ns, ks, ntx, nty, rkb, d_xx, d_a);
   } else {
     constexpr int GB = 8;
     k_group_convol3<GB><<<grid_of(per * ((ng + GB - 1) / GB), 256), 256, 0, s>>>(ng, ns, ks, rkb, d_xx, d_a);

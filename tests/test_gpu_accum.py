@@ -69,10 +69,24 @@ def test_rk_evaluate(setup):
 
 
 def test_r_data_channel_blocks(setup):
+    g, d = setup
+    _check_r_data(g, d)
+
+
+def test_r_data_wide_band(gpu_available):
+    """ks = 10: the exchange convolution takes its untiled kernel (the LDS
+    tiles hold halos for ks <= 9 only)"""
+    from dbsr3_amd import DBS
+    ns, ks = 24, 10
+    t = O.mkgrid_02(ns, ks, 0.25, 20.0, 4.0)
+    print(f"[accum grid ns={ns} ks={ks}]", flush=True)
+    _check_r_data(O.Grid(t, ns, ks, ks - 1, ks), DBS(t, ns, ks, ks - 1, ks))
+
+
+def _check_r_data(g, d):
     from dbsr3_amd import CData
     from dbsr3_amd.cdata import r_data_34
     from dbsr3_amd.dbs import device_to_fortran, fortran_to_device
-    g, d = setup
     nbound, nch = 5, 4
     norb = nbound + nch
     fs = random_orbitals(g, nbound, seed=33)
