@@ -288,7 +288,7 @@ __global__ void k_group_density(int ng, const int* __restrict__ goff, const int*
 // k_group_convol3_tile (one (i,ip) range's chunks on one XCD, its rkb slice
 // in that XCD's L2). a_g(i,ip) = sum_jp sum_j xx_g(j,jp) rkb(i,j,ip,jp),
 // summed over jp, then j (convol case 1, dbsr_lib_dbs.f90:2249-2253); outputs with
-// i >= ns-ip are 0. Needs a grid that is a multiple of 8.
+// i >= ns-ip are 0; fused multiply-adds. Needs a grid that is a multiple of 8.
 constexpr int CV1_GB = 16;
 
 template <int GB>
@@ -296,7 +296,6 @@ __global__ __launch_bounds__(256) void k_group_convol1_u(int ng, int ns, int ks,
                                                          const double* __restrict__ rkb,
                                                          const double* __restrict__ xx,
                                                          double* __restrict__ out) {
-#pragma clang fp contract(off)
   const int nchunk = (ng + GB - 1) / GB;
   const int lid = (int)(blockIdx.x % 8) * (int)(gridDim.x / 8) + (int)(blockIdx.x / 8);
   if (lid >= nqb * nchunk) return;
@@ -326,7 +325,7 @@ __global__ __launch_bounds__(256) void k_group_convol1_u(int ng, int ns, int ks,
         if (j + u >= nj) break;
         const double* xv = xj + (long long)(j + u) * GB;
 #pragma unroll
-        for (int b = 0; b < GB; ++b) acc[b] = acc[b] + xv[b] * r[u];
+        for (int b = 0; b < GB; ++b) acc[b] = fma(xv[b], r[u], acc[b]);
       }
     }
   }
