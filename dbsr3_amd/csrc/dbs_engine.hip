@@ -453,17 +453,22 @@ __device__ inline int partner_row(int x, int nv, int onesided) {
   return (x & 1) ? x - 1 : -1;
 }
 
-__global__ void __launch_bounds__(256) k_contract_dense(
+// one workgroup per 16x16 tile; its DBS_DENSE_SPLIT wavefronts take every
+// DBS_DENSE_SPLIT-th 4-row step of the feature rows (split K, so a tile's
+// long feature sum runs on many SIMDs), loads issued in batches of 4 steps
+// ahead of their MFMAs; the partial tiles are added in wave order through
+// LDS.
+#define DBS_DENSE_SPLIT 16
+__global__ void __launch_bounds__(64 * DBS_DENSE_SPLIT) k_contract_dense(
     int ntiles, const int4* __restrict__ tiles, const KGroupDev* __restrict__ kgs,
     const InstDev* __restrict__ inst, const double* __restrict__ F, const int* __restrict__ pairrow, const long long* __restrict__ pairoff,
     const int* __restrict__ nextrow, const long long* __restrict__ row_out, int onesided,
     double* __restrict__ vals) {
   // tile = (group, Ltile, Rtile, 0); pairrow[pairoff[g] + l*ndens + r] = row or -1
-  const int waves = blockDim.x >> 6;
+  __shared__ dbl4 red[DBS_DENSE_SPLIT][64];
   const int lane = threadIdx.x & 63;
-  // wave index made scalar (uniform across the wave)
-  for (int t = blockIdx.x * waves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); t < ntiles;
-       t += gridDim.x * waves) {
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // scalar wave index
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const int4 T = tiles[t];
     const KGroupDev G = kgs[T.x];
     const InstDev I = inst[G.inst];
@@ -474,24 +479,37 @@ __global__ void __launch_bounds__(256) k_contract_dense(
     const int li = l0 + (lane & 15), ri = r0 + (lane & 15), kq = lane >> 4;
     const bool lv = li < nd, rv = ri < nd;
     dbl4 acc = {0.0, 0.0, 0.0, 0.0};
-    for (int x0 = 0; x0 < D; x0 += 4) {
-      const int x = x0 + kq;
-      const int xp = x < D ? partner_row(x, nv, onesided) : -1;
-      double a = (lv && xp >= 0) ? Fg[(long long)x * nd + li] : 0.0;
-      double b = (rv && xp >= 0) ? Fg[(long long)xp * nd + ri] : 0.0;
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    constexpr int STEP = 4 * DBS_DENSE_SPLIT;
+    for (int xb = 4 * w; xb < D; xb += 4 * STEP) {
+      double a[4], b[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int x = xb + u * STEP + kq;
+        const int xp = x < D ? partner_row(x, nv, onesided) : -1;
+        a[u] = (lv && xp >= 0) ? Fg[(long long)x * nd + li] : 0.0;
+        b[u] = (rv && xp >= 0) ? Fg[(long long)xp * nd + ri] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc, 0, 0, 0);
     }
-    // output layout of the 16x16x4 f64 MFMA (one output group per
-    // register): lane holds column lane%16 and rows 4*j + lane/16, j = 0..3
-    const int col = lane & 15;
-    for (int j = 0; j < 4; ++j) {
-      const int row = 4 * j + (lane >> 4);
-      const int L = l0 + row, R = r0 + col;
-      if (L < nd && R < nd) {
-        for (int rr = pairrow[pairoff[T.x] + (long long)L * nd + R]; rr >= 0; rr = nextrow[rr])
-          vals[row_out[rr]] = acc[j];
+    red[w][lane] = acc;
+    __syncthreads();
+    if (w == 0) {
+      dbl4 sum = red[0][lane];
+      for (int q = 1; q < DBS_DENSE_SPLIT; ++q) sum += red[q][lane];
+      // output layout of the 16x16x4 f64 MFMA (one output group per
+      // register): lane holds column lane%16 and rows 4*j + lane/16, j = 0..3
+      const int col = lane & 15;
+      for (int j = 0; j < 4; ++j) {
+        const int row = 4 * j + (lane >> 4);
+        const int L = l0 + row, R = r0 + col;
+        if (L < nd && R < nd) {
+          for (int rr = pairrow[pairoff[T.x] + (long long)L * nd + R]; rr >= 0; rr = nextrow[rr])
+            vals[row_out[rr]] = sum[j];
+        }
       }
     }
+    __syncthreads();  // red is rewritten by the next tile
   }
 }
 
@@ -2093,7 +2111,7 @@ static int run_forward(dbs_engine* e, int with_adj) {
     mark(e, "rows_reduce");
   }
   if (e->ntiles > 0) {
-    k_contract_dense<<<grid_for(e->ntiles, 4, cap * 4), 256, 0, s>>>(
+    k_contract_dense<<<grid_for(e->ntiles, 1, cap * 4), 64 * DBS_DENSE_SPLIT, 0, s>>>(
         e->ntiles, e->d_tiles, e->d_kg, e->d_inst, e->F, e->d_pairrow, e->d_pairoff, e->d_nextrow,
         e->d_row_out, e->onesided, e->vals);
     mark(e, "contract_dense");
