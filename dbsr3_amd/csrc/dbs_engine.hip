@@ -1404,16 +1404,25 @@ __global__ void __launch_bounds__(256) k_adj_orb_mfma(
 // read straight from V through the pair index d(a,c).  Items are laid out
 // XCD-major (workgroup b runs item (b&7)*per + (b>>3)) so the ks items of an
 // interval share one L2 for the interval's S/Mm rows.
-template <int U>
+template <int U, int SG>
 __global__ void __launch_bounds__(256) k_adj_orb_acc(
     int ninst, const long long* __restrict__ seg, const InstDev* __restrict__ inst,
     const KGroupDev* __restrict__ kgs, const double* __restrict__ orbv, const double* __restrict__ wt,
     const double* __restrict__ Gb, double* __restrict__ gorb, long long total, long long per,
-    const int* __restrict__ accf) {
-  constexpr int KSM = 10;
+    const int* __restrict__ accf, int slice) {
+  // SG threads per item: 256 (the workgroup, U*256 densities) or 64 (one
+  // wavefront per item, four items per workgroup, up to U*64 densities; the
+  // item's LDS is the wave's slice and its barriers are wave-level)
+  constexpr int KSM = 10, NSG = 256 / SG;
   extern __shared__ double sm[];
+  const int sg = SG == 256 ? 0 : __builtin_amdgcn_readfirstlane((int)threadIdx.x / SG);
+  const int tid = (int)threadIdx.x % SG;
+  const auto sync = [] {
+    if (SG == 256) __syncthreads();
+    else wave_lds_sync();
+  };
   const int xcd = blockIdx.x & 7;
-  for (long long j = blockIdx.x >> 3; j < per; j += gridDim.x >> 3) {
+  for (long long j = (long long)(blockIdx.x >> 3) * NSG + sg; j < per; j += (long long)(gridDim.x >> 3) * NSG) {
     const long long item = xcd * per + j;
     if (item >= total) break;
     const int s = find_seg(seg, ninst, item);
@@ -1424,10 +1433,10 @@ __global__ void __launch_bounds__(256) k_adj_orb_acc(
     const int norb = I.norb, ks = I.ks, np = I.np, nv = I.nv, nk = nv * ks;
     const int nd = norb * (norb + 1) / 2, FW = 2 * (ks + 1);
     const int QH = (ks + 2) / 2;             // points per half of the table
-    double* V = sm;                          // [QH][nd]: the points of one half
+    double* V = sm + (size_t)sg * slice;     // [QH][nd]: the points of one half
     double* FV = V + (size_t)QH * nd;        // [norb][FW]: column 2q + comp
-    __syncthreads();                         // the previous item's readers are done
-    for (int q = threadIdx.x; q < norb * FW; q += blockDim.x) {
+    sync();                                  // the previous item's readers are done
+    for (int q = tid; q < norb * FW; q += SG) {
       const int c = q / FW, col = q % FW, pt = col >> 1, comp = col & 1;
       const long long base = I.off_orbv + ((long long)(2 * c + comp) * nv + iv) * np;
       FV[q] = orbv[base + (pt == 0 ? m : ks + m * ks + pt - 1)];
@@ -1450,7 +1459,7 @@ __global__ void __launch_bounds__(256) k_adj_orb_acc(
       const double* gb = Gb + G.off_F;
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const int d = threadIdx.x + u * 256;
+        const int d = tid + u * SG;
         if (d < nd) {
           vo[u] += gb[2LL * iv * nd + d] * wkv + (gb[(2LL * iv + 1) * nd + d] + gb[xr * nd + d]) * wmv;
           const double gy = gb[(xr + 1) * nd + d];
@@ -1462,10 +1471,10 @@ __global__ void __launch_bounds__(256) k_adj_orb_acc(
     // the points in two halves (q0 = 0: points 0..QH-1, q0 = QH: the rest)
     for (int q0 = 0; q0 <= ks; q0 += QH) {
       const int nq = min(QH, ks + 1 - q0);
-      if (q0 > 0) __syncthreads();  // the first half's readers are done
+      if (q0 > 0) sync();  // the first half's readers are done
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const int d = threadIdx.x + u * 256;
+        const int d = tid + u * SG;
         if (d < nd) {
           if (q0 == 0) V[d] = vo[u];
 #pragma unroll
@@ -1475,10 +1484,10 @@ __global__ void __launch_bounds__(256) k_adj_orb_acc(
           }
         }
       }
-      __syncthreads();
+      sync();
       // outputs (orbital a, component comp, point q0 + q): consecutive
       // threads write the consecutive points of one orbital component
-      for (int t = threadIdx.x; t < nq * 2 * norb; t += blockDim.x) {
+      for (int t = tid; t < nq * 2 * norb; t += SG) {
         const int o = t / nq, q = t % nq, a = o >> 1, comp = o & 1, qq = q0 + q;
         const double* Vq = V + (size_t)q * nd;
         const double* f = FV + 2 * qq + comp;
@@ -1653,6 +1662,7 @@ struct dbs_engine {
   size_t adjm_shm = 0;       // LDS of the matrix-core adjoint kernel (0: not usable)
   int adjacc_u = 0;          // densities per thread of k_adj_orb_acc (0: no instance uses it)
   size_t adjacc_shm = 0;
+  int adjacc_sg = 256;       // threads per item of k_adj_orb_acc: 256, or 64 when every group has <= 64 densities
   bool adjacc_all = false;   // every instance uses k_adj_orb_acc
   int* d_accf = nullptr;     // per instance: 1 = adjoint by k_adj_orb_acc
   int featm_wave = 0;        // LDS doubles per wave of the matrix-core kernel
@@ -2133,16 +2143,22 @@ static int run_adjoint(dbs_engine* e) {
   mark(e, "adj_suffix_scan");
   if (e->adjacc_u > 0) {
     const long long per = (e->adj_items + 7) / 8;
-    const unsigned grid = 8u * (unsigned)std::min<long long>(per, std::max(1, cap * 4 / 8));
-    if (e->adjacc_u == 1)
-      k_adj_orb_acc<1><<<grid, 256, e->adjacc_shm, s>>>(e->ninst, e->d_seg_adj, e->d_inst, e->d_kg, e->orbv,
-                                                        e->wt, e->G, e->gorb, e->adj_items, per, e->d_accf);
+    const int nsg = e->adjacc_sg == 64 ? 4 : 1;  // items per workgroup
+    const unsigned grid = 8u * (unsigned)std::min<long long>((per + nsg - 1) / nsg, std::max(1, cap * 4 / 8));
+    const int sl = (int)(e->adjacc_shm / sizeof(double) / nsg);
+    const size_t shm = e->adjacc_shm;
+    if (e->adjacc_sg == 64)
+      k_adj_orb_acc<1, 64><<<grid, 256, shm, s>>>(e->ninst, e->d_seg_adj, e->d_inst, e->d_kg, e->orbv, e->wt,
+                                                  e->G, e->gorb, e->adj_items, per, e->d_accf, sl);
+    else if (e->adjacc_u == 1)
+      k_adj_orb_acc<1, 256><<<grid, 256, shm, s>>>(e->ninst, e->d_seg_adj, e->d_inst, e->d_kg, e->orbv, e->wt,
+                                                   e->G, e->gorb, e->adj_items, per, e->d_accf, sl);
     else if (e->adjacc_u == 2)
-      k_adj_orb_acc<2><<<grid, 256, e->adjacc_shm, s>>>(e->ninst, e->d_seg_adj, e->d_inst, e->d_kg, e->orbv,
-                                                        e->wt, e->G, e->gorb, e->adj_items, per, e->d_accf);
+      k_adj_orb_acc<2, 256><<<grid, 256, shm, s>>>(e->ninst, e->d_seg_adj, e->d_inst, e->d_kg, e->orbv, e->wt,
+                                                   e->G, e->gorb, e->adj_items, per, e->d_accf, sl);
     else
-      k_adj_orb_acc<4><<<grid, 256, e->adjacc_shm, s>>>(e->ninst, e->d_seg_adj, e->d_inst, e->d_kg, e->orbv,
-                                                        e->wt, e->G, e->gorb, e->adj_items, per, e->d_accf);
+      k_adj_orb_acc<4, 256><<<grid, 256, shm, s>>>(e->ninst, e->d_seg_adj, e->d_inst, e->d_kg, e->orbv, e->wt,
+                                                   e->G, e->gorb, e->adj_items, per, e->d_accf, sl);
   }
   if (e->adjm_shm > 0) {
     if (!e->adjacc_all)
@@ -2630,12 +2646,13 @@ int dbs_engine_create(dbs_engine** out, int ninst, dbs_ctx* const* ctxs, const i
       const int U = maxnd <= 256 ? 1 : (maxnd <= 512 ? 2 : 4);
       if (aneed2 * sizeof(double) > (size_t)lmx) { std::fill(accf.begin(), accf.end(), 0); nacc = 0; }
       e->adjacc_u = nacc > 0 ? U : 0;
+      e->adjacc_sg = maxnd <= 64 && 4 * aneed2 * sizeof(double) <= (size_t)lmx ? 64 : 256;
       e->adjacc_all = nacc == ninst;
-      e->adjacc_shm = e->adjacc_u ? aneed2 * sizeof(double) : 0;
+      e->adjacc_shm = e->adjacc_u ? aneed2 * sizeof(double) * (e->adjacc_sg == 64 ? 4 : 1) : 0;
       rc |= upload(s, &e->d_accf, accf);
       if (e->adjacc_shm > 64 * 1024) {
-        const void* fs[] = {(const void*)k_adj_orb_acc<1>, (const void*)k_adj_orb_acc<2>,
-                            (const void*)k_adj_orb_acc<4>};
+        const void* fs[] = {(const void*)k_adj_orb_acc<1, 64>, (const void*)k_adj_orb_acc<1, 256>,
+                            (const void*)k_adj_orb_acc<2, 256>, (const void*)k_adj_orb_acc<4, 256>};
         for (const void* f : fs)
           if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)e->adjacc_shm) != hipSuccess)
             rc |= fail(DBS_ERR_HIP, "dbs_engine_create: LDS size of the register-summed adjoint");
