@@ -52,7 +52,7 @@ PHASE_KERNEL = {
     "prefix_scan": "k_scan_cols", "pair_panel": "{pair}<{w}, {v}, true>",
     "contract_rows": "{pair}<{w}, {v}, false>", "rows_reduce": "k_rows_reduce",
     "contract_dense": "k_contract_dense", "energy": "k_energy",
-    "adj_suffix_scan": "k_scan_cols", "adj_orb_points": "k_adj_orb_acc<2>", "adj_coef": "k_adj_proj",
+    "adj_suffix_scan": "k_scan_cols", "adj_orb_points": "k_adj_orb_acc<2, 256>", "adj_coef": "k_adj_proj",
 }
 # rocprofv3 FETCH_SIZE/WRITE_SIZE summary of this workload (HBM bytes per
 # dispatch, copy-kernel scaled); rows "| kernel | tuning | ... | bytes |"
