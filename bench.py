@@ -346,7 +346,7 @@ def run_config2(args):
                             "blocks on the device; one step = every list once",
                 "entries": nent, "merged_entries": int(nl), "lists": len(lists), "ns": ns, "channels": nch}
     roof = {"bound": "hbm", "achieved": None, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": None,
-            "traffic": None, "note": "many small launches per list; see the plain-path kernels"}
+            "traffic": None, "note": "per list: group densities, two convolutions (k_group_convol3_tile dominates), UPDATE_HS; kernel times from rocprofv3 --kernel-trace"}
     return config_line(2, value, "list entries/s", ms, args.steps, 1, workload,
                        {"roofline": roof, "cpu_baseline": cpu})
 
