@@ -49,8 +49,8 @@ SEED = 20261203 + 4
 # pair-panel width)
 PHASE_KERNEL = {
     "orb_points": "k_orb_values", "coef_gather": "k_adj_cgather", "features": "k_features_mfma",
-    "prefix_scan": "k_scan_cols", "pair_panel": "{pair}<{w}, {v}, true>",
-    "contract_rows": "{pair}<{w}, {v}, false>", "rows_reduce": "k_rows_reduce",
+    "prefix_scan": "k_scan_cols", "pair_panel": "{pair}<{w}, {v}, true{d}>",
+    "contract_rows": "{pair}<{w}, {v}, false{d}>", "rows_reduce": "k_rows_reduce",
     "contract_dense": "k_contract_dense", "energy": "k_energy",
     "adj_suffix_scan": "k_scan_cols", "adj_orb_points": "k_adj_orb_acc<2, 256>", "adj_coef": "k_adj_proj",
 }
@@ -62,31 +62,41 @@ TRAFFIC_FILE = "engine_rk_fwd_adj_ns200_norb30_rows5e4_pmc.md"
 
 def pair_kernel(rows, tuning):
     """the pair-panel kernel the engine launches for this row set (total
-    densities of each multipole; one density per thread when they fit a
-    workgroup of up to 512)"""
+    densities of each multipole): the rotated kernel when the largest group
+    fits its workgroup (512 threads, 1024 at W = 4) at one density per
+    thread, or at two for W < 16; returns (name, ONE argument, trailing
+    template arguments)"""
     nd = 0
     for k in np.unique(rows[:, 0]):
         r = rows[rows[:, 0] == k]
         pairs = {(min(a, c), max(a, c)) for a, c in r[:, [1, 3]]} | {(min(b, d), max(b, d)) for b, d in r[:, [2, 4]]}
         nd = max(nd, len(pairs))
     w = tuning[2]
-    if w >= 8 and nd <= 512:
-        return "k_pair_rot", "false"
-    return "k_pair_panel", "true"
+    tmax = 1024 if w == 4 else 512
+    if nd <= (tmax if w >= 16 else 2 * tmax):
+        return "k_pair_rot", "false", ", 1" if nd <= tmax else ", 2"
+    return "k_pair_panel", "true", ""
 
 
 def measured_traffic(path, kernel, tuning):
-    """HBM bytes per launch of `kernel` (name with its template arguments)
-    from the committed counter summary of this workload, else None."""
+    """(HBM bytes per launch of `kernel`, the engine tuning they were counted
+    with) from the committed counter summary of this workload: the row of
+    the same tuning, else a row of the same kernel (the name carries the
+    panel width) under another tuning; (None, None) when there is none."""
+    tune = ",".join(str(int(x)) for x in tuning)
+    other = (None, None)
     try:
         with open(path) as f:
             for line in f:
                 cells = [c.strip() for c in line.strip().strip("|").split("|")]
                 if len(cells) >= 6 and cells[0] == kernel:
-                    return float(cells[-1])
+                    if cells[1] == tune:
+                        return float(cells[-1]), tune
+                    if other[0] is None:
+                        other = (float(cells[-1]), cells[1])
     except (OSError, ValueError):
         pass
-    return None
+    return other
 
 
 def make_workload(ns=200, ks=9, ksp=8, ksq=9, norb=30, kmax=8, nrows=50000, seed=SEED):
@@ -533,9 +543,9 @@ def main():
         roof = {"bound": "hbm", "achieved": dby / (dms * 1e-3) / 1e9, "peak": PEAK_HBM_GBS,
                 "unit": "GB/s"}
     roof["frac"] = roof["achieved"] / roof["peak"]
-    pk, pv = pair_kernel(rows, tuning)
-    kname = PHASE_KERNEL.get(dom, dom).format(w=tuning[2], pair=pk, v=pv)
-    roof["traffic"] = measured_traffic(args.traffic_file, kname, tuning)
+    pk, pv, pd = pair_kernel(rows, tuning)
+    kname = PHASE_KERNEL.get(dom, dom).format(w=tuning[2], pair=pk, v=pv, d=pd)
+    roof["traffic"], roof["traffic_tuning"] = measured_traffic(args.traffic_file, kname, tuning)
     roof["kernel"] = kname
     roof["phase"] = dom
     roof["kernel_ms"] = dms
