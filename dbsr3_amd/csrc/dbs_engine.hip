@@ -323,8 +323,11 @@ __global__ void __launch_bounds__(256) k_features_mfma(
     const int npt = q < ks ? ks + 1 : ks;  // Y(q): nested points of q, then the outer point q
     double* Wa = V + 2 * npt * NO;         // [ng][npt] weight of each point; W[ks] = Wm(q)
     wave_lds_sync();                        // the previous item's reads are done
+    // e = oc*npt + lp split with a float reciprocal: exact for e < 2^12,
+    // npt <= 17 ((e + 0.5)/npt is at least 0.5/npt away from an integer)
+    const float rnpt = 1.0f / (float)npt;
     for (int e = lane; e < 2 * NO * npt; e += 64) {
-      const int lp = e % npt, oc = e / npt;  // oc = 2*orb + comp
+      const int oc = (int)(((float)e + 0.5f) * rnpt), lp = e - oc * npt;  // oc = 2*orb + comp
       const int o = oc >> 1, comp = oc & 1;
       const int p = q < ks ? (lp < ks ? ks + q * ks + lp : q) : lp;
       V[vsw(comp, lp, o, npt, NO)] = o < norb ? orbv[I.off_orbv + ((long long)oc * nv + iv) * np + p] : 0.0;
@@ -341,53 +344,74 @@ __global__ void __launch_bounds__(256) k_features_mfma(
     wave_lds_sync();
     const int ntl = NO / 16;
     const bool sym = kind != DK_PQ;
-    for (int gi = 0; gi < ng; ++gi) {
-    const KGroupDev G = kgs[g0 + gi];
-    const int nd = G.ndens;
-    const double* W = Wa + gi * npt;
-    const int slot = __popc(G.gmask & ((1 << kind) - 1));
-    const int* pt = pidx + G.off_pidx + (long long)slot * NO * NO;
-    double* Fg = F + G.off_F;
-    long long xrow;
-    if (q < ks) xrow = 2LL * (nv + iv * ks + q) + 1;  // Y(iv,q); Ro(iv,q) is the row before
-    else xrow = 2LL * iv + (q - ks);                   // M^k (the S slot), M^-(k+1)
-    // components of the A (row orbital) and B (column orbital) operands
+    const int npass = q < ks ? 2 : 1;
     for (int tt = 0; tt < ntl * ntl; ++tt) {
       const int ta = tt / ntl, tc = tt % ntl;
       if (sym && ta > tc) continue;  // densities of symmetric kinds have a <= c
       const int ra = ta * 16 + (lane & 15), cb = tc * 16 + (lane & 15);
-      for (int pass = 0; pass < (q < ks ? 2 : 1); ++pass) {
-        // pass 0: the Gram over the ks points; pass 1 (Y items): Ro over the outer point
-        const int K = pass == 0 ? (kind == DK_TOT ? 2 * ks : ks) : (kind == DK_TOT ? 2 : 1);
-        dbl4 acc = {0.0, 0.0, 0.0, 0.0};
-        for (int k0 = 0; k0 < K; k0 += 4) {
-          // lanes past K read an in-range entry and contribute zero (no
-          // divergent branch around the matrix-core chain)
-          const int kk = k0 + (lane >> 4);
-          const bool ok = kk < K;
-          const int kc = ok ? kk : 0;
-          const int per = pass == 0 ? ks : 1;
-          int ca, cbp, lp;
-          if (kind == DK_TOT) { ca = cbp = kc >= per ? 1 : 0; lp = kc - ca * per; }
-          else if (kind == DK_PP) { ca = cbp = 0; lp = kc; }
-          else if (kind == DK_QQ) { ca = cbp = 1; lp = kc; }
-          else { ca = 0; cbp = 1; lp = kc; }
-          if (pass == 1) lp = ks;  // the outer point
-          const double a = W[lp] * V[vsw(ca, lp, ra, npt, NO)];
-          const double b = V[vsw(cbp, lp, cb, npt, NO)];
-          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(ok ? a : 0.0, ok ? b : 0.0, acc, 0, 0, 0);
-        }
-        // lane holds column cb and rows ta*16 + 4j + lane/16
-        const long long x = pass == 0 ? xrow : xrow - 1;
+      // this lane's operand offsets for every k-step, the same for all
+      // groups of the item: A = V[ao]*W[wl], B = V[bo]; pass 0 the Gram over
+      // the ks points (up to 2ks = 32 entries: 8 steps), pass 1 (Y items)
+      // Ro over the outer point (1 step); lanes past K contribute zero
+      constexpr int MAXS = (2 * DBS_MAXK + 3) / 4;
+      int ao[MAXS + 1], bo[MAXS + 1], wl[MAXS + 1];
+      bool okk[MAXS + 1];
+      const int K0 = kind == DK_TOT ? 2 * ks : ks, ns0 = (K0 + 3) / 4;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const int row = ta * 16 + 4 * j + (lane >> 4);
-          const int d = pt[row * NO + cb];
-          if (d >= 0) Fg[x * nd + d] = acc[j];
-        }
+      for (int st = 0; st <= MAXS; ++st) {
+        const bool p1 = st == MAXS;  // the pass-1 step
+        const int K = p1 ? (kind == DK_TOT ? 2 : 1) : K0;
+        const int kk = (p1 ? 0 : 4 * st) + (lane >> 4);
+        const bool ok = kk < K;
+        const int kc = ok ? kk : 0;
+        const int per = p1 ? 1 : ks;
+        int ca, cbp, lp;
+        if (kind == DK_TOT) { ca = cbp = kc >= per ? 1 : 0; lp = kc - ca * per; }
+        else if (kind == DK_PP) { ca = cbp = 0; lp = kc; }
+        else if (kind == DK_QQ) { ca = cbp = 1; lp = kc; }
+        else { ca = 0; cbp = 1; lp = kc; }
+        if (p1) lp = q < ks ? ks : 0;  // the outer point (only Y items run pass 1)
+        ao[st] = vsw(ca, lp, ra, npt, NO);
+        bo[st] = vsw(cbp, lp, cb, npt, NO);
+        wl[st] = lp;
+        okk[st] = ok;
       }
-    }
-    }  // groups of the item
+      for (int gi = 0; gi < ng; ++gi) {
+        const KGroupDev G = kgs[g0 + gi];
+        const int nd = G.ndens;
+        const double* W = Wa + gi * npt;
+        const int slot = __popc(G.gmask & ((1 << kind) - 1));
+        const int* pt = pidx + G.off_pidx + (long long)slot * NO * NO;
+        double* Fg = F + G.off_F;
+        long long xrow;
+        if (q < ks) xrow = 2LL * (nv + iv * ks + q) + 1;  // Y(iv,q); Ro(iv,q) is the row before
+        else xrow = 2LL * iv + (q - ks);                   // M^k (the S slot), M^-(k+1)
+        for (int pass = 0; pass < npass; ++pass) {
+          dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+          if (pass == 0) {
+#pragma unroll
+            for (int st = 0; st < MAXS; ++st) {
+              if (st >= ns0) break;
+              const double a = W[wl[st]] * V[ao[st]];
+              const double b = V[bo[st]];
+              acc = __builtin_amdgcn_mfma_f64_16x16x4f64(okk[st] ? a : 0.0, okk[st] ? b : 0.0, acc, 0, 0, 0);
+            }
+          } else {
+            const double a = W[wl[MAXS]] * V[ao[MAXS]];
+            const double b = V[bo[MAXS]];
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(okk[MAXS] ? a : 0.0, okk[MAXS] ? b : 0.0, acc, 0, 0, 0);
+          }
+          // lane holds column cb and rows ta*16 + 4j + lane/16
+          const long long x = pass == 0 ? xrow : xrow - 1;
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const int row = ta * 16 + 4 * j + (lane >> 4);
+            const int d = pt[row * NO + cb];
+            if (d >= 0) Fg[x * nd + d] = acc[j];
+          }
+        }
+      }  // groups of the item
+    }    // tiles
   }
 }
 
