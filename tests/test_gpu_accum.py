@@ -73,6 +73,14 @@ def test_r_data_channel_blocks(setup):
     _check_r_data(g, d)
 
 
+def test_r_data_many_blocks(setup):
+    """9 channels (45 blocks): lists of more groups than one convolution
+    chunk holds (16 band groups, 8 exchange groups), with a partial last
+    chunk"""
+    g, d = setup
+    _check_r_data(g, d, nch=9, nent=900, min_groups=17)
+
+
 def test_r_data_wide_band(gpu_available):
     """ks = 10: the exchange convolution takes its untiled kernel (the LDS
     tiles hold halos for ks <= 9 only)"""
@@ -83,11 +91,11 @@ def test_r_data_wide_band(gpu_available):
     _check_r_data(O.Grid(t, ns, ks, ks - 1, ks), DBS(t, ns, ks, ks - 1, ks))
 
 
-def _check_r_data(g, d):
+def _check_r_data(g, d, nch=4, nent=400, min_groups=1):
     from dbsr3_amd import CData
     from dbsr3_amd.cdata import r_data_34
     from dbsr3_amd.dbs import device_to_fortran, fortran_to_device
-    nbound, nch = 5, 4
+    nbound = 5
     norb = nbound + nch
     fs = random_orbitals(g, nbound, seed=33)
     pq = np.stack(fs, axis=2)
@@ -104,7 +112,7 @@ def _check_r_data(g, d):
             icc[ich - 1, jch - 1] = nblk
     lists = []
     cd = CData(4, 0, 3, 300, 40, 5, 1e-12, lambda it, k, l: lists.append((it, k, l)))
-    for _ in range(400):
+    for _ in range(nent):
         k = int(rng.integers(0, 4))
         ch = [int(x) for x in rng.integers(nbound + 1, norb + 1, 2)]
         bd = [int(x) for x in rng.integers(1, nbound + 1, 2)]
@@ -122,14 +130,17 @@ def _check_r_data(g, d):
     hch_dev = fortran_to_device(hch_ref)
     pq_dev = fortran_to_device(pq)
     n_done = 0
+    ng_max = 0  # most channel blocks (groups) in one list
     for it, k, lst in lists:
         if it not in (3, 4) or len(lst[0]) == 0:
             continue
+        ng_max = max(ng_max, len(np.unique(np.asarray(lst[1]))))
         for atype in range(4):
             O.rdata_34(g, it, atype, k, lst[:4], pq, IBI, icc, hch_ref)
             r_data_34(d, it, atype, k, lst, pq_dev, nbound, icc, hch_dev, nblk)
             n_done += 1
     assert n_done >= 4
+    assert ng_max >= min_groups, ng_max
     hch = device_to_fortran(hch_dev)
     s = np.max(np.abs(hch_ref))
     assert s > 0
