@@ -968,12 +968,18 @@ int dbs_rdata_34_dev(dbs_ctx* ctx, int itype, int atype, int k, int nc, const do
   int* d_j1 = d_goff + goff.size();
   int* d_j2 = d_j1 + ej1.size();
   int4* d_gi = reinterpret_cast<int4*>(ctx->scratch + ((off_g + 1) & ~(size_t)1));
-  auto put = [&](void* dst, const void* src, size_t bytes) {
-    return bytes == 0 || hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s) == hipSuccess;
+  // the tables in the scratch layout from off_c on, one upload
+  std::vector<double> blob(need - off_c, 0.0);
+  char* hb = reinterpret_cast<char*>(blob.data());
+  auto stage = [&](const void* dev, const void* src, size_t bytes) {
+    if (bytes) std::memcpy(hb + (reinterpret_cast<const char*>(dev) - reinterpret_cast<const char*>(d_c)), src, bytes);
   };
-  if (!put(d_c, ec.data(), nce * sizeof(double)) || !put(d_goff, goff.data(), goff.size() * sizeof(int)) ||
-      !put(d_j1, ej1.data(), ej1.size() * sizeof(int)) || !put(d_j2, ej2.data(), ej2.size() * sizeof(int)) ||
-      !put(d_gi, ginfo.data(), ginfo.size() * sizeof(int4)))
+  stage(d_c, ec.data(), nce * sizeof(double));
+  stage(d_goff, goff.data(), goff.size() * sizeof(int));
+  stage(d_j1, ej1.data(), ej1.size() * sizeof(int));
+  stage(d_j2, ej2.data(), ej2.size() * sizeof(int));
+  stage(d_gi, ginfo.data(), ginfo.size() * sizeof(int4));
+  if (hipMemcpyAsync(d_c, blob.data(), blob.size() * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess)
     return fail(DBS_ERR_HIP, "R_data: upload");
   k_group_density<<<grid_of(per * ng, 256), 256, 0, s>>>(ng, d_goff, d_j1, d_j2, d_c, pq_dev, ns, ks, jp1, jp2,
                                                           sym, sym == 0 ? CV1_GB : 0, d_xx);
