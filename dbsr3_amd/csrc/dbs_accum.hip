@@ -248,7 +248,7 @@ __global__ void k_accumulate_targets(int ntarget, const long long* __restrict__ 
 // xx_g = sum_e c_e * density(pq(jp1,j1), pq(jp2,j2)) in list order;
 // sym 0: band 's' (ns,ks); 1: full 'x' (ns,ns). tgb = 0: group-major
 // xx[g][r]; tgb > 0: chunks of tgb groups with the group index fastest,
-// xx[g/tgb][r][g%tgb] (the last chunk's missing groups are never written)
+// xx[g/tgb][r][g%tgb], the last chunk's missing groups written as zeros
 __global__ void k_group_density(int ng, const int* __restrict__ goff, const int* __restrict__ ej1,
                                 const int* __restrict__ ej2, const double* __restrict__ ec,
                                 const double* __restrict__ pq, int ns, int ks, int jp1, int jp2,
@@ -256,14 +256,16 @@ __global__ void k_group_density(int ng, const int* __restrict__ goff, const int*
 #pragma clang fp contract(off)
   const int ncol = sym == 0 ? ks : ns;
   const long long per = (long long)ns * ncol;
-  const long long total = per * ng;
+  const int ngw = tgb > 0 ? (ng + tgb - 1) / tgb * tgb : ng;  // groups written
+  const long long total = per * ngw;
   for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
        q += (long long)gridDim.x * blockDim.x) {
     const int g = (int)(q / per);
     const int r = (int)(q % per);
     const int i = r % ns, j = r / ns;
     double acc = 0.0;
-    for (int e = goff[g]; e < goff[g + 1]; ++e) {
+    const int e0 = g < ng ? goff[g] : 0, e1 = g < ng ? goff[g + 1] : 0;
+    for (int e = e0; e < e1; ++e) {
       const double* p1 = pq + (long long)ns * ((jp1 - 1) + 2LL * (ej1[e] - 1));
       const double* p2 = pq + (long long)ns * ((jp2 - 1) + 2LL * (ej2[e] - 1));
       double d;
@@ -981,7 +983,7 @@ int dbs_rdata_34_dev(dbs_ctx* ctx, int itype, int atype, int k, int nc, const do
   stage(d_gi, ginfo.data(), ginfo.size() * sizeof(int4));
   if (hipMemcpyAsync(d_c, blob.data(), blob.size() * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess)
     return fail(DBS_ERR_HIP, "R_data: upload");
-  k_group_density<<<grid_of(per * ng, 256), 256, 0, s>>>(ng, d_goff, d_j1, d_j2, d_c, pq_dev, ns, ks, jp1, jp2,
+  k_group_density<<<grid_of(per * (sym == 0 ? (ng + CV1_GB - 1) / CV1_GB * CV1_GB : ng), 256), 256, 0, s>>>(ng, d_goff, d_j1, d_j2, d_c, pq_dev, ns, ks, jp1, jp2,
                                                           sym, sym == 0 ? CV1_GB : 0, d_xx);
   if (sym == 0) {
     constexpr int GB = CV1_GB;
