@@ -143,6 +143,33 @@ def cpu_baseline(t, ns, ks, ksp, ksq, pq, rows, budget_s):
     return n, t1 - t0, order[:n], vals
 
 
+def cpu_baseline_ref(t, ns, ks, ksp, ksq, pq, rows, budget_s):
+    """The reference's own rk() (DBSR_HF/dbsr_lib_dbs.f90:3981-4026, compiled
+    from the reference's Fortran into oracle/_ref): per row four densities,
+    four convolutions and four SUM_AmB, the arrays of each k built by mrk_*
+    on first use; rows in multipole order, single thread, until the time
+    budget is spent.  None when the reference build is absent."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    if not O.ref_available():
+        return None
+    R = O.ref()
+    tt = O.f64(t)
+    R.ref_setup(ns, ks, ksp, ksq, O._d(tt))
+    fs = [np.asfortranarray(pq[:, :, o]) for o in range(pq.shape[2])]
+    order = np.argsort(rows[:, 0], kind="stable")
+    vals = []
+    t0 = time.perf_counter()
+    for r in rows[order]:
+        k, a, b, c, d = (int(x) for x in r)
+        vals.append(R.ref_rk(O._d(fs[a - 1]), O._d(fs[b - 1]), O._d(fs[c - 1]), O._d(fs[d - 1]), k))
+        if time.perf_counter() - t0 > budget_s:
+            break
+    t1 = time.perf_counter()
+    n = len(vals)
+    return n, t1 - t0, order[:n], np.array(vals)
+
+
 def roofline_of(prof):
     """the dominant phase of an engine profile and its roofline (bytes and
     flops from the engine's algorithmic counts, time from HIP events)"""
@@ -569,14 +596,30 @@ def main():
 
     cpu = None
     if not args.no_cpu:
+        # the port: the oracle's restatement, convolutions shared by rows of
+        # one left density (stronger than the reference's own rk)
         n, secs, sel, cvals = cpu_baseline(t, ns, ks, ksp, ksq, pq, rows, args.cpu_seconds)
         rel = np.abs(vals[sel] - cvals) / np.maximum(np.abs(cvals), 1e-3 * np.max(np.abs(cvals)))
         kk = sorted(set(int(x) for x in rows[sel, 0]))
         cpu = {"value": n / secs, "unit": "R^k integrals/s", "cores": 1, "kind": "port",
                "sample": f"first {n} of the rows in multipole order (k={kk[0]}..{kk[-1]}; "
-                         f"R^k arrays of each k built inside the timed region), "
-                         f"{secs:.1f} s single-thread"}
+                         f"R^k arrays of each k built inside the timed region; convolutions "
+                         f"shared by the rows of one left density), {secs:.1f} s single-thread"}
         out_extra["max_rel_err_vs_cpu_oracle"] = float(rel.max())
+        out_extra["cpu_port_integrals_per_s"] = cpu["value"]
+        out_extra["batched_vs_cpu_port"] = value / cpu["value"]
+        # the reference's own rk() from oracle/_ref, when built: the baseline
+        ref = cpu_baseline_ref(t, ns, ks, ksp, ksq, pq, rows, args.cpu_seconds / 2)
+        if ref is not None:
+            n, secs, sel, rvals = ref
+            rel = np.abs(vals[sel] - rvals) / np.maximum(np.abs(rvals), 1e-3 * np.max(np.abs(rvals)))
+            kk = sorted(set(int(x) for x in rows[sel, 0]))
+            cpu = {"value": n / secs, "unit": "R^k integrals/s", "cores": 1, "kind": "reference",
+                   "sample": f"the reference's rk() (DBSR_HF/dbsr_lib_dbs.f90, compiled into oracle/_ref) "
+                             f"on the first {n} of the rows in multipole order (k={kk[0]}..{kk[-1]}; "
+                             f"arrays of each k built by mrk_* inside the timed region), "
+                             f"{secs:.1f} s single-thread"}
+            out_extra["max_rel_err_vs_cpu_reference"] = float(rel.max())
         out_extra["batched_vs_cpu"] = value / cpu["value"]
 
     line = {
