@@ -170,6 +170,23 @@ def cpu_baseline_ref(t, ns, ks, ksp, ksq, pq, rows, budget_s):
     return n, t1 - t0, order[:n], np.array(vals)
 
 
+def with_reference(cpu, t, ns, ks, ksp, ksq, pq, rows, budget_s, note=""):
+    """(baseline, port): the reference's own rk() on a sample of the rows
+    when oracle/_ref is built, with the port's figure `cpu` kept beside it;
+    (cpu, None) otherwise"""
+    if cpu is None:
+        return cpu, None
+    ref = cpu_baseline_ref(t, ns, ks, ksp, ksq, pq, rows, budget_s)
+    if ref is None or ref[0] == 0:
+        return cpu, None
+    n, secs, sel, _ = ref
+    kk = sorted(set(int(x) for x in rows[sel, 0]))
+    return ({"value": n / secs, "unit": cpu["unit"], "cores": 1, "kind": "reference",
+             "sample": f"the reference's rk() (DBSR_HF/dbsr_lib_dbs.f90, compiled into oracle/_ref) on the "
+                       f"first {n} rows in multipole order (k={kk[0]}..{kk[-1]}; arrays of each k built by "
+                       f"mrk_* inside the timed region), {secs:.1f} s single-thread{note}"}, cpu)
+
+
 def roofline_of(prof):
     """the dominant phase of an engine profile and its roofline (bytes and
     flops from the engine's algorithmic counts, time from HIP events)"""
@@ -297,8 +314,10 @@ def run_config1(args):
                             "forward only, one hipGraph per step; value: R^k on the recipe grid",
                 "rows": int(len(rows)), "norb": norb, "kmax": kmax, "grids": {"recipe": grids["recipe"][1],
                                                                              "fe": 113}}
+    cpu, port = with_reference(cpu, t, ns, 9, 8, 9, pq, rows[np.argsort(rows[:, 0], kind="stable")],
+                               args.cpu_seconds / 2)
     return config_line(1, value, "R^k integrals/s", ms, args.steps, args.warmup, workload,
-                       {"sets": rates, "roofline": roof, "cpu_baseline": cpu})
+                       {"sets": rates, "roofline": roof, "cpu_baseline": cpu, "cpu_port": port})
 
 
 def run_config2(args):
@@ -481,8 +500,10 @@ def run_config5(args):
     workload = {"workload": "SCF-style: ns=600, ks=9/8/9, 60 orbitals, k=0..12, 1e6 rows; 20 iterations of "
                             "new orbitals + one forward+adjoint hipGraph replay (default tuning)",
                 "rows": nrows, "ns": ns, "norb": norb, "kmax": kmax, "iterations": 20, "tuning": tuning}
+    cpu, port = with_reference(cpu, t, ns, ks, ksp, ksq, pq0, rows, args.cpu_seconds / 2,
+                               "; the GPU figure is forward+adjoint")
     return config_line(5, value, "R^k integrals/s", ms, 20, args.warmup, workload,
-                       {"roofline": roof, "cpu_baseline": cpu})
+                       {"roofline": roof, "cpu_baseline": cpu, "cpu_port": port})
 
 
 def main():
