@@ -430,7 +430,11 @@ def run_config3(args):
     for i, h in enumerate(host):
         eng.set_orbitals(i, h[3])
         eng.set_coef(i, h[5])
-    tuning = eng.autotune(with_adjoint=True, reps=2)
+    if args.tuning:  # a fixed tuning (profiling runs without the autotuner's trials)
+        eng.set_tuning([int(x) for x in args.tuning.split(",")])
+        tuning = eng.get_tuning()
+    else:
+        tuning = eng.autotune(with_adjoint=True, reps=2)
     ms = time_graph(eng, True, args.steps, args.warmup)
     value = nrows / (ms * 1e-3)
     roof, _ = roofline_of(eng.profile(with_adjoint=True, reps=3))
