@@ -600,7 +600,7 @@ __global__ void k_energy(int nchunks, const int2* __restrict__ chunks, const lon
 // (workgroups of at most 256 threads); W <= 8 runs 512-thread workgroups
 template <int W>
 constexpr int pair_block_max() { return W >= 16 ? 256 : 512; }
-constexpr int kFeatGroups = 4;  // groups per item of k_features_mfma
+constexpr int kFeatGroups = 8;  // groups per item of k_features_mfma
 
 template <int W, bool LDS, bool ADJ>
 __global__ void __launch_bounds__(pair_block_max<W>()) k_pair_panel(
