@@ -600,7 +600,7 @@ __global__ void k_energy(int nchunks, const int2* __restrict__ chunks, const lon
 // (workgroups of at most 256 threads); W <= 8 runs 512-thread workgroups
 template <int W>
 constexpr int pair_block_max() { return W >= 16 ? 256 : 512; }
-constexpr int kFeatGroups = 8;  // groups per item of k_features_mfma
+constexpr int kFeatGroups = 16;  // most groups per item of k_features_mfma (LDS sizing)
 
 template <int W, bool LDS, bool ADJ>
 __global__ void __launch_bounds__(pair_block_max<W>()) k_pair_panel(
@@ -2593,26 +2593,34 @@ int dbs_engine_create(dbs_engine** out, int ninst, dbs_ctx* const* ctxs, const i
         }
       }
       // runs of consecutive groups of one instance with the kind, at most
-      // kFeatGroups per item, spread evenly
-      for (int i = 0; i < ninst; ++i) {
-        const InstDev& I = e->h_inst[i];
-        for (int kd = 0; kd < 4; ++kd) {
-          int gi = 0;
-          while (gi < I.nkg) {
-            if (!((e->h_kg[I.kg0 + gi].gmask >> kd) & 1)) { ++gi; continue; }
-            int ge = gi;
-            while (ge < I.nkg && ((e->h_kg[I.kg0 + ge].gmask >> kd) & 1)) ++ge;
-            const int run = ge - gi, nch = (run + kFeatGroups - 1) / kFeatGroups;
-            for (int c = 0; c < nch; ++c) {
-              const int a = gi + run * c / nch, b = gi + run * (c + 1) / nch;
-              for (int iv = 0; iv < I.nv; ++iv)
-                for (int q = 0; q < I.ks + 2; ++q)
-                  fit.push_back(make_int4(I.kg0 + a, iv, kd | ((b - a) << 8), q));
+      // fg per item, spread evenly: fg = 16 (the item's staged orbital
+      // values shared by more groups) while that leaves at least 4 items per
+      // SIMD (1024 SIMDs), else 8
+      auto make_items = [&](int fg, std::vector<int4>* out) {
+        long long n = 0;
+        for (int i = 0; i < ninst; ++i) {
+          const InstDev& I = e->h_inst[i];
+          for (int kd = 0; kd < 4; ++kd) {
+            int gi = 0;
+            while (gi < I.nkg) {
+              if (!((e->h_kg[I.kg0 + gi].gmask >> kd) & 1)) { ++gi; continue; }
+              int ge = gi;
+              while (ge < I.nkg && ((e->h_kg[I.kg0 + ge].gmask >> kd) & 1)) ++ge;
+              const int run = ge - gi, nch = (run + fg - 1) / fg;
+              n += (long long)nch * I.nv * (I.ks + 2);
+              for (int c = 0; out && c < nch; ++c) {
+                const int a = gi + run * c / nch, b = gi + run * (c + 1) / nch;
+                for (int iv = 0; iv < I.nv; ++iv)
+                  for (int q = 0; q < I.ks + 2; ++q)
+                    out->push_back(make_int4(I.kg0 + a, iv, kd | ((b - a) << 8), q));
+              }
+              gi = ge;
             }
-            gi = ge;
           }
         }
-      }
+        return n;
+      };
+      make_items(make_items(kFeatGroups, nullptr) >= 4096 ? kFeatGroups : kFeatGroups / 2, &fit);
       e->nfitems = (int)fit.size();
       rc |= upload(s, &e->d_fitems, fit);
       rc |= upload(s, &e->d_pidx, pidx);
