@@ -291,7 +291,7 @@ __global__ void k_group_density(int ng, const int* __restrict__ goff, const int*
 // in that XCD's L2). a_g(i,ip) = sum_jp sum_j xx_g(j,jp) rkb(i,j,ip,jp),
 // summed over jp, then j (convol case 1, dbsr_lib_dbs.f90:2249-2253); outputs with
 // i >= ns-ip are 0; fused multiply-adds. Needs a grid that is a multiple of 8.
-constexpr int CV1_GB = 16;
+constexpr int CV1_GB = 4;
 
 template <int GB>
 __global__ __launch_bounds__(256) void k_group_convol1_u(int ng, int ns, int ks, int nqb,

@@ -75,7 +75,7 @@ def test_r_data_channel_blocks(setup):
 
 def test_r_data_many_blocks(setup):
     """9 channels (45 blocks): lists of more groups than one convolution
-    chunk holds (16 band groups, 8 exchange groups), with a partial last
+    chunk holds (4 band groups, 8 exchange groups), with a partial last
     chunk"""
     g, d = setup
     _check_r_data(g, d, nch=9, nent=900, min_groups=17)
