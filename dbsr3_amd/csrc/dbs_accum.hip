@@ -993,7 +993,7 @@ int dbs_rdata_34_dev(dbs_ctx* ctx, int itype, int atype, int k, int nc, const do
     const unsigned grid = (unsigned)((nblk + 7) / 8 * 8);
     k_group_convol1_u<GB><<<grid, 256, 0, s>>>(ng, ns, ks, nqb, rkb, d_xx, d_a);
   } else if (ks <= CV_KS) {
-    constexpr int GB = 8;
+    constexpr int GB = 4;  // 37 KB of LDS, four workgroups per CU (measured best of 2, 4, 8, 16)
     const int ntx = (ns + CV_TX - 1) / CV_TX, nty = (ns + CV_TY - 1) / CV_TY;
     const long long nblk = (long long)ntx * nty * ((ng + GB - 1) / GB);
     if (nblk > (1ll << 30)) return fail(DBS_ERR_ARG, "R_data: too many convolution tiles");
