@@ -21,6 +21,7 @@ sz = C.c_size_t
 
 GEN_MATRIX_FN = C.CFUNCTYPE(None, vp, i32, i32, i32, dp, ip, ip, ip, ip)
 KBORT_FN = C.CFUNCTYPE(i32, vp, i32, i32)
+QUADR_PQ_FN = C.CFUNCTYPE(f64, vp, i32, i32, i32)
 
 
 class MatBlocksC(C.Structure):
@@ -38,7 +39,7 @@ class Mat3AuxC(C.Structure):
     """struct dbs_mat3_aux (bound-orbital and target data of a MAT3 run)"""
     _fields_ = [("ibo", i32), ("ipbs", ip), ("kbort", KBORT_FN), ("kbort_user", vp),
                 ("vch_dev", vp), ("nvch", i32), ("fppqq_dev", vp), ("iitar", i32),
-                ("iptar", ip), ("htarg", dp)]
+                ("iptar", ip), ("htarg", dp), ("quadr_pq", QUADR_PQ_FN), ("quadr_pq_user", vp)]
 
 
 auxp = C.POINTER(Mat3AuxC)

@@ -947,11 +947,18 @@ int dbs_rdata_34_dev(dbs_ctx* ctx, int itype, int atype, int k, int nc, const do
   }
   const int ng = (int)ginfo.size();
   if (ng == 0) return DBS_OK;
+  const int sym = itype == 3 ? 0 : 1;
+  // launch sizes, checked before anything is queued on the stream
+  const int nqb = (ns * ks + 255) / 256;
+  const int ntx = (ns + CV_TX - 1) / CV_TX, nty = (ns + CV_TY - 1) / CV_TY;
+  if (sym == 0 && (long long)nqb * ((ng + CV1_GB - 1) / CV1_GB) > (1ll << 30))
+    return fail(DBS_ERR_ARG, "R_data: too many convolution blocks");
+  if (sym == 1 && ks <= CV_KS && (long long)ntx * nty * ((ng + 3) / 4) > (1ll << 30))
+    return fail(DBS_ERR_ARG, "R_data: too many convolution tiles");
   double* rkb = nullptr;
   int rc = build_rkb(ctx, type, k, &rkb);
   if (rc) return rc;
   hipStream_t s = ctx->stream;
-  const int sym = itype == 3 ? 0 : 1;
   const long long per = (long long)ns * (sym == 0 ? ks : ns);
   // every buffer of the call in the context's grow-only scratch (no device
   // allocation per call): densities, convolutions, coefficients, then the
@@ -981,22 +988,20 @@ int dbs_rdata_34_dev(dbs_ctx* ctx, int itype, int atype, int k, int nc, const do
   stage(d_j1, ej1.data(), ej1.size() * sizeof(int));
   stage(d_j2, ej2.data(), ej2.size() * sizeof(int));
   stage(d_gi, ginfo.data(), ginfo.size() * sizeof(int4));
-  if (hipMemcpyAsync(d_c, blob.data(), blob.size() * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess)
+  if (hipMemcpyAsync(d_c, blob.data(), blob.size() * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess) {
+    (void)hipStreamSynchronize(s);  // blob is a pageable host buffer that ends with this call
     return fail(DBS_ERR_HIP, "R_data: upload");
+  }
   k_group_density<<<grid_of(per * (sym == 0 ? (ng + CV1_GB - 1) / CV1_GB * CV1_GB : ng), 256), 256, 0, s>>>(ng, d_goff, d_j1, d_j2, d_c, pq_dev, ns, ks, jp1, jp2,
                                                           sym, sym == 0 ? CV1_GB : 0, d_xx);
   if (sym == 0) {
     constexpr int GB = CV1_GB;
-    const int nqb = (ns * ks + 255) / 256;
     const long long nblk = (long long)nqb * ((ng + GB - 1) / GB);
-    if (nblk > (1ll << 30)) return fail(DBS_ERR_ARG, "R_data: too many convolution blocks");
     const unsigned grid = (unsigned)((nblk + 7) / 8 * 8);
     k_group_convol1_u<GB><<<grid, 256, 0, s>>>(ng, ns, ks, nqb, rkb, d_xx, d_a);
   } else if (ks <= CV_KS) {
     constexpr int GB = 4;  // 37 KB of LDS, four workgroups per CU (measured best of 2, 4, 8, 16)
-    const int ntx = (ns + CV_TX - 1) / CV_TX, nty = (ns + CV_TY - 1) / CV_TY;
     const long long nblk = (long long)ntx * nty * ((ng + GB - 1) / GB);
-    if (nblk > (1ll << 30)) return fail(DBS_ERR_ARG, "R_data: too many convolution tiles");
     const unsigned grid = (unsigned)((nblk + 7) / 8 * 8);
     k_group_convol3_tile<GB><<<grid, CV_TX * CV_TY, 0, s>>>(ng, ns, ks, ntx, nty, rkb, d_xx, d_a);
   } else {

@@ -584,9 +584,31 @@ int rs_data(dbs_ctx* ctx, const dbs_mat_blocks* m, const dbs_mat3_aux* aux, int 
   if (nc <= 0) return DBS_OK;
   DBS_CHECK(c && K1 && K2 && K3 && K4, DBS_ERR_ARG, std::string(who) + ": null list");
   Aux X{aux, stype ? "S: KBORT = 0" : "RK: KBORT =0"};
-  if (stype == 0 && (itype == 3 || itype == 4))
-    return dbs_rdata_34_dev(ctx, itype, atype, k, nc, c, K1, K2, K3, aux->ibo, pq_dev, norb, m->icc, m->nch,
-                            m->hch_dev, m->nblocks);
+  if (stype == 0 && (itype == 3 || itype == 4)) {
+    // asymptotic coefficients of the direct P-P / Q-Q structure
+    // (R_data.f90:222-226, 240): per run of equal K1, CA = sum c*QUADR_PQ
+    // in list order, then UPDATE_ACF(kpol,ich,jch,CA)
+    const bool acf = itype == 3 && atype <= 1 && m->acf_dev;
+    std::vector<int> ak, aich, ajch;
+    std::vector<double> aC;
+    if (acf) {
+      DBS_CHECK(aux->quadr_pq, DBS_ERR_ARG, "R_data: QUADR_PQ callback needed for the asymptotic coefficients");
+      double CA = 0.0;
+      for (int j = 0; j < nc; ++j) {
+        CA = CA + c[j] * aux->quadr_pq(aux->quadr_pq_user, K2[j], K3[j], k);
+        if (j < nc - 1 && K1[j] == K1[j + 1]) continue;
+        ak.push_back(k);
+        aich.push_back(K1[j] / aux->ibo);
+        ajch.push_back(K1[j] % aux->ibo);
+        aC.push_back(CA);
+        CA = 0.0;
+      }
+    }
+    rc = dbs_rdata_34_dev(ctx, itype, atype, k, nc, c, K1, K2, K3, aux->ibo, pq_dev, norb, m->icc, m->nch,
+                          m->hch_dev, m->nblocks);
+    if (rc || !acf) return rc;
+    return dbs_update_acf_dev(ctx, m, (int)aC.size(), ak.data(), aich.data(), ajch.data(), aC.data());
+  }
   if (itype == 1) return structure1(ctx, m, X, P, stype, k, nc, c, K1, K2, K3, K4, pq_dev, norb);
   if (itype <= 5) return structure2(ctx, m, X, P, stype, k, nc, c, K1, K2, K3, K4, pq_dev, norb);
   return structure6(ctx, m, X, P, k, nc, c, K1, K2, K3, pq_dev, norb);

@@ -808,14 +808,7 @@ int dbs_rk_dev(dbs_ctx* c, const double* f1, const double* f2, const double* f3,
   double* d4 = d3 + nb;
   double* cv = d4 + nb;
   double* parts = cv + nb;  // 4 scalars
-  const double* rkb[4];
   const int types[4] = {DBS_PPPP, DBS_QQQQ, DBS_QPQP, DBS_PQPQ};
-  for (int t = 0; t < 4; ++t) {
-    double* p = nullptr;
-    rc = build_rkb(c, types[t], k, &p);
-    if (rc) return rc;
-    rkb[t] = p;
-  }
   if ((rc = dbs_density_dev(c, f1, f3, 's', d1))) return rc;
   if ((rc = dbs_density_dev(c, f2, f4, 's', d2))) return rc;
   if ((rc = dbs_density_dev(c, f1 + ns, f3 + ns, 's', d3))) return rc;
@@ -823,9 +816,14 @@ int dbs_rk_dev(dbs_ctx* c, const double* f1, const double* f2, const double* f3,
   const double* left[4] = {d1, d3, d3, d1};
   const double* right[4] = {d2, d4, d2, d4};
   size_t threads = (size_t)ns * ks * 64;
+  // each array is fetched just before its convolution: building the next
+  // one may evict this one, and the cache synchronises the stream before it
+  // frees an array, so no kernel can read a freed array
   for (int t = 0; t < 4; ++t) {
+    double* rkb = nullptr;
+    if ((rc = build_rkb(c, types[t], k, &rkb))) return rc;
     k_convol2<<<(unsigned)((threads + 255) / 256), 256, 0, c->stream>>>(ns, ks, ks, 's', 's',
-                                                                         rkb[t], left[t], cv);
+                                                                         rkb, left[t], cv);
     k_sum_amb<<<1, 1024, 0, c->stream>>>(ns, ks, 's', cv, right[t], parts + t);
   }
   // rk = (((0 + S_pppp) + S_qqqq) + S_qpqp) + S_pqpq

@@ -8,8 +8,10 @@
 !     Call mrk_pppp(k); Call Convol(ns,ks,a,d,2,'s','s'); S = SUM_AmB(...)
 ! runs them on the GPU when it links these instead of the library's own
 ! definitions.  Module dbs_gpu_state plays the part of the reference's
-! module state (DBS_grid sizes and the rkb pointer of DBS_integrals): it
-! holds the context and the (type,k) that the last mrk_*/msk_* selected.
+! module state (DBS_grid sizes and the two rkb pointers): it holds the
+! context, the (type,k) that the last mrk_* selected (DBS_integrals::rkb,
+! read by convol) and the one the last msk_* selected
+! (DBS_sk_integrals::rkb, read by convol_sk).
 ! A nonzero status from the library stops with the library's message, as
 ! the reference's Stop statements do.
 !======================================================================
@@ -91,7 +93,8 @@
       Implicit none
       Type(c_ptr) :: ctx = c_null_ptr
       Integer :: ns_g = 0, ks_g = 0
-      Integer :: itype_g = 0, krk_g = -100
+      Integer :: itype_r = 0, krk_r = -100   ! last mrk_* (R^k array)
+      Integer :: itype_s = 0, krk_s = -100   ! last msk_* (S^k array)
       End Module dbs_gpu_state
 
 !----------------------------------------------------------------------
@@ -129,7 +132,8 @@
       if (c_associated(ctx)) Call dbs_gpu_check(dbs_ctx_destroy(ctx))
       tt = t
       Call dbs_gpu_check(dbs_ctx_create(ctx, ns, ks, ksp, ksq, tt, 0))
-      ns_g = ns; ks_g = ks; itype_g = 0; krk_g = -100
+      ns_g = ns; ks_g = ks
+      itype_r = 0; krk_r = -100; itype_s = 0; krk_s = -100
       End Subroutine dbs_gpu_init
 
       Subroutine dbs_gpu_close
@@ -152,7 +156,11 @@
       Type(c_ptr) :: p
       if (.not. c_associated(ctx)) Stop 'dbs_gpu: call dbs_gpu_init first'
       Call dbs_gpu_check(dbs_mrk(ctx, itype, k, p))
-      itype_g = itype; krk_g = k
+      if (itype <= 4) then
+       itype_r = itype; krk_r = k
+      else
+       itype_s = itype; krk_s = k
+      end if
       End Subroutine dbs_gpu_select
 
       Subroutine mrk_pppp(k)
@@ -201,26 +209,30 @@
       Subroutine convol(ns, ks, a, d, icase, sym_i, sym_j)
 !----------------------------------------------------------------------
       Use dbs_gpu_api
-      Use dbs_gpu_state, only: ctx, itype_g, krk_g
+      Use dbs_gpu_state, only: ctx, itype_r, krk_r
       Implicit none
       Integer, intent(in) :: ns, ks, icase
       Character, intent(in) :: sym_i, sym_j
       Real(8), intent(in) :: d(ns, *)
       Real(8), intent(out) :: a(ns, *)
-      if (itype_g == 0) Stop 'convol: no array selected'
+      if (itype_r == 0) Stop 'convol: no array selected'
       if (icase < 1 .or. icase > 4) Stop 'convol: unknown case'
-      Call dbs_gpu_check(dbs_convol(ctx, itype_g, krk_g, icase, sym_i, sym_j, d, a))
+      Call dbs_gpu_check(dbs_convol(ctx, itype_r, krk_r, icase, sym_i, sym_j, d, a))
       End Subroutine convol
 
 !----------------------------------------------------------------------
       Subroutine convol_sk(ns, ks, a, d, icase, sym_i, sym_j)
 !----------------------------------------------------------------------
+      Use dbs_gpu_api
+      Use dbs_gpu_state, only: ctx, itype_s, krk_s
       Implicit none
       Integer, intent(in) :: ns, ks, icase
       Character, intent(in) :: sym_i, sym_j
       Real(8), intent(in) :: d(ns, *)
       Real(8), intent(out) :: a(ns, *)
-      Call convol(ns, ks, a, d, icase, sym_i, sym_j)
+      if (itype_s == 0) Stop 'convol_sk: no array selected'
+      if (icase < 1 .or. icase > 4) Stop 'convol: unknown case'
+      Call dbs_gpu_check(dbs_convol(ctx, itype_s, krk_s, icase, sym_i, sym_j, d, a))
       End Subroutine convol_sk
 
 !----------------------------------------------------------------------

@@ -15,7 +15,7 @@ import ctypes as C
 
 import numpy as np
 
-from .capi import KBORT_FN, MatBlocksC, Mat3AuxC, check, lib
+from .capi import KBORT_FN, QUADR_PQ_FN, MatBlocksC, Mat3AuxC, check, lib
 
 _ip = C.POINTER(C.c_int)
 _dp = C.POINTER(C.c_double)
@@ -134,37 +134,42 @@ class MatBlocks:
 
 
     # -- R_data.f90 / S_data.f90 ----------------------------------------------
-    def _aux(self, ibo, ipbs, kbort, vch, fppqq, iitar, iptar, htarg):
+    def _aux(self, ibo, ipbs, kbort, vch, fppqq, iitar, iptar, htarg, quadr_pq=None):
         self._keep = []
         ip = _i32(ipbs) if ipbs is not None else None
         it = _i32(iptar) if iptar is not None else None
         cb = KBORT_FN(lambda user, i1, i2: int(kbort(i1, i2))) if kbort is not None else KBORT_FN()
-        self._keep += [ip, it, cb]
+        qb = (QUADR_PQ_FN(lambda user, i1, i2, m: float(quadr_pq(i1, i2, m))) if quadr_pq is not None
+              else QUADR_PQ_FN())
+        self._keep += [ip, it, cb, qb]
         if htarg is not None:
             assert htarg.dtype == np.float64 and htarg.flags.c_contiguous
         return Mat3AuxC(ibo, ip.ctypes.data_as(_ip) if ip is not None else None, cb, None, _vp(vch),
                         int(vch.shape[0]) if vch is not None else 0, _vp(fppqq), iitar,
                         it.ctypes.data_as(_ip) if it is not None else None,
-                        htarg.ctypes.data_as(_dp) if htarg is not None else None)
+                        htarg.ctypes.data_as(_dp) if htarg is not None else None, qb, None)
 
-    def _rs(self, name, tkey, k, lst, pq, norb, ibo, ipbs, kbort, vch, fppqq, iitar, iptar, htarg):
+    def _rs(self, name, tkey, k, lst, pq, norb, ibo, ipbs, kbort, vch, fppqq, iitar, iptar, htarg,
+            quadr_pq=None):
         c, K1, K2, K3, K4 = lst
         c = np.ascontiguousarray(c, dtype=np.float64)
         K = [_i32(x) for x in (K1, K2, K3, K4)]
-        aux = self._aux(ibo, ipbs, kbort, vch, fppqq, iitar, iptar, htarg)
+        aux = self._aux(ibo, ipbs, kbort, vch, fppqq, iitar, iptar, htarg, quadr_pq)
         self._pre()
         check(getattr(lib(), name)(self.dbs.ctx, C.byref(self._desc), C.byref(aux), *tkey, k, len(c),
                                    c.ctypes.data_as(_dp), *[x.ctypes.data_as(_ip) for x in K], _vp(pq), norb))
         self._post()
 
     def R_data(self, itype, kpol, btype, lst, pq, norb, ibo=2 ** 15, ipbs=None, kbort=None, vch=None,
-               fppqq=None, iitar=0, iptar=None, htarg=None):
+               fppqq=None, iitar=0, iptar=None, htarg=None, quadr_pq=None):
         """R_data(itype, kpol, btype) of one merged list lst = (c, K1, K2, K3, K4)
         in IPT order; pq: device tensor holding pq(ns,2,norb); vch: device
         tensor (nvch, ms) holding V_ch(ms,nvch); ipbs/iptar 1-based tables
-        (element i-1 for orbital/channel i)."""
+        (element i-1 for orbital/channel i); quadr_pq(i1, i2, m): the host
+        program's QUADR_PQ (needed by itype 3, btype 0/1 for the asymptotic
+        coefficients acf)."""
         self._rs("dbs_rdata_dev", (itype, btype), kpol, lst, pq, norb, ibo, ipbs, kbort, vch, fppqq, iitar,
-                 iptar, htarg)
+                 iptar, htarg, quadr_pq)
 
     def S_data(self, ktype, kpol, lst, pq, norb, ibo=2 ** 15, ipbs=None, kbort=None, vch=None,
                fppqq=None, iitar=0, iptar=None, htarg=None):

@@ -82,7 +82,10 @@ int dbs_diag_cells_dev(dbs_ctx *ctx, int type, int k, double *rkd_dev);
 /* mrk_pppp/qqqq/pqpq/qpqp and msk_ppqq/pqqp (dbsr_lib_dbs.f90:2898-2959,
  * 3431-3492, 3067-3131, 3249-3313, 3601-3662, 3771-3833): returns the
  * context's cached device array rkb(ns,ns,kb,kb), kb = ks (R) or 2ks-1 (S).
- * The pointer stays valid until the context evicts or is destroyed. */
+ * The pointer stays valid only until the next array build on this context
+ * (any entry point that needs another (type,k) may evict it when the cache
+ * limit is reached; eviction synchronises the context stream first) or
+ * until the context is destroyed. */
 int dbs_mrk(dbs_ctx *ctx, int type, int k, const double **rkb_dev);
 /* same, copied into a host array of ns*ns*kb*kb doubles */
 int dbs_mrk_host(dbs_ctx *ctx, int type, int k, double *rkb);
@@ -255,6 +258,11 @@ int dbs_o_data_dev(dbs_ctx *ctx, const dbs_mat_blocks *m, int itype, int n, cons
 int dbs_overlap_ci_dev(dbs_ctx *ctx, int n, const int *ic, const int *jc, const double *cc,
                        double eps_c, int nzero, double *SM_dev, int ldsm);
 
+/* QUADR_PQ(i1,i2,m) of the host program: <i1|r^m|i2> over the P and Q
+ * components of orbitals i1, i2 (1-based), the reference's external
+ * QUADR_PQ (called at DBSR_MAT3/R_data.f90:224; defined in the un-vendored
+ * Libraries_03). */
+typedef double (*dbs_quadr_pq_fn)(void *user, int i1, int i2, int m);
 /* Bound-orbital and target data of a MAT3 run (module dbsr_mat,
  * DBSR_MAT3/mod_dbsr_mat.f90): pair packing base ibo (:61), ipbs(orbital)
  * (channel of a channel orbital, 0 for bound; 1-based orbitals), the host
@@ -272,13 +280,19 @@ typedef struct dbs_mat3_aux {
   int iitar;
   const int *iptar;
   double *htarg;
+  dbs_quadr_pq_fn quadr_pq; /* QUADR_PQ for the asymptotic coefficients */
+  void *quadr_pq_user;
 } dbs_mat3_aux;
 /* R_data(itype, kpol, btype) (DBSR_MAT3/R_data.f90:2-290): one merged list
  * (IPT order, host arrays) of structure itype 1..4 and atype 0..3 into the
  * MAT3 matrices: 1 integral x {UPDATE_HB, UPDATE_HV, UPDATE_HW, Target_h +
  * UPDATE_HX(fppqq)}; 2 the bound-channel vectors BAV(convol(case 1,
- * density(K1)), pq(K2)) into UPDATE_HW / UPDATE_HV; 3/4 as dbs_rdata_34_dev.
- * pq_dev(ns,2,norb). */
+ * density(K1)), pq(K2)) into UPDATE_HW / UPDATE_HV; 3/4 as dbs_rdata_34_dev,
+ * and for itype 3 with atype 0/1 also the asymptotic coefficients
+ * (R_data.f90:222-226, 240): per run of equal K1, CA = sum of
+ * c*QUADR_PQ(K2,K3,k) in list order, then UPDATE_ACF(k,ich,jch,CA).  That
+ * needs aux->quadr_pq when m->acf_dev is set (error otherwise); runs are
+ * applied in list order.  pq_dev(ns,2,norb). */
 int dbs_rdata_dev(dbs_ctx *ctx, const dbs_mat_blocks *m, const dbs_mat3_aux *aux, int itype, int atype,
                   int k, int nc, const double *c, const int *K1, const int *K2, const int *K3,
                   const int *K4, const double *pq_dev, int norb);

@@ -1217,6 +1217,31 @@ void orc_rdata_34(int ns, int ks, const double *rkb, int itype, int atype,
   free(xx);
 }
 
+/* Asymptotic coefficients of R_data itype 3 with atype 0/1
+ * (DBSR_MAT3/R_data.f90:214-240): walking the list in IPT order, CA =
+ * CA + c*QUADR_PQ(K2,K3,kpol); at the end of each run of equal K1,
+ * UPDATE_ACF(kpol,ich,jch,CA) (DBSR_MAT3/updates.f90:270-292: acf(i,k) += C
+ * with i = icc(ich,jch); blocks not stored here are skipped) and CA = 0.
+ * qpq(norb,norb) holds QUADR_PQ(i,j,kpol) of the host program.  acf is
+ * (nblocks, 0:mk) column-major.  Returns 0, or 1 for k > mk ('Update_cf:
+ * k > kmk'). */
+int orc_rdata_acf(int nc, const double *cdata, const int *K1, const int *K2, const int *K3, int ibo,
+                  int kpol, int mk, const double *qpq, int norb, const int *icc, int nch, int nblocks,
+                  double *acf) {
+  if (kpol > mk) return 1;
+  double CA = 0.0;
+  for (int j = 0; j < nc; ++j) {
+    const double S = qpq[(K2[j] - 1) + (size_t)norb * (K3[j] - 1)];
+    CA = CA + cdata[j] * S;
+    if (j < nc - 1 && K1[j] == K1[j + 1]) continue;
+    const int ich = K1[j] / ibo, jch = K1[j] % ibo;
+    const int b = icc[(ich - 1) + (size_t)nch * (jch - 1)];
+    if (b > 0) acf[(b - 1) + (size_t)nblocks * kpol] += CA;
+    CA = 0.0;
+  }
+  return 0;
+}
+
 /* Idef_Rtype for the channel-channel structures (ic=jc=io=jo=0):
  * DBSR_MAT3/idef_type.f90:142-302.  ipbs(orbital) > 0 marks a channel
  * orbital (1-based orbital index; ipbs[0] unused).  Returns itype (3 or 4),

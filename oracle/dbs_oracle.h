@@ -136,6 +136,9 @@ int  orc_rs_data(int ns, int ks, const double *rkb, int stype, int itype, int at
                  const double *vch, const double *fppqq, int iitar, const int *iptar, double *htarg,
                  int nch, int npert, const int *icc, double *hch, const int *icb, double *hcp,
                  const int *ibb, double *hp);
+int  orc_rdata_acf(int nc, const double *cdata, const int *K1, const int *K2, const int *K3, int ibo,
+                   int kpol, int mk, const double *qpq, int norb, const int *icc, int nch, int nblocks,
+                   double *acf);
 
 #ifdef __cplusplus
 }

@@ -344,6 +344,20 @@ def rs_data(g, stype, itype, atype, k, lst, pq, ibo, ipbs, kbort, vch, fppqq, ii
                              _d(mats["hch"]), _i(icb), _d(mats["hcp"]), _i(ibb), _d(mats["hp"]))
 
 
+def rdata_acf(lst, ibo, kpol, mk, qpq, icc, acf):
+    """asymptotic coefficients of R_data itype 3, atype 0/1 (R_data.f90:214-240):
+    lst = (c, K1, K2, K3) in IPT order; qpq(norb,norb) = QUADR_PQ(i,j,kpol);
+    acf (nblocks, mk+1) Fortran order, updated in place; returns the status"""
+    c, K1, K2, K3 = lst[:4]
+    ci = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+    icc_f = np.asfortranarray(icc, dtype=np.int32)
+    q = np.asfortranarray(qpq, dtype=np.float64)
+    assert acf.flags.f_contiguous and acf.dtype == np.float64
+    return lib().orc_rdata_acf(len(c), _d(np.ascontiguousarray(c, dtype=np.float64)), _i(ci(K1)), _i(ci(K2)),
+                               _i(ci(K3)), ibo, kpol, mk, _d(q), q.shape[0], _i(icc_f), icc_f.shape[0],
+                               acf.shape[0], _d(acf))
+
+
 def idef_rtype(j1, j2, j3, j4, ipbs, ibo):
     ip = np.ascontiguousarray(ipbs, dtype=np.int32)
     out = [np.zeros(1, dtype=np.int32) for _ in range(4)]

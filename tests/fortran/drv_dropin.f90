@@ -9,6 +9,7 @@
       Implicit none
       Integer :: ns, ks, ksp, ksq, norb, k, n, i
       Real(8), allocatable :: t(:), pq(:,:,:), d1(:,:), d2(:,:), conv(:,:), out(:)
+      Real(8), allocatable :: dn1(:,:), dn2(:,:), convn(:,:)
       Real(8), external :: rk, SUM_AmB, quadr
 
       Open(11, file='drv_in.bin', access='stream', form='unformatted', status='old')
@@ -19,7 +20,8 @@
       Close(11)
 
       Call dbs_gpu_init(ns, ks, ksp, ksq, t)
-      Allocate(d1(ns, ks), d2(ns, ks), conv(ns, ks), out(8))
+      Allocate(d1(ns, ks), d2(ns, ks), conv(ns, ks), out(10))
+      Allocate(dn1(ns, 2*ks-1), dn2(ns, 2*ks-1), convn(ns, 2*ks-1))
 
 ! ... R^k(1,2;3,4) as rk() assembles it
       out(1) = rk(pq(1,1,1), pq(1,1,2), pq(1,1,3), pq(1,1,4), k)
@@ -58,6 +60,21 @@
       Call Convol(ns, ks, conv, d2, 1, 's', 's')
       out(6) = SUM_AmB(ns, ks, conv, d1, 's')
       out(7) = dble(ns); out(8) = dble(k)
+! ... interleaved selections, as sk_ppqq (DBSR_HF/hf_sk.f90) and a direct
+!     potential would mix them: convol reads the last mrk_* array,
+!     convol_sk the last msk_* array, whichever was selected last
+      Call msk_ppqq(k)
+      Call mrk_pppp(k)
+      Call Density(ns, ks, dn1, pq(1,1,1), pq(1,2,3), 'n')
+      Call Convol_sk(ns, ks, convn, dn1, 2, 'n', 'n')
+      Call Density(ns, ks, dn2, pq(1,1,2), pq(1,2,4), 'n')
+      out(9) = SUM_AmB(ns, ks, convn, dn2, 'n')
+      Call mrk_qqqq(k)
+      Call msk_pqqp(k)
+      Call Density(ns, ks, d1, pq(1,2,1), pq(1,2,3), 's')
+      Call Density(ns, ks, d2, pq(1,2,2), pq(1,2,4), 's')
+      Call Convol(ns, ks, conv, d1, 2, 's', 's')
+      out(10) = SUM_AmB(ns, ks, conv, d2, 's')
 
       Open(12, file='drv_out.bin', access='stream', form='unformatted', status='replace')
       Write(12) out

@@ -44,3 +44,14 @@ def test_fortran_dropin(tmp_path, gpu_available):
     v = O.convol(ns, ks, g.mrk(O.PPPP, k), d2, 1, "s", "s")
     x = O.sum_amb(ns, ks, v, d1, "s")
     assert abs(out[5] - x) <= 1e-12 * abs(x)
+    # interleaved mrk_/msk_ selections: convol_sk used ppqq, convol used qqqq
+    dn1 = O.density(ns, ks, fs[0][:, 0], fs[2][:, 1], "n")
+    dn2 = O.density(ns, ks, fs[1][:, 0], fs[3][:, 1], "n")
+    v = O.convol(ns, ks, g.mrk(O.PPQQ, k), dn1, 2, "n", "n")
+    x = O.sum_amb(ns, ks, v, dn2, "n")
+    assert abs(out[8] - x) <= 1e-12 * np.sum(np.abs(v * dn2)), (out[8], x)
+    d1 = O.density(ns, ks, fs[0][:, 1], fs[2][:, 1], "s")
+    d2 = O.density(ns, ks, fs[1][:, 1], fs[3][:, 1], "s")
+    v = O.convol(ns, ks, g.mrk(O.QQQQ, k), d1, 2, "s", "s")
+    x = O.sum_amb(ns, ks, v, d2, "s")
+    assert abs(out[9] - x) <= 1e-12 * abs(x), (out[9], x)

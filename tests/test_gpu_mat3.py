@@ -82,8 +82,11 @@ def _check(stype, itype, atype, spec, seed, k=2, n=120):
     htarg_gpu, htarg_ref = np.zeros(3), np.zeros(3)
     kw = dict(ibo=IBO, ipbs=IPBS[1:], kbort=kbort, vch=dev["vch"], fppqq=dev["fppqq"], iitar=1,
               iptar=IPTAR[1:], htarg=htarg_gpu)
+    # QUADR_PQ of the host program: the oracle's quadr on the orbitals (the
+    # library only sums and places what the callback returns)
+    qpq = np.array([[g.quadr(pq[:, :, i], pq[:, :, j], k) for j in range(NORB)] for i in range(NORB)])
     if stype == 0:
-        m.R_data(itype, k, atype, lst, dev["pq"], NORB, **kw)
+        m.R_data(itype, k, atype, lst, dev["pq"], NORB, quadr_pq=lambda i1, i2, mm: qpq[i1 - 1, i2 - 1], **kw)
     else:
         m.S_data(atype * 10 + itype, k, lst, dev["pq"], NORB, **kw)
     mats = dict(icc=m.icc, icb=m.icb, ibb=m.ibb, hch=np.zeros((g.ms, g.ms, m.nblocks), order="F"),
@@ -99,6 +102,13 @@ def _check(stype, itype, atype, spec, seed, k=2, n=120):
         assert np.max(np.abs(got[key] - ref)) <= TOL * max(scale, 1e-300), key
     assert np.max(np.abs(htarg_gpu - htarg_ref)) <= TOL * max(np.max(np.abs(htarg_ref)), 1e-300)
     assert touched > 0
+    # asymptotic coefficients: only R_data itype 3 with atype 0/1 produces them
+    acf_ref = np.zeros((m.nblocks, m.mk + 1), order="F")
+    if stype == 0 and itype == 3 and atype <= 1:
+        assert O.rdata_acf(lst, IBO, k, m.mk, qpq, m.icc, acf_ref) == 0
+        assert np.max(np.abs(acf_ref)) > 0
+    acf_gpu = m.acf.cpu().numpy().T
+    assert np.array_equal(acf_gpu, acf_ref), "acf"
 
 
 @pytest.mark.parametrize("itype", [1, 2, 3, 4])

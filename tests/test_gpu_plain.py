@@ -152,3 +152,25 @@ def test_update_hs(grids):
             got = device_to_fortran(d.UPDATE_HS(fortran_to_device(h0), ii, jj,
                                                 fortran_to_device(dd), sym))
             assert np.array_equal(got, ref), (sym, ii, jj)
+
+
+def test_rk_under_small_cache_limit(grids):
+    """rk() needs four arrays of one multipole; with the cache limited to
+    about one array every build evicts the previous one.  Each array must be
+    built just before its convolution so that no kernel reads an evicted
+    (freed) array: the values must still equal the oracle's."""
+    import ctypes as C
+    from dbsr3_amd.capi import check, lib
+    from dbsr3_amd.dbs import fortran_to_device
+    g, d = grids
+    one = g.ns * g.ns * g.ks * g.ks * 8
+    check(lib().dbs_ctx_set_cache_limit(d.ctx, C.c_size_t(one + one // 2)))
+    try:
+        fs = random_orbitals(g, 4, seed=23)
+        fd = [fortran_to_device(f) for f in fs]
+        for k in (3, 1, 3):
+            x = d.rk(*fd, k)
+            r = g.rk(*fs, k)
+            assert abs(x - r) <= TOL * abs(r), (k, x, r)
+    finally:
+        check(lib().dbs_ctx_set_cache_limit(d.ctx, C.c_size_t(8 << 30)))

@@ -14,6 +14,10 @@ routes other than their own formulas:
   and sum is exact, the blocks equal the dense algebra (C d, C d^T, C v w^T,
   its transpose, the symmetrised form, the sum of the list), with the
   eps_C cut, the not-stored blocks, iitar and the KBORT=0 stop.
+* R_data's asymptotic coefficients (DBSR_MAT3/R_data.f90:214-240): on
+  small-integer data, acf(block, kpol) equals the plain sum of c*QUADR_PQ
+  over every list entry of that block, a sum with no run logic, so a run
+  that is not closed, not reset or closed at the wrong entry fails.
 """
 import numpy as np
 import pytest
@@ -219,3 +223,35 @@ def test_o_data_drops_unstored_blocks_and_stops_on_kbort0():
     assert np.array_equal(m["hch"][:, :, 1], 2.0 * x) and not np.any(m["hch"][:, :, 0])
     assert O.o_data(ms, 2, (c, io, np.array([1, 1])), 1e-10, IBO, ipbs, lambda a, b: 0, vch, vch, 0,
                     ipbs, np.zeros(3), m) == 2
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_rdata_acf_against_block_sums(seed):
+    rng = np.random.default_rng(seed)
+    nch, norb, ibo, n, mk, kpol = 5, 6, 2 ** 15, 300, 4, 3
+    icc = np.zeros((nch, nch), dtype=np.int32, order="F")
+    b = 0
+    for ich in range(nch):
+        for jch in range(ich + 1):
+            if (ich + jch) % 4 == 3:
+                continue  # some blocks not stored here
+            b += 1
+            icc[ich, jch] = icc[jch, ich] = b
+    ich, jch = rng.integers(1, nch + 1, n), rng.integers(1, nch + 1, n)
+    K1 = np.maximum(ich, jch) * ibo + np.minimum(ich, jch)
+    K2, K3 = rng.integers(1, norb + 1, n), rng.integers(1, norb + 1, n)
+    order = np.lexsort((K2, K1))
+    order = np.concatenate([order[:n - 7], order[n - 7:][::-1]])  # runs that repeat later
+    c = rng.integers(-8, 9, n).astype(np.float64)
+    lst = (c[order], K1[order], K2[order], K3[order])
+    qpq = rng.integers(-5, 6, (norb, norb)).astype(np.float64)
+    acf = np.zeros((b, mk + 1), order="F")
+    assert O.rdata_acf(lst, ibo, kpol, mk, qpq, icc, acf) == 0
+    want = np.zeros((b, mk + 1))
+    for cc, k1, k2, k3 in zip(*lst):
+        blk = icc[k1 // ibo - 1, k1 % ibo - 1]
+        if blk:
+            want[blk - 1, kpol] += cc * qpq[k2 - 1, k3 - 1]
+    assert np.array_equal(acf, want)
+    assert np.any(want != 0)
+    assert O.rdata_acf(lst, ibo, mk + 1, mk, qpq, icc, acf) == 1  # 'Update_cf: k > kmk'
