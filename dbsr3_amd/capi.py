@@ -71,6 +71,7 @@ PROTOTYPES = {
     "dbs_convol_dev": [vp, i32, i32, i32, ch, ch, vp, vp],
     "dbs_sum_amb_dev": [vp, vp, vp, ch, vp],
     "dbs_rk_dev": [vp, vp, vp, vp, vp, i32, vp],
+    "dbs_rk_noalloc_dev": [vp, vp, vp, vp, vp, i32, vp],
     "dbs_quadr_dev": [vp, vp, vp, i32, vp],
     "dbs_update_hs_dev": [vp, vp, i32, i32, vp, ch],
     "dbs_density": [vp, dp, dp, ch, dp],

@@ -8,6 +8,8 @@
 // summation order per element, so rkb matches the reference bit for bit.
 #include <algorithm>
 #include <cstring>
+#include <mutex>
+#include <string>
 
 #include "dbs_device.h"
 #include "dbs_internal.h"
@@ -827,6 +829,86 @@ int dbs_rk_dev(dbs_ctx* c, const double* f1, const double* f2, const double* f3,
     k_sum_amb<<<1, 1024, 0, c->stream>>>(ns, ks, 's', cv, right[t], parts + t);
   }
   // rk = (((0 + S_pppp) + S_qqqq) + S_qpqp) + S_pqpq
+  k_rk_sum<<<1, 64, 0, c->stream>>>(parts, out);
+  DBS_HIP(hipGetLastError());
+  return DBS_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// rk() without any allocation: the four densities, the convolution and the
+// four partial sums live in `scratch`, a fixed device array of this file
+// (one copy per GPU, held until the process ends).  Its device address is
+// taken once per device with hipGetSymbolAddress and kept in scratch_dev;
+// host code never passes the array's own name to a kernel.  The kernels are
+// those of dbs_rk_dev and overwrite every element they later read, so no
+// clearing is needed and results are bit-identical to dbs_rk_dev.
+// ---------------------------------------------------------------------------
+static constexpr int kRkNoallocMaxBand = 32768;  // ns*ks
+static constexpr size_t kRkNoallocDoubles = 5 * (size_t)kRkNoallocMaxBand + 8;
+static __device__ __align__(256) double scratch[kRkNoallocDoubles];  // 1.25 MiB per GPU
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mu;
+double* scratch_dev[kMaxDevices] = {};
+
+int scratch_on(int device, double** out) {
+  DBS_CHECK(device >= 0 && device < kMaxDevices, DBS_ERR_ARG, "rk (no-alloc): device index out of range");
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_dev[device]) {
+    int cur = 0;
+    DBS_HIP(hipGetDevice(&cur));
+    DBS_HIP(hipSetDevice(device));
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    (void)hipSetDevice(cur);
+    if (e != hipSuccess) return fail(DBS_ERR_HIP, std::string("rk (no-alloc): hipGetSymbolAddress: ") +
+                                                      hipGetErrorString(e));
+    scratch_dev[device] = static_cast<double*>(p);
+  }
+  *out = scratch_dev[device];
+  return DBS_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int dbs_rk_noalloc_dev(dbs_ctx* c, const double* f1, const double* f2, const double* f3, const double* f4,
+                       int k, double* out) {
+  DBS_CHECK(c && f1 && f2 && f3 && f4 && out, DBS_ERR_ARG, "dbs_rk_noalloc_dev: null argument");
+  const int ns = c->ns, ks = c->ks;
+  const size_t nb = (size_t)ns * ks;
+  DBS_CHECK(nb <= (size_t)kRkNoallocMaxBand, DBS_ERR_CAPACITY,
+            "rk (no-alloc): ns*ks = " + std::to_string(nb) + " exceeds the fixed scratch (" +
+                std::to_string(kRkNoallocMaxBand) + ")");
+  const int types[4] = {DBS_PPPP, DBS_QQQQ, DBS_QPQP, DBS_PQPQ};
+  const double* rkb[4];
+  for (int t = 0; t < 4; ++t) {
+    auto it = c->rkb_cache.find(types[t] * 1000 + k);
+    DBS_CHECK(it != c->rkb_cache.end(), DBS_ERR_STATE,
+              "rk (no-alloc): the R^k arrays of multipole " + std::to_string(k) +
+                  " are not built; build types 1-4 first with dbs_mrk");
+    rkb[t] = it->second;
+  }
+  double* s = nullptr;
+  int rc = scratch_on(c->device, &s);
+  if (rc) return rc;
+  double *d1 = s, *d2 = d1 + nb, *d3 = d2 + nb, *d4 = d3 + nb, *cv = d4 + nb, *parts = cv + nb;
+  const unsigned gd = (unsigned)((nb + 255) / 256);
+  k_density<<<gd, 256, 0, c->stream>>>(ns, ks, 's', f1, f3, d1);
+  k_density<<<gd, 256, 0, c->stream>>>(ns, ks, 's', f2, f4, d2);
+  k_density<<<gd, 256, 0, c->stream>>>(ns, ks, 's', f1 + ns, f3 + ns, d3);
+  k_density<<<gd, 256, 0, c->stream>>>(ns, ks, 's', f2 + ns, f4 + ns, d4);
+  const double* left[4] = {d1, d3, d3, d1};
+  const double* right[4] = {d2, d4, d2, d4};
+  const size_t threads = nb * 64;
+  for (int t = 0; t < 4; ++t) {
+    k_convol2<<<(unsigned)((threads + 255) / 256), 256, 0, c->stream>>>(ns, ks, ks, 's', 's', rkb[t], left[t],
+                                                                         cv);
+    k_sum_amb<<<1, 1024, 0, c->stream>>>(ns, ks, 's', cv, right[t], parts + t);
+  }
   k_rk_sum<<<1, 64, 0, c->stream>>>(parts, out);
   DBS_HIP(hipGetLastError());
   return DBS_OK;

@@ -100,6 +100,20 @@ int dbs_sum_amb_dev(dbs_ctx *ctx, const double *a_dev, const double *b_dev, char
 /* rk (dbsr_lib_dbs.f90:3981-4022): f*(ns,2) device; out_dev[0] */
 int dbs_rk_dev(dbs_ctx *ctx, const double *f1, const double *f2, const double *f3,
                const double *f4, int k, double *out_dev);
+/* rk() that allocates nothing, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops).  No
+ * counterpart in the reference.  Same kernels and results (bit for bit) as
+ * dbs_rk_dev, but the densities, the convolution and the partial sums live
+ * in a fixed device array of the library (1.25 MiB, one copy per GPU, kept
+ * until the process ends) instead of the context's scratch.  Limits: ns*ks
+ * <= 32768 (DBS_ERR_CAPACITY otherwise); the four R^k arrays of multipole
+ * k (types 1-4) must already be in the context's cache, e.g. built by
+ * dbs_mrk (DBS_ERR_STATE otherwise; the call neither builds nor evicts
+ * arrays).  Every call on one GPU shares that array: calls that overlap on
+ * one GPU (other streams or threads) race and must be serialised by the
+ * caller. */
+int dbs_rk_noalloc_dev(dbs_ctx *ctx, const double *f1, const double *f2, const double *f3,
+                       const double *f4, int k, double *out_dev);
 /* quadr (dbsr_lib_dbs.f90:3940-3976): <fi|r^m|fj>, out_dev[0] */
 int dbs_quadr_dev(dbs_ctx *ctx, const double *fi_dev, const double *fj_dev, int m, double *out_dev);
 /* UPDATE_HS, library form (dbsr_lib_dbs.f90:4137-4193): hm(ms,ms) += y/2 and
