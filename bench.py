@@ -239,9 +239,44 @@ def seeded_orbitals(t, ns, ks, ksp, ksq, norb, seed):
 
 
 def all_pair_rows(norb, kmax):
-    pairs = [(a, c) for a in range(1, norb + 1) for c in range(a, norb + 1)]
-    return np.array([(k, a, b, c, d) for k in range(kmax + 1) for (a, c) in pairs for (b, d) in pairs],
-                    dtype=np.int32)
+    """every (k, a, b, c, d) with a <= c, b <= d, k = 0..kmax, in (k, (a,c), (b,d)) order"""
+    a, c = np.triu_indices(norb)
+    npair = len(a)
+    L = np.repeat(np.arange(npair), npair)
+    R = np.tile(np.arange(npair), npair)
+    one = np.stack([a[L] + 1, a[R] + 1, c[L] + 1, c[R] + 1], axis=1)
+    rows = np.zeros(((kmax + 1) * len(one), 5), dtype=np.int32)
+    for k in range(kmax + 1):
+        rows[k * len(one):(k + 1) * len(one), 0] = k
+        rows[k * len(one):(k + 1) * len(one), 1:] = one
+    return rows
+
+
+CONFIG3_ROWS_MAX = 16384
+
+
+def config3_instances(seed=20261203 + 3, ninst=256):
+    """config 3: per instance ns ~ U{60..400}, ks ~ {7,8,9} (ksp = ks-1, ksq =
+    ks), kmax ~ U{2..10}, 4..24 orbitals (seeded, P,Q ~ N(0,1), zero ends, not
+    normalised) on an mkgrid_02 grid (Z=26, r_max=30); the integral set is
+    every R^k(a,b;c,d) with a <= c, b <= d, k <= kmax, or a seeded subset of
+    CONFIG3_ROWS_MAX of them (kept in order) where there are more; c ~ U[-1,1].
+    Returns [(t, ns, ks, kmax, pq, rows, coef)]."""
+    from dbsr3_amd.grid import mkgrid_02
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(ninst):
+        ns = int(rng.integers(60, 401))
+        ks = int(rng.choice([7, 8, 9]))
+        kmax = int(rng.integers(2, 11))
+        norb = int(rng.integers(4, 25))
+        t = mkgrid_02(ns, ks, 26.0, 0.25, 30.0)
+        pq = seeded_orbitals(t, ns, ks, ks - 1, ks, norb, seed + 1000 * (i + 1))
+        rows = all_pair_rows(norb, kmax)
+        if len(rows) > CONFIG3_ROWS_MAX:
+            rows = rows[np.sort(rng.choice(len(rows), CONFIG3_ROWS_MAX, replace=False))]
+        out.append((t, ns, ks, kmax, pq, rows, rng.uniform(-1.0, 1.0, len(rows))))
+    return out
 
 
 def cpu_sample(fn, budget_s):
@@ -320,111 +355,181 @@ def run_config1(args):
                        {"sets": rates, "roofline": roof, "cpu_baseline": cpu, "cpu_port": port})
 
 
+CONFIG2 = dict(ns=150, ks=9, ksp=8, ksq=9, nch=40, nbound=20, kmax=6, nent=200000, ibo=2 ** 15)
+# 0-based positions of the two channel orbitals in R^k(j1,j2;j3,j4): (1,3) and
+# (2,4) give the direct structure Rk(i . j .), (1,4) and (2,3) the exchange Rk(i . . j)
+CH_POS = ((0, 2), (1, 3), (0, 3), (1, 2))
+
+
+def config2_lists(seed=20261203 + 2):
+    """Config 2's MAT3 channel-channel R-integrals, built the reference's way:
+    2e5 seeded terms (c ~ U[-1,1], k ~ U{0..6}, two of 20 bound orbitals, a
+    channel block ich >= jch uniform over the 820 blocks of 40 channels, the
+    two channel orbitals in a direct or exchange position; the second bound
+    orbital and the parity of k are chosen so that every term obeys the
+    parity rule on l, as an angular program only emits such terms) pass through the
+    library's MAT3 Add_integral (parity rule on l, Idef_Rtype) into c_data
+    (DBSR_MAT3/MOD_c_data.f90), whose Add_matrix hands every merged list
+    (itype 3/4, kpol) to the Gen_matrix hook.  Returns (t, bound-orbital
+    pq (not normalised), [(itype, kpol, (c, K1, K2, K3, K4))] in hook order)."""
+    from dbsr3_amd import CData
+    from dbsr3_amd.grid import mkgrid_02
+    P = CONFIG2
+    ns, ks, ksp, ksq, nch, nb, kmax = P["ns"], P["ks"], P["ksp"], P["ksq"], P["nch"], P["nbound"], P["kmax"]
+    t = mkgrid_02(ns, ks, 26.0, 0.25, 30.0)
+    pq = seeded_orbitals(t, ns, ks, ksp, ksq, nb, seed)
+    rng = np.random.default_rng(seed)
+    norb = nb + nch
+    ipbs = np.zeros(norb + 1, dtype=np.int32)
+    ipbs[nb + 1:] = np.arange(1, nch + 1)
+    lorb = np.zeros(norb + 1, dtype=np.int32)
+    lorb[1:] = rng.integers(0, 4, norb)
+    n = P["nent"]
+    c = rng.uniform(-1.0, 1.0, n)
+    k = rng.integers(0, kmax + 1, n)
+    blk = rng.integers(0, nch * (nch + 1) // 2, n)
+    ich_of = np.concatenate([np.arange(j, nch + 1) for j in range(1, nch + 1)])
+    jch_of = np.concatenate([np.full(nch + 1 - j, j) for j in range(1, nch + 1)])
+    bound = rng.integers(1, nb + 1, (n, 2))
+    form = rng.integers(0, 4, n)
+    by_parity = [np.nonzero(lorb[1:nb + 1] % 2 == p)[0] + 1 for p in (0, 1)]
+    alt = rng.integers(0, 1 << 30, n)
+    lists = []
+    cd = CData(4, 0, kmax, 2000, 1000, 100, 1e-12, lambda it, kk, lst: lists.append((it, kk, lst)))
+    for e in range(n):
+        j = [0, 0, 0, 0]
+        p, q = CH_POS[form[e]]
+        j[p], j[q] = nb + int(ich_of[blk[e]]), nb + int(jch_of[blk[e]])
+        r = [x for x in range(4) if x not in (p, q)]
+        j[r[0]], j[r[1]] = int(bound[e, 0]), int(bound[e, 1])
+        p13, p24 = (lorb[j[0]] + lorb[j[2]]) % 2, (lorb[j[1]] + lorb[j[3]]) % 2
+        if p13 != p24:  # second bound orbital of the other l parity
+            cand = by_parity[(lorb[j[r[1]]] + 1) % 2]
+            j[r[1]] = int(cand[alt[e] % len(cand)])
+            p13 = (lorb[j[0]] + lorb[j[2]]) % 2
+        kk = int(k[e])
+        if kk % 2 != p13:
+            kk = kk + 1 if kk < kmax else kk - 1
+        cd.add_integral_mat(kk, *j, float(c[e]), lorb, ipbs, ibo=P["ibo"], mk=kmax)
+    for it in (3, 4):
+        for kk in range(kmax + 1):
+            cd.add_matrix(it, kk)
+    cd.close()
+    return t, pq, lists
+
+
+def config2_runs(lists):
+    """runs of equal K1 per list (one convolution each), by itype"""
+    out = {3: 0, 4: 0}
+    for it, _, lst in lists:
+        K1 = np.asarray(lst[1])
+        out[it] += int(np.count_nonzero(K1[1:] != K1[:-1]) + 1) if len(K1) else 0
+    return out
+
+
+def config2_sample(lists, blocks, icc, ibo):
+    """the lists cut down to the entries whose channel block is in `blocks`
+    (order kept; blocks are independent, so their values are unchanged)"""
+    keep = set(int(b) for b in blocks)
+    out = []
+    for it, kk, lst in lists:
+        K1 = np.asarray(lst[1])
+        b = icc[K1 // ibo - 1, K1 % ibo - 1]
+        sel = np.array([int(x) in keep for x in b], dtype=bool)
+        if sel.any():
+            out.append((it, kk, tuple(np.asarray(x)[sel] for x in lst)))
+    return out
+
+
 def run_config2(args):
     import torch
-    from dbsr3_amd import DBS
-    from dbsr3_amd.cdata import r_data_34
+    from dbsr3_amd import DBS, MatBlocks
     from dbsr3_amd.dbs import fortran_to_device
-    from dbsr3_amd.grid import mkgrid_02
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    ns, ks, ksp, ksq, nch, norb, kmax, nent, ibo = 150, 9, 8, 9, 40, 20, 6, 200000, 2 ** 15
-    t = mkgrid_02(ns, ks, 26.0, 0.25, 30.0)
+    P = CONFIG2
+    ns, ks, ksp, ksq, nch, nb, kmax, ibo = (P[x] for x in ("ns", "ks", "ksp", "ksq", "nch", "nbound", "kmax",
+                                                            "ibo"))
+    t, pq0, lists = config2_lists()
     d = DBS(t, ns, ks, ksp, ksq)
-    pq = normalise_on_gpu(d, seeded_orbitals(t, ns, ks, ksp, ksq, norb, 20261203 + 2))
-    rng = np.random.default_rng(20261203 + 2)
-    # entries: (c, k, bound orbitals i, j, channel block ich >= jch, itype 3/4, atype)
-    c = rng.uniform(-1, 1, nent)
-    k = rng.integers(0, kmax + 1, nent)
-    i2 = rng.integers(1, norb + 1, nent)
-    i3 = rng.integers(1, norb + 1, nent)
-    blk = rng.integers(0, nch * (nch + 1) // 2, nent)
-    it = rng.integers(3, 5, nent)
-    at = rng.integers(0, 4, nent)
-    ich_of, jch_of = [], []
-    for jch in range(1, nch + 1):
-        for ich in range(jch, nch + 1):
-            ich_of.append(ich)
-            jch_of.append(jch)
-    ich_of, jch_of = np.array(ich_of), np.array(jch_of)
-    K1 = ich_of[blk] * ibo + jch_of[blk]
-    icc = np.zeros((nch, nch), dtype=np.int32, order="F")
-    for b in range(len(ich_of)):
-        icc[ich_of[b] - 1, jch_of[b] - 1] = b + 1
-    # merged lists per (itype, atype, k): sorted by (K1, K2, K3), equal keys summed
-    lists = []
-    for itype in (3, 4):
-        for atype in range(4):
-            for kk in range(kmax + 1):
-                sel = (it == itype) & (at == atype) & (k == kk)
-                keys = np.stack([K1[sel], i2[sel], i3[sel]], axis=1)
-                uk, inv = np.unique(keys, axis=0, return_inverse=True)
-                cs = np.zeros(len(uk))
-                np.add.at(cs, inv.ravel(), c[sel])
-                lists.append((itype, atype, kk, (cs, uk[:, 0].astype(np.int32), uk[:, 1].astype(np.int32),
-                                                 uk[:, 2].astype(np.int32))))
-    ms2 = 2 * ns
-    nblk = len(ich_of)
-    hch = torch.zeros(nblk * ms2 * ms2, dtype=torch.float64, device="cuda")
+    pq = normalise_on_gpu(d, pq0)
     pq_dev = fortran_to_device(pq)
+    fd = [fortran_to_device(pq[:, :, o]) for o in range(nb)]
+    # QUADR_PQ(i,j,k) of the bound orbitals for the asymptotic coefficients
+    qpq = np.zeros((nb, nb, kmax + 1), order="F")
+    for i in range(nb):
+        for j in range(nb):
+            for m in range(kmax + 1):
+                qpq[i, j, m] = d.quadr(fd[i], fd[j], m)
+    m = MatBlocks(d, nch, 0, mk=kmax)
+    nl = sum(len(l[2][0]) for l in lists)
+
     def sweep():
-        for itype, atype, kk, lst in lists:
-            r_data_34(d, itype, atype, kk, lst, pq_dev, norb, icc, hch, nblk)
-    sweep()  # warm: builds and caches the R^k arrays
+        for it, kk, lst in lists:
+            for atype in range(4):
+                m.R_data(it, kk, atype, lst, pq_dev, nb, ibo=ibo, quadr_pq=qpq)
+
+    sweep()  # warm: builds and caches the R^k arrays; blocks checked below
     torch.cuda.synchronize()
+    rng = np.random.default_rng(20261203 + 2)
+    blocks = sorted(set(rng.choice(m.nblocks, 7, replace=False) + 1) | {int(m.icc[5, 5])})
+    got = {b: m.hch[b - 1].cpu().numpy().T.copy() for b in blocks}
     t0 = time.perf_counter()
     for _ in range(args.steps):
         sweep()
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / args.steps
-    nl = sum(len(l[3][0]) for l in lists)
     value = nl / (ms * 1e-3)
-    cpu = None
-    if not args.no_cpu:
-        g = O.Grid(t, ns, ks, ksp, ksq)
-        pqh = pq.copy(order="F")
-        href = np.zeros((ms2, ms2, nblk), order="F")
-        def work(deadline):
-            n = 0
-            for itype, atype, kk, lst in lists:
-                if time.perf_counter() > deadline:
-                    break
-                O.rdata_34(g, itype, atype, kk, lst, pqh, ibo, icc, href)
-                n += len(lst[0])
-            return n
-        n, secs = cpu_sample(work, args.cpu_seconds)
-        if n > 0:
-            cpu = {"value": n / secs, "unit": "list entries/s", "cores": 1, "kind": "port",
-                   "sample": f"first {n} merged entries in list order (arrays built inside the "
-                             f"timed region), {secs:.1f} s single-thread"}
-    workload = {"workload": "MAT3 R_data itype 3/4 into 820 channel blocks (ms=300): 2e5 entries, "
-                            "40 channels, 20 bound orbitals, ns=150, k=0..6, atype 0..3; host lists in, "
-                            "blocks on the device; one step = every list once",
-                "entries": nent, "merged_entries": int(nl), "lists": len(lists), "ns": ns, "channels": nch}
-    roof = {"bound": "hbm", "achieved": None, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": None,
-            "traffic": None, "note": "per list: group densities, two convolutions (k_group_convol3_tile dominates), UPDATE_HS; kernel times from rocprofv3 --kernel-trace"}
+    # algorithmic work of one sweep: per run one convolution (band: 2(ns ks)^2
+    # flops; exchange case 3: 4 updates of 2 flops per rkb element, 8 ns^2 ks^2),
+    # four P/Q types each; HBM: every rkb element read once per run
+    runs = config2_runs(lists)
+    flops = 4 * (runs[3] * 2.0 * (ns * ks) ** 2 + runs[4] * 8.0 * ns * ns * ks * ks)
+    roof = {"bound": "mfma", "achieved": flops / (ms * 1e-3) / 1e12, "peak": PEAK_FP64_TFS, "unit": "TFLOP/s"}
+    roof["frac"] = roof["achieved"] / roof["peak"]
+    roof.update({"traffic": None, "runs": runs, "flops_per_sweep": flops,
+                 "note": "whole sweep (all lists, four P/Q types): densities, band and exchange convolutions, "
+                         "UPDATE_HS; flops counted as the reference's convol loops do them"})
+    # oracle on the sampled blocks (every entry of those blocks, every type)
+    g = O.Grid(t, ns, ks, ksp, ksq)
+    icc_s = np.zeros((nch, nch), dtype=np.int32, order="F")
+    for i, b in enumerate(blocks):
+        icc_s[m.icc == b] = i + 1
+    sample = config2_sample(lists, blocks, m.icc, ibo)
+    href = np.zeros((2 * ns, 2 * ns, len(blocks)), order="F")
+    ns_entries = sum(len(l[2][0]) for l in sample)
+    t0 = time.perf_counter()
+    for it, kk, lst in sample:
+        for atype in range(4):
+            O.rdata_34(g, it, atype, kk, lst[:4], pq, ibo, icc_s, href)
+    secs = time.perf_counter() - t0
+    err = max(np.max(np.abs(got[b] - href[:, :, i])) / np.max(np.abs(href[:, :, i]))
+              for i, b in enumerate(blocks))
+    if not err <= 1e-12:
+        raise SystemExit(f"config 2: channel blocks differ from the oracle (max rel err {err:.3e})")
+    cpu = {"value": ns_entries / secs, "unit": "list entries/s", "cores": 1, "kind": "port",
+           "sample": f"every entry of {len(blocks)} seeded channel blocks ({ns_entries} merged entries, four "
+                     f"P/Q types each, arrays built inside the timed region), {secs:.1f} s single-thread"}
+    workload = {"workload": "MAT3 channel blocks: 2e5 seeded R^k terms (40 channels, 20 bound orbitals, ns=150, "
+                            "ks=9/8/9, k=0..6) through Add_integral + c_data into itype 3/4 lists; each list by "
+                            "R_data for atype 0..3 into 820 blocks of 300x300; one step = every list once",
+                "entries": P["nent"], "merged_entries": int(nl), "lists": len(lists), "ns": ns, "channels": nch}
     return config_line(2, value, "list entries/s", ms, args.steps, 1, workload,
-                       {"roofline": roof, "cpu_baseline": cpu})
+                       {"roofline": roof, "cpu_baseline": cpu, "max_rel_err_sampled_blocks": err,
+                        "sampled_blocks": [int(b) for b in blocks]})
 
 
 def run_config3(args):
     from dbsr3_amd import DBS, Engine
-    from dbsr3_amd.grid import mkgrid_02
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    rng = np.random.default_rng(20261203 + 3)
+    spec = config3_instances()
     insts, host = [], []
-    for i in range(256):
-        ns = int(rng.integers(60, 401))
-        ks = int(rng.choice([7, 8, 9]))
-        kmax = int(rng.integers(2, 11))
-        norb = int(rng.integers(4, 9))
-        t = mkgrid_02(ns, ks, 26.0, 0.25, 30.0)
+    for (t, ns, ks, kmax, pq0, rows, coef) in spec:
         d = DBS(t, ns, ks, ks - 1, ks)
-        pq = normalise_on_gpu(d, seeded_orbitals(t, ns, ks, ks - 1, ks, norb, 20261203 + 3 + 1000 * i))
-        rows = all_pair_rows(norb, kmax)
-        insts.append((d, norb, rows))
-        host.append((t, ns, ks, pq, rows, rng.uniform(-1, 1, len(rows))))
+        pq = normalise_on_gpu(d, pq0)
+        insts.append((d, pq.shape[2], rows))
+        host.append((t, ns, ks, pq, rows, coef))
     nrows = sum(len(h[4]) for h in host)
     eng = Engine(insts)
     for i, h in enumerate(host):
@@ -456,9 +561,10 @@ def run_config3(args):
             cpu = {"value": n / secs, "unit": "R^k integrals/s", "cores": 1, "kind": "port",
                    "sample": f"{n} rows of the smallest instances first (arrays built inside the timed "
                              f"region), {secs:.1f} s single-thread; the GPU figure is forward+adjoint"}
-    workload = {"workload": "256 uneven instances (ns 60..400, ks 7..9, kmax 2..10, 4..8 orbitals, all "
-                            "pairs) forward+adjoint, one hipGraph per step, autotuned",
-                "instances": 256, "rows": int(nrows), "tuning": tuning}
+    workload = {"workload": "256 uneven instances (ns 60..400, ks 7..9, kmax 2..10, 4..24 orbitals; all "
+                            f"a<=c, b<=d pairs, or a seeded subset of {CONFIG3_ROWS_MAX} rows per instance) "
+                            "forward+adjoint, one hipGraph per step, autotuned",
+                "instances": len(spec), "rows": int(nrows), "tuning": tuning}
     return config_line(3, value, "R^k integrals/s", ms, args.steps, args.warmup, workload,
                        {"roofline": roof, "cpu_baseline": cpu, "energy_0": eng.energy(0)})
 

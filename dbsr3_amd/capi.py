@@ -45,6 +45,11 @@ class Mat3AuxC(C.Structure):
 auxp = C.POINTER(Mat3AuxC)
 
 
+class QpqTableC(C.Structure):
+    """struct dbs_qpq_table (tabulated QUADR_PQ)"""
+    _fields_ = [("norb", i32), ("mmax", i32), ("q", dp)]
+
+
 class Mat3OrbsC(C.Structure):
     """struct dbs_mat3_orbs (orbital tables of MAT3's Add_integral)"""
     _fields_ = [("norb", i32), ("lbs", ip), ("kbs", ip), ("ipbs", ip), ("ibo", i32), ("mk", i32),
@@ -140,6 +145,8 @@ def lib():
         L = C.CDLL(LIB_PATH)
         L.dbs_last_error.restype = C.c_char_p
         L.dbs_last_error.argtypes = []
+        L.dbs_quadr_pq_table.restype = C.c_double
+        L.dbs_quadr_pq_table.argtypes = [vp, i32, i32, i32]
         for name, args in PROTOTYPES.items():
             f = getattr(L, name, None)
             if f is None:  # reported by tests/test_capi.py
@@ -159,4 +166,4 @@ def check(rc):
 
 def exported_symbols():
     """Every function name the public header declares."""
-    return ["dbs_last_error"] + list(PROTOTYPES)
+    return ["dbs_last_error", "dbs_quadr_pq_table"] + list(PROTOTYPES)

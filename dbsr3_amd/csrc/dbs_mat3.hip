@@ -621,6 +621,12 @@ using namespace dbs;
 
 extern "C" {
 
+double dbs_quadr_pq_table(void* user, int i1, int i2, int m) {
+  const dbs_qpq_table* t = static_cast<const dbs_qpq_table*>(user);
+  if (!t || !t->q || i1 < 1 || i2 < 1 || i1 > t->norb || i2 > t->norb || m < 0 || m > t->mmax) return 0.0;
+  return t->q[(i1 - 1) + (size_t)t->norb * ((i2 - 1) + (size_t)t->norb * m)];
+}
+
 int dbs_rdata_dev(dbs_ctx* ctx, const dbs_mat_blocks* m, const dbs_mat3_aux* aux, int itype, int atype, int k,
                   int nc, const double* c, const int* K1, const int* K2, const int* K3, const int* K4,
                   const double* pq_dev, int norb) {

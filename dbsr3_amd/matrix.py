@@ -15,7 +15,7 @@ import ctypes as C
 
 import numpy as np
 
-from .capi import KBORT_FN, QUADR_PQ_FN, MatBlocksC, Mat3AuxC, check, lib
+from .capi import KBORT_FN, QUADR_PQ_FN, MatBlocksC, Mat3AuxC, QpqTableC, check, lib
 
 _ip = C.POINTER(C.c_int)
 _dp = C.POINTER(C.c_double)
@@ -139,15 +139,26 @@ class MatBlocks:
         ip = _i32(ipbs) if ipbs is not None else None
         it = _i32(iptar) if iptar is not None else None
         cb = KBORT_FN(lambda user, i1, i2: int(kbort(i1, i2))) if kbort is not None else KBORT_FN()
-        qb = (QUADR_PQ_FN(lambda user, i1, i2, m: float(quadr_pq(i1, i2, m))) if quadr_pq is not None
-              else QUADR_PQ_FN())
+        quser = None
+        if isinstance(quadr_pq, np.ndarray):
+            # tabulated QUADR_PQ q(norb,norb,0:mmax): the library's own lookup
+            # callback, no Python call per list entry
+            q = np.asfortranarray(quadr_pq, dtype=np.float64)
+            tab = QpqTableC(q.shape[0], q.shape[2] - 1, q.ctypes.data_as(_dp))
+            qb = C.cast(lib().dbs_quadr_pq_table, QUADR_PQ_FN)
+            quser = C.cast(C.pointer(tab), C.c_void_p)
+            self._keep += [q, tab]
+        elif quadr_pq is not None:
+            qb = QUADR_PQ_FN(lambda user, i1, i2, m: float(quadr_pq(i1, i2, m)))
+        else:
+            qb = QUADR_PQ_FN()
         self._keep += [ip, it, cb, qb]
         if htarg is not None:
             assert htarg.dtype == np.float64 and htarg.flags.c_contiguous
         return Mat3AuxC(ibo, ip.ctypes.data_as(_ip) if ip is not None else None, cb, None, _vp(vch),
                         int(vch.shape[0]) if vch is not None else 0, _vp(fppqq), iitar,
                         it.ctypes.data_as(_ip) if it is not None else None,
-                        htarg.ctypes.data_as(_dp) if htarg is not None else None, qb, None)
+                        htarg.ctypes.data_as(_dp) if htarg is not None else None, qb, quser)
 
     def _rs(self, name, tkey, k, lst, pq, norb, ibo, ipbs, kbort, vch, fppqq, iitar, iptar, htarg,
             quadr_pq=None):
@@ -165,9 +176,10 @@ class MatBlocks:
         """R_data(itype, kpol, btype) of one merged list lst = (c, K1, K2, K3, K4)
         in IPT order; pq: device tensor holding pq(ns,2,norb); vch: device
         tensor (nvch, ms) holding V_ch(ms,nvch); ipbs/iptar 1-based tables
-        (element i-1 for orbital/channel i); quadr_pq(i1, i2, m): the host
-        program's QUADR_PQ (needed by itype 3, btype 0/1 for the asymptotic
-        coefficients acf)."""
+        (element i-1 for orbital/channel i); quadr_pq: the host program's
+        QUADR_PQ (needed by itype 3, btype 0/1 for the asymptotic
+        coefficients acf), a callable quadr_pq(i1, i2, m) or a table
+        q(norb,norb,0:mmax) looked up by the library."""
         self._rs("dbs_rdata_dev", (itype, btype), kpol, lst, pq, norb, ibo, ipbs, kbort, vch, fppqq, iitar,
                  iptar, htarg, quadr_pq)
 

@@ -277,6 +277,15 @@ int dbs_overlap_ci_dev(dbs_ctx *ctx, int n, const int *ic, const int *jc, const 
  * QUADR_PQ (called at DBSR_MAT3/R_data.f90:224; defined in the un-vendored
  * Libraries_03). */
 typedef double (*dbs_quadr_pq_fn)(void *user, int i1, int i2, int m);
+/* A ready QUADR_PQ for callers that tabulate it: pass dbs_quadr_pq_table as
+ * the callback and a dbs_qpq_table as its user pointer; returns
+ * q(i1,i2,m), column-major (norb,norb,0:mmax), 1-based orbitals (0.0 for
+ * indices or m outside the table). */
+typedef struct dbs_qpq_table {
+  int norb, mmax;
+  const double *q;
+} dbs_qpq_table;
+double dbs_quadr_pq_table(void *user, int i1, int i2, int m);
 /* Bound-orbital and target data of a MAT3 run (module dbsr_mat,
  * DBSR_MAT3/mod_dbsr_mat.f90): pair packing base ibo (:61), ipbs(orbital)
  * (channel of a channel orbital, 0 for bound; 1-based orbitals), the host

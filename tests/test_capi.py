@@ -26,7 +26,24 @@ def test_library_exports_every_declared_symbol():
     missing = [f for f in declared_functions() if not hasattr(L, f)]
     assert not missing, missing
     # the Python bindings cover the header
-    assert set(declared_functions()) <= set(PROTOTYPES) | {"dbs_last_error"}
+    assert set(declared_functions()) <= set(PROTOTYPES) | {"dbs_last_error", "dbs_quadr_pq_table"}
+
+
+def test_quadr_pq_table_lookup():
+    """the tabulated QUADR_PQ callback reads q(i1,i2,m) column-major, 1-based,
+    and returns 0 outside the table (host code, no GPU)"""
+    import ctypes as C
+    from dbsr3_amd.capi import QpqTableC, lib
+    norb, mmax = 3, 2
+    q = np.asfortranarray(np.arange(norb * norb * (mmax + 1), dtype=np.float64).reshape(norb, norb, mmax + 1,
+                                                                                            order="F"))
+    t = QpqTableC(norb, mmax, q.ctypes.data_as(C.POINTER(C.c_double)))
+    f = lib().dbs_quadr_pq_table
+    for i in range(1, norb + 1):
+        for j in range(1, norb + 1):
+            for m in range(mmax + 1):
+                assert f(C.byref(t), i, j, m) == q[i - 1, j - 1, m]
+    assert f(C.byref(t), 0, 1, 0) == 0.0 and f(C.byref(t), 1, 4, 0) == 0.0 and f(C.byref(t), 1, 1, 3) == 0.0
 
 
 def test_product_has_no_oracle_dependency():

@@ -4,10 +4,10 @@
   (30 420 rows) and the S^k ppqq / pqqp sets with the same indices, on the
   reference's Fe.knot grid (ns=113) and on the Z=26 mkgrid_01 recipe grid
   (ns=208: exponential step e^0.125 from 1e-3/Z, then 0.25 to r=30);
-* config 3 -- 256 uneven instances (ns 60..400, ks 7..9, kmax 2..10) in one
-  engine, one hipGraph, autotuned; orbitals per instance 4..8 (fewer than
-  the 4..24 of the full configuration, to keep the all-pairs row count at
-  about 2e6);
+* config 3 -- 256 uneven instances (ns 60..400, ks 7..9, kmax 2..10, 4..24
+  orbitals) in one engine, one hipGraph, autotuned; the integral set of an
+  instance is every a<=c, b<=d row, or a seeded subset of 16384 of them
+  where there are more (3.2e6 rows in all);
 * config 5 -- ns=600, 60 orbitals, k = 0..12, 1e6 rows, 20 SCF-style
   iterations with perturbed orbitals replayed through one hipGraph.
 
@@ -121,30 +121,25 @@ def test_config1_all_pairs(config1_grid, kind):
 
 
 def test_config3_batched_uneven_instances(gpu_available):
+    from bench import config3_instances
     from dbsr3_amd import DBS, Engine
-    from dbsr3_amd.grid import mkgrid_02
-    rng = np.random.default_rng(SEED + 3)
-    ninst = 256
-    spec = []
-    for _ in range(ninst):
-        ns = int(rng.integers(60, 401))
-        ks = int(rng.choice([7, 8, 9]))
-        kmax = int(rng.integers(2, 11))
-        norb = int(rng.integers(4, 9))
-        spec.append((ns, ks, kmax, norb))
+    from conftest import cs_bound, err_report
+    spec = config3_instances()
+    ninst = len(spec)
     insts, grids, pqs, coefs = [], [], [], []
-    for i, (ns, ks, kmax, norb) in enumerate(spec):
-        t = mkgrid_02(ns, ks, 26.0, 0.25, 30.0)
+    for (t, ns, ks, kmax, pq0, rows, coef) in spec:
         g = O.Grid(t, ns, ks, ks - 1, ks)
-        fs = orbitals(g, norb, SEED + 3 + 1000 * i)
-        rows = all_pair_rows(norb, kmax)
+        pq = np.asfortranarray(pq0.copy())
+        for o in range(pq.shape[2]):
+            pq[:, :, o] /= math.sqrt(g.quadr(pq[:, :, o], pq[:, :, o], 0))
         d = DBS(t, ns, ks, ks - 1, ks)
-        insts.append((d, norb, rows))
-        grids.append((g, fs))
-        pqs.append(np.stack(fs, axis=2))
-        coefs.append(rng.uniform(-1, 1, len(rows)))
+        insts.append((d, pq.shape[2], rows))
+        grids.append((g, [np.asfortranarray(pq[:, :, o]) for o in range(pq.shape[2])]))
+        pqs.append(pq)
+        coefs.append(coef)
     nrows = sum(len(r) for _, _, r in insts)
-    print(f"[config3 {ninst} instances, {nrows} rows]", flush=True)
+    print(f"[config3 {ninst} instances, {nrows} rows, orbitals "
+          f"{min(x[1] for x in insts)}..{max(x[1] for x in insts)}]", flush=True)
     eng = Engine(insts)
     for i in range(ninst):
         eng.set_orbitals(i, pqs[i])
@@ -159,8 +154,10 @@ def test_config3_batched_uneven_instances(gpu_available):
     Es = [eng.energy(i) for i in range(ninst)]
     for i in range(ninst):
         assert abs(Es[i] - vals[i] @ coefs[i]) <= TOL * np.sum(np.abs(vals[i] * coefs[i]))
-    # batched = single instance, same launch shape
-    for i in rng.choice(ninst, 6, replace=False):
+    rng = np.random.default_rng(SEED + 33)
+    # batched = single instance, same launch shape; the largest orbital count included
+    big = int(np.argmax([x[1] for x in insts]))
+    for i in list(rng.choice(ninst, 5, replace=False)) + [big]:
         single = Engine([insts[i]])
         single.set_orbitals(0, pqs[i])
         single.set_coef(0, coefs[i])
@@ -169,9 +166,15 @@ def test_config3_batched_uneven_instances(gpu_available):
         assert np.max(relerr(single.values(), vals[i])) < 1e-14, i
         gs = np.max(np.abs(grads[i]))
         assert np.max(np.abs(single.grad() - grads[i])) <= 1e-14 * gs, i
+        # every row of this instance against the plain path
+        single.forward_plain()
+        plain = single.values()
+        cs = cs_bound(insts[i][0], insts[i][1], pqs[i], insts[i][2])
+        r = err_report(f"config3 inst {i} (ns={spec[i][1]}, norb={insts[i][1]}) vs plain", vals[i], plain, scale=cs)
+        assert r["floored"] < TOL and r["scaled"] < TOL, i
         single.close()
     # oracle on rows of the three smallest grids, one multipole each
-    for i in sorted(range(ninst), key=lambda j: spec[j][0])[:3]:
+    for i in sorted(range(ninst), key=lambda j: spec[j][1])[:3]:
         g, fs = grids[i]
         rows = insts[i][2]
         sel = np.nonzero(rows[:, 0] == 1)[0][::11][:30]
