@@ -158,12 +158,24 @@ void orc_zbsplvd(int ns, int ks, int nv, const double *t, int kg, int ni,
 }
 
 /* ===================================================================== */
-/* mkgrid_01 (point nucleus): DBSR_HF/dbsr_lib_dbs.f90:724-783            */
+/* first knot step of the grid recipes by nucleus (mkgrid_01 :737-745,    */
+/* mkgrid_02 :800-807): nucl 0 point hi/z, 1 Fermi 4 ln3 a_fermi hi,      */
+/* 2 uniform r_uniform hi                                                 */
+/* ===================================================================== */
+double orc_grid_start(int nucl, double hi, double z, double a_fermi, double r_uniform) {
+  if (nucl == 0) return hi / z;
+  if (nucl == 1) {
+    double ts = 4.0 * log(3.0) * a_fermi;
+    return ts * hi;
+  }
+  return r_uniform * hi;
+}
+
+/* ===================================================================== */
+/* mkgrid_01: DBSR_HF/dbsr_lib_dbs.f90:724-783, from the first step hs    */
 /* returns ns; t must hold ns+ks entries (tcap checked)                   */
 /* ===================================================================== */
-int orc_mkgrid_01(int ks, int ml, double hi, double he, double hmax, double rmax,
-                  double z, double *t, int tcap) {
-  double hs = hi / z;
+int orc_mkgrid_01s(int ks, int ml, double hs, double he, double hmax, double rmax, double *t, int tcap) {
   double ti = hs * ml, tj, ht = 1.0 + he;
   int ns = ks + ml, ne = ns;
   for (;;) {
@@ -189,14 +201,25 @@ int orc_mkgrid_01(int ks, int ml, double hi, double he, double hmax, double rmax
   return ns;
 }
 
-/* mkgrid_02 (point nucleus): DBSR_HF/dbsr_lib_dbs.f90:787-823 */
-void orc_mkgrid_02(int ns, int ks, double hi, double rmax, double z, double *t) {
-  double ti = hi / z, tmax = rmax;
+/* mkgrid_01, point nucleus */
+int orc_mkgrid_01(int ks, int ml, double hi, double he, double hmax, double rmax,
+                  double z, double *t, int tcap) {
+  return orc_mkgrid_01s(ks, ml, orc_grid_start(0, hi, z, 0.0, 0.0), he, hmax, rmax, t, tcap);
+}
+
+/* mkgrid_02: DBSR_HF/dbsr_lib_dbs.f90:787-823, from the first knot ti */
+void orc_mkgrid_02s(int ns, int ks, double ti, double rmax, double *t) {
+  double tmax = rmax;
   double he = exp((log(tmax) - log(ti)) / (ns - ks));
   for (int i = 1; i <= ks; ++i) T1(i) = 0.0;
   T1(ks + 1) = ti;
   for (int i = ks + 2; i <= ns + 1; ++i) T1(i) = T1(i - 1) * he;
   for (int i = ns + 1; i <= ns + ks; ++i) T1(i) = tmax;
+}
+
+/* mkgrid_02, point nucleus */
+void orc_mkgrid_02(int ns, int ks, double hi, double rmax, double z, double *t) {
+  orc_mkgrid_02s(ns, ks, orc_grid_start(0, hi, z, 0.0, 0.0), rmax, t);
 }
 #undef T1
 
@@ -1509,9 +1532,10 @@ int orc_o_data(int ms, int itype, int n, const int *k1, const int *k2, const dou
  * lives in the un-vendored Libraries_03.  Restated from its name and call
  * sites: A(ns,ns) unfolded from storage sym as Full_mat_sym does, then
  * c = A b (dir 'r') or c = A^T b (dir 'l'), each element summed over the
- * full index range in ascending order.  PARITY UNPINNED for 'n' storage and
- * the 'l' direction (for symmetric storage both directions coincide and the
- * contraction property pins it). */
+ * full index range in ascending order.  Pinned through S_data structures
+ * 2-5 ('n' storage, both directions) and R_data structure 2 ('s'): their
+ * vectors contracted with a channel function equal the compiled
+ * reference's sk_ppqq / sk_pqqp and rk() with it in the channel slot. */
 void orc_bav(int ns, int ks, const double *a, const double *b, double *c, char sym, char dir) {
   double *y = (double *)malloc(sizeof(double) * (size_t)ns * ns);
   unfold(ns, ks, a, sym, y);

@@ -75,6 +75,12 @@ def lib():
         L.orc_mkgrid_01.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                     C.c_double, C.c_double, _dp, C.c_int]
         L.orc_mkgrid_02.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, _dp]
+        L.orc_grid_start.restype = C.c_double
+        L.orc_grid_start.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]
+        L.orc_mkgrid_01s.restype = C.c_int
+        L.orc_mkgrid_01s.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, _dp,
+                                     C.c_int]
+        L.orc_mkgrid_02s.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, _dp]
         L.orc_rk_evaluate.argtypes = [C.c_int, C.c_int, C.c_int, _dp, C.c_int, C.c_int, _dp,
                                       _ip, _ip, _ip, _ip, C.c_double, _dp, C.c_int,
                                       _dp, C.c_int, C.c_int, _dp]
@@ -124,6 +130,44 @@ def mkgrid_01(ks, ml, hi, he, hmax, rmax, z):
 def mkgrid_02(ns, ks, hi, rmax, z):
     t = np.zeros(ns + ks)
     lib().orc_mkgrid_02(ns, ks, hi, rmax, z, _d(t))
+    return t
+
+
+NUCLEAR = {"point": 0, "Fermi": 1, "uniform": 2}
+
+
+def mkgrid(grid_type, ks, hi, rmax, z, nuclear="point", a_fermi=0.0, r_uniform=0.0, ml=1, he=0.25, hmax=1.0,
+           ns=0):
+    """mkgrid_01 (grid_type 1) / mkgrid_02 (grid_type 2, given ns) with the
+    first step of the nucleus model; returns (t, ns)"""
+    L = lib()
+    h0 = L.orc_grid_start(NUCLEAR[nuclear], hi, z, a_fermi, r_uniform)
+    if grid_type == 1:
+        ns = L.orc_mkgrid_01s(ks, ml, h0, he, hmax, rmax, None, 0)
+        t = np.zeros(ns + ks)
+        L.orc_mkgrid_01s(ks, ml, h0, he, hmax, rmax, _d(t), ns + ks)
+    else:
+        t = np.zeros(ns + ks)
+        L.orc_mkgrid_02s(ns, ks, h0, rmax, _d(t))
+    return t, ns
+
+
+def ref_mkgrid(grid_type, ks, hi, rmax, z, nuclear="point", a_fermi=0.0, r_uniform=0.0, ml=1, he=0.25,
+               hmax=1.0, ns=0):
+    """the same by the reference's own mkgrid_01/02 (oracle/_ref)"""
+    R = ref()
+    args = (grid_type, ks, ml, hi, he, hmax, rmax, z, NUCLEAR[nuclear], a_fermi, r_uniform, ns)
+    n = R.ref_mkgrid(*args, None, 0)
+    t = np.zeros(n + ks)
+    R.ref_mkgrid(*args, _d(t), n + ks)
+    return t, n
+
+
+def ref_read_grid(path, ns, ks):
+    """the reference's read_grid on a knot file"""
+    t = np.zeros(ns + ks)
+    b = path.encode()
+    assert ref().ref_read_grid(b, len(b), ns, ks, _d(t)) == 0
     return t
 
 
@@ -358,6 +402,38 @@ def rdata_acf(lst, ibo, kpol, mk, qpq, icc, acf):
                                acf.shape[0], _d(acf))
 
 
+def idef_itype(atype, i1, i2, i3, i4, c, ic, jc, io, jo, ipbs, ibo):
+    """Idef_itype (DBSR_MAT3/idef_type.f90:306-437); returns (itype + 10*atype,
+    key) (itype 0: dropped, < 0: the reference stops)"""
+    L = lib()
+    L.orc_idef_itype.restype = C.c_int
+    L.orc_idef_itype.argtypes = [C.c_int] * 5 + [C.c_double] + [C.c_int] * 4 + [_ip, C.c_int, _ip]
+    key = np.zeros(4, dtype=np.int32)
+    ip = np.ascontiguousarray(ipbs, dtype=np.int32)
+    it = L.orc_idef_itype(atype, i1, i2, i3, i4, c, ic, jc, io, jo, _i(ip), ibo, _i(key))
+    return it, tuple(int(x) for x in key)
+
+
+def idef_otype(c, ic, jc, io, jo, eps_c=1e-10, check_target=1):
+    """Idef_Otype (DBSR_MAT3/idef_type.f90:2-50); returns (itype, key)"""
+    L = lib()
+    L.orc_idef_otype.restype = C.c_int
+    L.orc_idef_otype.argtypes = [C.c_double] + [C.c_int] * 4 + [C.c_double, C.c_int, _ip]
+    key = np.zeros(4, dtype=np.int32)
+    it = L.orc_idef_otype(c, ic, jc, io, jo, eps_c, check_target, _i(key))
+    return it, tuple(int(x) for x in key)
+
+
+def add_integral_ci(cd, kpol, ic, jc, i1, i2, i3, i4, c, kappa, ibi=2 ** 15, mbreit=0):
+    """CI3 Add_integral (DBSR_CI3/add_integral.f90:2-43) into the oracle's
+    CData cd; kappa indexed by orbital (element 0 unused)"""
+    L = lib()
+    L.orc_add_integral_ci.restype = C.c_int
+    L.orc_add_integral_ci.argtypes = [C.c_void_p] + [C.c_int] * 7 + [C.c_double, _ip, C.c_int, C.c_int]
+    k = np.ascontiguousarray(kappa, dtype=np.int32)
+    return L.orc_add_integral_ci(cd.h, kpol, ic, jc, i1, i2, i3, i4, c, _i(k), ibi, mbreit)
+
+
 def idef_rtype(j1, j2, j3, j4, ipbs, ibo):
     ip = np.ascontiguousarray(ipbs, dtype=np.int32)
     out = [np.zeros(1, dtype=np.int32) for _ in range(4)]
@@ -459,6 +535,13 @@ def ref():
             raise RuntimeError("reference build missing: " + REF_PATH)
         L = C.CDLL(REF_PATH)
         L.ref_setup.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _dp]
+        L.ref_mkgrid.restype = C.c_int
+        L.ref_mkgrid.argtypes = [C.c_int, C.c_int, C.c_int] + [C.c_double] * 5 + [C.c_int, C.c_double,
+                                                                                C.c_double, C.c_int, _dp, C.c_int]
+        L.ref_breit_energy.restype = C.c_double
+        L.ref_breit_energy.argtypes = [C.c_double]
+        L.ref_read_grid.restype = C.c_int
+        L.ref_read_grid.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, _dp]
         L.ref_tables.argtypes = [_dp, _dp, _dp, _dp]
         L.ref_gauleg.argtypes = [C.c_int, _dp, _dp]
         L.ref_moments.argtypes = [C.c_int, C.c_int, _dp]

@@ -293,3 +293,75 @@
       k_dhl = kbs(i)
       Call hf_matrix(i, hfm)
       End Subroutine ref_hf_matrix
+
+      Function ref_mkgrid(gtype, ks_, ml_, hi_, he_, hmax_, rmax_, z, nucl, afermi, runiform, ns_in, t_, tcap) &
+               result(nsout) bind(C, name='ref_mkgrid')
+! ... the reference's mkgrid_01 (gtype 1) / mkgrid_02 (gtype 2) with the
+! ... nucleus of DBS_nuclear: nucl 0 'point', 1 'Fermi' (a_fermi), 2
+! ... 'uniform' (r_uniform); mkgrid_02 takes ns_in.  Returns ns and, when
+! ... tcap >= ns+ks, the knots.
+      Use iso_c_binding
+      Use DBS_grid
+      Use DBS_nuclear
+      Implicit none
+      Integer(c_int), value :: gtype, ks_, ml_, nucl, ns_in, tcap
+      Real(c_double), value :: hi_, he_, hmax_, rmax_, z, afermi, runiform
+      Real(c_double) :: t_(*)
+      Integer(c_int) :: nsout
+      ks = ks_; ml = ml_; hi = hi_; he = he_; hmax = hmax_; rmax = rmax_
+      atomic_number = z; a_fermi = afermi; r_uniform = runiform
+      nuclear = 'point'
+      if (nucl == 1) nuclear = 'Fermi'
+      if (nucl == 2) nuclear = 'uniform'
+      if (allocated(t)) Deallocate(t)
+      if (gtype == 1) then
+        Call mkgrid_01
+      else
+        ns = ns_in
+        Call mkgrid_02
+      end if
+      nsout = ns
+      if (tcap >= ns + ks) t_(1:ns + ks) = t(1:ns + ks)
+      End Function ref_mkgrid
+
+      Function ref_read_grid(path, plen, ns_, ks_, t_) result(rc) bind(C, name='ref_read_grid')
+! ... the reference's read_grid on a knot file (grid points listed after
+! ... 'grid points:'); ns_ and ks_ as the file states them
+      Use iso_c_binding
+      Use DBS_grid
+      Implicit none
+      Integer(c_int), value :: plen, ns_, ks_
+      Character(kind=c_char) :: path(plen)
+      Real(c_double) :: t_(ns_ + ks_)
+      Integer(c_int) :: rc
+      Character(512) :: fname
+      Integer :: i, nu
+      fname = ' '
+      Do i = 1, plen
+       fname(i:i) = path(i)
+      End do
+      nu = 77
+      Open(nu, file=trim(fname), status='old')
+      ns = ns_; ks = ks_
+      Call read_grid(nu)
+      Close(nu)
+      t_ = t(1:ns + ks)
+      rc = 0
+      End Function ref_read_grid
+
+      Function ref_breit_energy(eps) result(e) bind(C, name='ref_breit_energy')
+! ... the reference's Breit_energy (DBSR_HF/hf_energy_BR.f90:2-50) for the
+! ... state set by ref_hf_setup (orbitals, kbs, coef(i,j,k), kmax); eps is
+! ... its cut on the merged coefficients (module dbsr_hf eps_c)
+      Use iso_c_binding
+      Use dbsr_hf, only: eps_c
+      Use rk4_data
+      Implicit none
+      Real(c_double), value :: eps
+      Real(c_double) :: e
+      Real(8) :: eb
+      eps_c = eps
+      nrk = 0
+      Call Breit_energy(eb)
+      e = eb
+      End Function ref_breit_energy

@@ -8,11 +8,12 @@ compiled here.  Each check reaches the matrices by another route:
   into hp / hcp (V_ch columns) / hch (rank-1) / htarg + fppqq;
 * structure 2 (R): the channel-bound vectors contracted with a random
   channel function u equal sum c * R^k with u in the channel slot, by rk();
-* structures 6 and 8 (S): the channel block contracted with random channel
+* structures 6-9 (S): the channel block contracted with random channel
   functions u, w equals sum c * S^k with u, w in the channel slots, by the
-  reference's sk_ppqq / sk_pqqp.
-BAV's 'n' storage and 'l' direction (S structures 2-5) stay unpinned: BAV
-is defined in the un-vendored Libraries_03.
+  reference's sk_ppqq / sk_pqqp;
+* structures 2-5 (S, the BAV vectors, which use BAV's 'n' storage and both
+  directions): the vectors contracted with a channel function equal sum c *
+  S^k with it in the channel slot, by the reference's sk_ppqq / sk_pqqp.
 """
 import numpy as np
 import pytest
@@ -230,3 +231,86 @@ def test_sdata_channel_blocks_contract_to_reference_sk(atype, itype):
     cols = slice((ip2 - 1) * ns, ip2 * ns)
     got = u[:, ip1 - 1] @ blk[rows, cols] @ w[:, ip2 - 1]
     assert abs(got - ref) <= 1e-13 * max(abs(ref), 1e-300)
+
+
+def _vec(f):
+    return np.concatenate([f[:, 0], f[:, 1]])
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference build (oracle/_ref) absent")
+@pytest.mark.parametrize("atype", [0, 1])
+@pytest.mark.parametrize("itype", [6, 7, 8, 9])
+def test_sdata_channel_blocks_full_contraction(atype, itype):
+    """every channel-channel S structure: U^T H(2,1) W over the whole ms x ms
+    block (both channel functions with P and Q parts) equals sum c * S^k with
+    U, W in the slots the reference's structure table gives them
+    (DBSR_MAT3/idef_type.f90:350-356): 6 Sk(i . j .): S(U,k2;W,k3); 7 Sk(. i
+    . j): S(k2,U;k3,W); 8 Sk(i . . j): S(U,k3;k2,W); 9 Sk(. i j .), k1 =
+    (ipbs(i3), ipbs(i2)): S(k2,W;U,k3)"""
+    g = _grid()
+    ns, ms = g.ns, g.ms
+    fs = random_orbitals(g, NORB, 120 + 10 * itype + atype)
+    rng = np.random.default_rng(130 + 10 * itype + atype)
+    u, w = (np.asfortranarray(rng.standard_normal((ns, 2))) for _ in range(2))
+    R = _ref_setup(g, fs + [u, w])
+    U, W = NORB + 1, NORB + 2
+    n, k = 14, 2
+    a2, a3 = rng.integers(1, NORB + 1, n), rng.integers(1, NORB + 1, n)
+    c = rng.uniform(-1, 1, n)
+    lst = (c, np.full(n, 2 * IBO + 1), a2, a3, np.zeros(n, dtype=np.int64))
+    m = _mats(2, 1, ms)
+    z = np.zeros((ms, 1), order="F")
+    assert O.rs_data(g, 1, itype, atype, k, lst, _pq(fs), IBO, np.zeros(NORB + 1, np.int32), lambda a, b: 1, z, z,
+                     0, np.zeros(3, np.int32), np.zeros(1), m) == 0
+    blk = m["hch"][:, :, m["icc"][1, 0] - 1]  # (ich, jch) = (2, 1): channel 2 <-> U, 1 <-> W
+    slots = {6: lambda p, q: (U, p, W, q), 7: lambda p, q: (p, U, q, W),
+             8: lambda p, q: (U, q, p, W), 9: lambda p, q: (p, W, U, q)}[itype]
+    ref = sum(cc * R.ref_sk(5 + atype, *(int(x) for x in slots(p, q)), k) for cc, p, q in zip(c, a2, a3))
+    got = _vec(u) @ blk @ _vec(w)
+    assert abs(got - ref) <= 1e-13 * max(abs(ref), 1e-300), (got, ref)
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference build (oracle/_ref) absent")
+@pytest.mark.parametrize("atype", [0, 1])
+@pytest.mark.parametrize("itype", [2, 3, 4, 5])
+def test_sdata_vectors_contract_to_reference_sk(atype, itype):
+    """the bound-channel S structures 2.1-5.1 (BAV vectors into hcp,
+    DBSR_MAT3/S_data.f90:111-236, including BAV's 'n' storage and 'l'
+    direction): U^T hcp(:, icb(ich, ic)) equals sum c * S^k with the channel
+    function U in the channel slot the structure table gives
+    (idef_type.f90:339-342): 2 S(U,a;k2,c), 3 S(a,U;c,k2), 4 S(k2,a;U,c),
+    5 S(a,k2;c,U), with (a,c) the pair K1.  The integral uses U's component
+    ip1 (the channel's), and the reference stores the vector in the ip2 half
+    of the channel column (S_data.f90:213-215, kept as written), so U's ip1
+    component is contracted with that half; the other half stays zero."""
+    g = _grid()
+    ns, ms = g.ns, g.ms
+    fs = random_orbitals(g, NORB, 140 + 10 * itype + atype)
+    rng = np.random.default_rng(150 + 10 * itype + atype)
+    u = np.asfortranarray(rng.standard_normal((ns, 2)))
+    R = _ref_setup(g, fs + [u])
+    U = NORB + 1
+    n, k, npert = 16, 1, 2
+    K1 = rng.integers(1, NORB + 1, n) * IBO + rng.integers(1, NORB + 1, n)
+    K2 = rng.integers(1, NORB + 1, n)
+    K4 = -rng.integers(1, npert + 1, n)
+    c = rng.uniform(-1, 1, n)
+    order = np.lexsort((K2, K1))
+    lst = tuple(x[order] for x in (c, K1, K2, np.ones(n, dtype=np.int64), K4))
+    m = _mats(1, npert, ms)
+    z = np.zeros((ms, 1), order="F")
+    assert O.rs_data(g, 1, itype, atype, k, lst, _pq(fs), IBO, np.zeros(NORB + 1, np.int32), lambda a, b: 1, z, z,
+                     0, np.zeros(3, np.int32), np.zeros(1), m) == 0
+    slots = {2: lambda a, cc, q: (U, a, q, cc), 3: lambda a, cc, q: (a, U, cc, q),
+             4: lambda a, cc, q: (q, a, U, cc), 5: lambda a, cc, q: (a, q, cc, U)}[itype]
+    ref = np.zeros(npert)
+    for cc, k1, k2, _, k4 in zip(*lst):
+        a, c2 = divmod(int(k1), IBO)
+        ref[-k4 - 1] += cc * R.ref_sk(5 + atype, *slots(a, c2, int(k2)), k)
+    ip1, ip2 = {(0, 2): (1, 2), (0, 3): (1, 2), (0, 4): (2, 1), (0, 5): (2, 1),
+                (1, 2): (1, 2), (1, 3): (2, 1), (1, 4): (2, 1), (1, 5): (1, 2)}[(atype, itype)]
+    half = slice((ip2 - 1) * ns, ip2 * ns)
+    other = slice((2 - ip2) * ns, (3 - ip2) * ns)
+    got = np.array([u[:, ip1 - 1] @ m["hcp"][half, m["icb"][0, ic] - 1] for ic in range(npert)])
+    assert np.max(np.abs(got - ref)) <= 1e-13 * np.max(np.abs(ref)), (got, ref)
+    assert not np.any(m["hcp"][other, :])

@@ -206,3 +206,45 @@ def test_overlaps_and_full_mat_sym(name):
         y = np.zeros((ns, ns), order="F")
         R.ref_full_mat_sym(ns, ks, O._d(x), O._d(y), sym.encode())
         assert np.array_equal(O.full_mat_sym(ns, ks, x, sym), y), sym
+
+
+GRID_CASES = [
+    (1, "point", 26.0, 0.25, 0.25, 1.0, 50.0, 1, 0.0, 0.0, 0),
+    (1, "point", 37.0, 0.25, 0.25, 0.65, 50.0, 2, 0.0, 0.0, 0),
+    (1, "Fermi", 26.0, 0.25, 0.25, 1.0, 50.0, 1, 9.890591209340532e-06, 0.0, 0),
+    (1, "uniform", 54.0, 0.3, 0.2, 0.8, 40.0, 1, 0.0, 1.2e-4, 0),
+    (2, "point", 26.0, 0.25, 0.25, 1.0, 30.0, 1, 0.0, 0.0, 150),
+    (2, "Fermi", 2.0, 0.25, 0.25, 1.0, 25.0, 1, 6.520314073119532e-06, 0.0, 90),
+    (2, "uniform", 10.0, 0.25, 0.25, 1.0, 30.0, 1, 0.0, 3e-5, 75),
+]
+
+
+@pytest.mark.parametrize("case", GRID_CASES)
+def test_mkgrid_matches_reference(case):
+    """mkgrid_01 / mkgrid_02 (DBSR_HF/dbsr_lib_dbs.f90:724-823) with each
+    nucleus model: the oracle and the host grid module equal the
+    reference's own routines bit for bit"""
+    from dbsr3_amd import grid as G
+    gt, nuc, z, hi, he, hmax, rmax, ml, af, ru, ns = case
+    for ks in (7, 8, 9):
+        tr, nr = O.ref_mkgrid(gt, ks, hi, rmax, z, nuc, af, ru, ml, he, hmax, ns)
+        to, no = O.mkgrid(gt, ks, hi, rmax, z, nuc, af, ru, ml, he, hmax, ns)
+        assert nr == no and np.array_equal(tr, to), (case, ks)
+        if gt == 1:
+            tg, ng = G.mkgrid_01(ks, z, hi, he, hmax, rmax, ml, nuc, af, ru)
+        else:
+            tg, ng = G.mkgrid_02(ns, ks, z, hi, rmax, nuc, af, ru), ns
+        assert ng == nr and np.array_equal(tg, tr), (case, ks)
+
+
+@pytest.mark.parametrize("name", ["Fe.knot", "He.knot", "Rb_cc27_knot.dat", "Be_092m1.knot"])
+def test_read_grid_matches_reference(name):
+    """read_grid (DBSR_HF/dbsr_lib_dbs.f90:611-631) on the reference's own
+    knot files: the host reader lists the same knots bit for bit"""
+    import os
+    from conftest import GOLDEN
+    from dbsr3_amd.grid import read_knot
+    path = os.path.join(GOLDEN, "knots", name)
+    p = read_knot(path)
+    t = O.ref_read_grid(path, p["ns"], p["ks"])
+    assert np.array_equal(p["points"], t)
