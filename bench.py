@@ -126,65 +126,100 @@ def normalise_on_gpu(d, pq):
     return out
 
 
-def cpu_baseline(t, ns, ks, ksp, ksq, pq, rows, budget_s):
-    """The oracle (C restatement of mrk_*/convol/SUM_AmB as rk() and
-    Rk_evaluate use them) on the rows taken in multipole order, single
-    thread, until the time budget is spent: the R^k arrays of each k are
-    built inside the timed region as the reference builds them on first use."""
+def host_cpu():
+    """(threads the all-cores baseline uses, logical CPUs available, CPU model):
+    OMP_NUM_THREADS when set (the GPU box sets it to its CPU share), else all
+    CPUs this process may run on"""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    threads = int(os.environ.get("OMP_NUM_THREADS", n))
+    return max(1, min(threads, n)), n, model
+
+
+def cpu_rows(kind, t, ns, ks, ksp, ksq, pq, rows, budget_s, nthreads=1):
+    """CPU evaluation of `rows` (rk() type sum) in Rk_evaluate's organisation
+    (one convolution per left pair, DBSR_CI3/rk_evaluate.f90:31-56), lowest
+    multipoles first, until the time budget is spent, with the arrays of
+    each multipole built inside the timed region: kind 'reference' -- the
+    reference's own density/convol/SUM_AmB (oracle/_ref) on one thread;
+    'port' -- the oracle's OpenMP port of the same on nthreads threads.
+    Returns (rows done, seconds, their indices in `rows`, values) or None."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    g = O.Grid(t, ns, ks, ksp, ksq)
-    fs = [np.asfortranarray(pq[:, :, o]) for o in range(pq.shape[2])]
-    order = np.argsort(rows[:, 0], kind="stable")
-    t0 = time.perf_counter()
-    vals = O.rk_rows(g, fs, rows[order], deadline=t0 + budget_s)
-    t1 = time.perf_counter()
-    n = len(vals)
-    return n, t1 - t0, order[:n], vals
+    rs, perm = O.sort_rows_by_left_pair(rows)
+    if kind == "reference":
+        if not O.ref_available():
+            return None
+        R = O.ref()
+        R.ref_setup(ns, ks, ksp, ksq, O._d(O.f64(t)))
+        t0 = time.perf_counter()
+        v = O.ref_rk_rows(pq, rs, budget_s)
+    else:
+        g = O.Grid(t, ns, ks, ksp, ksq)
+        t0 = time.perf_counter()
+        v = O.rk_rows_par(g, pq, rs, nthreads, budget_s)
+    secs = time.perf_counter() - t0
+    return len(v), secs, perm[:len(v)], v
 
 
-def cpu_baseline_ref(t, ns, ks, ksp, ksq, pq, rows, budget_s):
-    """The reference's own rk() (DBSR_HF/dbsr_lib_dbs.f90:3981-4026, compiled
-    from the reference's Fortran into oracle/_ref): per row four densities,
-    four convolutions and four SUM_AmB, the arrays of each k built by mrk_*
-    on first use; rows in multipole order, single thread, until the time
-    budget is spent.  None when the reference build is absent."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O
-    if not O.ref_available():
-        return None
-    R = O.ref()
-    tt = O.f64(t)
-    R.ref_setup(ns, ks, ksp, ksq, O._d(tt))
-    fs = [np.asfortranarray(pq[:, :, o]) for o in range(pq.shape[2])]
-    order = np.argsort(rows[:, 0], kind="stable")
-    vals = []
-    t0 = time.perf_counter()
-    for r in rows[order]:
-        k, a, b, c, d = (int(x) for x in r)
-        vals.append(R.ref_rk(O._d(fs[a - 1]), O._d(fs[b - 1]), O._d(fs[c - 1]), O._d(fs[d - 1]), k))
-        if time.perf_counter() - t0 > budget_s:
-            break
-    t1 = time.perf_counter()
-    n = len(vals)
-    return n, t1 - t0, order[:n], np.array(vals)
+def cpu_lines(t, ns, ks, ksp, ksq, pq, rows, budget_s, gpu_vals=None, note=""):
+    """(cpu_baseline, all-cores baseline, extras): the reference's own
+    routines on one thread and the port on all host cores, each on a bounded
+    sample (whole multipoles, lowest first), with their agreement with the
+    GPU values of the same rows"""
+    threads, ncpu, model = host_cpu()
+    extra = {}
+    base = None
+    ref = cpu_rows("reference", t, ns, ks, ksp, ksq, pq, rows, budget_s)
+    par = cpu_rows("port", t, ns, ks, ksp, ksq, pq, rows, budget_s, threads)
+    for name, res, cores, kind in (("reference", ref, 1, "reference"), ("port", par, threads, "port")):
+        if res is None or res[0] == 0:
+            continue
+        n, secs, sel, v = res
+        ks_done = sorted(set(int(x) for x in rows[sel, 0]))
+        line = {"value": n / secs, "unit": "R^k integrals/s", "cores": cores, "kind": kind,
+                "sample": (f"{'the reference' + chr(39) + 's own density/convol/SUM_AmB (oracle/_ref)' if kind == 'reference' else 'the oracle port (OpenMP over (k, type) units)'}"
+                           f" in Rk_evaluate's organisation (one convolution per left pair), arrays built inside "
+                           f"the timed region, on {n} of {len(rows)} rows (k={ks_done[0]}..{ks_done[-1]}), "
+                           f"{secs:.1f} s, {cores} thread(s){note}")}
+        if kind == "port":
+            line.update({"nproc": ncpu, "cpu_model": model})
+        if gpu_vals is not None:
+            scale = np.max(np.abs(v)) if len(v) else 1.0
+            extra[f"max_rel_err_vs_cpu_{name}"] = float(np.max(np.abs(gpu_vals[sel] - v) /
+                                                             np.maximum(np.abs(v), 1e-3 * scale)))
+        if kind == "reference":
+            base = line
+        else:
+            extra["cpu_all_cores"] = line
+    if base is None:
+        base = extra.pop("cpu_all_cores", None)
+    return base, extra
 
 
-def with_reference(cpu, t, ns, ks, ksp, ksq, pq, rows, budget_s, note=""):
-    """(baseline, port): the reference's own rk() on a sample of the rows
-    when oracle/_ref is built, with the port's figure `cpu` kept beside it;
-    (cpu, None) otherwise"""
-    if cpu is None:
-        return cpu, None
-    ref = cpu_baseline_ref(t, ns, ks, ksp, ksq, pq, rows, budget_s)
-    if ref is None or ref[0] == 0:
-        return cpu, None
-    n, secs, sel, _ = ref
-    kk = sorted(set(int(x) for x in rows[sel, 0]))
-    return ({"value": n / secs, "unit": cpu["unit"], "cores": 1, "kind": "reference",
-             "sample": f"the reference's rk() (DBSR_HF/dbsr_lib_dbs.f90, compiled into oracle/_ref) on the "
-                       f"first {n} rows in multipole order (k={kk[0]}..{kk[-1]}; arrays of each k built by "
-                       f"mrk_* inside the timed region), {secs:.1f} s single-thread{note}"}, cpu)
+def step_work(eng, prof, ms, with_adjoint=True):
+    """whole-step algorithmic work from the engine's per-phase counts (each
+    phase: flops of the factored form; bytes read and written once) at the
+    measured step time, with the minimum bytes of the problem (orbitals,
+    coefficients, outputs)"""
+    fl = sum(v[1] for v in prof.values())
+    by = sum(v[2] for v in prof.values())
+    _, bmin = eng.work(with_adjoint=with_adjoint)
+    sec = ms * 1e-3
+    return {"flops": fl, "tflops": fl / sec / 1e12, "fp64_frac": fl / sec / 1e12 / PEAK_FP64_TFS,
+            "bytes_phases": by, "gbs_phases": by / sec / 1e9, "hbm_frac_phases": by / sec / 1e9 / PEAK_HBM_GBS,
+            "bytes_min": bmin}
 
 
 def roofline_of(prof):
@@ -279,13 +314,6 @@ def config3_instances(seed=20261203 + 3, ninst=256):
     return out
 
 
-def cpu_sample(fn, budget_s):
-    """run fn(deadline) single-thread; returns (units done, seconds)"""
-    t0 = time.perf_counter()
-    n = fn(t0 + budget_s)
-    return n, time.perf_counter() - t0
-
-
 def config_line(cfg, value, unit, ms, steps, warmup, workload, extra):
     line = {"metric": f"config {cfg}: {unit}", "value": value, "unit": unit, "n_gpus": 1,
             "steps": steps, "warmup": warmup, "ms_per_step": ms, "higher_is_better": True,
@@ -300,8 +328,6 @@ def run_config1(args):
     import torch
     from dbsr3_amd import DBS, Engine
     from dbsr3_amd.grid import mkgrid_01
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O
     norb, kmax = 12, 4
     rows = all_pair_rows(norb, kmax)
     grids = {"recipe": None, "fe": None}
@@ -319,7 +345,9 @@ def run_config1(args):
             ms = time_graph(eng, False, args.steps, args.warmup)
             r = {"ms": ms, "integrals_per_s": len(rows) / (ms * 1e-3)}
             if kind == 0 and gname == "recipe":
-                roof, _ = roofline_of(eng.profile(with_adjoint=False, reps=3))
+                prof = eng.profile(with_adjoint=False, reps=3)
+                roof, _ = roofline_of(prof)
+                work = step_work(eng, prof, ms, with_adjoint=False)
                 vals = eng.values()
                 eng.forward_plain()
                 torch.cuda.synchronize()
@@ -327,32 +355,22 @@ def run_config1(args):
                 eng.forward_plain()
                 plain_ms = (time.perf_counter() - p0) * 1e3
                 r["plain_gpu_ms"] = plain_ms
-                main = (ms, roof, t, ns, pq, vals)
+                main = (ms, roof, t, ns, pq, vals, work)
             rates[f"{gname}_ns{ns}_{kname}"] = r
             eng.close()
-    ms, roof, t, ns, pq, vals = main
+    ms, roof, t, ns, pq, vals, work = main
     value = len(rows) / (ms * 1e-3)
-    cpu = None
-    if not args.no_cpu:
-        g = O.Grid(t, ns, 9, 8, 9)
-        fs = [np.asfortranarray(pq[:, :, o]) for o in range(norb)]
-        got = {}
-        def work(deadline):
-            got["v"] = O.rk_rows(g, fs, rows, deadline=deadline)
-            return len(got["v"])
-        n, secs = cpu_sample(work, args.cpu_seconds)
-        if n > 0:
-            cpu = {"value": n / secs, "unit": "R^k integrals/s", "cores": 1, "kind": "port",
-                   "sample": f"first {n} rows (k=0 first; arrays built inside the timed region), "
-                             f"{secs:.1f} s single-thread"}
     workload = {"workload": "all R^k/S^k of 12 orbitals, a<=c, b<=d, k=0..4 (30420 rows per set), "
                             "forward only, one hipGraph per step; value: R^k on the recipe grid",
                 "rows": int(len(rows)), "norb": norb, "kmax": kmax, "grids": {"recipe": grids["recipe"][1],
                                                                              "fe": 113}}
-    cpu, port = with_reference(cpu, t, ns, 9, 8, 9, pq, rows[np.argsort(rows[:, 0], kind="stable")],
-                               args.cpu_seconds / 2)
+    cpu, extra = (None, {}) if args.no_cpu else cpu_lines(t, ns, 9, 8, 9, pq, rows, args.cpu_seconds, vals)
+    if cpu:
+        extra["batched_vs_cpu"] = value / cpu["value"]
+    if "cpu_all_cores" in extra:
+        extra["batched_vs_cpu_all_cores"] = value / extra["cpu_all_cores"]["value"]
     return config_line(1, value, "R^k integrals/s", ms, args.steps, args.warmup, workload,
-                       {"sets": rates, "roofline": roof, "cpu_baseline": cpu, "cpu_port": port})
+                       {"sets": rates, "roofline": roof, "step_work": work, "cpu_baseline": cpu, **extra})
 
 
 CONFIG2 = dict(ns=150, ks=9, ksp=8, ksq=9, nch=40, nbound=20, kmax=6, nent=200000, ibo=2 ** 15)
@@ -521,8 +539,6 @@ def run_config2(args):
 
 def run_config3(args):
     from dbsr3_amd import DBS, Engine
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O
     spec = config3_instances()
     insts, host = [], []
     for (t, ns, ks, kmax, pq0, rows, coef) in spec:
@@ -542,38 +558,57 @@ def run_config3(args):
         tuning = eng.autotune(with_adjoint=True, reps=2)
     ms = time_graph(eng, True, args.steps, args.warmup)
     value = nrows / (ms * 1e-3)
-    roof, _ = roofline_of(eng.profile(with_adjoint=True, reps=3))
+    prof = eng.profile(with_adjoint=True, reps=3)
+    roof, _ = roofline_of(prof)
+    work = step_work(eng, prof, ms)
+    extra = {}
     cpu = None
     if not args.no_cpu:
+        # whole multipoles of the smallest instances first, until the budget is spent
+        threads, ncpu, model = host_cpu()
         order = sorted(range(len(host)), key=lambda j: host[j][1])
-        def work(deadline):
+        for kind, cores in (("reference", 1), ("port", threads)):
             n = 0
+            t0 = time.perf_counter()
             for j in order:
                 t, ns, ks, pq, rows, _ = host[j]
-                g = O.Grid(t, ns, ks, ks - 1, ks)
-                fs = [np.asfortranarray(pq[:, :, o]) for o in range(pq.shape[2])]
-                n += len(O.rk_rows(g, fs, rows, deadline=deadline))
-                if time.perf_counter() > deadline:
+                left = args.cpu_seconds - (time.perf_counter() - t0)
+                if left <= 0:
                     break
-            return n
-        n, secs = cpu_sample(work, args.cpu_seconds)
-        if n > 0:
-            cpu = {"value": n / secs, "unit": "R^k integrals/s", "cores": 1, "kind": "port",
-                   "sample": f"{n} rows of the smallest instances first (arrays built inside the timed "
-                             f"region), {secs:.1f} s single-thread; the GPU figure is forward+adjoint"}
+                res = cpu_rows(kind, t, ns, ks, ks - 1, ks, pq, rows, left, cores)
+                if res is None:
+                    break
+                n += res[0]
+            secs = time.perf_counter() - t0
+            if n == 0:
+                continue
+            line = {"value": n / secs, "unit": "R^k integrals/s", "cores": cores, "kind": kind,
+                    "sample": f"{n} rows of the smallest instances first, whole multipoles (arrays built inside "
+                              f"the timed region), Rk_evaluate's organisation, {secs:.1f} s, {cores} thread(s); "
+                              f"the GPU figure is forward+adjoint"}
+            if kind == "reference":
+                cpu = line
+            else:
+                line.update({"nproc": ncpu, "cpu_model": model})
+                extra["cpu_all_cores"] = line
+        if cpu is None:
+            cpu = extra.pop("cpu_all_cores", None)
+        if cpu:
+            extra["batched_vs_cpu"] = value / cpu["value"]
+        if "cpu_all_cores" in extra:
+            extra["batched_vs_cpu_all_cores"] = value / extra["cpu_all_cores"]["value"]
     workload = {"workload": "256 uneven instances (ns 60..400, ks 7..9, kmax 2..10, 4..24 orbitals; all "
                             f"a<=c, b<=d pairs, or a seeded subset of {CONFIG3_ROWS_MAX} rows per instance) "
                             "forward+adjoint, one hipGraph per step, autotuned",
                 "instances": len(spec), "rows": int(nrows), "tuning": tuning}
     return config_line(3, value, "R^k integrals/s", ms, args.steps, args.warmup, workload,
-                       {"roofline": roof, "cpu_baseline": cpu, "energy_0": eng.energy(0)})
+                       {"roofline": roof, "step_work": work, "cpu_baseline": cpu, "energy_0": eng.energy(0),
+                        **extra})
 
 
 def run_config5(args):
     from dbsr3_amd import DBS, Engine
     from dbsr3_amd.grid import mkgrid_02
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O
     ns, ks, ksp, ksq, norb, kmax, nrows = 600, 9, 8, 9, 60, 12, 1_000_000
     t = mkgrid_02(ns, ks, 26.0, 0.25, 30.0)
     d = DBS(t, ns, ks, ksp, ksq)
@@ -594,26 +629,26 @@ def run_config5(args):
     # one SCF iteration = new orbitals (a 576 KB upload) + the captured step
     ms = time_graph(eng, True, 20, args.warmup, before=lambda i: eng.set_orbitals(0, pq0 + i * bump))
     value = nrows / (ms * 1e-3)
-    roof, _ = roofline_of(eng.profile(with_adjoint=True, reps=2))
+    prof = eng.profile(with_adjoint=True, reps=2)
+    roof, _ = roofline_of(prof)
+    work = step_work(eng, prof, ms)
+    extra = {}
     cpu = None
     if not args.no_cpu:
-        g = O.Grid(t, ns, ks, ksp, ksq)
-        fs = [np.asfortranarray(pq0[:, :, o]) for o in range(norb)]
-        order = np.argsort(rows[:, 0], kind="stable")
-        def work(deadline):
-            return len(O.rk_rows(g, fs, rows[order], deadline=deadline))
-        n, secs = cpu_sample(work, args.cpu_seconds)
-        if n > 0:
-            cpu = {"value": n / secs, "unit": "R^k integrals/s", "cores": 1, "kind": "port",
-                   "sample": f"first {n} rows in multipole order (arrays built inside the timed region), "
-                             f"{secs:.1f} s single-thread; the GPU figure is forward+adjoint"}
+        eng.set_orbitals(0, pq0)
+        eng.forward()
+        vals = eng.values()
+        cpu, extra = cpu_lines(t, ns, ks, ksp, ksq, pq0, rows, args.cpu_seconds, vals,
+                               "; the GPU figure is forward+adjoint")
+        if cpu:
+            extra["batched_vs_cpu"] = value / cpu["value"]
+        if "cpu_all_cores" in extra:
+            extra["batched_vs_cpu_all_cores"] = value / extra["cpu_all_cores"]["value"]
     workload = {"workload": "SCF-style: ns=600, ks=9/8/9, 60 orbitals, k=0..12, 1e6 rows; 20 iterations of "
                             "new orbitals + one forward+adjoint hipGraph replay (default tuning)",
                 "rows": nrows, "ns": ns, "norb": norb, "kmax": kmax, "iterations": 20, "tuning": tuning}
-    cpu, port = with_reference(cpu, t, ns, ks, ksp, ksq, pq0, rows, args.cpu_seconds / 2,
-                               "; the GPU figure is forward+adjoint")
     return config_line(5, value, "R^k integrals/s", ms, 20, args.warmup, workload,
-                       {"roofline": roof, "cpu_baseline": cpu, "cpu_port": port})
+                       {"roofline": roof, "step_work": work, "cpu_baseline": cpu, **extra})
 
 
 def main():
@@ -725,33 +760,14 @@ def main():
         eng.forward_adjoint()
         eng.sync()
 
-    cpu = None
-    if not args.no_cpu:
-        # the port: the oracle's restatement, convolutions shared by rows of
-        # one left density (stronger than the reference's own rk)
-        n, secs, sel, cvals = cpu_baseline(t, ns, ks, ksp, ksq, pq, rows, args.cpu_seconds)
-        rel = np.abs(vals[sel] - cvals) / np.maximum(np.abs(cvals), 1e-3 * np.max(np.abs(cvals)))
-        kk = sorted(set(int(x) for x in rows[sel, 0]))
-        cpu = {"value": n / secs, "unit": "R^k integrals/s", "cores": 1, "kind": "port",
-               "sample": f"first {n} of the rows in multipole order (k={kk[0]}..{kk[-1]}; "
-                         f"R^k arrays of each k built inside the timed region; convolutions "
-                         f"shared by the rows of one left density), {secs:.1f} s single-thread"}
-        out_extra["max_rel_err_vs_cpu_oracle"] = float(rel.max())
-        out_extra["cpu_port_integrals_per_s"] = cpu["value"]
-        out_extra["batched_vs_cpu_port"] = value / cpu["value"]
-        # the reference's own rk() from oracle/_ref, when built: the baseline
-        ref = cpu_baseline_ref(t, ns, ks, ksp, ksq, pq, rows, args.cpu_seconds / 2)
-        if ref is not None:
-            n, secs, sel, rvals = ref
-            rel = np.abs(vals[sel] - rvals) / np.maximum(np.abs(rvals), 1e-3 * np.max(np.abs(rvals)))
-            kk = sorted(set(int(x) for x in rows[sel, 0]))
-            cpu = {"value": n / secs, "unit": "R^k integrals/s", "cores": 1, "kind": "reference",
-                   "sample": f"the reference's rk() (DBSR_HF/dbsr_lib_dbs.f90, compiled into oracle/_ref) "
-                             f"on the first {n} of the rows in multipole order (k={kk[0]}..{kk[-1]}; "
-                             f"arrays of each k built by mrk_* inside the timed region), "
-                             f"{secs:.1f} s single-thread"}
-            out_extra["max_rel_err_vs_cpu_reference"] = float(rel.max())
+    cpu, cpu_extra = (None, {}) if args.no_cpu else cpu_lines(t, ns, ks, ksp, ksq, pq, rows, args.cpu_seconds,
+                                                               vals)
+    out_extra.update(cpu_extra)
+    if cpu:
         out_extra["batched_vs_cpu"] = value / cpu["value"]
+    if "cpu_all_cores" in out_extra:
+        out_extra["batched_vs_cpu_all_cores"] = value / out_extra["cpu_all_cores"]["value"]
+    out_extra["step_work"] = step_work(eng, prof, ms)
 
     line = {
         "metric": "R^k integrals/s + Fock-gradient time, batched vs plain FP64 GPU, <=1e-12 rel err",

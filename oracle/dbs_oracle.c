@@ -18,6 +18,7 @@
 #include "dbs_oracle.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -1938,4 +1939,85 @@ void orc_hf_matrix(const orc_grid *g, int i, int norb, const double *pq, const d
   for (size_t q = 0; q < (size_t)ms * ms; ++q) hfm[q] = hfm[q] / qsum;
   free(d);
   free(rkb);
+}
+
+/* ===================================================================== */
+/* R^k of a row set the way Rk_evaluate organises the work               */
+/* (DBSR_CI3/rk_evaluate.f90:31-56) for rk()'s type sum                  */
+/* (DBSR_HF/dbsr_lib_dbs.f90:3981-4022): rows sorted by k and left pair;  */
+/* per multipole the four arrays are built (mrk_*), then per (left pair,  */
+/* type) one density + case-2 convolution and, per row, one density +     */
+/* SUM_AmB.  OpenMP over the four array builds and over the (left pair,   */
+/* type) tasks on nthreads threads; the four type values of a row are     */
+/* added in rk()'s order, so results do not depend on the thread count.   */
+/* Work stops at the first multipole boundary or task after `budget`      */
+/* seconds (budget <= 0: no limit); returns the number of leading rows     */
+/* whose values are complete.  rows (n,5) row-major (k,a,b,c,d), 1-based;  */
+/* pq(ns,2,norb).  CPU baseline only.                                     */
+/* ===================================================================== */
+int orc_rk_rows_par(const orc_grid *g, int n, const int *rows, const double *pq, int nthreads, double budget,
+                    double *out) {
+  const int ns = g->ns, ks = g->ks;
+  static const int types[4] = {ORC_PPPP, ORC_QQQQ, ORC_QPQP, ORC_PQPQ};
+  static const int compl_[4] = {0, 1, 1, 0}, compr_[4] = {0, 1, 0, 1};
+  const double t0 = omp_get_wtime();
+  double *part = (double *)calloc((size_t)4 * (n > 0 ? n : 1), sizeof(double));
+  int done = 0, lo = 0;
+  while (lo < n) {
+    const int k = rows[5 * lo];
+    int hi = lo;
+    while (hi < n && rows[5 * hi] == k) ++hi;
+    /* left-pair groups of this multipole */
+    int ng = 0;
+    int *gs = (int *)malloc(sizeof(int) * (size_t)(hi - lo + 1));
+    for (int i = lo; i < hi; ++i)
+      if (i == lo || rows[5 * i + 1] != rows[5 * (i - 1) + 1] || rows[5 * i + 3] != rows[5 * (i - 1) + 3])
+        gs[ng++] = i;
+    gs[ng] = hi;
+    double *rkb[4];
+#pragma omp parallel for num_threads(nthreads < 4 ? nthreads : 4)
+    for (int t = 0; t < 4; ++t) {
+      rkb[t] = (double *)malloc(sizeof(double) * orc_rkb_size(g, types[t]));
+      orc_mrk(g, types[t], k, rkb[t]);
+    }
+    char *ok = (char *)calloc((size_t)ng * 4, 1);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+    for (int task = 0; task < 4 * ng; ++task) {
+      if (budget > 0 && omp_get_wtime() - t0 > budget) continue;
+      const int gi = task / 4, t = task % 4;
+      double *d1 = (double *)malloc(sizeof(double) * (size_t)ns * ks);
+      double *d2 = (double *)malloc(sizeof(double) * (size_t)ns * ks);
+      double *cv = (double *)malloc(sizeof(double) * (size_t)ns * ks);
+      const int a = rows[5 * gs[gi] + 1], c = rows[5 * gs[gi] + 3];
+      orc_density(ns, ks, d1, pq + (size_t)ns * (compl_[t] + 2 * (size_t)(a - 1)),
+                  pq + (size_t)ns * (compl_[t] + 2 * (size_t)(c - 1)), 's');
+      orc_convol(ns, ks, ks, rkb[t], cv, d1, 2, 's', 's');
+      for (int i = gs[gi]; i < gs[gi + 1]; ++i) {
+        const int b = rows[5 * i + 2], d = rows[5 * i + 4];
+        orc_density(ns, ks, d2, pq + (size_t)ns * (compr_[t] + 2 * (size_t)(b - 1)),
+                    pq + (size_t)ns * (compr_[t] + 2 * (size_t)(d - 1)), 's');
+        part[4 * (size_t)i + t] = orc_sum_amb(ns, ks, cv, d2, 's');
+      }
+      ok[task] = 1;
+      free(d1); free(d2); free(cv);
+    }
+    for (int t = 0; t < 4; ++t) free(rkb[t]);
+    int gdone = 0;
+    while (gdone < ng && ok[4 * gdone] && ok[4 * gdone + 1] && ok[4 * gdone + 2] && ok[4 * gdone + 3]) ++gdone;
+    done = gs[gdone];
+    const int all = gdone == ng;
+    free(ok); free(gs);
+    if (!all || (budget > 0 && omp_get_wtime() - t0 > budget)) break;
+    lo = hi;
+  }
+  for (int i = 0; i < done; ++i) {
+    double r = 0.0;
+    r = r + part[4 * (size_t)i];
+    r = r + part[4 * (size_t)i + 1];
+    r = r + part[4 * (size_t)i + 2];
+    r = r + part[4 * (size_t)i + 3];
+    out[i] = r;
+  }
+  free(part);
+  return done;
 }

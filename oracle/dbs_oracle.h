@@ -139,6 +139,8 @@ int  orc_rs_data(int ns, int ks, const double *rkb, int stype, int itype, int at
 double orc_grid_start(int nucl, double hi, double z, double a_fermi, double r_uniform);
 int  orc_mkgrid_01s(int ks, int ml, double hs, double he, double hmax, double rmax, double *t, int tcap);
 void orc_mkgrid_02s(int ns, int ks, double ti, double rmax, double *t);
+int  orc_rk_rows_par(const orc_grid *g, int n, const int *rows, const double *pq, int nthreads, double budget,
+                     double *out);
 int  orc_rdata_acf(int nc, const double *cdata, const int *K1, const int *K2, const int *K3, int ibo,
                    int kpol, int mk, const double *qpq, int norb, const int *icc, int nch, int nblocks,
                    double *acf);

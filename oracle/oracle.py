@@ -538,6 +538,7 @@ def ref():
         L.ref_mkgrid.restype = C.c_int
         L.ref_mkgrid.argtypes = [C.c_int, C.c_int, C.c_int] + [C.c_double] * 5 + [C.c_int, C.c_double,
                                                                                 C.c_double, C.c_int, _dp, C.c_int]
+        L.ref_rk_rows.argtypes = [C.c_int, _ip, C.c_int, _dp, _dp]
         L.ref_breit_energy.restype = C.c_double
         L.ref_breit_energy.argtypes = [C.c_double]
         L.ref_read_grid.restype = C.c_int
@@ -579,6 +580,58 @@ def ref():
 # --------------------------------------------------------------------------
 _SET_TYPES = {0: [(PPPP, 0, 0), (QQQQ, 1, 1), (QPQP, 1, 0), (PQPQ, 0, 1)],
               1: [(PPPP, 0, 0)], 2: [(QQQQ, 1, 1)], 3: [(PQPQ, 0, 1)], 4: [(QPQP, 1, 0)]}
+
+
+def sort_rows_by_left_pair(rows):
+    """rows with each pair put as (min, max) (R^k(a,b;c,d) = R^k(c,b;a,d) =
+    R^k(a,d;c,b): 's' densities are symmetric) and sorted by (k, left pair,
+    right pair), the order in which Rk_evaluate shares convolutions;
+    returns (sorted rows, permutation)"""
+    r = np.array(rows, dtype=np.int32).reshape(-1, 5)
+    a, c = np.minimum(r[:, 1], r[:, 3]), np.maximum(r[:, 1], r[:, 3])
+    b, d = np.minimum(r[:, 2], r[:, 4]), np.maximum(r[:, 2], r[:, 4])
+    can = np.stack([r[:, 0], a, b, c, d], axis=1).astype(np.int32)
+    perm = np.lexsort((can[:, 4], can[:, 2], can[:, 3], can[:, 1], can[:, 0]))
+    return np.ascontiguousarray(can[perm]), perm
+
+
+def rk_rows_par(g, pq, rows_sorted, nthreads, budget=0.0):
+    """rk() values of rows sorted by sort_rows_by_left_pair, Rk_evaluate's
+    organisation, OpenMP on nthreads threads; with a budget (seconds) only
+    the leading rows finished in time are returned"""
+    L = lib()
+    L.orc_rk_rows_par.restype = C.c_int
+    L.orc_rk_rows_par.argtypes = [C.c_void_p, C.c_int, _ip, _dp, C.c_int, C.c_double, _dp]
+    r = np.ascontiguousarray(rows_sorted, dtype=np.int32)
+    out = np.zeros(len(r))
+    n = L.orc_rk_rows_par(g.h, len(r), _i(r), _d(f64(pq)), nthreads, budget, _d(out))
+    return out[:n]
+
+
+def ref_rk_rows(pq, rows_sorted, budget=0.0):
+    """the same by the reference's own density/convol/SUM_AmB (oracle/_ref),
+    one thread, in chunks of whole left pairs until the budget (seconds) is
+    spent (0: all rows); ref_setup must have set the grid"""
+    import time
+    R = ref()
+    r = np.ascontiguousarray(rows_sorted, dtype=np.int32)
+    p = f64(pq)
+    if budget <= 0:
+        out = np.zeros(len(r))
+        R.ref_rk_rows(len(r), _i(r), p.shape[2], _d(p), _d(out))
+        return out
+    starts = np.flatnonzero(np.concatenate([[True], np.any(r[1:, [0, 1, 3]] != r[:-1, [0, 1, 3]], axis=1)]))
+    ends = np.concatenate([starts[1:], [len(r)]])
+    outs = []
+    t0 = time.perf_counter()
+    for lo, hi in zip(starts, ends):
+        sub = np.ascontiguousarray(r[lo:hi])
+        o = np.zeros(hi - lo)
+        R.ref_rk_rows(hi - lo, _i(sub), p.shape[2], _d(p), _d(o))
+        outs.append(o)
+        if time.perf_counter() - t0 > budget:
+            break
+    return np.concatenate(outs) if outs else np.zeros(0)
 
 
 def rk_rows(g, fs, rows, kind=0, deadline=None):

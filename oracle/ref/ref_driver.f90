@@ -365,3 +365,54 @@
       Call Breit_energy(eb)
       e = eb
       End Function ref_breit_energy
+
+      Subroutine ref_rk_rows(n, rows, norb, pq, out) bind(C, name='ref_rk_rows')
+! ... R^k of the rows (k,a,b,c,d) (5,n), sorted by k and left pair (a,c),
+! ... with the reference's own routines in Rk_evaluate's organisation
+! ... (DBSR_CI3/rk_evaluate.f90:31-56) for the rk() type sum
+! ... (dbsr_lib_dbs.f90:3981-4022): per type and multipole mrk_*, one
+! ... density + convol case 2 per left pair, density + SUM_AmB per row;
+! ... the four type values of a row added in rk()'s order
+      Use iso_c_binding
+      Use DBS_grid, only: ns, ks
+      Implicit none
+      Integer(c_int), value :: n, norb
+      Integer(c_int), intent(in) :: rows(5, n)
+      Real(c_double), intent(in) :: pq(ns, 2, norb)
+      Real(c_double) :: out(n)
+      Real(8), allocatable :: part(:,:), d1(:,:), d2(:,:), cv(:,:)
+      Real(8), external :: SUM_AmB
+      Integer :: it, i, k, kcur, la, lc, cl, cr
+      Integer, parameter :: compl(4) = (/1, 2, 2, 1/), compr(4) = (/1, 2, 1, 2/)
+      Allocate(part(4, n), d1(ns, ks), d2(ns, ks), cv(ns, ks))
+      Do it = 1, 4
+       kcur = -1; la = -1; lc = -1
+       cl = compl(it); cr = compr(it)
+       Do i = 1, n
+        k = rows(1, i)
+        if (k /= kcur) then
+         Select case(it)
+          Case(1); Call mrk_pppp(k)
+          Case(2); Call mrk_qqqq(k)
+          Case(3); Call mrk_qpqp(k)
+          Case(4); Call mrk_pqpq(k)
+         End select
+         kcur = k; la = -1
+        end if
+        if (rows(2, i) /= la .or. rows(4, i) /= lc) then
+         la = rows(2, i); lc = rows(4, i)
+         Call density(ns, ks, d1, pq(1, cl, la), pq(1, cl, lc), 's')
+         Call convol(ns, ks, cv, d1, 2, 's', 's')
+        end if
+        Call density(ns, ks, d2, pq(1, cr, rows(3, i)), pq(1, cr, rows(5, i)), 's')
+        part(it, i) = SUM_AmB(ns, ks, cv, d2, 's')
+       End do
+      End do
+      Do i = 1, n
+       out(i) = 0.d0 + part(1, i)
+       out(i) = out(i) + part(2, i)
+       out(i) = out(i) + part(3, i)
+       out(i) = out(i) + part(4, i)
+      End do
+      Deallocate(part, d1, d2, cv)
+      End Subroutine ref_rk_rows

@@ -248,3 +248,24 @@ def test_read_grid_matches_reference(name):
     p = read_knot(path)
     t = O.ref_read_grid(path, p["ns"], p["ks"])
     assert np.array_equal(p["points"], t)
+
+
+def test_cpu_baselines_match_reference_rk():
+    """the CPU baselines: the reference's own density/convol/SUM_AmB in
+    Rk_evaluate's organisation (ref_rk_rows) and the oracle's OpenMP port of
+    it (orc_rk_rows_par, any thread count) give every row's rk() bit for bit"""
+    from conftest import random_orbitals
+    t, ns, ks, ksp, ksq = fe_grid()
+    R, g = setup_both((t, ns, ks, ksp, ksq))
+    fs = random_orbitals(g, 5, 77)
+    pq = np.asfortranarray(np.stack(fs, axis=2))
+    rng = np.random.default_rng(78)
+    rows = np.zeros((150, 5), np.int32)
+    rows[:, 0] = rng.integers(0, 3, 150)
+    rows[:, 1:] = rng.integers(1, 6, (150, 4))
+    rs, _ = O.sort_rows_by_left_pair(rows)
+    ref = O.ref_rk_rows(pq, rs)
+    per_row = np.array([R.ref_rk(*[O._d(fs[x - 1]) for x in r[1:]], int(r[0])) for r in rs])
+    assert np.array_equal(ref, per_row)
+    for nt in (1, 3, 8):
+        assert np.array_equal(O.rk_rows_par(g, pq, rs, nt), ref), nt
