@@ -56,15 +56,22 @@ def build(verbose=False):
 
 
 FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
-FSRC = os.path.join(HERE, "fortran", "dbs_gpu.f90")
+FDIR = os.path.join(HERE, "fortran")
 FLIB = os.path.join(HERE, "libdbsr3_amd_fortran.so")
-DRIVER_SRC = os.path.join(ROOT, "tests", "fortran", "drv_dropin.f90")
-DRIVER = os.path.join(ROOT, "tests", "fortran", "drv_dropin")
+TDIR = os.path.join(ROOT, "tests", "fortran")
+# drop-in layers with the reference's names for one host program each
+# (DBSR_CI3 and DBSR_MAT3 define Add_integral / Add_matrix / Gen_matrix
+# differently): static archives linked into the host program
+ARCHIVES = {"ci3": "dbs_gpu_ci3.f90", "mat3": "dbs_gpu_mat3.f90"}
+# test host programs: source, archive it links (or None)
+DRIVERS = {"drv_dropin": None, "drv_ci3": "ci3", "drv_mat3": "mat3"}
 
 
 def build_fortran(verbose=False):
-    """the ISO_C_BINDING drop-in layer (libdbsr3_amd_fortran.so) and the
-    Fortran host program used by the drop-in test; skipped without flang"""
+    """the ISO_C_BINDING drop-in layers (libdbsr3_amd_fortran.so with the DBS
+    routines, hf_matrix and the engine; libdbsr3_amd_ci3.a and
+    libdbsr3_amd_mat3.a with the accumulation routines of each host program)
+    and the Fortran host programs of the drop-in tests; skipped without flang"""
     if not os.path.exists(FLANG):
         if verbose:
             print("flang not found: Fortran drop-in layer not built")
@@ -73,18 +80,26 @@ def build_fortran(verbose=False):
     os.makedirs(mod, exist_ok=True)
     obj = os.path.join(OBJDIR, "dbs_gpu.o")
     steps = [
-        [FLANG, "-O2", "-fPIC", "-module-dir", mod, "-c", FSRC, "-o", obj],
-        [FLANG, "-shared", "-fPIC", obj, "-L" + HERE, "-ldbsr3_amd",
-         "-Wl,-rpath,$ORIGIN", "-o", FLIB],
-        [FLANG, "-O2", DRIVER_SRC, "-I" + mod, "-L" + HERE, "-ldbsr3_amd_fortran", "-ldbsr3_amd",
-         "-Wl,-rpath,$ORIGIN/../../dbsr3_amd", "-o", DRIVER],
+        [FLANG, "-O2", "-fPIC", "-module-dir", mod, "-c", os.path.join(FDIR, "dbs_gpu.f90"), "-o", obj],
+        [FLANG, "-shared", "-fPIC", obj, "-L" + HERE, "-ldbsr3_amd", "-Wl,-rpath,$ORIGIN", "-o", FLIB],
     ]
+    for name, src in ARCHIVES.items():
+        o = os.path.join(OBJDIR, f"dbs_gpu_{name}.o")
+        a = os.path.join(HERE, f"libdbsr3_amd_{name}.a")
+        steps.append([FLANG, "-O2", "-fPIC", "-module-dir", mod, "-I" + mod, "-c", os.path.join(FDIR, src), "-o", o])
+        steps.append(["rm", "-f", a])
+        steps.append(["ar", "rcs", a, o])
+    for drv, arch in DRIVERS.items():
+        libs = ([os.path.join(HERE, f"libdbsr3_amd_{arch}.a")] if arch else [])
+        steps.append([FLANG, "-O2", os.path.join(TDIR, drv + ".f90"), "-I" + mod, *libs, "-L" + HERE,
+                      "-ldbsr3_amd_fortran", "-ldbsr3_amd", "-Wl,-rpath,$ORIGIN/../../dbsr3_amd",
+                      "-o", os.path.join(TDIR, drv)])
     for cmd in steps:
-        r = subprocess.run(cmd, capture_output=True, text=True)
+        r = subprocess.run(cmd, capture_output=True, text=True, cwd=OBJDIR)
         if r.returncode != 0:
             raise RuntimeError("flang failed: " + " ".join(cmd) + "\n" + r.stderr[-3000:])
     if verbose:
-        print("built", FLIB, "and", DRIVER)
+        print("built", FLIB, "the ci3/mat3 archives and the test drivers")
     return FLIB
 
 

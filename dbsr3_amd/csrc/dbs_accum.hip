@@ -226,25 +226,6 @@ double smu_factors(int ka, int kb, int kc, int kd, int L, int v, double* S) {
 // ===========================================================================
 namespace dbs {
 
-// HM/DM += S*c per target, entries of one target in list order
-__global__ void k_accumulate_targets(int ntarget, const long long* __restrict__ tpos,
-                                     const int* __restrict__ toff, const int* __restrict__ tent,
-                                     const double* __restrict__ S, const double* __restrict__ c,
-                                     double* __restrict__ HM, double* __restrict__ DM) {
-#pragma clang fp contract(off)
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < ntarget; t += gridDim.x * blockDim.x) {
-    const long long p = tpos[t];
-    double* dst = p >= 0 ? HM + p : DM + (-p - 1);
-    double acc = *dst;
-    for (int e = toff[t]; e < toff[t + 1]; ++e) {
-      const int q = tent[e];
-      const double ss = S[q] * c[q];
-      acc = acc + ss;
-    }
-    *dst = acc;
-  }
-}
-
 // xx_g = sum_e c_e * density(pq(jp1,j1), pq(jp2,j2)) in list order;
 // sym 0: band 's' (ns,ks); 1: full 'x' (ns,ns). tgb = 0: group-major
 // xx[g][r]; tgb > 0: chunks of tgb groups with the group index fastest,
@@ -499,14 +480,6 @@ __global__ void k_update_hs_mat(int ng, const int4* __restrict__ gi, int ns, int
 }  // namespace dbs
 
 using namespace dbs;
-
-template <class T>
-static int dev_upload(T** p, const std::vector<T>& v, hipStream_t s) {
-  const size_t n = v.empty() ? 1 : v.size();
-  DBS_HIP(hipMalloc(p, n * sizeof(T)));
-  if (!v.empty()) DBS_HIP(hipMemcpyAsync(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
-  return DBS_OK;
-}
 
 static unsigned grid_of(long long n, int block) {
   long long g = (n + block - 1) / block;
@@ -821,86 +794,6 @@ int dbs_decode_int(int n, const int* code, int* met, int* k, int* i1, int* i2, i
     i1[q] = ii / 32;
   }
   return DBS_OK;
-}
-
-int dbs_rk_evaluate_dev(dbs_ctx* ctx, int itype, int k, int nc, const double* c, const int* K1,
-                        const int* K2, const int* K3, const int* K4, double eps_c, int ibi,
-                        const double* pq_dev, int norb, double* HM_dev, int ldhm, int nzero,
-                        double* DM_dev) {
-  DBS_CHECK(ctx && pq_dev && HM_dev, DBS_ERR_ARG, "Rk_evaluate: null argument");
-  DBS_CHECK(itype >= 1 && itype <= 6, DBS_ERR_ARG, "Rk_evaluate: unknown type of intgrals");
-  if (nc <= 0) return DBS_OK;
-  // entries that survive |c| >= eps_c, as rows (k, a, b, c, d)
-  std::vector<int> rows, keep;
-  for (int j = 0; j < nc; ++j) {
-    if (std::fabs(c[j]) < eps_c) continue;
-    const int a = K1[j] / ibi, cc = K1[j] % ibi, b = K2[j] / ibi, d = K2[j] % ibi;
-    DBS_CHECK(a >= 1 && a <= norb && cc >= 1 && cc <= norb && b >= 1 && b <= norb && d >= 1 && d <= norb,
-              DBS_ERR_ARG, "Rk_evaluate: orbital index out of range");
-    rows.insert(rows.end(), {k, a, b, cc, d});
-    keep.push_back(j);
-  }
-  const int n = (int)keep.size();
-  if (n == 0) return DBS_OK;
-  // targets in first-touch order, entries in list order
-  std::map<long long, int> tindex;
-  std::vector<long long> tpos;
-  std::vector<std::vector<int>> tent;
-  for (int q = 0; q < n; ++q) {
-    const int j = keep[q];
-    const int ic = K3[j], jc = K4[j];
-    long long pos;
-    if (jc <= nzero) pos = (long long)(ic - 1) + (long long)ldhm * (jc - 1);
-    else if (ic == jc) {
-      DBS_CHECK(DM_dev, DBS_ERR_ARG, "Rk_evaluate: DM needed");
-      pos = -(long long)jc;  // DM(jc) -> -(jc-1)-1
-    } else continue;
-    auto it = tindex.find(pos);
-    int t;
-    if (it == tindex.end()) {
-      t = (int)tpos.size();
-      tindex[pos] = t;
-      tpos.push_back(pos);
-      tent.emplace_back();
-    } else {
-      t = it->second;
-    }
-    tent[t].push_back(q);
-  }
-  std::vector<int> toff(1, 0), tflat;
-  for (auto& v : tent) {
-    tflat.insert(tflat.end(), v.begin(), v.end());
-    toff.push_back((int)tflat.size());
-  }
-  std::vector<double> ck(n);
-  for (int q = 0; q < n; ++q) ck[q] = c[keep[q]];
-  // integral values by the engine
-  dbs_engine* eng = nullptr;
-  const int nr = n;
-  int rc = dbs_engine_create(&eng, 1, &ctx, &norb, &nr, rows.data(), itype);
-  if (rc) return rc;
-  double *epq = nullptr, *evals = nullptr;
-  dbs_engine_buffers(eng, 0, &epq, nullptr, &evals, nullptr, nullptr);
-  hipStream_t s = ctx->stream;
-  long long *d_tpos = nullptr;
-  int *d_toff = nullptr, *d_tent = nullptr;
-  double* d_c = nullptr;
-  do {
-    if (hipMemcpyAsync(epq, pq_dev, sizeof(double) * (size_t)ctx->ns * 2 * norb, hipMemcpyDeviceToDevice, s) != hipSuccess ||
-        hipStreamSynchronize(s) != hipSuccess) { rc = fail(DBS_ERR_HIP, "Rk_evaluate: copy"); break; }
-    if ((rc = dbs_engine_forward(eng))) break;
-    if ((rc = dbs_engine_sync(eng))) break;
-    if ((rc = dev_upload(&d_tpos, tpos, s)) || (rc = dev_upload(&d_toff, toff, s)) ||
-        (rc = dev_upload(&d_tent, tflat, s)) || (rc = dev_upload(&d_c, ck, s))) break;
-    const int nt = (int)tpos.size();
-    k_accumulate_targets<<<grid_of(nt, 256), 256, 0, s>>>(nt, d_tpos, d_toff, d_tent, evals, d_c, HM_dev, DM_dev);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    if (e != hipSuccess) rc = fail(DBS_ERR_HIP, std::string("Rk_evaluate: ") + hipGetErrorString(e));
-  } while (0);
-  (void)hipFree(d_tpos); (void)hipFree(d_toff); (void)hipFree(d_tent); (void)hipFree(d_c);
-  dbs_engine_destroy(eng);
-  return rc;
 }
 
 int dbs_rdata_34_dev(dbs_ctx* ctx, int itype, int atype, int k, int nc, const double* c,

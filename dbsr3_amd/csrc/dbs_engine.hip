@@ -3033,6 +3033,171 @@ static int plain_group(dbs_engine* e, int g, int type, int accumulate) {
   return rc;
 }
 
+// HM/DM += S*c per target, the entries of one target in list order
+__global__ void k_list_accumulate(int ntarget, const long long* __restrict__ tpos, const int* __restrict__ toff,
+                                  const int* __restrict__ tent, const double* __restrict__ S,
+                                  const double* __restrict__ c, double* __restrict__ HM, double* __restrict__ DM) {
+#pragma clang fp contract(off)
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < ntarget; t += gridDim.x * blockDim.x) {
+    const long long p = tpos[t];
+    double* dst = p >= 0 ? HM + p : DM + (-p - 1);
+    double acc = *dst;
+    for (int e = toff[t]; e < toff[t + 1]; ++e) {
+      const int q = tent[e];
+      const double ss = S[q] * c[q];
+      acc = acc + ss;
+    }
+    *dst = acc;
+  }
+}
+
+// Rk_evaluate / Sk_evaluate (DBSR_CI3/rk_evaluate.f90:2-63,
+// sk_evaluate.f90:2-63) in the reference's organisation: the list's
+// distinct densities (one batched launch), one case-2 convolution per
+// distinct left density K1 (one batched launch against the context's
+// cached array of (itype, k)), one SUM_AmB per entry, then S*c added to
+// HM(ic,jc) / DM(jc) per target in list order.  Every buffer lives in the
+// context's grow-only scratch and the host tables go up in one copy, so a
+// flush allocates nothing once the scratch has grown.
+int dbs_rk_evaluate_dev(dbs_ctx* ctx, int itype, int k, int nc, const double* c, const int* K1, const int* K2,
+                        const int* K3, const int* K4, double eps_c, int ibi, const double* pq_dev, int norb,
+                        double* HM_dev, int ldhm, int nzero, double* DM_dev) {
+  DBS_CHECK(ctx && pq_dev && HM_dev, DBS_ERR_ARG, "Rk_evaluate: null argument");
+  DBS_CHECK(itype >= 1 && itype <= 6, DBS_ERR_ARG, "Rk_evaluate: unknown type of intgrals");
+  DBS_CHECK(nc <= 0 || (c && K1 && K2 && K3 && K4), DBS_ERR_ARG, "Rk_evaluate: null list");
+  DBS_CHECK(ibi > 0, DBS_ERR_ARG, "Rk_evaluate: ibi < 1");
+  if (nc <= 0) return DBS_OK;
+  const int ns = ctx->ns, ks = ctx->ks;
+  const int sym = itype >= DBS_PPQQ ? 1 : 0;
+  const int nb = sym ? 2 * ks - 1 : ks;
+  int lc1, lc2, rc1, rc2;  // components of the left (K1) and right (K2) densities
+  switch (itype) {
+    case DBS_PPPP: lc1 = lc2 = 0; rc1 = rc2 = 0; break;
+    case DBS_QQQQ: lc1 = lc2 = 1; rc1 = rc2 = 1; break;
+    case DBS_PQPQ: lc1 = lc2 = 0; rc1 = rc2 = 1; break;
+    case DBS_QPQP: lc1 = lc2 = 1; rc1 = rc2 = 0; break;
+    case DBS_PPQQ: lc1 = 0; lc2 = 1; rc1 = 0; rc2 = 1; break;
+    default: lc1 = 0; lc2 = 1; rc1 = 1; rc2 = 0; break;
+  }
+  std::map<std::tuple<int, int, int>, int> slot;
+  std::vector<int4> spec;
+  std::vector<int> lslots;
+  std::map<int, int> lindex;
+  std::vector<int2> lr;
+  std::vector<double> ck;
+  std::map<long long, int> tindex;
+  std::vector<long long> tpos;
+  std::vector<std::vector<int>> tent;
+  auto get_slot = [&](int a, int b, int cm) {
+    auto key = std::make_tuple(a, b, cm);
+    auto it = slot.find(key);
+    if (it != slot.end()) return it->second;
+    const int id = (int)spec.size();
+    spec.push_back(make_int4(a - 1, b - 1, cm, id));
+    slot[key] = id;
+    return id;
+  };
+  for (int j = 0; j < nc; ++j) {
+    if (std::fabs(c[j]) < eps_c) continue;
+    const int a = K1[j] / ibi, cc = K1[j] % ibi, b = K2[j] / ibi, d = K2[j] % ibi;
+    DBS_CHECK(a >= 1 && a <= norb && cc >= 1 && cc <= norb && b >= 1 && b <= norb && d >= 1 && d <= norb,
+              DBS_ERR_ARG, "Rk_evaluate: orbital index out of range");
+    const int ic = K3[j], jc = K4[j];
+    long long pos;
+    if (jc <= nzero) {
+      DBS_CHECK(ic >= 1 && jc >= 1 && ic <= ldhm, DBS_ERR_ARG, "Rk_evaluate: HM index out of range");
+      pos = (long long)(ic - 1) + (long long)ldhm * (jc - 1);
+    } else if (ic == jc) {
+      DBS_CHECK(DM_dev, DBS_ERR_ARG, "Rk_evaluate: DM needed");
+      pos = -(long long)jc;  // DM(jc) at -(jc-1)-1
+    } else {
+      continue;
+    }
+    const int ls = get_slot(a, cc, lc1 | (lc2 << 1));
+    const int rs = get_slot(b, d, rc1 | (rc2 << 1));
+    auto li = lindex.find(ls);
+    int l;
+    if (li == lindex.end()) {
+      l = (int)lslots.size();
+      lslots.push_back(ls);
+      lindex[ls] = l;
+    } else {
+      l = li->second;
+    }
+    const int q = (int)lr.size();
+    lr.push_back(make_int2(l, rs));
+    ck.push_back(c[j]);
+    auto ti = tindex.find(pos);
+    if (ti == tindex.end()) {
+      tindex[pos] = (int)tpos.size();
+      tpos.push_back(pos);
+      tent.emplace_back(1, q);
+    } else {
+      tent[ti->second].push_back(q);
+    }
+  }
+  const int n = (int)lr.size();
+  if (n == 0) return DBS_OK;
+  std::vector<int> toff(1, 0), tflat;
+  for (auto& v : tent) {
+    tflat.insert(tflat.end(), v.begin(), v.end());
+    toff.push_back((int)tflat.size());
+  }
+  std::vector<long long> rowout(n);
+  for (int q = 0; q < n; ++q) rowout[q] = q;
+  double* rkb = nullptr;
+  int rc = build_rkb(ctx, itype, k, &rkb);
+  if (rc) return rc;
+  // scratch (doubles): densities | convolutions | values | then the host
+  // tables (coefficients, spec, left slots, row pairs, row outputs, targets)
+  const size_t per = (size_t)ns * nb;
+  const size_t o_conv = spec.size() * per, o_vals = o_conv + lslots.size() * per, o_tab = o_vals + n;
+  std::vector<char> blob;
+  auto place = [&](const void* src, size_t bytes) {
+    const size_t off = (blob.size() + 15) & ~(size_t)15;
+    blob.resize(off + bytes + 16);
+    if (bytes) std::memcpy(blob.data() + off, src, bytes);
+    return off;
+  };
+  const size_t b_c = place(ck.data(), ck.size() * sizeof(double));
+  const size_t b_spec = place(spec.data(), spec.size() * sizeof(int4));
+  const size_t b_ls = place(lslots.data(), lslots.size() * sizeof(int));
+  const size_t b_lr = place(lr.data(), lr.size() * sizeof(int2));
+  const size_t b_out = place(rowout.data(), rowout.size() * sizeof(long long));
+  const size_t b_tpos = place(tpos.data(), tpos.size() * sizeof(long long));
+  const size_t b_toff = place(toff.data(), toff.size() * sizeof(int));
+  const size_t b_tent = place(tflat.data(), tflat.size() * sizeof(int));
+  if ((rc = ensure_scratch(ctx, o_tab + (blob.size() + 7) / 8 + 2))) return rc;
+  char* base = reinterpret_cast<char*>(ctx->scratch + o_tab);
+  base = reinterpret_cast<char*>(((uintptr_t)base + 15) & ~(uintptr_t)15);
+  hipStream_t s = ctx->stream;
+  double* dens = ctx->scratch;
+  double* conv = ctx->scratch + o_conv;
+  double* vals = ctx->scratch + o_vals;
+  hipError_t e = hipMemcpyAsync(base, blob.data(), blob.size(), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) {
+    const int nsp = (int)spec.size(), nl = (int)lslots.size(), nt = (int)tpos.size();
+    k_plain_dens<<<grid_for((long long)nsp * per, 256, 8192), 256, 0, s>>>(
+        nsp, reinterpret_cast<const int4*>(base + b_spec), ns, ks, sym, pq_dev, 0, dens);
+    k_plain_conv2<<<grid_for((long long)nl * per, 4, 65535), 256, 0, s>>>(
+        nl, reinterpret_cast<const int*>(base + b_ls), ns, ks, sym, rkb, dens, conv);
+    k_plain_rows<<<grid_for(n, 4, 65535), 256, 0, s>>>(n, reinterpret_cast<const int2*>(base + b_lr), ns, ks,
+                                                         sym, conv, dens,
+                                                         reinterpret_cast<const long long*>(base + b_out), 0,
+                                                         vals);
+    k_list_accumulate<<<grid_for(nt, 256, 65535), 256, 0, s>>>(
+        nt, reinterpret_cast<const long long*>(base + b_tpos), reinterpret_cast<const int*>(base + b_toff),
+        reinterpret_cast<const int*>(base + b_tent), vals, reinterpret_cast<const double*>(base + b_c), HM_dev,
+        DM_dev);
+    e = hipGetLastError();
+  }
+  // the host tables must outlive their copy
+  const hipError_t e2 = hipStreamSynchronize(s);
+  if (e == hipSuccess) e = e2;
+  if (e != hipSuccess) return fail(DBS_ERR_HIP, std::string("Rk_evaluate: ") + hipGetErrorString(e));
+  return DBS_OK;
+}
+
 int dbs_engine_forward_plain(dbs_engine* e) {
   DBS_CHECK(e, DBS_ERR_ARG, "dbs_engine_forward_plain: null engine");
   DBS_HIP(hipStreamSynchronize(e->stream));

@@ -85,6 +85,163 @@
         Integer(c_int), value :: m
         Integer(c_int) :: dbs_quadr
        End Function
+       Function dbs_ctx_sync(ctx) bind(C, name='dbs_ctx_sync')
+        Import :: c_ptr, c_int
+        Type(c_ptr), value :: ctx
+        Integer(c_int) :: dbs_ctx_sync
+       End Function
+       Function dbs_malloc(ctx, p, bytes) bind(C, name='dbs_malloc')
+        Import :: c_ptr, c_int, c_size_t
+        Type(c_ptr), value :: ctx
+        Type(c_ptr) :: p
+        Integer(c_size_t), value :: bytes
+        Integer(c_int) :: dbs_malloc
+       End Function
+       Function dbs_free(ctx, p) bind(C, name='dbs_free')
+        Import :: c_ptr, c_int
+        Type(c_ptr), value :: ctx, p
+        Integer(c_int) :: dbs_free
+       End Function
+       Function dbs_memcpy_h2d(ctx, dst, src, bytes) bind(C, name='dbs_memcpy_h2d')
+        Import :: c_ptr, c_int, c_size_t
+        Type(c_ptr), value :: ctx, dst, src
+        Integer(c_size_t), value :: bytes
+        Integer(c_int) :: dbs_memcpy_h2d
+       End Function
+       Function dbs_memcpy_d2h(ctx, dst, src, bytes) bind(C, name='dbs_memcpy_d2h')
+        Import :: c_ptr, c_int, c_size_t
+        Type(c_ptr), value :: ctx, dst, src
+        Integer(c_size_t), value :: bytes
+        Integer(c_int) :: dbs_memcpy_d2h
+       End Function
+! ... coefficient lists (module c_data) and Add_integral
+       Function dbs_cdata_create(cd, ntype, kpol1, kpol2, mblock, nblock, kblock, eps_c, gen, user) &
+                bind(C, name='dbs_cdata_create')
+        Import :: c_ptr, c_int, c_double, c_funptr
+        Type(c_ptr) :: cd
+        Integer(c_int), value :: ntype, kpol1, kpol2, mblock, nblock, kblock
+        Real(c_double), value :: eps_c
+        Type(c_funptr), value :: gen
+        Type(c_ptr), value :: user
+        Integer(c_int) :: dbs_cdata_create
+       End Function
+       Function dbs_cdata_destroy(cd) bind(C, name='dbs_cdata_destroy')
+        Import :: c_ptr, c_int
+        Type(c_ptr), value :: cd
+        Integer(c_int) :: dbs_cdata_destroy
+       End Function
+       Function dbs_cdata_add_coef(cd, c, kpol, j1, j2, j3, j4, itype) bind(C, name='dbs_cdata_add_coef')
+        Import :: c_ptr, c_int, c_double
+        Type(c_ptr), value :: cd
+        Real(c_double), value :: c
+        Integer(c_int), value :: kpol, j1, j2, j3, j4, itype
+        Integer(c_int) :: dbs_cdata_add_coef
+       End Function
+       Function dbs_cdata_add_matrix(cd, itype, kpol) bind(C, name='dbs_cdata_add_matrix')
+        Import :: c_ptr, c_int
+        Type(c_ptr), value :: cd
+        Integer(c_int), value :: itype, kpol
+        Integer(c_int) :: dbs_cdata_add_matrix
+       End Function
+       Function dbs_add_integral_ci(cd, kpol, ic, jc, i1, i2, i3, i4, c, kappa, ibi, mbreit) &
+                bind(C, name='dbs_add_integral_ci')
+        Import :: c_ptr, c_int, c_double
+        Type(c_ptr), value :: cd
+        Integer(c_int), value :: kpol, ic, jc, i1, i2, i3, i4, ibi, mbreit
+        Real(c_double), value :: c
+        Integer(c_int) :: kappa(*)
+        Integer(c_int) :: dbs_add_integral_ci
+       End Function
+       Function dbs_add_integral_mat3(cd, orbs, icase, kpol, i1, i2, i3, i4, c, ic, jc, io, jo) &
+                bind(C, name='dbs_add_integral_mat3')
+        Import :: c_ptr, c_int, c_double
+        Type(c_ptr), value :: cd, orbs
+        Integer(c_int), value :: icase, kpol, i1, i2, i3, i4, ic, jc, io, jo
+        Real(c_double), value :: c
+        Integer(c_int) :: dbs_add_integral_mat3
+       End Function
+! ... list evaluation and accumulation
+       Function dbs_rk_evaluate_dev(ctx, itype, k, nc, c, K1, K2, K3, K4, eps_c, ibi, pq_dev, norb, &
+                HM_dev, ldhm, nzero, DM_dev) bind(C, name='dbs_rk_evaluate_dev')
+        Import :: c_ptr, c_int, c_double
+        Type(c_ptr), value :: ctx, c, K1, K2, K3, K4, pq_dev, HM_dev, DM_dev
+        Integer(c_int), value :: itype, k, nc, ibi, norb, ldhm, nzero
+        Real(c_double), value :: eps_c
+        Integer(c_int) :: dbs_rk_evaluate_dev
+       End Function
+       Function dbs_rdata_dev(ctx, m, aux, itype, atype, k, nc, c, K1, K2, K3, K4, pq_dev, norb) &
+                bind(C, name='dbs_rdata_dev')
+        Import :: c_ptr, c_int
+        Type(c_ptr), value :: ctx, m, aux, c, K1, K2, K3, K4, pq_dev
+        Integer(c_int), value :: itype, atype, k, nc, norb
+        Integer(c_int) :: dbs_rdata_dev
+       End Function
+       Function dbs_sdata_dev(ctx, m, aux, ktype, k, nc, c, K1, K2, K3, K4, pq_dev, norb) &
+                bind(C, name='dbs_sdata_dev')
+        Import :: c_ptr, c_int
+        Type(c_ptr), value :: ctx, m, aux, c, K1, K2, K3, K4, pq_dev
+        Integer(c_int), value :: ktype, k, nc, norb
+        Integer(c_int) :: dbs_sdata_dev
+       End Function
+       Function dbs_o_data_dev(ctx, m, itype, n, k1, k2, c, eps_c, ibo, ipbs, kbort, kbort_user, vch_dev, &
+                nvch, fppqq_dev, iitar, iptar, otarg) bind(C, name='dbs_o_data_dev')
+        Import :: c_ptr, c_int, c_double, c_funptr
+        Type(c_ptr), value :: ctx, m, k1, k2, c, ipbs, kbort_user, vch_dev, fppqq_dev, iptar, otarg
+        Type(c_funptr), value :: kbort
+        Integer(c_int), value :: itype, n, ibo, nvch, iitar
+        Real(c_double), value :: eps_c
+        Integer(c_int) :: dbs_o_data_dev
+       End Function
+       Function dbs_hf_matrix_dev(ctx, i, norb, pq_dev, acoef, bcoef, kmax, qsum, mbreit, nrk, kr1, kr2, &
+                kr3, kr4, crk, hfm_dev) bind(C, name='dbs_hf_matrix_dev')
+        Import :: c_ptr, c_int, c_double
+        Type(c_ptr), value :: ctx, pq_dev, acoef, bcoef, kr1, kr2, kr3, kr4, crk, hfm_dev
+        Integer(c_int), value :: i, norb, kmax, mbreit, nrk
+        Real(c_double), value :: qsum
+        Integer(c_int) :: dbs_hf_matrix_dev
+       End Function
+! ... the batched engine
+       Function dbs_engine_create(eng, ninst, ctxs, norb, nrows, rows, kind) bind(C, name='dbs_engine_create')
+        Import :: c_ptr, c_int
+        Type(c_ptr) :: eng
+        Integer(c_int), value :: ninst, kind
+        Type(c_ptr) :: ctxs(*)
+        Integer(c_int) :: norb(*), nrows(*), rows(*)
+        Integer(c_int) :: dbs_engine_create
+       End Function
+       Function dbs_engine_destroy(eng) bind(C, name='dbs_engine_destroy')
+        Import :: c_ptr, c_int
+        Type(c_ptr), value :: eng
+        Integer(c_int) :: dbs_engine_destroy
+       End Function
+       Function dbs_engine_buffers(eng, inst, pq, coef, vals, grad, energy) bind(C, name='dbs_engine_buffers')
+        Import :: c_ptr, c_int
+        Type(c_ptr), value :: eng
+        Integer(c_int), value :: inst
+        Type(c_ptr) :: pq, coef, vals, grad, energy
+        Integer(c_int) :: dbs_engine_buffers
+       End Function
+       Function dbs_engine_forward_adjoint(eng) bind(C, name='dbs_engine_forward_adjoint')
+        Import :: c_ptr, c_int
+        Type(c_ptr), value :: eng
+        Integer(c_int) :: dbs_engine_forward_adjoint
+       End Function
+       Function dbs_engine_graph_capture(eng, with_adjoint) bind(C, name='dbs_engine_graph_capture')
+        Import :: c_ptr, c_int
+        Type(c_ptr), value :: eng
+        Integer(c_int), value :: with_adjoint
+        Integer(c_int) :: dbs_engine_graph_capture
+       End Function
+       Function dbs_engine_graph_launch(eng) bind(C, name='dbs_engine_graph_launch')
+        Import :: c_ptr, c_int
+        Type(c_ptr), value :: eng
+        Integer(c_int) :: dbs_engine_graph_launch
+       End Function
+       Function dbs_engine_sync(eng) bind(C, name='dbs_engine_sync')
+        Import :: c_ptr, c_int
+        Type(c_ptr), value :: eng
+        Integer(c_int) :: dbs_engine_sync
+       End Function
       End Interface
       End Module dbs_gpu_api
 
@@ -274,3 +431,118 @@
       Call dbs_gpu_check(dbs_quadr(ctx, fi, fj, m, x))
       quadr = x
       End Function quadr
+
+!----------------------------------------------------------------------
+!     device buffers for the routines below (context stream)
+!----------------------------------------------------------------------
+      Module dbs_gpu_mem
+      Use iso_c_binding
+      Implicit none
+      Contains
+
+      Function dbs_gpu_put(a, n) result(p)
+      Use dbs_gpu_api
+      Use dbs_gpu_state, only: ctx
+      Integer, intent(in) :: n
+      Real(8), intent(in), target :: a(n)
+      Type(c_ptr) :: p
+      Call dbs_gpu_check(dbs_malloc(ctx, p, int(8 * max(n, 1), c_size_t)))
+      if (n > 0) Call dbs_gpu_check(dbs_memcpy_h2d(ctx, p, c_loc(a), int(8 * n, c_size_t)))
+      End Function dbs_gpu_put
+
+      Subroutine dbs_gpu_get(p, a, n)
+      Use dbs_gpu_api
+      Use dbs_gpu_state, only: ctx
+      Type(c_ptr), intent(in) :: p
+      Integer, intent(in) :: n
+      Real(8), intent(out), target :: a(n)
+      if (n > 0) Call dbs_gpu_check(dbs_memcpy_d2h(ctx, c_loc(a), p, int(8 * n, c_size_t)))
+      End Subroutine dbs_gpu_get
+
+      End Module dbs_gpu_mem
+
+!----------------------------------------------------------------------
+      Subroutine dbs_gpu_hf_matrix(i, norb, pq, acoef, bcoef, kmax, qsum, mbreit, nrk, kr1, kr2, kr3, kr4, &
+                                   crk, hfm)
+!----------------------------------------------------------------------
+!     the two-electron part of hf_matrix(i,hfm) (DBSR_HF/hf_matrix.f90:2-78,
+!     Breit block :97-163 when mbreit = 2) on the GPU, with the module data
+!     the reference reads passed explicitly: orbitals pq(ns,2,norb), the
+!     coefficients a(i,j,k), b(i,j,k) as acoef/bcoef(norb,0:kmax), qsum(i)
+!     and the Breit list of rk4_data.  hfm(ms,ms) holds on entry what the
+!     host put there (qsum(i)*dhl in the reference) and on return the
+!     potential, divided by qsum, as hf_matrix leaves it.
+      Use dbs_gpu_api
+      Use dbs_gpu_state, only: ctx, ns_g
+      Use dbs_gpu_mem
+      Implicit none
+      Integer, intent(in) :: i, norb, kmax, mbreit, nrk
+      Real(8), intent(in), target :: pq(ns_g, 2, norb), acoef(norb, 0:kmax), bcoef(norb, 0:kmax), qsum
+      Integer, intent(in), target :: kr1(*), kr2(*), kr3(*), kr4(*)
+      Real(8), intent(in), target :: crk(*)
+      Real(8), intent(inout), target :: hfm(2 * ns_g, 2 * ns_g)
+      Type(c_ptr) :: pqd, hd
+      pqd = dbs_gpu_put(pq, ns_g * 2 * norb)
+      hd = dbs_gpu_put(hfm, 4 * ns_g * ns_g)
+      Call dbs_gpu_check(dbs_hf_matrix_dev(ctx, i, norb, pqd, c_loc(acoef), c_loc(bcoef), kmax, qsum, mbreit, &
+                         nrk, c_loc(kr1), c_loc(kr2), c_loc(kr3), c_loc(kr4), c_loc(crk), hd))
+      Call dbs_gpu_get(hd, hfm, 4 * ns_g * ns_g)
+      Call dbs_gpu_check(dbs_free(ctx, pqd))
+      Call dbs_gpu_check(dbs_free(ctx, hd))
+      End Subroutine dbs_gpu_hf_matrix
+
+!----------------------------------------------------------------------
+!     the batched engine for an SCF loop: E = sum_n coef_n R^k(rows_n)
+!     and dE/d(pq), the direct+exchange potentials applied to the
+!     orbitals (what hf_matrix builds), replayed as one captured hipGraph
+!     per call.  rows(5,nrows) = (k,a,b,c,d), 1-based orbitals.
+!----------------------------------------------------------------------
+      Module dbs_gpu_engine_state
+      Use iso_c_binding
+      Implicit none
+      Type(c_ptr) :: eng = c_null_ptr, pq_d = c_null_ptr, coef_d = c_null_ptr, grad_d = c_null_ptr, &
+                     e_d = c_null_ptr
+      Integer :: norb_e = 0, nrows_e = 0
+      End Module dbs_gpu_engine_state
+
+      Subroutine dbs_gpu_engine_setup(norb, nrows, rows)
+      Use dbs_gpu_api
+      Use dbs_gpu_state, only: ctx
+      Use dbs_gpu_engine_state
+      Implicit none
+      Integer, intent(in) :: norb, nrows
+      Integer(c_int), intent(in) :: rows(5, nrows)
+      Type(c_ptr) :: ctxs(1), vals
+      Integer(c_int) :: no(1), nr(1)
+      if (c_associated(eng)) Call dbs_gpu_check(dbs_engine_destroy(eng))
+      ctxs(1) = ctx; no(1) = norb; nr(1) = nrows
+      Call dbs_gpu_check(dbs_engine_create(eng, 1, ctxs, no, nr, rows, 0))
+      Call dbs_gpu_check(dbs_engine_buffers(eng, 0, pq_d, coef_d, vals, grad_d, e_d))
+      Call dbs_gpu_check(dbs_engine_graph_capture(eng, 1))
+      norb_e = norb; nrows_e = nrows
+      End Subroutine dbs_gpu_engine_setup
+
+      Subroutine dbs_gpu_engine_step(pq, coef, energy, grad)
+      Use dbs_gpu_api
+      Use dbs_gpu_state, only: ctx, ns_g
+      Use dbs_gpu_engine_state
+      Implicit none
+      Real(8), intent(in), target :: pq(ns_g, 2, norb_e), coef(nrows_e)
+      Real(8), intent(out), target :: energy, grad(ns_g, 2, norb_e)
+      if (.not. c_associated(eng)) Stop 'dbs_gpu_engine_step: call dbs_gpu_engine_setup first'
+      Call dbs_gpu_check(dbs_memcpy_h2d(ctx, pq_d, c_loc(pq), int(8 * ns_g * 2 * norb_e, c_size_t)))
+      Call dbs_gpu_check(dbs_memcpy_h2d(ctx, coef_d, c_loc(coef), int(8 * nrows_e, c_size_t)))
+      Call dbs_gpu_check(dbs_ctx_sync(ctx))
+      Call dbs_gpu_check(dbs_engine_graph_launch(eng))
+      Call dbs_gpu_check(dbs_engine_sync(eng))
+      Call dbs_gpu_check(dbs_memcpy_d2h(ctx, c_loc(energy), e_d, int(8, c_size_t)))
+      Call dbs_gpu_check(dbs_memcpy_d2h(ctx, c_loc(grad), grad_d, int(8 * ns_g * 2 * norb_e, c_size_t)))
+      End Subroutine dbs_gpu_engine_step
+
+      Subroutine dbs_gpu_engine_close
+      Use dbs_gpu_api
+      Use dbs_gpu_engine_state
+      Implicit none
+      if (c_associated(eng)) Call dbs_gpu_check(dbs_engine_destroy(eng))
+      eng = c_null_ptr
+      End Subroutine dbs_gpu_engine_close

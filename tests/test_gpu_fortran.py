@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from conftest import ROOT, fe_grid, random_orbitals
+from conftest import ROOT, fe_grid, random_orbitals, small_grid
 
 pytestmark = pytest.mark.gpu
 DRIVER = os.path.join(ROOT, "tests", "fortran", "drv_dropin")
@@ -55,3 +55,157 @@ def test_fortran_dropin(tmp_path, gpu_available):
     v = O.convol(ns, ks, g.mrk(O.QQQQ, k), d1, 2, "s", "s")
     x = O.sum_amb(ns, ks, v, d2, "s")
     assert abs(out[9] - x) <= 1e-12 * abs(x), (out[9], x)
+
+
+def _write(f, *arrs):
+    for a in arrs:
+        a = np.asarray(a)
+        (np.asfortranarray(a).T if a.ndim > 1 else a).tofile(f)
+
+
+CI3 = os.path.join(ROOT, "tests", "fortran", "drv_ci3")
+MAT3 = os.path.join(ROOT, "tests", "fortran", "drv_mat3")
+
+
+@pytest.mark.skipif(not os.path.exists(CI3), reason="Fortran driver not built (no flang)")
+def test_fortran_ci3_setup_matrix(tmp_path, gpu_available):
+    """a DBSR_CI3 Setup_matrix-style host loop (Add_integral with Breit terms,
+    one-electron HM/DM terms added between them, small c_data blocks so
+    lists flush early, Add_matrix at the end) through the drop-in routines
+    of libdbsr3_amd_ci3.a; HM and DM against the oracle's replay of the same
+    loop (c_data, Add_integral, Rk_evaluate / Sk_evaluate)"""
+    t, ns, ks, ksp, ksq = fe_grid()
+    g = O.Grid(t, ns, ks, ksp, ksq)
+    norb, ncj, nzero, mbreit, mtype, mpol, mb, nb, kb = 6, 9, 7, 1, 6, 4, 64, 40, 6
+    fs = random_orbitals(g, norb, seed=61)
+    pq = np.stack(fs, axis=2)
+    rng = np.random.default_rng(62)
+    kef = np.array([-1, 1, -2, 2, -3, -1], dtype=np.int32)
+    nterm = 300
+    term = np.zeros((nterm, 7), dtype=np.int32)
+    for n in range(nterm):
+        i, j = sorted((int(x) for x in rng.integers(1, ncj + 1, 2)), reverse=True)
+        term[n] = [rng.integers(0, 4), i, j, *rng.integers(1, norb + 1, 4)]
+    C = rng.uniform(-1, 1, nterm)
+    nl = 40
+    lterm = np.stack([np.sort(rng.integers(1, nterm + 1, nl)), rng.integers(1, ncj + 1, nl),
+                      rng.integers(1, ncj + 1, nl)], axis=1).astype(np.int32)
+    lterm[:, 2] = np.minimum(lterm[:, 1], lterm[:, 2])
+    lval = rng.uniform(-1, 1, nl)
+    with open(tmp_path / "drv_ci3_in.bin", "wb") as f:
+        _write(f, np.array([ns, ks, ksp, ksq], np.int32), np.asarray(t, np.float64),
+               np.array([norb, ncj, nzero, mbreit, mtype, mpol, mb, nb, kb], np.int32), pq, kef,
+               np.array([nterm], np.int32), term, C, np.array([nl], np.int32), lterm, lval)
+    print("[fortran CI3 driver]", flush=True)
+    r = subprocess.run([CI3], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = np.fromfile(tmp_path / "drv_ci3_out.bin", dtype=np.float64)
+    HM = out[:ncj * nzero].reshape((ncj, nzero), order="F")
+    DM = out[ncj * nzero:]
+    # the oracle's replay of the same loop
+    HMr, DMr = np.zeros((ncj, nzero), order="F"), np.zeros(ncj)
+    flushes = []
+
+    def gen(it, k, lst):
+        if len(lst[0]):
+            O.rk_evaluate(g, it, k, lst, pq, 2 ** 15, HMr, nzero, DMr, eps_c=1e-10)
+            flushes.append(it)
+    cd = O.CData(mtype, 0, mpol, mb, nb, kb, 1e-12, gen)
+    kap = np.concatenate([[0], kef]).astype(np.int32)
+    li = 0
+    for n in range(nterm):
+        while li < nl and lterm[li, 0] == n + 1:
+            _, a, b = lterm[li]
+            if b <= nzero:
+                HMr[a - 1, b - 1] += lval[li]
+            elif a == b:
+                DMr[b - 1] += lval[li]
+            li += 1
+        k, i, j, j1, j2, j3, j4 = (int(x) for x in term[n])
+        assert O.add_integral_ci(cd, k, i, j, j1, j2, j3, j4, float(C[n]), kap, 2 ** 15, mbreit) == 0
+    early = len(flushes)
+    for it in range(1, mtype + 1):
+        for k in range(mpol + 1):
+            cd.add_matrix(it, k)
+    assert early > 0 and set(flushes) >= {1, 2, 3, 4, 5, 6}  # lists flushed early and at the end, all types
+    s = max(np.max(np.abs(HMr)), np.max(np.abs(DMr)))
+    assert np.max(np.abs(HM - HMr)) <= 1e-12 * s
+    assert np.max(np.abs(DM - DMr)) <= 1e-12 * s
+
+
+@pytest.mark.skipif(not os.path.exists(MAT3), reason="Fortran driver not built (no flang)")
+def test_fortran_mat3_state_res(tmp_path, gpu_available):
+    """a DBSR_MAT3 State_res-style host program (overlap, R and S terms of
+    every structure through Add_integral, then Add_matrix over all types)
+    with its own KBORT and QUADR_PQ, through libdbsr3_amd_mat3.a; every
+    matrix against the oracle's replay (Idef_*, c_data, O_data, R_data for
+    the four atypes with the asymptotic coefficients, S_data)"""
+    from test_mat3_routing import IBO, IPB, KAPPA, LBS, NORB, _terms
+    from test_oracle_mat3 import _mats
+    t, ns, ks, ksp, ksq = small_grid()
+    g = O.Grid(t, ns, ks, ksp, ksq)
+    ms, nch, npert, nvch, mk = 2 * ns, 3, 4, 5, 3
+    fs = random_orbitals(g, NORB, seed=71)
+    pq = np.stack(fs, axis=2)
+    rng = np.random.default_rng(72)
+    vch = np.asfortranarray(rng.standard_normal((ms, nvch)))
+    fppqq = O.overlaps(g)[2]
+    iptar = np.array([0, 1, 2, 2], dtype=np.int32)
+    kb = np.array([[(3 * i + j) % nvch + 1 for j in range(1, NORB + 1)] for i in range(1, NORB + 1)], np.int32)
+    m = _mats(nch, npert, ms)
+    terms = []
+    for (orbs, c, ic, jc, io, jo) in _terms(rng, 150):
+        # overlap terms need a configuration pair or a channel overlap (Idef_Otype)
+        ok0 = (ic > 0 and jc > 0) or io > 0
+        icase = int(rng.choice([0, 2, 3] if ok0 else [2, 3]))
+        terms.append([icase, int(rng.integers(0, 3)), *orbs, ic, jc, io, jo])
+    terms = np.array(terms, dtype=np.int32)
+    C = rng.uniform(-1, 1, len(terms))
+    ntarg = nch * (nch + 1) // 2
+    with open(tmp_path / "drv_mat3_in.bin", "wb") as f:
+        _write(f, np.array([ns, ks, ksp, ksq], np.int32), np.asarray(t, np.float64),
+               np.array([NORB, NORB, IBO, mk, nch, npert, nvch, 1, ntarg], np.int32), pq, LBS[1:], KAPPA[1:],
+               IPB[1:], kb, m["icc"], m["icb"], m["ibb"], iptar[1:], vch, fppqq,
+               np.array([len(terms)], np.int32), terms, C)
+    print("[fortran MAT3 driver]", flush=True)
+    r = subprocess.run([MAT3], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = np.fromfile(tmp_path / "drv_mat3_out.bin", dtype=np.float64)
+    nblk, ncb, nbb = int(m["icc"].max()), int(m["icb"].max()), int(m["ibb"].max())
+    sizes = [ms * ms * nblk, nblk * (mk + 1), ms * ncb, nbb, ntarg, ntarg]
+    parts = np.split(out, np.cumsum(sizes)[:-1])
+    got = dict(hch=parts[0].reshape((ms, ms, nblk), order="F"), acf=parts[1].reshape((nblk, mk + 1), order="F"),
+               hcp=parts[2].reshape((ms, ncb), order="F"), hp=parts[3], htarg=parts[4], otarg=parts[5])
+    # the oracle's replay
+    kbort = lambda i1, i2: int(kb[i1 - 1, i2 - 1])
+    qpq = np.array([[[g.quadr(fs[i], fs[j], k) for k in range(mk + 1)] for j in range(NORB)] for i in range(NORB)])
+    acf, htarg, otarg = np.zeros((nblk, mk + 1), order="F"), np.zeros(ntarg), np.zeros(ntarg)
+    for icase in (0, 2, 3):
+        def gen(it, k, lst):
+            if not len(lst[0]):
+                return
+            if icase == 0:
+                assert O.o_data(ms, it, lst[:3], 1e-10, IBO, IPB, kbort, vch, fppqq, 1, iptar, otarg, m) == 0
+            elif icase == 2:
+                for atype in range(4):
+                    assert O.rs_data(g, 0, it, atype, k, lst, pq, IBO, IPB, kbort, vch, fppqq, 1, iptar, htarg,
+                                     m) == 0
+                    if it == 3 and atype <= 1:
+                        assert O.rdata_acf(lst, IBO, k, mk, qpq[:, :, k], m["icc"], acf) == 0
+            else:
+                assert O.rs_data(g, 1, it % 10, it // 10, k, lst, pq, IBO, IPB, kbort, vch, fppqq, 1, iptar, htarg,
+                                 m) == 0
+        cd = O.CData(19, 0, mk + 1, 64, 400, 40, 1e-14, gen)
+        for row, c in zip(terms, C):
+            if row[0] == icase:
+                assert cd.add_integral_mat3(icase, *(int(x) for x in row[1:6]), float(c),
+                                            *(int(x) for x in row[6:10]), LBS, KAPPA, IPB, mk=mk, eps_c=1e-10,
+                                            check_target=1) == 0
+        for k in range(mk + 2):
+            for it in range(1, 20):
+                cd.add_matrix(it, k)
+    want = dict(hch=m["hch"], acf=acf, hcp=m["hcp"], hp=m["hp"], htarg=htarg, otarg=otarg)
+    for key, ref in want.items():
+        s = np.max(np.abs(ref))
+        assert s > 0, key
+        assert np.max(np.abs(got[key] - ref)) <= 1e-12 * s, key
