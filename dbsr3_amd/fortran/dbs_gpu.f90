@@ -272,7 +272,7 @@
        text(i:i) = msg(i)
       End do
       write(*, '(a)') trim(text)
-      Stop 'dbs_gpu: library error'
+      Error stop 'dbs_gpu: library error'
       End Subroutine dbs_gpu_check
 
 !----------------------------------------------------------------------

@@ -10,6 +10,9 @@
       Integer :: ns, ks, ksp, ksq, norb, k, n, i
       Real(8), allocatable :: t(:), pq(:,:,:), d1(:,:), d2(:,:), conv(:,:), out(:)
       Real(8), allocatable :: dn1(:,:), dn2(:,:), convn(:,:)
+      Real(8), allocatable :: acoef(:,:), bcoef(:,:), hfm(:,:), grad(:,:,:), crk(:)
+      Integer, allocatable :: rows(:,:), kr(:)
+      Real(8) :: energy, coef(3)
       Real(8), external :: rk, SUM_AmB, quadr
 
       Open(11, file='drv_in.bin', access='stream', form='unformatted', status='old')
@@ -76,8 +79,27 @@
       Call Convol(ns, ks, conv, d1, 2, 's', 's')
       out(10) = SUM_AmB(ns, ks, conv, d2, 's')
 
+! ... hf_matrix's two-electron potential of orbital 1: a(1,2,0) = 1,
+!     b(1,3,1) = 0.5, qsum = 2, no Breit terms
+      Allocate(acoef(norb, 0:2), bcoef(norb, 0:2), hfm(2 * ns, 2 * ns), kr(1), crk(1))
+      acoef = 0.d0; bcoef = 0.d0; acoef(2, 0) = 1.d0; bcoef(3, 1) = 0.5d0
+      hfm = 0.d0; kr = 0; crk = 0.d0
+      Call dbs_gpu_hf_matrix(1, norb, pq, acoef, bcoef, 2, 2.d0, 0, 0, kr, kr, kr, kr, crk, hfm)
+! ... the engine: E = sum c R^k and dE/d(pq) for three rows, twice (a
+!     captured graph replayed with new coefficients)
+      Allocate(rows(5, 3), grad(ns, 2, norb))
+      rows(:, 1) = (/0, 1, 2, 3, 4/); rows(:, 2) = (/1, 1, 1, 2, 2/); rows(:, 3) = (/2, 2, 3, 2, 3/)
+      Call dbs_gpu_engine_setup(norb, 3, rows)
+      coef = (/1.d0, -0.5d0, 0.25d0/)
+      Call dbs_gpu_engine_step(pq, 2.d0 * coef, energy, grad)
+      Call dbs_gpu_engine_step(pq, coef, energy, grad)
+      Call dbs_gpu_engine_close
+
       Open(12, file='drv_out.bin', access='stream', form='unformatted', status='replace')
       Write(12) out
+      Write(12) hfm
+      Write(12) energy
+      Write(12) grad
       Close(12)
       Call dbs_gpu_close
       End Program drv_dropin

@@ -30,6 +30,7 @@ def test_fortran_dropin(tmp_path, gpu_available):
     print("[fortran drop-in driver]", flush=True)
     r = subprocess.run([DRIVER], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
+    assert (tmp_path / "drv_out.bin").exists(), r.stdout + r.stderr
     out = np.fromfile(tmp_path / "drv_out.bin", dtype=np.float64)
     assert out[6] == ns and out[7] == k
     ref = g.rk(fs[0], fs[1], fs[2], fs[3], k)
@@ -55,6 +56,21 @@ def test_fortran_dropin(tmp_path, gpu_available):
     v = O.convol(ns, ks, g.mrk(O.QQQQ, k), d1, 2, "s", "s")
     x = O.sum_amb(ns, ks, v, d2, "s")
     assert abs(out[9] - x) <= 1e-12 * abs(x), (out[9], x)
+    # hf_matrix of orbital 1 with a(1,2,0) = 1, b(1,3,1) = 0.5, qsum = 2
+    ms = 2 * ns
+    hfm = out[10:10 + ms * ms].reshape((ms, ms), order="F")
+    A, B = np.zeros((4, 3), order="F"), np.zeros((4, 3), order="F")
+    A[1, 0], B[2, 1] = 1.0, 0.5
+    ref = O.hf_matrix(g, 1, np.asfortranarray(pq), A, B, 2, 2.0)
+    assert np.max(np.abs(hfm - ref)) <= 1e-12 * np.max(np.abs(ref))
+    # the engine: E = R^0(1,2;3,4) - 0.5 R^1(1,1;2,2) + 0.25 R^2(2,3;2,3)
+    energy = out[10 + ms * ms]
+    rows = np.array([(0, 1, 2, 3, 4), (1, 1, 1, 2, 2), (2, 2, 3, 2, 3)], dtype=np.int32)
+    vals = O.rk_rows(g, fs, rows)
+    e_ref = vals @ np.array([1.0, -0.5, 0.25])
+    assert abs(energy - e_ref) <= 1e-12 * np.sum(np.abs(vals))
+    grad = out[11 + ms * ms:].reshape((ns, 2, 4), order="F")
+    assert np.all(np.isfinite(grad)) and np.any(grad != 0)
 
 
 def _write(f, *arrs):
@@ -76,7 +92,7 @@ def test_fortran_ci3_setup_matrix(tmp_path, gpu_available):
     loop (c_data, Add_integral, Rk_evaluate / Sk_evaluate)"""
     t, ns, ks, ksp, ksq = fe_grid()
     g = O.Grid(t, ns, ks, ksp, ksq)
-    norb, ncj, nzero, mbreit, mtype, mpol, mb, nb, kb = 6, 9, 7, 1, 6, 4, 64, 40, 6
+    norb, ncj, nzero, mbreit, mtype, mpol, mb, nb, kb = 6, 9, 7, 1, 6, 4, 8, 70, 2
     fs = random_orbitals(g, norb, seed=61)
     pq = np.stack(fs, axis=2)
     rng = np.random.default_rng(62)
@@ -95,10 +111,11 @@ def test_fortran_ci3_setup_matrix(tmp_path, gpu_available):
     with open(tmp_path / "drv_ci3_in.bin", "wb") as f:
         _write(f, np.array([ns, ks, ksp, ksq], np.int32), np.asarray(t, np.float64),
                np.array([norb, ncj, nzero, mbreit, mtype, mpol, mb, nb, kb], np.int32), pq, kef,
-               np.array([nterm], np.int32), term, C, np.array([nl], np.int32), lterm, lval)
+               np.array([nterm], np.int32), term.T, C, np.array([nl], np.int32), lterm.T, lval)
     print("[fortran CI3 driver]", flush=True)
     r = subprocess.run([CI3], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
+    assert (tmp_path / "drv_ci3_out.bin").exists(), r.stdout + r.stderr
     out = np.fromfile(tmp_path / "drv_ci3_out.bin", dtype=np.float64)
     HM = out[:ncj * nzero].reshape((ncj, nzero), order="F")
     DM = out[ncj * nzero:]
@@ -158,6 +175,11 @@ def test_fortran_mat3_state_res(tmp_path, gpu_available):
         # overlap terms need a configuration pair or a channel overlap (Idef_Otype)
         ok0 = (ic > 0 and jc > 0) or io > 0
         icase = int(rng.choice([0, 2, 3] if ok0 else [2, 3]))
+        # <i|j> alone is the target structure -- for overlaps always, for R and
+        # S when no channel orbital is in the integral: io pairs two channel
+        # orbitals there
+        if io > 0 and jo == 0 and ic == 0 and (icase == 0 or not any(IPB[o] > 0 for o in orbs)):
+            io = (io // IBO) * IBO + int(rng.integers(1, 4))
         terms.append([icase, int(rng.integers(0, 3)), *orbs, ic, jc, io, jo])
     terms = np.array(terms, dtype=np.int32)
     C = rng.uniform(-1, 1, len(terms))
@@ -166,10 +188,11 @@ def test_fortran_mat3_state_res(tmp_path, gpu_available):
         _write(f, np.array([ns, ks, ksp, ksq], np.int32), np.asarray(t, np.float64),
                np.array([NORB, NORB, IBO, mk, nch, npert, nvch, 1, ntarg], np.int32), pq, LBS[1:], KAPPA[1:],
                IPB[1:], kb, m["icc"], m["icb"], m["ibb"], iptar[1:], vch, fppqq,
-               np.array([len(terms)], np.int32), terms, C)
+               np.array([len(terms)], np.int32), terms.T, C)
     print("[fortran MAT3 driver]", flush=True)
     r = subprocess.run([MAT3], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
+    assert (tmp_path / "drv_mat3_out.bin").exists(), r.stdout + r.stderr
     out = np.fromfile(tmp_path / "drv_mat3_out.bin", dtype=np.float64)
     nblk, ncb, nbb = int(m["icc"].max()), int(m["icb"].max()), int(m["ibb"].max())
     sizes = [ms * ms * nblk, nblk * (mk + 1), ms * ncb, nbb, ntarg, ntarg]
