@@ -25,7 +25,9 @@ host (one multipole's rows).
   3  256 uneven instances (ns 60..400, ks 7..9, kmax 2..10, 4..8 orbitals,
      all pairs) forward+adjoint in one hipGraph, autotuned;
   5  SCF-style: ns=600, 60 orbitals, k=0..12, 1e6 rows, 20 iterations of
-     new orbitals + one forward+adjoint hipGraph replay.
+     new orbitals + one forward+adjoint hipGraph replay;
+  6  the CI3 accumulation path: Add_integral (with Breit) of 2e5 terms,
+     c_data, every list by Rk_evaluate / Sk_evaluate into HM/DM.
 """
 from __future__ import annotations
 
@@ -651,6 +653,84 @@ def run_config5(args):
                        {"roofline": roof, "step_work": work, "cpu_baseline": cpu, **extra})
 
 
+def run_config6(args):
+    """the CI3 accumulation path: 2e5 seeded angular terms (c ~ U[-1,1],
+    k ~ U{0..4}, a CSF pair ic >= jc of 400, four of 30 orbitals with
+    seeded kappa) through the library's CI3 Add_integral with the Breit
+    terms (DBSR_CI3/add_integral.f90) and c_data; every merged list by
+    Rk_evaluate / Sk_evaluate on the device into HM(400,400) and DM; ns=200
+    (ks=9/8/9).  One step = every list once."""
+    import torch
+    from dbsr3_amd import CData, DBS
+    from dbsr3_amd.cdata import rk_evaluate
+    from dbsr3_amd.dbs import fortran_to_device
+    from dbsr3_amd.grid import mkgrid_02
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    ns, ks, ksp, ksq, norb, ncsf, nterm, kmax = 200, 9, 8, 9, 30, 400, 200000, 4
+    t = mkgrid_02(ns, ks, 26.0, 0.25, 30.0)
+    d = DBS(t, ns, ks, ksp, ksq)
+    pq = normalise_on_gpu(d, seeded_orbitals(t, ns, ks, ksp, ksq, norb, 20261203 + 6))
+    rng = np.random.default_rng(20261203 + 6)
+    kappa = np.concatenate([[0], rng.choice([-1, 1, -2, 2, -3, 3], norb)]).astype(np.int32)
+    lists = []
+    cd = CData(6, 0, kmax + 1, 2000, 1000, 100, 1e-12, lambda it, k, l: lists.append((it, k, l)))
+    ic = rng.integers(1, ncsf + 1, nterm)
+    jc = rng.integers(1, ncsf + 1, nterm)
+    orbs = rng.integers(1, norb + 1, (nterm, 4))
+    kk = rng.integers(0, kmax + 1, nterm)
+    cc = rng.uniform(-1.0, 1.0, nterm)
+    for n in range(nterm):
+        i, j = max(ic[n], jc[n]), min(ic[n], jc[n])
+        cd.add_integral_ci(int(kk[n]), int(i), int(j), *(int(x) for x in orbs[n]), float(cc[n]), kappa, mbreit=1)
+    for it in range(1, 7):
+        for k in range(kmax + 2):
+            cd.add_matrix(it, k)
+    cd.close()
+    nl = sum(len(l[2][0]) for l in lists)
+    pq_dev = fortran_to_device(pq)
+    HM = torch.zeros((ncsf, ncsf), dtype=torch.float64, device="cuda")
+    DM = torch.zeros(ncsf, dtype=torch.float64, device="cuda")
+
+    def sweep():
+        for it, k, lst in lists:
+            rk_evaluate(d, it, k, lst, pq_dev, norb, HM, ncsf, ncsf, DM)
+    sweep()  # warm: builds the arrays
+    torch.cuda.synchronize()
+    hm = HM.cpu().numpy().T.copy()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sweep()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    value = nl / (ms * 1e-3)
+    # the oracle on the lists of one multipole of each R type (CPU baseline and check)
+    g = O.Grid(t, ns, ks, ksp, ksq)
+    HMr, DMr = np.zeros((ncsf, ncsf), order="F"), np.zeros(ncsf)
+    sample = [(it, k, lst) for it, k, lst in lists if k == 1 and it <= 4]
+    ns_entries = sum(len(l[2][0]) for l in sample)
+    t0 = time.perf_counter()
+    for it, k, lst in sample:
+        O.rk_evaluate(g, it, k, lst, pq, 2 ** 15, HMr, ncsf, DMr)
+    secs = time.perf_counter() - t0
+    HMs = torch.zeros((ncsf, ncsf), dtype=torch.float64, device="cuda")
+    DMs = torch.zeros(ncsf, dtype=torch.float64, device="cuda")
+    for it, k, lst in sample:
+        rk_evaluate(d, it, k, lst, pq_dev, norb, HMs, ncsf, ncsf, DMs)
+    err = float(np.max(np.abs(HMs.cpu().numpy().T - HMr)) / np.max(np.abs(HMr)))
+    if not err <= 1e-12:
+        raise SystemExit(f"config 6: HM differs from the oracle (max rel err {err:.3e})")
+    cpu = {"value": ns_entries / secs, "unit": "list entries/s", "cores": 1, "kind": "port",
+           "sample": f"the R^k lists of k=1 ({ns_entries} entries, arrays built inside the timed region), "
+                     f"{secs:.1f} s single-thread"}
+    workload = {"workload": "CI3 Add_integral (with Breit) of 2e5 seeded terms, 30 orbitals, 400 CSFs, ns=200, "
+                            "k=0..4; every merged list by Rk_evaluate/Sk_evaluate into HM/DM; one step = every list",
+                "terms": nterm, "merged_entries": int(nl), "lists": len(lists), "ns": ns, "csfs": ncsf}
+    return config_line(6, value, "list entries/s", ms, args.steps, 1, workload,
+                       {"cpu_baseline": cpu, "max_rel_err_sampled_lists": err,
+                        "hm_norm": float(np.abs(hm).sum())})
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -668,7 +748,7 @@ def main():
                          "(a plain copy kernel of known size for hardware-counter scaling)")
     ap.add_argument("--traffic-file", type=str,
                     default=os.path.join(ROOT, "profiles", TRAFFIC_FILE))
-    ap.add_argument("--config", type=int, default=4, choices=[1, 2, 3, 4, 5],
+    ap.add_argument("--config", type=int, default=4, choices=[1, 2, 3, 4, 5, 6],
                     help="integral-set configuration (4: the headline workload)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -676,7 +756,7 @@ def main():
               "but no multi-GPU figure is produced in this build (use --gpus 1)", file=sys.stderr)
         return 2
     if args.config != 4:
-        run = {1: run_config1, 2: run_config2, 3: run_config3, 5: run_config5}[args.config]
+        run = {1: run_config1, 2: run_config2, 3: run_config3, 5: run_config5, 6: run_config6}[args.config]
         print(json.dumps(run(args)), flush=True)
         return 0
 
