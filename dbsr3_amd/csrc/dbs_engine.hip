@@ -74,10 +74,11 @@ struct KGroupDev {
   long long off_part;          // contraction partials: npr x nrows
   int nrows, row0;             // the group's rows row0 .. row0+nrows-1 (group order)
   int npr, dense;              // partial slots per row; rows contracted on the matrix cores
-  int chunk0, nchunk;          // adjacency chunks of 64 densities
+  int chunk0, nchunk;          // adjacency segments: nchunk blocks of 64 densities x nch
   int gmask, full;             // density kinds present (bit DK_TOT..DK_PQ); 1: every
                                // orbital pair a <= c is a total density of the group
   long long off_pidx;          // orbital pair -> density tables, one NO x NO per kind present
+  int nch, pcs;                // partner chunks of the pair panels, densities per chunk
 };
 
 struct DensDev {
@@ -93,23 +94,34 @@ struct OrbAdjDev {
   int dens, partner, factor, kind;
 };
 
-// adjacency of the densities of a group in 64-wide column blocks (entry i
-// of density 64c+t at base + 64 i + t): partner density (group-local, -1
-// pads the column) and the row's slot in the group when this density is
-// the row's left one and its value is wanted (-1: the density is the
-// row's right one; -2: left one, value produced elsewhere)
+// adjacency of the densities of a group in segments of 64-wide column
+// blocks: the entries of density X = 64b+t whose partner lies in partner
+// chunk c are the first entries of column t of segment chunk0 + b*nch + c,
+// entry i at base + 64 i + t; the entries where X is the row's left density
+// come first.  A segment has len rows (its longest column, rounded up to
+// DBS_PAIR_PF) plus DBS_PAIR_PF more, the rest of every column filled with
+// inert entries (the chunk's first partner, no slot, coefficient 0), so the
+// 64 lanes run the same trip count: len with the adjoint, lenl (the longest
+// run of left entries, rounded up) without.  An entry holds the partner
+// density (group-local) and,
+// when X is the row's left density and the row's value is wanted, the
+// row's partial slot >= 0: the group's left entries numbered in position
+// order, so the lanes of a wavefront that update partials touch
+// consecutive doubles.  -1: X is the row's right density; -2: left one,
+// value produced elsewhere.
 struct AdjChunk {
   long long base;
-  int len, pad;
+  int len, lenl;
 };
 struct AdjEnt {
   int P, slot;
   double c;  // the row's coefficient, gathered at every launch
 };
 
-// pair-panel work: slots [j0, j1) of group g, partial column pr
+// pair-panel work: slots [j0, j1) of group g, partial column pr, owner
+// densities [o0, o1) (the chunked kernel)
 struct PairWork {
-  int g, j0, j1, pr;
+  int g, j0, j1, pr, o0, o1;
 };
 
 // sparse-row reduction: value = sum_p part[pbase + p*stride], p < npr
@@ -583,161 +595,163 @@ __global__ void k_energy(int nchunks, const int2* __restrict__ chunks, const lon
 // ---------------------------------------------------------------------------
 // pair panels: contraction of the rows and (with the adjoint) the gather
 //   G_X = sum over rows touching X of coef * (partner features, u<->v)
-// in one pass over the features.  A workgroup owns pair slots [j0, j1) of
-// one group and walks them W slots at a time: the W-slot panel of every
-// density of the group is staged in LDS as [density][u(W) v(W)], then each
-// thread takes one density X, keeps its own panel and gradient in
-// registers and streams its adjacency: per entry the partner's 2W values
-// come from LDS, the row's partial value (X = left density) and the
-// gradient are accumulated in a fixed order.  Partial values of a row over
-// the workgroup's panels go to part[off_part + pr*nrows + slot]; the row
-// value is their sum in pr order (k_rows_reduce).  Every feature is read
-// from HBM once per launch and every gradient written once.
-// LDS = false reads the partner panels from global memory (groups whose
-// panel does not fit in LDS).
+// in one pass over the features.  A workgroup takes pair slots [j0, j1) of
+// one group W slots at a time: the W-slot panels of the densities are
+// staged in LDS, each thread owns a density X, keeps its own panel and
+// gradient in registers and streams its adjacency: per entry the partner's
+// 2W values come from LDS, the row's partial value (X = left density) and
+// the gradient are accumulated in a fixed order.  Partial values of a row
+// over the workgroup's panels go to the row's partial column pr; the row
+// value is their sum in pr order (k_rows_reduce).
 // ---------------------------------------------------------------------------
-// the register panel of W = 16 needs the full register file of a lane
-// (workgroups of at most 256 threads); W <= 8 runs 512-thread workgroups
-template <int W>
-constexpr int pair_block_max() { return W >= 16 ? 256 : 512; }
 constexpr int kFeatGroups = 16;  // most groups per item of k_features_mfma (LDS sizing)
 
-template <int W, bool LDS, bool ADJ>
-__global__ void __launch_bounds__(pair_block_max<W>()) k_pair_panel(
-    int nwork, const PairWork* __restrict__ work, const KGroupDev* __restrict__ kgs,
-    const InstDev* __restrict__ inst, const AdjChunk* __restrict__ chunks,
-    const AdjEnt* __restrict__ ent, const int2* __restrict__ alen,
-    const double* __restrict__ F, double* __restrict__ Gout, double* __restrict__ part, int onesided) {
-  extern __shared__ double sp[];
-  constexpr int LD = 2 * W + 2;  // LDS row: u(W) v(W) + pad (16-byte aligned rows)
-  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
-    const PairWork Wk = work[w];
-    const KGroupDev Gp = kgs[Wk.g];
-    const int nv = inst[Gp.inst].nv, nd = Gp.ndens;
-    const double* Fg = F + Gp.off_F;
-    bool first = true;
-    for (int j0 = Wk.j0; j0 < Wk.j1; j0 += W) {
-      const int nj = min(W, Wk.j1 - j0);
-      if (LDS) {
-        __syncthreads();
-        const int tot = 2 * W * nd;
-        for (int q = threadIdx.x; q < tot; q += blockDim.x) {
-          const int x = q / nd, d = q - x * nd;  // x = 2 jl + h
-          const int jl = x >> 1, h = x & 1;
-          sp[d * LD + h * W + jl] = jl < nj ? Fg[(2LL * j0 + x) * nd + d] : 0.0;
-        }
-        __syncthreads();
+// Lane order of the pair panels.  The panel of a density is E = 2W doubles
+// [u0 v0 u1 v1 ...] (element e = 2 jl + h) and sits in LDS as a row of E
+// doubles, so the bank of an element does not depend on the row.  Lane l
+// takes the E/2 element pairs in the order k ^ r, r = l mod min(E/2, 16),
+// and moves a pair with one 16-byte access: the 16 lanes that share an LDS
+// cycle (64 banks) touch 16 different pairs, which for rows of 32 doubles
+// (W = 16) are the 64 banks exactly, wherever their rows are.  Registers
+// t = 2k + h hold element 2 (k ^ r) + h; the byte address of that pair of
+// a row at byte offset R (a multiple of E*8) is (R | r << 4) ^ (k << 4): one
+// XOR per access.
+template <int E>
+__device__ inline int pair_lane(int lane) {
+  return lane & ((E / 2 < 16 ? E / 2 : 16) - 1);
+}
+
+// v[2k + h] <- v[2 (k ^ r) + h] for r < E/2: per bit b of r, pairs k and
+// k | b swap where the bit is set (an involution: the same call maps back).
+// Written as explicit swaps, one select per register and bit.
+template <int E>
+__device__ inline void permute_pairs(double (&v)[E], int r) {
+#pragma unroll
+  for (int b = 1; b < E / 2; b <<= 1) {
+    const bool sw = (r & b) != 0;
+#pragma unroll
+    for (int k = 0; k < E / 2; ++k) {
+      if (k & b) continue;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const double x = v[2 * k + h], y = v[2 * (k | b) + h];
+        v[2 * k + h] = sw ? y : x;
+        v[2 * (k | b) + h] = sw ? x : y;
       }
-      // panel value of density d at (h, jl)
-      auto pv = [&](int d, int h, int jl) -> double {
-        if (LDS) return sp[d * LD + h * W + jl];
-        return jl < nj ? Fg[(2LL * (j0 + jl) + h) * nd + d] : 0.0;
-      };
-      for (int X = threadIdx.x; X < nd; X += blockDim.x) {
-        double fu[W], fv[W], gu[W], gv[W];
-#pragma unroll
-        for (int jl = 0; jl < W; ++jl) {
-          fu[jl] = pv(X, 0, jl);
-          fv[jl] = pv(X, 1, jl);
-          gu[jl] = 0.0;
-          gv[jl] = 0.0;
-        }
-        const AdjChunk C = chunks[Gp.chunk0 + (X >> 6)];
-        const long long eb = C.base + (X & 63);
-        // the forward stops after the entries where X is the row's left one
-        const int n = ADJ ? alen[Gp.dens0 + X].x : alen[Gp.dens0 + X].y;
-        // entries are fetched PF at a time so their loads overlap; the tail
-        // of the last group is padded with inert entries (own row, c = 0,
-        // no row value)
-        constexpr int PF = 4;
-        for (int i0 = 0; i0 < n; i0 += PF) {
-          AdjEnt Aq[PF];
-          double cq[PF];
-#pragma unroll
-          for (int q = 0; q < PF; ++q) {
-            const int i = i0 + q;
-            if (i < n) {
-              Aq[q] = ent[eb + 64LL * i];
-              cq[q] = ADJ ? Aq[q].c : 0.0;
-            } else {
-              Aq[q].P = X;
-              Aq[q].slot = -1;
-              cq[q] = 0.0;
-            }
-          }
-#pragma unroll
-          for (int q = 0; q < PF; ++q) {
-            const AdjEnt A = Aq[q];
-            if (!ADJ && A.slot < 0) continue;
-            const double c = cq[q];
-            double dot = 0.0;
-#pragma unroll
-            for (int jl = 0; jl < W; ++jl) {
-              const double pu = pv(A.P, 0, jl), pvv = pv(A.P, 1, jl);
-              if (!onesided) {
-                dot += fu[jl] * pvv + fv[jl] * pu;
-                if (ADJ) {
-                  gu[jl] += c * pvv;
-                  gv[jl] += c * pu;
-                }
-              } else {
-                // S^k: S_L Mm_R (j < nv) and Y_L Ro_R (j >= nv)
-                const bool lower = j0 + jl < nv;
-                const bool left = A.slot != -1;
-                if (lower) {
-                  dot += fu[jl] * pvv;
-                  if (ADJ) {
-                    if (left) gu[jl] += c * pvv;
-                    else gv[jl] += c * pu;
-                  }
-                } else {
-                  dot += fv[jl] * pu;
-                  if (ADJ) {
-                    if (left) gv[jl] += c * pu;
-                    else gu[jl] += c * pvv;
-                  }
-                }
-              }
-            }
-            if (A.slot >= 0) {
-              double* o = part + Gp.off_part + (long long)Wk.pr * Gp.nrows + A.slot;
-              *o = first ? dot : *o + dot;
-            }
-          }
-        }
-        if (ADJ) {
-          double* g = Gout + Gp.off_F + X;
-#pragma unroll
-          for (int jl = 0; jl < W; ++jl) {
-            if (jl < nj) {
-              g[(2LL * (j0 + jl)) * nd] = gu[jl];
-              g[(2LL * (j0 + jl) + 1) * nd] = gv[jl];
-            }
-          }
-        }
-      }
-      first = false;
     }
   }
 }
 
-// Bank-conflict-free pair panels for groups of up to T*DPT densities
-// (thread t owns densities t + q blockDim, q < DPT; T = 512 threads, 1024
-// at W = 4; DPT = 2 for the narrow panels W = 8, 4 of groups up to 1024 or
-// 2048 densities).  The panel of density X is E = 2W doubles
-// [u0 v0 u1 v1 ...] (element e = 2 jl + h), kept in LDS as B[X][e] with
-// rows of E doubles, so the LDS bank of element e does not depend on the
-// row.  Lane l works on the elements in the rotated order
-// e_t = (t + rot) mod E, rot = 2 (l mod 8), and reads its partner's element
-// pair (e_t, e_t + 1) with one 16-byte access: the 8 lanes that share an LDS
-// cycle touch 8 different groups of 4 banks, wherever their partner rows
-// are.  Registers fx[t], gx[t] hold element e_t: the panel is loaded
-// coalesced in element order and rotated in registers by three conditional
-// stages (rot bits 2, 4, 8), and the gradient rotated back before its
-// coalesced store, so the LDS holds only the panels (E*ndens doubles).
-// Adjacency entries are loaded one iteration ahead of their use.
-// workgroup size limit of the rotated kernel: W = 4 panels leave room for
-// 1024 threads (more waves to hide the adjacency loads of narrow panels)
+// LDS accesses by 32-bit byte address: the address of an element pair is
+// one XOR away from its row's, with no base to add.  The pair-panel kernels
+// use dynamic LDS only, which starts at address 0, so row offsets keep the
+// zero low bits the XOR relies on.
+typedef double pair_v __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) pair_v lds_pair_v;
+__device__ inline unsigned lds_addr(const double* p) {
+  return (unsigned)(size_t)((const __attribute__((address_space(3))) double*)p);
+}
+__device__ inline pair_v lds_get(unsigned a) { return *(const lds_pair_v*)(size_t)a; }
+__device__ inline void lds_put(unsigned a, double x, double y) {
+  pair_v v;
+  v.x = x;
+  v.y = y;
+  *(lds_pair_v*)(size_t)a = v;
+}
+
+// S^k pair panels: bit t set where the element of register t of a panel at
+// slot j0 pairs as the row's left density (S_L Mm_R below nv, Y_L Ro_R above)
+template <int E>
+__device__ inline unsigned left_mask(int j0, int nv, int r) {
+  unsigned mL = 0;
+#pragma unroll
+  for (int t = 0; t < E; ++t) {
+    const int e = (((t >> 1) ^ r) << 1) | (t & 1);
+    const bool lower = j0 + (e >> 1) < nv;
+    if (lower ? (e & 1) == 0 : (e & 1) == 1) mL |= 1u << t;
+  }
+  return mL;
+}
+
+// one adjacency entry against the partner row at LDS byte address a (the
+// row's address | r << 4): the row's partial dot and, with the adjoint, the
+// gradient update; the pairing swaps u and v.  The dot runs in four
+// accumulators (pairs k mod 4), summed as (d0 + d1) + (d2 + d3).
+template <int E, bool ONE, bool ADJ>
+__device__ inline double pair_entry(unsigned a, const double (&fx)[E], double (&gx)[E], double c, bool left,
+                                    unsigned mL) {
+  double d[4] = {0.0, 0.0, 0.0, 0.0};
+  if (!ONE) {
+#pragma unroll
+    for (int t = 0; t < E; t += 2) {
+      const pair_v pp = lds_get(a ^ (unsigned)(t << 3));
+      d[(t >> 1) & 3] += fx[t] * pp.y;
+      d[(t >> 1) & 3] += fx[t + 1] * pp.x;
+      if (ADJ) {
+        gx[t] += c * pp.y;
+        gx[t + 1] += c * pp.x;
+      }
+    }
+  } else {
+    const unsigned mg = left ? mL : ~mL;
+#pragma unroll
+    for (int t = 0; t < E; t += 2) {
+      const pair_v pp = lds_get(a ^ (unsigned)(t << 3));
+      if ((mL >> t) & 1u) d[(t >> 1) & 3] += fx[t] * pp.y;
+      if ((mL >> (t + 1)) & 1u) d[(t >> 1) & 3] += fx[t + 1] * pp.x;
+      if (ADJ && ((mg >> t) & 1u)) gx[t] += c * pp.y;
+      if (ADJ && ((mg >> (t + 1)) & 1u)) gx[t + 1] += c * pp.x;
+    }
+  }
+  return (d[0] + d[1]) + (d[2] + d[3]);
+}
+
+// adjacency entries in flight per thread of the pair-panel kernels
+#ifndef DBS_PAIR_PF
+#define DBS_PAIR_PF 4
+#endif
+
+// one column of an adjacency segment, len entries (a multiple of
+// DBS_PAIR_PF) DBS_PAIR_PF at a time, with the same trip count on every
+// lane.  Each entry is loaded one group ahead of its use, into the
+// registers it is used from.  A row's partial over the workgroup's panels
+// is stored at its first panel and added to with a no-return atomic add at
+// the others: nothing waits on a partial, and since only this thread adds
+// to the slot, in program order, the sum is formed in panel order.
+template <int E, bool ONE, bool ADJ>
+__device__ inline void pair_entries(const AdjEnt* __restrict__ ent, long long eb, int len, unsigned rowbase, int P0,
+                                    int rsh, const double (&fx)[E], double (&gx)[E], double* __restrict__ pcol,
+                                    unsigned mL, bool first) {
+  constexpr int PF = DBS_PAIR_PF;
+  AdjEnt a[PF];
+#pragma unroll
+  for (int u = 0; u < PF; ++u) a[u] = ent[eb + 64LL * u];
+  for (int i = 0; i < len; i += PF) {
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+      if (i + u >= len) break;
+      const unsigned ad = (rowbase + (unsigned)((a[u].P - P0) * E * 8)) | (unsigned)rsh;
+      const double dot = pair_entry<E, ONE, ADJ>(ad, fx, gx, ADJ ? a[u].c : 0.0, a[u].slot != -1, mL);
+      if (a[u].slot >= 0) {
+        if (first) pcol[a[u].slot] = dot;
+        else unsafeAtomicAdd(pcol + a[u].slot, dot);
+      }
+      a[u] = ent[eb + 64LL * (i + u + PF)];
+      // one entry's partner reads in flight at a time (registers)
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+}
+
+// Pair panels for groups of up to T*DPT densities (thread t owns densities
+// t + q blockDim, q < DPT; T = 512 threads, 1024 at W = 4; DPT = 2 for the
+// narrow panels W = 8, 4 of groups up to 1024 or 2048 densities), the
+// panels of all the group's densities in LDS (E*ndens doubles).  Each
+// thread loads its densities' panels coalesced in element order, permutes
+// them into its lane order in registers and stores them as pairs; the
+// gradient is permuted back before its coalesced store.
+// workgroup size limit of the kernel: W = 4 panels leave room for 1024
+// threads (more waves to hide the adjacency loads of narrow panels)
 template <int W>
 constexpr int rot_threads_max() { return W == 4 ? 1024 : 512; }
 
@@ -745,19 +759,19 @@ template <int W, bool ONE, bool ADJ, int DPT>
 __global__ void __launch_bounds__(rot_threads_max<W>()) k_pair_rot(
     int nwork, const PairWork* __restrict__ work, const KGroupDev* __restrict__ kgs,
     const InstDev* __restrict__ inst, const AdjChunk* __restrict__ chunks,
-    const AdjEnt* __restrict__ ent, const int2* __restrict__ alen,
+    const AdjEnt* __restrict__ ent,
     const double* __restrict__ F, double* __restrict__ Gout, double* __restrict__ part) {
   extern __shared__ double sm[];
   constexpr int E = 2 * W;
-  // even: element pairs (u,v) stay 16-byte aligned
-  const int rot = (2 * (threadIdx.x & 7)) & (E - 1);
+  const int r = pair_lane<E>(threadIdx.x);
+  const unsigned sbase = lds_addr(sm);
   for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
     const PairWork Wk = work[w];
     const KGroupDev Gp = kgs[Wk.g];
     const int nv = inst[Gp.inst].nv, nd = Gp.ndens;
-    double* B = sm;  // [nd][E]
     const double* Fg = F + Gp.off_F;
-    // the thread's densities X = threadIdx.x + q blockDim.x, q < DPT
+    // the thread's densities X = threadIdx.x + q blockDim.x, q < DPT (one
+    // partner chunk: segment chunk0 + X/64)
     bool act[DPT];
     int n[DPT];
     long long eb[DPT];
@@ -771,24 +785,16 @@ __global__ void __launch_bounds__(rot_threads_max<W>()) k_pair_rot(
         const AdjChunk C = chunks[Gp.chunk0 + (X >> 6)];
         eb[q] = C.base + (X & 63);
         // the forward stops after the entries where X is the row's left one
-        n[q] = ADJ ? alen[Gp.dens0 + X].x : alen[Gp.dens0 + X].y;
+        n[q] = ADJ ? C.len : C.lenl;
       }
     }
     bool first = true;
     for (int j0 = Wk.j0; j0 < Wk.j1; j0 += W) {
       const int nj = min(W, Wk.j1 - j0);
-      // B of the previous panel is no longer read
+      // the previous panel is no longer read
       __syncthreads();
       double fx[DPT][E], gx[DPT][E];
-      unsigned mL = 0;  // S^k: elements that pair as the row's left density
-      if (ONE) {
-#pragma unroll
-        for (int t = 0; t < E; ++t) {
-          const int e = (t + rot) & (E - 1);
-          const bool lower = j0 + (e >> 1) < nv;
-          if (lower ? (e & 1) == 0 : (e & 1) == 1) mL |= 1u << t;
-        }
-      }
+      const unsigned mL = ONE ? left_mask<E>(j0, nv, r) : 0u;
 #pragma unroll
       for (int q = 0; q < DPT; ++q) {
         const int X = threadIdx.x + q * blockDim.x;
@@ -797,87 +803,118 @@ __global__ void __launch_bounds__(rot_threads_max<W>()) k_pair_rot(
         if (act[q]) {
 #pragma unroll
           for (int e = 0; e < E; ++e) fx[q][e] = (e >> 1) < nj ? Fg[(2LL * j0 + e) * nd + X] : 0.0;
-          // fx[t] = element (t + rot) mod E
+          permute_pairs<E>(fx[q], r);
+          const unsigned a = (sbase + (unsigned)(X * E * 8)) | (unsigned)(r << 4);
 #pragma unroll
-          for (int b = 2; b < E; b <<= 1) {
-            if (rot & b) {
-              double tmp[E];
-#pragma unroll
-              for (int t = 0; t < E; ++t) tmp[t] = fx[q][(t + b) & (E - 1)];
-#pragma unroll
-              for (int t = 0; t < E; ++t) fx[q][t] = tmp[t];
-            }
-          }
-#pragma unroll
-          for (int t = 0; t < E; ++t) B[X * E + ((t + rot) & (E - 1))] = fx[q][t];
+          for (int t = 0; t < E; t += 2) lds_put(a ^ (unsigned)(t << 3), fx[q][t], fx[q][t + 1]);
         }
       }
       __syncthreads();
+      double* const pcol = part + Gp.off_part + (long long)Wk.pr * Gp.nrows;
 #pragma unroll
       for (int q = 0; q < DPT; ++q) {
-        // the next entry is loaded while the current one is processed, so
-        // its partner index is known before its LDS reads are issued
-        const int nq = n[q];
-        AdjEnt nxt;
-        if (nq > 0) nxt = ent[eb[q]];
-        for (int i = 0; i < nq; ++i) {
-          const AdjEnt Aen = nxt;
-          if (i + 1 < nq) nxt = ent[eb[q] + 64LL * (i + 1)];
-          if (!ADJ && Aen.slot < 0) continue;
-          const double c = ADJ ? Aen.c : 0.0;
-          const double* pr = B + Aen.P * E;
-          // the row's partial of the previous panels, requested before the
-          // partner reads (workgroups with several panels)
-          double* const o = part + Gp.off_part + (long long)Wk.pr * Gp.nrows + (Aen.slot >= 0 ? Aen.slot : 0);
-          const double old = (!first && Aen.slot >= 0) ? *o : 0.0;
-          double dot = 0.0;
-          // partner pair (u,v) of elements t, t+1 in one 16-byte read; the
-          // pairing swaps u and v
-          if (!ONE) {
-#pragma unroll
-            for (int t = 0; t < E; t += 2) {
-              const double2 pp = *reinterpret_cast<const double2*>(pr + ((t + rot) & (E - 1)));
-              dot += fx[q][t] * pp.y;
-              dot += fx[q][t + 1] * pp.x;
-              if (ADJ) {
-                gx[q][t] += c * pp.y;
-                gx[q][t + 1] += c * pp.x;
-              }
-            }
-          } else {
-            const unsigned mg = Aen.slot != -1 ? mL : ~mL;
-#pragma unroll
-            for (int t = 0; t < E; t += 2) {
-              const double2 pp = *reinterpret_cast<const double2*>(pr + ((t + rot) & (E - 1)));
-              if ((mL >> t) & 1u) dot += fx[q][t] * pp.y;
-              if ((mL >> (t + 1)) & 1u) dot += fx[q][t + 1] * pp.x;
-              if (ADJ && ((mg >> t) & 1u)) gx[q][t] += c * pp.y;
-              if (ADJ && ((mg >> (t + 1)) & 1u)) gx[q][t + 1] += c * pp.x;
-            }
-          }
-          if (Aen.slot >= 0) *o = first ? dot : old + dot;
-        }
+        if (!act[q]) continue;
+        pair_entries<E, ONE, ADJ>(ent, eb[q], n[q], sbase, 0, r << 4, fx[q], gx[q], pcol, mL, first);
       }
       if (ADJ) {
 #pragma unroll
         for (int q = 0; q < DPT; ++q) {
           if (!act[q]) continue;
           // back to element order, then coalesced rows of G
-#pragma unroll
-          for (int b = 2; b < E; b <<= 1) {
-            if (rot & b) {
-              double tmp[E];
-#pragma unroll
-              for (int t = 0; t < E; ++t) tmp[t] = gx[q][(t - b) & (E - 1)];
-#pragma unroll
-              for (int t = 0; t < E; ++t) gx[q][t] = tmp[t];
-            }
-          }
+          permute_pairs<E>(gx[q], r);
           double* g = Gout + Gp.off_F + threadIdx.x + q * blockDim.x;
 #pragma unroll
           for (int e = 0; e < E; ++e)
             if ((e >> 1) < nj) g[(2LL * j0 + e) * nd] = gx[q][e];
         }
+      }
+      first = false;
+    }
+  }
+}
+
+// Pair panels of groups too large for k_pair_rot.  The group's densities
+// form nch partner chunks of pcs; per panel each chunk is staged in LDS in
+// turn, in the same row form and lane order, so the panel width does not
+// depend on the group size.  A workgroup owns the densities [o0, o1) of the
+// group (one per thread, o0 a multiple of 64); per chunk each thread
+// streams the entries of its density whose partner lies in that chunk, its
+// panel and gradient staying in registers across the chunks.  Staging
+// moves 8 element pairs (128 bytes) per step, lane l taking pair k ^ (l mod
+// 8): for W = 16 lanes with l mod 16 >= 8 take the other half of the row
+// first, for W = 8 they stage partners of the other parity, so the 16 lanes
+// of an LDS cycle write all 64 banks.
+template <int W, bool ONE, bool ADJ>
+__global__ void __launch_bounds__(512) k_pair_chunk(
+    int nwork, const PairWork* __restrict__ work, const KGroupDev* __restrict__ kgs,
+    const InstDev* __restrict__ inst, const AdjChunk* __restrict__ chunks,
+    const AdjEnt* __restrict__ ent,
+    const double* __restrict__ F, double* __restrict__ Gout, double* __restrict__ part) {
+  static_assert(W == 8 || W == 16, "chunked pair panels: W = 8 or 16");
+  extern __shared__ double sm[];
+  constexpr int E = 2 * W;
+  const int r = pair_lane<E>(threadIdx.x);
+  const int r8 = threadIdx.x & 7;
+  const int hsw = (threadIdx.x >> 3) & 1;
+  const unsigned sbase = lds_addr(sm);
+  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+    const PairWork Wk = work[w];
+    const KGroupDev Gp = kgs[Wk.g];
+    const int nv = inst[Gp.inst].nv, nd = Gp.ndens, nch = Gp.nch, pcs = Gp.pcs;
+    const double* Fg = F + Gp.off_F;
+    const int X = Wk.o0 + threadIdx.x;
+    const bool act = X < Wk.o1;
+    double* const pcol = part + Gp.off_part + (long long)Wk.pr * Gp.nrows;
+    bool first = true;
+    for (int j0 = Wk.j0; j0 < Wk.j1; j0 += W) {
+      const int nj = min(W, Wk.j1 - j0);
+      const unsigned mL = ONE ? left_mask<E>(j0, nv, r) : 0u;
+      double fx[E], gx[E];
+#pragma unroll
+      for (int t = 0; t < E; ++t) {
+        fx[t] = 0.0;
+        gx[t] = 0.0;
+      }
+      if (act) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) fx[e] = (e >> 1) < nj ? Fg[(2LL * j0 + e) * nd + X] : 0.0;
+        permute_pairs<E>(fx, r);
+      }
+      for (int c = 0; c < nch; ++c) {
+        const int c0 = c * pcs, cn = min(pcs, nd - c0);
+        // the previous chunk is no longer read
+        __syncthreads();
+        const int cn2 = (cn + 1) & ~1;
+        for (int s = threadIdx.x; s < cn2; s += blockDim.x) {
+          const int p = W == 8 ? s ^ hsw : s;
+          if (p >= cn) continue;
+#pragma unroll 1
+          for (int hh = 0; hh < E / 16; ++hh) {
+            const int h0 = 16 * (W == 16 ? hh ^ hsw : hh);
+            // one half at a time: its element rows from one base
+            const double* fr = Fg + (size_t)(2 * j0 + h0) * nd + (c0 + p);
+            double v[16];
+#pragma unroll
+            for (int e = 0; e < 16; ++e) v[e] = ((h0 + e) >> 1) < nj ? fr[(size_t)e * nd] : 0.0;
+            permute_pairs<16>(v, r8);
+            const unsigned a = (sbase + (unsigned)(p * E * 8) + (unsigned)(h0 * 8)) | (unsigned)(r8 << 4);
+#pragma unroll
+            for (int t = 0; t < 16; t += 2) lds_put(a ^ (unsigned)(t << 3), v[t], v[t + 1]);
+          }
+        }
+        __syncthreads();
+        if (!act) continue;
+        const AdjChunk S = chunks[Gp.chunk0 + (X >> 6) * nch + c];
+        // the forward stops after the entries where X is the row's left one
+        const int n = ADJ ? S.len : S.lenl;
+        pair_entries<E, ONE, ADJ>(ent, S.base + (X & 63), n, sbase, c0, r << 4, fx, gx, pcol, mL, first);
+      }
+      if (ADJ && act) {
+        permute_pairs<E>(gx, r);
+        double* g = Gout + Gp.off_F + X;
+#pragma unroll
+        for (int e = 0; e < E; ++e)
+          if ((e >> 1) < nj) g[(2LL * j0 + e) * nd] = gx[e];
       }
       first = false;
     }
@@ -1700,16 +1737,18 @@ struct dbs_engine {
   AdjEnt* d_ent = nullptr;
   long long* d_ecidx = nullptr;
   long long nent = 0;
-  int2* d_alen = nullptr;  // per density: adjacency length, entries as the row's left density
   // work lists [adjoint][variant]: 0 partner panels read from global
   // memory, 1 LDS panels, 2 LDS panels with the conflict-free rotation
-  PairWork* d_work[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
-  int nwork[2][3] = {{0, 0, 0}, {0, 0, 0}};
-  int pair_w = 16;        // panel width (pair slots) of this launch shape
-  size_t pair_shm = 0;    // LDS bytes of the LDS pair-panel kernel
+  // pair-panel work lists [forward / with adjoint][rotated / chunked kernel]
+  PairWork* d_work[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+  int nwork[2][2] = {{0, 0}, {0, 0}};
+  int pair_w = 16;        // panel width (pair slots) of the rotated kernel
   size_t rot_shm = 0;     // LDS bytes of the rotated kernel
   int rot_block = 64;     // its workgroup size (rot_block * rot_dpt >= ndens of its groups)
   int rot_dpt = 1;        // densities per thread of the rotated kernel
+  int chunk_w = 16;       // panel width of the chunked kernel
+  size_t chunk_shm = 0;   // its LDS bytes (one partner chunk)
+  int chunk_block = 64;   // its workgroup size (owner densities per workgroup)
   size_t lds_block = 0;   // device limit of LDS per workgroup (bytes)
   RowRed* d_red = nullptr;
   int nred = 0;
@@ -1841,8 +1880,9 @@ static size_t rot_lds_bytes(int W, int ndens) {
 }
 
 template <int W>
-static hipError_t pair_lds_attr(size_t bytes) {
-  const void* fs[] = {(const void*)k_pair_panel<W, true, true>, (const void*)k_pair_panel<W, true, false>};
+static hipError_t chunk_lds_attr(size_t bytes) {
+  const void* fs[] = {(const void*)k_pair_chunk<W, false, true>, (const void*)k_pair_chunk<W, false, false>,
+                      (const void*)k_pair_chunk<W, true, true>, (const void*)k_pair_chunk<W, true, false>};
   for (const void* f : fs) {
     const hipError_t err = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     if (err != hipSuccess) return err;
@@ -1850,13 +1890,21 @@ static hipError_t pair_lds_attr(size_t bytes) {
   return hipSuccess;
 }
 
-static size_t pair_lds_bytes(int W, int ndens) { return (size_t)ndens * (2 * W + 2) * sizeof(double); }
+// LDS of one partner chunk of the chunked kernel (KiB, capped by the
+// device's limit per workgroup): the whole of a CU's 160 KiB.  The kernel's
+// registers allow one 512-thread workgroup per CU either way, and fewer,
+// larger chunks balance the entry counts of the 64 lanes better.
+#ifndef DBS_CHUNK_LDS_KB
+#define DBS_CHUNK_LDS_KB 160
+#endif
 
-// Pair-panel tables for the current tuning: adjacency blocks (the slot
-// code depends on which groups are dense), work lists, partial layout and
-// the sparse-row reduction list.
+// Pair-panel tables for the current tuning: the kernel of each group (the
+// rotated one when the group's panel fits in LDS, the chunked one
+// otherwise), adjacency segments (the slot code depends on which groups
+// are dense and on the kernel), work lists, partial layout and the
+// sparse-row reduction list.
 static int build_pair_tables(dbs_engine* e) {
-  int W = e->tune[2];
+  const int W = e->tune[2];
   const int NP = e->tune[5];
   if (e->lds_block == 0) {
     int v = 0;
@@ -1865,76 +1913,20 @@ static int build_pair_tables(dbs_engine* e) {
     e->lds_block = (size_t)v;
   }
   const size_t kPairLdsMax = e->lds_block;
-  // largest width <= tuning whose panels fit LDS for the groups (groups that
-  // do not fit even at width 4 use the global-memory variant)
-  int maxnd = 0;
-  for (int g = 0; g < e->nkg; ++g)
-    if (pair_lds_bytes(4, e->h_kg[g].ndens) <= kPairLdsMax) maxnd = std::max(maxnd, e->h_kg[g].ndens);
-  while (W > 4 && pair_lds_bytes(W, maxnd) > kPairLdsMax) W /= 2;
-  e->pair_w = W;
-  e->pair_shm = pair_lds_bytes(W, std::max(maxnd, 1));
-  if (e->pair_shm > 64 * 1024) {
-    const hipError_t err = W == 4 ? pair_lds_attr<4>(e->pair_shm)
-                                  : (W == 8 ? pair_lds_attr<8>(e->pair_shm) : pair_lds_attr<16>(e->pair_shm));
-    if (err != hipSuccess) return fail(DBS_ERR_HIP, std::string("pair tables: LDS size: ") + hipGetErrorString(err));
-  }
-  // adjacency in 64-wide column blocks
-  std::vector<AdjChunk> chunks;
-  std::vector<AdjEnt> ent;
-  std::vector<long long> ecidx;
-  long long base = 0;
-  for (int g = 0; g < e->nkg; ++g) {
-    KGroupDev& G = e->h_kg[g];
-    G.chunk0 = (int)chunks.size();
-    G.nchunk = (G.ndens + 63) / 64;
-    for (int c = 0; c < G.nchunk; ++c) {
-      int len = 0;
-      for (int t = 0; t < 64 && 64 * c + t < G.ndens; ++t)
-        len = std::max(len, (int)e->h_adj[G.dens0 + 64 * c + t].size());
-      AdjChunk C;
-      C.base = base;
-      C.len = len;
-      C.pad = 0;
-      chunks.push_back(C);
-      ent.resize((size_t)(base + 64LL * len), AdjEnt{-1, -1, 0.0});
-      ecidx.resize((size_t)(base + 64LL * len), -1);
-      for (int t = 0; t < 64 && 64 * c + t < G.ndens; ++t) {
-        const int X = G.dens0 + 64 * c + t;
-        // entries where X is the row's left density first (they also form
-        // the row value), each part in row order
-        std::vector<int2> lst;
-        for (const int2& a : e->h_adj[X]) if (a.x >= 0) lst.push_back(a);
-        for (const int2& a : e->h_adj[X]) if (a.x < 0) lst.push_back(a);
-        for (size_t i = 0; i < lst.size(); ++i) {
-          const int r = lst[i].y;
-          const RowDev& R = e->h_rows[r];
-          const bool left = lst[i].x >= 0;
-          const int partner = left ? R.R : R.L;
-          AdjEnt A;
-          A.P = partner - G.dens0;
-          A.slot = left ? (e->group_dense[g] ? -2 : r - G.row0) : -1;
-          A.c = 0.0;
-          ent[(size_t)(base + 64LL * (long long)i + t)] = A;
-          ecidx[(size_t)(base + 64LL * (long long)i + t)] = e->h_row_out[r];
-        }
-      }
-      base += 64LL * len;
-    }
-  }
-  e->nent = base;
-  // work lists and partial layout
-  std::vector<PairWork> work[2][3];
-  std::vector<RowRed> red;
-  long long poff = 0;
-  // groups for the rotated kernel: up to T threads (512, 1024 at W = 4)
-  // with DPT densities each (DPT = 1 at W = 16, up to 2 at W = 8 and 4)
+  // rotated kernel: up to T threads (512, 1024 at W = 4) with DPT densities
+  // each (DPT = 1 at W = 16, up to 2 at W = 8 and 4)
   const int tmax = W == 4 ? 1024 : 512;
   const int rot_max = W >= 16 ? tmax : 2 * tmax;
+  std::vector<char> rot(std::max(e->nkg, 1), 0);
   int rot_nd = 0;
   for (int g = 0; g < e->nkg; ++g) {
     const int nd = e->h_kg[g].ndens;
-    if (nd <= rot_max && rot_lds_bytes(W, nd) <= kPairLdsMax) rot_nd = std::max(rot_nd, nd);
+    if (nd <= rot_max && rot_lds_bytes(W, nd) <= kPairLdsMax) {
+      rot[g] = 1;
+      rot_nd = std::max(rot_nd, nd);
+    }
   }
+  e->pair_w = W;
   e->rot_dpt = rot_nd <= tmax ? 1 : 2;
   e->rot_block = std::max(64, ((rot_nd + e->rot_dpt - 1) / e->rot_dpt + 63) / 64 * 64);
   e->rot_shm = rot_lds_bytes(W, std::max(rot_nd, 1));
@@ -1945,42 +1937,147 @@ static int build_pair_tables(dbs_engine* e) {
     else err = e->rot_dpt == 1 ? rot_lds_attr<4, 1>(e->rot_shm) : rot_lds_attr<4, 2>(e->rot_shm);
     if (err != hipSuccess) return fail(DBS_ERR_HIP, std::string("pair tables: LDS size: ") + hipGetErrorString(err));
   }
+  // chunked kernel: panel width at least 8 (128-byte LDS rows), partner
+  // chunks within the LDS budget, owner ranges of at most tuning[4]
+  // densities in whole 64-density blocks
+  const int CW = W < 8 ? 8 : W;
+  const size_t rowb = 2 * (size_t)CW * sizeof(double);
+  const size_t cbudget = std::min(kPairLdsMax, (size_t)DBS_CHUNK_LDS_KB * 1024);
+  const int omax = e->tune[4];
+  e->chunk_w = CW;
+  e->chunk_shm = 0;
+  e->chunk_block = 64;
+  std::vector<int> osz(std::max(e->nkg, 1), 0);
+  for (int g = 0; g < e->nkg; ++g) {
+    KGroupDev& G = e->h_kg[g];
+    G.nch = 1;
+    G.pcs = std::max(G.ndens, 1);
+    if (rot[g]) continue;
+    G.nch = (int)std::max<size_t>(1, ((size_t)G.ndens * rowb + cbudget - 1) / cbudget);
+    G.pcs = (G.ndens + G.nch - 1) / G.nch;
+    const int nor = (G.ndens + omax - 1) / omax;
+    osz[g] = ((G.ndens + nor - 1) / nor + 63) / 64 * 64;
+    e->chunk_shm = std::max(e->chunk_shm, (size_t)G.pcs * rowb);
+    e->chunk_block = std::max(e->chunk_block, osz[g]);
+  }
+  if (e->chunk_shm > 64 * 1024) {
+    const hipError_t err = CW == 8 ? chunk_lds_attr<8>(e->chunk_shm) : chunk_lds_attr<16>(e->chunk_shm);
+    if (err != hipSuccess) return fail(DBS_ERR_HIP, std::string("pair tables: LDS size: ") + hipGetErrorString(err));
+  }
+  // adjacency segments: per block of 64 densities and partner chunk
+  const int PF = DBS_PAIR_PF;
+  std::vector<AdjChunk> chunks;
+  std::vector<AdjEnt> ent;
+  std::vector<long long> ecidx;
+  // per group: the row of each partial slot (left entries in position order)
+  std::vector<std::vector<int>> slot_row(std::max(e->nkg, 1));
+  long long base = 0;
+  for (int g = 0; g < e->nkg; ++g) {
+    KGroupDev& G = e->h_kg[g];
+    const int nd = G.ndens, nch = G.nch, pcs = G.pcs;
+    G.chunk0 = (int)chunks.size();
+    G.nchunk = (nd + 63) / 64;
+    const long long ebase = base;
+    std::vector<std::vector<int2>> lst((size_t)64 * nch);
+    for (int b = 0; b < G.nchunk; ++b) {
+      // per density and partner chunk: the entries where the density is the
+      // row's left one (they also form the row value), then the others, each
+      // part in row order
+      for (auto& v : lst) v.clear();
+      for (int t = 0; t < 64 && 64 * b + t < nd; ++t) {
+        const int X = G.dens0 + 64 * b + t;
+        for (int pass = 0; pass < 2; ++pass)
+          for (const int2& a : e->h_adj[X]) {
+            const bool left = a.x >= 0;
+            if (left != (pass == 0)) continue;
+            const RowDev& R = e->h_rows[a.y];
+            const int partner = (left ? R.R : R.L) - G.dens0;
+            lst[(size_t)t * nch + partner / pcs].push_back(a);
+          }
+      }
+      for (int c = 0; c < nch; ++c) {
+        int len = 0, lenl = 0;
+        for (int t = 0; t < 64; ++t) {
+          const std::vector<int2>& L = lst[(size_t)t * nch + c];
+          int nl = 0;
+          for (const int2& a : L) nl += a.x >= 0 ? 1 : 0;
+          len = std::max(len, (int)L.size());
+          lenl = std::max(lenl, nl);
+        }
+        len = (len + PF - 1) / PF * PF;
+        lenl = (lenl + PF - 1) / PF * PF;
+        chunks.push_back(AdjChunk{base, len, lenl});
+        // inert entries: the chunk's first partner, no slot, coefficient 0
+        const long long rows_alloc = 64LL * (len + PF);
+        ent.resize((size_t)(base + rows_alloc), AdjEnt{c * pcs, -1, 0.0});
+        ecidx.resize((size_t)(base + rows_alloc), -1);
+        for (int t = 0; t < 64 && 64 * b + t < nd; ++t) {
+          const std::vector<int2>& L = lst[(size_t)t * nch + c];
+          for (size_t i = 0; i < L.size(); ++i) {
+            const int r = L[i].y;
+            const RowDev& R = e->h_rows[r];
+            const bool left = L[i].x >= 0;
+            AdjEnt A;
+            A.P = (left ? R.R : R.L) - G.dens0;
+            // slot: the row itself for now, numbered below
+            A.slot = left ? (e->group_dense[g] ? -2 : r) : -1;
+            A.c = 0.0;
+            const long long pos = base + 64LL * (long long)i + t;
+            ent[(size_t)pos] = A;
+            ecidx[(size_t)pos] = e->h_row_out[r];
+          }
+        }
+        base += rows_alloc;
+      }
+    }
+    // partial slots of the left entries in position order
+    for (long long pos = ebase; pos < base; ++pos) {
+      AdjEnt& A = ent[(size_t)pos];
+      if (A.slot < 0) continue;
+      slot_row[g].push_back(A.slot);
+      A.slot = (int)slot_row[g].size() - 1;
+    }
+  }
+  e->nent = base;
+  // work lists and partial layout
+  std::vector<PairWork> work[2][2];
+  std::vector<RowRed> red;
+  long long poff = 0;
   for (int g = 0; g < e->nkg; ++g) {
     KGroupDev& G = e->h_kg[g];
     const InstDev& I = e->h_inst[G.inst];
     const int J = I.nv * (1 + I.ks);
+    const int Wg = rot[g] ? W : CW;
     // partial values of the group's rows are bounded to kPartMax doubles:
     // large row sets take more panels per workgroup
     const long long kPartMax = 32LL << 20;
     int np_g = NP;
     if (!e->group_dense[g] && G.nrows > 0) {
       const long long maxpr = std::max(1LL, kPartMax / G.nrows);
-      const long long need = (J + (long long)W * np_g - 1) / ((long long)W * np_g);
-      if (need > maxpr) np_g = (int)((J + W * maxpr - 1) / (W * maxpr));
+      const long long need = (J + (long long)Wg * np_g - 1) / ((long long)Wg * np_g);
+      if (need > maxpr) np_g = (int)((J + Wg * maxpr - 1) / (Wg * maxpr));
     }
-    const int span = W * np_g;
+    const int span = Wg * np_g;
     const int npr = (J + span - 1) / span;
-    int var = pair_lds_bytes(W, G.ndens) <= kPairLdsMax ? 1 : 0;
-    if (G.ndens <= rot_nd) var = 2;
     G.npr = e->group_dense[g] ? 0 : npr;
     G.dense = e->group_dense[g];
     G.off_part = poff;
     if (!G.dense) poff += (long long)npr * G.nrows;
-    for (int p = 0; p < npr; ++p) {
-      PairWork Wk;
-      Wk.g = g;
-      Wk.j0 = p * span;
-      Wk.j1 = std::min(J, (p + 1) * span);
-      Wk.pr = p;
-      if (G.nrows == 0) continue;
-      work[1][var].push_back(Wk);
-      if (!G.dense) work[0][var].push_back(Wk);
-    }
+    if (G.nrows == 0) continue;
+    const int var = rot[g] ? 0 : 1;
+    const int ostep = rot[g] ? std::max(G.ndens, 1) : osz[g];
+    for (int p = 0; p < npr; ++p)
+      for (int o0 = 0; o0 < G.ndens; o0 += ostep) {
+        const PairWork Wk{g, p * span, std::min(J, (p + 1) * span), p, o0, std::min(G.ndens, o0 + ostep)};
+        work[1][var].push_back(Wk);
+        if (!G.dense) work[0][var].push_back(Wk);
+      }
+    // in slot order: consecutive rows read consecutive partials
     if (!G.dense)
-      for (int q = 0; q < G.nrows; ++q) {
+      for (size_t q = 0; q < slot_row[g].size(); ++q) {
         RowRed R;
-        R.pbase = G.off_part + q;
-        R.out = e->h_row_out[G.row0 + q];
+        R.pbase = G.off_part + (long long)q;
+        R.out = e->h_row_out[slot_row[g][q]];
         R.stride = G.nrows;
         R.npr = npr;
         red.push_back(R);
@@ -1988,7 +2085,7 @@ static int build_pair_tables(dbs_engine* e) {
   }
   // longest groups first so the tail of the launch is short
   for (int a = 0; a < 2; ++a)
-    for (int l = 0; l < 3; ++l)
+    for (int l = 0; l < 2; ++l)
       std::stable_sort(work[a][l].begin(), work[a][l].end(), [&](const PairWork& x, const PairWork& y) {
         return e->h_kg[x.g].nrows > e->h_kg[y.g].nrows;
       });
@@ -1999,7 +2096,7 @@ static int build_pair_tables(dbs_engine* e) {
   dfree(&e->d_part);
   dfree(&e->d_kg);
   for (int a = 0; a < 2; ++a)
-    for (int l = 0; l < 3; ++l) {
+    for (int l = 0; l < 2; ++l) {
       dfree(&e->d_work[a][l]);
       e->nwork[a][l] = (int)work[a][l].size();
     }
@@ -2010,17 +2107,7 @@ static int build_pair_tables(dbs_engine* e) {
   if (!rc) rc = upload(e->stream, &e->d_red, red);
   if (!rc) rc = upload(e->stream, &e->d_kg, e->h_kg);
   for (int a = 0; a < 2 && !rc; ++a)
-    for (int l = 0; l < 3 && !rc; ++l) rc = upload(e->stream, &e->d_work[a][l], work[a][l]);
-  {
-    std::vector<int2> alen(std::max(e->ndens, 1), make_int2(0, 0));
-    for (int d = 0; d < e->ndens; ++d) {
-      int nl = 0;
-      for (const int2& a : e->h_adj[d]) nl += a.x >= 0 ? 1 : 0;
-      alen[d] = make_int2((int)e->h_adj[d].size(), nl);
-    }
-    dfree(&e->d_alen);
-    if (!rc) rc = upload(e->stream, &e->d_alen, alen);
-  }
+    for (int l = 0; l < 2 && !rc; ++l) rc = upload(e->stream, &e->d_work[a][l], work[a][l]);
   if (!rc && hipMalloc(&e->d_part, sizeof(double) * (size_t)std::max<long long>(poff, 1)) != hipSuccess)
     rc = fail(DBS_ERR_HIP, "pair tables: hipMalloc");
   return rc;
@@ -2046,61 +2133,64 @@ static void launch_rot(dbs_engine* e, int adj, unsigned grid, const PairWork* wk
   if (e->onesided) {
     if (adj)
       k_pair_rot<W, true, true, DPT><<<grid, b, shm, s>>>(n, wk, e->d_kg, e->d_inst, e->d_chunks, e->d_ent,
-                                                          e->d_alen, e->F, e->G, e->d_part);
+                                                          e->F, e->G, e->d_part);
     else
       k_pair_rot<W, true, false, DPT><<<grid, b, shm, s>>>(n, wk, e->d_kg, e->d_inst, e->d_chunks, e->d_ent,
-                                                           e->d_alen, e->F, e->G, e->d_part);
+                                                           e->F, e->G, e->d_part);
   } else {
     if (adj)
       k_pair_rot<W, false, true, DPT><<<grid, b, shm, s>>>(n, wk, e->d_kg, e->d_inst, e->d_chunks, e->d_ent,
-                                                           e->d_alen, e->F, e->G, e->d_part);
+                                                           e->F, e->G, e->d_part);
     else
       k_pair_rot<W, false, false, DPT><<<grid, b, shm, s>>>(n, wk, e->d_kg, e->d_inst, e->d_chunks, e->d_ent,
-                                                            e->d_alen, e->F, e->G, e->d_part);
+                                                            e->F, e->G, e->d_part);
   }
 }
 
 template <int W>
-static void launch_pair_w(dbs_engine* e, int adj, int var, hipStream_t s) {
-  const int n = e->nwork[adj][var];
-  if (n == 0) return;
-  const unsigned grid = (unsigned)std::min(n, e->tune[3] * 4);
-  const PairWork* wk = e->d_work[adj][var];
-  if (var == 2) {
-    if (e->rot_dpt == 1) launch_rot<W, 1>(e, adj, grid, wk, n, s);
-    else if constexpr (W <= 8) launch_rot<W, 2>(e, adj, grid, wk, n, s);
-    return;
-  }
-  const int block = std::min(e->tune[4], pair_block_max<W>());
-  const size_t shm = var == 1 ? e->pair_shm : 0;
-  if (adj) {
-    if (var == 1)
-      k_pair_panel<W, true, true><<<grid, block, shm, s>>>(n, wk, e->d_kg, e->d_inst, e->d_chunks, e->d_ent,
-                                                           e->d_alen, e->F, e->G, e->d_part,
-                                                           e->onesided);
+static void launch_chunk(dbs_engine* e, int adj, unsigned grid, const PairWork* wk, int n, hipStream_t s) {
+  const int b = e->chunk_block;
+  const size_t shm = e->chunk_shm;
+  if (e->onesided) {
+    if (adj)
+      k_pair_chunk<W, true, true><<<grid, b, shm, s>>>(n, wk, e->d_kg, e->d_inst, e->d_chunks, e->d_ent,
+                                                       e->F, e->G, e->d_part);
     else
-      k_pair_panel<W, false, true><<<grid, block, 0, s>>>(n, wk, e->d_kg, e->d_inst, e->d_chunks, e->d_ent,
-                                                          e->d_alen, e->F, e->G, e->d_part,
-                                                          e->onesided);
+      k_pair_chunk<W, true, false><<<grid, b, shm, s>>>(n, wk, e->d_kg, e->d_inst, e->d_chunks, e->d_ent,
+                                                        e->F, e->G, e->d_part);
   } else {
-    if (var == 1)
-      k_pair_panel<W, true, false><<<grid, block, shm, s>>>(n, wk, e->d_kg, e->d_inst, e->d_chunks, e->d_ent,
-                                                            e->d_alen, e->F, e->G, e->d_part,
-                                                            e->onesided);
+    if (adj)
+      k_pair_chunk<W, false, true><<<grid, b, shm, s>>>(n, wk, e->d_kg, e->d_inst, e->d_chunks, e->d_ent,
+                                                        e->F, e->G, e->d_part);
     else
-      k_pair_panel<W, false, false><<<grid, block, 0, s>>>(n, wk, e->d_kg, e->d_inst, e->d_chunks, e->d_ent,
-                                                           e->d_alen, e->F, e->G, e->d_part,
-                                                           e->onesided);
+      k_pair_chunk<W, false, false><<<grid, b, shm, s>>>(n, wk, e->d_kg, e->d_inst, e->d_chunks, e->d_ent,
+                                                         e->F, e->G, e->d_part);
   }
 }
 
 static void launch_pair(dbs_engine* e, int adj, hipStream_t s) {
-  for (int var = 2; var >= 0; --var) {
+  const int cap = e->tune[3] * 4;
+  const int nr = e->nwork[adj][0], nc = e->nwork[adj][1];
+  if (nr > 0) {
+    const unsigned grid = (unsigned)std::min(nr, cap);
+    const PairWork* wk = e->d_work[adj][0];
     switch (e->pair_w) {
-      case 4: launch_pair_w<4>(e, adj, var, s); break;
-      case 8: launch_pair_w<8>(e, adj, var, s); break;
-      default: launch_pair_w<16>(e, adj, var, s); break;
+      case 4:
+        if (e->rot_dpt == 1) launch_rot<4, 1>(e, adj, grid, wk, nr, s);
+        else launch_rot<4, 2>(e, adj, grid, wk, nr, s);
+        break;
+      case 8:
+        if (e->rot_dpt == 1) launch_rot<8, 1>(e, adj, grid, wk, nr, s);
+        else launch_rot<8, 2>(e, adj, grid, wk, nr, s);
+        break;
+      default: launch_rot<16, 1>(e, adj, grid, wk, nr, s); break;
     }
+  }
+  if (nc > 0) {
+    const unsigned grid = (unsigned)std::min(nc, cap);
+    const PairWork* wk = e->d_work[adj][1];
+    if (e->chunk_w == 8) launch_chunk<8>(e, adj, grid, wk, nc, s);
+    else launch_chunk<16>(e, adj, grid, wk, nc, s);
   }
 }
 
@@ -2135,7 +2225,7 @@ static int run_forward(dbs_engine* e, int with_adj) {
                                                                e->d_inst, e->F, 0);
   mark(e, "prefix_scan");
   const int wa = with_adj ? 1 : 0;
-  if (e->nwork[wa][0] + e->nwork[wa][1] + e->nwork[wa][2] > 0) {
+  if (e->nwork[wa][0] + e->nwork[wa][1] > 0) {
     launch_pair(e, with_adj ? 1 : 0, s);
     mark(e, with_adj ? "pair_panel" : "contract_rows");
   }
@@ -2764,9 +2854,8 @@ int dbs_engine_destroy(dbs_engine* e) {
                   e->d_seg_w, e->d_roff, e->d_items, e->d_scan_items, e->d_oadjoff, e->d_oadj,
                   e->d_tiles, e->d_pairrow, e->d_pairoff, e->d_nextrow, e->plain_d, e->plain_c,
                   e->d_echunks, e->d_echoff, e->d_epart, e->d_seg_adj, e->d_chunks, e->d_ent,
-                  e->d_ecidx, e->d_red, e->d_part, e->d_alen, e->d_fitems, e->d_pidx, e->d_dpk, e->d_accf, e->d_seg_ivc, e->d_ppoff, e->d_ppart, e->d_edone,
-                  e->d_work[0][0], e->d_work[0][1], e->d_work[0][2],
-                  e->d_work[1][0], e->d_work[1][1], e->d_work[1][2]};
+                  e->d_ecidx, e->d_red, e->d_part, e->d_fitems, e->d_pidx, e->d_dpk, e->d_accf, e->d_seg_ivc, e->d_ppoff, e->d_ppart, e->d_edone,
+                  e->d_work[0][0], e->d_work[0][1], e->d_work[1][0], e->d_work[1][1]};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   if (e->stream) (void)hipStreamDestroy(e->stream);

@@ -341,8 +341,8 @@ def test_pair_panel_shapes(setups, kind):
 @pytest.mark.parametrize("norb,width,fblock", [(24, 16, 256), (24, 8, 64), (40, 8, 256)])
 def test_group_panel_variants(setups, norb, width, fblock):
     """one multipole with 300 (24 orbitals) or 820 (40 orbitals) total
-    densities: the LDS pair panels in their padded form (more densities
-    than one per thread) or the conflict-free rotated form; the features
+    densities: the pair panels of the rotated kernel (the group's panel in
+    LDS) or of the chunked kernel (partners staged chunk by chunk); the features
     by the matrix-core Gram kernel or (64-thread blocks, more than four
     densities per thread) the per-density kernel; 40 rows carry
     coefficients and are checked with their gradient against the oracle"""
@@ -370,6 +370,46 @@ def test_group_panel_variants(setups, norb, width, fblock):
     assert np.max(relerr(eng.values(), v1)) < 1e-13
 
 
+@pytest.mark.parametrize("kind", [0, 5])
+def test_chunked_panels_match_rotated(setups, kind):
+    """one multipole with 1830 densities (60 orbitals): at panel width 4 the
+    group's panel fits in LDS (rotated kernel, two densities per thread);
+    at widths 8 and 16 its partners are staged in LDS chunk by chunk
+    (chunked kernel; owner ranges of 256 or 512 densities, one to three
+    panels per workgroup).  Values, energy and gradient agree to 1e-13
+    across the launch shapes; 30 rows are checked against the oracle."""
+    from dbsr3_amd import Engine
+    g, d = setups["small"]
+    norb = 60
+    fs = random_orbitals(g, norb, seed=600 + kind)
+    rng = np.random.default_rng(600 + kind)
+    rows = make_rows(rng, norb, 0, 30000)
+    rows[:, 0] = 2
+    coef = rng.uniform(-1, 1, len(rows))
+    print(f"[chunked panels kind={kind} rows={len(rows)}]", flush=True)
+    eng = Engine([(d, norb, rows)], kind=kind)
+    eng.set_orbitals(0, pq_of(fs))
+    eng.set_coef(0, coef)
+    eng.set_tuning([0, 256, 4, 4096, 512, 1])
+    eng.forward_adjoint()
+    v0, g0, e0 = eng.values(), eng.grad(), eng.energy()
+    sel = rng.choice(len(rows), 30, replace=False)
+    ref = oracle_values(g, fs, rows[sel], kind)
+    assert relerr(v0[sel], ref).max() < TOL
+    gs = np.max(np.abs(g0))
+    assert gs > 0
+    for t in ([0, 256, 8, 4096, 512, 1], [0, 256, 16, 4096, 512, 1], [0, 256, 16, 4096, 256, 3],
+              [0, 256, 8, 64, 256, 2]):
+        print(f"  tuning {t}", flush=True)
+        eng.set_tuning(t)
+        eng.forward_adjoint()
+        assert np.max(relerr(eng.values(), v0)) < 1e-13, t
+        assert np.max(np.abs(eng.grad() - g0)) <= 1e-13 * gs, t
+        assert abs(eng.energy() - e0) <= 1e-13 * np.sum(np.abs(coef * v0)), t
+        eng.forward()
+        assert np.max(relerr(eng.values(), v0)) < 1e-13, t
+
+
 @pytest.mark.parametrize("kind", [0, 3])
 def test_features_beyond_64_orbitals(setups, kind):
     """more than 64 orbitals: the features come from the per-density kernel
@@ -395,9 +435,9 @@ def test_features_beyond_64_orbitals(setups, kind):
 
 def test_large_group_global_panels(setups):
     """one multipole with ~2080 total densities (64 orbitals): the panel of
-    the group exceeds LDS and the pair panels read partners from global
-    memory.  Only 40 rows carry a coefficient, so the gradient is checked
-    against the oracle's potentials of those rows."""
+    the group exceeds LDS and the pair panels stage its partners in LDS
+    chunk by chunk.  Only 40 rows carry a coefficient, so the gradient is
+    checked against the oracle's potentials of those rows."""
     from dbsr3_amd import Engine
     g, d = setups["small"]
     norb = 64
