@@ -1769,7 +1769,8 @@ struct dbs_engine {
   std::vector<std::vector<int>> group_rows;
   // tuning: [0] contraction path (0 pair panels, 1 dense MFMA, 2 auto),
   //         [1] features block size, [2] pair-panel width W (4, 8, 16),
-  //         [3] persistent grid cap (workgroups), [4] pair-panel block size,
+  //         [3] persistent grid cap (workgroups), [4] owner densities per
+  //         workgroup of the chunked pair-panel kernel (its block size),
   //         [5] W-panels per pair-panel workgroup
   int tune[6] = {2, 256, 16, 4096, 512, 1};
   std::vector<char> group_dense;  // per group: use the dense path
@@ -2925,11 +2926,12 @@ int dbs_engine_set_tuning(dbs_engine* e, int n, const int* p) {
   DBS_CHECK(t[1] >= 64 && t[1] <= 256 && t[1] % 64 == 0, DBS_ERR_ARG, "tuning[1]: block size 64..256");
   DBS_CHECK(t[2] == 4 || t[2] == 8 || t[2] == 16, DBS_ERR_ARG, "tuning[2]: panel width 4, 8 or 16");
   DBS_CHECK(t[3] >= 64 && t[3] <= 65536, DBS_ERR_ARG, "tuning[3]: grid cap");
-  DBS_CHECK(t[4] >= 64 && t[4] <= 512 && t[4] % 64 == 0, DBS_ERR_ARG, "tuning[4]: block size 64..512");
+  DBS_CHECK(t[4] >= 64 && t[4] <= 512 && t[4] % 64 == 0, DBS_ERR_ARG,
+            "tuning[4]: owner densities per chunked pair-panel workgroup, 64..512");
   DBS_CHECK(t[5] >= 1 && t[5] <= 256, DBS_ERR_ARG, "tuning[5]: panels per workgroup 1..256");
   DBS_HIP(hipStreamSynchronize(e->stream));
   const bool path_changed = t[0] != e->tune[0];
-  const bool pair_changed = path_changed || t[2] != e->tune[2] || t[5] != e->tune[5];
+  const bool pair_changed = path_changed || t[2] != e->tune[2] || t[4] != e->tune[4] || t[5] != e->tune[5];
   std::memcpy(e->tune, t, sizeof(t));
   if (path_changed) {
     for (int g = 0; g < e->nkg; ++g) e->group_dense[g] = (char)choose_dense(e, e->h_kg[g], e->group_rows[g].size());
