@@ -45,6 +45,14 @@ class Mat3AuxC(C.Structure):
 auxp = C.POINTER(Mat3AuxC)
 
 
+class NucleusC(C.Structure):
+    """dbs_nucleus: model 0 point, 1 uniform, 2 Fermi; z; r_uniform, a_fermi, c_fermi (a.u.)"""
+    _fields_ = [("model", i32), ("z", f64), ("r_uniform", f64), ("a_fermi", f64), ("c_fermi", f64)]
+
+
+nucp = C.POINTER(NucleusC)
+
+
 class QpqTableC(C.Structure):
     """struct dbs_qpq_table (tabulated QUADR_PQ)"""
     _fields_ = [("norb", i32), ("mmax", i32), ("q", dp)]
@@ -97,6 +105,8 @@ PROTOTYPES = {
                             i32, vp],
     "dbs_rdata_34_dev": [vp, i32, i32, i32, i32, dp, ip, ip, ip, i32, vp, i32, ip, i32, vp, i32],
     "dbs_bspline_overlaps_dev": [vp, vp, vp, vp],
+    "dbs_vnucl_dev": [vp, nucp, vp],
+    "dbs_dhl_pq_dev": [vp, nucp, i32, i32, f64, f64, vp],
     "dbs_full_mat_sym_dev": [vp, vp, ch, vp],
     "dbs_update_hx_dev": [vp, mbp, i32, i32, vp, f64],
     "dbs_update_hw_dev": [vp, mbp, i32, i32, vp, vp, f64],

@@ -3,6 +3,11 @@
 //   ZINTYM overlaps fpbs/fqbs and the block matrix fppqq
 //     (DBSR_HF/dbsr_lib_dbs.f90:4472-4497; Def_pbsp/Def_qbsp :1026, :1072;
 //      Def_pq_arrays :1078-1102)
+//   the Dirac one-electron operator with a finite nucleus: V_nuclear and
+//     SKFUN (DBSR_HF/dbsr_lib_dbs.f90:220-310), Def_Vnucl (:2756-2780),
+//     the derivative tables of Def_pbsp/Def_qbsp (:987-1074, zbsplvd with
+//     two derivatives :4197-4351), MAT_dhl_pq with the Bloch term
+//     (:1164-1206); exp() in the Fermi potential is the device library's
 //   Full_mat_sym (DBSR_HF/dbsr_lib_zcom.f90:1247-1294)
 //   UPDATE_HX / HW / HV / HB / ACF (DBSR_MAT3/updates.f90:79-106, 183-292)
 //   O_data (DBSR_MAT3/O_data.f90:2-93) and the CI overlap SM(i,j) += CC
@@ -15,6 +20,7 @@
 #include <string>
 #include <vector>
 
+#include "dbs_device.h"
 #include "dbs_internal.h"
 
 
@@ -397,6 +403,177 @@ int apply_mat3(dbs_ctx* ctx, const dbs_mat_blocks* m, const std::vector<MatUpd>&
   return DBS_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Dirac one-electron operator
+// ---------------------------------------------------------------------------
+// the constants of module zconst (DBSR_HF/dbsr_lib_zcom.f90:72-98)
+constexpr double kCau = 137.03599976;
+constexpr double kPi = 3.141592653589793238462643;
+
+// S_k(x) = sum_{n>=1} (-1)^n e^{nx} / n^k, summed until the relative term
+// falls below 10 eps (SKFUN)
+__device__ inline double skfun(int k, double x) {
+#pragma clang fp contract(off)
+  if (x < -500.0) return 0.0;
+  const double eps = 10.0 * 2.220446049250313e-16;
+  const double base = -exp(x);
+  double dnum = base, s = base;
+  int n = 1;
+  for (;;) {
+    ++n;
+    dnum = dnum * base;
+    const double dk = ipow((double)n, k);
+    const double delta = dnum / dk;
+    s = s + delta;
+    if (fabs(delta / s) <= eps) break;
+  }
+  return s;
+}
+
+// nuclear potential at r: point, uniform sphere or Fermi distribution
+// (Parpia and Mohanty, Phys Rev A46, 3735 (1992)), as V_nuclear
+__device__ inline double v_nuclear(const dbs_nucleus& nu, double r) {
+#pragma clang fp contract(off)
+  const double Z = nu.z;
+  if (r <= 0.0) return 0.0;
+  if (nu.model == DBS_NUCLEUS_POINT) return -Z / r;
+  if (nu.model == DBS_NUCLEUS_UNIFORM) {
+    const double b = nu.r_uniform;
+    if (r > b) return -Z / r;
+    return -(3.0 * Z) / (2.0 * b) * (1.0 - r * r / (3.0 * b * b));
+  }
+  const double a = nu.a_fermi, c = nu.c_fermi;
+  const double ac = a / c, ac2 = ac * ac, ac3 = ac2 * ac;
+  const double p = kPi * kPi * ac2, s = 6.0 * ac3 * skfun(3, -c / a);
+  const double N = 1.0 + p - s;
+  const double rc = r / c;
+  double V;
+  if (r <= c) {
+    const double ra = (r - c) / a;
+    const double s2 = ac2 * skfun(2, ra), s3 = ac3 * skfun(3, ra);
+    V = (3.0 - rc * rc + p) / 2.0 - 3.0 * s2 - s / rc + 6.0 / rc * s3;
+    V = -(Z * V) / (N * c);
+  } else {
+    const double ra = (c - r) / a;
+    const double s2 = ac2 * skfun(2, ra), s3 = ac3 * skfun(3, ra);
+    V = N + 6.0 * s3 + 3.0 * rc * s2;
+    V = -Z / r * (V / N);
+  }
+  return V;
+}
+
+// per outer Gauss point: vr = V_nuclear(gr) * grw (VR_nucl of Def_Vnucl)
+// and ygw = grm * grw (the 1/r weights of Def_pq_arrays)
+__global__ void k_dirac_weights(int nv, int ks, const double* __restrict__ gr, const double* __restrict__ grw,
+                                const double* __restrict__ grm, dbs_nucleus nu, double* __restrict__ vr,
+                                double* __restrict__ ygw) {
+#pragma clang fp contract(off)
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nv * ks) return;
+  vr[q] = v_nuclear(nu, gr[q]) * grw[q];
+  ygw[q] = grm[q] * grw[q];
+}
+
+// first derivatives of the kx B-splines that do not vanish on
+// [T(left), T(left+1)) at x: bsplvd with two derivatives (order kx-1
+// values, then the B-coefficients w31 of the derivative), tx 0-based
+__device__ inline void bspline_d1(const double* __restrict__ tx, int kx, int left, double x, double* d) {
+#pragma clang fp contract(off)
+  double lo[DBS_MAXK];
+  bspline_values(tx, kx - 1, left, x, lo);
+  double w31[DBS_MAXK + 1][DBS_MAXK + 1];
+  int jlow = 1;
+  for (int i = 1; i <= kx; ++i) {
+    for (int j = jlow; j <= kx; ++j) w31[j][i] = 0.0;
+    jlow = i;
+    w31[i][i] = 1.0;
+  }
+  const int kpimm = kx - 1;
+  const double fkpimm = kpimm;
+  int i = kx, il = 0;
+  for (int ld = 1; ld <= kpimm; ++ld) {
+    const int n = left - il;
+    const double w1 = fkpimm / (tx[n + kpimm - 1] - tx[n - 1]);
+    for (int j = 1; j <= i; ++j) w31[i][j] = (w31[i][j] - w31[i - 1][j]) * w1;
+    ++il;
+    --i;
+  }
+  // DB(j,2) = order kx-1 value j-1 for j = 2..kx
+  for (int ii = 1; ii <= kx; ++ii) {
+    double w2 = 0.0;
+    for (int j = ii > 2 ? ii : 2; j <= kx; ++j) w2 += w31[j][ii] * lo[j - 2];
+    d[ii - 1] = w2;
+  }
+}
+
+// derivative table xbsd(nv+1,ks,kx) in the layout of pbsp/qbsp (row nv,
+// the last knot, left zero: no ZINTYm term reads it)
+__global__ void k_outer_bspline_d1(int nv, int ks, int kx, const double* __restrict__ tx,
+                                   const double* __restrict__ gr, double* __restrict__ xbsd) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nw = nv + 1;
+  if (q >= nw * ks) return;
+  const int iv = q % nw, m = q / nw;
+  double d[DBS_MAXK];
+  if (iv < nv) {
+    bspline_d1(tx, kx, iv + kx, gr[iv + nv * m], d);
+  } else {
+    for (int j = 0; j < kx; ++j) d[j] = 0.0;
+  }
+  for (int j = 0; j < kx; ++j) xbsd[iv + (size_t)nw * (m + (size_t)ks * j)] = d[j];
+}
+
+// dhl(ms,ms): (1:1) <B|V|B>; (2:2) <B|V|B> - 2c^2 <B|B>; (2:1)
+// c<B|d/dr|B> + kc<B|1/r|B>; (1:2) -c<B|d/dr|B> + kc<B|1/r|B>; then the
+// Bloch term at the last P and Q splines
+struct DiracMats {
+  const double *fpbn, *fqbn, *fqbs, *fpqbs, *fqpbs, *fpqbsd, *fqpbsd;
+};
+__global__ void k_dhl_assemble(int ns, int nsp, int nsq, int kappa, DiracMats M, int bloch, double rn, double pnu,
+                               double* __restrict__ dhl) {
+#pragma clang fp contract(off)
+  const int ms = 2 * ns;
+  const long long total = (long long)ms * ms;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+       q += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(q % ms), j = (int)(q / ms);
+    const int ii = i % ns, jj = j % ns;
+    const long long e = ii + (long long)ns * jj;
+    double v;
+    if (i < ns && j < ns) {
+      v = M.fpbn[e];
+    } else if (i >= ns && j >= ns) {
+      const double sb = -2.0 * kCau * kCau;
+      v = M.fqbn[e] + sb * M.fqbs[e];
+    } else if (i >= ns) {
+      const double sa = kCau, sb = (double)kappa * kCau;
+      v = sa * M.fqpbsd[e] + sb * M.fqpbs[e];
+    } else {
+      const double sa = -kCau, sb = (double)kappa * kCau;
+      v = sa * M.fpqbsd[e] + sb * M.fpqbs[e];
+    }
+    if (bloch) {
+      const int bi = nsp - 1, bj = ns + nsq - 1;
+      if (i == bi && j == bi) v = v - pnu * rn * kCau;
+      if (i == bi && j == bj) v = v + pnu * kCau;
+      if (i == bj && j == bi) v = v + (pnu - 1.0) * kCau;
+      if (i == bj && j == bj) v = v - (pnu - 1.0) / rn * kCau;
+    }
+    dhl[q] = v;
+  }
+}
+
+static int check_nucleus(const dbs_nucleus* nu) {
+  DBS_CHECK(nu, DBS_ERR_ARG, "dirac: null nucleus");
+  DBS_CHECK(nu->z > 0.0, DBS_ERR_ARG, "dirac: Z <= 0");
+  DBS_CHECK(nu->model == DBS_NUCLEUS_POINT || nu->model == DBS_NUCLEUS_UNIFORM || nu->model == DBS_NUCLEUS_FERMI,
+            DBS_ERR_ARG, "dirac: unknown charge distribution");
+  if (nu->model == DBS_NUCLEUS_UNIFORM) DBS_CHECK(nu->r_uniform > 0.0, DBS_ERR_ARG, "dirac: r_uniform <= 0");
+  if (nu->model == DBS_NUCLEUS_FERMI)
+    DBS_CHECK(nu->a_fermi > 0.0 && nu->c_fermi > 0.0, DBS_ERR_ARG, "dirac: Fermi a, c must be > 0");
+  return DBS_OK;
+}
+
 }  // namespace dbs
 
 using namespace dbs;
@@ -568,3 +745,63 @@ int dbs_overlap_ci_dev(dbs_ctx* ctx, int n, const int* ic, const int* jc, const 
 }
 
 }  // extern "C"
+
+int dbs_vnucl_dev(dbs_ctx* ctx, const dbs_nucleus* nuc, double* vr_dev) {
+  DBS_CHECK(ctx && vr_dev, DBS_ERR_ARG, "dbs_vnucl_dev: null argument");
+  int rc = check_nucleus(nuc);
+  if (rc) return rc;
+  const int nv = ctx->nv, ks = ctx->ks;
+  rc = ensure_scratch(ctx, (size_t)nv * ks);
+  if (rc) return rc;
+  k_dirac_weights<<<grid_n((long long)nv * ks, 256), 256, 0, ctx->stream>>>(nv, ks, ctx->gr, ctx->grw, ctx->grm, *nuc,
+                                                                        vr_dev, ctx->scratch);
+  DBS_HIP(hipGetLastError());
+  return DBS_OK;
+}
+
+int dbs_dhl_pq_dev(dbs_ctx* ctx, const dbs_nucleus* nuc, int kappa, int bloch, double rb, double pnu,
+                   double* dhl_dev) {
+  DBS_CHECK(ctx && dhl_dev, DBS_ERR_ARG, "dbs_dhl_pq_dev: null argument");
+  DBS_CHECK(kappa != 0, DBS_ERR_ARG, "dbs_dhl_pq_dev: kappa = 0");
+  int rc = check_nucleus(nuc);
+  if (rc) return rc;
+  const int ns = ctx->ns, nv = ctx->nv, ks = ctx->ks, ksp = ctx->ksp, ksq = ctx->ksq;
+  const size_t nk = (size_t)nv * ks, nwk = (size_t)(nv + 1) * ks, nn = (size_t)ns * ns;
+  // scratch: vr, ygw | pbsd, qbsd | 7 matrices
+  rc = ensure_scratch(ctx, 2 * nk + nwk * (ksp + ksq) + 7 * nn);
+  if (rc) return rc;
+  double* vr = ctx->scratch;
+  double* ygw = vr + nk;
+  double* pbsd = ygw + nk;
+  double* qbsd = pbsd + nwk * ksp;
+  double* mats = qbsd + nwk * ksq;
+  double* fpbn = mats;
+  double* fqbn = fpbn + nn;
+  double* fqbs = fqbn + nn;
+  double* fpqbs = fqbs + nn;
+  double* fqpbs = fpqbs + nn;
+  double* fpqbsd = fqpbs + nn;
+  double* fqpbsd = fpqbsd + nn;
+  hipStream_t s = ctx->stream;
+  k_dirac_weights<<<grid_n((long long)nk, 256), 256, 0, s>>>(nv, ks, ctx->gr, ctx->grw, ctx->grm, *nuc, vr, ygw);
+  k_outer_bspline_d1<<<grid_n((long long)nwk, 256), 256, 0, s>>>(nv, ks, ksp, ctx->tp, ctx->gr, pbsd);
+  k_outer_bspline_d1<<<grid_n((long long)nwk, 256), 256, 0, s>>>(nv, ks, ksq, ctx->tq, ctx->gr, qbsd);
+  const unsigned g = grid_n((long long)nn, 256);
+  // Def_Vnucl
+  k_zintym<<<g, 256, 0, s>>>(nv, ks, ksp, ksp, ctx->pbsp, ctx->pbsp, vr, ns, ns, fpbn);
+  k_zintym<<<g, 256, 0, s>>>(nv, ks, ksq, ksq, ctx->qbsp, ctx->qbsp, vr, ns, ns, fqbn);
+  // Def_qbsp's overlap and Def_pq_arrays
+  k_zintym<<<g, 256, 0, s>>>(nv, ks, ksq, ksq, ctx->qbsp, ctx->qbsp, ctx->grw, ns, ns, fqbs);
+  k_zintym<<<g, 256, 0, s>>>(nv, ks, ksp, ksq, ctx->pbsp, ctx->qbsp, ygw, ns, ns, fpqbs);
+  k_zintym<<<g, 256, 0, s>>>(nv, ks, ksq, ksp, ctx->qbsp, ctx->pbsp, ygw, ns, ns, fqpbs);
+  k_zintym<<<g, 256, 0, s>>>(nv, ks, ksp, ksq, ctx->pbsp, qbsd, ctx->grw, ns, ns, fpqbsd);
+  k_zintym<<<g, 256, 0, s>>>(nv, ks, ksq, ksp, ctx->qbsp, pbsd, ctx->grw, ns, ns, fqpbsd);
+  // RN = (RB + k) / (2 tmax c), tmax = t(ns+1)
+  const double tmax = ctx->h_t[ns];
+  const double rn = (rb + (double)kappa) / (2.0 * tmax * kCau);
+  DiracMats M{fpbn, fqbn, fqbs, fpqbs, fqpbs, fpqbsd, fqpbsd};
+  k_dhl_assemble<<<grid_n(4LL * (long long)nn, 256), 256, 0, s>>>(ns, ctx->nsp, ctx->nsq, kappa, M, bloch ? 1 : 0, rn,
+                                                              pnu, dhl_dev);
+  DBS_HIP(hipGetLastError());
+  return DBS_OK;
+}

@@ -201,6 +201,35 @@ class DBS:
         self._post()
         return fp, fq, fpq
 
+    # -- the Dirac operator -------------------------------------------------
+    @staticmethod
+    def nucleus(model, z, r_uniform=0.0, a_fermi=0.0, c_fermi=0.0):
+        """module DBS_nuclear's charge distribution: model 'point',
+        'uniform' or 'Fermi' (or 0, 1, 2), atomic number z, radii in a.u."""
+        from .capi import NucleusC
+        codes = {"point": 0, "uniform": 1, "fermi": 2}
+        m = codes[model.lower()] if isinstance(model, str) else int(model)
+        return NucleusC(m, float(z), float(r_uniform), float(a_fermi), float(c_fermi))
+
+    def vnucl(self, nucleus):
+        """VR_nucl(nv,ks) = V_nuclear(gr)*grw (Def_Vnucl) as a device tensor"""
+        vr = self._empty(self.nv, self.ks)
+        self._pre()
+        check(lib().dbs_vnucl_dev(self.ctx, C.byref(nucleus), _ptr(vr)))
+        self._post()
+        return vr
+
+    def dhl_pq(self, nucleus, kappa, bloch=True, rb=0.0, pnu=1.0):
+        """MAT_dhl_pq(kappa): the Dirac operator dhl(ms,ms) as a device
+        tensor (row-major view of the column-major matrix: dhl[j, i] =
+        dhl(i,j)); bloch/rb/pnu are module DBS_dhl_pq's mbloch, RB, pnu"""
+        d = self._empty(self.ms, self.ms)
+        self._pre()
+        check(lib().dbs_dhl_pq_dev(self.ctx, C.byref(nucleus), int(kappa), 1 if bloch else 0, float(rb),
+                                   float(pnu), _ptr(d)))
+        self._post()
+        return d
+
     def Full_mat_sym(self, x, sym):
         """y(ns,ns) from band/full storage x (device tensor) in mode sym"""
         y = self._empty(self.ns, self.ns)

@@ -213,6 +213,30 @@ int dbs_rdata_34_dev(dbs_ctx *ctx, int itype, int atype, int k, int nc, const do
  * fqbs(ns,ns) = <B^Q_i|B^Q_j>, fppqq(ms,ms) = diag(fpbs, fqbs).  Any output
  * pointer may be NULL. */
 int dbs_bspline_overlaps_dev(dbs_ctx *ctx, double *fpbs_dev, double *fqbs_dev, double *fppqq_dev);
+/* the nucleus of module DBS_nuclear (DBSR_HF/dbsr_lib_dbs.f90:47-98): its
+ * charge distribution, atomic number z and, in atomic units, the radius of
+ * the uniform sphere or the Fermi parameters a, c (the a_fermi, c_fermi of
+ * a knot.dat or .bsw file). */
+#define DBS_NUCLEUS_POINT 0
+#define DBS_NUCLEUS_UNIFORM 1
+#define DBS_NUCLEUS_FERMI 2
+typedef struct dbs_nucleus {
+  int model;
+  double z, r_uniform, a_fermi, c_fermi;
+} dbs_nucleus;
+/* VR_nucl of Def_Vnucl (dbsr_lib_dbs.f90:2756-2780): vr(nv,ks) =
+ * V_nuclear(gr) * grw at the outer Gauss points (V_nuclear and SKFUN
+ * :220-310), device output. */
+int dbs_vnucl_dev(dbs_ctx *ctx, const dbs_nucleus *nuc, double *vr_dev);
+/* MAT_dhl_pq(kappa) (dbsr_lib_dbs.f90:1164-1206): the Dirac operator
+ * dhl(ms,ms) of the grid, column-major on the device: <P|V|P>,
+ * c<P|-d/dr+k/r|Q>, c<Q|d/dr+k/r|P>, <Q|V-2c^2|Q> from Def_Vnucl and the
+ * mixed arrays of Def_pq_arrays (:1078-1102).  bloch = 1 adds the Bloch
+ * boundary term with parameters rb, pnu (module DBS_dhl_pq: mbloch = 1,
+ * RB = 0, pnu = 1 by default).  <v1|dhl|v2> (Vp_dhl, :1210-1227) is then a
+ * matrix product on the caller's side. */
+int dbs_dhl_pq_dev(dbs_ctx *ctx, const dbs_nucleus *nuc, int kappa, int bloch, double rb, double pnu,
+                   double *dhl_dev);
 /* Full_mat_sym (DBSR_HF/dbsr_lib_zcom.f90:1247-1294): y(ns,ns)
  * from x in storage sym 's' (upper band x(ns,ks)), 'l' (lower band), 'n'
  * (non-symmetric band x(ns,2ks-1)) or 'x' (full). */

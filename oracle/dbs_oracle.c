@@ -17,6 +17,7 @@
  */
 #include "dbs_oracle.h"
 
+#include <float.h>
 #include <math.h>
 #include <omp.h>
 #include <stdio.h>
@@ -2020,4 +2021,129 @@ int orc_rk_rows_par(const orc_grid *g, int n, const int *rows, const double *pq,
   }
   free(part);
   return done;
+}
+
+/* ===================================================================== */
+/* Dirac one-electron operator with a finite nucleus                     */
+/* V_nuclear / SKFUN: DBSR_HF/dbsr_lib_dbs.f90:220-310                   */
+/* Def_Vnucl :2756-2780; derivative tables of Def_pbsp/Def_qbsp          */
+/* :987-1074 (zbsplvd with two derivatives); Def_pq_arrays :1078-1102;   */
+/* MAT_dhl_pq :1164-1206 (Bloch term with RB, pnu); constants of zconst  */
+/* (DBSR_HF/dbsr_lib_zcom.f90:72-98).                                    */
+/* ===================================================================== */
+#define ORC_CAU 137.03599976
+#define ORC_PI 3.141592653589793238462643
+
+double orc_skfun(int k, double x) {
+  if (x < -500.0) return 0.0;
+  const double eps = 10.0 * DBL_EPSILON;
+  const double base = -exp(x);
+  double dnum = base, s = base;
+  int n = 1;
+  for (;;) {
+    ++n;
+    dnum = dnum * base;
+    const double dk = ipow((double)n, k);
+    const double delta = dnum / dk;
+    s = s + delta;
+    if (fabs(delta / s) <= eps) break;
+  }
+  return s;
+}
+
+/* model 0 point, 1 uniform (radius b), 2 Fermi (a, c) */
+double orc_vnuclear(int model, double z, double b, double a, double c, double r) {
+  if (r <= 0.0) return 0.0;
+  if (model == 0) return -z / r;
+  if (model == 1) {
+    if (r > b) return -z / r;
+    return -(3.0 * z) / (2.0 * b) * (1.0 - r * r / (3.0 * b * b));
+  }
+  const double ac = a / c, ac2 = ac * ac, ac3 = ac2 * ac;
+  const double p = ORC_PI * ORC_PI * ac2, s = 6.0 * ac3 * orc_skfun(3, -c / a);
+  const double N = 1.0 + p - s, rc = r / c;
+  double V;
+  if (r <= c) {
+    const double ra = (r - c) / a;
+    const double s2 = ac2 * orc_skfun(2, ra), s3 = ac3 * orc_skfun(3, ra);
+    V = (3.0 - rc * rc + p) / 2.0 - 3.0 * s2 - s / rc + 6.0 / rc * s3;
+    V = -(z * V) / (N * c);
+  } else {
+    const double ra = (c - r) / a;
+    const double s2 = ac2 * orc_skfun(2, ra), s3 = ac3 * orc_skfun(3, ra);
+    V = N + 6.0 * s3 + 3.0 * rc * s2;
+    V = -z / r * (V / N);
+  }
+  return V;
+}
+
+/* first-derivative table xbsd(nv+1,ks,kx) at the outer Gauss points */
+static void def_xbsd(const orc_grid *g, int kx, int nsx, const double *tx, double *xbsd) {
+  const int nv = g->nv, ks = g->ks, nw = nv + 1;
+  double *dbi = (double *)calloc((size_t)nv * kx * kx, sizeof(double));
+  double *xm = (double *)malloc(sizeof(double) * nv);
+  for (size_t q = 0; q < (size_t)nw * ks * kx; ++q) xbsd[q] = 0.0;
+  for (int m = 0; m < ks; ++m) {
+    for (int iv = 0; iv < nv; ++iv) xm[iv] = g->gr[iv + (size_t)nv * m];
+    orc_zbsplvd(nsx, kx, nv, tx, kx, nv, xm, 2, dbi);
+    for (int j = 0; j < kx; ++j)
+      for (int iv = 0; iv < nv; ++iv)
+        xbsd[iv + (size_t)nw * (m + (size_t)ks * j)] = dbi[iv + (size_t)nv * (j + (size_t)kx * 1)];
+  }
+  free(dbi);
+  free(xm);
+}
+
+void orc_vnucl(const orc_grid *g, int model, double z, double b, double a, double c, double *vr) {
+  const int nv = g->nv, ks = g->ks;
+  for (int j = 0; j < ks; ++j)
+    for (int i = 0; i < nv; ++i) {
+      const size_t q = i + (size_t)nv * j;
+      vr[q] = orc_vnuclear(model, z, b, a, c, g->gr[q]) * g->grw[q];
+    }
+}
+
+void orc_dhl_pq(const orc_grid *g, int model, double z, double b, double a, double c, int kappa, int bloch,
+                double rb, double pnu, double *dhl) {
+  const int ns = g->ns, ms = g->ms, nv = g->nv, ks = g->ks, ksp = g->ksp, ksq = g->ksq;
+  const size_t nn = (size_t)ns * ns, nk = (size_t)nv * ks, nwk = (size_t)(nv + 1) * ks;
+  double *vr = (double *)malloc(sizeof(double) * 2 * nk), *ygw = vr + nk;
+  double *pbsd = (double *)malloc(sizeof(double) * nwk * (ksp + ksq)), *qbsd = pbsd + nwk * ksp;
+  double *m7 = (double *)malloc(sizeof(double) * 7 * nn);
+  double *fpbn = m7, *fqbn = m7 + nn, *fqbs = m7 + 2 * nn, *fpqbs = m7 + 3 * nn, *fqpbs = m7 + 4 * nn,
+         *fpqbsd = m7 + 5 * nn, *fqpbsd = m7 + 6 * nn;
+  orc_vnucl(g, model, z, b, a, c, vr);
+  for (size_t q = 0; q < nk; ++q) ygw[q] = g->grm[q] * g->grw[q];
+  def_xbsd(g, ksp, g->nsp, g->tp, pbsd);
+  def_xbsd(g, ksq, g->nsq, g->tq, qbsd);
+  orc_zintym(nv, ks, ksp, ksp, g->pbsp, g->pbsp, vr, ns, fpbn);
+  orc_zintym(nv, ks, ksq, ksq, g->qbsp, g->qbsp, vr, ns, fqbn);
+  orc_zintym(nv, ks, ksq, ksq, g->qbsp, g->qbsp, g->grw, ns, fqbs);
+  orc_zintym(nv, ks, ksp, ksq, g->pbsp, g->qbsp, ygw, ns, fpqbs);
+  orc_zintym(nv, ks, ksq, ksp, g->qbsp, g->pbsp, ygw, ns, fqpbs);
+  orc_zintym(nv, ks, ksp, ksq, g->pbsp, qbsd, g->grw, ns, fpqbsd);
+  orc_zintym(nv, ks, ksq, ksp, g->qbsp, pbsd, g->grw, ns, fqpbsd);
+#define DH(i, j) dhl[(i) + (size_t)ms * (j)]
+  for (int j = 0; j < ns; ++j)
+    for (int i = 0; i < ns; ++i) {
+      const size_t e = i + (size_t)ns * j;
+      DH(i, j) = fpbn[e];
+      const double sq = -2.0 * ORC_CAU * ORC_CAU;
+      DH(ns + i, ns + j) = fqbn[e] + sq * fqbs[e];
+      DH(ns + i, j) = ORC_CAU * fqpbsd[e] + ((double)kappa * ORC_CAU) * fqpbs[e];
+      DH(i, ns + j) = (-ORC_CAU) * fpqbsd[e] + ((double)kappa * ORC_CAU) * fpqbs[e];
+    }
+  if (bloch) {
+    const double tmax = g->t[ns];
+    const double rn = (rb + kappa) / (2.0 * tmax * ORC_CAU);
+    const int i = g->nsp - 1, j = ns + g->nsq - 1;
+    DH(i, i) = DH(i, i) - pnu * rn * ORC_CAU;
+    DH(i, j) = DH(i, j) + pnu * ORC_CAU;
+    DH(j, i) = DH(j, i) + (pnu - 1.0) * ORC_CAU;
+    DH(j, j) = DH(j, j) - (pnu - 1.0) / rn * ORC_CAU;
+  }
+#undef DH
+  free(vr);
+  free(pbsd);
+  free(m7);
 }

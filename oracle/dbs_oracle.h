@@ -109,6 +109,14 @@ void orc_full_mat_sym(int ns, int ks, const double *x, double *y, char sym);
 void orc_zintym(int nv, int ks, int ks1, int ks2, const double *b1, const double *b2,
                 const double *ygr, int mm, double *ym);
 void orc_overlaps(const orc_grid *g, double *fpbs, double *fqbs, double *fppqq);
+/* Dirac operator with a finite nucleus (model 0 point, 1 uniform sphere of
+ * radius b, 2 Fermi a, c): V_nuclear, Def_Vnucl's VR_nucl(nv,ks),
+ * MAT_dhl_pq(kappa) into dhl(ms,ms) */
+double orc_skfun(int k, double x);
+double orc_vnuclear(int model, double z, double b, double a, double c, double r);
+void orc_vnucl(const orc_grid *g, int model, double z, double b, double a, double c, double *vr);
+void orc_dhl_pq(const orc_grid *g, int model, double z, double b, double a, double c, int kappa, int bloch,
+                double rb, double pnu, double *dhl);
 void orc_update_hx(int ms, double *blk, int ich, int jch, const double *d, double C);
 void orc_update_hw(int ms, double *blk, int ich, int jch, const double *v, const double *w, double C);
 void orc_update_hv(int ms, double *col, const double *v, double C);

@@ -663,3 +663,203 @@ def rk_rows(g, fs, rows, kind=0, deadline=None):
         if deadline is not None and time.perf_counter() > deadline:
             return out[:n + 1]
     return out
+
+
+# --------------------------------------------------------------------------
+# DBSR_HF's average-of-configuration energy (term = 'LS'/'AV'), restated:
+# the Dirac operator with a finite nucleus (orc_dhl_pq: V_nuclear,
+# Def_Vnucl, Def_pq_arrays, MAT_dhl_pq, DBSR_HF/dbsr_lib_dbs.f90:220-310,
+# 1078-1206, 2756-2780) and Vp_dhl (:1210-1227); hf_fk / hf_gk
+# (DBSR_HF/hf_rk.f90:43-113) with the reference's densities, convolutions
+# and SUM_AmB in its order (hf_gk's cross term as 2 qpqp); av_energy_coef,
+# WW, a, b, update_int, energy (DBSR_HF/hf_MOD_energy.f90:19-255);
+# Conf_energies (DBSR_HF/hf_summry.f90:95-134).  The 3j symbol is Racah's
+# formula in floating point (the closed forms of the tests pin it).
+# --------------------------------------------------------------------------
+def vnuclear(model, z, b, a, c, r):
+    L = lib()
+    L.orc_vnuclear.restype = C.c_double
+    L.orc_vnuclear.argtypes = [C.c_int] + [C.c_double] * 5
+    return L.orc_vnuclear(model, z, b, a, c, r)
+
+
+def vnucl(g, model, z, b, a, c):
+    L = lib()
+    L.orc_vnucl.argtypes = [C.c_void_p, C.c_int] + [C.c_double] * 4 + [_dp]
+    vr = np.zeros((g.nv, g.ks), order="F")
+    L.orc_vnucl(g.h, model, z, b, a, c, _d(vr))
+    return vr
+
+
+def dhl_pq(g, model, z, b, a, c, kappa, bloch=1, rb=0.0, pnu=1.0):
+    L = lib()
+    L.orc_dhl_pq.argtypes = [C.c_void_p, C.c_int] + [C.c_double] * 4 + [C.c_int, C.c_int, C.c_double,
+                                                                        C.c_double, _dp]
+    d = np.zeros((g.ms, g.ms), order="F")
+    L.orc_dhl_pq(g.h, model, z, b, a, c, kappa, bloch, rb, pnu, _d(d))
+    return d
+
+
+def threej_sq(j1, j2, j3, m1, m2, m3):
+    """(j1/2 j2/2 j3/2; m1/2 m2/2 m3/2)^2, doubled arguments, floating point"""
+    from math import factorial as fa
+    if m1 + m2 + m3 != 0 or j3 > j1 + j2 or j3 < abs(j1 - j2) or (j1 + j2 + j3) % 2:
+        return 0.0
+    if any(abs(m) > j or (j + m) % 2 for j, m in ((j1, m1), (j2, m2), (j3, m3))):
+        return 0.0
+    a, b, c = (j1 + j2 - j3) // 2, (j1 - j2 + j3) // 2, (-j1 + j2 + j3) // 2
+    tri = fa(a) * fa(b) * fa(c) / fa((j1 + j2 + j3) // 2 + 1)
+    mm = 1.0
+    for j, m in ((j1, m1), (j2, m2), (j3, m3)):
+        mm *= fa((j + m) // 2) * fa((j - m) // 2)
+    s = 0.0
+    for t in range(0, a + 1):
+        args = (t, (j3 - j2 + m1) // 2 + t, (j3 - j1 - m2) // 2 + t, a - t, (j1 - m1) // 2 - t, (j2 + m2) // 2 - t)
+        if min(args) < 0:
+            continue
+        den = 1.0
+        for x in args:
+            den *= fa(x)
+        s += (-1) ** t / den
+    return tri * mm * s * s
+
+
+def cjkj_sq(jj1, k, jj2):
+    return threej_sq(jj1, 2 * k, jj2, 1, 0, -1) * (jj1 + 1) * (jj2 + 1)
+
+
+def av_energy_coef(jbs, iqconf, weight):
+    n, kmax = len(jbs), max(jbs)
+    iq = np.asarray(iqconf, dtype=float)
+    w = np.asarray(weight, dtype=float)
+    coef = np.zeros((n, n, kmax + 1))
+    for i in range(n):
+        for j in range(i + 1):
+            if i == j:
+                c = float(np.sum(iq[:, i] * (iq[:, i] - 1) * w)) / 2.0
+                coef[i, i, 0] = c
+                for k in range(1, jbs[i] + 1):
+                    coef[i, i, k] = -c * cjkj_sq(jbs[i], k, jbs[i]) / (jbs[i] * (jbs[i] + 1))
+            else:
+                c = float(np.sum(iq[:, i] * iq[:, j] * w))
+                coef[i, j, 0] = c
+                for k in range(abs(jbs[i] - jbs[j]) // 2, (jbs[i] + jbs[j]) // 2 + 1):
+                    coef[j, i, k] = -c * cjkj_sq(jbs[i], k, jbs[j]) / ((jbs[i] + 1) * (jbs[j] + 1))
+    return coef
+
+
+def hf_fk(g, fs, i, j, k):
+    ns, ks = g.ns, g.ks
+    ppi, ppj = density(ns, ks, fs[i][:, 0], fs[i][:, 0], "s"), density(ns, ks, fs[j][:, 0], fs[j][:, 0], "s")
+    qqi, qqj = density(ns, ks, fs[i][:, 1], fs[i][:, 1], "s"), density(ns, ks, fs[j][:, 1], fs[j][:, 1], "s")
+    s = 0.0
+    s += sum_amb(ns, ks, convol(ns, ks, g.mrk(PPPP, k), ppi, 2, "s", "s"), ppj, "s")
+    s += sum_amb(ns, ks, convol(ns, ks, g.mrk(QQQQ, k), qqi, 2, "s", "s"), qqj, "s")
+    s += sum_amb(ns, ks, convol(ns, ks, g.mrk(QPQP, k), qqi, 2, "s", "s"), ppj, "s")
+    s += sum_amb(ns, ks, convol(ns, ks, g.mrk(PQPQ, k), ppi, 2, "s", "s"), qqj, "s")
+    return s
+
+
+def hf_gk(g, fs, i, j, k):
+    ns, ks = g.ns, g.ks
+    pp = density(ns, ks, fs[i][:, 0], fs[j][:, 0], "s")
+    qq = density(ns, ks, fs[i][:, 1], fs[j][:, 1], "s")
+    s = 0.0
+    s += sum_amb(ns, ks, convol(ns, ks, g.mrk(PPPP, k), pp, 2, "s", "s"), pp, "s")
+    s += sum_amb(ns, ks, convol(ns, ks, g.mrk(QQQQ, k), qq, 2, "s", "s"), qq, "s")
+    s += 2.0 * sum_amb(ns, ks, convol(ns, ks, g.mrk(QPQP, k), qq, 2, "s", "s"), pp, "s")
+    return s
+
+
+def av_energy(g, pq, kbs, lbs, jbs, nucleus, iqconf, weight):
+    """(Etotal, E1body, E2body, [configuration energies]) of the orbitals
+    pq(ns,2,nbf); nucleus = (model, z, r_uniform, a_fermi, c_fermi)"""
+    n = len(kbs)
+    fs = [np.asfortranarray(pq[:, :, i]) for i in range(n)]
+    valueI = np.zeros(n)
+    dhl = {}
+    for i in range(n):
+        if kbs[i] not in dhl:
+            dhl[kbs[i]] = dhl_pq(g, *nucleus, kbs[i])
+        v = np.concatenate([fs[i][:, 0], fs[i][:, 1]])
+        valueI[i] = float(np.sum(v * (dhl[kbs[i]] @ v)))
+    F = {(i, j, k): hf_fk(g, fs, i, j, k) for i in range(n) for j in range(i + 1)
+         for k in range(0, 2 * min(lbs[i], lbs[j]) + 1, 2)}
+    G = {(i, j, k): hf_gk(g, fs, i, j, k) for i in range(n) for j in range(i)
+         for k in range(abs(lbs[i] - lbs[j]), lbs[i] + lbs[j] + 1, 2)}
+
+    def energy(iq, w):
+        w = np.asarray(w, dtype=float) / np.sum(w)
+        qsum = np.asarray(iq, dtype=float).T @ w
+        coef = av_energy_coef(jbs, iq, w)
+        et = e1 = e2 = 0.0
+        for i in range(n):
+            et += qsum[i] * valueI[i]
+            e1 += qsum[i] * valueI[i]
+            for j in range(i + 1):
+                for k in range(0, 2 * min(lbs[i], lbs[j]) + 1, 2):
+                    if k % 2 or k > (jbs[i] + jbs[j]) // 2:
+                        continue
+                    c = coef[i, j, k]
+                    if c != 0.0:
+                        et += c * F[(i, j, k)]
+                        e2 += c * F[(i, j, k)]
+            for j in range(i):
+                for k in range(abs(lbs[i] - lbs[j]), lbs[i] + lbs[j] + 1, 2):
+                    if (k + lbs[i] + lbs[j]) % 2 or k < abs(jbs[i] - jbs[j]) // 2 or k > (jbs[i] + jbs[j]) // 2:
+                        continue
+                    c = coef[j, i, k]
+                    if c != 0.0:
+                        et += c * G[(i, j, k)]
+                        e2 += c * G[(i, j, k)]
+        return et, e1, e2
+
+    iq = np.asarray(iqconf)
+    et, e1, e2 = energy(iq, weight)
+    confs = []
+    for ic in range(len(iq)):
+        w = np.zeros(len(iq))
+        w[ic] = 1.0
+        confs.append(energy(iq, w)[0])
+    return et, e1, e2, confs, valueI, F, G
+
+
+def ref_av_energy(pq, kbs, lbs, jbs, nucleus, iqconf, weight):
+    """the same by the compiled reference (ref_setup must have set the
+    grid): ref_hf_setup for the orbitals, ref_set_nucleus, then DBSR_HF's
+    av_energy_coef / update_int / energy / Conf_energies"""
+    R = ref()
+    R.ref_set_nucleus.argtypes = [C.c_int] + [C.c_double] * 4
+    R.ref_av_energy.argtypes = [C.c_int, _ip, _dp, _dp]
+    n = len(kbs)
+    kmax = max(jbs)
+    coef0 = np.zeros((n, n, kmax + 1), order="F")
+    z = np.zeros(1, dtype=np.int32)
+    R.ref_hf_setup(n, kmax, 0, _i(np.asarray(kbs, dtype=np.int32)), _i(np.asarray(lbs, dtype=np.int32)),
+                   _i(np.asarray(jbs, dtype=np.int32)), _d(np.ones(n)), _d(coef0), _d(f64(pq)), 0,
+                   _i(z), _i(z), _i(z), _i(z), _d(np.zeros(1)))
+    R.ref_set_nucleus(*nucleus)
+    iq = np.asfortranarray(np.asarray(iqconf, dtype=np.int32))
+    w = np.asarray(weight, dtype=float)
+    out = np.zeros(3 + len(w))
+    R.ref_av_energy(len(w), _i(iq), _d(w), _d(out))
+    return out[0], out[1], out[2], list(out[3:])
+
+
+def ref_dhl_pq(kappa, nucleus, ms):
+    R = ref()
+    R.ref_set_nucleus.argtypes = [C.c_int] + [C.c_double] * 4
+    R.ref_dhl_pq.argtypes = [C.c_int, _dp]
+    R.ref_set_nucleus(*nucleus)
+    d = np.zeros((ms, ms), order="F")
+    R.ref_dhl_pq(kappa, _d(d))
+    return d
+
+
+def ref_vnuclear(nucleus, r):
+    R = ref()
+    R.ref_set_nucleus.argtypes = [C.c_int] + [C.c_double] * 4
+    R.ref_vnuclear.restype = C.c_double
+    R.ref_vnuclear.argtypes = [C.c_double]
+    R.ref_set_nucleus(*nucleus)
+    return R.ref_vnuclear(r)

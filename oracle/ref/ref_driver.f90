@@ -416,3 +416,101 @@
       End do
       Deallocate(part, d1, d2, cv)
       End Subroutine ref_rk_rows
+
+      Subroutine ref_set_nucleus(model, z, b, a, c) bind(C, name='ref_set_nucleus')
+! ... module DBS_nuclear: 0 point, 1 uniform (radius b), 2 Fermi (a, c)
+      Use iso_c_binding
+      Use DBS_nuclear
+      Implicit none
+      Integer(c_int), value :: model
+      Real(c_double), value :: z, b, a, c
+      atomic_number = z; r_uniform = b; a_fermi = a; c_fermi = c
+      if (model == 0) nuclear = 'point'
+      if (model == 1) nuclear = 'uniform'
+      if (model == 2) nuclear = 'Fermi'
+      End Subroutine ref_set_nucleus
+
+      Function ref_vnuclear(r) result(v) bind(C, name='ref_vnuclear')
+      Use iso_c_binding
+      Implicit none
+      Real(c_double), value :: r
+      Real(c_double) :: v
+      Real(8), external :: V_nuclear
+      v = V_nuclear(r)
+      End Function ref_vnuclear
+
+      Subroutine ref_dhl_pq(kappa, out) bind(C, name='ref_dhl_pq')
+! ... Def_Vnucl, then MAT_dhl_pq(kappa) with the module's Bloch defaults
+      Use iso_c_binding
+      Use DBS_grid
+      Use DBS_dhl_pq
+      Implicit none
+      Integer(c_int), value :: kappa
+      Real(c_double) :: out(ms, ms)
+      Call Def_Vnucl
+      Call alloc_dhl_pq(ms)
+      Call MAT_dhl_pq(kappa)
+      out = dhl
+      End Subroutine ref_dhl_pq
+
+      Subroutine ref_av_energy(nconf_, iqconf_, weight_, out) bind(C, name='ref_av_energy')
+! ... DBSR_HF's average-term energy of the orbitals set by ref_hf_setup:
+! ... configurations iqconf(nconf,nbf) with weights, qsum from them
+! ... (Read_confs), av_energy_coef, update_int(0) (Vp_dhl, hf_fk, hf_gk),
+! ... ENERGY; then the configuration energies as Conf_energies forms them.
+! ... out = (Etotal, E1body, E2body, E(conf 1..nconf))
+      Use iso_c_binding
+      Use DBS_grid
+      Use DBS_dhl_pq
+      Use df_orbitals
+      Use hf_energy
+      Use dbsr_hf, only: nconf, iqconf, weight, kmax, Etotal, E1body, E2body
+      Implicit none
+      Integer(c_int), value :: nconf_
+      Integer(c_int), intent(in) :: iqconf_(nconf_, nbf)
+      Real(c_double), intent(in) :: weight_(nconf_)
+      Real(c_double) :: out(3 + nconf_)
+      Integer :: io, ic
+      nconf = nconf_
+      if (allocated(iqconf)) Deallocate(iqconf)
+      if (allocated(weight)) Deallocate(weight)
+      Allocate(iqconf(nconf, nbf), weight(nconf))
+      iqconf = iqconf_; weight = weight_ / SUM(weight_)
+      qsum = 0.d0
+      Do io = 1, nbf
+       Do ic = 1, nconf
+        qsum(io) = qsum(io) + iqconf(ic, io) * weight(ic)
+       End do
+      End do
+      kmax = maxval(jbs(1:nbf))
+      if (allocated(coef)) Deallocate(coef)
+      if (allocated(value)) Deallocate(value)
+      if (allocated(valueI)) Deallocate(valueI)
+      Allocate(coef(nbf, nbf, 0:kmax), value(nbf, nbf, 0:kmax), valueI(nbf))
+      coef = 0.d0; value = 0.d0; valueI = 0.d0
+      Call Def_Vnucl
+      Call alloc_dhl_pq(ms)
+      Call av_energy_coef(nbf)
+      Call update_int(0)
+      Call energy
+      out(1) = Etotal; out(2) = E1body; out(3) = E2body
+      Do ic = 1, nconf
+       qsum(:) = iqconf(ic, :)
+       weight = 0.d0; weight(ic) = 1.d0; coef = 0.d0
+       Call av_energy_coef(nbf)
+       Call energy
+       out(3 + ic) = Etotal
+      End do
+      End Subroutine ref_av_energy
+
+      Subroutine ref_av_tables(kmx, vals, vali, coefo) bind(C, name='ref_av_tables')
+! ... module hf_energy's tables after ref_av_energy: value(nbf,nbf,0:kmax),
+! ... valueI(nbf) and coef(nbf,nbf,0:kmax) of the last configuration set
+      Use iso_c_binding
+      Use df_orbitals, only: nbf
+      Use hf_energy
+      Implicit none
+      Integer(c_int), value :: kmx
+      Real(c_double) :: vals(nbf, nbf, 0:kmx), vali(nbf), coefo(nbf, nbf, 0:kmx)
+      vals = value; vali = valueI; coefo = coef
+      End Subroutine ref_av_tables
