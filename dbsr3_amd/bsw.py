@@ -1,4 +1,5 @@
-"""B-spline wave-function files (.bsw) of DBSR_HF, read on the host.
+"""B-spline wave-function files (.bsw) of DBSR_HF and DBSR_MCHF, read on
+the host.
 
 Layout, as write_pqbs writes it (DBSR_HF/hf_read_pqbs.f90:61-96) and
 Read_pqbs / read_grid_bsw read it (:2-57; DBSR_HF/dbsr_lib_dbs.f90:635-651):
@@ -8,6 +9,8 @@ Fortran sequential unformatted records, each framed by its byte length in
     record 1:  grid_type, ns, ks (int32), t(1:ns+ks), ksp, ksq (int32),
                atomic_number, atomic_weight, nuclear (20 characters),
                r_uniform, a_fermi, c_fermi, t_fermi, rrms (float64)
+               -- DBSR_MCHF's Write_pqbs stops after ksp, ksq
+               (DBSR_MCHF/read_pqbs.f90:84); the nucleus keys are then None
     per orbital, three records:
                label (5 characters), mbs (int32), e (float64)
                P(1:mbs)
@@ -75,6 +78,11 @@ def read_bsw(path):
     off += 8 * (ns + ks)
     ksp, ksq = struct.unpack_from("<2i", h, off)
     off += 8
+    if off == len(h):
+        nucl = dict(atomic_number=None, atomic_weight=None, nuclear=None, r_uniform=None, a_fermi=None,
+                    c_fermi=None, t_fermi=None, rrms=None)
+        return _orbitals(dict(grid_type=grid_type, ns=ns, ks=ks, ksp=ksp, ksq=ksq, t=t, **nucl, orbitals=[]),
+                         recs[1:])
     z, atw = struct.unpack_from("<2d", h, off)
     off += 16
     nuclear = h[off:off + 20].decode("ascii").strip()
@@ -86,7 +94,11 @@ def read_bsw(path):
     out = dict(grid_type=grid_type, ns=ns, ks=ks, ksp=ksp, ksq=ksq, t=t, atomic_number=z, atomic_weight=atw,
                nuclear=nuclear, r_uniform=r_uniform, a_fermi=a_fermi, c_fermi=c_fermi, t_fermi=t_fermi,
                rrms=rrms, orbitals=[])
-    body = recs[1:]
+    return _orbitals(out, recs[1:])
+
+
+def _orbitals(out, body):
+    ns = out["ns"]
     if len(body) % 3:
         raise ValueError("bsw: orbital records come in threes")
     for q in range(0, len(body), 3):

@@ -120,6 +120,9 @@ PROTOTYPES = {
     "dbs_sdata_dev": [vp, mbp, auxp, i32, i32, i32, dp, ip, ip, ip, ip, vp, i32],
     "dbs_bav_dev": [vp, vp, vp, ch, ch, vp],
     "dbs_hf_matrix_dev": [vp, i32, i32, vp, dp, dp, i32, f64, i32, i32, ip, ip, ip, ip, dp, vp],
+    "dbs_mchf_matrix_dev": [vp, i32, i32, vp, vp, i32, ip, ip, dp, i32, ip, ip, ip, ip, ip, dp, i32, f64, f64,
+                            i32, i32, vp, vp, vp, vp, dp, dp],
+    "dbs_core_matrix_dev": [vp, i32, i32, vp, vp, dp, dp, i32, dp, i32, vp, vp],
     "dbs_engine_create": [C.POINTER(vp), i32, C.POINTER(vp), ip, ip, ip, i32],
     "dbs_engine_destroy": [vp],
     "dbs_engine_buffers": [vp, i32, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp),

@@ -142,6 +142,125 @@ __global__ void k_div(long long n, double* __restrict__ h, double q) {
     h[e] = h[e] / q;
 }
 
+
+// ---------------------------------------------------------------------------
+// DBSR_MCHF orbital matrices: small dense pieces of mchf_matrix / core_matrix
+// ---------------------------------------------------------------------------
+// p = scale * src, or zero when src is null
+__global__ void k_set(long long n, double* __restrict__ p, const double* __restrict__ src, double scale) {
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x)
+    p[e] = src ? scale * src[e] : 0.0;
+}
+
+// y(r) = y(r) + sum_c A(r,c) x(c), A (m,m) column-major, c ascending (MATMUL)
+__global__ void k_gemv_add(int m, const double* __restrict__ A, const double* __restrict__ x, double* __restrict__ y) {
+#pragma clang fp contract(off)
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < m; r += gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int c = 0; c < m; ++c) s += A[r + (long long)m * c] * x[c];
+    y[r] = y[r] + s;
+  }
+}
+
+// y(r) = y(r) + sum_c Full_mat_sym(dd,'s')(r,c) p(c): the band columns of row r
+__global__ void k_band_mv_add(int ns, int ks, const double* __restrict__ dd, const double* __restrict__ p,
+                              double* __restrict__ y) {
+#pragma clang fp contract(off)
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < ns; r += gridDim.x * blockDim.x) {
+    double s = 0.0;
+    const int c0 = max(0, r - ks + 1), c1 = min(ns - 1, r + ks - 1);
+    for (int c = c0; c <= c1; ++c) s += band_elem(dd, ns, ks, 0, r, c) * p[c];
+    y[r] = y[r] + s;
+  }
+}
+
+// vv = sum_e v(orb_e) * coef_e / S / 2, v = (P, Q) of the orbital (Get_pv_df)
+__global__ void k_rhs_vec(int ms, int n, const int* __restrict__ orb, const double* __restrict__ coef, double S,
+                          const double* __restrict__ pq, double* __restrict__ vv) {
+#pragma clang fp contract(off)
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < ms; r += gridDim.x * blockDim.x) {
+    double a = 0.0;
+    for (int e = 0; e < n; ++e) a = a + pq[r + (long long)ms * (orb[e] - 1)] * coef[e] / S / 2.0;
+    vv[r] = a;
+  }
+}
+
+// one workgroup of 256: sum over 256 strided partial sums
+__device__ double block_sum256(double x, double* sh) {
+  const int t = threadIdx.x;
+  sh[t] = x;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) sh[t] = sh[t] + sh[t + w];
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+// mchf_matrix.f90:238-246: srhs = sum|rhs|, rhs dropped below 1e-8;
+// scal[0] = 1 when rhs moves into hfm (irhs > 0, srhs > 0), scal[1] = SS =
+// -rhs.v.  One workgroup of 256.
+__global__ void k_mchf_rhs_scan(int ms, int irhs, const double* __restrict__ v, double* __restrict__ rhs,
+                                double* __restrict__ scal) {
+#pragma clang fp contract(off)
+  __shared__ double sh[256];
+  double a = 0.0;
+  for (int r = threadIdx.x; r < ms; r += 256) a += fabs(rhs[r]);
+  double srhs = block_sum256(a, sh);
+  if (srhs < 1e-8) {
+    for (int r = threadIdx.x; r < ms; r += 256) rhs[r] = 0.0;
+    srhs = 0.0;
+  }
+  __syncthreads();
+  a = 0.0;
+  for (int r = threadIdx.x; r < ms; r += 256) a += rhs[r] * v[r];
+  const double rv = block_sum256(a, sh);
+  if (threadIdx.x == 0) {
+    scal[0] = (irhs > 0 && srhs > 0.0) ? 1.0 : 0.0;
+    scal[1] = -rv;
+  }
+}
+
+// irhs conversion (mchf_matrix.f90:243-251): h = h + SS B, then
+// h(a,b) = h(a,b) + rhs(a) vv(b) + rhs(b) vv(a), vv = B v
+__global__ void k_mchf_irhs(int ms, const double* __restrict__ B, const double* __restrict__ rhs,
+                            const double* __restrict__ vv, const double* __restrict__ scal, double* __restrict__ h) {
+#pragma clang fp contract(off)
+  if (scal[0] == 0.0) return;
+  const double SS = scal[1];
+  const long long n = (long long)ms * ms;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
+    const int a = (int)(e % ms), b = (int)(e / ms);
+    double x = h[e] + SS * B[e];
+    h[e] = x + rhs[a] * vv[b] + rhs[b] * vv[a];
+  }
+}
+
+// hfm = dhl + hfm; rhs = -rhs (zero after the irhs conversion)
+__global__ void k_mchf_final(int ms, const double* __restrict__ dhl, const double* __restrict__ scal,
+                             double* __restrict__ h, double* __restrict__ rhs) {
+#pragma clang fp contract(off)
+  const long long n = (long long)ms * ms;
+  const bool moved = scal[0] != 0.0;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
+    h[e] = dhl[e] + h[e];
+    if (e < ms) rhs[e] = moved ? -0.0 : -rhs[e];
+  }
+}
+
+// e(i) = v.(w - rhs), w = hfm v; one workgroup of 256 -> scal[2]
+__global__ void k_mchf_energy(int ms, const double* __restrict__ v, const double* __restrict__ w,
+                              const double* __restrict__ rhs, double* __restrict__ scal) {
+#pragma clang fp contract(off)
+  __shared__ double sh[256];
+  double a = 0.0;
+  for (int r = threadIdx.x; r < ms; r += 256) a += v[r] * (w[r] - rhs[r]);
+  const double e = block_sum256(a, sh);
+  if (threadIdx.x == 0) scal[2] = e;
+}
+
 namespace {
 
 unsigned grid_for(long long n, int block) {
@@ -572,6 +691,287 @@ int hf_matrix(dbs_ctx* ctx, int i, int norb, const double* pq_dev, const double*
   return DBS_OK;
 }
 
+
+// ---------------------------------------------------------------------------
+// mchf_matrix (DBSR_MCHF/mchf_matrix.f90:2-248).  The angular list arrives
+// flattened, one entry per coefficient with c = WW(ic,jc)*coef.  The
+// collection list (module id4_data, un-vendored Libraries_03) keeps entries
+// in order of first insertion and sums equal keys (k, id2, id3, id4).
+// ---------------------------------------------------------------------------
+struct Id4 {
+  int id2, id3, id4;
+  double c;
+};
+
+void id4_add(std::vector<Id4>& l, int id2, int id3, int id4, double c) {
+  for (Id4& e : l)
+    if (e.id2 == id2 && e.id3 == id3 && e.id4 == id4) {
+      e.c = e.c + c;
+      return;
+    }
+  l.push_back({id2, id3, id4, c});
+}
+
+// mchf_matrix.f90:89-144 for one k
+void id4_collect(int i, int k, int nr, const int* rk, const int* r1, const int* r2, const int* r3, const int* r4,
+                 const double* rc, double S, double eps_c, std::vector<Id4>& l) {
+  for (int n = 0; n < nr; ++n) {
+    if (rk[n] != k) continue;
+    const int i1 = r1[n], i2 = r2[n], i3 = r3[n], i4 = r4[n];
+    const double C = rc[n] / S;
+    if (fabs(C) < eps_c) continue;
+    const int ii = (i == i1 ? 1000 : 0) + (i == i2 ? 100 : 0) + (i == i3 ? 10 : 0) + (i == i4 ? 1 : 0);
+    switch (ii) {
+      case 1111: id4_add(l, -1, i1, i3, 2 * C); id4_add(l, -3, i1, i3, 4 * C); break;
+      case 1010: id4_add(l, -1, i2, i4, C); break;
+      case 101: id4_add(l, -1, i1, i3, C); break;
+      case 1001: id4_add(l, -2, i2, i3, C); break;
+      case 1100: id4_add(l, -2, i3, i4, C); break;
+      case 110: id4_add(l, -2, i1, i4, C); break;
+      case 11: id4_add(l, -2, i1, i2, C); break;
+      case 111:
+      case 1101: id4_add(l, -1, i1, i3, C); id4_add(l, -2, i1, i3, C / 2); id4_add(l, -3, i1, i3, 3 * C / 2); break;
+      case 1011:
+      case 1110: id4_add(l, -1, i2, i4, C); id4_add(l, -2, i2, i4, C / 2); id4_add(l, -3, i2, i4, 3 * C / 2); break;
+      case 1000: id4_add(l, i3, i2, i4, C / 2); break;
+      case 100: id4_add(l, i4, i1, i3, C / 2); break;
+      case 10: id4_add(l, i1, i2, i4, C / 2); break;
+      case 1: id4_add(l, i2, i1, i3, C / 2); break;
+      default: break;
+    }
+  }
+}
+
+// group lists over k (g = k) or over (k, orbital) for one P/Q block
+struct Groups {
+  std::vector<int> off{0}, j1, j2;
+  std::vector<double> c;
+  void close() { off.push_back((int)c.size()); }
+  void add(int a, int b, double x) {
+    j1.push_back(a);
+    j2.push_back(b);
+    c.push_back(x);
+  }
+  bool has(int g) const { return off[g + 1] > off[g]; }
+};
+
+int launch_groups(DevMem& mem, hipStream_t s, const Groups& G, const double* pq, int ns, int ks, int jp1, int jp2,
+                  int type, double** out) {
+  const int ng = (int)G.off.size() - 1;
+  const int ncol = type == 0 ? ks : ns;
+  int rc;
+  if ((rc = mem.alloc(out, (size_t)ns * ncol * std::max(ng, 1)))) return rc;
+  if (G.c.empty()) return DBS_OK;
+  int *d_off, *d_j1, *d_j2;
+  double* d_c;
+  if ((rc = mem.put(G.off, &d_off)) || (rc = mem.put(G.j1, &d_j1)) || (rc = mem.put(G.j2, &d_j2)) ||
+      (rc = mem.put(G.c, &d_c)))
+    return rc;
+  k_run_density<<<grid_for((long long)ns * ncol * ng, 256), 256, 0, s>>>(ng, d_off, d_j1, d_j2, d_c, pq, ns, ks, jp1,
+                                                                         jp2, type, *out);
+  DBS_HIP(hipGetLastError());
+  return DBS_OK;
+}
+
+int mchf_matrix(dbs_ctx* ctx, int i, int norb, const double* pq, const double* dhl, int nl, const int* li1,
+                const int* li2, const double* lc, int nr, const int* rk, const int* r1, const int* r2, const int* r3,
+                const int* r4, const double* rc_, int kmax, double eps_c, double qsum_min, int newton, int irhs,
+                const double* fppqq, double* hfm, double* rhs, double* hx, double* e_out, double* qsum_out) {
+  const int ns = ctx->ns, ks = ctx->ks, ms = ctx->ms;
+  hipStream_t s = ctx->stream;
+  const long long mm = (long long)ms * ms;
+  // qsum(i) and CI (mchf_matrix.f90:59-72)
+  double S = 0.0;
+  std::vector<double> CI(norb, 0.0);
+  for (int n = 0; n < nl; ++n) {
+    const double C = lc[n];
+    if (li1[n] == i && li2[n] == i) S = S + C;
+    else if (li1[n] == i) CI[li2[n] - 1] = CI[li2[n] - 1] + C;
+    else if (li2[n] == i) CI[li1[n] - 1] = CI[li1[n] - 1] + C;
+  }
+  if (fabs(S) < qsum_min) S = qsum_min;
+  DevMem mem(s);
+  double *Dt, *Xt, *W, *scal;
+  int rc;
+  if ((rc = mem.alloc(&Dt, (size_t)ns * ks)) || (rc = mem.alloc(&Xt, (size_t)ns * ns)) || (rc = mem.alloc(&W, ms)) ||
+      (rc = mem.alloc(&scal, 4)))
+    return rc;
+  k_set<<<grid_for(mm, 256), 256, 0, s>>>(mm, hfm, nullptr, 0.0);
+  k_set<<<grid_for(mm, 256), 256, 0, s>>>(mm, hx, nullptr, 0.0);
+  k_set<<<1, 256, 0, s>>>(ms, rhs, nullptr, 0.0);
+  // one-electron rhs (mchf_matrix.f90:74-82)
+  double sci = 0.0;
+  std::vector<int> io;
+  std::vector<double> ci;
+  for (int o = 1; o <= norb; ++o) {
+    sci += fabs(CI[o - 1]);
+    if (CI[o - 1] != 0.0) {
+      io.push_back(o);
+      ci.push_back(CI[o - 1]);
+    }
+  }
+  if (sci != 0.0) {
+    int* d_io;
+    double* d_ci;
+    if ((rc = mem.put(io, &d_io)) || (rc = mem.put(ci, &d_ci))) return rc;
+    k_rhs_vec<<<grid_for(ms, 256), 256, 0, s>>>(ms, (int)io.size(), d_io, d_ci, S, pq, W);
+    k_gemv_add<<<grid_for(ms, 64), 64, 0, s>>>(ms, dhl, W, rhs);
+  }
+  // collection per k, then per P/Q block the densities of all k in one
+  // launch each: direct, exchange, rhs (per orbital) and Newton
+  std::vector<std::vector<Id4>> lists(kmax + 1);
+  for (int k = 0; k <= kmax; ++k) id4_collect(i, k, nr, rk, r1, r2, r3, r4, rc_, S, eps_c, lists[k]);
+  static const int blk[4][3] = {{1, 1, DBS_PPPP}, {1, 2, DBS_PQPQ}, {2, 1, DBS_QPQP}, {2, 2, DBS_QQQQ}};
+  Groups Gd[4], Gx[4], Gr[4], Gh[4];
+  std::vector<std::vector<int>> rj[4];  // rhs group orbitals per k
+  for (int b = 0; b < 4; ++b) rj[b].resize(kmax + 1);
+  for (int k = 0; k <= kmax; ++k) {
+    const std::vector<Id4>& l = lists[k];
+    int last3 = -1;
+    for (int n = 0; n < (int)l.size(); ++n)
+      if (l[n].id2 == -3 && l[n].c != 0.0) last3 = n;
+    std::vector<int> rorb;
+    for (const Id4& e : l)
+      if (e.id2 > 0 && std::find(rorb.begin(), rorb.end(), e.id2) == rorb.end()) rorb.push_back(e.id2);
+    std::sort(rorb.begin(), rorb.end());
+    for (int b = 0; b < 4; ++b) {
+      for (const Id4& e : l)
+        if (e.id2 == -1 && e.c != 0.0) Gd[b].add(e.id3, e.id4, e.c);
+      Gd[b].close();
+      for (const Id4& e : l)
+        if (e.id2 == -2 && e.c != 0.0) Gx[b].add(e.id3, e.id4, e.c);
+      Gx[b].close();
+      for (int j : rorb) {
+        for (const Id4& e : l)
+          if (e.id2 == j) Gr[b].add(e.id3, e.id4, e.c);
+        Gr[b].close();
+        rj[b][k].push_back(j);
+      }
+      // the Newton term takes the last -3 density without its coefficient
+      // (mchf_matrix.f90:222-231: Convol's output is xx, its input x)
+      if (newton && last3 >= 0) Gh[b].add(l[last3].id3, l[last3].id4, 1.0);
+      Gh[b].close();
+    }
+  }
+  double *DD[4], *XX[4], *RA[4], *HX[4];
+  for (int b = 0; b < 4; ++b) {
+    const int ii = blk[b][0], jj = blk[b][1];
+    if ((rc = launch_groups(mem, s, Gd[b], pq, ns, ks, jj, jj, 0, &DD[b])) ||
+        (rc = launch_groups(mem, s, Gx[b], pq, ns, ks, ii, jj, 3, &XX[b])) ||
+        (rc = launch_groups(mem, s, Gr[b], pq, ns, ks, jj, jj, 0, &RA[b])) ||
+        (rc = launch_groups(mem, s, Gh[b], pq, ns, ks, jj, jj, 3, &HX[b])))
+      return rc;
+  }
+  std::vector<int> rg(4, 0);  // next rhs group per block
+  for (int k = 0; k <= kmax; ++k) {
+    if (lists[k].empty()) continue;
+    for (int b = 0; b < 4; ++b) {
+      const int ii = blk[b][0], jj = blk[b][1], type = blk[b][2];
+      const bool dir = Gd[b].has(k), exch = Gx[b].has(k), nwt = Gh[b].has(k);
+      const int nrg = (int)rj[b][k].size();
+      if (!dir && !exch && !nwt && nrg == 0) continue;
+      double* rkb = nullptr;
+      if ((rc = build_rkb(ctx, type, k, &rkb))) return rc;
+      if (dir) {
+        if ((rc = convol_launch(ctx, rkb, type, 1, 's', 's', DD[b] + (size_t)ns * ks * k, Dt))) return rc;
+        if ((rc = dbs_update_hs_dev(ctx, hfm, ii, ii, Dt, 's'))) return rc;
+      }
+      if (exch) {
+        if ((rc = convol_launch(ctx, rkb, type, 4, 's', 's', XX[b] + (size_t)ns * ns * k, Xt))) return rc;
+        if ((rc = dbs_update_hs_dev(ctx, hfm, ii, jj, Xt, 'x'))) return rc;
+      }
+      for (int q = 0; q < nrg; ++q) {
+        const int g = rg[b]++, j = rj[b][k][q];
+        if ((rc = convol_launch(ctx, rkb, type, 1, 's', 's', RA[b] + (size_t)ns * ks * g, Dt))) return rc;
+        k_band_mv_add<<<grid_for(ns, 64), 64, 0, s>>>(ns, ks, Dt, pq + (size_t)ns * ((ii - 1) + 2 * (size_t)(j - 1)),
+                                                     rhs + (size_t)(ii - 1) * ns);
+      }
+      if (nwt) {
+        if ((rc = convol_launch(ctx, rkb, type, 4, 's', 's', HX[b] + (size_t)ns * ns * k, Xt))) return rc;
+        if ((rc = dbs_update_hs_dev(ctx, hx, ii, jj, Xt, 'x'))) return rc;
+      }
+    }
+  }
+  // rhs into hfm, final matrix and e(i) (mchf_matrix.f90:236-258)
+  const double* v = pq + (size_t)ms * (i - 1);
+  k_mchf_rhs_scan<<<1, 256, 0, s>>>(ms, irhs, v, rhs, scal);
+  if (irhs > 0) {
+    k_set<<<grid_for(ms, 256), 256, 0, s>>>(ms, W, nullptr, 0.0);
+    k_gemv_add<<<grid_for(ms, 64), 64, 0, s>>>(ms, fppqq, v, W);
+    k_mchf_irhs<<<grid_for(mm, 256), 256, 0, s>>>(ms, fppqq, rhs, W, scal, hfm);
+  }
+  k_mchf_final<<<grid_for(mm, 256), 256, 0, s>>>(ms, dhl, scal, hfm, rhs);
+  k_set<<<grid_for(ms, 256), 256, 0, s>>>(ms, W, nullptr, 0.0);
+  k_gemv_add<<<grid_for(ms, 64), 64, 0, s>>>(ms, hfm, v, W);
+  k_mchf_energy<<<1, 256, 0, s>>>(ms, v, W, rhs, scal);
+  double h_scal[4];
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(h_scal, scal, sizeof h_scal, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return fail(DBS_ERR_HIP, std::string("mchf_matrix: ") + hipGetErrorString(e));
+  if (e_out) *e_out = h_scal[2];
+  if (qsum_out) *qsum_out = S;
+  return DBS_OK;
+}
+
+// core_matrix (DBSR_MCHF/mchf_matrix.f90:253-345): hf_matrix's loops with
+// a(i,j,k) qsum(i) qsum(j) (qsum(i)(qsum(i)-1) for j = i, which hf_matrix
+// doubles, so half of it goes in) and b likewise; hfm starts as qsum(i) hd
+int core_matrix(dbs_ctx* ctx, int i, int norb, const double* pq, const double* hd, const double* A, const double* B,
+                int kmax, const double* qsum, int newton, double* hfm, double* hx) {
+  const int ns = ctx->ns, ms = ctx->ms;
+  hipStream_t s = ctx->stream;
+  const long long mm = (long long)ms * ms;
+  const double q = qsum[i - 1];
+  std::vector<double> A2((size_t)norb * (kmax + 1)), B2((size_t)norb * (kmax + 1));
+  for (int k = 0; k <= kmax; ++k)
+    for (int j = 1; j <= norb; ++j) {
+      const size_t x = (j - 1) + (size_t)norb * k;
+      if (j != i) {
+        A2[x] = A[x] * q * qsum[j - 1];
+        B2[x] = B[x] * q * qsum[j - 1];
+      } else {
+        A2[x] = A[x] * q * (q - 1) / 2.0;
+        B2[x] = B[x] * q * (q - 1);
+      }
+    }
+  k_set<<<grid_for(mm, 256), 256, 0, s>>>(mm, hfm, hd, q);
+  k_set<<<grid_for(mm, 256), 256, 0, s>>>(mm, hx, nullptr, 0.0);
+  DBS_HIP(hipGetLastError());
+  int rc = hf_matrix(ctx, i, norb, pq, A2.data(), B2.data(), kmax, q, 0, 0, nullptr, nullptr, nullptr, nullptr,
+                     nullptr, hfm);
+  if (rc || !newton) return rc;
+  // H_aa terms (:323-331): c = 2 a(i,i,k) q (q-1), exchange density of i
+  DevMem mem(s);
+  double *X, *Xt;
+  if ((rc = mem.alloc(&X, (size_t)ns * ns)) || (rc = mem.alloc(&Xt, (size_t)ns * ns))) return rc;
+  static const int blk[4][3] = {{1, 1, DBS_PPPP}, {1, 2, DBS_PQPQ}, {2, 1, DBS_QPQP}, {2, 2, DBS_QQQQ}};
+  std::vector<int> off{0, 1}, jl{i};
+  for (int b = 0; b < 4; ++b)
+    for (int k = 0; k <= kmax; ++k) {
+      double c = 2.0 * A[(i - 1) + (size_t)norb * k];
+      if (c == 0.0) continue;
+      c = c * q * (q - 1);
+      const int ii = blk[b][0], jj = blk[b][1], type = blk[b][2];
+      std::vector<double> cl{c};
+      DevMem lists(s);
+      int *d_off, *d_j;
+      double* d_c;
+      if ((rc = lists.put(off, &d_off)) || (rc = lists.put(jl, &d_j)) || (rc = lists.put(cl, &d_c))) return rc;
+      k_run_density<<<grid_for((long long)ns * ns, 256), 256, 0, s>>>(1, d_off, d_j, d_j, d_c, pq, ns, ctx->ks, ii,
+                                                                       jj, 3, X);
+      double* rkb = nullptr;
+      if ((rc = build_rkb(ctx, type, k, &rkb))) return rc;
+      if ((rc = convol_launch(ctx, rkb, type, 4, 's', 's', X, Xt))) return rc;
+      if ((rc = dbs_update_hs_dev(ctx, hx, ii, jj, Xt, 'x'))) return rc;
+    }
+  k_div<<<grid_for(mm, 256), 256, 0, s>>>(mm, hx, q);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return fail(DBS_ERR_HIP, std::string("core_matrix: ") + hipGetErrorString(e));
+  return DBS_OK;
+}
+
 int rs_data(dbs_ctx* ctx, const dbs_mat_blocks* m, const dbs_mat3_aux* aux, int stype, int itype, int atype, int k,
             int nc, const double* c, const int* K1, const int* K2, const int* K3, const int* K4,
             const double* pq_dev, int norb) {
@@ -649,6 +1049,39 @@ int dbs_hf_matrix_dev(dbs_ctx* ctx, int i, int norb, const double* pq_dev, const
   DBS_CHECK(mbreit != 2 || nrk <= 0 || (kr1 && kr2 && kr3 && kr4 && crk), DBS_ERR_ARG,
             "hf_matrix: Breit list missing");
   return hf_matrix(ctx, i, norb, pq_dev, acoef, bcoef, kmax, qsum, mbreit, nrk, kr1, kr2, kr3, kr4, crk, hfm_dev);
+}
+
+int dbs_mchf_matrix_dev(dbs_ctx* ctx, int i, int norb, const double* pq_dev, const double* dhl_dev, int nl,
+                        const int* l_i1, const int* l_i2, const double* l_c, int nr, const int* r_k, const int* r_i1,
+                        const int* r_i2, const int* r_i3, const int* r_i4, const double* r_c, int kmax, double eps_c,
+                        double qsum_min, int newton, int irhs, const double* fppqq_dev, double* hfm_dev,
+                        double* rhs_dev, double* hx_dev, double* e_i, double* qsum_i) {
+  DBS_CHECK(ctx && pq_dev && dhl_dev && hfm_dev && rhs_dev && hx_dev, DBS_ERR_ARG, "mchf_matrix: null argument");
+  DBS_CHECK(i >= 1 && i <= norb && kmax >= 0 && nl >= 0 && nr >= 0, DBS_ERR_ARG,
+            "mchf_matrix: orbital, kmax or list length out of range");
+  DBS_CHECK(nl == 0 || (l_i1 && l_i2 && l_c), DBS_ERR_ARG, "mchf_matrix: L list missing");
+  DBS_CHECK(nr == 0 || (r_k && r_i1 && r_i2 && r_i3 && r_i4 && r_c), DBS_ERR_ARG, "mchf_matrix: Rk list missing");
+  DBS_CHECK(irhs <= 0 || fppqq_dev, DBS_ERR_ARG, "mchf_matrix: irhs > 0 needs fppqq");
+  DBS_CHECK(qsum_min > 0.0, DBS_ERR_ARG, "mchf_matrix: qsum_min <= 0");
+  for (int n = 0; n < nl; ++n)
+    DBS_CHECK(l_i1[n] >= 1 && l_i1[n] <= norb && l_i2[n] >= 1 && l_i2[n] <= norb, DBS_ERR_ARG,
+              "mchf_matrix: L orbital index out of range");
+  for (int n = 0; n < nr; ++n)
+    DBS_CHECK(r_k[n] >= 0 && r_i1[n] >= 1 && r_i1[n] <= norb && r_i2[n] >= 1 && r_i2[n] <= norb && r_i3[n] >= 1 &&
+                  r_i3[n] <= norb && r_i4[n] >= 1 && r_i4[n] <= norb,
+              DBS_ERR_ARG, "mchf_matrix: Rk orbital index or k out of range");
+  return mchf_matrix(ctx, i, norb, pq_dev, dhl_dev, nl, l_i1, l_i2, l_c, nr, r_k, r_i1, r_i2, r_i3, r_i4, r_c, kmax,
+                     eps_c, qsum_min, newton, irhs, fppqq_dev, hfm_dev, rhs_dev, hx_dev, e_i, qsum_i);
+}
+
+int dbs_core_matrix_dev(dbs_ctx* ctx, int i, int norb, const double* pq_dev, const double* hd_dev,
+                        const double* acoef, const double* bcoef, int kmax, const double* qsum, int newton,
+                        double* hfm_dev, double* hx_dev) {
+  DBS_CHECK(ctx && pq_dev && hd_dev && acoef && bcoef && qsum && hfm_dev && hx_dev, DBS_ERR_ARG,
+            "core_matrix: null argument");
+  DBS_CHECK(i >= 1 && i <= norb && kmax >= 0, DBS_ERR_ARG, "core_matrix: orbital or kmax out of range");
+  DBS_CHECK(qsum[i - 1] != 0.0, DBS_ERR_ARG, "core_matrix: qsum = 0");
+  return core_matrix(ctx, i, norb, pq_dev, hd_dev, acoef, bcoef, kmax, qsum, newton, hfm_dev, hx_dev);
 }
 
 int dbs_bav_dev(dbs_ctx* ctx, const double* a_dev, const double* b_dev, char sym, char dir, double* c_dev) {

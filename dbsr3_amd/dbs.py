@@ -265,6 +265,57 @@ class DBS:
         self._post()
         return hfm
 
+    def mchf_matrix(self, i, pq, dhl, llist, rlist, kmax, eps_c=1e-8, qsum_min=1e-8, newton=0, irhs=0,
+                    fppqq=None):
+        """mchf_matrix for the non-core orbital i (1-based): pq device tensor
+        holding pq(ns,2,norb), dhl device tensor holding Get_hd_core(kbs(i))
+        (ms,ms); llist rows (i1, i2, c) and rlist rows (k, i1, i2, i3, i4, c)
+        with c = WW(ic,jc)*coef per angular coefficient.  Returns (hfm, rhs,
+        hx, e(i), qsum(i)), the matrices as device tensors."""
+        norb = int(pq.shape[0])
+        L = np.asarray(llist, dtype=np.float64).reshape(-1, 3)
+        R = np.asarray(rlist, dtype=np.float64).reshape(-1, 6)
+        ci = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+        cd = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        l1, l2, lc = ci(L[:, 0]), ci(L[:, 1]), cd(L[:, 2])
+        rk, r1, r2, r3, r4, rc = ci(R[:, 0]), ci(R[:, 1]), ci(R[:, 2]), ci(R[:, 3]), ci(R[:, 4]), cd(R[:, 5])
+        hfm = self._empty(self.ms, self.ms)
+        hx = self._empty(self.ms, self.ms)
+        rhs = self._empty(self.ms)
+        e = C.c_double()
+        q = C.c_double()
+        ip_ = C.POINTER(C.c_int)
+        dp_ = C.POINTER(C.c_double)
+        self._pre()
+        check(lib().dbs_mchf_matrix_dev(self.ctx, i, norb, _ptr(pq), _ptr(dhl), len(lc), l1.ctypes.data_as(ip_),
+                                        l2.ctypes.data_as(ip_), lc.ctypes.data_as(dp_), len(rc),
+                                        rk.ctypes.data_as(ip_), r1.ctypes.data_as(ip_), r2.ctypes.data_as(ip_),
+                                        r3.ctypes.data_as(ip_), r4.ctypes.data_as(ip_), rc.ctypes.data_as(dp_),
+                                        kmax, float(eps_c), float(qsum_min), int(newton), int(irhs),
+                                        _ptr(fppqq) if fppqq is not None else None, _ptr(hfm), _ptr(rhs),
+                                        _ptr(hx), C.byref(e), C.byref(q)))
+        self._post()
+        return hfm, rhs, hx, e.value, q.value
+
+    def core_matrix(self, i, pq, hd, acoef, bcoef, kmax, qsum, newton=0):
+        """core_matrix for the core orbital i (1-based): hd device tensor
+        holding Get_hd(kbs(i)) (ms,ms); acoef/bcoef (norb, kmax+1) host
+        a(i,j,k), b(i,j,k); qsum (norb) host.  Returns (hfm, hx) device
+        tensors."""
+        norb = int(pq.shape[0])
+        A = np.asfortranarray(acoef, dtype=np.float64)
+        B = np.asfortranarray(bcoef, dtype=np.float64)
+        q = np.ascontiguousarray(qsum, dtype=np.float64)
+        hfm = self._empty(self.ms, self.ms)
+        hx = self._empty(self.ms, self.ms)
+        dp_ = C.POINTER(C.c_double)
+        self._pre()
+        check(lib().dbs_core_matrix_dev(self.ctx, i, norb, _ptr(pq), _ptr(hd), A.ctypes.data_as(dp_),
+                                        B.ctypes.data_as(dp_), kmax, q.ctypes.data_as(dp_), int(newton),
+                                        _ptr(hfm), _ptr(hx)))
+        self._post()
+        return hfm, hx
+
     def UPDATE_HS(self, hm, ii, jj, d, sym):
         """UPDATE_HS(ms,hm,ii,jj,ns,ks,d,sym): hm device tensor holding (ms,ms)"""
         self._pre()

@@ -362,6 +362,34 @@ int dbs_hf_matrix_dev(dbs_ctx *ctx, int i, int norb, const double *pq_dev, const
                       const double *bcoef, int kmax, double qsum, int mbreit, int nrk, const int *kr1,
                       const int *kr2, const int *kr3, const int *kr4, const double *crk,
                       double *hfm_dev);
+/* mchf_matrix (DBSR_MCHF/mchf_matrix.f90:2-248) for the non-core orbital i
+ * (1-based) of pq_dev(ns,2,norb): hfm_dev(ms,ms), rhs_dev(ms), hx_dev(ms,ms)
+ * (device, overwritten) and, on the host, e(i) = v.(hfm v - rhs) and
+ * qsum(i).  dhl_dev(ms,ms) is the one-electron operator Get_hd_core(kbs(i))
+ * (MOD_hd_core.f90:337-356).  The angular lists are flattened, one entry
+ * per coefficient with c = WW(ic,jc)*coef (mchf_matrix.f90:64, 94): L
+ * integrals (l_i1, l_i2, l_c) and R^k integrals (r_k, r_i1..r_i4, r_c);
+ * entries with |c/qsum| < eps_c are dropped, |qsum| < qsum_min takes
+ * qsum_min (DBSR_MCHF/MOD_dbsr_mchf.f90:52-55: 1e-8, 1e-8).  newton != 0
+ * builds hx; irhs > 0 moves rhs into hfm with fppqq_dev (ms,ms), the B-spline
+ * overlap matrix.  Equal keys (k, flag, id3, id4) of the collection list sum
+ * their coefficients, in order of first insertion (module id4_data is in the
+ * un-vendored Libraries_03); hx is the case-4 convolution of the LAST Newton
+ * density of each (k, P/Q block) without its coefficient, as the reference
+ * computes it (:222-231). */
+int dbs_mchf_matrix_dev(dbs_ctx *ctx, int i, int norb, const double *pq_dev, const double *dhl_dev, int nl,
+                        const int *l_i1, const int *l_i2, const double *l_c, int nr, const int *r_k,
+                        const int *r_i1, const int *r_i2, const int *r_i3, const int *r_i4, const double *r_c,
+                        int kmax, double eps_c, double qsum_min, int newton, int irhs, const double *fppqq_dev,
+                        double *hfm_dev, double *rhs_dev, double *hx_dev, double *e_i, double *qsum_i);
+/* core_matrix (DBSR_MCHF/mchf_matrix.f90:253-345) for the core orbital i:
+ * hfm_dev = (qsum(i) hd + direct + exchange)/qsum(i) with the coefficients
+ * a(i,j,k) qsum(i) qsum(j) (qsum(i)(qsum(i)-1) for j = i) and b likewise;
+ * hd_dev(ms,ms) = Get_hd(kbs(i)); acoef/bcoef(norb, kmax+1) host; qsum(norb)
+ * host; hx_dev the Newton term (zero unless newton != 0). */
+int dbs_core_matrix_dev(dbs_ctx *ctx, int i, int norb, const double *pq_dev, const double *hd_dev,
+                        const double *acoef, const double *bcoef, int kmax, const double *qsum, int newton,
+                        double *hfm_dev, double *hx_dev);
 /* BAV (called at DBSR_MAT3/R_data.f90:181, S_data.f90:205; defined in the
  * un-vendored Libraries_03): c(ns) = A b (dir 'r') or A^T b (dir 'l'), A the
  * (ns,ns) matrix held in storage sym 's','l','n','x'. */

@@ -863,3 +863,219 @@ def ref_vnuclear(nucleus, r):
     R.ref_vnuclear.argtypes = [C.c_double]
     R.ref_set_nucleus(*nucleus)
     return R.ref_vnuclear(r)
+
+
+# --------------------------------------------------------------------------
+# DBSR_MCHF's orbital matrices, restated with the primitives above:
+# mchf_matrix (DBSR_MCHF/mchf_matrix.f90:2-248) and core_matrix (:253-345).
+# The integral list of the reference (i1_int..i4_int, ip_int, Rk_coef with
+# the level factor WW(ic,jc), :89-95) arrives flattened: one entry per
+# angular coefficient, c = WW*coef.  The collection list id4_data lives in
+# the un-vendored Libraries_03; it is restated as a list in order of first
+# insertion whose equal keys (k, id2, id3, id4) sum their coefficients.
+# Only the Newton term depends on that order (below).
+# --------------------------------------------------------------------------
+_MCHF_BLOCKS = ((1, 1, PPPP), (1, 2, PQPQ), (2, 1, QPQP), (2, 2, QQQQ))
+
+
+def id4_lists(i, rlist, S, eps_c, kmax):
+    """per k: the (id2, id3, id4, cid) list mchf_matrix collects for orbital
+    i (mchf_matrix.f90:89-144); flags -1 direct, -2 exchange, -3 Newton, >0
+    the rhs orbital"""
+    out = []
+    for k in range(kmax + 1):
+        keys, vals = [], {}
+
+        def add(id2, id3, id4, c):
+            key = (id2, id3, id4)
+            if key in vals:
+                vals[key] += c
+            else:
+                keys.append(key)
+                vals[key] = c
+
+        for kk, i1, i2, i3, i4, c0 in rlist:
+            if kk != k:
+                continue
+            C_ = c0 / S
+            if abs(C_) < eps_c:
+                continue
+            ii = (1000 if i == i1 else 0) + (100 if i == i2 else 0) + (10 if i == i3 else 0) + (1 if i == i4 else 0)
+            if ii == 1111:
+                add(-1, i1, i3, 2 * C_); add(-3, i1, i3, 4 * C_)
+            elif ii == 1010:
+                add(-1, i2, i4, C_)
+            elif ii == 101:
+                add(-1, i1, i3, C_)
+            elif ii == 1001:
+                add(-2, i2, i3, C_)
+            elif ii == 1100:
+                add(-2, i3, i4, C_)
+            elif ii == 110:
+                add(-2, i1, i4, C_)
+            elif ii == 11:
+                add(-2, i1, i2, C_)
+            elif ii in (111, 1101):
+                add(-1, i1, i3, C_); add(-2, i1, i3, C_ / 2); add(-3, i1, i3, 3 * C_ / 2)
+            elif ii in (1011, 1110):
+                add(-1, i2, i4, C_); add(-2, i2, i4, C_ / 2); add(-3, i2, i4, 3 * C_ / 2)
+            elif ii == 1000:
+                add(i3, i2, i4, C_ / 2)
+            elif ii == 100:
+                add(i4, i1, i3, C_ / 2)
+            elif ii == 10:
+                add(i1, i2, i4, C_ / 2)
+            elif ii == 1:
+                add(i2, i1, i3, C_ / 2)
+        out.append([(a, b, c, vals[(a, b, c)]) for (a, b, c) in keys])
+    return out
+
+
+def mchf_qsum(i, llist, norb, qsum_min=1e-8):
+    """S = qsum(i) and CI(io) from the L-integral coefficients
+    (mchf_matrix.f90:59-72)"""
+    S = 0.0
+    CI = np.zeros(norb)
+    for i1, i2, c in llist:
+        if i1 == i and i2 == i:
+            S = S + c
+        elif i1 == i:
+            CI[i2 - 1] += c
+        elif i2 == i:
+            CI[i1 - 1] += c
+    if abs(S) < qsum_min:
+        S = qsum_min
+    return S, CI
+
+
+def mchf_matrix(g, i, pq, dhl, llist, rlist, kmax, eps_c=1e-8, qsum_min=1e-8, newton=0, irhs=0, fppqq=None):
+    """mchf_matrix for orbital i (1-based): llist (i1, i2, c), rlist
+    (k, i1, i2, i3, i4, c) with c = WW*coef; dhl (ms,ms) = Get_hd_core(kbs(i)).
+    Returns hfm, rhs, hx, e(i), qsum(i).
+
+    The Newton term hx of a (k, block) is, as the reference computes it
+    (mchf_matrix.f90:220-234), the case-4 convolution of the LAST collected
+    -3 density, without its coefficient: Convol's output and input are
+    xx and x there, so the accumulated xx is overwritten."""
+    ns, ks, ms = g.ns, g.ks, g.ms
+    norb = pq.shape[2]
+    hfm = np.zeros((ms, ms), order="F")
+    hx = np.zeros((ms, ms), order="F")
+    rhs = np.zeros(ms)
+    S, CI = mchf_qsum(i, llist, norb, qsum_min)
+    v_of = lambda o: np.concatenate([pq[:, 0, o - 1], pq[:, 1, o - 1]])
+    if np.sum(np.abs(CI)) != 0.0:
+        vv = np.zeros(ms)
+        for io in range(1, norb + 1):
+            if CI[io - 1] == 0.0:
+                continue
+            vv = vv + v_of(io) * CI[io - 1] / S / 2.0
+        rhs = rhs + np.asarray(dhl) @ vv
+    lists = id4_lists(i, rlist, S, eps_c, kmax)
+    for k in range(kmax + 1):
+        lst = lists[k]
+        if not lst:
+            continue
+        for ii, jj, typ in _MCHF_BLOCKS:
+            rkb = g.mrk(typ, k)
+            dd, met = np.zeros((ns, ks), order="F"), False
+            for id2, id3, id4, cid in lst:
+                if id2 != -1 or cid == 0.0:
+                    continue
+                dd = dd + cid * density(ns, ks, pq[:, jj - 1, id3 - 1], pq[:, jj - 1, id4 - 1], "s")
+                met = True
+            if met:
+                d = convol(ns, ks, rkb, dd, 1, "s", "s")
+                update_hs_lib(hfm, ii, ii, ns, ks, d, "s")
+            xx, met = np.zeros((ns, ns), order="F"), False
+            for id2, id3, id4, cid in lst:
+                if id2 != -2 or cid == 0.0:
+                    continue
+                xx = xx + cid * density(ns, ks, pq[:, ii - 1, id3 - 1], pq[:, jj - 1, id4 - 1], "x")
+                met = True
+            if met:
+                x = convol(ns, ks, rkb, xx, 4, "s", "s")
+                update_hs_lib(hfm, ii, jj, ns, ks, x, "x")
+            aa = {}
+            for id2, id3, id4, cid in lst:
+                if id2 <= 0:
+                    continue
+                d = density(ns, ks, pq[:, jj - 1, id3 - 1], pq[:, jj - 1, id4 - 1], "s")
+                aa[id2] = (aa[id2] + d * cid) if id2 in aa else (np.zeros((ns, ks), order="F") + d * cid)
+            for j in sorted(aa):
+                dd = convol(ns, ks, rkb, aa[j], 1, "s", "s")
+                y = full_mat_sym(ns, ks, dd, "s")
+                v1 = y @ pq[:, ii - 1, j - 1]
+                rhs[(ii - 1) * ns:ii * ns] = rhs[(ii - 1) * ns:ii * ns] + v1
+            if newton:
+                last = None
+                for id2, id3, id4, cid in lst:
+                    if id2 != -3 or cid == 0.0:
+                        continue
+                    last = (id3, id4)
+                if last is not None:
+                    x = density(ns, ks, pq[:, jj - 1, last[0] - 1], pq[:, jj - 1, last[1] - 1], "x")
+                    xx = convol(ns, ks, rkb, x, 4, "s", "s")
+                    update_hs_lib(hx, ii, jj, ns, ks, xx, "x")
+    srhs = np.sum(np.abs(rhs))
+    if srhs < 1e-8:
+        rhs[:] = 0.0
+        srhs = 0.0
+    if irhs > 0 and srhs > 0.0:
+        v = v_of(i)
+        B = np.asarray(fppqq)
+        SS = -np.sum(rhs * v)
+        hfm = hfm + SS * B
+        vv = B @ v
+        hfm = hfm + np.outer(rhs, vv) + np.outer(vv, rhs)
+        rhs[:] = 0.0
+    hfm = np.asfortranarray(np.asarray(dhl) + hfm)
+    rhs = -rhs
+    v = v_of(i)
+    e = float(np.dot(v, hfm @ v - rhs))
+    return hfm, rhs, hx, e, S
+
+
+def core_matrix(g, i, pq, hd, acoef, bcoef, kmax, qsum, newton=0):
+    """core_matrix for core orbital i (1-based): hd (ms,ms) = Get_hd(kbs(i)),
+    acoef/bcoef (norb, kmax+1) = a(i,j,k), b(i,j,k), qsum (norb).  Returns
+    hfm, hx."""
+    ns, ks, ms = g.ns, g.ks, g.ms
+    norb = pq.shape[2]
+    q = qsum[i - 1]
+    hfm = np.asfortranarray(q * np.asarray(hd))
+    hx = np.zeros((ms, ms), order="F")
+    for ii, jj, typ in _MCHF_BLOCKS:
+        for k in range(kmax + 1):
+            rkb = g.mrk(typ, k)
+            dd, met = np.zeros((ns, ks), order="F"), False
+            for j in range(1, norb + 1):
+                c = acoef[j - 1, k]
+                c = c * q * qsum[j - 1] if i != j else c * q * (q - 1)
+                if c == 0.0:
+                    continue
+                dd = dd + c * density(ns, ks, pq[:, jj - 1, j - 1], pq[:, jj - 1, j - 1], "s")
+                met = True
+            if met:
+                d = convol(ns, ks, rkb, dd, 1, "s", "s")
+                update_hs_lib(hfm, ii, ii, ns, ks, d, "s")
+            xx, met = np.zeros((ns, ns), order="F"), False
+            for j in range(1, norb + 1):
+                c = bcoef[j - 1, k]
+                c = c * q * qsum[j - 1] if i != j else c * q * (q - 1)
+                if c == 0.0:
+                    continue
+                xx = xx + density(ns, ks, pq[:, ii - 1, j - 1], pq[:, jj - 1, j - 1], "x") * c
+                met = True
+            if met:
+                x = convol(ns, ks, rkb, xx, 4, "s", "s")
+                update_hs_lib(hfm, ii, jj, ns, ks, x, "x")
+            if newton:
+                c = 2.0 * acoef[i - 1, k]
+                if c == 0.0:
+                    continue
+                c = c * q * (q - 1)
+                x = density(ns, ks, pq[:, ii - 1, i - 1], pq[:, jj - 1, i - 1], "x")
+                x = convol(ns, ks, rkb, c * x, 4, "s", "s")
+                update_hs_lib(hx, ii, jj, ns, ks, x, "x")
+    return np.asfortranarray(hfm / q), np.asfortranarray(hx / q)
