@@ -98,3 +98,35 @@ def test_core_matrix_vs_oracle(he, newton):
             assert close(device_to_fortran(hx), x0)
         else:
             assert not np.any(device_to_fortran(hx))
+
+
+def test_df_integrals_vs_oracle(he):
+    """rk_df and its single-type, S^k and hf_fk / hf_gk relatives as one
+    request set, against the oracle's densities, convolutions and SUM_AmB in
+    the reference's order"""
+    from dbsr3_amd.mchf import DfIntegrals
+    from test_gpu_engine import oracle_values
+    d, g, pq, pqd, kbs, nuc = he
+    n = pq.shape[2]
+    rng = np.random.default_rng(8)
+    names = ["rk_df", "rk_pppp_df", "rk_qqqq_df", "rk_pqpq_df", "rk_qpqp_df", "sk_ppqq_df", "sk_pqqp_df"]
+    reqs = []
+    for nm in names:
+        for _ in range(12):
+            a, b, c, e = (int(x) for x in rng.integers(1, n + 1, 4))
+            reqs.append((nm, a, b, c, e, int(rng.integers(0, 4))))
+    for _ in range(10):
+        i, j = (int(x) for x in rng.integers(1, n + 1, 2))
+        k = int(rng.integers(0, 4))
+        reqs += [("hf_fk", i, j, k), ("hf_gk", i, j, k)]
+    got = DfIntegrals(d, n, reqs).values(pq)
+    fs = [np.asfortranarray(pq[:, :, j]) for j in range(n)]
+    for q, r in enumerate(reqs):
+        if r[0] == "hf_fk":
+            ref = O.hf_fk(g, fs, r[1] - 1, r[2] - 1, r[3])
+        elif r[0] == "hf_gk":
+            ref = O.hf_gk(g, fs, r[1] - 1, r[2] - 1, r[3])
+        else:
+            kind = names.index(r[0])
+            ref = oracle_values(g, fs, [(r[5], r[1], r[2], r[3], r[4])], kind)[0]
+        assert abs(got[q] - ref) <= 1e-12 * max(abs(ref), 1e-3 * np.max(np.abs(got))), r
