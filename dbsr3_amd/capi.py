@@ -45,6 +45,15 @@ class Mat3AuxC(C.Structure):
 auxp = C.POINTER(Mat3AuxC)
 
 
+class LDataAuxC(C.Structure):
+    """struct dbs_ldata_aux (L_data's one-electron inputs)"""
+    _fields_ = [("mat3", auxp), ("otarg", dp), ("check_target", i32), ("norb", i32), ("lval", dp),
+                ("kbs", ip), ("nkap", i32), ("kap_list", ip), ("hd_dev", vp)]
+
+
+ldp = C.POINTER(LDataAuxC)
+
+
 class NucleusC(C.Structure):
     """dbs_nucleus: model 0 point, 1 uniform, 2 Fermi; z; r_uniform, a_fermi, c_fermi (a.u.)"""
     _fields_ = [("model", i32), ("z", f64), ("r_uniform", f64), ("a_fermi", f64), ("c_fermi", f64)]
@@ -116,6 +125,7 @@ PROTOTYPES = {
     "dbs_o_data_dev": [vp, mbp, i32, i32, ip, ip, dp, f64, i32, ip, KBORT_FN, vp, vp, i32, vp, i32,
                        ip, dp],
     "dbs_overlap_ci_dev": [vp, i32, ip, ip, dp, f64, i32, vp, i32],
+    "dbs_ldata_dev": [vp, mbp, ldp, i32, i32, ip, ip, ip, ip, dp, f64, vp],
     "dbs_rdata_dev": [vp, mbp, auxp, i32, i32, i32, i32, dp, ip, ip, ip, ip, vp, i32],
     "dbs_sdata_dev": [vp, mbp, auxp, i32, i32, i32, dp, ip, ip, ip, ip, vp, i32],
     "dbs_bav_dev": [vp, vp, vp, ch, ch, vp],

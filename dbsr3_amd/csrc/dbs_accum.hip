@@ -719,6 +719,45 @@ int idef_otype(double eps_c, int check_target, double C, int ic, int jc, int io,
   return itype;
 }
 
+// Idef_Ltype (idef_type.f90:54-138): one-electron structures 1..3 with
+// key (k1,k2,k3,k4); 0 = dropped (|C| < eps_C, or a target term with
+// check_target = 0)
+template <class IPB>
+int idef_ltype(const IPB& ipb, double eps_c, int check_target, int ii, int jj, double C, int ic, int jc, int io,
+               int jo, int* key) {
+  if (std::fabs(C) < eps_c) return 0;
+  if (jc > 0 && ic <= 0) return -fail(DBS_ERR_ARG, "jc>0, ic<=0");
+  if (jo > 0 && io <= 0) return -fail(DBS_ERR_ARG, "jo>0, io<=0");
+  if (io > 0 && jc != 0) return -fail(DBS_ERR_ARG, "jc>0, io>0");
+  if (jo > 0 && ic != 0) return -fail(DBS_ERR_ARG, "jc>0, jo>0");
+  int itype = 0, k1 = 0, k2 = 0, k3 = 0, k4 = 0;
+  const int i = std::min(ii, jj), j = std::max(ii, jj), ich = ipb(i), jch = ipb(j);
+  if (jc > 0) {
+    k1 = i; k2 = j; k3 = ic; k4 = jc; itype = 1;
+  } else if (ic > 0) {
+    if (io == 0) {
+      if (ich != 0 && jch == 0) { k1 = j; k2 = ich; k3 = ic; k4 = 0; itype = 2; }
+      else if (ich == 0 && jch != 0) { k1 = i; k2 = jch; k3 = ic; k4 = 0; itype = 2; }
+    } else if (io > 0) {
+      k1 = i; k2 = j; k3 = -io; k4 = ic; itype = 1;
+    }
+  } else if (io == 0) {
+    k1 = j; k2 = i; itype = 3;
+  } else if (io > 0 && jo == 0) {
+    if (ich != 0 && jch == 0) { k1 = j; k2 = ich; k3 = -io; itype = 2; }
+    else if (jch != 0 && ich == 0) { k1 = i; k2 = jch; k3 = -io; itype = 2; }
+    else if (ich == 0 && jch == 0) {
+      k1 = i; k2 = j; k3 = -io; itype = 1;
+      if (check_target == 0) return 0;
+    }
+  } else if (io > 0 && jo > 0) {
+    k1 = i; k2 = j; k3 = -io; k4 = -jo; itype = 1;
+  }
+  if (itype == 0) return -fail(DBS_ERR_ARG, "Idef_Ltype: itype=0");
+  key[0] = k1; key[1] = k2; key[2] = k3; key[3] = k4;
+  return itype;
+}
+
 }  // namespace
 
 extern "C" {
@@ -746,8 +785,11 @@ int dbs_add_integral_mat3(dbs_cdata* cd, const dbs_mat3_orbs* o, int icase, int 
       t = idef_otype(o->eps_c, o->check_target, C, ic, jc, io, jo, key);
       if (t <= 0) return -t;
       return dbs_cdata_add_coef(cd, C, 0, key[0], key[1], key[2], key[3], t);
-    case 1:
-      return fail(DBS_ERR_ARG, "Add_integral: L-integrals (one-electron) are not part of this library");
+    case 1:  // L-integrals (add_integral.f90:24-26): Idef_Ltype of (i1, i3)
+      for (int q : {i1, i3}) DBS_CHECK(q >= 1 && q <= o->norb, DBS_ERR_ARG, "Add_integral: orbital index out of range");
+      t = idef_ltype(ipb, o->eps_c, o->check_target, i1, i3, C, ic, jc, io, jo, key);
+      if (t <= 0) return -t;
+      return dbs_cdata_add_coef(cd, C, 0, key[0], key[1], key[2], key[3], t);
     case 2: {  // R-integrals: the parity rule, then Idef_Rtype
       DBS_CHECK(o->lbs, DBS_ERR_ARG, "Add_integral: lbs missing");
       for (int q : {i1, i2, i3, i4}) DBS_CHECK(q >= 1 && q <= o->norb, DBS_ERR_ARG, "Add_integral: orbital index out of range");

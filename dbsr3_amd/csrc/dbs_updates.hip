@@ -192,6 +192,23 @@ __global__ void k_vec_list(int ms, int ntarget, const int* __restrict__ tcol, co
   }
 }
 
+// Gen_Lvec for a list: W(:,q) = hd(kappa index qv.x) (P,Q)(orbital qv.y),
+// columns summed in order (MATMUL)
+__global__ void k_lvec(int ms, int nq, const int2* __restrict__ qv, const double* __restrict__ hd,
+                       const double* __restrict__ pq, double* __restrict__ W) {
+#pragma clang fp contract(off)
+  const long long total = (long long)ms * nq;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int q = (int)(e / ms), r = (int)(e % ms);
+    const double* A = hd + (long long)ms * ms * qv[q].x;
+    const double* v = pq + (long long)ms * (qv[q].y - 1);
+    double acc = 0.0;
+    for (int c = 0; c < ms; ++c) acc += A[r + (long long)ms * c] * v[c];
+    W[e] = acc;
+  }
+}
+
 namespace {
 
 unsigned grid_n(long long n, int block) {
@@ -728,6 +745,163 @@ int dbs_o_data_dev(dbs_ctx* ctx, const dbs_mat_blocks* m, int itype, int n, cons
     }
   }
   return apply_mat3(ctx, m, u, vch_dev, ctx->ms, nullptr, fppqq_dev, "O_data");
+}
+
+int dbs_ldata_dev(dbs_ctx* ctx, const dbs_mat_blocks* m, const dbs_ldata_aux* aux, int itype, int n, const int* k1,
+                  const int* k2, const int* k3, const int* k4, const double* c, double eps_c, const double* pq_dev) {
+  DBS_CHECK(ctx && aux && aux->mat3 && (n <= 0 || (k1 && k2 && k3 && k4 && c)), DBS_ERR_ARG,
+            "L_data: null argument");
+  int rc = check_blocks(m);
+  if (rc || n <= 0) return rc;
+  DBS_CHECK(itype >= 1 && itype <= 3, DBS_ERR_ARG, " L_data: uknown itype ");
+  const dbs_mat3_aux* X = aux->mat3;
+  const int ibo = X->ibo, norb = aux->norb;
+  auto ipb = [&](int o) -> int { return X->ipbs[o - 1]; };
+  auto orb_ok = [&](int o) { return o >= 1 && o <= norb; };
+  // io = i1*ibo + i2 -> column KBORT(i1,i2)-1 of V_ch (L_data.f90:58-60)
+  auto vcol = [&](int io, int* col, int* i1) -> int {
+    DBS_CHECK(ibo > 0 && X->ipbs && X->kbort && X->vch_dev, DBS_ERR_ARG, "L_data: bound-orbital tables missing");
+    *i1 = io / ibo;
+    DBS_CHECK(orb_ok(*i1), DBS_ERR_ARG, "L_data: orbital index out of range");
+    const int k = X->kbort(X->kbort_user, io / ibo, io % ibo);
+    if (k == 0) return fail(DBS_ERR_ARG, "O: KBORT=0");
+    DBS_CHECK(k >= 1 && k <= X->nvch, DBS_ERR_ARG, "L_data: KBORT column out of range");
+    *col = k - 1;
+    return DBS_OK;
+  };
+  auto kap_index = [&](int o, int* q) -> int {  // Ipointer(nkap, kap_list, kbs(o))
+    DBS_CHECK(aux->kbs && aux->kap_list && aux->hd_dev, DBS_ERR_ARG, "L_data: one-electron operators missing");
+    for (int t = 0; t < aux->nkap; ++t)
+      if (aux->kap_list[t] == aux->kbs[o - 1]) { *q = t; return DBS_OK; }
+    *q = -1;
+    return DBS_OK;
+  };
+  std::vector<MatUpd> u;
+  if (itype == 1) {
+    DBS_CHECK(aux->lval, DBS_ERR_ARG, "L_data: Lval table missing");
+    for (int e = 0; e < n; ++e) {
+      double C = c[e];
+      if (std::fabs(C) < eps_c) continue;
+      DBS_CHECK(orb_ok(k1[e]) && orb_ok(k2[e]), DBS_ERR_ARG, "L_data: orbital index out of range");
+      C = C * aux->lval[(k1[e] - 1) + (size_t)norb * (k2[e] - 1)];
+      const int ic = k3[e], jc = k4[e];
+      if (ic > 0 && jc > 0) {  // L(..) ic,jc
+        long long t = -1;
+        if ((rc = hb_target(m, ic, jc, &t))) return rc;
+        u.push_back({MatUpd::HB, 0, 0, 0, t, C});
+      } else if (ic < 0 && jc > 0) {  // L(..) <i|.> jc
+        int vc = 0, i1 = 0, hc = 0;
+        if ((rc = vcol(-ic, &vc, &i1))) return rc;
+        if ((rc = hv_column(m, ipb(i1), jc, &hc))) return rc;
+        u.push_back({MatUpd::HV, 0, vc, 0, (long long)hc - 1, C});
+      } else if (ic < 0 && jc < 0) {  // L(..) <i|.> <.|j>
+        int vi = 0, i1 = 0, wj = 0, j1 = 0, b = 0;
+        if ((rc = vcol(-ic, &vi, &i1)) || (rc = vcol(-jc, &wj, &j1))) return rc;
+        const int ich = ipb(i1), jch = ipb(j1);
+        if ((rc = chan_block(m, ich, jch, "UPDATE_HW", &b))) return rc;
+        u.push_back({MatUpd::HW, ich > jch ? 0 : (jch > ich ? 1 : 2), vi, wj, (long long)b - 1, C});
+      } else if (ic < 0 && jc == 0) {  // L(..) <i|j>
+        DBS_CHECK(ibo > 0 && X->ipbs, DBS_ERR_ARG, "L_data: bound-orbital tables missing");
+        const int io = -ic, i1 = io / ibo, i2 = io % ibo;
+        DBS_CHECK(orb_ok(i1) && orb_ok(i2), DBS_ERR_ARG, "L_data: orbital index out of range");
+        const int ich = ipb(i1), jch = ipb(i2);
+        if (X->iitar > 0 && ich != jch) {
+          int b = 0;
+          if ((rc = chan_block(m, ich, jch, "UPDATE_HX", &b))) return rc;
+          u.push_back({MatUpd::HX, ich >= jch ? 0 : 1, 0, 0, (long long)b - 1, C});
+        }
+        if ((rc = target_h(m, ich, jch, X->iptar, X->htarg, C))) return rc;
+      } else {
+        return fail(DBS_ERR_ARG, "L_data:  wrong combination of ic,jc for itype=1");
+      }
+    }
+    return apply_mat3(ctx, m, u, X->vch_dev, ctx->ms, nullptr, X->fppqq_dev, "L_data");
+  }
+  if (itype == 2) {
+    // Gen_Lvec(i1, v): v = hd(kappa of i1) (P,Q)(i1), one column per orbital
+    DBS_CHECK(pq_dev, DBS_ERR_ARG, "L_data: orbitals missing");
+    std::vector<int> orbs;
+    std::vector<int2> qv;
+    std::vector<int> col_of(norb + 1, -1);
+    for (int e = 0; e < n; ++e) {
+      if (std::fabs(c[e]) < eps_c) continue;
+      const int i1 = k1[e];
+      DBS_CHECK(orb_ok(i1), DBS_ERR_ARG, "L_data: orbital index out of range");
+      if (col_of[i1] >= 0) continue;
+      int q = -1;
+      if ((rc = kap_index(i1, &q))) return rc;
+      DBS_CHECK(q >= 0, DBS_ERR_ARG, "L_data: unknown kappa, itype=2");
+      col_of[i1] = (int)qv.size();
+      qv.push_back(make_int2(q, i1));
+    }
+    for (int e = 0; e < n; ++e) {
+      const double C = c[e];
+      if (std::fabs(C) < eps_c) continue;
+      const int ich = k2[e], jc = k3[e], wq = -(col_of[k1[e]] + 1);
+      if (jc > 0) {  // L(.i) jc
+        int hc = 0;
+        if ((rc = hv_column(m, ich, jc, &hc))) return rc;
+        u.push_back({MatUpd::HV, 0, wq, 0, (long long)hc - 1, C});
+      } else if (jc < 0) {  // L(.i) <.|j>
+        int wj = 0, j1 = 0, b = 0;
+        if ((rc = vcol(-jc, &wj, &j1))) return rc;
+        const int jch = ipb(j1);
+        if ((rc = chan_block(m, ich, jch, "UPDATE_HW", &b))) return rc;
+        u.push_back({MatUpd::HW, ich > jch ? 0 : (jch > ich ? 1 : 2), wq, wj, (long long)b - 1, C});
+      } else {
+        return fail(DBS_ERR_ARG, "L_data:  wrong value jc = 0 for itype=2");
+      }
+    }
+    if (u.empty()) return DBS_OK;
+    double* W = nullptr;
+    if (hipMalloc(&W, sizeof(double) * (size_t)ctx->ms * qv.size()) != hipSuccess)
+      return fail(DBS_ERR_HIP, "L_data: hipMalloc");
+    int2* d_qv = nullptr;
+    if (hipMalloc(&d_qv, sizeof(int2) * qv.size()) != hipSuccess) {
+      (void)hipFree(W);
+      return fail(DBS_ERR_HIP, "L_data: hipMalloc");
+    }
+    hipStream_t s = ctx->stream;
+    rc = DBS_OK;
+    if (hipMemcpyAsync(d_qv, qv.data(), sizeof(int2) * qv.size(), hipMemcpyHostToDevice, s) != hipSuccess)
+      rc = fail(DBS_ERR_HIP, "L_data: hipMemcpyAsync");
+    if (!rc) {
+      k_lvec<<<grid_n((long long)ctx->ms * qv.size(), 256), 256, 0, s>>>(ctx->ms, (int)qv.size(), d_qv, aux->hd_dev,
+                                                                          pq_dev, W);
+      rc = apply_mat3(ctx, m, u, X->vch_dev, ctx->ms, W, X->fppqq_dev, "L_data");
+    }
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(W);
+    (void)hipFree(d_qv);
+    return rc;
+  }
+  // itype 3: L(ij) -> UPDATE_HX(ich,jch,hd(kappa)) or the target overlap;
+  // one apply per kappa (a block's channels fix its kappa)
+  std::vector<std::vector<MatUpd>> per(aux->nkap);
+  for (int e = 0; e < n; ++e) {
+    const double C = c[e];
+    if (std::fabs(C) < eps_c) continue;
+    const int i1 = k1[e], j1 = k2[e];
+    DBS_CHECK(orb_ok(i1) && orb_ok(j1) && X->ipbs, DBS_ERR_ARG, "L_data: orbital index out of range");
+    const int ich = ipb(i1), jch = ipb(j1);
+    DBS_CHECK(aux->kbs, DBS_ERR_ARG, "L_data: one-electron operators missing");
+    if (aux->kbs[i1 - 1] != aux->kbs[j1 - 1]) return fail(DBS_ERR_ARG, "L_data: different kappa, itype=3");
+    int q = -1;
+    if ((rc = kap_index(i1, &q))) return rc;
+    if (q < 0) return fail(DBS_ERR_ARG, "L_data: unknown kappa, itype=3");
+    if (ich == jch || X->iitar > 1) {
+      int b = 0;
+      if ((rc = chan_block(m, ich, jch, "UPDATE_HX", &b))) return rc;
+      per[q].push_back({MatUpd::HX, ich >= jch ? 0 : 1, 0, 0, (long long)b - 1, C});
+    } else if (aux->check_target == 1) {
+      if ((rc = target_h(m, ich, jch, X->iptar, aux->otarg, C))) return rc;
+    }
+  }
+  const size_t mm = (size_t)ctx->ms * ctx->ms;
+  for (int q = 0; q < aux->nkap; ++q)
+    if (!per[q].empty() && (rc = apply_mat3(ctx, m, per[q], nullptr, ctx->ms, nullptr, aux->hd_dev + mm * q, "L_data")))
+      return rc;
+  return DBS_OK;
 }
 
 int dbs_overlap_ci_dev(dbs_ctx* ctx, int n, const int* ic, const int* jc, const double* cc, double eps_c,

@@ -133,6 +133,32 @@ class MatBlocks:
                    otarg.ctypes.data_as(_dp) if otarg is not None else None)
 
 
+    # -- L_data.f90 --------------------------------------------------------
+    def L_data(self, itype, lst, norb, lval, kbs, kap_list, hd, pq=None, eps_c=1e-10, ibo=2 ** 15, ipbs=None,
+               kbort=None, vch=None, fppqq=None, iitar=0, iptar=None, htarg=None, otarg=None, check_target=1):
+        """L_data(itype) of one merged list lst = (c, k1, k2, k3, k4) in IPT
+        order.  lval: (norb,norb) host table of Lval(i,j); kbs: kappa of each
+        orbital; hd: device tensor (nkap, ms, ms) holding hd(ms,ms,nkap) of
+        kap_list; pq: device tensor holding pq(ns,2,norb) (itype 2);
+        htarg/otarg: packed target matrices (numpy), updated in place."""
+        from .capi import LDataAuxC
+        c, k1, k2, k3, k4 = lst
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        K = [_i32(x) for x in (k1, k2, k3, k4)]
+        aux = self._aux(ibo, ipbs, kbort, vch, fppqq, iitar, iptar, htarg)
+        lv = np.asfortranarray(lval, dtype=np.float64)
+        kb, kl = _i32(kbs), _i32(kap_list)
+        if otarg is not None:
+            assert otarg.dtype == np.float64 and otarg.flags.c_contiguous
+        la = LDataAuxC(C.pointer(aux), otarg.ctypes.data_as(_dp) if otarg is not None else None, check_target,
+                       int(norb), lv.ctypes.data_as(_dp), kb.ctypes.data_as(_ip), len(kl), kl.ctypes.data_as(_ip),
+                       _vp(hd))
+        self._keep += [lv, kb, kl, aux]
+        self._pre()
+        check(lib().dbs_ldata_dev(self.dbs.ctx, C.byref(self._desc), C.byref(la), itype, len(c),
+                                  *[x.ctypes.data_as(_ip) for x in K], c.ctypes.data_as(_dp), eps_c, _vp(pq)))
+        self._post()
+
     # -- R_data.f90 / S_data.f90 ----------------------------------------------
     def _aux(self, ibo, ipbs, kbort, vch, fppqq, iitar, iptar, htarg, quadr_pq=None):
         self._keep = []

@@ -173,11 +173,12 @@ typedef struct dbs_mat3_orbs {
   int check_target;
 } dbs_mat3_orbs;
 /* Add_integral, MAT3 form (DBSR_MAT3/add_integral.f90:2-57): icase 0
- * overlaps -> Idef_Otype; 2 R-integrals -> parity rule, Idef_Rtype (all 16
- * structures with ic, jc, io, jo); 3 S-integrals -> SMU and Idef_itype for
- * the eight permuted terms of multipoles kpol-1..kpol+1 (atype 0 / 1,
- * itype + 10*atype) (DBSR_MAT3/idef_type.f90:2-437).  icase 1 (one-electron
- * L-integrals) returns an error. */
+ * overlaps -> Idef_Otype; 1 L-integrals -> Idef_Ltype of (i1, i3)
+ * (idef_type.f90:54-138, structures 1..3 with multipole 0); 2 R-integrals
+ * -> parity rule, Idef_Rtype (all 16 structures with ic, jc, io, jo); 3
+ * S-integrals -> SMU and Idef_itype for the eight permuted terms of
+ * multipoles kpol-1..kpol+1 (atype 0 / 1, itype + 10*atype)
+ * (DBSR_MAT3/idef_type.f90:2-437). */
 int dbs_add_integral_mat3(dbs_cdata *cd, const dbs_mat3_orbs *o, int icase, int kpol, int i1, int i2,
                           int i3, int i4, double C, int ic, int jc, int io, int jo);
 /* Decode_int (DBSR_CI3/setup_matrix.f90:206-223) for n packed int_bnk codes */
@@ -349,6 +350,37 @@ int dbs_rdata_dev(dbs_ctx *ctx, const dbs_mat_blocks *m, const dbs_mat3_aux *aux
 int dbs_sdata_dev(dbs_ctx *ctx, const dbs_mat_blocks *m, const dbs_mat3_aux *aux, int ktype, int k,
                   int nc, const double *c, const int *K1, const int *K2, const int *K3, const int *K4,
                   const double *pq_dev, int norb);
+/* L_data (DBSR_MAT3/L_data.f90:2-145) inputs beyond the MAT3 tables: the
+ * target overlaps otarg (host, may be NULL; Target_h's CC), check_target,
+ * Lval(i,j) of bound orbitals as a host table lval(norb,norb) (Lval is in
+ * the un-vendored Libraries_03), the orbitals' kappas kbs(norb), and the
+ * one-electron operators hd_dev(ms,ms,nkap) of kap_list(nkap) (module
+ * dhl_core; Gen_dhl_core, DBSR_MAT3/sub1.f90:23). */
+typedef struct dbs_ldata_aux {
+  const dbs_mat3_aux *mat3;
+  double *otarg;
+  int check_target;
+  int norb;
+  const double *lval;
+  const int *kbs;
+  int nkap;
+  const int *kap_list;
+  const double *hd_dev;
+} dbs_ldata_aux;
+/* L_data(itype) (DBSR_MAT3/L_data.f90:2-145): one merged list (IPT order,
+ * host arrays) of L-integral structure itype into the MAT3 matrices.
+ * 1: C*Lval(k1,k2) into UPDATE_HB (k3,k4 > 0), UPDATE_HV (k3 < 0, k4 > 0),
+ *    UPDATE_HW (k3, k4 < 0), or htarg and, for iitar > 0 and ich != jch,
+ *    UPDATE_HX(fppqq) (k3 < 0, k4 = 0);
+ * 2: Gen_Lvec(k1) = hd(kappa of k1) (P,Q)(k1) into UPDATE_HV(k2, k3) (k3 >
+ *    0) or UPDATE_HW(k2, ipbs(j1), ., V_ch(KBORT)) (k3 < 0);
+ * 3: UPDATE_HX(ipbs(k1), ipbs(k2), hd(kappa)) when the channels are equal or
+ *    iitar > 1, else the target overlap (check_target = 1).
+ * Entries with |c| < eps_c are skipped; updates of one element in list
+ * order.  pq_dev(ns,2,norb) (itype 2). */
+int dbs_ldata_dev(dbs_ctx *ctx, const dbs_mat_blocks *m, const dbs_ldata_aux *aux, int itype, int n,
+                  const int *k1, const int *k2, const int *k3, const int *k4, const double *c, double eps_c,
+                  const double *pq_dev);
 /* hf_matrix, two-electron part (DBSR_HF/hf_matrix.f90:2-78 with
  * mrk_aaaa :82-92; Breit block hf_matrix_breit :97-163 when mbreit == 2):
  * for orbital i (1-based) of pq_dev(ns,2,norb), with acoef/bcoef(norb,

@@ -1079,3 +1079,152 @@ def core_matrix(g, i, pq, hd, acoef, bcoef, kmax, qsum, newton=0):
                 x = convol(ns, ks, rkb, c * x, 4, "s", "s")
                 update_hs_lib(hx, ii, jj, ns, ks, x, "x")
     return np.asfortranarray(hfm / q), np.asfortranarray(hx / q)
+
+
+# --------------------------------------------------------------------------
+# One-electron (L) integrals of DBSR_MAT3, restated: Idef_Ltype
+# (DBSR_MAT3/idef_type.f90:54-138) and L_data (DBSR_MAT3/L_data.f90:2-145).
+# Lval and Gen_Lvec are in the un-vendored Libraries_03: Lval comes as a
+# table lval[i-1, j-1]; Gen_Lvec(i, v) is restated as v = hd(kappa of i)
+# (P_i, Q_i), the one-electron operator of the orbital's kappa applied to it.
+# Tables follow o_data's conventions: ipbs / iptar 1-based (element 0 unused).
+# --------------------------------------------------------------------------
+def idef_ltype(ii, jj, C_, ic, jc, io, jo, ipbs, eps_c=1e-10, check_target=1):
+    """(itype, (k1, k2, k3, k4)), itype 0 when dropped; raises on the
+    reference's Stop conditions"""
+    if abs(C_) < eps_c:
+        return 0, None
+    if jc > 0 and ic <= 0:
+        raise ValueError("jc>0, ic<=0")
+    if jo > 0 and io <= 0:
+        raise ValueError("jo>0, io<=0")
+    if io > 0 and jc != 0:
+        raise ValueError("jc>0, io>0")
+    if jo > 0 and ic != 0:
+        raise ValueError("jc>0, jo>0")
+    i, j = min(ii, jj), max(ii, jj)
+    ich, jch = ipbs[i], ipbs[j]
+    t, key = 0, None
+    if jc > 0:
+        t, key = 1, (i, j, ic, jc)
+    elif ic > 0:
+        if io == 0:
+            if ich != 0 and jch == 0:
+                t, key = 2, (j, ich, ic, 0)
+            elif ich == 0 and jch != 0:
+                t, key = 2, (i, jch, ic, 0)
+        elif io > 0:
+            t, key = 1, (i, j, -io, ic)
+    elif io == 0:
+        t, key = 3, (j, i, 0, 0)
+    elif io > 0 and jo == 0:
+        if ich != 0 and jch == 0:
+            t, key = 2, (j, ich, -io, 0)
+        elif jch != 0 and ich == 0:
+            t, key = 2, (i, jch, -io, 0)
+        elif ich == 0 and jch == 0:
+            t, key = 1, (i, j, -io, 0)
+            if check_target == 0:
+                return 0, None
+    elif io > 0 and jo > 0:
+        t, key = 1, (i, j, -io, -jo)
+    if t == 0:
+        raise ValueError("Idef_Ltype: itype=0")
+    return t, key
+
+
+def _target(mats, iptar, tgt, ich, jch, C_):
+    if mats["icc"][ich - 1, jch - 1] == 0 or tgt is None:
+        return
+    it, jt = iptar[ich], iptar[jch]
+    hi, lo = max(it, jt), min(it, jt)
+    tgt[(hi - 1) * hi // 2 + lo - 1] += C_
+
+
+def l_data(ms, itype, lst, eps_c, ibo, ipbs, kbort, vch, fppqq, iitar, iptar, htarg, otarg, check_target,
+           lval, kbs, kap_list, hd, pq, mats):
+    """L_data(itype) of lst = (c, k1, k2, k3, k4) in IPT order into mats =
+    dict(icc, hch, icb, hcp, ibb, hp) (updated in place); hd (ms,ms,nkap);
+    pq (ns,2,norb); kbs 1-based (element 0 unused)"""
+    c, k1, k2, k3, k4 = lst
+    hch, hcp, hp = mats["hch"], mats["hcp"], mats["hp"]
+    icc, icb, ibb = mats["icc"], mats["icb"], mats["ibb"]
+
+    def blk_add(ich, jch, x):
+        b = icc[ich - 1, jch - 1]
+        if b:
+            hch[:, :, b - 1] += x
+
+    def hx(ich, jch, d, C_):
+        blk_add(ich, jch, C_ * (d if ich >= jch else d.T))
+
+    def hw(ich, jch, v, w, C_):
+        x = np.outer(v, w)
+        blk_add(ich, jch, C_ * x if ich > jch else (C_ * x.T if jch > ich else (C_ / 2) * (x + x.T)))
+
+    def hv(ich, ip, v, C_):
+        col = icb[ich - 1, ip - 1]
+        if col:
+            hcp[:, col - 1] += C_ * v
+
+    def vcol(io):
+        i1, i2 = divmod(int(io), ibo)
+        k = kbort(i1, i2)
+        if k == 0:
+            raise ValueError("O: KBORT=0")
+        return i1, vch[:, k - 1]
+
+    def kap(o):
+        return list(kap_list).index(kbs[o]) if kbs[o] in list(kap_list) else -1
+
+    for e in range(len(c)):
+        C_ = c[e]
+        if abs(C_) < eps_c:
+            continue
+        if itype == 1:
+            i1, i2, ic, jc = int(k1[e]), int(k2[e]), int(k3[e]), int(k4[e])
+            C_ = C_ * lval[i1 - 1, i2 - 1]
+            if ic > 0 and jc > 0:
+                b = ibb[ic - 1, jc - 1]
+                if b:
+                    hp[b - 1] += C_
+            elif ic < 0 < jc:
+                o1, v = vcol(-ic)
+                hv(ipbs[o1], jc, v, C_)
+            elif ic < 0 and jc < 0:
+                o1, v = vcol(-ic)
+                o2, w = vcol(-jc)
+                hw(ipbs[o1], ipbs[o2], v, w, C_)
+            elif ic < 0 and jc == 0:
+                o1, o2 = divmod(-ic, ibo)
+                ich, jch = ipbs[o1], ipbs[o2]
+                if iitar > 0 and ich != jch:
+                    hx(ich, jch, fppqq, C_)
+                _target(mats, iptar, htarg, ich, jch, C_)
+            else:
+                raise ValueError("L_data:  wrong combination of ic,jc for itype=1")
+        elif itype == 2:
+            o1, ich, jc = int(k1[e]), int(k2[e]), int(k3[e])
+            q = kap(o1)
+            if q < 0:
+                raise ValueError("L_data: unknown kappa, itype=2")
+            v = hd[:, :, q] @ np.concatenate([pq[:, 0, o1 - 1], pq[:, 1, o1 - 1]])
+            if jc > 0:
+                hv(ich, jc, v, C_)
+            elif jc < 0:
+                j1, w = vcol(-jc)
+                hw(ich, ipbs[j1], v, w, C_)
+            else:
+                raise ValueError("L_data:  wrong value jc = 0 for itype=2")
+        else:
+            o1, o2 = int(k1[e]), int(k2[e])
+            ich, jch = ipbs[o1], ipbs[o2]
+            if kbs[o1] != kbs[o2]:
+                raise ValueError("L_data: different kappa, itype=3")
+            q = kap(o1)
+            if q < 0:
+                raise ValueError("L_data: unknown kappa, itype=3")
+            if ich == jch or iitar > 1:
+                hx(ich, jch, hd[:, :, q], C_)
+            elif check_target == 1:
+                _target(mats, iptar, otarg, ich, jch, C_)
