@@ -743,13 +743,15 @@ __device__ inline void pair_entries(const AdjEnt* __restrict__ ent, long long eb
   }
 }
 
-// Pair panels for groups of up to T*DPT densities (thread t owns densities
-// t + q blockDim, q < DPT; T = 512 threads, 1024 at W = 4; DPT = 2 for the
-// narrow panels W = 8, 4 of groups up to 1024 or 2048 densities), the
-// panels of all the group's densities in LDS (E*ndens doubles).  Each
-// thread loads its densities' panels coalesced in element order, permutes
-// them into its lane order in registers and stores them as pairs; the
-// gradient is permuted back before its coalesced store.
+// Pair panels for groups whose panels all fit in LDS (E*ndens doubles).  A
+// workgroup owns the densities [o0, o1) of the group (thread t owns o0 + t
+// + q blockDim, q < DPT; T = 512 threads, 1024 at W = 4; DPT = 2 for the
+// narrow panels W = 8, 4 of owner ranges up to 1024 or 2048 densities) and
+// stages the panels of all the group's densities: owner ranges split a
+// group's rows between workgroups that read the same panels (from L2 when
+// they run on one XCD), so a workgroup can take more panels, and a row
+// fewer partial values, at the same number of workgroups.  The gradient is
+// permuted back to element order before its coalesced store.
 // workgroup size limit of the kernel: W = 4 panels leave room for 1024
 // threads (more waves to hide the adjacency loads of narrow panels)
 template <int W>
@@ -770,15 +772,15 @@ __global__ void __launch_bounds__(rot_threads_max<W>()) k_pair_rot(
     const KGroupDev Gp = kgs[Wk.g];
     const int nv = inst[Gp.inst].nv, nd = Gp.ndens;
     const double* Fg = F + Gp.off_F;
-    // the thread's densities X = threadIdx.x + q blockDim.x, q < DPT (one
-    // partner chunk: segment chunk0 + X/64)
+    // the thread's densities X = o0 + threadIdx.x + q blockDim.x < o1, q <
+    // DPT (one partner chunk: segment chunk0 + X/64)
     bool act[DPT];
     int n[DPT];
     long long eb[DPT];
 #pragma unroll
     for (int q = 0; q < DPT; ++q) {
-      const int X = threadIdx.x + q * blockDim.x;
-      act[q] = X < nd;
+      const int X = Wk.o0 + threadIdx.x + q * blockDim.x;
+      act[q] = X < Wk.o1;
       n[q] = 0;
       eb[q] = 0;
       if (act[q]) {
@@ -793,23 +795,38 @@ __global__ void __launch_bounds__(rot_threads_max<W>()) k_pair_rot(
       const int nj = min(W, Wk.j1 - j0);
       // the previous panel is no longer read
       __syncthreads();
+      // every density of the group is a partner: the workgroup stages all
+      // their panels (rows read coalesced in element order, permuted into
+      // the staging lane's order, stored as pairs: a row's layout in LDS
+      // is element order whichever lane stores it)
+      for (int X = threadIdx.x; X < nd; X += blockDim.x) {
+        double st[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) st[e] = (e >> 1) < nj ? Fg[(2LL * j0 + e) * nd + X] : 0.0;
+        permute_pairs<E>(st, r);
+        const unsigned a = (sbase + (unsigned)(X * E * 8)) | (unsigned)(r << 4);
+#pragma unroll
+        for (int t = 0; t < E; t += 2) lds_put(a ^ (unsigned)(t << 3), st[t], st[t + 1]);
+      }
+      __syncthreads();
+      // the owned densities' panels, in this lane's order, from LDS
       double fx[DPT][E], gx[DPT][E];
       const unsigned mL = ONE ? left_mask<E>(j0, nv, r) : 0u;
 #pragma unroll
       for (int q = 0; q < DPT; ++q) {
-        const int X = threadIdx.x + q * blockDim.x;
+        const int X = Wk.o0 + threadIdx.x + q * blockDim.x;
 #pragma unroll
         for (int t = 0; t < E; ++t) gx[q][t] = 0.0;
         if (act[q]) {
-#pragma unroll
-          for (int e = 0; e < E; ++e) fx[q][e] = (e >> 1) < nj ? Fg[(2LL * j0 + e) * nd + X] : 0.0;
-          permute_pairs<E>(fx[q], r);
           const unsigned a = (sbase + (unsigned)(X * E * 8)) | (unsigned)(r << 4);
 #pragma unroll
-          for (int t = 0; t < E; t += 2) lds_put(a ^ (unsigned)(t << 3), fx[q][t], fx[q][t + 1]);
+          for (int t = 0; t < E; t += 2) {
+            const pair_v pp = lds_get(a ^ (unsigned)(t << 3));
+            fx[q][t] = pp.x;
+            fx[q][t + 1] = pp.y;
+          }
         }
       }
-      __syncthreads();
       double* const pcol = part + Gp.off_part + (long long)Wk.pr * Gp.nrows;
 #pragma unroll
       for (int q = 0; q < DPT; ++q) {
@@ -822,7 +839,7 @@ __global__ void __launch_bounds__(rot_threads_max<W>()) k_pair_rot(
           if (!act[q]) continue;
           // back to element order, then coalesced rows of G
           permute_pairs<E>(gx[q], r);
-          double* g = Gout + Gp.off_F + threadIdx.x + q * blockDim.x;
+          double* g = Gout + Gp.off_F + Wk.o0 + threadIdx.x + q * blockDim.x;
 #pragma unroll
           for (int e = 0; e < E; ++e)
             if ((e >> 1) < nj) g[(2LL * j0 + e) * nd] = gx[q][e];
@@ -1771,8 +1788,9 @@ struct dbs_engine {
   //         [1] features block size, [2] pair-panel width W (4, 8, 16),
   //         [3] persistent grid cap (workgroups), [4] owner densities per
   //         workgroup of the chunked pair-panel kernel (its block size),
-  //         [5] W-panels per pair-panel workgroup
-  int tune[6] = {2, 256, 16, 4096, 512, 1};
+  //         [5] W-panels per pair-panel workgroup, [6] owner ranges per
+  //         group of the rotated pair-panel kernel
+  int tune[7] = {2, 256, 16, 4096, 512, 1, 1};
   std::vector<char> group_dense;  // per group: use the dense path
   double flops_fwd = 0, flops_adj = 0, bytes_min = 0;
   // graph
@@ -1928,8 +1946,19 @@ static int build_pair_tables(dbs_engine* e) {
     }
   }
   e->pair_w = W;
-  e->rot_dpt = rot_nd <= tmax ? 1 : 2;
-  e->rot_block = std::max(64, ((rot_nd + e->rot_dpt - 1) / e->rot_dpt + 63) / 64 * 64);
+  // owner ranges of the rotated groups: tuning[6] ranges of whole
+  // 64-density blocks
+  const int osplit = e->tune[6];
+  std::vector<int> rot_os(std::max(e->nkg, 1), 0);
+  int rot_own = 0;
+  for (int g = 0; g < e->nkg; ++g) {
+    if (!rot[g]) continue;
+    const int nd = std::max(e->h_kg[g].ndens, 1);
+    rot_os[g] = std::min(nd, ((nd + osplit - 1) / osplit + 63) / 64 * 64);
+    rot_own = std::max(rot_own, rot_os[g]);
+  }
+  e->rot_dpt = rot_own <= tmax ? 1 : 2;
+  e->rot_block = std::max(64, ((rot_own + e->rot_dpt - 1) / e->rot_dpt + 63) / 64 * 64);
   e->rot_shm = rot_lds_bytes(W, std::max(rot_nd, 1));
   if (e->rot_shm > 64 * 1024) {
     hipError_t err;
@@ -2066,13 +2095,29 @@ static int build_pair_tables(dbs_engine* e) {
     if (!G.dense) poff += (long long)npr * G.nrows;
     if (G.nrows == 0) continue;
     const int var = rot[g] ? 0 : 1;
-    const int ostep = rot[g] ? std::max(G.ndens, 1) : osz[g];
-    for (int p = 0; p < npr; ++p)
-      for (int o0 = 0; o0 < G.ndens; o0 += ostep) {
-        const PairWork Wk{g, p * span, std::min(J, (p + 1) * span), p, o0, std::min(G.ndens, o0 + ostep)};
-        work[1][var].push_back(Wk);
-        if (!G.dense) work[0][var].push_back(Wk);
-      }
+    const int ostep = rot[g] ? rot_os[g] : osz[g];
+    auto push = [&](const PairWork& Wk) {
+      work[1][var].push_back(Wk);
+      if (!G.dense) work[0][var].push_back(Wk);
+    };
+    if (rot[g] && ostep < G.ndens) {
+      // the owner ranges of one slot range 8 items apart, so that they
+      // run on one XCD (workgroups go to the XCDs in turn) and read its
+      // panels from one L2; blocks of 8 slot ranges, padded with empty
+      // items so every group keeps the alignment
+      for (int p0 = 0; p0 < npr; p0 += 8)
+        for (int o0 = 0; o0 < G.ndens; o0 += ostep)
+          for (int p = p0; p < p0 + 8; ++p) {
+            if (p < npr)
+              push(PairWork{g, p * span, std::min(J, (p + 1) * span), p, o0, std::min(G.ndens, o0 + ostep)});
+            else
+              push(PairWork{g, 0, 0, 0, 0, 0});
+          }
+    } else {
+      for (int p = 0; p < npr; ++p)
+        for (int o0 = 0; o0 < G.ndens; o0 += ostep)
+          push(PairWork{g, p * span, std::min(J, (p + 1) * span), p, o0, std::min(G.ndens, o0 + ostep)});
+    }
     // in slot order: consecutive rows read consecutive partials
     if (!G.dense)
       for (size_t q = 0; q < slot_row[g].size(); ++q) {
@@ -2087,9 +2132,22 @@ static int build_pair_tables(dbs_engine* e) {
   // longest groups first so the tail of the launch is short
   for (int a = 0; a < 2; ++a)
     for (int l = 0; l < 2; ++l)
+    {
       std::stable_sort(work[a][l].begin(), work[a][l].end(), [&](const PairWork& x, const PairWork& y) {
         return e->h_kg[x.g].nrows > e->h_kg[y.g].nrows;
       });
+      // a group with owner ranges starts at a multiple of 8
+      std::vector<PairWork> al;
+      al.reserve(work[a][l].size());
+      for (size_t i = 0; i < work[a][l].size(); ++i) {
+        const PairWork& x = work[a][l][i];
+        const bool start = i == 0 || work[a][l][i - 1].g != x.g;
+        if (start && rot[x.g] && rot_os[x.g] < e->h_kg[x.g].ndens)
+          while (al.size() % 8) al.push_back(PairWork{x.g, 0, 0, 0, 0, 0});
+        al.push_back(x);
+      }
+      work[a][l].swap(al);
+    }
   dfree(&e->d_chunks);
   dfree(&e->d_ent);
   dfree(&e->d_ecidx);
@@ -2918,8 +2976,8 @@ int dbs_engine_graph_launch(dbs_engine* e) {
 }
 
 int dbs_engine_set_tuning(dbs_engine* e, int n, const int* p) {
-  DBS_CHECK(e && p && n >= 1 && n <= 6, DBS_ERR_ARG, "dbs_engine_set_tuning: bad arguments");
-  int t[6];
+  DBS_CHECK(e && p && n >= 1 && n <= 7, DBS_ERR_ARG, "dbs_engine_set_tuning: bad arguments");
+  int t[7];
   std::memcpy(t, e->tune, sizeof(t));
   for (int i = 0; i < n; ++i) t[i] = p[i];
   DBS_CHECK(t[0] >= 0 && t[0] <= 2, DBS_ERR_ARG, "tuning[0]: path 0 pair panels, 1 dense, 2 auto");
@@ -2929,9 +2987,12 @@ int dbs_engine_set_tuning(dbs_engine* e, int n, const int* p) {
   DBS_CHECK(t[4] >= 64 && t[4] <= 512 && t[4] % 64 == 0, DBS_ERR_ARG,
             "tuning[4]: owner densities per chunked pair-panel workgroup, 64..512");
   DBS_CHECK(t[5] >= 1 && t[5] <= 256, DBS_ERR_ARG, "tuning[5]: panels per workgroup 1..256");
+  DBS_CHECK(t[6] == 1 || t[6] == 2 || t[6] == 4 || t[6] == 8, DBS_ERR_ARG,
+            "tuning[6]: owner ranges per rotated group 1, 2, 4 or 8");
   DBS_HIP(hipStreamSynchronize(e->stream));
   const bool path_changed = t[0] != e->tune[0];
-  const bool pair_changed = path_changed || t[2] != e->tune[2] || t[4] != e->tune[4] || t[5] != e->tune[5];
+  const bool pair_changed = path_changed || t[2] != e->tune[2] || t[4] != e->tune[4] || t[5] != e->tune[5] ||
+                            t[6] != e->tune[6];
   std::memcpy(e->tune, t, sizeof(t));
   if (path_changed) {
     for (int g = 0; g < e->nkg; ++g) e->group_dense[g] = (char)choose_dense(e, e->h_kg[g], e->group_rows[g].size());
@@ -2951,7 +3012,7 @@ int dbs_engine_set_tuning(dbs_engine* e, int n, const int* p) {
 }
 
 int dbs_engine_get_tuning(dbs_engine* e, int n, int* p) {
-  DBS_CHECK(e && p && n >= 1 && n <= 6, DBS_ERR_ARG, "dbs_engine_get_tuning: bad arguments");
+  DBS_CHECK(e && p && n >= 1 && n <= 7, DBS_ERR_ARG, "dbs_engine_get_tuning: bad arguments");
   for (int i = 0; i < n; ++i) p[i] = e->tune[i];
   return DBS_OK;
 }
@@ -2983,7 +3044,8 @@ int dbs_engine_autotune(dbs_engine* e, int with_adjoint, int reps) {
   const int widths[] = {8, 16};
   const int pblocks[] = {256, 512};
   const int nps[] = {1, 2, 4};
-  int best[6];
+  const int splits[] = {1, 2, 4};
+  int best[7];
   std::memcpy(best, e->tune, sizeof(best));
   float best_ms = 1e30f;
   hipEvent_t a, b;
@@ -2993,9 +3055,10 @@ int dbs_engine_autotune(dbs_engine* e, int with_adjoint, int reps) {
     for (int fb : fblocks)
       for (int w : widths)
         for (int pb : pblocks)
-          for (int np : nps) {
-            int t[6] = {p, fb, w, e->tune[3], pb, np};
-            if (dbs_engine_set_tuning(e, 6, t)) continue;
+          for (int np : nps)
+          for (int sp : splits) {
+            int t[7] = {p, fb, w, e->tune[3], pb, np, sp};
+            if (dbs_engine_set_tuning(e, 7, t)) continue;
             int rc = run_all(e, with_adjoint ? 1 : 0);  // warm
             if (rc) continue;
             std::vector<float> ms;
@@ -3018,7 +3081,7 @@ int dbs_engine_autotune(dbs_engine* e, int with_adjoint, int reps) {
           }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
-  return dbs_engine_set_tuning(e, 6, best);
+  return dbs_engine_set_tuning(e, 7, best);
 }
 
 // ---------------------------------------------------------------------------

@@ -114,8 +114,8 @@ class Engine:
         check(lib().dbs_engine_set_tuning(self.h, len(p), p.ctypes.data_as(C.POINTER(C.c_int))))
 
     def get_tuning(self):
-        p = np.zeros(6, dtype=np.int32)
-        check(lib().dbs_engine_get_tuning(self.h, 6, p.ctypes.data_as(C.POINTER(C.c_int))))
+        p = np.zeros(7, dtype=np.int32)
+        check(lib().dbs_engine_get_tuning(self.h, 7, p.ctypes.data_as(C.POINTER(C.c_int))))
         return [int(x) for x in p]
 
     def autotune(self, with_adjoint=True, reps=3):

@@ -297,7 +297,7 @@ def test_autotune_keeps_results(setups):
     eng.forward_adjoint()
     v0, g0 = eng.values(), eng.grad()
     best = eng.autotune(with_adjoint=True, reps=2)
-    assert len(best) == 6
+    assert len(best) == 7
     eng.forward_adjoint()
     assert np.max(relerr(eng.values(), v0)) < 1e-13
     assert np.max(np.abs(eng.grad() - g0)) <= 1e-13 * np.max(np.abs(g0))
@@ -327,7 +327,8 @@ def test_pair_panel_shapes(setups, kind):
     gs = np.max(np.abs(g0))
     assert gs > 0
     for t in ([0, 128, 4, 4096, 64, 1], [0, 256, 8, 4096, 512, 3], [0, 128, 16, 64, 128, 64],
-              [0, 256, 8, 4096, 192, 2], [1, 256, 16, 4096, 256, 1], [2, 256, 8, 4096, 256, 4]):
+              [0, 256, 8, 4096, 192, 2], [1, 256, 16, 4096, 256, 1], [2, 256, 8, 4096, 256, 4],
+              [0, 256, 16, 4096, 256, 4, 2]):
         print(f"  tuning {t}", flush=True)
         eng.set_tuning(t)
         eng.forward_adjoint()
@@ -365,9 +366,18 @@ def test_group_panel_variants(setups, norb, width, fblock):
     assert relerr(eng.values()[sel], ref).max() < TOL
     Gref = oracle_gradient(g, fs, rows[sel], coef[sel])
     assert np.max(np.abs(eng.grad() - Gref)) < TOL * np.max(np.abs(Gref))
-    v1 = eng.values()
+    v1, g1 = eng.values(), eng.grad()
     eng.forward()
     assert np.max(relerr(eng.values(), v1)) < 1e-13
+    # owner ranges of the rotated kernel (2, 4, 8 workgroups per slot range,
+    # several panels each): the same values and gradient
+    for sp, npn in ((2, 4), (4, 2), (8, 1)):
+        eng.set_tuning([0, fblock, width, 4096, 256, npn, sp])
+        eng.forward_adjoint()
+        assert np.max(relerr(eng.values(), v1)) < 1e-13, sp
+        assert np.max(np.abs(eng.grad() - g1)) <= 1e-13 * np.max(np.abs(g1)), sp
+        eng.forward()
+        assert np.max(relerr(eng.values(), v1)) < 1e-13, sp
 
 
 @pytest.mark.parametrize("kind", [0, 5])
@@ -399,7 +409,7 @@ def test_chunked_panels_match_rotated(setups, kind):
     gs = np.max(np.abs(g0))
     assert gs > 0
     for t in ([0, 256, 8, 4096, 512, 1], [0, 256, 16, 4096, 512, 1], [0, 256, 16, 4096, 256, 3],
-              [0, 256, 8, 64, 256, 2]):
+              [0, 256, 8, 64, 256, 2], [0, 256, 4, 4096, 512, 2, 2], [0, 256, 4, 4096, 512, 4, 4]):
         print(f"  tuning {t}", flush=True)
         eng.set_tuning(t)
         eng.forward_adjoint()
