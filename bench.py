@@ -631,6 +631,15 @@ def run_config5(args):
         tuning = eng.get_tuning()
     else:
         tuning = eng.autotune(with_adjoint=True, reps=2)
+    budget = {}
+    if args.memory_budget_mb > 0:
+        # the large-ns path: (instance, k) groups in batches whose features
+        # and gradients fit the budget
+        nb0, by0 = eng.memory()
+        eng.set_memory_budget(int(args.memory_budget_mb * (1 << 20)))
+        nb, by = eng.memory()
+        budget = {"memory_budget_mb": args.memory_budget_mb, "batches": nb, "feature_bytes": by,
+                  "feature_bytes_unbudgeted": by0}
     bump = np.zeros_like(pq0)
     bump[1:-2] = np.sin(np.arange(1, ns - 2))[:, None, None] * 1e-3
     # one SCF iteration = new orbitals (a 576 KB upload) + the captured step
@@ -653,7 +662,8 @@ def run_config5(args):
             extra["batched_vs_cpu_all_cores"] = value / extra["cpu_all_cores"]["value"]
     workload = {"workload": "SCF-style: ns=600, ks=9/8/9, 60 orbitals, k=0..12, 1e6 rows; 20 iterations of "
                             "new orbitals + one forward+adjoint hipGraph replay (autotuned once)",
-                "rows": nrows, "ns": ns, "norb": norb, "kmax": kmax, "iterations": 20, "tuning": tuning}
+                "rows": nrows, "ns": ns, "norb": norb, "kmax": kmax, "iterations": 20, "tuning": tuning,
+                **budget}
     return config_line(5, value, "R^k integrals/s", ms, 20, args.warmup, workload,
                        {"roofline": roof, "step_work": work, "cpu_baseline": cpu, **extra})
 
@@ -753,6 +763,8 @@ def main():
                          "(a plain copy kernel of known size for hardware-counter scaling)")
     ap.add_argument("--traffic-file", type=str,
                     default=os.path.join(ROOT, "profiles", TRAFFIC_FILE))
+    ap.add_argument("--memory-budget-mb", type=float, default=0.0,
+                    help="config 5: bytes of features + gradients held at once (MiB; 0 = all groups)")
     ap.add_argument("--config", type=int, default=4, choices=[1, 2, 3, 4, 5, 6],
                     help="integral-set configuration (4: the headline workload)")
     args = ap.parse_args()

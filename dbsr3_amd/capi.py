@@ -143,6 +143,8 @@ PROTOTYPES = {
     "dbs_engine_graph_capture": [vp, i32],
     "dbs_engine_graph_launch": [vp],
     "dbs_engine_set_tuning": [vp, i32, ip],
+    "dbs_engine_set_memory_budget": [vp, sz],
+    "dbs_engine_memory": [vp, ip, C.POINTER(sz)],
     "dbs_engine_get_tuning": [vp, i32, ip],
     "dbs_engine_autotune": [vp, i32, i32],
     "dbs_engine_work": [vp, i32, dp, dp],

@@ -978,11 +978,14 @@ __global__ void k_adj_orb_points(int ninst, const long long* __restrict__ seg,
                                  const DensDev* __restrict__ dens, const int* __restrict__ oadjoff,
                                  const OrbAdjDev* __restrict__ oadj, const double* __restrict__ orbv,
                                  const double* __restrict__ wt, const double* __restrict__ Gb,
-                                 double* __restrict__ gorb, long long total) {
+                                 double* __restrict__ gorb, long long total, int gb0, int gb1) {
   for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
        q += (long long)gridDim.x * blockDim.x) {
     const int s = find_seg(seg, ninst, q);
     const InstDev I = inst[s];
+    // groups [gb0, gb1) only; earlier groups' sums continue from gorb
+    if (max(I.kg0, gb0) >= min(I.kg0 + I.nkg, gb1)) continue;
+    const bool cont = I.kg0 < gb0;
     long long r = q - seg[s];
     const int p = (int)(r % I.np);
     r /= I.np;
@@ -991,7 +994,7 @@ __global__ void k_adj_orb_points(int ninst, const long long* __restrict__ seg,
     const int comp = (int)(r % 2);
     const int orb = (int)(r / 2);
     const int ks = I.ks, nv = I.nv, nk = nv * ks;
-    double acc = 0.0;
+    double acc = cont ? gorb[I.off_orbv + q - seg[s]] : 0.0;
     const int gslot = I.orb_base + orb;
     const int e0 = oadjoff[gslot], e1 = oadjoff[gslot + 1];
     const int m = p < ks ? p : (p - ks) / ks;
@@ -999,6 +1002,7 @@ __global__ void k_adj_orb_points(int ninst, const long long* __restrict__ seg,
     for (int e = e0; e < e1; ++e) {
       const OrbAdjDev A = oadj[e];
       const DensDev D = dens[A.dens];
+      if (D.kg < gb0 || D.kg >= gb1) continue;
       // component selection: TOT both comps, PP only P, QQ only Q, PQ: P for a, Q for c
       double partner;
       const double* vp = orbv + I.off_orbv + ((long long)(2 * A.partner) * nv + iv) * I.np;
@@ -1199,11 +1203,15 @@ __global__ void __launch_bounds__(256) k_adj_orb_lds(
     int ninst, const long long* __restrict__ seg, const InstDev* __restrict__ inst,
     const KGroupDev* __restrict__ kgs, const DensDev* __restrict__ dens,
     const double* __restrict__ orbv, const double* __restrict__ wt, const double* __restrict__ Gb,
-    int mode, double* __restrict__ gorb, long long total) {
+    int mode, double* __restrict__ gorb, long long total, int gb0, int gb1) {
   extern __shared__ double sm[];
   for (long long item = blockIdx.x; item < total; item += gridDim.x) {
     const int s = find_seg(seg, ninst, item);
     const InstDev I = inst[s];
+    // groups [gb0, gb1) only; earlier groups' sums continue from gorb
+    const int ga = max(I.kg0, gb0) - I.kg0, gz = min(I.kg0 + I.nkg, gb1) - I.kg0;
+    if (ga >= gz) continue;
+    const bool cont = ga > 0;
     const int r = (int)(item - seg[s]);
     const int iv = r / I.ks, m = r % I.ks;
     const int norb = I.norb, ks = I.ks, np = I.np, nv = I.nv, nk = nv * ks;
@@ -1220,14 +1228,14 @@ __global__ void __launch_bounds__(256) k_adj_orb_lds(
     for (int q = threadIdx.x; q < 2 * norb; q += blockDim.x) {
       const int comp = q / norb, a = q % norb;
       fo[q] = orbv[I.off_orbv + ((long long)(2 * a + comp) * nv + iv) * np + m];
-      oo[q] = 0.0;
+      oo[q] = cont ? gorb[I.off_orbv + ((long long)(2 * a + comp) * nv + iv) * np + m] : 0.0;
     }
     for (int q = threadIdx.x; q < 2 * norb * ks; q += blockDim.x) {
       const int i = q % ks, a = (q / ks) % norb, comp = q / (ks * norb);
       fi[q] = orbv[I.off_orbv + ((long long)(2 * a + comp) * nv + iv) * np + ks + m * ks + i];
-      oi[q] = 0.0;
+      oi[q] = cont ? gorb[I.off_orbv + ((long long)(2 * a + comp) * nv + iv) * np + ks + m * ks + i] : 0.0;
     }
-    for (int gi = 0; gi < I.nkg; ++gi) {
+    for (int gi = ga; gi < gz; ++gi) {
       const KGroupDev G = kgs[I.kg0 + gi];
       __syncthreads();
       if (threadIdx.x < ks + 2) {
@@ -1318,7 +1326,7 @@ __global__ void __launch_bounds__(256) k_adj_orb_mfma(
     int ninst, const long long* __restrict__ seg, const InstDev* __restrict__ inst,
     const KGroupDev* __restrict__ kgs, const int* __restrict__ dpk, const double* __restrict__ orbv,
     const double* __restrict__ wt, const double* __restrict__ Gb, double* __restrict__ gorb, long long total,
-    const int* __restrict__ accf) {
+    const int* __restrict__ accf, int gb0, int gb1) {
   extern __shared__ double sm[];
   // wave index uniform across the wave (scalar for the compiler)
   const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), nw = blockDim.x >> 6;
@@ -1326,6 +1334,11 @@ __global__ void __launch_bounds__(256) k_adj_orb_mfma(
     const int s = find_seg(seg, ninst, item);
     if (accf[s]) continue;  // instance done by k_adj_orb_acc
     const InstDev I = inst[s];
+    // groups [gb0, gb1) only; the sums of earlier groups continue from gorb
+    // (the accumulators are reloaded with the values they were stored as)
+    const int ga = max(I.kg0, gb0) - I.kg0, gz = min(I.kg0 + I.nkg, gb1) - I.kg0;
+    if (ga >= gz) continue;
+    const bool cont = ga > 0;
     const int r = (int)(item - seg[s]);
     const int iv = r / I.ks, m = r % I.ks;
     const int norb = I.norb, ks = I.ks, np = I.np, nv = I.nv, nk = nv * ks;
@@ -1354,10 +1367,25 @@ __global__ void __launch_bounds__(256) k_adj_orb_mfma(
     for (int q = threadIdx.x; q < 4 * NO * LM; q += blockDim.x) MB[q] = 0.0;
     __syncthreads();
     const int ntl = NO / 16, ntask = ntl * 2;  // nested-point tiles; the outer point on VALU
-    double oacc = 0.0;                            // thread t < 2 norb: orbital t/2, component t%2
+    // thread t < 2 norb: orbital t/2, component t%2
+    double oacc = cont && (int)threadIdx.x < 2 * norb
+                      ? gorb[I.off_orbv + ((long long)threadIdx.x * nv + iv) * np + m] : 0.0;
     dbl4 accs[3];
 #pragma unroll
-    for (int n = 0; n < 3; ++n) accs[n] = dbl4{0.0, 0.0, 0.0, 0.0};
+    for (int n = 0; n < 3; ++n) {
+      accs[n] = dbl4{0.0, 0.0, 0.0, 0.0};
+      const int t = wv + n * nw;
+      if (!cont || t >= ntask) continue;
+      const int rt = t / 2, ct = t % 2;
+      const int col = ct * 16 + (lane & 15);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int a = rt * 16 + 4 * j + (lane >> 4);
+        if (a >= norb || col >= 2 * ks) continue;
+        const int comp = col / ks, i = col % ks;
+        accs[n][j] = gorb[I.off_orbv + ((long long)(2 * a + comp) * nv + iv) * np + ks + m * ks + i];
+      }
+    }
     const long long xr = 2LL * (nv + iv * ks + m);
     // gradients of the first DBS_ADJ_PF densities per thread, one group ahead
     double pgo[DBS_ADJ_PF], pgy[DBS_ADJ_PF];
@@ -1380,8 +1408,8 @@ __global__ void __launch_bounds__(256) k_adj_orb_mfma(
         }
       }
     };
-    if (I.nkg > 0) fetch(0);
-    for (int gi = 0; gi < I.nkg; ++gi) {
+    fetch(ga);
+    for (int gi = ga; gi < gz; ++gi) {
       const KGroupDev G = kgs[I.kg0 + gi];
       const int nd = G.ndens;
       const int b = gi & 1;
@@ -1401,7 +1429,7 @@ __global__ void __launch_bounds__(256) k_adj_orb_mfma(
         gy[u] = pgy[u];
         pk[u] = ppk[u];
       }
-      if (gi + 1 < I.nkg) fetch(gi + 1);
+      if (gi + 1 < gz) fetch(gi + 1);
       auto put = [&](int p, double vo, double vy) {
         const int a = p & 255, c = (p >> 8) & 255;
         if (a == c) {
@@ -1791,6 +1819,20 @@ struct dbs_engine {
   //         [5] W-panels per pair-panel workgroup, [6] owner ranges per
   //         group of the rotated pair-panel kernel
   int tune[7] = {2, 256, 16, 4096, 512, 1, 1};
+  // memory budget of the features and their gradients (bytes of F and G
+  // together); 0: every group's are held at once.  Otherwise the groups run
+  // in batches of consecutive groups that fit, F and G reused per batch
+  // (dbs_engine_set_memory_budget).
+  long long mem_budget = 0;
+  std::vector<int> bat{0, 0};  // batch b: groups [bat[b], bat[b+1])
+  // per batch: first item of the feature, scan, matrix-core feature,
+  // dense-tile, row-reduction and pair-work lists (one entry past the end)
+  std::vector<int> bat_items, bat_scan, bat_fit, bat_tiles, bat_red;
+  std::vector<int> bat_work[2][2];
+  std::vector<int> h_accf;         // adjoint by k_adj_orb_acc per instance, as chosen at creation
+  int adjacc_u0 = 0;
+  bool adjacc_all0 = false;
+  int feat_fg = 16;                // groups per matrix-core feature item
   std::vector<char> group_dense;  // per group: use the dense path
   double flops_fwd = 0, flops_adj = 0, bytes_min = 0;
   // graph
@@ -1842,6 +1884,50 @@ int choose_dense(const dbs_engine* e, const KGroupDev& G, size_t nrow) {
   return (double)nrow >= 0.08 * n * n ? 1 : 0;
 }
 
+// Batches of consecutive groups whose features and gradients fit the
+// memory budget (at least one group each), the groups' feature offsets
+// within their batch, and the first feature / scan item of every batch
+// (both lists are in group order).  Returns the largest batch's feature
+// count.
+long long layout_batches(dbs_engine* e) {
+  e->bat.assign(1, 0);
+  long long cur = 0, big = 0;
+  for (int g = 0; g < e->nkg; ++g) {
+    KGroupDev& G = e->h_kg[g];
+    const InstDev& I = e->h_inst[G.inst];
+    const long long nf = (long long)feat_dim(I.nv, I.ks) * G.ndens;
+    if (e->mem_budget > 0 && cur > 0 && 2 * (cur + nf) * (long long)sizeof(double) > e->mem_budget) {
+      e->bat.push_back(g);
+      big = std::max(big, cur);
+      cur = 0;
+    }
+    G.off_F = cur;
+    cur += nf;
+  }
+  e->bat.push_back(e->nkg);
+  big = std::max(big, cur);
+  const int nb = (int)e->bat.size() - 1;
+  e->bat_items.assign(nb + 1, 0);
+  e->bat_scan.assign(nb + 1, 0);
+  int ni = 0, nsc = 0;
+  for (int b = 0; b < nb; ++b) {
+    e->bat_items[b] = ni;
+    e->bat_scan[b] = nsc;
+    for (int g = e->bat[b]; g < e->bat[b + 1]; ++g) {
+      ni += e->h_inst[e->h_kg[g].inst].nv;
+      nsc += (e->h_kg[g].ndens + DBS_SCAN_COLS - 1) / DBS_SCAN_COLS;
+    }
+  }
+  e->bat_items[nb] = ni;
+  e->bat_scan[nb] = nsc;
+  return big;
+}
+
+// the batch of group g
+int batch_of(const dbs_engine* e, int g) {
+  return (int)(std::upper_bound(e->bat.begin(), e->bat.end(), g) - e->bat.begin()) - 1;
+}
+
 int build_dense_tables(dbs_engine* e) {
   std::vector<int4> tiles;
   std::vector<long long> pairoff(e->nkg + 1, 0);
@@ -1853,7 +1939,10 @@ int build_dense_tables(dbs_engine* e) {
   pairoff[e->nkg] = tot;
   std::vector<int> pairrow((size_t)std::max<long long>(tot, 1), -1);
   std::vector<int> nextrow(std::max<size_t>(e->h_rows.size(), 1), -1);
+  const int nbat = (int)e->bat.size() - 1;
+  e->bat_tiles.assign(nbat + 1, 0);
   for (int g = 0; g < e->nkg; ++g) {
+    if (g == e->bat[batch_of(e, g)]) e->bat_tiles[batch_of(e, g)] = (int)tiles.size();
     if (!e->group_dense[g]) continue;
     const int n = e->h_kg[g].ndens, nt = (n + 15) / 16;
     for (int a = 0; a < nt; ++a)
@@ -1869,6 +1958,7 @@ int build_dense_tables(dbs_engine* e) {
     }
   }
   e->ntiles = (int)tiles.size();
+  e->bat_tiles[nbat] = e->ntiles;
   dfree(&e->d_tiles);
   dfree(&e->d_pairrow);
   dfree(&e->d_pairoff);
@@ -2069,85 +2159,99 @@ static int build_pair_tables(dbs_engine* e) {
     }
   }
   e->nent = base;
-  // work lists and partial layout
+  // work lists and partial layout, batch by batch: a batch's partials and
+  // work items start afresh (the partials of a batch are reduced before the
+  // next batch's pair panels run)
   std::vector<PairWork> work[2][2];
   std::vector<RowRed> red;
-  long long poff = 0;
-  for (int g = 0; g < e->nkg; ++g) {
-    KGroupDev& G = e->h_kg[g];
-    const InstDev& I = e->h_inst[G.inst];
-    const int J = I.nv * (1 + I.ks);
-    const int Wg = rot[g] ? W : CW;
-    // partial values of the group's rows are bounded to kPartMax doubles:
-    // large row sets take more panels per workgroup
-    const long long kPartMax = 32LL << 20;
-    int np_g = NP;
-    if (!e->group_dense[g] && G.nrows > 0) {
-      const long long maxpr = std::max(1LL, kPartMax / G.nrows);
-      const long long need = (J + (long long)Wg * np_g - 1) / ((long long)Wg * np_g);
-      if (need > maxpr) np_g = (int)((J + Wg * maxpr - 1) / (Wg * maxpr));
+  long long pmax = 0;
+  const int nbat = (int)e->bat.size() - 1;
+  e->bat_red.assign(nbat + 1, 0);
+  for (int a = 0; a < 2; ++a)
+    for (int l = 0; l < 2; ++l) e->bat_work[a][l].assign(nbat + 1, 0);
+  for (int bt = 0; bt < nbat; ++bt) {
+    std::vector<PairWork> wb[2][2];
+    long long poff = 0;
+    e->bat_red[bt] = (int)red.size();
+    for (int g = e->bat[bt]; g < e->bat[bt + 1]; ++g) {
+      KGroupDev& G = e->h_kg[g];
+      const InstDev& I = e->h_inst[G.inst];
+      const int J = I.nv * (1 + I.ks);
+      const int Wg = rot[g] ? W : CW;
+      // partial values of the group's rows are bounded to kPartMax doubles:
+      // large row sets take more panels per workgroup
+      const long long kPartMax = 32LL << 20;
+      int np_g = NP;
+      if (!e->group_dense[g] && G.nrows > 0) {
+        const long long maxpr = std::max(1LL, kPartMax / G.nrows);
+        const long long need = (J + (long long)Wg * np_g - 1) / ((long long)Wg * np_g);
+        if (need > maxpr) np_g = (int)((J + Wg * maxpr - 1) / (Wg * maxpr));
+      }
+      const int span = Wg * np_g;
+      const int npr = (J + span - 1) / span;
+      G.npr = e->group_dense[g] ? 0 : npr;
+      G.dense = e->group_dense[g];
+      G.off_part = poff;
+      if (!G.dense) poff += (long long)npr * G.nrows;
+      if (G.nrows == 0) continue;
+      const int var = rot[g] ? 0 : 1;
+      const int ostep = rot[g] ? rot_os[g] : osz[g];
+      auto push = [&](const PairWork& Wk) {
+        wb[1][var].push_back(Wk);
+        if (!G.dense) wb[0][var].push_back(Wk);
+      };
+      if (rot[g] && ostep < G.ndens) {
+        // the owner ranges of one slot range 8 items apart, so that they
+        // run on one XCD (workgroups go to the XCDs in turn) and read its
+        // panels from one L2; blocks of 8 slot ranges, padded with empty
+        // items so every group keeps the alignment
+        for (int p0 = 0; p0 < npr; p0 += 8)
+          for (int o0 = 0; o0 < G.ndens; o0 += ostep)
+            for (int p = p0; p < p0 + 8; ++p) {
+              if (p < npr)
+                push(PairWork{g, p * span, std::min(J, (p + 1) * span), p, o0, std::min(G.ndens, o0 + ostep)});
+              else
+                push(PairWork{g, 0, 0, 0, 0, 0});
+            }
+      } else {
+        for (int p = 0; p < npr; ++p)
+          for (int o0 = 0; o0 < G.ndens; o0 += ostep)
+            push(PairWork{g, p * span, std::min(J, (p + 1) * span), p, o0, std::min(G.ndens, o0 + ostep)});
+      }
+      // in slot order: consecutive rows read consecutive partials
+      if (!G.dense)
+        for (size_t q = 0; q < slot_row[g].size(); ++q) {
+          RowRed R;
+          R.pbase = G.off_part + (long long)q;
+          R.out = e->h_row_out[slot_row[g][q]];
+          R.stride = G.nrows;
+          R.npr = npr;
+          red.push_back(R);
+        }
     }
-    const int span = Wg * np_g;
-    const int npr = (J + span - 1) / span;
-    G.npr = e->group_dense[g] ? 0 : npr;
-    G.dense = e->group_dense[g];
-    G.off_part = poff;
-    if (!G.dense) poff += (long long)npr * G.nrows;
-    if (G.nrows == 0) continue;
-    const int var = rot[g] ? 0 : 1;
-    const int ostep = rot[g] ? rot_os[g] : osz[g];
-    auto push = [&](const PairWork& Wk) {
-      work[1][var].push_back(Wk);
-      if (!G.dense) work[0][var].push_back(Wk);
-    };
-    if (rot[g] && ostep < G.ndens) {
-      // the owner ranges of one slot range 8 items apart, so that they
-      // run on one XCD (workgroups go to the XCDs in turn) and read its
-      // panels from one L2; blocks of 8 slot ranges, padded with empty
-      // items so every group keeps the alignment
-      for (int p0 = 0; p0 < npr; p0 += 8)
-        for (int o0 = 0; o0 < G.ndens; o0 += ostep)
-          for (int p = p0; p < p0 + 8; ++p) {
-            if (p < npr)
-              push(PairWork{g, p * span, std::min(J, (p + 1) * span), p, o0, std::min(G.ndens, o0 + ostep)});
-            else
-              push(PairWork{g, 0, 0, 0, 0, 0});
-          }
-    } else {
-      for (int p = 0; p < npr; ++p)
-        for (int o0 = 0; o0 < G.ndens; o0 += ostep)
-          push(PairWork{g, p * span, std::min(J, (p + 1) * span), p, o0, std::min(G.ndens, o0 + ostep)});
-    }
-    // in slot order: consecutive rows read consecutive partials
-    if (!G.dense)
-      for (size_t q = 0; q < slot_row[g].size(); ++q) {
-        RowRed R;
-        R.pbase = G.off_part + (long long)q;
-        R.out = e->h_row_out[slot_row[g][q]];
-        R.stride = G.nrows;
-        R.npr = npr;
-        red.push_back(R);
+    pmax = std::max(pmax, poff);
+    // longest groups first so the tail of the launch is short
+    for (int a = 0; a < 2; ++a)
+      for (int l = 0; l < 2; ++l) {
+        std::stable_sort(wb[a][l].begin(), wb[a][l].end(), [&](const PairWork& x, const PairWork& y) {
+          return e->h_kg[x.g].nrows > e->h_kg[y.g].nrows;
+        });
+        // a group with owner ranges starts at a multiple of 8
+        e->bat_work[a][l][bt] = (int)work[a][l].size();
+        std::vector<PairWork>& al = work[a][l];
+        const size_t base = al.size();
+        for (size_t i = 0; i < wb[a][l].size(); ++i) {
+          const PairWork& x = wb[a][l][i];
+          const bool start = i == 0 || wb[a][l][i - 1].g != x.g;
+          if (start && rot[x.g] && rot_os[x.g] < e->h_kg[x.g].ndens)
+            while ((al.size() - base) % 8) al.push_back(PairWork{x.g, 0, 0, 0, 0, 0});
+          al.push_back(x);
+        }
       }
   }
-  // longest groups first so the tail of the launch is short
+  e->bat_red[nbat] = (int)red.size();
   for (int a = 0; a < 2; ++a)
-    for (int l = 0; l < 2; ++l)
-    {
-      std::stable_sort(work[a][l].begin(), work[a][l].end(), [&](const PairWork& x, const PairWork& y) {
-        return e->h_kg[x.g].nrows > e->h_kg[y.g].nrows;
-      });
-      // a group with owner ranges starts at a multiple of 8
-      std::vector<PairWork> al;
-      al.reserve(work[a][l].size());
-      for (size_t i = 0; i < work[a][l].size(); ++i) {
-        const PairWork& x = work[a][l][i];
-        const bool start = i == 0 || work[a][l][i - 1].g != x.g;
-        if (start && rot[x.g] && rot_os[x.g] < e->h_kg[x.g].ndens)
-          while (al.size() % 8) al.push_back(PairWork{x.g, 0, 0, 0, 0, 0});
-        al.push_back(x);
-      }
-      work[a][l].swap(al);
-    }
+    for (int l = 0; l < 2; ++l) e->bat_work[a][l][nbat] = (int)work[a][l].size();
   dfree(&e->d_chunks);
   dfree(&e->d_ent);
   dfree(&e->d_ecidx);
@@ -2167,7 +2271,7 @@ static int build_pair_tables(dbs_engine* e) {
   if (!rc) rc = upload(e->stream, &e->d_kg, e->h_kg);
   for (int a = 0; a < 2 && !rc; ++a)
     for (int l = 0; l < 2 && !rc; ++l) rc = upload(e->stream, &e->d_work[a][l], work[a][l]);
-  if (!rc && hipMalloc(&e->d_part, sizeof(double) * (size_t)std::max<long long>(poff, 1)) != hipSuccess)
+  if (!rc && hipMalloc(&e->d_part, sizeof(double) * (size_t)std::max<long long>(pmax, 1)) != hipSuccess)
     rc = fail(DBS_ERR_HIP, "pair tables: hipMalloc");
   return rc;
 }
@@ -2227,12 +2331,13 @@ static void launch_chunk(dbs_engine* e, int adj, unsigned grid, const PairWork* 
   }
 }
 
-static void launch_pair(dbs_engine* e, int adj, hipStream_t s) {
+static void launch_pair(dbs_engine* e, int adj, int bt, hipStream_t s) {
   const int cap = e->tune[3] * 4;
-  const int nr = e->nwork[adj][0], nc = e->nwork[adj][1];
+  const int r0 = e->bat_work[adj][0][bt], nr = e->bat_work[adj][0][bt + 1] - r0;
+  const int c0 = e->bat_work[adj][1][bt], nc = e->bat_work[adj][1][bt + 1] - c0;
   if (nr > 0) {
     const unsigned grid = (unsigned)std::min(nr, cap);
-    const PairWork* wk = e->d_work[adj][0];
+    const PairWork* wk = e->d_work[adj][0] + r0;
     switch (e->pair_w) {
       case 4:
         if (e->rot_dpt == 1) launch_rot<4, 1>(e, adj, grid, wk, nr, s);
@@ -2247,20 +2352,20 @@ static void launch_pair(dbs_engine* e, int adj, hipStream_t s) {
   }
   if (nc > 0) {
     const unsigned grid = (unsigned)std::min(nc, cap);
-    const PairWork* wk = e->d_work[adj][1];
+    const PairWork* wk = e->d_work[adj][1] + c0;
     if (e->chunk_w == 8) launch_chunk<8>(e, adj, grid, wk, nc, s);
     else launch_chunk<16>(e, adj, grid, wk, nc, s);
   }
 }
 
-// the forward (with_adj: fused with the gather of the adjoint)
-static int run_forward(dbs_engine* e, int with_adj) {
+// orbital values at the points; with the adjoint the coefficient gather
+// into the adjacency entries rides in the same launch
+static void run_points(dbs_engine* e, int with_adj) {
   hipStream_t s = e->stream;
   mark(e, "start");
   const int cap = e->tune[3];
   const long long ng = with_adj ? e->nent : 0;
   if (e->ov_shm <= 64 * 1024) {
-    // the coefficient gather rides in the same launch (extra workgroups)
     const int nbo = (int)grid_for(e->n_ivc, 1, cap * 4);
     const int nbg = ng > 0 ? (int)grid_for(ng, 256, cap * 4) : 0;
     k_orb_values<<<nbo + nbg, 256, e->ov_shm, s>>>(e->ninst, e->d_seg_ivc, e->d_inst, e->pq, e->orbv, e->n_ivc,
@@ -2273,47 +2378,13 @@ static int run_forward(dbs_engine* e, int with_adj) {
     if (ng > 0) k_adj_cgather<<<grid_for(ng, 256, cap * 4), 256, 0, s>>>(ng, e->d_ecidx, e->coef, e->d_ent);
     mark(e, "coef_gather");
   }
-  if (e->featm_shm > 0)
-    k_features_mfma<<<grid_for(e->nfitems, 4, cap * 4), 256, e->featm_shm, s>>>(
-        e->nfitems, e->d_fitems, e->d_kg, e->d_inst, e->d_pidx, e->orbv, e->wt, e->F, e->featm_wave);
-  else
-    k_features<<<grid_for(e->nitems, 1, cap), e->tune[1], e->feat_shm, s>>>(
-        e->nitems, e->d_items, e->d_kg, e->d_inst, e->d_dens, e->orbv, e->wt, e->F, e->feat_stage);
-  mark(e, "features");
-  k_scan_cols<<<grid_for(e->nscan_items, 1, cap), 256, 0, s>>>(e->nscan_items, e->d_scan_items, e->d_kg,
-                                                               e->d_inst, e->F, 0);
-  mark(e, "prefix_scan");
-  const int wa = with_adj ? 1 : 0;
-  if (e->nwork[wa][0] + e->nwork[wa][1] > 0) {
-    launch_pair(e, with_adj ? 1 : 0, s);
-    mark(e, with_adj ? "pair_panel" : "contract_rows");
-  }
-  if (e->nred > 0) {
-    k_rows_reduce<<<grid_for((long long)e->nred * DBS_RED_LANES, 256, cap * 4), 256, 0, s>>>(
-        e->nred, e->d_red, e->d_part, e->vals);
-    mark(e, "rows_reduce");
-  }
-  if (e->ntiles > 0) {
-    k_contract_dense<<<grid_for(e->ntiles, 1, cap * 4), 64 * DBS_DENSE_SPLIT, 0, s>>>(
-        e->ntiles, e->d_tiles, e->d_kg, e->d_inst, e->F, e->d_pairrow, e->d_pairoff, e->d_nextrow,
-        e->d_row_out, e->onesided, e->vals);
-    mark(e, "contract_dense");
-  }
-  k_energy<<<grid_for(e->nechunks, 1, cap), 256, 0, s>>>(e->nechunks, e->d_echunks, e->d_roff, e->coef, e->vals,
-                                                         e->d_epart, e->ninst, e->d_echoff, e->energy,
-                                                         e->d_edone);
-  mark(e, "energy");
-  DBS_HIP(hipGetLastError());
-  return DBS_OK;
 }
 
-// the rest of the adjoint after run_forward(e, 1)
-static int run_adjoint(dbs_engine* e) {
+// the orbital point gradients of groups [g0, g1) (all groups, or one batch:
+// a batch continues the sums of the earlier batches from gorb)
+static void run_adj_orb(dbs_engine* e, int g0, int g1) {
   hipStream_t s = e->stream;
   const int cap = e->tune[3];
-  k_scan_cols<<<grid_for(e->nscan_items, 1, cap), 256, 0, s>>>(e->nscan_items, e->d_scan_items, e->d_kg,
-                                                               e->d_inst, e->G, 1);
-  mark(e, "adj_suffix_scan");
   if (e->adjacc_u > 0) {
     const long long per = (e->adj_items + 7) / 8;
     const int nsg = e->adjacc_sg == 64 ? 4 : 1;  // items per workgroup
@@ -2337,36 +2408,91 @@ static int run_adjoint(dbs_engine* e) {
     if (!e->adjacc_all)
       k_adj_orb_mfma<<<grid_for(e->adj_items, 1, cap * 4), 256, e->adjm_shm, s>>>(
           e->ninst, e->d_seg_adj, e->d_inst, e->d_kg, e->d_dpk, e->orbv, e->wt, e->G, e->gorb, e->adj_items,
-          e->d_accf);
+          e->d_accf, g0, g1);
   } else if (e->adj_shm > 0) {
     k_adj_orb_lds<<<grid_for(e->adj_items, 1, cap * 4), 256, e->adj_shm, s>>>(
         e->ninst, e->d_seg_adj, e->d_inst, e->d_kg, e->d_dens, e->orbv, e->wt, e->G, e->adj_mode, e->gorb,
-        e->adj_items);
+        e->adj_items, g0, g1);
   } else {
     k_adj_orb_points<<<grid_for(e->total_orbv, 256, cap * 4), 256, 0, s>>>(
         e->ninst, e->d_seg_orb, e->d_inst, e->d_kg, e->d_dens, e->d_oadjoff, e->d_oadj, e->orbv, e->wt,
-        e->G, e->gorb, e->total_orbv);
+        e->G, e->gorb, e->total_orbv, g0, g1);
   }
   mark(e, "adj_orb_points");
-  if (e->proj_shm <= 64 * 1024) {
-    k_adj_proj<<<grid_for(e->n_ivc, 1, cap * 4), 256, e->proj_shm, s>>>(e->ninst, e->d_seg_ivc, e->d_inst, e->gorb,
-                                                                       e->d_ppoff, e->d_ppart, e->n_ivc);
-    k_adj_coef_sum<<<grid_for(e->total_grad, 256, cap * 4), 256, 0, s>>>(e->ninst, e->d_seg_coef, e->d_inst,
-                                                                         e->d_ppoff, e->d_ppart, e->grad,
-                                                                         e->total_grad);
+}
+
+// one batch of groups: features, prefix scans, pair panels (fused with the
+// gradient gather of the adjoint) and row values; with the adjoint the
+// suffix scans and the batch's share of the orbital point gradients
+static void run_batch(dbs_engine* e, int bt, int with_adj) {
+  hipStream_t s = e->stream;
+  const int cap = e->tune[3];
+  if (e->featm_shm > 0) {
+    const int f0 = e->bat_fit[bt], nf = e->bat_fit[bt + 1] - f0;
+    if (nf > 0)
+      k_features_mfma<<<grid_for(nf, 4, cap * 4), 256, e->featm_shm, s>>>(
+          nf, e->d_fitems + f0, e->d_kg, e->d_inst, e->d_pidx, e->orbv, e->wt, e->F, e->featm_wave);
   } else {
-    k_adj_coef<<<grid_for(e->total_grad, 4, cap * 4), 256, 0, s>>>(e->ninst, e->d_seg_coef, e->d_inst,
-                                                                    e->gorb, e->grad, e->total_grad);
+    const int i0 = e->bat_items[bt], ni = e->bat_items[bt + 1] - i0;
+    if (ni > 0)
+      k_features<<<grid_for(ni, 1, cap), e->tune[1], e->feat_shm, s>>>(
+          ni, e->d_items + i0, e->d_kg, e->d_inst, e->d_dens, e->orbv, e->wt, e->F, e->feat_stage);
   }
-  mark(e, "adj_coef");
-  DBS_HIP(hipGetLastError());
-  return DBS_OK;
+  mark(e, "features");
+  const int s0 = e->bat_scan[bt], nsc = e->bat_scan[bt + 1] - s0;
+  if (nsc > 0)
+    k_scan_cols<<<grid_for(nsc, 1, cap), 256, 0, s>>>(nsc, e->d_scan_items + s0, e->d_kg, e->d_inst, e->F, 0);
+  mark(e, "prefix_scan");
+  const int wa = with_adj ? 1 : 0;
+  if (e->bat_work[wa][0][bt + 1] - e->bat_work[wa][0][bt] + e->bat_work[wa][1][bt + 1] - e->bat_work[wa][1][bt] > 0) {
+    launch_pair(e, wa, bt, s);
+    mark(e, with_adj ? "pair_panel" : "contract_rows");
+  }
+  const int r0 = e->bat_red[bt], nred = e->bat_red[bt + 1] - r0;
+  if (nred > 0) {
+    k_rows_reduce<<<grid_for((long long)nred * DBS_RED_LANES, 256, cap * 4), 256, 0, s>>>(nred, e->d_red + r0,
+                                                                                          e->d_part, e->vals);
+    mark(e, "rows_reduce");
+  }
+  const int t0 = e->bat_tiles[bt], nt = e->bat_tiles[bt + 1] - t0;
+  if (nt > 0) {
+    k_contract_dense<<<grid_for(nt, 1, cap * 4), 64 * DBS_DENSE_SPLIT, 0, s>>>(
+        nt, e->d_tiles + t0, e->d_kg, e->d_inst, e->F, e->d_pairrow, e->d_pairoff, e->d_nextrow, e->d_row_out,
+        e->onesided, e->vals);
+    mark(e, "contract_dense");
+  }
+  if (with_adj) {
+    if (nsc > 0)
+      k_scan_cols<<<grid_for(nsc, 1, cap), 256, 0, s>>>(nsc, e->d_scan_items + s0, e->d_kg, e->d_inst, e->G, 1);
+    mark(e, "adj_suffix_scan");
+    run_adj_orb(e, e->bat[bt], e->bat[bt + 1]);
+  }
 }
 
 static int run_all(dbs_engine* e, int with_adj) {
-  int rc = run_forward(e, with_adj);
-  if (!rc && with_adj) rc = run_adjoint(e);
-  return rc;
+  hipStream_t s = e->stream;
+  const int cap = e->tune[3];
+  run_points(e, with_adj);
+  for (int bt = 0; bt + 1 < (int)e->bat.size(); ++bt) run_batch(e, bt, with_adj);
+  k_energy<<<grid_for(e->nechunks, 1, cap), 256, 0, s>>>(e->nechunks, e->d_echunks, e->d_roff, e->coef, e->vals,
+                                                         e->d_epart, e->ninst, e->d_echoff, e->energy,
+                                                         e->d_edone);
+  mark(e, "energy");
+  if (with_adj) {
+    if (e->proj_shm <= 64 * 1024) {
+      k_adj_proj<<<grid_for(e->n_ivc, 1, cap * 4), 256, e->proj_shm, s>>>(e->ninst, e->d_seg_ivc, e->d_inst,
+                                                                         e->gorb, e->d_ppoff, e->d_ppart, e->n_ivc);
+      k_adj_coef_sum<<<grid_for(e->total_grad, 256, cap * 4), 256, 0, s>>>(e->ninst, e->d_seg_coef, e->d_inst,
+                                                                           e->d_ppoff, e->d_ppart, e->grad,
+                                                                           e->total_grad);
+    } else {
+      k_adj_coef<<<grid_for(e->total_grad, 4, cap * 4), 256, 0, s>>>(e->ninst, e->d_seg_coef, e->d_inst,
+                                                                      e->gorb, e->grad, e->total_grad);
+    }
+    mark(e, "adj_coef");
+  }
+  DBS_HIP(hipGetLastError());
+  return DBS_OK;
 }
 
 static void compute_phase_work(dbs_engine* e) {
@@ -2420,6 +2546,48 @@ static void compute_phase_work(dbs_engine* e) {
 }
 
 extern "C" {
+
+// Items of the matrix-core feature kernel, batch by batch: runs of
+// consecutive groups of one instance (within one batch) with the kind, at
+// most fg per item, spread evenly.  Returns the item count.
+static long long feature_items(dbs_engine* e, int fg, std::vector<int4>* out) {
+  long long n = 0;
+  const int nbat = (int)e->bat.size() - 1;
+  if (out) e->bat_fit.assign(nbat + 1, 0);
+  for (int bt = 0; bt < nbat; ++bt) {
+    if (out) e->bat_fit[bt] = (int)out->size();
+    for (int i = 0; i < e->ninst; ++i) {
+      const InstDev& I = e->h_inst[i];
+      const int g0 = std::max(I.kg0, e->bat[bt]) - I.kg0, g1 = std::min(I.kg0 + I.nkg, e->bat[bt + 1]) - I.kg0;
+      for (int kd = 0; kd < 4; ++kd) {
+        int gi = g0;
+        while (gi < g1) {
+          if (!((e->h_kg[I.kg0 + gi].gmask >> kd) & 1)) { ++gi; continue; }
+          int ge = gi;
+          while (ge < g1 && ((e->h_kg[I.kg0 + ge].gmask >> kd) & 1)) ++ge;
+          const int run = ge - gi, nch = (run + fg - 1) / fg;
+          n += (long long)nch * I.nv * (I.ks + 2);
+          for (int c = 0; out && c < nch; ++c) {
+            const int a = gi + run * c / nch, b = gi + run * (c + 1) / nch;
+            for (int iv = 0; iv < I.nv; ++iv)
+              for (int q = 0; q < I.ks + 2; ++q) out->push_back(make_int4(I.kg0 + a, iv, kd | ((b - a) << 8), q));
+          }
+          gi = ge;
+        }
+      }
+    }
+  }
+  if (out) e->bat_fit[nbat] = (int)out->size();
+  return n;
+}
+
+static int build_feature_items(dbs_engine* e) {
+  std::vector<int4> fit;
+  feature_items(e, e->feat_fg, &fit);
+  e->nfitems = (int)fit.size();
+  dfree(&e->d_fitems);
+  return upload(e->stream, &e->d_fitems, fit);
+}
 
 int dbs_engine_create(dbs_engine** out, int ninst, dbs_ctx* const* ctxs, const int* norb,
                       const int* nrows, const int* rows, int kind) {
@@ -2596,7 +2764,8 @@ int dbs_engine_create(dbs_engine** out, int ninst, dbs_ctx* const* ctxs, const i
     while (g < e->nkg && e->h_kg[g].inst == i) { ++g; ++n; }
     e->h_inst[i].nkg = n;
   }
-  e->total_feat = foff;
+  e->total_feat = layout_batches(e);  // one batch: off_F as above
+  (void)foff;
   e->total_w = woff;
   // work items (group, iv)
   std::vector<int2> items;
@@ -2724,7 +2893,6 @@ int dbs_engine_create(dbs_engine** out, int ninst, dbs_ctx* const* ctxs, const i
                             (int)e->featm_shm) != hipSuccess)
       e->featm_shm = 0;
     if (e->featm_shm > 0) {
-      std::vector<int4> fit;
       std::vector<int> pidx;
       for (int g = 0; g < e->nkg; ++g) {
         KGroupDev& G = e->h_kg[g];
@@ -2741,37 +2909,8 @@ int dbs_engine_create(dbs_engine** out, int ninst, dbs_ctx* const* ctxs, const i
           }
         }
       }
-      // runs of consecutive groups of one instance with the kind, at most
-      // fg per item, spread evenly: fg = 16 (the item's staged orbital
-      // values shared by more groups) while that leaves at least 4 items per
-      // SIMD (1024 SIMDs), else 8
-      auto make_items = [&](int fg, std::vector<int4>* out) {
-        long long n = 0;
-        for (int i = 0; i < ninst; ++i) {
-          const InstDev& I = e->h_inst[i];
-          for (int kd = 0; kd < 4; ++kd) {
-            int gi = 0;
-            while (gi < I.nkg) {
-              if (!((e->h_kg[I.kg0 + gi].gmask >> kd) & 1)) { ++gi; continue; }
-              int ge = gi;
-              while (ge < I.nkg && ((e->h_kg[I.kg0 + ge].gmask >> kd) & 1)) ++ge;
-              const int run = ge - gi, nch = (run + fg - 1) / fg;
-              n += (long long)nch * I.nv * (I.ks + 2);
-              for (int c = 0; out && c < nch; ++c) {
-                const int a = gi + run * c / nch, b = gi + run * (c + 1) / nch;
-                for (int iv = 0; iv < I.nv; ++iv)
-                  for (int q = 0; q < I.ks + 2; ++q)
-                    out->push_back(make_int4(I.kg0 + a, iv, kd | ((b - a) << 8), q));
-              }
-              gi = ge;
-            }
-          }
-        }
-        return n;
-      };
-      make_items(make_items(kFeatGroups, nullptr) >= 4096 ? kFeatGroups : kFeatGroups / 2, &fit);
-      e->nfitems = (int)fit.size();
-      rc |= upload(s, &e->d_fitems, fit);
+      e->feat_fg = feature_items(e, kFeatGroups, nullptr) >= 4096 ? kFeatGroups : kFeatGroups / 2;
+      rc |= build_feature_items(e);
       rc |= upload(s, &e->d_pidx, pidx);
     }
 
@@ -2831,6 +2970,9 @@ int dbs_engine_create(dbs_engine** out, int ninst, dbs_ctx* const* ctxs, const i
       e->adjacc_all = nacc == ninst;
       e->adjacc_shm = e->adjacc_u ? aneed2 * sizeof(double) * (e->adjacc_sg == 64 ? 4 : 1) : 0;
       rc |= upload(s, &e->d_accf, accf);
+      e->h_accf = accf;
+      e->adjacc_u0 = e->adjacc_u;
+      e->adjacc_all0 = e->adjacc_all;
       if (e->adjacc_shm > 64 * 1024) {
         const void* fs[] = {(const void*)k_adj_orb_acc<1, 64>, (const void*)k_adj_orb_acc<1, 256>,
                             (const void*)k_adj_orb_acc<2, 256>, (const void*)k_adj_orb_acc<4, 256>};
@@ -3008,6 +3150,51 @@ int dbs_engine_set_tuning(dbs_engine* e, int n, const int* p) {
   // a captured graph holds the old launch shapes
   if (e->gexec) { (void)hipGraphExecDestroy(e->gexec); e->gexec = nullptr; }
   if (e->graph) { (void)hipGraphDestroy(e->graph); e->graph = nullptr; }
+  return DBS_OK;
+}
+
+int dbs_engine_set_memory_budget(dbs_engine* e, size_t bytes) {
+  DBS_CHECK(e, DBS_ERR_ARG, "dbs_engine_set_memory_budget: null engine");
+  DBS_HIP(hipStreamSynchronize(e->stream));
+  e->mem_budget = (long long)bytes;
+  const long long nf = layout_batches(e);
+  dfree(&e->F);
+  dfree(&e->G);
+  DBS_HIP(hipMalloc(&e->F, (size_t)std::max(nf, 1LL) * sizeof(double)));
+  DBS_HIP(hipMalloc(&e->G, (size_t)std::max(nf, 1LL) * sizeof(double)));
+  e->total_feat = nf;
+  // the register-summed adjoint (k_adj_orb_acc) adds all of an instance's
+  // groups in registers before the orbital products, which a batch cannot
+  // continue: under a budget the other adjoint kernels run
+  std::vector<int> accf(e->h_accf);
+  if (bytes > 0) {
+    std::fill(accf.begin(), accf.end(), 0);
+    e->adjacc_u = 0;
+    e->adjacc_all = false;
+  } else {
+    e->adjacc_u = e->adjacc_u0;
+    e->adjacc_all = e->adjacc_all0;
+  }
+  int rc = DBS_OK;
+  if (!accf.empty()) {
+    dfree(&e->d_accf);
+    rc = upload(e->stream, &e->d_accf, accf);
+  }
+  if (!rc) rc = build_dense_tables(e);
+  if (!rc) rc = build_pair_tables(e);
+  if (!rc && e->featm_shm > 0) rc = build_feature_items(e);
+  if (rc) return rc;
+  compute_phase_work(e);
+  DBS_HIP(hipStreamSynchronize(e->stream));
+  if (e->gexec) { (void)hipGraphExecDestroy(e->gexec); e->gexec = nullptr; }
+  if (e->graph) { (void)hipGraphDestroy(e->graph); e->graph = nullptr; }
+  return DBS_OK;
+}
+
+int dbs_engine_memory(dbs_engine* e, int* nbatch, size_t* feature_bytes) {
+  DBS_CHECK(e, DBS_ERR_ARG, "dbs_engine_memory: null engine");
+  if (nbatch) *nbatch = (int)e->bat.size() - 1;
+  if (feature_bytes) *feature_bytes = (size_t)(2 * e->total_feat * (long long)sizeof(double));
   return DBS_OK;
 }
 

@@ -113,6 +113,18 @@ class Engine:
         p = np.ascontiguousarray(params, dtype=np.int32)
         check(lib().dbs_engine_set_tuning(self.h, len(p), p.ctypes.data_as(C.POINTER(C.c_int))))
 
+    def set_memory_budget(self, nbytes):
+        """bytes of features + gradients held at once (0: no budget); the
+        groups then run in batches that fit"""
+        check(lib().dbs_engine_set_memory_budget(self.h, int(nbytes)))
+
+    def memory(self):
+        """(batches, bytes of features + gradients held)"""
+        nb = C.c_int()
+        by = C.c_size_t()
+        check(lib().dbs_engine_memory(self.h, C.byref(nb), C.byref(by)))
+        return nb.value, by.value
+
     def get_tuning(self):
         p = np.zeros(7, dtype=np.int32)
         check(lib().dbs_engine_get_tuning(self.h, 7, p.ctypes.data_as(C.POINTER(C.c_int))))

@@ -465,6 +465,18 @@ int dbs_engine_graph_launch(dbs_engine *eng);
 int dbs_engine_set_tuning(dbs_engine *eng, int nparam, const int *params);
 int dbs_engine_get_tuning(dbs_engine *eng, int nparam, int *params);
 int dbs_engine_autotune(dbs_engine *eng, int with_adjoint, int reps);
+/* Memory budget of the features and their gradients (bytes of both; 0 =
+ * no budget, the default): the groups then run in batches of consecutive
+ * (instance, k) groups whose features and gradients fit, the two buffers
+ * reused by every batch: each batch's features are computed, contracted and
+ * turned into its share of the orbital gradient before the next batch's
+ * overwrite them.  A group larger than the budget is a batch of its own.
+ * The orbital gradient of later batches continues the sums of earlier ones,
+ * so the results do not depend on the batching; under a budget the
+ * register-summed adjoint kernel is not used.  memory: the batch count and
+ * the feature + gradient bytes held. */
+int dbs_engine_set_memory_budget(dbs_engine *eng, size_t bytes);
+int dbs_engine_memory(dbs_engine *eng, int *nbatch, size_t *feature_bytes);
 /* algorithmic flop count and minimum bytes of one forward (+adjoint) */
 int dbs_engine_work(dbs_engine *eng, int with_adjoint, double *flops, double *bytes);
 int dbs_engine_sync(dbs_engine *eng);

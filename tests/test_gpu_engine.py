@@ -538,3 +538,68 @@ def test_complete_groups_register_adjoint(setups, name, norb):
             e_pm.append(eng.energy())
         fd = (e_pm[0] - e_pm[1]) / 2e-5
         assert abs(fd - G[j, comp, o]) <= 1e-6 * max(abs(fd), 1e-3 * np.max(np.abs(G)))
+
+
+@pytest.mark.parametrize("kind", [0, 1, 5])
+def test_memory_budget_is_bit_exact(setups, kind):
+    """the large-ns path: under a memory budget the (instance, k) groups
+    run in batches whose features and gradients fit, the buffers reused;
+    two instances of different grids (fe and small knots) so batches cut
+    across an instance.  Values, energies and gradients equal the path that
+    holds every group's features at once, bit for bit (the same adjoint
+    kernels, a batch continuing the orbital-point sums of the earlier ones),
+    in eager runs and in graph replays; against the default path (which may
+    take the register-summed adjoint kernel) within 1e-13."""
+    from dbsr3_amd import Engine
+    (gf, df), (gs, ds) = setups["fe"], setups["small"]
+    rng = np.random.default_rng(700 + kind)
+    insts, pqs, coefs = [], [], []
+    for (g, d, norb, kmax, n) in ((gf, df, 9, 5, 900), (gs, ds, 12, 6, 1500)):
+        fs = random_orbitals(g, norb, seed=701 + kind + norb)
+        rows = make_rows(rng, norb, kmax, n)
+        insts.append((d, norb, rows))
+        pqs.append(pq_of(fs))
+        coefs.append(rng.uniform(-1, 1, n))
+    print(f"[memory budget kind={kind}]", flush=True)
+    eng = Engine(insts, kind=kind)
+    for i in range(2):
+        eng.set_orbitals(i, pqs[i])
+        eng.set_coef(i, coefs[i])
+
+    def run():
+        eng.forward_adjoint()
+        return ([eng.values(i) for i in range(2)], [eng.grad(i) for i in range(2)],
+                [eng.energy(i) for i in range(2)])
+
+    v0, g0, e0 = run()                       # default
+    eng.set_memory_budget(1 << 40)           # one batch, no register-summed adjoint
+    nb1, by1 = eng.memory()
+    v1, g1, e1 = run()
+    assert nb1 == 1
+    for i in range(2):
+        assert np.max(relerr(v1[i], v0[i])) < 1e-13
+        assert np.max(np.abs(g1[i] - g0[i])) <= 1e-13 * np.max(np.abs(g0[i]))
+    for frac in (0.45, 0.2, 1e-9):           # several batches; one group per batch
+        eng.set_memory_budget(max(1, int(by1 * frac)))
+        nb, by = eng.memory()
+        print(f"  budget {frac}: {nb} batches, {by} bytes of {by1}", flush=True)
+        assert nb > 1 and (by <= by1 * frac or frac < 1e-6)
+        v2, g2, e2 = run()
+        for i in range(2):
+            assert np.array_equal(v2[i], v1[i]), (frac, i)
+            assert np.array_equal(g2[i], g1[i]), (frac, i)
+            assert e2[i] == e1[i], (frac, i)
+        eng.forward()
+        for i in range(2):
+            assert np.array_equal(eng.values(i), v1[i])
+    # graph replays under a budget
+    eng.graph_capture(with_adjoint=True)
+    for _ in range(3):
+        eng.graph_launch()
+        eng.sync()
+        for i in range(2):
+            assert np.array_equal(eng.grad(i), g1[i]) and np.array_equal(eng.values(i), v1[i])
+    eng.set_memory_budget(0)
+    v3, g3, e3 = run()
+    for i in range(2):
+        assert np.array_equal(v3[i], v0[i]) and np.array_equal(g3[i], g0[i])
