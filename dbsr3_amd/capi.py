@@ -133,6 +133,7 @@ PROTOTYPES = {
     "dbs_mchf_matrix_dev": [vp, i32, i32, vp, vp, i32, ip, ip, dp, i32, ip, ip, ip, ip, ip, dp, i32, f64, f64,
                             i32, i32, vp, vp, vp, vp, dp, dp],
     "dbs_core_matrix_dev": [vp, i32, i32, vp, vp, dp, dp, i32, dp, i32, vp, vp],
+    "dbs_hf_eiv_dev": [vp, vp, i32, vp, ip, i32, vp, vp, dp],
     "dbs_engine_create": [C.POINTER(vp), i32, C.POINTER(vp), ip, ip, ip, i32],
     "dbs_engine_destroy": [vp],
     "dbs_engine_buffers": [vp, i32, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp),

@@ -245,6 +245,7 @@ int dbs_ctx_destroy(dbs_ctx* c) {
   for (double* p : bufs)
     if (p) (void)hipFree(p);
   for (auto& kv : c->rkb_cache) (void)hipFree(kv.second);
+  if (c->solver && c->solver_free) c->solver_free(c->solver);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return DBS_OK;

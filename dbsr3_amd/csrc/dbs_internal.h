@@ -55,6 +55,10 @@ struct dbs_ctx {
   bool own_stream = false;
   int ns = 0, ks = 0, ksp = 0, ksq = 0, nv = 0, nsp = 0, nsq = 0, ms = 0;
 
+  // dense-solver handle of the context (created on first use by the entry
+  // point that needs it) and the routine that releases it
+  void* solver = nullptr;
+  void (*solver_free)(void*) = nullptr;
   // knots (1-based in the reference; here t[i-1])
   double* t = nullptr;   // ns+ks
   double* tp = nullptr;  // nsp+ksp

@@ -19,6 +19,9 @@
 #include <string>
 #include <vector>
 
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
 #include "dbs_internal.h"
 
 namespace dbs {
@@ -259,6 +262,78 @@ __global__ void k_mchf_energy(int ms, const double* __restrict__ v, const double
   for (int r = threadIdx.x; r < ms; r += 256) a += v[r] * (w[r] - rhs[r]);
   const double e = block_sum256(a, sh);
   if (threadIdx.x == 0) scal[2] = e;
+}
+
+
+// ---------------------------------------------------------------------------
+// hf_eiv: orbital update of DBSR_HF (hf_solve_HF.f90:118-202)
+// ---------------------------------------------------------------------------
+// u(c) = sum_r v(r) h(r,c), rows in order (thread per column)
+__global__ void k_vt_h(int m, const double* __restrict__ v, const double* __restrict__ h, double* __restrict__ u) {
+#pragma clang fp contract(off)
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < m; c += gridDim.x * blockDim.x) {
+    double s = 0.0;
+    const double* hc = h + (long long)m * c;
+    for (int r = 0; r < m; ++r) s += v[r] * hc[r];
+    u[c] = s;
+  }
+}
+
+// h(r,c) = h(r,c) - x(r) y(c)
+__global__ void k_rank1_sub(int m, const double* __restrict__ x, const double* __restrict__ y, double* __restrict__ h) {
+#pragma clang fp contract(off)
+  const long long n = (long long)m * m;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x)
+    h[e] = h[e] - x[e % m] * y[e / m];
+}
+
+// the kept rows/columns of h and B (boundary conditions): a(p,q) = h(keep(p), keep(q))
+__global__ void k_compact2(int m, int kk, const int* __restrict__ keep, const double* __restrict__ h,
+                           const double* __restrict__ B, double* __restrict__ a, double* __restrict__ s) {
+  const long long n = (long long)kk * kk;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
+    const int p = (int)(e % kk), q = (int)(e / kk);
+    const long long src = keep[p] + (long long)m * keep[q];
+    a[e] = h[src];
+    s[e] = B[src];
+  }
+}
+
+// v = column col of Z expanded onto the kept B-splines, then the sign that
+// makes the entry of largest magnitude (first one) positive.  One workgroup
+// of 256.
+__global__ void k_expand_sign(int m, int kk, const int* __restrict__ keep, const double* __restrict__ Z, int col,
+                              double* __restrict__ v) {
+  __shared__ double sa[256];
+  __shared__ int si[256];
+  for (int r = threadIdx.x; r < m; r += 256) v[r] = 0.0;
+  __syncthreads();
+  for (int p = threadIdx.x; p < kk; p += 256) v[keep[p]] = Z[p + (long long)kk * col];
+  __syncthreads();
+  double best = -1.0;
+  int bi = m;
+  for (int r = threadIdx.x; r < m; r += 256) {
+    const double a = fabs(v[r]);
+    if (a > best) { best = a; bi = r; }
+  }
+  sa[threadIdx.x] = best;
+  si[threadIdx.x] = bi;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      const double a = sa[threadIdx.x + w];
+      const int i = si[threadIdx.x + w];
+      if (a > sa[threadIdx.x] || (a == sa[threadIdx.x] && i < si[threadIdx.x])) {
+        sa[threadIdx.x] = a;
+        si[threadIdx.x] = i;
+      }
+    }
+    __syncthreads();
+  }
+  const bool flip = si[0] < m && v[si[0]] < 0.0;
+  __syncthreads();
+  if (flip)
+    for (int r = threadIdx.x; r < m; r += 256) v[r] = -v[r];
 }
 
 namespace {
@@ -972,6 +1047,87 @@ int core_matrix(dbs_ctx* ctx, int i, int norb, const double* pq, const double* h
   return DBS_OK;
 }
 
+
+void free_solver(void* h) { (void)rocblas_destroy_handle((rocblas_handle)h); }
+
+int solver_of(dbs_ctx* ctx, rocblas_handle* h) {
+  if (!ctx->solver) {
+    rocblas_handle x = nullptr;
+    if (rocblas_create_handle(&x) != rocblas_status_success) return fail(DBS_ERR_HIP, "hf_eiv: solver handle");
+    ctx->solver = x;
+    ctx->solver_free = free_solver;
+  }
+  *h = (rocblas_handle)ctx->solver;
+  if (rocblas_set_stream(*h, ctx->stream) != rocblas_status_success)
+    return fail(DBS_ERR_HIP, "hf_eiv: solver stream");
+  return DBS_OK;
+}
+
+// hf_eiv (hf_solve_HF.f90:136-202): orthogonality to each vorth column
+// (H <- (1 - B v v^T) H (1 - v v^T B), orthogonality in hf_orthogonality.f90),
+// the B-splines with iprm = 0 removed, H x = e B x (lower triangles), the
+// m-th eigenvalue above -1.999 c^2 and its eigenvector on the full net with
+// its largest entry positive
+int hf_eiv(dbs_ctx* ctx, double* hfm, int northo, const double* vorth, const int* iprm, int m, const double* B,
+           double* v, double* e_out) {
+  const int ms = ctx->ms;
+  hipStream_t s = ctx->stream;
+  DevMem mem(s);
+  double *w, *u;
+  int rc;
+  if ((rc = mem.alloc(&w, ms)) || (rc = mem.alloc(&u, ms))) return rc;
+  for (int j = 0; j < northo; ++j) {
+    const double* vj = vorth + (size_t)ms * j;
+    k_set<<<grid_for(ms, 256), 256, 0, s>>>(ms, w, nullptr, 0.0);
+    k_gemv_add<<<grid_for(ms, 64), 64, 0, s>>>(ms, B, vj, w);  // w = B v
+    k_vt_h<<<grid_for(ms, 64), 64, 0, s>>>(ms, vj, hfm, u);     // u = v^T H
+    k_rank1_sub<<<grid_for((long long)ms * ms, 256), 256, 0, s>>>(ms, w, u, hfm);  // H - w u^T
+    k_set<<<grid_for(ms, 256), 256, 0, s>>>(ms, u, nullptr, 0.0);
+    k_gemv_add<<<grid_for(ms, 64), 64, 0, s>>>(ms, hfm, vj, u);  // u = H v
+    k_rank1_sub<<<grid_for((long long)ms * ms, 256), 256, 0, s>>>(ms, u, w, hfm);  // H - u w^T
+  }
+  std::vector<int> keep;
+  for (int r = 0; r < ms; ++r)
+    if (iprm[r] != 0) keep.push_back(r);
+  const int kk = (int)keep.size();
+  DBS_CHECK(kk > 0, DBS_ERR_ARG, "hf_eiv: every B-spline removed");
+  int* d_keep;
+  double *A, *S, *D, *E;
+  rocblas_int* info;
+  if ((rc = mem.put(keep, &d_keep)) || (rc = mem.alloc(&A, (size_t)kk * kk)) || (rc = mem.alloc(&S, (size_t)kk * kk)) ||
+      (rc = mem.alloc(&D, kk)) || (rc = mem.alloc(&E, kk)) || (rc = mem.alloc(&info, 1)))
+    return rc;
+  k_compact2<<<grid_for((long long)kk * kk, 256), 256, 0, s>>>(ms, kk, d_keep, hfm, B, A, S);
+  DBS_HIP(hipGetLastError());
+  rocblas_handle h;
+  if ((rc = solver_of(ctx, &h))) return rc;
+  if (rocsolver_dsygvd(h, rocblas_eform_ax, rocblas_evect_original, rocblas_fill_lower, kk, A, kk, S, kk, D, E,
+                       info) != rocblas_status_success)
+    return fail(DBS_ERR_HIP, "hf_eiv: rocsolver_dsygvd");
+  std::vector<double> ev(kk);
+  rocblas_int hinfo = 0;
+  hipError_t he = hipMemcpyAsync(ev.data(), D, sizeof(double) * kk, hipMemcpyDeviceToHost, s);
+  if (he == hipSuccess) he = hipMemcpyAsync(&hinfo, info, sizeof(hinfo), hipMemcpyDeviceToHost, s);
+  if (he == hipSuccess) he = hipStreamSynchronize(s);
+  if (he != hipSuccess) return fail(DBS_ERR_HIP, std::string("hf_eiv: ") + hipGetErrorString(he));
+  if (hinfo != 0) return fail(DBS_ERR_HIP, "Error in Eigenvalue routine, dsygv " + std::to_string(hinfo));
+  // the first eigenvalue above the negative-energy continuum, then m - 1 more
+  const double zz = -1.999 * 137.03599976 * 137.03599976;
+  int mm = 0;
+  for (int j = 1; j <= kk; ++j) {
+    mm = j;
+    if (ev[j - 1] > zz) break;
+  }
+  mm = m + mm - 1;
+  DBS_CHECK(mm >= 1 && mm <= kk, DBS_ERR_ARG, "hf_eiv: eigenvalue index out of range");
+  k_expand_sign<<<1, 256, 0, s>>>(ms, kk, d_keep, A, mm - 1, v);
+  he = hipGetLastError();
+  if (he == hipSuccess) he = hipStreamSynchronize(s);
+  if (he != hipSuccess) return fail(DBS_ERR_HIP, std::string("hf_eiv: ") + hipGetErrorString(he));
+  if (e_out) *e_out = ev[mm - 1];
+  return DBS_OK;
+}
+
 int rs_data(dbs_ctx* ctx, const dbs_mat_blocks* m, const dbs_mat3_aux* aux, int stype, int itype, int atype, int k,
             int nc, const double* c, const int* K1, const int* K2, const int* K3, const int* K4,
             const double* pq_dev, int norb) {
@@ -1082,6 +1238,14 @@ int dbs_core_matrix_dev(dbs_ctx* ctx, int i, int norb, const double* pq_dev, con
   DBS_CHECK(i >= 1 && i <= norb && kmax >= 0, DBS_ERR_ARG, "core_matrix: orbital or kmax out of range");
   DBS_CHECK(qsum[i - 1] != 0.0, DBS_ERR_ARG, "core_matrix: qsum = 0");
   return core_matrix(ctx, i, norb, pq_dev, hd_dev, acoef, bcoef, kmax, qsum, newton, hfm_dev, hx_dev);
+}
+
+int dbs_hf_eiv_dev(dbs_ctx* ctx, double* hfm_dev, int northo, const double* vorth_dev, const int* iprm, int m,
+                   const double* fppqq_dev, double* v_dev, double* e_i) {
+  DBS_CHECK(ctx && hfm_dev && iprm && fppqq_dev && v_dev && (northo <= 0 || vorth_dev), DBS_ERR_ARG,
+            "hf_eiv: null argument");
+  DBS_CHECK(m >= 1, DBS_ERR_ARG, "hf_eiv: m < 1");
+  return hf_eiv(ctx, hfm_dev, northo, vorth_dev, iprm, m, fppqq_dev, v_dev, e_i);
 }
 
 int dbs_bav_dev(dbs_ctx* ctx, const double* a_dev, const double* b_dev, char sym, char dir, double* c_dev) {

@@ -316,6 +316,25 @@ class DBS:
         self._post()
         return hfm, hx
 
+    def hf_eiv(self, hfm, vorth, iprm, m, fppqq):
+        """hf_eiv (DBSR_HF/hf_solve_HF.f90:118-202): hfm device tensor holding
+        hfm(ms,ms) (overwritten by the projections), vorth a list of device
+        tensors (ms) to project out, iprm (ms) host 0/1, m the wanted
+        positive-energy eigenvalue, fppqq device tensor (ms,ms).  Returns
+        (v device tensor (ms), e)."""
+        torch = _torch()
+        ms = self.ms
+        vo = torch.stack(list(vorth)) if len(vorth) else None
+        ip = np.ascontiguousarray(iprm, dtype=np.int32)
+        v = self._empty(ms)
+        e = C.c_double()
+        self._pre()
+        check(lib().dbs_hf_eiv_dev(self.ctx, _ptr(hfm), 0 if vo is None else int(vo.shape[0]),
+                                   _ptr(vo) if vo is not None else None, ip.ctypes.data_as(C.POINTER(C.c_int)),
+                                   int(m), _ptr(fppqq), _ptr(v), C.byref(e)))
+        self._post()
+        return v, e.value
+
     def UPDATE_HS(self, hm, ii, jj, d, sym):
         """UPDATE_HS(ms,hm,ii,jj,ns,ks,d,sym): hm device tensor holding (ms,ms)"""
         self._pre()

@@ -422,6 +422,19 @@ int dbs_mchf_matrix_dev(dbs_ctx *ctx, int i, int norb, const double *pq_dev, con
 int dbs_core_matrix_dev(dbs_ctx *ctx, int i, int norb, const double *pq_dev, const double *hd_dev,
                         const double *acoef, const double *bcoef, int kmax, const double *qsum, int newton,
                         double *hfm_dev, double *hx_dev);
+/* hf_eiv (DBSR_HF/hf_solve_HF.f90:118-202): the orbital update of DBSR_HF.
+ * hfm_dev(ms,ms) (overwritten) is first projected orthogonal to each of the
+ * northo vectors vorth_dev(ms,northo) (orthogonality,
+ * DBSR_HF/hf_orthogonality.f90:2-29: H <- (1 - B v v^T) H (1 - v v^T B),
+ * B = fppqq_dev), the B-splines with iprm[r] = 0 (host, ms entries;
+ * Boundary_conditions, DBSR_HF/hf_boundary.f90:2-32) are removed, H x = e B x
+ * is solved on the device (rocSOLVER dsygvd, lower triangles, as dsygv there),
+ * and the m-th eigenvalue above -1.999 c^2 (m = n - l minus the orthogonality
+ * conditions to lower orbitals) is returned in *e_i with its eigenvector,
+ * B-normalised, on the full B-spline net in v_dev(ms), its largest entry
+ * positive.  Stop texts: "Error in Eigenvalue routine, dsygv". */
+int dbs_hf_eiv_dev(dbs_ctx *ctx, double *hfm_dev, int northo, const double *vorth_dev, const int *iprm, int m,
+                   const double *fppqq_dev, double *v_dev, double *e_i);
 /* BAV (called at DBSR_MAT3/R_data.f90:181, S_data.f90:205; defined in the
  * un-vendored Libraries_03): c(ns) = A b (dir 'r') or A^T b (dir 'l'), A the
  * (ns,ns) matrix held in storage sym 's','l','n','x'. */

@@ -1228,3 +1228,68 @@ def l_data(ms, itype, lst, eps_c, ibo, ipbs, kbort, vch, fppqq, iitar, iptar, ht
                 hx(ich, jch, hd[:, :, q], C_)
             elif check_target == 1:
                 _target(mats, iptar, otarg, ich, jch, C_)
+
+
+# --------------------------------------------------------------------------
+# DBSR_HF's orbital update, restated: Boundary_conditions
+# (DBSR_HF/hf_boundary.f90:2-32), orthogonality (hf_orthogonality.f90:2-29)
+# and hf_eiv (hf_solve_HF.f90:118-202), the generalised eigenproblem by
+# LAPACK through scipy (dsygvd; the reference calls dsygv: the same
+# problem, another algorithm).
+# --------------------------------------------------------------------------
+C_AU = 137.03599976
+
+
+def boundary_conditions(ns, nsp, nsq, ksp, ksq, lbs, kbs, nuclear="Fermi", ilzero=1, ibzero=2):
+    """iprm(ms, nbf) of 0/1 (0 = B-spline removed)"""
+    nbf = len(lbs)
+    ms = 2 * ns
+    iprm = np.ones((ms, nbf), dtype=np.int32)
+    for i in range(nbf):
+        if nuclear == "point" or ilzero == 0:
+            iprm[0, i] = 0
+            iprm[nsp - ibzero:ns, i] = 0
+            iprm[ns, i] = 0
+            iprm[nsq - ibzero + ns:ms, i] = 0
+        else:
+            j = lbs[i] + 1
+            if j > ksp - 1:
+                j = 1
+            iprm[0:j, i] = 0
+            iprm[nsp - ibzero:ns, i] = 0
+            j = lbs[i] + 2 if kbs[i] < 0 else lbs[i]
+            if j > ksq - 1:
+                j = 1
+            iprm[ns:ns + j, i] = 0
+            iprm[nsq - ibzero + ns:ms, i] = 0
+    return iprm
+
+
+def hf_eiv(hfm, vorth, iprm, m, fppqq):
+    """(v, e): the orbital update of hf_eiv for the matrix hfm (ms,ms), the
+    vectors to project out vorth (list of ms), iprm (ms) and m"""
+    import scipy.linalg
+    h = np.array(hfm, dtype=np.float64)
+    B = np.asarray(fppqq)
+    ms = h.shape[0]
+    for v in vorth:
+        w = B @ v
+        c = -np.outer(w, v) + np.eye(ms)
+        h = c @ h
+        h = h @ c.T
+    keep = np.nonzero(np.asarray(iprm))[0]
+    aa = h[np.ix_(keep, keep)]
+    ss = B[np.ix_(keep, keep)]
+    ev, z = scipy.linalg.eigh(aa, ss, lower=True)
+    zz = -1.999 * C_AU ** 2
+    mm = 0
+    for j in range(1, len(ev) + 1):
+        mm = j
+        if ev[j - 1] > zz:
+            break
+    mm = m + mm - 1
+    v = np.zeros(ms)
+    v[keep] = z[:, mm - 1]
+    if v[int(np.argmax(np.abs(v)))] < 0.0:
+        v = -v
+    return v, float(ev[mm - 1])
