@@ -192,6 +192,40 @@
         Real(c_double), value :: eps_c
         Integer(c_int) :: dbs_o_data_dev
        End Function
+       Function dbs_ldata_dev(ctx, m, aux, itype, n, k1, k2, k3, k4, c, eps_c, pq_dev) &
+                bind(C, name='dbs_ldata_dev')
+         Import :: c_int, c_ptr, c_double
+         Type(c_ptr), value :: ctx, m, aux, k1, k2, k3, k4, c, pq_dev
+         Integer(c_int), value :: itype, n
+         Real(c_double), value :: eps_c
+         Integer(c_int) :: dbs_ldata_dev
+       End Function
+       Function dbs_mchf_matrix_dev(ctx, i, norb, pq_dev, dhl_dev, nl, l_i1, l_i2, l_c, nr, r_k, r_i1, r_i2, &
+                r_i3, r_i4, r_c, kmax, eps_c, qsum_min, newton, irhs, fppqq_dev, hfm_dev, rhs_dev, hx_dev, &
+                e_i, qsum_i) bind(C, name='dbs_mchf_matrix_dev')
+         Import :: c_int, c_ptr, c_double
+         Type(c_ptr), value :: ctx, pq_dev, dhl_dev, l_i1, l_i2, l_c, r_k, r_i1, r_i2, r_i3, r_i4, r_c, &
+                               fppqq_dev, hfm_dev, rhs_dev, hx_dev
+         Integer(c_int), value :: i, norb, nl, nr, kmax, newton, irhs
+         Real(c_double), value :: eps_c, qsum_min
+         Real(c_double) :: e_i, qsum_i
+         Integer(c_int) :: dbs_mchf_matrix_dev
+       End Function
+       Function dbs_core_matrix_dev(ctx, i, norb, pq_dev, hd_dev, acoef, bcoef, kmax, qsum, newton, hfm_dev, &
+                hx_dev) bind(C, name='dbs_core_matrix_dev')
+         Import :: c_int, c_ptr
+         Type(c_ptr), value :: ctx, pq_dev, hd_dev, acoef, bcoef, qsum, hfm_dev, hx_dev
+         Integer(c_int), value :: i, norb, kmax, newton
+         Integer(c_int) :: dbs_core_matrix_dev
+       End Function
+       Function dbs_hf_eiv_dev(ctx, hfm_dev, northo, vorth_dev, iprm, m, fppqq_dev, v_dev, e_i) &
+                bind(C, name='dbs_hf_eiv_dev')
+         Import :: c_int, c_ptr, c_double
+         Type(c_ptr), value :: ctx, hfm_dev, vorth_dev, iprm, fppqq_dev, v_dev
+         Integer(c_int), value :: northo, m
+         Real(c_double) :: e_i
+         Integer(c_int) :: dbs_hf_eiv_dev
+       End Function
        Function dbs_hf_matrix_dev(ctx, i, norb, pq_dev, acoef, bcoef, kmax, qsum, mbreit, nrk, kr1, kr2, &
                 kr3, kr4, crk, hfm_dev) bind(C, name='dbs_hf_matrix_dev')
         Import :: c_ptr, c_int, c_double
@@ -490,6 +524,93 @@
       Call dbs_gpu_check(dbs_free(ctx, pqd))
       Call dbs_gpu_check(dbs_free(ctx, hd))
       End Subroutine dbs_gpu_hf_matrix
+
+!----------------------------------------------------------------------
+      Subroutine dbs_gpu_mchf_matrix(i, norb, pq, dhl, nl, l_i1, l_i2, l_c, nr, r_k, r_i1, r_i2, r_i3, r_i4, &
+                                     r_c, kmax, eps_c, qsum_min, newton, irhs, fppqq, hfm, rhs, hx, e_i, qsum_i)
+!----------------------------------------------------------------------
+!     mchf_matrix(i,hfm,rhs,hx) (DBSR_MCHF/mchf_matrix.f90:2-248) on the GPU
+!     with the module data passed explicitly: orbitals pq(ns,2,norb),
+!     dhl = Get_hd_core(kbs(i)), the L and R^k lists flattened per angular
+!     coefficient with c = WW(ic,jc)*coef, eps_c, qsum_min, newton, irhs,
+!     fppqq.  Returns hfm, rhs, hx, e(i) and qsum(i).
+      Use dbs_gpu_api
+      Use dbs_gpu_state, only: ctx, ns_g
+      Use dbs_gpu_mem
+      Implicit none
+      Integer, intent(in) :: i, norb, nl, nr, kmax, newton, irhs
+      Integer(c_int), intent(in), target :: l_i1(*), l_i2(*), r_k(*), r_i1(*), r_i2(*), r_i3(*), r_i4(*)
+      Real(8), intent(in), target :: pq(ns_g, 2, norb), dhl(2 * ns_g, 2 * ns_g), l_c(*), r_c(*), &
+                                     fppqq(2 * ns_g, 2 * ns_g)
+      Real(8), intent(in) :: eps_c, qsum_min
+      Real(8), intent(out), target :: hfm(2 * ns_g, 2 * ns_g), rhs(2 * ns_g), hx(2 * ns_g, 2 * ns_g)
+      Real(8), intent(out) :: e_i, qsum_i
+      Type(c_ptr) :: pqd, dd, bd, hd, rd, xd
+      Integer :: mm
+      mm = 4 * ns_g * ns_g
+      pqd = dbs_gpu_put(pq, ns_g * 2 * norb); dd = dbs_gpu_put(dhl, mm); bd = dbs_gpu_put(fppqq, mm)
+      hd = dbs_gpu_put(hfm, mm); rd = dbs_gpu_put(rhs, 2 * ns_g); xd = dbs_gpu_put(hx, mm)
+      Call dbs_gpu_check(dbs_mchf_matrix_dev(ctx, i, norb, pqd, dd, nl, c_loc(l_i1), c_loc(l_i2), c_loc(l_c), &
+                         nr, c_loc(r_k), c_loc(r_i1), c_loc(r_i2), c_loc(r_i3), c_loc(r_i4), c_loc(r_c), kmax, &
+                         eps_c, qsum_min, newton, irhs, bd, hd, rd, xd, e_i, qsum_i))
+      Call dbs_gpu_get(hd, hfm, mm); Call dbs_gpu_get(rd, rhs, 2 * ns_g); Call dbs_gpu_get(xd, hx, mm)
+      Call dbs_gpu_check(dbs_free(ctx, pqd)); Call dbs_gpu_check(dbs_free(ctx, dd))
+      Call dbs_gpu_check(dbs_free(ctx, bd)); Call dbs_gpu_check(dbs_free(ctx, hd))
+      Call dbs_gpu_check(dbs_free(ctx, rd)); Call dbs_gpu_check(dbs_free(ctx, xd))
+      End Subroutine dbs_gpu_mchf_matrix
+
+!----------------------------------------------------------------------
+      Subroutine dbs_gpu_core_matrix(i, norb, pq, hd, acoef, bcoef, kmax, qsum, newton, hfm, hx)
+!----------------------------------------------------------------------
+!     core_matrix(i,hfm,hx) (DBSR_MCHF/mchf_matrix.f90:253-345) on the GPU:
+!     hd = Get_hd(kbs(i)), a(i,j,k) / b(i,j,k) as acoef/bcoef(norb,0:kmax),
+!     qsum(norb)
+      Use dbs_gpu_api
+      Use dbs_gpu_state, only: ctx, ns_g
+      Use dbs_gpu_mem
+      Implicit none
+      Integer, intent(in) :: i, norb, kmax, newton
+      Real(8), intent(in), target :: pq(ns_g, 2, norb), hd(2 * ns_g, 2 * ns_g), acoef(norb, 0:kmax), &
+                                     bcoef(norb, 0:kmax), qsum(norb)
+      Real(8), intent(out), target :: hfm(2 * ns_g, 2 * ns_g), hx(2 * ns_g, 2 * ns_g)
+      Type(c_ptr) :: pqd, dd, hfd, xd
+      Integer :: mm
+      mm = 4 * ns_g * ns_g
+      pqd = dbs_gpu_put(pq, ns_g * 2 * norb); dd = dbs_gpu_put(hd, mm)
+      hfd = dbs_gpu_put(hfm, mm); xd = dbs_gpu_put(hx, mm)
+      Call dbs_gpu_check(dbs_core_matrix_dev(ctx, i, norb, pqd, dd, c_loc(acoef), c_loc(bcoef), kmax, &
+                         c_loc(qsum), newton, hfd, xd))
+      Call dbs_gpu_get(hfd, hfm, mm); Call dbs_gpu_get(xd, hx, mm)
+      Call dbs_gpu_check(dbs_free(ctx, pqd)); Call dbs_gpu_check(dbs_free(ctx, dd))
+      Call dbs_gpu_check(dbs_free(ctx, hfd)); Call dbs_gpu_check(dbs_free(ctx, xd))
+      End Subroutine dbs_gpu_core_matrix
+
+!----------------------------------------------------------------------
+      Subroutine dbs_gpu_hf_eiv(hfm, northo, vorth, iprm, m, fppqq, v, e_i)
+!----------------------------------------------------------------------
+!     hf_eiv(i,hfm,v) (DBSR_HF/hf_solve_HF.f90:118-202) on the GPU with the
+!     module data passed explicitly: the orbitals to project out
+!     vorth(ms,northo) (the lower orbitals of the same kappa), iprm(ms) of
+!     Boundary_conditions, m = n - l - northo.  Returns v(ms) and e(i,i).
+      Use dbs_gpu_api
+      Use dbs_gpu_state, only: ctx, ns_g
+      Use dbs_gpu_mem
+      Implicit none
+      Integer, intent(in) :: northo, m
+      Integer(c_int), intent(in), target :: iprm(2 * ns_g)
+      Real(8), intent(in), target :: hfm(2 * ns_g, 2 * ns_g), vorth(2 * ns_g, *), fppqq(2 * ns_g, 2 * ns_g)
+      Real(8), intent(out), target :: v(2 * ns_g)
+      Real(8), intent(out) :: e_i
+      Type(c_ptr) :: hd, od, bd, vd
+      Integer :: mm
+      mm = 4 * ns_g * ns_g
+      hd = dbs_gpu_put(hfm, mm); bd = dbs_gpu_put(fppqq, mm); vd = dbs_gpu_put(v, 2 * ns_g)
+      od = dbs_gpu_put(vorth(:, 1:max(northo, 1)), 2 * ns_g * max(northo, 1))
+      Call dbs_gpu_check(dbs_hf_eiv_dev(ctx, hd, northo, od, c_loc(iprm), m, bd, vd, e_i))
+      Call dbs_gpu_get(vd, v, 2 * ns_g)
+      Call dbs_gpu_check(dbs_free(ctx, hd)); Call dbs_gpu_check(dbs_free(ctx, bd))
+      Call dbs_gpu_check(dbs_free(ctx, vd)); Call dbs_gpu_check(dbs_free(ctx, od))
+      End Subroutine dbs_gpu_hf_eiv
 
 !----------------------------------------------------------------------
 !     the batched engine for an SCF loop: E = sum_n coef_n R^k(rows_n)

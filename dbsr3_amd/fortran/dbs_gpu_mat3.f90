@@ -6,6 +6,7 @@
 !     Call Add_matrix(jtype,jpol)     (DBSR_MAT3/MOD_c_data.f90:318-368)
 !     Call Gen_matrix(itype,kpol)     (DBSR_MAT3/gen_matrix.f90:2-17)
 !     Call O_data(itype)              (DBSR_MAT3/O_data.f90:2-93)
+!     Call L_data(itype)              (DBSR_MAT3/L_data.f90:2-145)
 !     Call R_data(itype,kpol,btype)   (DBSR_MAT3/R_data.f90:2-290)
 !     Call S_data(ktype,kpol)         (DBSR_MAT3/S_data.f90:2-287)
 ! Module dbs_gpu_mat plays the part of module dbsr_mat (icase, ibo, mk,
@@ -14,8 +15,10 @@
 ! hands it over with dbs_gpu_mat_begin, sets icase with dbs_gpu_mat_case
 ! as State_res does, and takes the matrices back with dbs_gpu_mat_fetch.
 ! The host program's own KBORT(i1,i2) and QUADR_PQ(i1,i2,m) (Libraries_03)
-! are called back through bind(C) trampolines.  L-integrals (icase 1)
-! stop, as the library does not provide L_data.  Built into
+! are called back through bind(C) trampolines.  For L-integrals (icase 1)
+! the host hands over, with dbs_gpu_mat_one_electron, what L_data reads
+! from Libraries_03: Lval(i,j) of the orbitals as a table and the
+! one-electron operators hd(ms,ms,nkap) of module dhl_core.  Built into
 ! libdbsr3_amd_mat3.a, linked into the host program.
 !======================================================================
       Module dbs_gpu_mat
@@ -50,6 +53,13 @@
        Real(c_double) :: eps_c
        Integer(c_int) :: check_target
       End Type
+      Type, bind(C) :: ldata_aux_t         ! struct dbs_ldata_aux
+       Type(c_ptr) :: mat3, otarg
+       Integer(c_int) :: check_target, norb
+       Type(c_ptr) :: lval, kbs
+       Integer(c_int) :: nkap
+       Type(c_ptr) :: kap_list, hd_dev
+      End Type
       Type(mat_blocks_t), target :: mb
       Type(mat3_aux_t), target :: aux
       Type(mat3_orbs_t), target :: orbs
@@ -59,6 +69,12 @@
       Integer(c_int), allocatable, target :: lbs(:), kbs(:), ipbs(:), ipbs0(:), icc(:,:), icb(:,:), &
                                              ibb(:,:), iptar(:)
       Type(c_ptr) :: otarg = c_null_ptr
+      Integer :: check_t = 1
+      ! L_data's one-electron data (dbs_gpu_mat_one_electron)
+      Type(ldata_aux_t), target :: laux
+      Real(8), allocatable, target :: lval(:,:)
+      Integer(c_int), allocatable, target :: kap_list(:)
+      Type(c_ptr) :: hd_dev = c_null_ptr
       Integer :: nc_cur = 0
       Type(c_ptr) :: c_cur, k1_cur, k2_cur, k3_cur, k4_cur
       End Module dbs_gpu_mat
@@ -167,6 +183,7 @@
       otarg = c_loc(otarg_)
       orbs%norb = norb; orbs%lbs = c_loc(lbs); orbs%kbs = c_loc(kbs); orbs%ipbs = c_loc(ipbs)
       orbs%ibo = ibo_; orbs%mk = mk_; orbs%eps_c = eps_c_; orbs%check_target = check_target_
+      check_t = check_target_
       Call dbs_gpu_check(dbs_cdata_create(cdata, ntype, 0, mk_ + 1, mblock, nblock, kblock, eps_cdata, &
                          c_funloc(dbs_gpu_mat_gen), c_null_ptr))
       End Subroutine dbs_gpu_mat_begin
@@ -182,6 +199,28 @@
       p = dbs_gpu_put(z, n)
       Deallocate(z)
       End Subroutine dbs_gpu_zeros
+
+      Subroutine dbs_gpu_mat_one_electron(lval_, nkap, kap_list_, hd)
+!     L_data's inputs from Libraries_03: Lval(i,j) of the orbitals 1..norb
+!     and the one-electron operators hd(ms,ms,nkap) of kappas kap_list
+      Use dbs_gpu_api
+      Use dbs_gpu_mem
+      Use dbs_gpu_state, only: ctx, ns_g
+      Use dbs_gpu_mat
+      Implicit none
+      Integer, intent(in) :: nkap, kap_list_(nkap)
+      Real(8), intent(in) :: lval_(norb, norb)
+      Real(8), intent(in), target :: hd(2 * ns_g, 2 * ns_g, nkap)
+      if (.not. c_associated(cdata)) Stop 'dbs_gpu_mat_one_electron: call dbs_gpu_mat_begin first'
+      if (allocated(lval)) Deallocate(lval, kap_list)
+      if (c_associated(hd_dev)) Call dbs_gpu_check(dbs_free(ctx, hd_dev))
+      Allocate(lval(norb, norb), kap_list(nkap))
+      lval = lval_; kap_list = kap_list_
+      hd_dev = dbs_gpu_put(hd, ms * ms * nkap)
+      laux%mat3 = c_loc(aux); laux%otarg = otarg; laux%check_target = check_t; laux%norb = norb
+      laux%lval = c_loc(lval); laux%kbs = c_loc(kbs); laux%nkap = nkap; laux%kap_list = c_loc(kap_list)
+      laux%hd_dev = hd_dev
+      End Subroutine dbs_gpu_mat_one_electron
 
       Subroutine dbs_gpu_mat_case(icase_)
 !     the kind of data State_res feeds next: 0 overlaps, 1 L, 2 R, 3 S
@@ -219,19 +258,22 @@
       end if
       cdata = c_null_ptr
       if (allocated(lbs)) Deallocate(lbs, kbs, ipbs, ipbs0, icc, icb, ibb, iptar)
+      if (allocated(lval)) Deallocate(lval, kap_list)
+      if (c_associated(hd_dev)) Call dbs_gpu_check(dbs_free(ctx, hd_dev))
+      hd_dev = c_null_ptr
       End Subroutine dbs_gpu_mat_end
 
 !======================================================================
       Subroutine Add_integral(kpol, i1, i2, i3, i4, C, ic, jc, io, jo)
 !======================================================================
-!     DBSR_MAT3/add_integral.f90:2-57: icase 0 -> Idef_Otype, 2 -> parity
-!     rule and Idef_Rtype, 3 -> SMU and Idef_itype, into the lists
+!     DBSR_MAT3/add_integral.f90:2-57: icase 0 -> Idef_Otype, 1 ->
+!     Idef_Ltype, 2 -> parity rule and Idef_Rtype, 3 -> SMU and Idef_itype,
+!     into the lists
       Use dbs_gpu_api
       Use dbs_gpu_mat
       Implicit none
       Integer, intent(in) :: kpol, i1, i2, i3, i4, ic, jc, io, jo
       Real(8), intent(in) :: C
-      if (icase == 1) Stop 'Add_integral: L-integrals are not provided by this library'
       Call dbs_gpu_check(dbs_add_integral_mat3(cdata, c_loc(orbs), icase, kpol, i1, i2, i3, i4, C, ic, jc, &
                          io, jo))
       End Subroutine Add_integral
@@ -253,7 +295,7 @@
       Integer, intent(in) :: itype, kpol
       Select case(icase)
        Case(0); Call O_data(itype)
-       Case(1); Stop 'Gen_matrix: L-integrals are not provided by this library'
+       Case(1); Call L_data(itype)
        Case(2)
         Call R_data(itype, kpol, 0)
         Call R_data(itype, kpol, 1)
@@ -282,6 +324,18 @@
                          aux%ipbs, c_funloc(dbs_gpu_kbort), c_null_ptr, aux%vch_dev, aux%nvch, aux%fppqq_dev, &
                          aux%iitar, aux%iptar, otarg))
       End Subroutine O_data
+
+      Subroutine L_data(itype)
+      Use dbs_gpu_api
+      Use dbs_gpu_state, only: ctx
+      Use dbs_gpu_mat
+      Implicit none
+      Integer, intent(in) :: itype
+      if (nc_cur <= 0) Return
+      if (.not. allocated(lval)) Stop 'L_data: call dbs_gpu_mat_one_electron first'
+      Call dbs_gpu_check(dbs_ldata_dev(ctx, c_loc(mb), c_loc(laux), itype, nc_cur, k1_cur, k2_cur, k3_cur, &
+                         k4_cur, c_cur, eps_c, pq_dev))
+      End Subroutine L_data
 
       Subroutine R_data(itype, kpol, btype)
       Use dbs_gpu_api

@@ -1,12 +1,13 @@
 !======================================================================
 ! drv_mat3.f90 -- a host program written the way DBSR_MAT3's State_res is
 ! (DBSR_MAT3/state_res.f90:160-191): per kind of data (icase 0 overlaps,
-! 2 R-integrals, 3 S-integrals) it calls Add_integral(kpol,j1,j2,j3,j4,C,
-! ic,jc,io,jo) for every term, then Add_matrix(itype,kpol) over all types
-! and multipoles.  Add_integral, Add_matrix, Gen_matrix, O_data, R_data
-! and S_data come from libdbsr3_amd_mat3.a (dbsr3_amd/fortran/
-! dbs_gpu_mat3.f90); KBORT and QUADR_PQ are this program's own, as in the
-! reference's host programs.  Reads drv_mat3_in.bin, writes
+! 1 L-integrals, 2 R-integrals, 3 S-integrals) it calls Add_integral(kpol,
+! j1,j2,j3,j4,C,ic,jc,io,jo) for every term, then Add_matrix(itype,kpol)
+! over all types and multipoles.  Add_integral, Add_matrix, Gen_matrix,
+! O_data, L_data, R_data and S_data come from libdbsr3_amd_mat3.a
+! (dbsr3_amd/fortran/dbs_gpu_mat3.f90); KBORT and QUADR_PQ are this
+! program's own, and it hands over Lval and the one-electron operators, as
+! the reference's host programs hold them.  Reads drv_mat3_in.bin, writes
 ! drv_mat3_out.bin (tests/test_gpu_fortran.py).
 !======================================================================
       Module drv_mat3_data
@@ -35,7 +36,9 @@
       Use drv_mat3_data
       Implicit none
       Integer :: ns, ks, ksp, ksq, nbound, ibo, mk, nch, npert, nvch, iitar, nterm, ntarg, n, icase, it, kp
-      Integer :: nblk, ncb, nbb
+      Integer :: nblk, ncb, nbb, nkap
+      Real(8), allocatable :: lval(:,:), hd(:,:,:)
+      Integer, allocatable :: kap_list(:)
       Real(8), allocatable :: t(:), vch(:,:), fppqq(:,:), C(:), htarg(:), otarg(:), hch(:), acf(:), hcp(:), hp(:)
       Integer, allocatable :: lbs(:), kbs(:), ipbs(:), icc(:,:), icb(:,:), ibb(:,:), iptar(:), term(:,:)
 
@@ -50,14 +53,17 @@
       Allocate(vch(2 * ns, nvch), fppqq(2 * ns, 2 * ns)); Read(11) vch; Read(11) fppqq
       Read(11) nterm
       Allocate(term(10, nterm), C(nterm)); Read(11) term; Read(11) C
+      Read(11) nkap
+      Allocate(lval(norb, norb), kap_list(nkap), hd(2 * ns, 2 * ns, nkap))
+      Read(11) lval; Read(11) kap_list; Read(11) hd
       Close(11)
 
       Allocate(htarg(ntarg), otarg(ntarg)); htarg = 0.d0; otarg = 0.d0
       Call dbs_gpu_init(ns, ks, ksp, ksq, t)
       Call dbs_gpu_mat_begin(norb, nbound, pq, lbs, kbs, ipbs, ibo, mk, 1.d-10, 1, nch, npert, icc, icb, ibb, &
                              nvch, vch, fppqq, iitar, iptar, htarg, otarg, 19, 64, 400, 40, 1.d-14)
+      Call dbs_gpu_mat_one_electron(lval, nkap, kap_list, hd)
       Do icase = 0, 3
-       if (icase == 1) Cycle
        Call dbs_gpu_mat_case(icase)
        Do n = 1, nterm
         if (term(1, n) /= icase) Cycle

@@ -22,11 +22,30 @@ def test_fortran_dropin(tmp_path, gpu_available):
     fs = random_orbitals(g, 4, seed=44)
     pq = np.stack(fs, axis=2)
     k = 2
+    ms = 2 * ns
+    # mchf_matrix lists for orbital 1 (every occurrence pattern), dhl, fppqq, iprm
+    rng = np.random.default_rng(45)
+    llist = [(1, 1, 1.25), (1, 1, 0.75), (2, 1, 0.3), (1, 3, -0.2)]
+    pats = [(1, 1, 1, 1), (1, 2, 1, 3), (2, 1, 3, 1), (1, 2, 3, 1), (1, 1, 2, 3), (2, 1, 1, 3), (2, 3, 1, 1),
+            (2, 1, 1, 1), (1, 2, 1, 1), (1, 2, 3, 4), (2, 1, 3, 4), (2, 3, 1, 4), (2, 3, 4, 1)]
+    rlist = [(int(rng.integers(0, 3)), *p, float(rng.uniform(-1, 1))) for p in pats]
+    dhl = O.dhl_pq(g, 0, 26.0, 0.0, 0.0, 0.0, -1)
+    dhl = np.asfortranarray((dhl + dhl.T) / 2)
+    fppqq = O.overlaps(g)[2]
+    iprm = O.boundary_conditions(ns, g.nsp, g.nsq, ksp, ksq, [0] * 4, [-1] * 4, "point")[:, 0]
     with open(tmp_path / "drv_in.bin", "wb") as f:
         np.array([ns, ks, ksp, ksq], dtype=np.int32).tofile(f)
         np.asarray(t, dtype=np.float64).tofile(f)
         np.array([4, k], dtype=np.int32).tofile(f)
         np.asfortranarray(pq).T.tofile(f)  # column-major bytes
+        np.array([len(llist), len(rlist)], dtype=np.int32).tofile(f)
+        np.array([r[:2] for r in llist], dtype=np.int32).tofile(f)      # l_i(2,nl)
+        np.array([r[2] for r in llist]).tofile(f)
+        np.array([r[:5] for r in rlist], dtype=np.int32).tofile(f)      # r_i(5,nr)
+        np.array([r[5] for r in rlist]).tofile(f)
+        dhl.T.tofile(f)
+        np.asfortranarray(fppqq).T.tofile(f)
+        np.asarray(iprm, dtype=np.int32).tofile(f)
     print("[fortran drop-in driver]", flush=True)
     r = subprocess.run([DRIVER], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
@@ -69,8 +88,45 @@ def test_fortran_dropin(tmp_path, gpu_available):
     vals = O.rk_rows(g, fs, rows)
     e_ref = vals @ np.array([1.0, -0.5, 0.25])
     assert abs(energy - e_ref) <= 1e-12 * np.sum(np.abs(vals))
-    grad = out[11 + ms * ms:].reshape((ns, 2, 4), order="F")
+    pos = 11 + ms * ms
+    grad = out[pos:pos + ns * 8].reshape((ns, 2, 4), order="F")
     assert np.all(np.isfinite(grad)) and np.any(grad != 0)
+    pos += ns * 8
+    take = lambda n: out[pos:pos + n]
+    mh = take(ms * ms).reshape((ms, ms), order="F"); pos += ms * ms
+    mr = take(ms); pos += ms
+    mx = take(ms * ms).reshape((ms, ms), order="F"); pos += ms * ms
+    e_i, q_i = out[pos], out[pos + 1]; pos += 2
+    ch = take(ms * ms).reshape((ms, ms), order="F"); pos += ms * ms
+    cx = take(ms * ms).reshape((ms, ms), order="F"); pos += ms * ms
+    ev = take(ms); pos += ms
+    e_v = out[pos]
+    assert pos + 1 == len(out)
+    # mchf_matrix (newton) and core_matrix (newton, a(1,1,1) = 0.75) of orbital 1
+    pqf = np.asfortranarray(pq)
+    h0, r0, x0, e0, s0 = O.mchf_matrix(g, 1, pqf, dhl, llist, rlist, 2, newton=1, fppqq=fppqq)
+    assert q_i == s0 and abs(e_i - e0) <= 1e-12 * abs(e0)
+    for a, b in ((mh, h0), (mr, r0), (mx, x0)):
+        assert np.any(b) and np.max(np.abs(a - b)) <= 1e-13 * np.max(np.abs(b))
+    qs = np.array([2.0, 1.5, 1.0, 1.0])
+    A2 = A.copy(order="F")
+    A2[0, 1] = 0.75
+    hc, xc = O.core_matrix(g, 1, pqf, dhl, A2, B, 2, qs, newton=1)
+    for a, b in ((ch, hc), (cx, xc)):
+        assert np.any(b) and np.max(np.abs(a - b)) <= 1e-13 * np.max(np.abs(b))
+    # hf_eiv of 2 dhl + the potential, orthogonal to orbital 2, m = 1
+    he = O.hf_matrix(g, 1, pqf, A, B, 2, 2.0, hfm=np.asfortranarray(2.0 * dhl))
+    vo = np.concatenate([pq[:, 0, 1], pq[:, 1, 1]])
+    vr, er = O.hf_eiv(he, [vo], iprm, 1, fppqq)
+    import scipy.linalg
+    keep = np.nonzero(iprm)[0]
+    # tolerances of the generalised eigen-solve: 0.1 ulp of the spectral
+    # radius for the eigenvalue, 4 ulps of it over the gap for the vector
+    allv = scipy.linalg.eigh(he[np.ix_(keep, keep)], fppqq[np.ix_(keep, keep)], lower=True, eigvals_only=True)
+    ulp = np.finfo(float).eps * np.max(np.abs(allv))
+    gap = np.min(np.abs(allv[np.abs(allv - er) > 1e-9 * abs(er)] - er))
+    assert abs(e_v - er) <= max(1e-12 * abs(er), 0.1 * ulp)
+    assert np.max(np.abs(ev - vr)) <= max(1e-10, 4 * ulp / gap) * np.max(np.abs(vr))
 
 
 def _write(f, *arrs):
@@ -152,11 +208,12 @@ def test_fortran_ci3_setup_matrix(tmp_path, gpu_available):
 
 @pytest.mark.skipif(not os.path.exists(MAT3), reason="Fortran driver not built (no flang)")
 def test_fortran_mat3_state_res(tmp_path, gpu_available):
-    """a DBSR_MAT3 State_res-style host program (overlap, R and S terms of
-    every structure through Add_integral, then Add_matrix over all types)
-    with its own KBORT and QUADR_PQ, through libdbsr3_amd_mat3.a; every
-    matrix against the oracle's replay (Idef_*, c_data, O_data, R_data for
-    the four atypes with the asymptotic coefficients, S_data)"""
+    """a DBSR_MAT3 State_res-style host program (overlap, L, R and S terms
+    of every structure through Add_integral, then Add_matrix over all types)
+    with its own KBORT and QUADR_PQ, Lval and one-electron operators,
+    through libdbsr3_amd_mat3.a; every matrix against the oracle's replay
+    (Idef_*, c_data, O_data, L_data, R_data for the four atypes with the
+    asymptotic coefficients, S_data)"""
     from test_mat3_routing import IBO, IPB, KAPPA, LBS, NORB, _terms
     from test_oracle_mat3 import _mats
     t, ns, ks, ksp, ksq = small_grid()
@@ -181,14 +238,25 @@ def test_fortran_mat3_state_res(tmp_path, gpu_available):
         if io > 0 and jo == 0 and ic == 0 and (icase == 0 or not any(IPB[o] > 0 for o in orbs)):
             io = (io // IBO) * IBO + int(rng.integers(1, 4))
         terms.append([icase, int(rng.integers(0, 3)), *orbs, ic, jc, io, jo])
+    # L-integrals (icase 1): every structure of Idef_Ltype; L(i j) between
+    # channel orbitals of one kappa
+    from test_oracle_ldata import l_terms
+    for (ii, jj, c, ic, jc, io, jo) in l_terms(rng, 80, valid_only=True, ibo=IBO):
+        if ic == jc == io == jo == 0 and KAPPA[ii] != KAPPA[jj]:
+            jj = ii
+        terms.append([1, 0, ii, 4, jj, 4, ic, jc, io, jo])
     terms = np.array(terms, dtype=np.int32)
     C = rng.uniform(-1, 1, len(terms))
     ntarg = nch * (nch + 1) // 2
+    kap_list = np.array(sorted(set(int(k) for k in KAPPA[1:])), dtype=np.int32)
+    hd = np.asfortranarray(np.stack([O.dhl_pq(g, 0, 4.0, 0.0, 0.0, 0.0, int(k)) for k in kap_list], axis=2))
+    lval = np.asfortranarray(rng.uniform(-1, 1, (NORB, NORB)))
     with open(tmp_path / "drv_mat3_in.bin", "wb") as f:
         _write(f, np.array([ns, ks, ksp, ksq], np.int32), np.asarray(t, np.float64),
                np.array([NORB, NORB, IBO, mk, nch, npert, nvch, 1, ntarg], np.int32), pq, LBS[1:], KAPPA[1:],
                IPB[1:], kb, m["icc"], m["icb"], m["ibb"], iptar[1:], vch, fppqq,
-               np.array([len(terms)], np.int32), terms.T, C)
+               np.array([len(terms)], np.int32), terms.T, C, np.array([len(kap_list)], np.int32), lval, kap_list,
+               hd)
     print("[fortran MAT3 driver]", flush=True)
     r = subprocess.run([MAT3], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
@@ -203,11 +271,14 @@ def test_fortran_mat3_state_res(tmp_path, gpu_available):
     kbort = lambda i1, i2: int(kb[i1 - 1, i2 - 1])
     qpq = np.array([[[g.quadr(fs[i], fs[j], k) for k in range(mk + 1)] for j in range(NORB)] for i in range(NORB)])
     acf, htarg, otarg = np.zeros((nblk, mk + 1), order="F"), np.zeros(ntarg), np.zeros(ntarg)
-    for icase in (0, 2, 3):
+    for icase in (0, 1, 2, 3):
         def gen(it, k, lst):
             if not len(lst[0]):
                 return
-            if icase == 0:
+            if icase == 1:
+                O.l_data(ms, it, lst, 1e-10, IBO, IPB, kbort, vch, fppqq, 1, iptar, htarg, otarg, 1, lval, KAPPA,
+                         kap_list, hd, pq, m)
+            elif icase == 0:
                 assert O.o_data(ms, it, lst[:3], 1e-10, IBO, IPB, kbort, vch, fppqq, 1, iptar, otarg, m) == 0
             elif icase == 2:
                 for atype in range(4):
@@ -220,7 +291,12 @@ def test_fortran_mat3_state_res(tmp_path, gpu_available):
                                  m) == 0
         cd = O.CData(19, 0, mk + 1, 64, 400, 40, 1e-14, gen)
         for row, c in zip(terms, C):
-            if row[0] == icase:
+            if row[0] == icase == 1:
+                lt, key = O.idef_ltype(int(row[2]), int(row[4]), float(c), *(int(x) for x in row[6:10]), IPB,
+                                       1e-10, 1)
+                if lt:
+                    cd.add_coef(float(c), 0, *key, lt)
+            elif row[0] == icase:
                 assert cd.add_integral_mat3(icase, *(int(x) for x in row[1:6]), float(c),
                                             *(int(x) for x in row[6:10]), LBS, KAPPA, IPB, mk=mk, eps_c=1e-10,
                                             check_target=1) == 0

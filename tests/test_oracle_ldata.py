@@ -31,9 +31,10 @@ KAPPA = np.array([0, -1, 1, -1, -1, 1, -1, 1, -1], dtype=np.int32)
 LBS = np.array([0] + [k if k > 0 else -k - 1 for k in KAPPA[1:]], dtype=np.int32)
 
 
-def l_terms(rng, n, valid_only=False):
+def l_terms(rng, n, valid_only=False, ibo=IBO):
     """(ii, jj, C, ic, jc, io, jo) over every structure of Idef_Ltype, some
-    of them invalid (the reference stops on those) unless valid_only"""
+    of them invalid (the reference stops on those) unless valid_only; the
+    overlaps io, jo packed with ibo"""
     bound = lambda: int(rng.integers(4, NORB + 1))
     chan = lambda: int(rng.integers(1, 4))
     out = []
@@ -44,17 +45,17 @@ def l_terms(rng, n, valid_only=False):
         if s == 0:                     # 1.1 L(..) ic,jc
             ic, jc = int(rng.integers(1, 4)), int(rng.integers(1, 4))
         elif s == 1:                   # 1.2 L(..) <i|.> ic
-            ic, io = int(rng.integers(1, 4)), chan() * IBO + bound()
+            ic, io = int(rng.integers(1, 4)), chan() * ibo + bound()
         elif s == 2:                   # 1.3 L(..) <i|.> <.|j>
-            io, jo = chan() * IBO + bound(), chan() * IBO + bound()
+            io, jo = chan() * ibo + bound(), chan() * ibo + bound()
         elif s == 3:                   # 1.4 L(..) <i|j>
-            io = chan() * IBO + chan()
+            io = chan() * ibo + chan()
         elif s == 4:                   # 2.1 L(i.) ic
             ii = chan()
             ic = int(rng.integers(1, 4))
         elif s == 5:                   # 2.2 L(i.) <j|.>
             jj = chan()
-            io = chan() * IBO + bound()
+            io = chan() * ibo + bound()
         elif s == 6:                   # 3.0 L(ij): channel orbitals of one kappa
             ii, jj = int(rng.choice([1, 3])), int(rng.choice([1, 3]))
         elif s == 7:                   # invalid: ic > 0, io = 0, two bound orbitals
