@@ -16,7 +16,7 @@ INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(HERE, "libdbsr3_amd.so")
 OBJDIR = os.path.join(HERE, "_build")
 
-SOURCES = ["dbs_context.hip", "dbs_plain.hip", "dbs_engine.hip", "dbs_accum.hip", "dbs_updates.hip", "dbs_mat3.hip"]
+SOURCES = ["dbs_context.hip", "dbs_plain.hip", "dbs_engine.hip", "dbs_accum.hip", "dbs_updates.hip", "dbs_mat3.hip", "dbs_channels.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

@@ -22,6 +22,9 @@ sz = C.c_size_t
 GEN_MATRIX_FN = C.CFUNCTYPE(None, vp, i32, i32, i32, dp, ip, ip, ip, ip)
 KBORT_FN = C.CFUNCTYPE(i32, vp, i32, i32)
 QUADR_PQ_FN = C.CFUNCTYPE(f64, vp, i32, i32, i32)
+IBORT_FN = C.CFUNCTYPE(i32, vp, i32, i32)
+i64 = C.c_longlong
+lp = C.POINTER(C.c_longlong)
 
 
 class MatBlocksC(C.Structure):
@@ -134,6 +137,11 @@ PROTOTYPES = {
                             i32, i32, vp, vp, vp, vp, dp, dp],
     "dbs_core_matrix_dev": [vp, i32, i32, vp, vp, dp, dp, i32, dp, i32, vp, vp],
     "dbs_hf_eiv_dev": [vp, vp, i32, vp, ip, i32, vp, vp, dp],
+    "dbs_zero_cond": [i32, i32, i32, i32, i32, i32, ip, ip, i32, i32, i32, i32, i32, ip, ip],
+    "dbs_orth_conditions": [i32, ip, i32, ip, ip, IBORT_FN, vp, i32, ip, ip],
+    "dbs_orth_dev": [vp, mbp, i32, ip, ip, i32, vp, vp, i32],
+    "dbs_diag_channels_dev": [vp, mbp, vp, i32, ip, f64, f64, f64, ip],
+    "dbs_record_matrix_dev": [vp, mbp, vp, i32, ip, vp, i64, lp, vp, i64, lp, dp],
     "dbs_engine_create": [C.POINTER(vp), i32, C.POINTER(vp), ip, ip, ip, i32],
     "dbs_engine_destroy": [vp],
     "dbs_engine_buffers": [vp, i32, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp),

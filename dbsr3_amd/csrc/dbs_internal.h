@@ -135,3 +135,10 @@ int hv_column(const dbs_mat_blocks* m, int ich, int ip, int* col);
 int check_blocks(const dbs_mat_blocks* m);
 int target_h(const dbs_mat_blocks* m, int ich, int jch, const int* iptar, double* tgt, double C);
 }  // namespace dbs
+
+// the context's dense-solver handle (rocBLAS/rocSOLVER), created on first
+// use and bound to the context stream (dbs_mat3.hip)
+struct _rocblas_handle;
+namespace dbs {
+int solver_of(dbs_ctx* ctx, _rocblas_handle** h);
+}  // namespace dbs

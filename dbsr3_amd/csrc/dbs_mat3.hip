@@ -1050,6 +1050,8 @@ int core_matrix(dbs_ctx* ctx, int i, int norb, const double* pq, const double* h
 
 void free_solver(void* h) { (void)rocblas_destroy_handle((rocblas_handle)h); }
 
+}  // namespace
+
 int solver_of(dbs_ctx* ctx, rocblas_handle* h) {
   if (!ctx->solver) {
     rocblas_handle x = nullptr;
@@ -1062,6 +1064,8 @@ int solver_of(dbs_ctx* ctx, rocblas_handle* h) {
     return fail(DBS_ERR_HIP, "hf_eiv: solver stream");
   return DBS_OK;
 }
+
+namespace {
 
 // hf_eiv (hf_solve_HF.f90:136-202): orthogonality to each vorth column
 // (H <- (1 - B v v^T) H (1 - v v^T B), orthogonality in hf_orthogonality.f90),

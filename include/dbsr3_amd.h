@@ -442,6 +442,69 @@ int dbs_bav_dev(dbs_ctx *ctx, const double *a_dev, const double *b_dev, char sym
                 double *c_dev);
 
 /* ---------------------------------------------------------------------
+ * Channel-block post-processing of DBSR_MAT3 (dbs_channels.hip): zero
+ * conditions, orthogonality conditions, diagonalisation of the diagonal
+ * blocks and the transformation of the other blocks to the channel
+ * solutions (DBSR_MAT3/sub1.f90:133-170, 258-265).
+ * --------------------------------------------------------------------- */
+/* Zero_cond (DBSR_MAT3/zero_cond.f90:2-56): iprm(nch*2*ns + npert) of 0/1
+ * (0 = B-spline removed) and nhm = the number kept + npert.  lch, kch: l and
+ * kappa of channel ich at element ich-1; ilzero/jlzero < 0 take the
+ * l-dependent rule at r = 0, ibzero/jbzero B-splines are removed at r = R;
+ * the perturber entries are 1.  Host only. */
+int dbs_zero_cond(int nch, int ns, int nsp, int nsq, int ksp, int ksq, const int *lch, const int *kch,
+                  int ilzero, int jlzero, int ibzero, int jbzero, int npert, int *iprm, int *nhm);
+/* IBORT(i,j) of the host program: 0 when orbital i is to be kept orthogonal
+ * to orbital j (the reference's external IBORT) */
+typedef int (*dbs_ibort_fn)(void *user, int i, int j);
+/* the conditions DBS_ORTH imposes (dbs_orth.f90:36-42): for channel ich,
+ * whose orbital is ipch[ich-1], the bound orbitals j = 1..nbf (ascending)
+ * with ipbs[j-1] <= 0, kbs[j-1] = kbs[ipch[ich-1]-1] and IBORT(i,j) = 0, into
+ * iort[(ich-1)*ldo + n], n < nort[ich-1].  Error when a channel has more
+ * than ldo.  Host only. */
+int dbs_orth_conditions(int nch, const int *ipch, int nbf, const int *ipbs, const int *kbs,
+                        dbs_ibort_fn ibort, void *ibort_user, int ldo, int *nort, int *iort);
+/* DBS_ORTH (DBSR_MAT3/dbs_orth.f90:2-84): with P_i, Q_i the columns
+ * pv_dev(:,j), qv_dev(:,j) (ms each: Get_pv / Get_qv of orbital j, nbf
+ * columns) of channel i's conditions iort (dbs_orth_conditions), every
+ * block idiag selects (+1 diagonal blocks, -1 the others, 0 all) becomes
+ * (1 - Q_ich P_ich^T) hch(:,:,icc(ich,jch)) (1 - P_jch Q_jch^T), ich >= jch
+ * (a side without conditions is left as it is), and for idiag = 0 every
+ * perturber column hcp(:,icb(ich,ip)) becomes (1 - Q_ich P_ich^T) hcp.
+ * Same products as the reference's matmul sequence, summed in another
+ * order. */
+int dbs_orth_dev(dbs_ctx *ctx, const dbs_mat_blocks *m, int idiag, const int *nort, const int *iort, int ldo,
+                 const double *pv_dev, const double *qv_dev, int nbf);
+/* Diag_channels (DBSR_MAT3/diag.f90:2-79) after Zero_cond (iprm from
+ * dbs_zero_cond, host): per diagonal block k = icc(ich,ich) the
+ * generalised problem hch(:,:,k) x = E diag(:,:,k) x on the B-splines with
+ * iprm = 1 (lower triangles), the solutions with edmin <= E <= edmax and
+ * |E| >= egap, in ascending E, written to diag(:,ksol,k) (B-normalised,
+ * largest entry positive, zero on removed B-splines) with E in
+ * diag(ksol,ns,k), in the reference's write order (a solution ns overwrites
+ * the eigenvalues stored in its column before it).  ipsol[0..nch]:
+ * ipsol[0] = 0, ipsol[ich] = ksol.  diag_dev(ms,ms,nd) device; error
+ * "channel diagonalization failed" when the overlap is not positive
+ * definite. */
+int dbs_diag_channels_dev(dbs_ctx *ctx, const dbs_mat_blocks *m, double *diag_dev, int nd, const int *iprm,
+                          double edmin, double edmax, double egap, int *ipsol);
+/* Record_matrix (DBSR_MAT3/record_matrix.f90:2-60) on the device, without
+ * the file: D_ich = diag_dev(:,1:ipsol[ich],ich) (channel solutions by
+ * channel, nd >= nch).  For ich > jch with k = icc(ich,jch) != 0 and
+ * max|hch(:,:,k)| >= 1e-20 the record c = D_ich^T hch(:,:,k) D_jch
+ * (ipsol[ich] x ipsol[jch], column-major) goes to c_dev + coff[(ich-1) +
+ * nch*(jch-1)]; for k = icb(ich,ip) != 0 and max|hcp(:,k)| >= 1e-20 the
+ * record w = D_ich^T hcp(:,k) goes to w_dev + woff[(ich-1) + nch*(ip-1)].
+ * Offsets (host, in doubles) run in record order, -1 = record not
+ * written; c_cap / w_cap: capacities of c_dev / w_dev in doubles.
+ * overlaps (host, (nch+npert)^2, column-major) or NULL: set as icase = 0
+ * sets it (max|c|, max|w|, |hp(k)| of the written records).  The
+ * perturber-perturber records are hp itself. */
+int dbs_record_matrix_dev(dbs_ctx *ctx, const dbs_mat_blocks *m, const double *diag_dev, int nd,
+                          const int *ipsol, double *c_dev, long long c_cap, long long *coff, double *w_dev,
+                          long long w_cap, long long *woff, double *overlaps);
+
+/* ---------------------------------------------------------------------
  * Engine: batched evaluation of integral sets and their adjoint.
  * An instance = (context, norb orbitals, rows (k,a,b,c,d)).  Row n is
  *   kind DBS_RK_TOTAL: R^k(a,b;c,d) = rk(f_a,f_b,f_c,f_d,k)

@@ -1293,3 +1293,192 @@ def hf_eiv(hfm, vorth, iprm, m, fppqq):
     if v[int(np.argmax(np.abs(v)))] < 0.0:
         v = -v
     return v, float(ev[mm - 1])
+
+
+# --------------------------------------------------------------------------
+# DBSR_MAT3's channel-block post-processing, restated: Zero_cond
+# (DBSR_MAT3/zero_cond.f90:2-56), the orthogonality conditions of DBS_ORTH
+# (dbs_orth.f90:2-84), Diag_channels (diag.f90:2-79; the eigen-solve is
+# LAPACK's DSYGV(1,'V','L') through scipy, the routine the reference calls)
+# and Record_matrix (record_matrix.f90:2-60).  Matrices hold the reference's
+# layouts: hch(ms,ms,nblocks), diag(ms,ms,nd), hcp(ms,ncb), hp(nbb);
+# icc/icb/ibb 1-based block numbers, 0 = not stored.
+# --------------------------------------------------------------------------
+EDMIN = -2 * C_AU * C_AU   # mod_dbsr_mat.f90:81
+EDMAX = 1.0e10             # :80
+EGAP = 0.001               # :82
+
+
+def zero_cond(nch, ns, nsp, nsq, ksp, ksq, lch, kch, ilzero=-1, jlzero=-1, ibzero=1, jbzero=1, npert=0):
+    """(iprm(mhm) of 0/1, nhm): zero conditions at r = 0 and at the boundary
+    per channel (lch, kch: l and kappa of channel ich at element ich-1)"""
+    ms = 2 * ns
+    iprm = np.ones(nch * ms + npert, dtype=np.int32)
+    ip = jp = 0
+    for ich in range(nch):
+        mi = 0
+        if ilzero >= 0:
+            il = ilzero
+        else:
+            il = lch[ich] + 1
+            if il > ksp - 1:
+                il, mi = 1, 1
+        mj = 0
+        if jlzero >= 0:
+            jl = jlzero
+        else:
+            if kch[ich] < 0:
+                jl = lch[ich] + 2
+            elif kch[ich] >= 1:
+                jl = lch[ich]
+            else:
+                raise ValueError("zero_cond: kappa = 0")
+            if jl > ksq - 1:
+                jl, mj = 1, 1
+        if mi == 1 or mj == 1:
+            il, jl = 1, 1
+        for lo, mb in ((il, nsp + 1 - ibzero), (jl, nsq + 1 - jbzero)):
+            for i in range(1, ns + 1):
+                ip += 1
+                if i <= lo:
+                    iprm[ip - 1] = 0
+                if i >= mb:
+                    iprm[ip - 1] = 0
+                jp += iprm[ip - 1]
+    return iprm, jp + npert
+
+
+def orth_conditions(nch, ipch, ipbs, kbs, ibort):
+    """per channel the 1-based bound orbitals j (ascending) it is kept
+    orthogonal to: ipbs(j) = 0, kbs(j) = kbs(ipch(ich)), IBORT(i,j) = 0
+    (dbs_orth.f90:36-42); ipch/ipbs/kbs indexed by orbital - 1"""
+    out = []
+    for ich in range(nch):
+        i = ipch[ich]
+        out.append([j for j in range(1, len(kbs) + 1)
+                    if ipbs[j - 1] <= 0 and kbs[j - 1] == kbs[i - 1] and ibort(i, j) == 0])
+    return out
+
+
+def dbs_orth(hch, icc, idiag, orth, pv, qv, hcp=None, icb=None, npert=0):
+    """hch (and hcp) updated in place; orth[ich-1] = the orbitals of channel
+    ich (orth_conditions); pv[:, j-1], qv[:, j-1] = Get_pv / Get_qv of
+    orbital j"""
+    nch = icc.shape[0]
+    ms = hch.shape[0]
+    for ich in range(1, nch + 1):
+        if not orth[ich - 1]:
+            continue
+        aa = np.zeros((ms, ms))
+        bb = np.zeros((ms, ms))
+        for j in orth[ich - 1]:
+            S = np.outer(qv[:, j - 1], pv[:, j - 1])
+            aa += S
+            bb += S.T
+        for jch in range(1, nch + 1):
+            if idiag == 1 and ich != jch:
+                continue
+            if idiag == -1 and ich == jch:
+                continue
+            ij = icc[ich - 1, jch - 1]
+            if ij == 0:
+                continue
+            x = hch[:, :, ij - 1].copy()
+            if ich > jch:
+                x = x - aa @ x
+            elif ich == jch:
+                x = x - aa @ x
+                x = x - x @ bb
+            else:
+                x = x - x @ bb
+            hch[:, :, ij - 1] = x
+        if npert > 0 and idiag == 0:
+            for ic in range(1, npert + 1):
+                jc = icb[ich - 1, ic - 1]
+                if jc == 0:
+                    continue
+                vv = hcp[:, jc - 1] @ bb
+                hcp[:, jc - 1] = hcp[:, jc - 1] - vv
+
+
+def diag_channels(hch, diag, icc, iprm, ns, edmin=EDMIN, edmax=EDMAX, egap=EGAP):
+    """diag (ms,ms,nd) updated in place: per diagonal block k = icc(ich,ich)
+    the solutions of H x = E S x on the B-splines with iprm = 1 (S = the
+    block's overlap in diag(:,:,k)), those with edmin <= E <= edmax and
+    |E| >= egap in columns 1..ksol (largest entry positive), E in
+    diag(ksol,ns,k).  Returns ipsol(0:nch) with ipsol[0] = 0."""
+    from scipy.linalg.lapack import dsygv
+    nch = icc.shape[0]
+    ms = 2 * ns
+    ipsol = np.zeros(nch + 1, dtype=np.int32)
+    for ich in range(1, nch + 1):
+        k = icc[ich - 1, ich - 1]
+        if k == 0:
+            continue
+        keep = np.nonzero(np.asarray(iprm[(ich - 1) * ms:ich * ms]))[0]
+        a = np.asfortranarray(hch[np.ix_(keep, keep, [k - 1])][:, :, 0])
+        b = np.asfortranarray(diag[np.ix_(keep, keep, [k - 1])][:, :, 0])
+        w, v, info = dsygv(a, b, itype=1, jobz="V", uplo="L")
+        if info != 0:
+            raise RuntimeError("channel diagonalization failed")
+        ksol = 0
+        for i in range(len(keep)):
+            if w[i] < edmin or w[i] > edmax or abs(w[i]) < egap:
+                continue
+            ksol += 1
+            x = np.zeros(ms)
+            x[keep] = v[:, i]
+            if x[int(np.argmax(np.abs(x)))] < 0.0:
+                x = -x
+            diag[:, ksol - 1, k - 1] = x
+            diag[ksol - 1, ns - 1, k - 1] = w[i]
+        ipsol[ich] = ksol
+    return ipsol
+
+
+def record_matrix(hch, icc, diag, ipsol, icase, hcp=None, icb=None, hp=None, ibb=None, npert=0):
+    """(records, overlaps): the records in the order Record_matrix writes
+    them, ((i, j), values) with values c(1:ni,1:nj), w(1:ni) or hp(k), ending
+    with ((0, 0), None); overlaps (nch+npert, nch+npert), set for icase 0.
+    diag(:,:,ich) holds channel ich's solutions."""
+    nch = icc.shape[0]
+    recs = []
+    ov = np.zeros((nch + npert, nch + npert))
+    for ich in range(2, nch + 1):
+        ni = ipsol[ich]
+        for jch in range(1, ich):
+            nj = ipsol[jch]
+            k = icc[ich - 1, jch - 1]
+            if k == 0:
+                continue
+            h = hch[:, :, k - 1]
+            if np.max(np.abs(h)) < 1e-20:
+                continue
+            w = diag[:, :ni, ich - 1].T @ h
+            c = w @ diag[:, :nj, jch - 1]
+            recs.append(((ich, jch), c))
+            if icase == 0:
+                ov[ich - 1, jch - 1] = np.max(np.abs(c)) if c.size else 0.0
+    if npert > 0:
+        for ich in range(1, nch + 1):
+            ni = ipsol[ich]
+            for ip in range(1, npert + 1):
+                k = icb[ich - 1, ip - 1]
+                if k == 0:
+                    continue
+                if np.max(np.abs(hcp[:, k - 1])) < 1e-20:
+                    continue
+                w = diag[:, :ni, ich - 1].T @ hcp[:, k - 1]
+                if icase == 0:
+                    ov[nch + ip - 1, ich - 1] = np.max(np.abs(w)) if w.size else 0.0
+                recs.append(((nch + ip, ich), w))
+        for ip in range(1, npert + 1):
+            for jp in range(1, ip + 1):
+                k = ibb[ip - 1, jp - 1]
+                if k == 0 or hp[k - 1] == 0.0:
+                    continue
+                recs.append(((ip + nch, jp + nch), hp[k - 1]))
+                if icase == 0:
+                    ov[nch + ip - 1, nch + jp - 1] = abs(hp[k - 1])
+    recs.append(((0, 0), None))
+    return recs, ov
