@@ -746,6 +746,241 @@ def run_config6(args):
                         "hm_norm": float(np.abs(hm).sum())})
 
 
+# edmax: the window's upper end (dbsr_par's Edmax; the Rb example keeps its
+# channel solutions below -1 au with the core energy in the blocks): here the
+# B-spline states above 1e4 au are disregarded, about 100 solutions per channel
+CONFIG7 = dict(ns=150, ks=9, ksp=8, ksq=9, nch=40, z=26.0, nbound=3, kappas=(-1, 1, -2, 2, -3, 3, -4),
+               cpu_channels=12, edmax=1e4)
+
+
+def config7_case(d, seed=20261203 + 7):
+    """Config 2's partial wave after accumulation, the input of MAT3's
+    post-processing: 40 channels (kappa cycling over -1, 1, -2, 2, -3, 3, -4)
+    on the ns=150 grid, the 820 blocks in alloc_dbsr_matrix's numbering
+    (diagonal blocks first); diagonal block = the Dirac-Coulomb operator of
+    the channel's kappa (point nucleus Z=26, MAT_dhl_pq on the device with
+    the Bloch term, so symmetric) plus a seeded symmetric 1e-3 N(0,1) part,
+    other blocks seeded 1e-2 N(0,1); overlaps fppqq; 3 seeded bound orbitals
+    per kappa that vanish on the B-splines Zero_cond removes for that kappa
+    (as bound orbitals do), B-orthonormal, as the orthogonality conditions
+    (IBORT = 0 except for seeded pairs)."""
+    from dbsr3_amd.dbs import device_to_fortran
+    P = CONFIG7
+    ns, nch = P["ns"], P["nch"]
+    ms = 2 * ns
+    rng = np.random.default_rng(seed)
+    kap = [P["kappas"][i % len(P["kappas"])] for i in range(nch)]
+    lch = [k if k > 0 else -k - 1 for k in kap]
+    nuc = d.nucleus("point", P["z"])
+    fppqq = np.asfortranarray(device_to_fortran(d.bspline_overlaps()[2]))
+    dhl = {k: device_to_fortran(d.dhl_pq(nuc, k, bloch=True)) for k in sorted(set(kap))}
+    nb = nch * (nch + 1) // 2
+    hch = np.asfortranarray(rng.standard_normal((ms, ms, nb)) * 1e-2)
+    for i in range(nch):
+        R = hch[:, :, i] * 0.1
+        hch[:, :, i] = dhl[kap[i]] + (R + R.T)
+    kset = sorted(set(kap))
+    nbound = P["nbound"] * len(kset)
+    pv = np.zeros((ms, nbound + nch), order="F")
+    kbs = []
+    from dbsr3_amd.channels import Zero_cond
+    for kp in kset:
+        first = len(kbs)
+        alive = Zero_cond(1, d, [kp if kp > 0 else -kp - 1], [kp])[0]
+        for n in range(P["nbound"]):
+            v = rng.standard_normal(ms) * alive
+            for j in range(first, len(kbs)):
+                v -= (pv[:, j] @ fppqq @ v) * pv[:, j]
+            pv[:, len(kbs)] = v / np.sqrt(v @ fppqq @ v)
+            kbs.append(kp)
+    kbs = kbs + kap
+    ipbs = [0] * nbound + list(range(1, nch + 1))
+    ipch = list(range(nbound + 1, nbound + nch + 1))
+    qv = np.asfortranarray(fppqq @ pv)
+    skip = set((int(a), int(b)) for a, b in zip(rng.integers(nbound + 1, nbound + nch + 1, 8),
+                                                rng.integers(1, nbound + 1, 8)))
+    ibort = lambda i, j: 1 if (i, j) in skip else 0
+    return dict(kap=kap, lch=lch, hch=hch, fppqq=fppqq, pv=pv, qv=qv, kbs=kbs, ipbs=ipbs, ipch=ipch,
+                ibort=ibort)
+
+
+def config7_icc(nch):
+    """alloc_dbsr_matrix's numbering for idiag = 0 (DBSR_MAT3/updates.f90:20-38)"""
+    icc = np.zeros((nch, nch), dtype=np.int32, order="F")
+    n = 0
+    for i in range(nch):
+        n += 1
+        icc[i, i] = n
+    for i in range(1, nch):
+        for j in range(i):
+            n += 1
+            icc[i, j] = icc[j, i] = n
+    return icc
+
+
+def config7_oracle(case, nch_s, ns, nsp, nsq):
+    """the oracle's post-processing of channels 1..nch_s (their blocks are
+    independent of the other channels): (projected hch, solutions, ipsol,
+    records), the reference's sequence DBS_ORTH(+1), Diag_channels,
+    DBS_ORTH(-1), Record_matrix"""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    icc_f = config7_icc(CONFIG7["nch"])
+    icc = config7_icc(nch_s)
+    ms = 2 * ns
+    nb = nch_s * (nch_s + 1) // 2
+    h = np.zeros((ms, ms, nb), order="F")
+    for i in range(nch_s):
+        for j in range(i + 1):
+            h[:, :, icc[i, j] - 1] = case["hch"][:, :, icc_f[i, j] - 1]
+    dg = np.asfortranarray(np.repeat(case["fppqq"][:, :, None], nch_s, axis=2))
+    orth = O.orth_conditions(nch_s, case["ipch"][:nch_s], case["ipbs"], case["kbs"], case["ibort"])
+    O.dbs_orth(h, icc, 1, orth, case["pv"], case["qv"])
+    iprm, _ = O.zero_cond(nch_s, ns, nsp, nsq, CONFIG7["ksp"], CONFIG7["ksq"], case["lch"], case["kap"])
+    ipsol = O.diag_channels(h, dg, icc, iprm, ns, edmax=CONFIG7["edmax"])
+    O.dbs_orth(h, icc, -1, orth, case["pv"], case["qv"])
+    recs, ov = O.record_matrix(h, icc, dg, ipsol, 1)
+    return h, dg, ipsol, recs, icc, iprm
+
+
+def run_config7(args):
+    """MAT3's channel-block post-processing of config 2's partial wave: one
+    step = DBS_ORTH on the diagonal blocks, Diag_channels, DBS_ORTH on the
+    other blocks and Record_matrix with the records left in HBM
+    (DBSR_MAT3/sub1.f90:133-170, 258-265)"""
+    import torch
+    from dbsr3_amd import DBS, MatBlocks
+    from dbsr3_amd.channels import (DBS_ORTH, Diag_channels, Zero_cond, orth_conditions, record_matrix_device,
+                                    record_sizes)
+    from dbsr3_amd.dbs import fortran_to_device
+    from dbsr3_amd.grid import mkgrid_02
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    P = CONFIG7
+    ns, ks, ksp, ksq, nch = P["ns"], P["ks"], P["ksp"], P["ksq"], P["nch"]
+    ms = 2 * ns
+    t = mkgrid_02(ns, ks, P["z"], 0.25, 30.0)
+    d = DBS(t, ns, ks, ksp, ksq)
+    case = config7_case(d)
+    m = MatBlocks(d, nch, 0)
+    assert np.array_equal(m.icc, config7_icc(nch))
+    h0 = torch.from_numpy(np.ascontiguousarray(case["hch"].T)).cuda()
+    d0 = torch.from_numpy(np.ascontiguousarray(np.repeat(case["fppqq"].T[None], nch, axis=0))).cuda()
+    dg = torch.empty_like(d0)
+    iprm, nhm = Zero_cond(nch, d, case["lch"], case["kap"])
+    nort, iort = orth_conditions(case["ipch"], case["ipbs"], case["kbs"], case["ibort"])
+    pv, qv = fortran_to_device(case["pv"]), fortran_to_device(case["qv"])
+    bufs = {}
+
+    def step():
+        m.hch.copy_(h0)
+        dg.copy_(d0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        DBS_ORTH(m, 1, nort, iort, pv, qv)
+        t1 = time.perf_counter()
+        ipsol = Diag_channels(m, dg, iprm, edmax=P["edmax"])
+        t2 = time.perf_counter()
+        DBS_ORTH(m, -1, nort, iort, pv, qv)
+        t3 = time.perf_counter()
+        if "c" not in bufs:
+            nc, nw = record_sizes(m, ipsol)
+            bufs["c"] = torch.zeros(max(nc, 1), dtype=torch.float64, device="cuda")
+            bufs["w"] = torch.zeros(max(nw, 1), dtype=torch.float64, device="cuda")
+        coff, woff, ov = record_matrix_device(m, dg, ipsol, bufs["c"], bufs["w"])
+        t4 = time.perf_counter()
+        return ipsol, coff, (t1 - t0, t2 - t1, t3 - t2, t4 - t3)
+
+    for _ in range(max(args.warmup, 1)):
+        ipsol, coff, _ = step()
+    phases = np.zeros(4)
+    for _ in range(args.steps):
+        ipsol, coff, ph = step()
+        phases += ph
+    phases *= 1e3 / args.steps
+    ms_step = float(phases.sum())
+    nblocks = m.nblocks
+    value = nblocks / (ms_step * 1e-3)
+    # algorithmic work per phase
+    nmo = int(np.max(nort))
+    kk = [int(np.count_nonzero(iprm[i * ms:(i + 1) * ms])) for i in range(nch)]
+    ipl = [int(x) for x in ipsol]
+    written = [(i, j) for i in range(2, nch + 1) for j in range(1, i) if coff[i - 1, j - 1] >= 0]
+    work = {
+        "orth_diag": (nch * 8.0 * nmo * ms * ms, nch * 2.0 * ms * ms * 8),
+        "diag_channels": (sum(7.0 * k ** 3 for k in kk), sum(2.0 * k * k * 8 + ms * ipl[i + 1] * 8
+                                                              for i, k in enumerate(kk))),
+        "orth_offdiag": ((nblocks - nch) * 8.0 * nmo * ms * ms, (nblocks - nch) * 2.0 * ms * ms * 8),
+        "record_matrix": (sum(2.0 * ipl[i] * ms * ms + 2.0 * ipl[i] * ipl[j] * ms for i, j in written),
+                          sum(ms * ms * 8.0 + ipl[i] * ipl[j] * 8.0 for i, j in written)),
+    }
+    names = list(work)
+    phases_ms = {k: round(float(v), 4) for k, v in zip(names, phases)}
+    dom = names[int(np.argmax(phases))]
+    fl, by = work[dom]
+    dt = phases_ms[dom] * 1e-3
+    if fl / by > PEAK_FP64_TFS * 1e12 / (PEAK_HBM_GBS * 1e9):
+        roof = {"bound": "mfma", "achieved": fl / dt / 1e12, "peak": PEAK_FP64_TFS, "unit": "TFLOP/s"}
+    else:
+        roof = {"bound": "hbm", "achieved": by / dt / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s"}
+    roof.update({"frac": roof["achieved"] / roof["peak"], "traffic": None, "phase": dom,
+                 "note": "host-timed phases (each call returns after its stream drains); flops: rank-2n "
+                         "projection 8 n ms^2 per block, eigen-solve 7 kk^3 per channel (dpotrf, dsygst, "
+                         "dsytrd, dstedc, dormtr, dtrsm), transform 2 ni ms^2 + 2 ni nj ms per record; bytes: "
+                         "each block read and written once"})
+    # parity and the CPU baseline: the oracle on channels 1..nch_s, one thread
+    nch_s = P["cpu_channels"]
+    from threadpoolctl import threadpool_limits
+    t0 = time.perf_counter()
+    with threadpool_limits(1):
+        h_o, d_o, ip_o, recs_o, icc_s, iprm_s = config7_oracle(case, nch_s, ns, d.nsp, d.nsq)
+    secs = time.perf_counter() - t0
+    nb_s = nch_s * (nch_s + 1) // 2
+    cpu = {"value": nb_s / secs, "unit": "channel blocks/s", "cores": 1, "kind": "port",
+           "sample": f"the oracle's DBS_ORTH, Diag_channels (LAPACK dsygv through scipy), DBS_ORTH and "
+                     f"Record_matrix on channels 1..{nch_s} ({nb_s} blocks), {secs:.1f} s, one thread"}
+    gh = m.hch.cpu().numpy().T
+    gd = dg.cpu().numpy().T
+    errs = {}
+    if not np.array_equal(ipsol[:nch_s + 1], ip_o):
+        raise SystemExit("config 7: ipsol differs from the oracle")
+    icc_f = m.icc
+    errs["projected_blocks"] = max(
+        float(np.max(np.abs(gh[:, :, icc_f[i, j] - 1] - h_o[:, :, icc_s[i, j] - 1]))
+              / np.max(np.abs(h_o[:, :, icc_s[i, j] - 1])))
+        for i in range(nch_s) for j in range(i))
+    # energies against the channel's largest |E| (the solvers' accuracy scale)
+    errs["energies"] = max(
+        float(np.max(np.abs(gd[:ip_o[c + 1], ns - 1, c] - d_o[:ip_o[c + 1], ns - 1, c]))
+              / np.max(np.abs(d_o[:ip_o[c + 1], ns - 1, c]))) for c in range(nch_s) if ip_o[c + 1] > 0)
+    # the records against the oracle's transform of the GPU's own projected
+    # blocks and solutions (the eigen-solvers' differences do not enter)
+    icc_g = np.zeros((nch_s, nch_s), dtype=np.int32, order="F")
+    for i in range(nch_s):
+        for j in range(nch_s):
+            icc_g[i, j] = icc_f[i, j]
+    recs_g, _ = O.record_matrix(gh, icc_g, gd, ipsol[:nch_s + 1], 1)
+    c = bufs["c"].cpu().numpy()
+    er = 0.0
+    for (key, r) in recs_g[:-1]:
+        i, j = key
+        o = coff[i - 1, j - 1]
+        v = c[o:o + r.size].reshape(r.shape, order="F")
+        er = max(er, float(np.max(np.abs(v - r)) / np.max(np.abs(r))))
+    errs["records"] = er
+    if not (errs["projected_blocks"] <= 1e-12 and errs["energies"] <= 1e-12 and er <= 1e-12):
+        raise SystemExit(f"config 7: results differ from the oracle {errs}")
+    workload = {"workload": "MAT3 channel post-processing of one partial wave: 40 channels, ns=150, ks=9/8/9, "
+                            "820 blocks of 300x300 in HBM; DBS_ORTH (diagonal), Diag_channels (Edmax = 1e4 au), "
+                            "DBS_ORTH (off-diagonal), Record_matrix (records left in HBM); one step = all four",
+                "channels": nch, "blocks": int(nblocks), "ns": ns, "nsol": int(np.sum(ipsol[1:])),
+                "conditions_per_channel": [int(x) for x in nort]}
+    return config_line(7, value, "channel blocks/s", ms_step, args.steps, max(args.warmup, 1), workload,
+                       {"roofline": roof, "phases_ms": phases_ms, "cpu_baseline": cpu,
+                        "batched_vs_cpu": value / cpu["value"], "max_rel_err_sample": errs,
+                        "phase_work": {k: {"flops": v[0], "bytes": v[1]} for k, v in work.items()}})
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -765,7 +1000,7 @@ def main():
                     default=os.path.join(ROOT, "profiles", TRAFFIC_FILE))
     ap.add_argument("--memory-budget-mb", type=float, default=0.0,
                     help="config 5: bytes of features + gradients held at once (MiB; 0 = all groups)")
-    ap.add_argument("--config", type=int, default=4, choices=[1, 2, 3, 4, 5, 6],
+    ap.add_argument("--config", type=int, default=4, choices=[1, 2, 3, 4, 5, 6, 7],
                     help="integral-set configuration (4: the headline workload)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -773,7 +1008,8 @@ def main():
               "but no multi-GPU figure is produced in this build (use --gpus 1)", file=sys.stderr)
         return 2
     if args.config != 4:
-        run = {1: run_config1, 2: run_config2, 3: run_config3, 5: run_config5, 6: run_config6}[args.config]
+        run = {1: run_config1, 2: run_config2, 3: run_config3, 5: run_config5, 6: run_config6,
+               7: run_config7}[args.config]
         print(json.dumps(run(args)), flush=True)
         return 0
 

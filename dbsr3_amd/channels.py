@@ -108,6 +108,27 @@ def record_sizes(m, ipsol):
     return nc, nw
 
 
+def record_matrix_device(m, diag, ipsol, cbuf, wbuf, overlaps=True):
+    """Record_matrix with the records left in the device buffers cbuf / wbuf
+    (float64 tensors of at least record_sizes' capacities): returns the host
+    offset tables coff(nch,nch), woff(nch,npert) (-1 = not written) and the
+    overlaps (nch+npert)^2 (None when overlaps is False)"""
+    ipsol = _i32(ipsol)
+    nch, npert = m.nch, m.npert
+    coff = np.full(nch * nch, -1, dtype=np.int64)
+    woff = np.full(max(nch * npert, 1), -1, dtype=np.int64)
+    nt = nch + npert
+    ov = np.zeros(nt * nt, dtype=np.float64) if overlaps else None
+    m._pre()
+    check(lib().dbs_record_matrix_dev(m.dbs.ctx, C.byref(m._desc), _vp(diag), int(diag.shape[0]),
+                                      ipsol.ctypes.data_as(_ip), _vp(cbuf), int(cbuf.numel()),
+                                      coff.ctypes.data_as(_lp), _vp(wbuf), int(wbuf.numel()),
+                                      woff.ctypes.data_as(_lp), ov.ctypes.data_as(_dp) if overlaps else None))
+    m._post()
+    return (coff.reshape((nch, nch), order="F"), woff[:nch * npert].reshape((nch, npert), order="F"),
+            ov.reshape((nt, nt), order="F") if overlaps else None)
+
+
 def Record_matrix(m, diag, ipsol, icase=0, hp=None):
     """the records of Record_matrix in write order, ((i, j), values) with
     values a numpy array (c(1:ni,1:nj), w(1:ni)) or hp(k), ending with
@@ -121,20 +142,12 @@ def Record_matrix(m, diag, ipsol, icase=0, hp=None):
     nc, nw = record_sizes(m, ipsol)
     cbuf = torch.zeros(max(nc, 1), dtype=torch.float64, device="cuda")
     wbuf = torch.zeros(max(nw, 1), dtype=torch.float64, device="cuda")
-    coff = np.full(nch * nch, -1, dtype=np.int64)
-    woff = np.full(max(nch * npert, 1), -1, dtype=np.int64)
-    nt = nch + npert
-    ov = np.zeros(nt * nt, dtype=np.float64)
-    m._pre()
-    check(lib().dbs_record_matrix_dev(m.dbs.ctx, C.byref(m._desc), _vp(diag), int(diag.shape[0]),
-                                      ipsol.ctypes.data_as(_ip), _vp(cbuf), nc, coff.ctypes.data_as(_lp),
-                                      _vp(wbuf), nw, woff.ctypes.data_as(_lp),
-                                      ov.ctypes.data_as(_dp) if icase == 0 else None))
-    m._post()
+    coff, woff, ov = record_matrix_device(m, diag, ipsol, cbuf, wbuf, overlaps=icase == 0)
+    if ov is None:
+        ov = np.zeros((nch + npert, nch + npert))
     c = cbuf.cpu().numpy()
     w = wbuf.cpu().numpy()
     recs = []
-    coff = coff.reshape((nch, nch), order="F")
     for ich in range(2, nch + 1):
         ni = int(ipsol[ich])
         for jch in range(1, ich):
@@ -144,7 +157,6 @@ def Record_matrix(m, diag, ipsol, icase=0, hp=None):
             nj = int(ipsol[jch])
             recs.append(((ich, jch), c[o:o + ni * nj].reshape((ni, nj), order="F")))
     if npert > 0:
-        woff = woff[:nch * npert].reshape((nch, npert), order="F")
         for ich in range(1, nch + 1):
             ni = int(ipsol[ich])
             for ip in range(1, npert + 1):
@@ -159,4 +171,4 @@ def Record_matrix(m, diag, ipsol, icase=0, hp=None):
                     continue
                 recs.append(((ip + nch, jp + nch), float(hpv[k - 1])))
     recs.append(((0, 0), None))
-    return recs, ov.reshape((nt, nt), order="F")
+    return recs, ov

@@ -142,18 +142,27 @@ __global__ void __launch_bounds__(256) k_orth_vec(int ms, int nmo, const int* __
 }
 
 // compacted pairs of the eigen-problems of one batch: A = hch(keep,keep,blk),
-// B = diag(keep,keep,blk)
-__global__ void k_compact_pair(int ms, int kk, int nb, const int* __restrict__ keep, const int* __restrict__ blk,
-                               const double* __restrict__ hch, const double* __restrict__ dg, double* __restrict__ A,
-                               double* __restrict__ B) {
+// B = diag(keep,keep,blk) in the leading kr x kr part; a channel with fewer
+// kept B-splines than the batch's order kk is padded with (sigma I, I), a
+// block that decouples exactly (its zeros stay zeros through the Cholesky
+// factor, the reduction and the Householder steps) and whose eigenvalue
+// sigma lies outside the energy window
+__global__ void k_compact_pair(int ms, int kk, int nb, const int* __restrict__ keep, const int* __restrict__ kr,
+                               const int* __restrict__ blk, double sigma, const double* __restrict__ hch,
+                               const double* __restrict__ dg, double* __restrict__ A, double* __restrict__ B) {
   const long long n = (long long)nb * kk * kk;
   for (long long x = blockIdx.x * (long long)blockDim.x + threadIdx.x; x < n; x += (long long)gridDim.x * blockDim.x) {
     const int i = (int)(x % kk);
     const long long r = x / kk;
     const int j = (int)(r % kk), e = (int)(r / kk);
-    const size_t src = (size_t)ms * ms * blk[e] + keep[(size_t)e * kk + i] + (size_t)ms * keep[(size_t)e * kk + j];
-    A[x] = hch[src];
-    B[x] = dg[src];
+    if (i < kr[e] && j < kr[e]) {
+      const size_t src = (size_t)ms * ms * blk[e] + keep[(size_t)e * kk + i] + (size_t)ms * keep[(size_t)e * kk + j];
+      A[x] = hch[src];
+      B[x] = dg[src];
+    } else {
+      A[x] = i == j ? sigma : 0.0;
+      B[x] = i == j ? 1.0 : 0.0;
+    }
   }
 }
 
@@ -162,15 +171,16 @@ __global__ void k_compact_pair(int ms, int kk, int nb, const int* __restrict__ k
 // it in diag(:,s,blk) (diag.f90:50-56)
 __global__ void __launch_bounds__(256) k_expand_solutions(int ms, int kk, const int* __restrict__ sol_e,
                                                           const int* __restrict__ sol_col, const int* __restrict__ sol_s,
-                                                          const int* __restrict__ blk, const int* __restrict__ inv,
-                                                          const double* __restrict__ Z, double* __restrict__ dg) {
+                                                          const int* __restrict__ blk, const int* __restrict__ kr,
+                                                          const int* __restrict__ inv, const double* __restrict__ Z,
+                                                          double* __restrict__ dg) {
   __shared__ double bv[256];
   __shared__ int bi[256];
   const int e = sol_e[blockIdx.x];
   const double* z = Z + (size_t)kk * kk * e + (size_t)kk * sol_col[blockIdx.x];
   double best = -1.0;
   int idx = 0x7fffffff;
-  for (int i = threadIdx.x; i < kk; i += blockDim.x) {
+  for (int i = threadIdx.x; i < kr[e]; i += blockDim.x) {
     const double a = fabs(z[i]);
     if (a > best) {
       best = a;
@@ -445,8 +455,10 @@ int diag_channels(dbs_ctx* ctx, const dbs_mat_blocks* m, double* dg, int nd, con
   const int ms = ctx->ms, ns = ctx->ns, nch = m->nch;
   hipStream_t s = ctx->stream;
   ipsol[0] = 0;
-  // channels by their number of kept B-splines: one strided batch each
-  std::map<int, std::vector<int>> groups;
+  // the channels' kept sizes; one strided batch of the largest order with
+  // the smaller problems padded by a decoupled (sigma I, I) block when a
+  // sigma outside the window of modest size exists, else one batch per size
+  std::vector<int> chans_all, kr_all;
   for (int ich = 1; ich <= nch; ++ich) {
     ipsol[ich] = 0;
     const int k = m->icc[(ich - 1) + (size_t)nch * (ich - 1)];
@@ -455,21 +467,41 @@ int diag_channels(dbs_ctx* ctx, const dbs_mat_blocks* m, double* dg, int nd, con
     int kk = 0;
     for (int i = 0; i < ms; ++i) kk += iprm[(size_t)(ich - 1) * ms + i] != 0;
     DBS_CHECK(kk > 0, DBS_ERR_ARG, "Diag_channels: every B-spline of a channel removed");
-    groups[kk].push_back(ich);
+    chans_all.push_back(ich);
+    kr_all.push_back(kk);
   }
+  double sigma = 0.0;
+  bool pad = false;
+  {
+    const double lim = 1e12;
+    double best = HUGE_VAL;
+    if (std::fabs(edmin) < lim) {
+      const double c = edmin - std::max(1.0, std::fabs(edmin));
+      if (std::fabs(c) < best) best = std::fabs(c), sigma = c, pad = true;
+    }
+    if (std::fabs(edmax) < lim) {
+      const double c = edmax + std::max(1.0, std::fabs(edmax));
+      if (std::fabs(c) < best) best = std::fabs(c), sigma = c, pad = true;
+    }
+  }
+  std::map<int, std::vector<int>> groups;  // order -> positions in chans_all
+  const int kmax = kr_all.empty() ? 0 : *std::max_element(kr_all.begin(), kr_all.end());
+  for (size_t x = 0; x < chans_all.size(); ++x) groups[pad ? kmax : kr_all[x]].push_back((int)x);
   rocblas_handle h;
   int rc;
   if (!groups.empty() && (rc = solver_of(ctx, &h))) return rc;
   for (auto& g : groups) {
     const int kk = g.first;
-    const std::vector<int>& chans = g.second;
+    const std::vector<int>& pos = g.second;
     const size_t per = (size_t)kk * kk * 2 + (size_t)kk * 2 + ms;
-    const int chunk = (int)std::max<size_t>(1, std::min<size_t>(chans.size(), kWorkBytes / (per * 8)));
-    for (size_t c0 = 0; c0 < chans.size(); c0 += chunk) {
-      const int nb = (int)std::min<size_t>(chunk, chans.size() - c0);
-      std::vector<int> keep((size_t)nb * kk), inv((size_t)nb * ms, -1), blk(nb);
+    const int chunk = (int)std::max<size_t>(1, std::min<size_t>(pos.size(), kWorkBytes / (per * 8)));
+    for (size_t c0 = 0; c0 < pos.size(); c0 += chunk) {
+      const int nb = (int)std::min<size_t>(chunk, pos.size() - c0);
+      std::vector<int> keep((size_t)nb * kk, -1), inv((size_t)nb * ms, -1), blk(nb), kr(nb), chans(nb);
       for (int e = 0; e < nb; ++e) {
-        const int ich = chans[c0 + e];
+        const int ich = chans_all[pos[c0 + e]];
+        chans[e] = ich;
+        kr[e] = kr_all[pos[c0 + e]];
         blk[e] = m->icc[(ich - 1) + (size_t)nch * (ich - 1)] - 1;
         int n = 0;
         for (int i = 0; i < ms; ++i)
@@ -479,16 +511,17 @@ int diag_channels(dbs_ctx* ctx, const dbs_mat_blocks* m, double* dg, int nd, con
           }
       }
       Mem mem(s);
-      int *d_keep, *d_inv, *d_blk;
+      int *d_keep, *d_inv, *d_blk, *d_kr;
       double *A, *B, *D, *E;
       rocblas_int* info;
       if ((rc = mem.put(keep, &d_keep)) || (rc = mem.put(inv, &d_inv)) || (rc = mem.put(blk, &d_blk)) ||
+          (rc = mem.put(kr, &d_kr)) ||
           (rc = mem.alloc(&A, (size_t)nb * kk * kk)) || (rc = mem.alloc(&B, (size_t)nb * kk * kk)) ||
           (rc = mem.alloc(&D, (size_t)nb * kk)) || (rc = mem.alloc(&E, (size_t)nb * kk)) ||
           (rc = mem.alloc(&info, nb)))
         return rc;
-      k_compact_pair<<<grid_n((long long)nb * kk * kk, 256), 256, 0, s>>>(ms, kk, nb, d_keep, d_blk, m->hch_dev, dg, A,
-                                                                          B);
+      k_compact_pair<<<grid_n((long long)nb * kk * kk, 256), 256, 0, s>>>(ms, kk, nb, d_keep, d_kr, d_blk, sigma,
+                                                                          m->hch_dev, dg, A, B);
       if ((rc = hip_ok("Diag_channels"))) return rc;
       if ((rc = blas(rocsolver_dsygvd_strided_batched(h, rocblas_eform_ax, rocblas_evect_original, rocblas_fill_lower,
                                                       kk, A, kk, (rocblas_stride)kk * kk, B, kk,
@@ -514,7 +547,7 @@ int diag_channels(dbs_ctx* ctx, const dbs_mat_blocks* m, double* dg, int nd, con
           ss.push_back(ksol[e]++);
           se_E.push_back(w);
         }
-      for (int e = 0; e < nb; ++e) ipsol[chans[c0 + e]] = ksol[e];
+      for (int e = 0; e < nb; ++e) ipsol[chans[e]] = ksol[e];
       if (se.empty()) continue;
       DBS_CHECK(*std::max_element(ksol.begin(), ksol.end()) <= ms, DBS_ERR_ARG, "Diag_channels: ksol > ms");
       int *d_se, *d_scol, *d_ss, *d_ksol;
@@ -522,7 +555,7 @@ int diag_channels(dbs_ctx* ctx, const dbs_mat_blocks* m, double* dg, int nd, con
       if ((rc = mem.put(se, &d_se)) || (rc = mem.put(scol, &d_scol)) || (rc = mem.put(ss, &d_ss)) ||
           (rc = mem.put(ksol, &d_ksol)) || (rc = mem.put(se_E, &d_E)))
         return rc;
-      k_expand_solutions<<<(unsigned)se.size(), 256, 0, s>>>(ms, kk, d_se, d_scol, d_ss, d_blk, d_inv, A, dg);
+      k_expand_solutions<<<(unsigned)se.size(), 256, 0, s>>>(ms, kk, d_se, d_scol, d_ss, d_blk, d_kr, d_inv, A, dg);
       k_store_energies<<<grid_n((long long)se.size(), 256), 256, 0, s>>>(ms, ns, (int)se.size(), d_se, d_ss, d_ksol,
                                                                          d_blk, d_E, dg);
       if ((rc = hip_ok("Diag_channels"))) return rc;

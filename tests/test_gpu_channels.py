@@ -99,8 +99,11 @@ def _diag_compare(g, hch, ovl, d_gpu, d_ref, ipsol, iprm, icc):
         k = icc[ich - 1, ich - 1] - 1
         ksol = ipsol[ich]
         keep = np.nonzero(iprm[(ich - 1) * ms:ich * ms])[0]
-        A = hch[np.ix_(keep, keep, [k])][:, :, 0]
-        B = ovl[np.ix_(keep, keep, [k])][:, :, 0]
+        # the problem both solvers see: the lower triangles (uplo = 'L')
+        A = np.tril(hch[np.ix_(keep, keep, [k])][:, :, 0])
+        A = A + np.tril(A, -1).T
+        B = np.tril(ovl[np.ix_(keep, keep, [k])][:, :, 0])
+        B = B + np.tril(B, -1).T
         w_all = dsygv(np.asfortranarray(A), np.asfortranarray(B), 1, "N", "L")[0]
         emax = np.max(np.abs(w_all))
         nA, nB = np.max(np.sum(np.abs(A), 1)), np.max(np.sum(np.abs(B), 1))
@@ -234,3 +237,67 @@ def test_record_matrix_matches_oracle(gpu_available):
         assert v.shape == r.shape and np.max(np.abs(v - r)) <= 1e-12 * np.max(np.abs(r)), key
     assert np.max(np.abs(ov - ovr)) <= 1e-12 * np.max(np.abs(ovr))
     assert np.array_equal(ov == 0, ovr == 0)
+
+
+def test_channel_post_processing_config2_size(gpu_available):
+    """config 2's partial wave (40 channels, ns=150, 820 blocks of 300x300;
+    bench.config7_case) through the four steps, each stage against the
+    oracle run on the same input: DBS_ORTH on the diagonal blocks, then
+    Diag_channels on the GPU's projected blocks, DBS_ORTH on the other
+    blocks, Record_matrix on the GPU's projected blocks and solutions"""
+    import torch
+    from bench import CONFIG7, config7_case
+    from dbsr3_amd import DBS, MatBlocks
+    from dbsr3_amd.channels import DBS_ORTH, Diag_channels, Record_matrix, Zero_cond, orth_conditions
+    from dbsr3_amd.dbs import fortran_to_device
+    from dbsr3_amd.grid import mkgrid_02
+    P = CONFIG7
+    ns, ks, ksp, ksq, nch = P["ns"], P["ks"], P["ksp"], P["ksq"], P["nch"]
+    t = mkgrid_02(ns, ks, P["z"], 0.25, 30.0)
+    d = DBS(t, ns, ks, ksp, ksq)
+    g = O.Grid(t, ns, ks, ksp, ksq)
+    case = config7_case(d)
+    print(f"[channels nch={nch} ns={ns}]", flush=True)
+    m = _blocks(d, nch, 0, 0, case["hch"])
+    ovl = np.asfortranarray(np.repeat(case["fppqq"][:, :, None], nch, axis=2))
+    dg = torch.from_numpy(np.ascontiguousarray(ovl.T)).cuda()
+    iprm, nhm = Zero_cond(nch, d, case["lch"], case["kap"])
+    iprm_o, nhm_o = O.zero_cond(nch, ns, g.nsp, g.nsq, ksp, ksq, case["lch"], case["kap"])
+    assert np.array_equal(iprm, iprm_o) and nhm == nhm_o
+    nort, iort = orth_conditions(case["ipch"], case["ipbs"], case["kbs"], case["ibort"])
+    orth = O.orth_conditions(nch, case["ipch"], case["ipbs"], case["kbs"], case["ibort"])
+    assert all(list(iort[i, :nort[i]]) == orth[i] for i in range(nch)) and max(nort) > 0
+    pv, qv = fortran_to_device(case["pv"]), fortran_to_device(case["qv"])
+    icc = m.icc
+    # DBS_ORTH, diagonal blocks
+    DBS_ORTH(m, 1, nort, iort, pv, qv)
+    h1 = _f(m.hch)
+    ref = case["hch"].copy(order="F")
+    O.dbs_orth(ref, icc, 1, orth, case["pv"], case["qv"])
+    err = max(float(np.max(np.abs(h1[:, :, b] - ref[:, :, b])) / np.max(np.abs(ref[:, :, b])))
+              for b in range(nch))
+    print(f"  orth(+1) max rel {err:.2e}", flush=True)
+    assert err <= 1e-12 and np.array_equal(h1[:, :, nch:], case["hch"][:, :, nch:])
+    # Diag_channels on the projected diagonal blocks
+    ipsol = Diag_channels(m, dg, iprm, edmax=P["edmax"])
+    d_ref = ovl.copy(order="F")
+    ip_ref = O.diag_channels(h1, d_ref, icc, iprm, ns, edmax=P["edmax"])
+    assert np.array_equal(ipsol, ip_ref) and np.all(ipsol[1:] < ns)
+    sol = _f(dg)
+    _diag_compare(g, h1, ovl, sol, d_ref, ipsol, iprm, icc)
+    # DBS_ORTH, the other blocks
+    DBS_ORTH(m, -1, nort, iort, pv, qv)
+    h2 = _f(m.hch)
+    O.dbs_orth(ref, icc, -1, orth, case["pv"], case["qv"])
+    err = max(float(np.max(np.abs(h2[:, :, b] - ref[:, :, b])) / np.max(np.abs(ref[:, :, b])))
+              for b in range(nch, m.nblocks))
+    print(f"  orth(-1) max rel {err:.2e}", flush=True)
+    assert err <= 1e-12 and np.array_equal(h2[:, :, :nch], h1[:, :, :nch])
+    # Record_matrix on the GPU's projected blocks and solutions
+    recs, ov = Record_matrix(m, dg, ipsol, icase=0)
+    rrefs, ovr = O.record_matrix(h2, icc, sol, ipsol, 0)
+    assert [r[0] for r in recs] == [r[0] for r in rrefs] and len(recs) == m.nblocks - nch + 1
+    err = max(float(np.max(np.abs(v - r)) / np.max(np.abs(r))) for (_, v), (_, r) in zip(recs[:-1], rrefs[:-1]))
+    print(f"  records max rel {err:.2e}", flush=True)
+    assert err <= 1e-12
+    assert np.max(np.abs(ov - ovr)) <= 1e-12 * np.max(np.abs(ovr))
