@@ -11,8 +11,9 @@
 // is projected by one rank-2*nmo update, X <- X - [Q_L | V - Q_L S] [U ; Q_R^T]
 // with U = P_L^T X, V = X P_R, S = P_L^T X P_R -- four passes over the block,
 // HBM-bound -- as batched rocBLAS products over all selected blocks.  The
-// eigen-solves of channels with the same number of kept B-splines run as one
-// strided batch of rocSOLVER dsygvd; the transformation D_i^T H D_j is two
+// eigen-solves of all channels run as one strided batch (smaller problems
+// padded by a decoupled block): Cholesky, reduction by two triangular solves,
+// rocSOLVER dsyevd, back-substitution; the transformation D_i^T H D_j is two
 // batched products per block.  Selection of the eigenvalue window and the
 // record offsets are host work on the eigenvalues and per-block maxima only.
 #include <rocblas/rocblas.h>
@@ -523,18 +524,38 @@ int diag_channels(dbs_ctx* ctx, const dbs_mat_blocks* m, double* dg, int nd, con
       k_compact_pair<<<grid_n((long long)nb * kk * kk, 256), 256, 0, s>>>(ms, kk, nb, d_keep, d_kr, d_blk, sigma,
                                                                           m->hch_dev, dg, A, B);
       if ((rc = hip_ok("Diag_channels"))) return rc;
-      if ((rc = blas(rocsolver_dsygvd_strided_batched(h, rocblas_eform_ax, rocblas_evect_original, rocblas_fill_lower,
-                                                      kk, A, kk, (rocblas_stride)kk * kk, B, kk,
-                                                      (rocblas_stride)kk * kk, D, kk, E, kk, info, nb),
-                     "Diag_channels: rocsolver_dsygvd")))
-        return rc;
-      std::vector<double> ev((size_t)nb * kk);
-      std::vector<rocblas_int> hinfo(nb);
-      if ((rc = to_host(hinfo.data(), info, sizeof(rocblas_int) * nb, s, "Diag_channels")) ||
-          (rc = to_host(ev.data(), D, sizeof(double) * ev.size(), s, "Diag_channels")))
+      // DSYGV's steps (itype 1): B = L L^T, C = L^-1 A L^-T, C z = E z,
+      // x = L^-T z; the reduction as two batched triangular solves (level 3)
+      // rather than rocSOLVER's column-by-column sygst
+      const rocblas_stride st2 = (rocblas_stride)kk * kk;
+      const double one = 1.0;
+      std::vector<rocblas_int> hinfo(nb), hinfo2(nb);
+      if ((rc = blas(rocsolver_dpotrf_strided_batched(h, rocblas_fill_lower, kk, B, kk, st2, info, nb),
+                     "Diag_channels: rocsolver_dpotrf")) ||
+          (rc = to_host(hinfo.data(), info, sizeof(rocblas_int) * nb, s, "Diag_channels")))
         return rc;
       for (int e = 0; e < nb; ++e)
         DBS_CHECK(hinfo[e] == 0, DBS_ERR_ARG, "channel diagonalization failed");
+      if ((rc = blas(rocblas_dtrsm_strided_batched(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
+                                                   rocblas_diagonal_non_unit, kk, kk, &one, B, kk, st2, A, kk, st2, nb),
+                     "Diag_channels: L^-1 A")) ||
+          (rc = blas(rocblas_dtrsm_strided_batched(h, rocblas_side_right, rocblas_fill_lower,
+                                                   rocblas_operation_transpose, rocblas_diagonal_non_unit, kk, kk, &one,
+                                                   B, kk, st2, A, kk, st2, nb),
+                     "Diag_channels: A L^-T")) ||
+          (rc = blas(rocsolver_dsyevd_strided_batched(h, rocblas_evect_original, rocblas_fill_lower, kk, A, kk, st2, D,
+                                                      kk, E, kk, info, nb),
+                     "Diag_channels: rocsolver_dsyevd")) ||
+          (rc = blas(rocblas_dtrsm_strided_batched(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_transpose,
+                                                   rocblas_diagonal_non_unit, kk, kk, &one, B, kk, st2, A, kk, st2, nb),
+                     "Diag_channels: L^-T z")))
+        return rc;
+      std::vector<double> ev((size_t)nb * kk);
+      if ((rc = to_host(hinfo2.data(), info, sizeof(rocblas_int) * nb, s, "Diag_channels")) ||
+          (rc = to_host(ev.data(), D, sizeof(double) * ev.size(), s, "Diag_channels")))
+        return rc;
+      for (int e = 0; e < nb; ++e)
+        DBS_CHECK(hinfo2[e] == 0, DBS_ERR_ARG, "channel diagonalization failed");
       // the window (diag.f90:44-48), solutions in ascending order
       std::vector<int> se, scol, ss, ksol(nb, 0);
       std::vector<double> se_E;
