@@ -84,6 +84,7 @@ PROTOTYPES = {
     "dbs_ctx_sync": [vp],
     "dbs_ctx_tables": [vp, dp, dp, dp, dp],
     "dbs_ctx_set_cache_limit": [vp, sz],
+    "dbs_ctx_set_workspace_limit": [vp, sz],
     "dbs_malloc": [vp, C.POINTER(vp), sz],
     "dbs_free": [vp, vp],
     "dbs_memcpy_h2d": [vp, vp, vp, sz],

@@ -30,8 +30,6 @@
 namespace dbs {
 namespace {
 
-constexpr size_t kWorkBytes = (size_t)256 << 20;  // workspace per batch of blocks
-
 unsigned grid_n(long long n, int block) {
   long long g = (n + block - 1) / block;
   return (unsigned)std::max<long long>(1, std::min<long long>(g, 1 << 16));
@@ -376,7 +374,7 @@ int orth(dbs_ctx* ctx, const dbs_mat_blocks* m, int idiag, const int* nort, cons
   rocblas_handle h;
   if ((rc0 = solver_of(ctx, &h))) return rc0;
   const size_t per = (size_t)ms * 2 * nmo * 2 + (size_t)nmo * nmo;
-  const int chunk = (int)std::max<size_t>(1, std::min<size_t>(blk.size(), kWorkBytes / (per * 8)));
+  const int chunk = (int)std::max<size_t>(1, std::min<size_t>(blk.size(), ctx->work_limit / (per * 8)));
   double *Lw, *Rw, *S;
   int *d_l, *d_r;
   const double **pp;
@@ -495,7 +493,7 @@ int diag_channels(dbs_ctx* ctx, const dbs_mat_blocks* m, double* dg, int nd, con
     const int kk = g.first;
     const std::vector<int>& pos = g.second;
     const size_t per = (size_t)kk * kk * 2 + (size_t)kk * 2 + ms;
-    const int chunk = (int)std::max<size_t>(1, std::min<size_t>(pos.size(), kWorkBytes / (per * 8)));
+    const int chunk = (int)std::max<size_t>(1, std::min<size_t>(pos.size(), ctx->work_limit / (per * 8)));
     for (size_t c0 = 0; c0 < pos.size(); c0 += chunk) {
       const int nb = (int)std::min<size_t>(chunk, pos.size() - c0);
       std::vector<int> keep((size_t)nb * kk, -1), inv((size_t)nb * ms, -1), blk(nb), kr(nb), chans(nb);
@@ -645,7 +643,7 @@ int record_matrix(dbs_ctx* ctx, const dbs_mat_blocks* m, const double* dg, const
       rocblas_handle h;
       if ((rc = solver_of(ctx, &h))) return rc;
       const size_t per = (size_t)nmax * ms + (size_t)nmax * nmax;
-      const int chunk = (int)std::max<size_t>(1, std::min<size_t>(wr.size(), kWorkBytes / (per * 8)));
+      const int chunk = (int)std::max<size_t>(1, std::min<size_t>(wr.size(), ctx->work_limit / (per * 8)));
       double *W, *Cp, *d_cm;
       const double** pp;
       int *d_ni, *d_nj;

@@ -281,6 +281,13 @@ int dbs_ctx_set_cache_limit(dbs_ctx* c, size_t bytes) {
   return DBS_OK;
 }
 
+int dbs_ctx_set_workspace_limit(dbs_ctx* c, size_t bytes) {
+  DBS_CHECK(c, DBS_ERR_ARG, "dbs_ctx_set_workspace_limit: null context");
+  DBS_CHECK(bytes >= ((size_t)64 << 10), DBS_ERR_ARG, "dbs_ctx_set_workspace_limit: below 64 KiB");
+  c->work_limit = bytes;
+  return DBS_OK;
+}
+
 int dbs_malloc(dbs_ctx* c, void** p, size_t bytes) {
   DBS_CHECK(c && p, DBS_ERR_ARG, "dbs_malloc: null argument");
   DBS_HIP(hipSetDevice(c->device));

@@ -88,6 +88,8 @@ struct dbs_ctx {
   std::vector<int> rkb_lru;
   size_t rkb_cache_limit = (size_t)8 << 30;  // bytes
   size_t rkb_cache_bytes = 0;
+  // device workspace per batch of blocks of the channel post-processing
+  size_t work_limit = (size_t)256 << 20;
   // scratch for the array build
   double* mom = nullptr;  // 4 x (ks*ks, nv)
   double* rkd = nullptr;  // (Tmax, Tmax, nv)

@@ -73,6 +73,11 @@ class DBS:
         except Exception:
             pass
 
+    def set_workspace_limit(self, nbytes):
+        """device workspace per batch of blocks of the channel
+        post-processing (default 256 MiB)"""
+        check(lib().dbs_ctx_set_workspace_limit(self.ctx, int(nbytes)))
+
     # -- plumbing --------------------------------------------------------
     def _pre(self):
         _torch().cuda.synchronize()

@@ -63,6 +63,10 @@ int dbs_ctx_sync(dbs_ctx *ctx);
 int dbs_ctx_tables(dbs_ctx *ctx, double *gr, double *grw, double *pbsp, double *qbsp);
 /* bytes of materialised arrays the context may cache (default 8 GiB) */
 int dbs_ctx_set_cache_limit(dbs_ctx *ctx, size_t bytes);
+/* bytes of device workspace the channel post-processing (DBS_ORTH,
+ * Diag_channels, Record_matrix) takes per batch of blocks (default 256 MiB);
+ * larger partial waves run in more batches with the same results; at least 64 KiB */
+int dbs_ctx_set_workspace_limit(dbs_ctx *ctx, size_t bytes);
 
 /* device memory helpers for callers without HIP headers (Fortran shim) */
 int dbs_malloc(dbs_ctx *ctx, void **dptr, size_t bytes);

@@ -301,3 +301,49 @@ def test_channel_post_processing_config2_size(gpu_available):
     print(f"  records max rel {err:.2e}", flush=True)
     assert err <= 1e-12
     assert np.max(np.abs(ov - ovr)) <= 1e-12 * np.max(np.abs(ovr))
+
+
+def test_post_processing_in_small_batches(gpu_available):
+    """a 64 KiB workspace limit splits DBS_ORTH (36 blocks, 18 per batch),
+    the eigen-solves (one channel per batch) and Record_matrix (one record
+    per batch) into batches, the pointer and host tables rewritten between
+    them: the same results as the oracle"""
+    import torch
+    from dbsr3_amd import DBS, MatBlocks
+    from dbsr3_amd.channels import DBS_ORTH, Diag_channels, Record_matrix, orth_conditions
+    from dbsr3_amd.dbs import fortran_to_device
+    # projections: 36 blocks of 36x36 with 3 conditions, 3.5 KB of workspace each
+    c = projector_case(9, nch=8, ns=18, npert=2)
+    ns = c["pv"].shape[0] // 2
+    d = DBS(O.mkgrid_02(ns, 5, 0.25, 20.0, 4.0), ns, 5, 4, 5)
+    d.set_workspace_limit(64 << 10)
+    m = _blocks(d, c["nch"], c["npert"], 0, c["hch"], c["hcp"])
+    nort, iort = orth_conditions(c["ipch"], c["ipbs"], c["kbs"], c["ibort"])
+    DBS_ORTH(m, 0, nort, iort, fortran_to_device(c["pv"]), fortran_to_device(c["qv"]))
+    h, hp = c["hch"].copy(order="F"), c["hcp"].copy(order="F")
+    O.dbs_orth(h, c["icc"], 0, c["orth"], c["pv"], c["qv"], hp, c["icb"], c["npert"])
+    gh = _f(m.hch)
+    for b in range(h.shape[2]):
+        assert np.max(np.abs(gh[:, :, b] - h[:, :, b])) <= 1e-12 * np.max(np.abs(h[:, :, b])), b
+    # eigen-solves (0.8 MB per channel) and records (0.3 MB each)
+    kappas = [-1, 1, -2, 2, -1]
+    g, fppqq, hch_d, diag, icc_d, iprm, lch = _perturbed_dirac(kappas, 26.0, None, 3)
+    t, ns, ks, ksp, ksq = fe_grid()
+    d2 = DBS(t, ns, ks, ksp, ksq)
+    d2.set_workspace_limit(64 << 10)
+    md = _blocks(d2, len(kappas), 0, 1, hch_d)
+    dg = torch.from_numpy(np.ascontiguousarray(diag.T)).cuda()
+    ipsol = Diag_channels(md, dg, iprm)
+    d_ref = diag.copy(order="F")
+    assert np.array_equal(ipsol, O.diag_channels(hch_d, d_ref, icc_d, iprm, ns))
+    sol = _f(dg)
+    _diag_compare(g, hch_d, diag, sol, d_ref, ipsol, iprm, icc_d)
+    mo = MatBlocks(d2, len(kappas), 0, idiag=-1)
+    rng = np.random.default_rng(8)
+    hch = np.asfortranarray(rng.standard_normal((g.ms, g.ms, mo.nblocks)))
+    m2 = _blocks(d2, len(kappas), 0, -1, hch)
+    recs, ov = Record_matrix(m2, dg, ipsol, icase=0)
+    rrefs, ovr = O.record_matrix(hch, m2.icc, sol, ipsol, 0)
+    assert [r[0] for r in recs] == [r[0] for r in rrefs]
+    for (key, v), (_, r) in zip(recs[:-1], rrefs[:-1]):
+        assert np.max(np.abs(v - r)) <= 1e-12 * np.max(np.abs(r)), key
