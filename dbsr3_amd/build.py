@@ -63,9 +63,9 @@ TDIR = os.path.join(ROOT, "tests", "fortran")
 # drop-in layers with the reference's names for one host program each
 # (DBSR_CI3 and DBSR_MAT3 define Add_integral / Add_matrix / Gen_matrix
 # differently): static archives linked into the host program
-ARCHIVES = {"ci3": "dbs_gpu_ci3.f90", "mat3": "dbs_gpu_mat3.f90"}
+ARCHIVES = {"ci3": ["dbs_gpu_ci3.f90"], "mat3": ["dbs_gpu_mat3.f90", "dbs_gpu_mat3_post.f90"]}
 # test host programs: source, archive it links (or None)
-DRIVERS = {"drv_dropin": None, "drv_ci3": "ci3", "drv_mat3": "mat3"}
+DRIVERS = {"drv_dropin": None, "drv_ci3": "ci3", "drv_mat3": "mat3", "drv_chan": "mat3"}
 
 
 def build_fortran(verbose=False):
@@ -84,12 +84,16 @@ def build_fortran(verbose=False):
         [FLANG, "-O2", "-fPIC", "-module-dir", mod, "-c", os.path.join(FDIR, "dbs_gpu.f90"), "-o", obj],
         [FLANG, "-shared", "-fPIC", obj, "-L" + HERE, "-ldbsr3_amd", "-Wl,-rpath,$ORIGIN", "-o", FLIB],
     ]
-    for name, src in ARCHIVES.items():
-        o = os.path.join(OBJDIR, f"dbs_gpu_{name}.o")
+    for name, srcs in ARCHIVES.items():
         a = os.path.join(HERE, f"libdbsr3_amd_{name}.a")
-        steps.append([FLANG, "-O2", "-fPIC", "-module-dir", mod, "-I" + mod, "-c", os.path.join(FDIR, src), "-o", o])
+        objs = []
+        for src in srcs:
+            o = os.path.join(OBJDIR, os.path.splitext(src)[0] + ".o")
+            steps.append([FLANG, "-O2", "-fPIC", "-module-dir", mod, "-I" + mod, "-c", os.path.join(FDIR, src),
+                          "-o", o])
+            objs.append(o)
         steps.append(["rm", "-f", a])
-        steps.append(["ar", "rcs", a, o])
+        steps.append(["ar", "rcs", a, *objs])
     for drv, arch in DRIVERS.items():
         libs = ([os.path.join(HERE, f"libdbsr3_amd_{arch}.a")] if arch else [])
         steps.append([FLANG, "-O2", os.path.join(TDIR, drv + ".f90"), "-I" + mod, *libs, "-L" + HERE,

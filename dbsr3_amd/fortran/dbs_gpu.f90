@@ -234,6 +234,45 @@
         Real(c_double), value :: qsum
         Integer(c_int) :: dbs_hf_matrix_dev
        End Function
+! ... channel-block post-processing (DBSR_MAT3)
+       Function dbs_zero_cond(nch, ns, nsp, nsq, ksp, ksq, lch, kch, ilzero, jlzero, ibzero, jbzero, npert, &
+                iprm, nhm) bind(C, name='dbs_zero_cond')
+        Import :: c_ptr, c_int
+        Integer(c_int), value :: nch, ns, nsp, nsq, ksp, ksq, ilzero, jlzero, ibzero, jbzero, npert
+        Type(c_ptr), value :: lch, kch, iprm
+        Integer(c_int) :: nhm
+        Integer(c_int) :: dbs_zero_cond
+       End Function
+       Function dbs_orth_conditions(nch, ipch, nbf, ipbs, kbs, ibort, ibort_user, ldo, nort, iort) &
+                bind(C, name='dbs_orth_conditions')
+        Import :: c_ptr, c_int, c_funptr
+        Integer(c_int), value :: nch, nbf, ldo
+        Type(c_ptr), value :: ipch, ipbs, kbs, ibort_user, nort, iort
+        Type(c_funptr), value :: ibort
+        Integer(c_int) :: dbs_orth_conditions
+       End Function
+       Function dbs_orth_dev(ctx, m, idiag, nort, iort, ldo, pv_dev, qv_dev, nbf) bind(C, name='dbs_orth_dev')
+        Import :: c_ptr, c_int
+        Type(c_ptr), value :: ctx, m, nort, iort, pv_dev, qv_dev
+        Integer(c_int), value :: idiag, ldo, nbf
+        Integer(c_int) :: dbs_orth_dev
+       End Function
+       Function dbs_diag_channels_dev(ctx, m, diag_dev, nd, iprm, edmin, edmax, egap, ipsol) &
+                bind(C, name='dbs_diag_channels_dev')
+        Import :: c_ptr, c_int, c_double
+        Type(c_ptr), value :: ctx, m, diag_dev, iprm, ipsol
+        Integer(c_int), value :: nd
+        Real(c_double), value :: edmin, edmax, egap
+        Integer(c_int) :: dbs_diag_channels_dev
+       End Function
+       Function dbs_record_matrix_dev(ctx, m, diag_dev, nd, ipsol, c_dev, c_cap, coff, w_dev, w_cap, woff, &
+                overlaps) bind(C, name='dbs_record_matrix_dev')
+        Import :: c_ptr, c_int, c_long_long
+        Type(c_ptr), value :: ctx, m, diag_dev, ipsol, c_dev, coff, w_dev, woff, overlaps
+        Integer(c_int), value :: nd
+        Integer(c_long_long), value :: c_cap, w_cap
+        Integer(c_int) :: dbs_record_matrix_dev
+       End Function
 ! ... the batched engine
        Function dbs_engine_create(eng, ninst, ctxs, norb, nrows, rows, kind) bind(C, name='dbs_engine_create')
         Import :: c_ptr, c_int
@@ -481,7 +520,7 @@
       Real(8), intent(in), target :: a(n)
       Type(c_ptr) :: p
       Call dbs_gpu_check(dbs_malloc(ctx, p, int(8 * max(n, 1), c_size_t)))
-      if (n > 0) Call dbs_gpu_check(dbs_memcpy_h2d(ctx, p, c_loc(a), int(8 * n, c_size_t)))
+      if (n > 0) Call dbs_gpu_check(dbs_memcpy_h2d(ctx, p, c_loc(a), 8_c_size_t * n))
       End Function dbs_gpu_put
 
       Subroutine dbs_gpu_get(p, a, n)
@@ -490,7 +529,7 @@
       Type(c_ptr), intent(in) :: p
       Integer, intent(in) :: n
       Real(8), intent(out), target :: a(n)
-      if (n > 0) Call dbs_gpu_check(dbs_memcpy_d2h(ctx, c_loc(a), p, int(8 * n, c_size_t)))
+      if (n > 0) Call dbs_gpu_check(dbs_memcpy_d2h(ctx, c_loc(a), p, 8_c_size_t * n))
       End Subroutine dbs_gpu_get
 
       End Module dbs_gpu_mem

@@ -9,6 +9,8 @@
 !     Call L_data(itype)              (DBSR_MAT3/L_data.f90:2-145)
 !     Call R_data(itype,kpol,btype)   (DBSR_MAT3/R_data.f90:2-290)
 !     Call S_data(ktype,kpol)         (DBSR_MAT3/S_data.f90:2-287)
+! and (dbs_gpu_mat3_post.f90) Zero_cond, DBS_ORTH, Diag_channels and
+! Record_matrix.
 ! Module dbs_gpu_mat plays the part of module dbsr_mat (icase, ibo, mk,
 ! eps_c, check_target, lbs/kbs/ipbs, V_ch, fppqq, iitar, iptar and the
 ! matrices hch, acf, hcp, hp with their tables icc, icb, ibb): the host
@@ -18,8 +20,15 @@
 ! are called back through bind(C) trampolines.  For L-integrals (icase 1)
 ! the host hands over, with dbs_gpu_mat_one_electron, what L_data reads
 ! from Libraries_03: Lval(i,j) of the orbitals as a table and the
-! one-electron operators hd(ms,ms,nkap) of module dhl_core.  Built into
-! libdbsr3_amd_mat3.a, linked into the host program.
+! one-electron operators hd(ms,ms,nkap) of module dhl_core.  For the
+! channel post-processing the host hands over, with dbs_gpu_mat_channels,
+! the grid sizes of DBS_grid, the channels' l, kappa and orbital
+! (channel_jj), the zero conditions, the energy window, idiag and the unit
+! Record_matrix writes; the diagonal blocks' overlaps are kept on the
+! device by dbs_gpu_mat_save_diag (sub1's diag = hch copy), and the host's
+! own IBORT(i,j), Get_pv(j,v,ns) and Get_qv(j,v,ns) are called for the
+! orthogonality conditions.  Built into libdbsr3_amd_mat3.a, linked into
+! the host program.
 !======================================================================
       Module dbs_gpu_mat
       Use iso_c_binding
@@ -77,6 +86,13 @@
       Type(c_ptr) :: hd_dev = c_null_ptr
       Integer :: nc_cur = 0
       Type(c_ptr) :: c_cur, k1_cur, k2_cur, k3_cur, k4_cur
+      ! channel post-processing (dbs_gpu_mat_channels, dbs_gpu_mat_save_diag)
+      Integer :: idiag = 0, nui = 0, ilzero = -1, jlzero = -1, ibzero = 1, jbzero = 1
+      Integer :: nsp_c = 0, nsq_c = 0, ksp_c = 0, ksq_c = 0, nch_c = 0, nhm = 0
+      Real(8) :: Edmin = -2.d0 * 137.03599976d0**2, Edmax = 1.d10, Egap = 1.d-3
+      Integer(c_int), allocatable, target :: lch(:), kch(:), ipch(:), iprm(:), ipsol(:)
+      Real(8), allocatable, target :: overlaps(:,:)
+      Type(c_ptr) :: diag_dev = c_null_ptr
       End Module dbs_gpu_mat
 
 !----------------------------------------------------------------------
@@ -151,7 +167,7 @@
         Integer(c_int), value :: i1, i2, m
         Real(c_double) :: s
        End Function
-       Subroutine dbs_gpu_mat_gen(user, itype, kpol, nc, c, K1, K2, K3, K4) bind(C)
+      Subroutine dbs_gpu_mat_gen(user, itype, kpol, nc, c, K1, K2, K3, K4) bind(C)
         Import :: c_ptr, c_int
         Type(c_ptr), value :: user, c, K1, K2, K3, K4
         Integer(c_int), value :: itype, kpol, nc
@@ -243,6 +259,105 @@
       Call dbs_gpu_get(mb%hcp_dev, hcp, ms * mb%ncb)
       Call dbs_gpu_get(mb%hp_dev, hp, mb%nbb)
       End Subroutine dbs_gpu_mat_fetch
+
+      Subroutine dbs_gpu_mat_put(hch, acf, hcp, hp)
+!     matrices from the host into the current storage (the inverse of
+!     dbs_gpu_mat_fetch)
+      Use dbs_gpu_api
+      Use dbs_gpu_mem
+      Use dbs_gpu_state, only: ctx
+      Use dbs_gpu_mat
+      Implicit none
+      Real(8), intent(in), target :: hch(*), acf(*), hcp(*), hp(*)
+      if (mb%nblocks > 0) Call dbs_gpu_check(dbs_memcpy_h2d(ctx, mb%hch_dev, c_loc(hch), &
+                                             int(8, c_size_t) * ms * ms * mb%nblocks))
+      if (mb%nblocks > 0) Call dbs_gpu_check(dbs_memcpy_h2d(ctx, mb%acf_dev, c_loc(acf), &
+                                             int(8, c_size_t) * mb%nblocks * (mb%mk + 1)))
+      if (mb%ncb > 0) Call dbs_gpu_check(dbs_memcpy_h2d(ctx, mb%hcp_dev, c_loc(hcp), int(8, c_size_t) * ms * mb%ncb))
+      if (mb%nbb > 0) Call dbs_gpu_check(dbs_memcpy_h2d(ctx, mb%hp_dev, c_loc(hp), int(8, c_size_t) * mb%nbb))
+      End Subroutine dbs_gpu_mat_put
+
+      Subroutine dbs_gpu_mat_channels(nsp, nsq, ksp, ksq, lch_, kch_, ipch_, ilzero_, jlzero_, ibzero_, jbzero_, &
+                                      edmin_, edmax_, egap_, idiag_, nui_)
+!     the post-processing's data of modules DBS_grid (nsp, nsq, ksp, ksq),
+!     channel_jj (l, kappa and orbital of each channel) and dbsr_mat (zero
+!     conditions, energy window, idiag of the current storage, unit nui)
+      Use dbs_gpu_mat
+      Implicit none
+      Integer, intent(in) :: nsp, nsq, ksp, ksq, lch_(*), kch_(*), ipch_(*), ilzero_, jlzero_, ibzero_, jbzero_, &
+                             idiag_, nui_
+      Real(8), intent(in) :: edmin_, edmax_, egap_
+      if (.not. c_associated(cdata)) Stop 'dbs_gpu_mat_channels: call dbs_gpu_mat_begin first'
+      nch_c = mb%nch
+      if (allocated(lch)) Deallocate(lch, kch, ipch)
+      Allocate(lch(max(nch_c, 1)), kch(max(nch_c, 1)), ipch(max(nch_c, 1)))
+      lch(1:nch_c) = lch_(1:nch_c); kch(1:nch_c) = kch_(1:nch_c); ipch(1:nch_c) = ipch_(1:nch_c)
+      nsp_c = nsp; nsq_c = nsq; ksp_c = ksp; ksq_c = ksq
+      ilzero = ilzero_; jlzero = jlzero_; ibzero = ibzero_; jbzero = jbzero_
+      Edmin = edmin_; Edmax = edmax_; Egap = egap_; idiag = idiag_; nui = nui_
+      End Subroutine dbs_gpu_mat_channels
+
+      Subroutine dbs_gpu_mat_save_diag
+!     diag(:,:,ich) = hch(:,:,icc(ich,ich)), kept on the device for
+!     Diag_channels and Record_matrix (DBSR_MAT3/sub1.f90:98-99)
+      Use dbs_gpu_api
+      Use dbs_gpu_mem
+      Use dbs_gpu_state, only: ctx
+      Use dbs_gpu_mat
+      Implicit none
+      Real(8), allocatable, target :: blk(:)
+      Integer :: ich, k
+      Integer(c_size_t) :: nb
+      if (.not. c_associated(cdata)) Stop 'dbs_gpu_mat_save_diag: call dbs_gpu_mat_begin first'
+      nb = int(8, c_size_t) * ms * ms
+      if (c_associated(diag_dev)) Call dbs_gpu_check(dbs_free(ctx, diag_dev))
+      Call dbs_gpu_check(dbs_malloc(ctx, diag_dev, nb * max(mb%nch, 1)))
+      Allocate(blk(ms * ms))
+      Do ich = 1, mb%nch
+       k = icc(ich, ich); blk = 0.d0
+       if (k > 0) Call dbs_gpu_check(dbs_memcpy_d2h(ctx, c_loc(blk), &
+                                     transfer(transfer(mb%hch_dev, 0_c_intptr_t) + nb * (k - 1), c_null_ptr), nb))
+       Call dbs_gpu_check(dbs_memcpy_h2d(ctx, transfer(transfer(diag_dev, 0_c_intptr_t) + nb * (ich - 1), &
+                          c_null_ptr), c_loc(blk), nb))
+      End do
+      End Subroutine dbs_gpu_mat_save_diag
+
+      Subroutine dbs_gpu_mat_fetch_channels(iprm_, nhm_, ipsol_, diag)
+!     Zero_cond's iprm(nch*ms+npert) and nhm, Diag_channels' ipsol(0:nch)
+!     and diag(ms,ms,nch) back to the host
+      Use dbs_gpu_mem
+      Use dbs_gpu_mat
+      Implicit none
+      Integer, intent(out) :: iprm_(*), nhm_, ipsol_(0:*)
+      Real(8), intent(out) :: diag(*)
+      if (.not. allocated(ipsol)) Stop 'dbs_gpu_mat_fetch_channels: call Diag_channels first'
+      iprm_(1:size(iprm)) = iprm; nhm_ = nhm; ipsol_(0:nch_c) = ipsol(0:nch_c)
+      Call dbs_gpu_get(diag_dev, diag, ms * ms * nch_c)
+      End Subroutine dbs_gpu_mat_fetch_channels
+
+      Subroutine dbs_gpu_mat_overlaps(ov)
+!     the overlaps(nch+npert,nch+npert) of the last Record_matrix with icase 0
+      Use dbs_gpu_mat
+      Implicit none
+      Real(8), intent(out) :: ov(*)
+      Integer :: n
+      if (.not. allocated(overlaps)) Stop 'dbs_gpu_mat_overlaps: no Record_matrix with icase 0'
+      n = size(overlaps, 1)
+      ov(1:n * n) = reshape(overlaps, [n * n])
+      End Subroutine dbs_gpu_mat_overlaps
+
+      Subroutine dbs_gpu_mat_channels_end
+      Use dbs_gpu_api
+      Use dbs_gpu_state, only: ctx
+      Use dbs_gpu_mat
+      Implicit none
+      if (c_associated(diag_dev)) Call dbs_gpu_check(dbs_free(ctx, diag_dev))
+      diag_dev = c_null_ptr
+      if (allocated(lch)) Deallocate(lch, kch, ipch)
+      if (allocated(iprm)) Deallocate(iprm)
+      if (allocated(ipsol)) Deallocate(ipsol)
+      if (allocated(overlaps)) Deallocate(overlaps)
+      End Subroutine dbs_gpu_mat_channels_end
 
       Subroutine dbs_gpu_mat_end
       Use dbs_gpu_api
@@ -358,3 +473,4 @@
       Call dbs_gpu_check(dbs_sdata_dev(ctx, c_loc(mb), c_loc(aux), ktype, kpol, nc_cur, c_cur, k1_cur, k2_cur, &
                          k3_cur, k4_cur, pq_dev, nbound))
       End Subroutine S_data
+

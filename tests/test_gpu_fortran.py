@@ -308,3 +308,174 @@ def test_fortran_mat3_state_res(tmp_path, gpu_available):
         s = np.max(np.abs(ref))
         assert s > 0, key
         assert np.max(np.abs(got[key] - ref)) <= 1e-12 * s, key
+
+
+CHAN = os.path.join(ROOT, "tests", "fortran", "drv_chan")
+
+
+def _fortran_records(path):
+    """the records of a Fortran unformatted sequential file (4-byte length
+    markers around each record), as raw bytes"""
+    raw = open(path, "rb").read()
+    out, o = [], 0
+    while o < len(raw):
+        n = int(np.frombuffer(raw, np.int32, 1, o)[0])
+        assert int(np.frombuffer(raw, np.int32, 1, o + 4 + n)[0]) == n
+        out.append(raw[o + 4:o + 4 + n])
+        o += n + 8
+    return out
+
+
+def _chan_case(seed):
+    """one partial wave's blocks as DBSR_MAT3's SUB1 holds them before the
+    post-processing, on the Fe grid: 4 channels (kappa -1, 1, -2, -1) and 2
+    perturbers; diagonal blocks = the Dirac-Coulomb operator with the Bloch
+    term (Z = 26) plus a seeded symmetric 1e-3 part, overlaps fppqq; the
+    other blocks, channel-perturber vectors and perturber elements seeded,
+    one of each zero (not recorded); 2 bound orbitals per kappa, zero on the
+    B-splines Zero_cond removes, B-orthonormal; IBORT = 0 except one pair"""
+    t, ns, ks, ksp, ksq = fe_grid()
+    g = O.Grid(t, ns, ks, ksp, ksq)
+    ms = g.ms
+    rng = np.random.default_rng(seed)
+    kch = [-1, 1, -2, -1]
+    nch, npert, nb = len(kch), 2, 2
+    lch = [k if k > 0 else -k - 1 for k in kch]
+    fppqq = O.overlaps(g)[2]
+    kset = sorted(set(kch))
+    hd = np.zeros((ms, ms, nch), order="F")
+    for i, kp in enumerate(kch):
+        R = rng.standard_normal((ms, ms)) * 1e-3
+        hd[:, :, i] = O.dhl_pq(g, 0, 26.0, 0.0, 0.0, 0.0, kp, bloch=1) + R + R.T
+    ovl = np.asfortranarray(np.repeat(fppqq[:, :, None], nch, axis=2))
+    nbound = nb * len(kset)
+    norb = nbound + nch
+    pv = np.zeros((ms, norb), order="F")
+    kbs = []
+    for kp in kset:
+        first = len(kbs)
+        alive = O.zero_cond(1, ns, g.nsp, g.nsq, ksp, ksq, [kp if kp > 0 else -kp - 1], [kp])[0]
+        for _ in range(nb):
+            v = rng.standard_normal(ms) * alive
+            for j in range(first, len(kbs)):
+                v -= (pv[:, j] @ fppqq @ v) * pv[:, j]
+            pv[:, len(kbs)] = v / np.sqrt(v @ fppqq @ v)
+            kbs.append(kp)
+    kbs = np.array(kbs + kch, np.int32)
+    lbs = np.array([k if k > 0 else -k - 1 for k in kbs], np.int32)
+    ipbs = np.array([0] * nbound + list(range(1, nch + 1)), np.int32)
+    ipch = np.array(range(nbound + 1, norb + 1), np.int32)
+    qv = np.asfortranarray(fppqq @ pv)
+    ib = np.zeros((norb, norb), np.int32, order="F")
+    ib[ipch[3] - 1, 2] = 1           # channel 4 (kappa -1) not kept orthogonal to orbital 3
+    # alloc_dbsr_matrix's numbering (DBSR_MAT3/updates.f90:20-70)
+    iccd = np.zeros((nch, nch), np.int32, order="F")
+    for i in range(nch):
+        iccd[i, i] = i + 1
+    icco = np.zeros((nch, nch), np.int32, order="F")
+    n = 0
+    for i in range(1, nch):
+        for j in range(i):
+            n += 1
+            icco[i, j] = icco[j, i] = n
+    icb = np.asfortranarray(np.arange(1, nch * npert + 1, dtype=np.int32).reshape((nch, npert)))
+    ibb = np.zeros((npert, npert), np.int32, order="F")
+    n = 0
+    for i in range(npert):
+        for j in range(i + 1):
+            n += 1
+            ibb[i, j] = ibb[j, i] = n
+    ho = np.asfortranarray(rng.standard_normal((ms, ms, int(icco.max()))) * 1e-2)
+    ho[:, :, icco[2, 1] - 1] = 0.0
+    hcp = np.asfortranarray(rng.standard_normal((ms, int(icb.max()))) * 1e-2)
+    hcp[:, icb[1, 0] - 1] = 0.0
+    hp = rng.standard_normal(int(ibb.max()))
+    hp[ibb[1, 0] - 1] = 0.0
+    return dict(t=t, g=g, ns=ns, ks=ks, ksp=ksp, ksq=ksq, kch=np.array(kch, np.int32),
+                lch=np.array(lch, np.int32), nch=nch, npert=npert, norb=norb, lbs=lbs, kbs=kbs, ipbs=ipbs,
+                ipch=ipch, ib=ib, pv=pv, qv=qv, iccd=iccd, ovl=ovl, hd=hd, icco=icco, icb=icb, ibb=ibb, ho=ho,
+                hcp=hcp, hp=hp, edmin=O.EDMIN, edmax=1e4, egap=O.EGAP)
+
+
+@pytest.mark.skipif(not os.path.exists(CHAN), reason="Fortran driver not built (no flang)")
+def test_fortran_mat3_channels(tmp_path, gpu_available):
+    """a host program run the way DBSR_MAT3's SUB1 runs one partial wave's
+    post-processing (DBS_ORTH, Diag_channels with Zero_cond, DBS_ORTH on the
+    other blocks, Record_matrix to an unformatted unit) through the
+    reference-named routines of libdbsr3_amd_mat3.a, with its own IBORT,
+    Get_pv and Get_qv: iprm, nhm, ipsol, the projected blocks, the
+    solutions, every record written and the overlaps against the oracle on
+    the same inputs, to the tolerances of the C-ABI channel tests"""
+    from test_gpu_channels import _diag_compare
+    c = _chan_case(31)
+    g, ns, nch, npert, norb = c["g"], c["ns"], c["nch"], c["npert"], c["norb"]
+    ms = g.ms
+    i32 = lambda *v: np.array(v, np.int32)
+    with open(tmp_path / "drv_chan_in.bin", "wb") as f:
+        _write(f, i32(ns, c["ks"], c["ksp"], c["ksq"], g.nsp, g.nsq), np.asarray(c["t"], np.float64),
+               i32(norb, nch, npert), c["lbs"], c["kbs"], c["ipbs"], c["ipch"], c["lch"], c["kch"], c["ib"],
+               c["pv"], c["qv"], c["iccd"], c["ovl"], c["hd"], c["icco"], c["icb"], c["ibb"], c["ho"], c["hcp"],
+               c["hp"], np.array([c["edmin"], c["edmax"], c["egap"]]))
+    print("[fortran MAT3 channel driver]", flush=True)
+    r = subprocess.run([CHAN], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert (tmp_path / "drv_chan_out.bin").exists(), r.stdout + r.stderr
+    raw = open(tmp_path / "drv_chan_out.bin", "rb").read()
+    ni = 1 + nch * ms + npert + nch + 1
+    ints = np.frombuffer(raw, np.int32, ni)
+    nhm, iprm, ipsol = int(ints[0]), ints[1:1 + nch * ms], ints[1 + nch * ms + npert:]
+    nbo, ncb = int(c["icco"].max()), int(c["icb"].max())
+    nt = nch + npert
+    dbl = np.frombuffer(raw, np.float64, offset=4 * ni)
+    sizes = [ms * ms * nch, ms * ms * nch, nt * nt, ms * ms * nbo, ms * ncb]
+    assert dbl.size == sum(sizes)
+    parts = np.split(dbl, np.cumsum(sizes)[:-1])
+    diag = parts[0].reshape((ms, ms, nch), order="F")
+    hdo = parts[1].reshape((ms, ms, nch), order="F")
+    ov = parts[2].reshape((nt, nt), order="F")
+    hout = parts[3].reshape((ms, ms, nbo), order="F")
+    hcpo = parts[4].reshape((ms, ncb), order="F")
+    # Zero_cond
+    iprm_o, nhm_o = O.zero_cond(nch, ns, g.nsp, g.nsq, c["ksp"], c["ksq"], c["lch"], c["kch"])
+    assert np.array_equal(iprm, iprm_o[:nch * ms]) and nhm == nhm_o
+    # DBS_ORTH on the diagonal blocks
+    ibort = lambda i, j: int(c["ib"][i - 1, j - 1])
+    orth = O.orth_conditions(nch, c["ipch"], c["ipbs"], c["kbs"], ibort)
+    assert orth[3] == [4] and orth[0] == [3, 4] and orth[1] == [5, 6] and orth[2] == [1, 2]
+    h1 = c["hd"].copy(order="F")
+    O.dbs_orth(h1, c["iccd"], 1, orth, c["pv"], c["qv"])
+    for b in range(nch):
+        assert np.max(np.abs(hdo[:, :, b] - h1[:, :, b])) <= 1e-12 * np.max(np.abs(h1[:, :, b])), b
+    # Diag_channels on the projected blocks
+    d_ref = c["ovl"].copy(order="F")
+    ip_ref = O.diag_channels(hdo, d_ref, c["iccd"], iprm_o, ns, edmin=c["edmin"], edmax=c["edmax"],
+                             egap=c["egap"])
+    assert np.array_equal(ipsol, ip_ref) and np.all(ipsol[1:] > 0) and np.all(ipsol[1:] < ns)
+    _diag_compare(g, hdo, c["ovl"], diag, d_ref, ipsol, iprm_o, c["iccd"])
+    # DBS_ORTH on the other blocks (idiag = -1: channel-perturber vectors kept)
+    h2, p2 = c["ho"].copy(order="F"), c["hcp"].copy(order="F")
+    O.dbs_orth(h2, c["icco"], -1, orth, c["pv"], c["qv"], p2, c["icb"], npert)
+    for b in range(nbo):
+        s = np.max(np.abs(h2[:, :, b]))
+        assert np.max(np.abs(hout[:, :, b] - h2[:, :, b])) <= 1e-12 * s if s > 0 else not np.any(hout[:, :, b])
+    assert np.array_equal(hcpo, c["hcp"])
+    # Record_matrix: the file's records against the oracle's on the GPU's
+    # own projected blocks and solutions
+    rrefs, ovr = O.record_matrix(hout, c["icco"], diag, ipsol, 0, hcpo, c["icb"], c["hp"], c["ibb"], npert)
+    recs = _fortran_records(tmp_path / "drv_chan_rec.bin")
+    assert len(recs) == 2 * len(rrefs) - 1
+    for n, (key, ref) in enumerate(rrefs):
+        assert tuple(int(x) for x in np.frombuffer(recs[2 * n], np.int32)) == key, n
+        if ref is None:
+            break
+        v = np.frombuffer(recs[2 * n + 1], np.float64)
+        if isinstance(ref, float):
+            assert v.size == 1 and v[0] == ref, key
+            continue
+        ref = np.asarray(ref).reshape(-1, order="F")
+        assert v.size == ref.size and np.max(np.abs(v - ref)) <= 1e-12 * np.max(np.abs(ref)), key
+    keys = [k for k, _ in rrefs]
+    assert (3, 2) not in keys and (nch + 1, 2) not in keys and (nch + 2, nch + 1) not in keys
+    assert (nch + 2, nch + 2) in keys and (nch + 1, 1) in keys
+    assert np.max(np.abs(ov - ovr)) <= 1e-12 * np.max(np.abs(ovr))
+    assert np.array_equal(ov == 0, ovr == 0)
