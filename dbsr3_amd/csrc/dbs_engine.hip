@@ -3392,14 +3392,44 @@ __global__ void k_list_accumulate(int ntarget, const long long* __restrict__ tpo
   }
 }
 
+// open-addressing table (linear probing) from 64-bit keys >= 0 to dense
+// ids 0, 1, ... in order of first insertion
+struct KeyIds {
+  std::vector<long long> key;
+  std::vector<int> id;
+  size_t mask = 0;
+  int n = 0;
+  void reset(size_t expect) {
+    size_t cap = 64;
+    while (cap < 2 * expect) cap <<= 1;
+    key.assign(cap, -1);
+    id.resize(cap);
+    mask = cap - 1;
+    n = 0;
+  }
+  // the id of k; a new id (== previous n) when k is new
+  int get(long long k) {
+    unsigned long long h = (unsigned long long)k * 0x9E3779B97F4A7C15ull;
+    size_t q = (size_t)(h ^ (h >> 29)) & mask;
+    while (key[q] >= 0) {
+      if (key[q] == k) return id[q];
+      q = (q + 1) & mask;
+    }
+    key[q] = k;
+    id[q] = n;
+    return n++;
+  }
+};
+
 // Rk_evaluate / Sk_evaluate (DBSR_CI3/rk_evaluate.f90:2-63,
 // sk_evaluate.f90:2-63) in the reference's organisation: the list's
-// distinct densities (one batched launch), one case-2 convolution per
-// distinct left density K1 (one batched launch against the context's
-// cached array of (itype, k)), one SUM_AmB per entry, then S*c added to
-// HM(ic,jc) / DM(jc) per target in list order.  Every buffer lives in the
-// context's grow-only scratch and the host tables go up in one copy, so a
-// flush allocates nothing once the scratch has grown.
+// distinct densities (one batched launch; the left densities K1 numbered
+// first), one case-2 convolution per distinct left density K1 (one matrix
+// product against the context's cached array of (itype, k), permuted to
+// K x K), one SUM_AmB per entry, then S*c added to HM(ic,jc) / DM(jc) per
+// target in list order.  Every buffer lives in the context's grow-only
+// scratch and the host tables go up in one copy, so a flush allocates
+// nothing once the scratch and the array cache have grown.
 int dbs_rk_evaluate_dev(dbs_ctx* ctx, int itype, int k, int nc, const double* c, const int* K1, const int* K2,
                         const int* K3, const int* K4, double eps_c, int ibi, const double* pq_dev, int norb,
                         double* HM_dev, int ldhm, int nzero, double* DM_dev) {
@@ -3420,79 +3450,84 @@ int dbs_rk_evaluate_dev(dbs_ctx* ctx, int itype, int k, int nc, const double* c,
     case DBS_PPQQ: lc1 = 0; lc2 = 1; rc1 = 0; rc2 = 1; break;
     default: lc1 = 0; lc2 = 1; rc1 = 1; rc2 = 0; break;
   }
-  std::map<std::tuple<int, int, int>, int> slot;
-  std::vector<int4> spec;
-  std::vector<int> lslots;
-  std::map<int, int> lindex;
-  std::vector<int2> lr;
-  std::vector<double> ck;
-  std::map<long long, int> tindex;
-  std::vector<long long> tpos;
-  std::vector<std::vector<int>> tent;
-  auto get_slot = [&](int a, int b, int cm) {
-    auto key = std::make_tuple(a, b, cm);
-    auto it = slot.find(key);
-    if (it != slot.end()) return it->second;
-    const int id = (int)spec.size();
-    spec.push_back(make_int4(a - 1, b - 1, cm, id));
-    slot[key] = id;
-    return id;
-  };
+  const int cmL = lc1 | (lc2 << 1), cmR = rc1 | (rc2 << 1);
+  // the entries kept (|c| >= eps_c, a target in HM or DM) and their targets
+  std::vector<int> keep;
+  std::vector<long long> pos;
+  keep.reserve(nc);
+  pos.reserve(nc);
   for (int j = 0; j < nc; ++j) {
     if (std::fabs(c[j]) < eps_c) continue;
     const int a = K1[j] / ibi, cc = K1[j] % ibi, b = K2[j] / ibi, d = K2[j] % ibi;
     DBS_CHECK(a >= 1 && a <= norb && cc >= 1 && cc <= norb && b >= 1 && b <= norb && d >= 1 && d <= norb,
               DBS_ERR_ARG, "Rk_evaluate: orbital index out of range");
     const int ic = K3[j], jc = K4[j];
-    long long pos;
+    long long p;
     if (jc <= nzero) {
       DBS_CHECK(ic >= 1 && jc >= 1 && ic <= ldhm, DBS_ERR_ARG, "Rk_evaluate: HM index out of range");
-      pos = (long long)(ic - 1) + (long long)ldhm * (jc - 1);
+      p = (long long)(ic - 1) + (long long)ldhm * (jc - 1);
     } else if (ic == jc) {
       DBS_CHECK(DM_dev, DBS_ERR_ARG, "Rk_evaluate: DM needed");
-      pos = -(long long)jc;  // DM(jc) at -(jc-1)-1
+      p = -(long long)jc;  // DM(jc) at -(jc-1)-1
     } else {
       continue;
     }
-    const int ls = get_slot(a, cc, lc1 | (lc2 << 1));
-    const int rs = get_slot(b, d, rc1 | (rc2 << 1));
-    auto li = lindex.find(ls);
-    int l;
-    if (li == lindex.end()) {
-      l = (int)lslots.size();
-      lslots.push_back(ls);
-      lindex[ls] = l;
-    } else {
-      l = li->second;
-    }
-    const int q = (int)lr.size();
-    lr.push_back(make_int2(l, rs));
-    ck.push_back(c[j]);
-    auto ti = tindex.find(pos);
-    if (ti == tindex.end()) {
-      tindex[pos] = (int)tpos.size();
-      tpos.push_back(pos);
-      tent.emplace_back(1, q);
-    } else {
-      tent[ti->second].push_back(q);
-    }
+    keep.push_back(j);
+    pos.push_back(p);
   }
-  const int n = (int)lr.size();
+  const int n = (int)keep.size();
   if (n == 0) return DBS_OK;
-  std::vector<int> toff(1, 0), tflat;
-  for (auto& v : tent) {
-    tflat.insert(tflat.end(), v.begin(), v.end());
-    toff.push_back((int)tflat.size());
+  // densities: key (component pattern, a, c); the left ones take ids
+  // 0..nl-1, the right ones follow (sharing a left id when equal)
+  const long long nob = (long long)norb + 1;
+  auto dkey = [&](int cm, int a, int b) { return ((long long)cm * nob + a) * nob + b; };
+  KeyIds dens_ids, targ_ids;
+  dens_ids.reset((size_t)n);
+  std::vector<int4> spec;
+  std::vector<int2> lr(n);
+  std::vector<double> ck(n);
+  for (int q = 0; q < n; ++q) {
+    const int j = keep[q];
+    const int l = dens_ids.get(dkey(cmL, K1[j] / ibi, K1[j] % ibi));
+    if (l == (int)spec.size()) spec.push_back(make_int4(K1[j] / ibi - 1, K1[j] % ibi - 1, cmL, l));
+    lr[q].x = l;
+    ck[q] = c[j];
   }
+  const int nl = (int)spec.size();
+  for (int q = 0; q < n; ++q) {
+    const int j = keep[q];
+    const int r = dens_ids.get(dkey(cmR, K2[j] / ibi, K2[j] % ibi));
+    if (r == (int)spec.size()) spec.push_back(make_int4(K2[j] / ibi - 1, K2[j] % ibi - 1, cmR, r));
+    lr[q].y = r;
+  }
+  // targets: HM/DM positions in order of first appearance, each with its
+  // entries in list order (CSR)
+  targ_ids.reset((size_t)n);
+  std::vector<int> tid(n);
+  std::vector<long long> tpos;
+  std::vector<int> toff(1, 0);
+  for (int q = 0; q < n; ++q) {
+    const long long key = pos[q] >= 0 ? 2 * pos[q] : 2 * (-pos[q] - 1) + 1;
+    tid[q] = targ_ids.get(key);
+    if (tid[q] == (int)tpos.size()) {
+      tpos.push_back(pos[q]);
+      toff.push_back(0);
+    }
+    ++toff[tid[q] + 1];
+  }
+  const int nt = (int)tpos.size();
+  for (int t = 0; t < nt; ++t) toff[t + 1] += toff[t];
+  std::vector<int> tflat(n), fill(toff.begin(), toff.end() - 1);
+  for (int q = 0; q < n; ++q) tflat[fill[tid[q]]++] = q;
   std::vector<long long> rowout(n);
   for (int q = 0; q < n; ++q) rowout[q] = q;
-  double* rkb = nullptr;
-  int rc = build_rkb(ctx, itype, k, &rkb);
+  double* rkm = nullptr;
+  int rc = build_rkm(ctx, itype, k, &rkm);
   if (rc) return rc;
   // scratch (doubles): densities | convolutions | values | then the host
-  // tables (coefficients, spec, left slots, row pairs, row outputs, targets)
+  // tables (coefficients, spec, row pairs, row outputs, targets)
   const size_t per = (size_t)ns * nb;
-  const size_t o_conv = spec.size() * per, o_vals = o_conv + lslots.size() * per, o_tab = o_vals + n;
+  const size_t o_conv = spec.size() * per, o_vals = o_conv + (size_t)nl * per, o_tab = o_vals + n;
   std::vector<char> blob;
   auto place = [&](const void* src, size_t bytes) {
     const size_t off = (blob.size() + 15) & ~(size_t)15;
@@ -3502,7 +3537,6 @@ int dbs_rk_evaluate_dev(dbs_ctx* ctx, int itype, int k, int nc, const double* c,
   };
   const size_t b_c = place(ck.data(), ck.size() * sizeof(double));
   const size_t b_spec = place(spec.data(), spec.size() * sizeof(int4));
-  const size_t b_ls = place(lslots.data(), lslots.size() * sizeof(int));
   const size_t b_lr = place(lr.data(), lr.size() * sizeof(int2));
   const size_t b_out = place(rowout.data(), rowout.size() * sizeof(long long));
   const size_t b_tpos = place(tpos.data(), tpos.size() * sizeof(long long));
@@ -3517,11 +3551,18 @@ int dbs_rk_evaluate_dev(dbs_ctx* ctx, int itype, int k, int nc, const double* c,
   double* vals = ctx->scratch + o_vals;
   hipError_t e = hipMemcpyAsync(base, blob.data(), blob.size(), hipMemcpyHostToDevice, s);
   if (e == hipSuccess) {
-    const int nsp = (int)spec.size(), nl = (int)lslots.size(), nt = (int)tpos.size();
+    const int nsp = (int)spec.size();
     k_plain_dens<<<grid_for((long long)nsp * per, 256, 8192), 256, 0, s>>>(
         nsp, reinterpret_cast<const int4*>(base + b_spec), ns, ks, sym, pq_dev, 0, dens);
-    k_plain_conv2<<<grid_for((long long)nl * per, 4, 65535), 256, 0, s>>>(
-        nl, reinterpret_cast<const int*>(base + b_ls), ns, ks, sym, rkb, dens, conv);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) {
+    // conv(:, l) = rkm^T dens(:, l) for the left densities 0..nl-1
+    rc = conv2_gemm(ctx, (int)per, nl, rkm, dens, conv);
+    if (rc) {
+      (void)hipStreamSynchronize(s);
+      return rc;
+    }
     k_plain_rows<<<grid_for(n, 4, 65535), 256, 0, s>>>(n, reinterpret_cast<const int2*>(base + b_lr), ns, ks,
                                                          sym, conv, dens,
                                                          reinterpret_cast<const long long*>(base + b_out), 0,

@@ -106,6 +106,12 @@ int ensure_scratch(dbs_ctx* c, size_t n);
 int ensure_stage(dbs_ctx* c, size_t n);
 int kband(int type, int ks);  // third/fourth dimension of rkb for the type
 int build_rkb(dbs_ctx* c, int type, int k, double** out);
+// rkb permuted to the K x K matrix rkm(i + ns*ip, j + ns*jp), K = ns*kb,
+// cached with the arrays (negative keys in rkb_cache, one LRU and limit)
+int build_rkm(dbs_ctx* c, int type, int k, double** out);
+// conv(K, n) = rkm^T dens(K, n) on the context stream (rocBLAS): convol
+// case 2 of n densities against one array
+int conv2_gemm(dbs_ctx* c, int K, int n, const double* rkm, const double* dens, double* conv);
 }  // namespace dbs
 
 namespace dbs {

@@ -1065,6 +1065,17 @@ int solver_of(dbs_ctx* ctx, rocblas_handle* h) {
   return DBS_OK;
 }
 
+int conv2_gemm(dbs_ctx* c, int K, int n, const double* rkm, const double* dens, double* conv) {
+  rocblas_handle h;
+  int rc = solver_of(c, &h);
+  if (rc) return rc;
+  const double one = 1.0, zero = 0.0;
+  if (rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, K, n, K, &one, rkm, K, dens, K, &zero,
+                    conv, K) != rocblas_status_success)
+    return fail(DBS_ERR_HIP, "Rk_evaluate: convolution product");
+  return DBS_OK;
+}
+
 namespace {
 
 // hf_eiv (hf_solve_HF.f90:136-202): orthogonality to each vorth column

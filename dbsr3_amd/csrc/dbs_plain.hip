@@ -359,7 +359,7 @@ int build_rkb(dbs_ctx* c, int type, int k, double** out) {
     int old = c->rkb_lru.front();
     c->rkb_lru.erase(c->rkb_lru.begin());
     DBS_HIP(hipStreamSynchronize(c->stream));
-    int okb = kband(old / 1000, ks);
+    int okb = kband((old >= 0 ? old : -old - 1) / 1000, ks);  // negative keys: build_rkm's matrices
     DBS_HIP(hipFree(c->rkb_cache[old]));
     c->rkb_cache_bytes -= (size_t)ns * ns * okb * okb * sizeof(double);
     c->rkb_cache.erase(old);
@@ -405,6 +405,61 @@ int build_rkb(dbs_ctx* c, int type, int k, double** out) {
   c->rkb_lru.push_back(key);
   c->rkb_cache_bytes += bytes;
   *out = rkb;
+  return DBS_OK;
+}
+
+// rkm(r, c) = rkb(i, j, ip, jp) with r = i + ns*ip, c = j + ns*jp (K x K,
+// K = ns*kb, column-major): the read side coalesced along i, the write side
+// along r
+__global__ void k_rkb_permute(int ns, int kb, const double* __restrict__ rkb, double* __restrict__ rkm) {
+  const long long K = (long long)ns * kb, n = K * K;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+       q += (long long)gridDim.x * blockDim.x) {
+    const long long r = q % K, cc = q / K;
+    const int i = (int)(r % ns), ip = (int)(r / ns), j = (int)(cc % ns), jp = (int)(cc / ns);
+    rkm[q] = rkb[i + (long long)ns * (j + (long long)ns * (ip + (long long)kb * jp))];
+  }
+}
+
+int build_rkm(dbs_ctx* c, int type, int k, double** out) {
+  const int key = -(type * 1000 + k) - 1;
+  auto it = c->rkb_cache.find(key);
+  if (it != c->rkb_cache.end()) {
+    *out = it->second;
+    auto p = std::find(c->rkb_lru.begin(), c->rkb_lru.end(), key);
+    if (p != c->rkb_lru.end()) c->rkb_lru.erase(p);
+    c->rkb_lru.push_back(key);
+    return DBS_OK;
+  }
+  double* rkb = nullptr;
+  int rc = build_rkb(c, type, k, &rkb);
+  if (rc) return rc;
+  const int ns = c->ns, kb = kband(type, c->ks);
+  const size_t bytes = (size_t)ns * ns * kb * kb * sizeof(double);
+  // evict least recently used arrays beyond the cache limit, never the
+  // array just built or touched (the back of the list)
+  while (c->rkb_lru.size() > 1 && c->rkb_cache_bytes + bytes > c->rkb_cache_limit) {
+    int old = c->rkb_lru.front();
+    c->rkb_lru.erase(c->rkb_lru.begin());
+    DBS_HIP(hipStreamSynchronize(c->stream));
+    int okb = kband((old >= 0 ? old : -old - 1) / 1000, c->ks);
+    DBS_HIP(hipFree(c->rkb_cache[old]));
+    c->rkb_cache_bytes -= (size_t)ns * ns * okb * okb * sizeof(double);
+    c->rkb_cache.erase(old);
+  }
+  double* rkm = nullptr;
+  DBS_HIP(hipMalloc(&rkm, bytes));
+  const long long n = (long long)ns * ns * kb * kb;
+  k_rkb_permute<<<(unsigned)std::min<long long>((n + 255) / 256, 1 << 20), 256, 0, c->stream>>>(ns, kb, rkb, rkm);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    (void)hipFree(rkm);
+    return fail(DBS_ERR_HIP, std::string("mrk: ") + hipGetErrorString(e));
+  }
+  c->rkb_cache[key] = rkm;
+  c->rkb_lru.push_back(key);
+  c->rkb_cache_bytes += bytes;
+  *out = rkm;
   return DBS_OK;
 }
 
