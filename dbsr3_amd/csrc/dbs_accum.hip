@@ -227,26 +227,22 @@ double smu_factors(int ka, int kb, int kc, int kd, int L, int v, double* S) {
 namespace dbs {
 
 // xx_g = sum_e c_e * density(pq(jp1,j1), pq(jp2,j2)) in list order;
-// sym 0: band 's' (ns,ks); 1: full 'x' (ns,ns). tgb = 0: group-major
-// xx[g][r]; tgb > 0: chunks of tgb groups with the group index fastest,
-// xx[g/tgb][r][g%tgb], the last chunk's missing groups written as zeros
+// sym 0: band 's' (ns,ks); 1: full 'x' (ns,ns); one group after another
 __global__ void k_group_density(int ng, const int* __restrict__ goff, const int* __restrict__ ej1,
                                 const int* __restrict__ ej2, const double* __restrict__ ec,
                                 const double* __restrict__ pq, int ns, int ks, int jp1, int jp2,
-                                int sym, int tgb, double* __restrict__ xx) {
+                                int sym, double* __restrict__ xx) {
 #pragma clang fp contract(off)
   const int ncol = sym == 0 ? ks : ns;
   const long long per = (long long)ns * ncol;
-  const int ngw = tgb > 0 ? (ng + tgb - 1) / tgb * tgb : ng;  // groups written
-  const long long total = per * ngw;
+  const long long total = per * ng;
   for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
        q += (long long)gridDim.x * blockDim.x) {
     const int g = (int)(q / per);
     const int r = (int)(q % per);
     const int i = r % ns, j = r / ns;
     double acc = 0.0;
-    const int e0 = g < ng ? goff[g] : 0, e1 = g < ng ? goff[g + 1] : 0;
-    for (int e = e0; e < e1; ++e) {
+    for (int e = goff[g]; e < goff[g + 1]; ++e) {
       const double* p1 = pq + (long long)ns * ((jp1 - 1) + 2LL * (ej1[e] - 1));
       const double* p2 = pq + (long long)ns * ((jp2 - 1) + 2LL * (ej2[e] - 1));
       double d;
@@ -259,63 +255,85 @@ __global__ void k_group_density(int ng, const int* __restrict__ goff, const int*
       }
       acc = acc + ec[e] * d;
     }
-    xx[tgb > 0 ? ((long long)(g / tgb) * per + r) * tgb + g % tgb : q] = acc;
+    xx[q] = acc;
   }
 }
 
-// convol case 1 for many groups, one workgroup per 256 outputs (i,ip) and
-// a chunk of GB groups: the chunk is uniform over the workgroup and its
-// densities are stored group-fastest (k_group_density with tgb = GB), so the
-// GB values of one (j,jp) are one contiguous wave-uniform (scalar) read
-// shared by all lanes, and each rkb value feeds GB products. XCD-aware block order as in
-// k_group_convol3_tile (one (i,ip) range's chunks on one XCD, its rkb slice
-// in that XCD's L2). a_g(i,ip) = sum_jp sum_j xx_g(j,jp) rkb(i,j,ip,jp),
-// summed over jp, then j (convol case 1, dbsr_lib_dbs.f90:2249-2253); outputs with
-// i >= ns-ip are 0; fused multiply-adds. Needs a grid that is a multiple of 8.
-constexpr int CV1_GB = 4;
+// the full 'x' densities of the groups, xx_g = sum_e (c_e p1_e) p2_e^T,
+// as register-tiled rank-k updates: one workgroup per 64x64 tile of one
+// group, the group's entries staged DX_E at a time in LDS (c_e p1_e over
+// the tile's rows, p2_e over its columns), each thread 4x4 outputs (rows
+// tx + 16a, columns ty + 16b) summed over the entries in list order
+// (fused multiply-adds); grid = ng * nt * nt, nt = ceil(ns / 64)
+constexpr int DX_T = 64, DX_E = 16;
 
-template <int GB>
-__global__ __launch_bounds__(256) void k_group_convol1_u(int ng, int ns, int ks, int nqb,
-                                                         const double* __restrict__ rkb,
-                                                         const double* __restrict__ xx,
-                                                         double* __restrict__ out) {
-  const int nchunk = (ng + GB - 1) / GB;
-  const int lid = (int)(blockIdx.x % 8) * (int)(gridDim.x / 8) + (int)(blockIdx.x / 8);
-  if (lid >= nqb * nchunk) return;
-  const int qb = lid / nchunk, chunk = lid % nchunk;
-  const int q = qb * 256 + (int)threadIdx.x;
-  if (q >= ns * ks) return;
-  const int i = q % ns, ip = q / ns;
-  const int g0 = chunk * GB, nb = min(GB, ng - g0);
-  const long long per = (long long)ns * ks, nn = (long long)ns * ns;
-  const double* xc = xx + (long long)chunk * per * GB;  // this chunk: [r][b]
-  double acc[GB];
+__global__ __launch_bounds__(256) void k_group_density_x(int ng, int nt, const int* __restrict__ goff,
+                                                         const int* __restrict__ ej1, const int* __restrict__ ej2,
+                                                         const double* __restrict__ ec,
+                                                         const double* __restrict__ pq, int ns, int jp1, int jp2,
+                                                         double* __restrict__ xx) {
+  __shared__ double A[DX_E][DX_T];
+  __shared__ double B[DX_E][DX_T];
+  const int nb = nt * nt;
+  const int g = (int)blockIdx.x / nb, tile = (int)blockIdx.x % nb;
+  if (g >= ng) return;
+  const int i0 = (tile % nt) * DX_T, j0 = (tile / nt) * DX_T;
+  const int tid = (int)threadIdx.x, tx = tid % 16, ty = tid / 16;
+  double acc[4][4];
 #pragma unroll
-  for (int b = 0; b < GB; ++b) acc[b] = 0.0;
-  // every lane runs the loop (rkb(i,j,ip,jp) is in bounds for all i, j <
-  // ns), so the density loads stay uniform; invalid outputs are zeroed below
-  for (int jp = 0; jp < ks; ++jp) {
-    const double* rr = rkb + i + nn * (ip + (long long)ks * jp);
-    const int nj = ns - jp;
-    const double* xj = xc + (long long)ns * jp * GB;
-    // rkb values in batches of 8 loads issued ahead of their products
-    for (int j = 0; j < nj; j += 8) {
-      double r[8];
+  for (int a = 0; a < 4; ++a)
 #pragma unroll
-      for (int u = 0; u < 8; ++u) r[u] = j + u < nj ? rr[(long long)ns * (j + u)] : 0.0;
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        if (j + u >= nj) break;
-        const double* xv = xj + (long long)(j + u) * GB;
-#pragma unroll
-        for (int b = 0; b < GB; ++b) acc[b] = fma(xv[b], r[u], acc[b]);
+    for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+  const int e0 = goff[g], e1 = goff[g + 1];
+  for (int eb = e0; eb < e1; eb += DX_E) {
+    const int ne = min(DX_E, e1 - eb);
+    __syncthreads();
+    for (int q = tid; q < DX_E * DX_T; q += 256) {
+      const int e = q / DX_T, r = q % DX_T;
+      double av = 0.0, bv = 0.0;
+      if (e < ne) {
+        const int ee = eb + e;
+        if (i0 + r < ns) av = ec[ee] * pq[(long long)ns * ((jp1 - 1) + 2LL * (ej1[ee] - 1)) + i0 + r];
+        if (j0 + r < ns) bv = pq[(long long)ns * ((jp2 - 1) + 2LL * (ej2[ee] - 1)) + j0 + r];
       }
+      A[e][r] = av;
+      B[e][r] = bv;
+    }
+    __syncthreads();
+    for (int e = 0; e < ne; ++e) {
+      double av[4], bv[4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) av[a] = A[e][tx + 16 * a];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) bv[b] = B[e][ty + 16 * b];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = fma(av[a], bv[b], acc[a][b]);
     }
   }
-  const bool valid = i < ns - ip;
+  double* out = xx + (long long)g * ns * ns;
 #pragma unroll
-  for (int b = 0; b < GB; ++b)
-    if (b < nb) out[(long long)(g0 + b) * per + i + (long long)ns * ip] = valid ? acc[b] : 0.0;
+  for (int b = 0; b < 4; ++b) {
+    const int j = j0 + ty + 16 * b;
+    if (j >= ns) continue;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const int i = i0 + tx + 16 * a;
+      if (i < ns) out[i + (long long)ns * j] = acc[a][b];
+    }
+  }
+}
+
+// the outputs of convol case 1 with sym 's' outside the band (i >= ns-ip)
+// set to zero, as the reference's convol leaves them
+__global__ void k_band_zero(int ng, int ns, int ks, double* __restrict__ a) {
+  const long long per = (long long)ns * ks, total = per * ng;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+       q += (long long)gridDim.x * blockDim.x) {
+    const int r = (int)(q % per), i = r % ns, ip = r / ns;
+    if (i >= ns - ip) a[q] = 0.0;
+  }
 }
 
 // tiled convol case 3: one workgroup computes a CV_TX x CV_TY block of
@@ -884,23 +902,23 @@ int dbs_rdata_34_dev(dbs_ctx* ctx, int itype, int atype, int k, int nc, const do
   if (ng == 0) return DBS_OK;
   const int sym = itype == 3 ? 0 : 1;
   // launch sizes, checked before anything is queued on the stream
-  const int nqb = (ns * ks + 255) / 256;
   const int ntx = (ns + CV_TX - 1) / CV_TX, nty = (ns + CV_TY - 1) / CV_TY;
-  if (sym == 0 && (long long)nqb * ((ng + CV1_GB - 1) / CV1_GB) > (1ll << 30))
-    return fail(DBS_ERR_ARG, "R_data: too many convolution blocks");
   if (sym == 1 && ks <= CV_KS && (long long)ntx * nty * ((ng + 3) / 4) > (1ll << 30))
     return fail(DBS_ERR_ARG, "R_data: too many convolution tiles");
+  if (sym == 1 && (long long)ng * ((ns + DX_T - 1) / DX_T) * ((ns + DX_T - 1) / DX_T) > (1ll << 31) - 1)
+    return fail(DBS_ERR_ARG, "R_data: too many density tiles");
+  // case 1 (itype 3): one matrix product against the permuted array;
+  // case 3 (itype 4): the array itself
   double* rkb = nullptr;
-  int rc = build_rkb(ctx, type, k, &rkb);
+  int rc = sym == 0 ? build_rkm(ctx, type, k, &rkb) : build_rkb(ctx, type, k, &rkb);
   if (rc) return rc;
   hipStream_t s = ctx->stream;
   const long long per = (long long)ns * (sym == 0 ? ks : ns);
   // every buffer of the call in the context's grow-only scratch (no device
   // allocation per call): densities, convolutions, coefficients, then the
-  // integer tables (run offsets, orbitals, run info), 16-byte aligned
-  // band densities in chunks of CV1_GB groups (group index fastest), padded
-  // to whole chunks; the full-square case keeps one group after another
-  const size_t nxx = (size_t)per * (sym == 0 ? (ng + CV1_GB - 1) / CV1_GB * CV1_GB : ng), nce = ec.size();
+  // integer tables (run offsets, orbitals, run info), 16-byte aligned;
+  // densities and convolutions one group after another
+  const size_t nxx = (size_t)per * ng, nce = ec.size();
   const size_t n_int = goff.size() + ej1.size() + ej2.size();
   const size_t off_c = 2 * nxx, off_i = off_c + nce, off_g = off_i + (n_int + 1) / 2 + 1;
   const size_t need = (off_g + 2 * (size_t)ng + 2) & ~(size_t)1;
@@ -927,13 +945,20 @@ int dbs_rdata_34_dev(dbs_ctx* ctx, int itype, int atype, int k, int nc, const do
     (void)hipStreamSynchronize(s);  // blob is a pageable host buffer that ends with this call
     return fail(DBS_ERR_HIP, "R_data: upload");
   }
-  k_group_density<<<grid_of(per * (sym == 0 ? (ng + CV1_GB - 1) / CV1_GB * CV1_GB : ng), 256), 256, 0, s>>>(ng, d_goff, d_j1, d_j2, d_c, pq_dev, ns, ks, jp1, jp2,
-                                                          sym, sym == 0 ? CV1_GB : 0, d_xx);
   if (sym == 0) {
-    constexpr int GB = CV1_GB;
-    const long long nblk = (long long)nqb * ((ng + GB - 1) / GB);
-    const unsigned grid = (unsigned)((nblk + 7) / 8 * 8);
-    k_group_convol1_u<GB><<<grid, 256, 0, s>>>(ng, ns, ks, nqb, rkb, d_xx, d_a);
+    k_group_density<<<grid_of(per * ng, 256), 256, 0, s>>>(ng, d_goff, d_j1, d_j2, d_c, pq_dev, ns, ks, jp1, jp2,
+                                                            sym, d_xx);
+  } else {
+    const int nt = (ns + DX_T - 1) / DX_T;
+    k_group_density_x<<<(unsigned)((long long)ng * nt * nt), 256, 0, s>>>(ng, nt, d_goff, d_j1, d_j2, d_c, pq_dev,
+                                                                          ns, jp1, jp2, d_xx);
+  }
+  if (sym == 0) {
+    if ((rc = conv_gemm(ctx, 1, ns * ks, ng, rkb, d_xx, d_a))) {
+      (void)hipStreamSynchronize(s);
+      return rc;
+    }
+    k_band_zero<<<grid_of(per * ng, 256), 256, 0, s>>>(ng, ns, ks, d_a);
   } else if (ks <= CV_KS) {
     constexpr int GB = 4;  // 37 KB of LDS, four workgroups per CU (measured best of 2, 4, 8, 16)
     const long long nblk = (long long)ntx * nty * ((ng + GB - 1) / GB);

@@ -3558,7 +3558,7 @@ int dbs_rk_evaluate_dev(dbs_ctx* ctx, int itype, int k, int nc, const double* c,
   }
   if (e == hipSuccess) {
     // conv(:, l) = rkm^T dens(:, l) for the left densities 0..nl-1
-    rc = conv2_gemm(ctx, (int)per, nl, rkm, dens, conv);
+    rc = conv_gemm(ctx, 2, (int)per, nl, rkm, dens, conv);
     if (rc) {
       (void)hipStreamSynchronize(s);
       return rc;

@@ -109,9 +109,11 @@ int build_rkb(dbs_ctx* c, int type, int k, double** out);
 // rkb permuted to the K x K matrix rkm(i + ns*ip, j + ns*jp), K = ns*kb,
 // cached with the arrays (negative keys in rkb_cache, one LRU and limit)
 int build_rkm(dbs_ctx* c, int type, int k, double** out);
-// conv(K, n) = rkm^T dens(K, n) on the context stream (rocBLAS): convol
-// case 2 of n densities against one array
-int conv2_gemm(dbs_ctx* c, int K, int n, const double* rkm, const double* dens, double* conv);
+// convol of n densities (K x n, one after another) against one array on the
+// context stream (rocBLAS dgemm): case 1 conv = rkm dens, case 2
+// conv = rkm^T dens (every output of the K x n result written; the entries
+// outside the band are the products' and not zero)
+int conv_gemm(dbs_ctx* c, int icase, int K, int n, const double* rkm, const double* dens, double* conv);
 }  // namespace dbs
 
 namespace dbs {

@@ -1065,14 +1065,15 @@ int solver_of(dbs_ctx* ctx, rocblas_handle* h) {
   return DBS_OK;
 }
 
-int conv2_gemm(dbs_ctx* c, int K, int n, const double* rkm, const double* dens, double* conv) {
+int conv_gemm(dbs_ctx* c, int icase, int K, int n, const double* rkm, const double* dens, double* conv) {
   rocblas_handle h;
   int rc = solver_of(c, &h);
   if (rc) return rc;
   const double one = 1.0, zero = 0.0;
-  if (rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, K, n, K, &one, rkm, K, dens, K, &zero,
-                    conv, K) != rocblas_status_success)
-    return fail(DBS_ERR_HIP, "Rk_evaluate: convolution product");
+  const rocblas_operation op = icase == 2 ? rocblas_operation_transpose : rocblas_operation_none;
+  if (rocblas_dgemm(h, op, rocblas_operation_none, K, n, K, &one, rkm, K, dens, K, &zero, conv, K) !=
+      rocblas_status_success)
+    return fail(DBS_ERR_HIP, "convol: convolution product");
   return DBS_OK;
 }
 
