@@ -495,6 +495,62 @@ __global__ void k_update_hs_mat(int ng, const int4* __restrict__ gi, int ns, int
   }
 }
 
+// UPDATE_HS for full 'x' convolutions (sym 1), both passes in one launch:
+// one workgroup per 32x32 tile (a, b) of one group's output block, y over
+// the tile staged in LDS as read (Td: y(a,b)) and, when the update needs
+// it, transposed (Tt: y(b,a)), so global reads and writes are coalesced.
+// Per element the reference's order: ich > jch h += y(a,b); jch > ich
+// h += y(b,a); ich == jch pass 0 h(i1+a,j1+b) += y(a,b)/2 then pass 1
+// h(j1+a,i1+b) += y(b,a)/2 (one element when i1 == j1, added in that
+// order by one thread).  grid = ng * nt * nt, nt = ceil(ns / 32).
+__global__ __launch_bounds__(256) void k_update_hs_x(int ng, int nt, const int4* __restrict__ gi, int ns, int ii,
+                                                     int jj, const double* __restrict__ d,
+                                                     double* __restrict__ hch) {
+#pragma clang fp contract(off)
+  __shared__ double Td[32][33];
+  __shared__ double Tt[32][33];
+  const int nb = nt * nt;
+  const int g = (int)blockIdx.x / nb, tile = (int)blockIdx.x % nb;
+  if (g >= ng) return;
+  const int a0 = (tile % nt) * 32, b0 = (tile / nt) * 32;
+  const int tx = (int)threadIdx.x % 32, ty = (int)threadIdx.x / 32;
+  const int4 G = gi[g];  // (ich, jch, block0, 0)
+  const int ich = G.x, jch = G.y;
+  const double* dg = d + (long long)g * ns * ns;
+  for (int m = ty; m < 32; m += 8) {
+    // Td[bl][al] = y(a0+al, b0+bl); Tt[al][bl] = y(b0+bl, a0+al)
+    const int a = a0 + tx, b = b0 + m;
+    Td[m][tx] = (a < ns && b < ns) ? dg[a + (long long)ns * b] : 0.0;
+    const int bt = b0 + tx, at = a0 + m;
+    Tt[m][tx] = (ich <= jch && bt < ns && at < ns) ? dg[bt + (long long)ns * at] : 0.0;
+  }
+  __syncthreads();
+  const int ms = 2 * ns;
+  double* h = hch + (long long)G.z * ms * ms;
+  const int i1 = (ii - 1) * ns, j1 = (jj - 1) * ns;
+  const int a = a0 + tx;
+  if (a >= ns) return;
+  for (int m = ty; m < 32; m += 8) {
+    const int b = b0 + m;
+    if (b >= ns) break;
+    const double yab = Td[m][tx], yba = Tt[tx][m];
+    double* p = h + (i1 + a) + (long long)ms * (j1 + b);
+    if (ich > jch) {
+      *p += yab;
+    } else if (jch > ich) {
+      *p += yba;
+    } else if (i1 == j1) {
+      double v = *p;
+      v = v + yab / 2.0;
+      v = v + yba / 2.0;
+      *p = v;
+    } else {
+      *p += yab / 2.0;
+      h[(j1 + a) + (long long)ms * (i1 + b)] += yba / 2.0;
+    }
+  }
+}
+
 }  // namespace dbs
 
 using namespace dbs;
@@ -905,7 +961,7 @@ int dbs_rdata_34_dev(dbs_ctx* ctx, int itype, int atype, int k, int nc, const do
   const int ntx = (ns + CV_TX - 1) / CV_TX, nty = (ns + CV_TY - 1) / CV_TY;
   if (sym == 1 && ks <= CV_KS && (long long)ntx * nty * ((ng + 3) / 4) > (1ll << 30))
     return fail(DBS_ERR_ARG, "R_data: too many convolution tiles");
-  if (sym == 1 && (long long)ng * ((ns + DX_T - 1) / DX_T) * ((ns + DX_T - 1) / DX_T) > (1ll << 31) - 1)
+  if (sym == 1 && (long long)ng * ((ns + 31) / 32) * ((ns + 31) / 32) > (1ll << 31) - 1)
     return fail(DBS_ERR_ARG, "R_data: too many density tiles");
   // case 1 (itype 3): one matrix product against the permuted array;
   // case 3 (itype 4): the array itself
@@ -976,14 +1032,29 @@ int dbs_rdata_34_dev(dbs_ctx* ctx, int itype, int atype, int k, int nc, const do
   std::sort(blks.begin(), blks.end());
   const bool shared = std::adjacent_find(blks.begin(), blks.end()) != blks.end();
   const long long nupd = (long long)ns * ns * ng;
+  bool any_diag = false;
+  for (const int4& G : ginfo) any_diag = any_diag || G.x == G.y;
+  const int npass = any_diag ? 2 : 1;  // pass 1 only adds for ich == jch
+  const int nt32 = (ns + 31) / 32;
   if (!shared) {
-    for (int pass = 0; pass < 2; ++pass)
-      k_update_hs_mat<<<grid_of(nupd, 256), 256, 0, s>>>(ng, d_gi, ns, ks, sym, ip1, ip2, pass, d_a, hch_dev);
+    if (sym == 1) {
+      k_update_hs_x<<<(unsigned)((long long)ng * nt32 * nt32), 256, 0, s>>>(ng, nt32, d_gi, ns, ip1, ip2, d_a,
+                                                                              hch_dev);
+    } else {
+      for (int pass = 0; pass < npass; ++pass)
+        k_update_hs_mat<<<grid_of(nupd, 256), 256, 0, s>>>(ng, d_gi, ns, ks, sym, ip1, ip2, pass, d_a, hch_dev);
+    }
   } else {
-    for (int g = 0; g < ng; ++g)
-      for (int pass = 0; pass < 2; ++pass)
+    for (int g = 0; g < ng; ++g) {
+      if (sym == 1) {
+        k_update_hs_x<<<(unsigned)(nt32 * nt32), 256, 0, s>>>(1, nt32, d_gi + g, ns, ip1, ip2, d_a + per * g,
+                                                               hch_dev);
+        continue;
+      }
+      for (int pass = 0; pass < (ginfo[g].x == ginfo[g].y ? 2 : 1); ++pass)
         k_update_hs_mat<<<grid_of((long long)ns * ns, 256), 256, 0, s>>>(1, d_gi + g, ns, ks, sym, ip1, ip2, pass,
                                                                          d_a + per * g, hch_dev);
+    }
   }
   // the host tables must outlive their copies
   hipError_t e = hipGetLastError();
