@@ -61,7 +61,10 @@ int dbs_ctx_sync(dbs_ctx *ctx);
 /* copies gr(nv,ks), grw(nv,ks), pbsp(nv+1,ks,ksp), qbsp(nv+1,ks,ksq) to host
  * (module DBS_gauss, dbsr_lib_dbs.f90:830-891); any pointer may be NULL */
 int dbs_ctx_tables(dbs_ctx *ctx, double *gr, double *grw, double *pbsp, double *qbsp);
-/* bytes of materialised arrays the context may cache (default 8 GiB) */
+/* bytes of materialised arrays the context may cache (default 8 GiB): the
+ * arrays rkb(ns,ns,kb,kb) of (type, k) and, for Rk_evaluate / Sk_evaluate
+ * and R_data's band convolutions, their K x K permutations (K = ns*kb, the
+ * same size), least recently used evicted first */
 int dbs_ctx_set_cache_limit(dbs_ctx *ctx, size_t bytes);
 /* bytes of device workspace the channel post-processing (DBS_ORTH,
  * Diag_channels, Record_matrix) takes per batch of blocks (default 256 MiB);
