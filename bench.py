@@ -708,8 +708,11 @@ def run_config6(args):
     DM = torch.zeros(ncsf, dtype=torch.float64, device="cuda")
 
     def sweep():
+        # lists queued without waiting: the host builds the next list's
+        # tables while the device runs the current one
         for it, k, lst in lists:
-            rk_evaluate(d, it, k, lst, pq_dev, norb, HM, ncsf, ncsf, DM)
+            rk_evaluate(d, it, k, lst, pq_dev, norb, HM, ncsf, ncsf, DM, sync=False)
+        d.sync()
     sweep()  # warm: builds the arrays
     torch.cuda.synchronize()
     hm = HM.cpu().numpy().T.copy()

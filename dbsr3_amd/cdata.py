@@ -92,20 +92,27 @@ def smu(ka, kb, kc, kd, L, v):
     return s.value, S
 
 
-def rk_evaluate(dbs, itype, k, lst, pq_dev, norb, HM_dev, ldhm, nzero, DM_dev, eps_c=1e-10, ibi=2 ** 15):
+def rk_evaluate(dbs, itype, k, lst, pq_dev, norb, HM_dev, ldhm, nzero, DM_dev, eps_c=1e-10, ibi=2 ** 15,
+                sync=True):
     """Rk_evaluate/Sk_evaluate of one merged list (c, K1..K4) on the GPU.
-    pq_dev, HM_dev, DM_dev: torch float64 device tensors (Fortran layouts)."""
+    pq_dev, HM_dev, DM_dev: torch float64 device tensors (Fortran layouts).
+    sync=False queues the list on the context's stream and returns at once
+    (the next list's tables are built while the device runs this one); then
+    nothing may touch HM/DM from another stream before dbs.sync() (or a
+    device-wide synchronize)."""
     import torch
     c, K1, K2, K3, K4 = lst
     c = np.ascontiguousarray(c, dtype=np.float64)
     K = [_i32(x) for x in (K1, K2, K3, K4)]
-    torch.cuda.synchronize()
+    if sync:
+        torch.cuda.synchronize()
     dm = C.c_void_p(DM_dev.data_ptr()) if DM_dev is not None else None
     check(lib().dbs_rk_evaluate_dev(dbs.ctx, itype, k, len(c), c.ctypes.data_as(_dp),
                                     *[x.ctypes.data_as(_ip) for x in K], eps_c, ibi,
                                     C.c_void_p(pq_dev.data_ptr()), norb,
                                     C.c_void_p(HM_dev.data_ptr()), ldhm, nzero, dm))
-    check(lib().dbs_ctx_sync(dbs.ctx))
+    if sync:
+        check(lib().dbs_ctx_sync(dbs.ctx))
 
 
 def r_data_34(dbs, itype, atype, k, lst, pq_dev, norb, icc, hch_dev, nblocks, ibo=2 ** 15):

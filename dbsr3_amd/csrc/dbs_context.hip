@@ -245,6 +245,10 @@ int dbs_ctx_destroy(dbs_ctx* c) {
   for (double* p : bufs)
     if (p) (void)hipFree(p);
   for (auto& kv : c->rkb_cache) (void)hipFree(kv.second);
+  for (int b = 0; b < 2; ++b) {
+    if (c->list_host[b]) (void)hipHostFree(c->list_host[b]);
+    if (c->list_ev[b]) (void)hipEventDestroy(c->list_ev[b]);
+  }
   if (c->solver && c->solver_free) c->solver_free(c->solver);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;

@@ -1696,7 +1696,7 @@ __global__ void k_plain_rows(int nrows, const int2* __restrict__ lr, int ns, int
         if (band_ok(sym, ns, ks, i, jp)) s += a[i + (long long)ns * jp] * b[i + (long long)ns * jp];
     s = wave_sum(s);
     if (lane == 0) {
-      double* o = vals + row_out[w];
+      double* o = vals + (row_out ? row_out[w] : w);  // no table: row w to vals[w]
       *o = accumulate ? *o + s : s;
     }
   }
@@ -3519,37 +3519,59 @@ int dbs_rk_evaluate_dev(dbs_ctx* ctx, int itype, int k, int nc, const double* c,
   for (int t = 0; t < nt; ++t) toff[t + 1] += toff[t];
   std::vector<int> tflat(n), fill(toff.begin(), toff.end() - 1);
   for (int q = 0; q < n; ++q) tflat[fill[tid[q]]++] = q;
-  std::vector<long long> rowout(n);
-  for (int q = 0; q < n; ++q) rowout[q] = q;
   double* rkm = nullptr;
   int rc = build_rkm(ctx, itype, k, &rkm);
   if (rc) return rc;
   // scratch (doubles): densities | convolutions | values | then the host
-  // tables (coefficients, spec, row pairs, row outputs, targets)
+  // tables (coefficients, spec, row pairs, targets), 16-byte aligned, staged
+  // in one of the context's two pinned buffers and uploaded in one copy
   const size_t per = (size_t)ns * nb;
   const size_t o_conv = spec.size() * per, o_vals = o_conv + (size_t)nl * per, o_tab = o_vals + n;
-  std::vector<char> blob;
-  auto place = [&](const void* src, size_t bytes) {
-    const size_t off = (blob.size() + 15) & ~(size_t)15;
-    blob.resize(off + bytes + 16);
-    if (bytes) std::memcpy(blob.data() + off, src, bytes);
+  size_t nblob = 0;
+  auto slot = [&](size_t bytes) {
+    const size_t off = (nblob + 15) & ~(size_t)15;
+    nblob = off + bytes;
     return off;
   };
-  const size_t b_c = place(ck.data(), ck.size() * sizeof(double));
-  const size_t b_spec = place(spec.data(), spec.size() * sizeof(int4));
-  const size_t b_lr = place(lr.data(), lr.size() * sizeof(int2));
-  const size_t b_out = place(rowout.data(), rowout.size() * sizeof(long long));
-  const size_t b_tpos = place(tpos.data(), tpos.size() * sizeof(long long));
-  const size_t b_toff = place(toff.data(), toff.size() * sizeof(int));
-  const size_t b_tent = place(tflat.data(), tflat.size() * sizeof(int));
-  if ((rc = ensure_scratch(ctx, o_tab + (blob.size() + 7) / 8 + 2))) return rc;
+  const size_t b_c = slot(ck.size() * sizeof(double));
+  const size_t b_spec = slot(spec.size() * sizeof(int4));
+  const size_t b_lr = slot(lr.size() * sizeof(int2));
+  const size_t b_tpos = slot(tpos.size() * sizeof(long long));
+  const size_t b_toff = slot(toff.size() * sizeof(int));
+  const size_t b_tent = slot(tflat.size() * sizeof(int));
+  nblob = (nblob + 15) & ~(size_t)15;
+  if ((rc = ensure_scratch(ctx, o_tab + nblob / 8 + 2))) return rc;
+  const int hb = ctx->list_cur;
+  if (!ctx->list_ev[hb]) DBS_HIP(hipEventCreateWithFlags(&ctx->list_ev[hb], hipEventDisableTiming));
+  // the upload that last read this buffer has finished
+  DBS_HIP(hipEventSynchronize(ctx->list_ev[hb]));
+  if (ctx->list_host_n[hb] < nblob) {
+    if (ctx->list_host[hb]) DBS_HIP(hipHostFree(ctx->list_host[hb]));
+    ctx->list_host[hb] = nullptr;
+    ctx->list_host_n[hb] = 0;
+    const size_t cap = std::max(nblob, (size_t)1 << 20);
+    DBS_HIP(hipHostMalloc(reinterpret_cast<void**>(&ctx->list_host[hb]), cap, hipHostMallocDefault));
+    ctx->list_host_n[hb] = cap;
+  }
+  char* blob = ctx->list_host[hb];
+  auto put = [&](size_t off, const void* src, size_t bytes) {
+    if (bytes) std::memcpy(blob + off, src, bytes);
+  };
+  put(b_c, ck.data(), ck.size() * sizeof(double));
+  put(b_spec, spec.data(), spec.size() * sizeof(int4));
+  put(b_lr, lr.data(), lr.size() * sizeof(int2));
+  put(b_tpos, tpos.data(), tpos.size() * sizeof(long long));
+  put(b_toff, toff.data(), toff.size() * sizeof(int));
+  put(b_tent, tflat.data(), tflat.size() * sizeof(int));
   char* base = reinterpret_cast<char*>(ctx->scratch + o_tab);
   base = reinterpret_cast<char*>(((uintptr_t)base + 15) & ~(uintptr_t)15);
   hipStream_t s = ctx->stream;
   double* dens = ctx->scratch;
   double* conv = ctx->scratch + o_conv;
   double* vals = ctx->scratch + o_vals;
-  hipError_t e = hipMemcpyAsync(base, blob.data(), blob.size(), hipMemcpyHostToDevice, s);
+  hipError_t e = hipMemcpyAsync(base, blob, nblob, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipEventRecord(ctx->list_ev[hb], s);
+  ctx->list_cur ^= 1;
   if (e == hipSuccess) {
     const int nsp = (int)spec.size();
     k_plain_dens<<<grid_for((long long)nsp * per, 256, 8192), 256, 0, s>>>(
@@ -3559,23 +3581,16 @@ int dbs_rk_evaluate_dev(dbs_ctx* ctx, int itype, int k, int nc, const double* c,
   if (e == hipSuccess) {
     // conv(:, l) = rkm^T dens(:, l) for the left densities 0..nl-1
     rc = conv_gemm(ctx, 2, (int)per, nl, rkm, dens, conv);
-    if (rc) {
-      (void)hipStreamSynchronize(s);
-      return rc;
-    }
+    if (rc) return rc;
     k_plain_rows<<<grid_for(n, 4, 65535), 256, 0, s>>>(n, reinterpret_cast<const int2*>(base + b_lr), ns, ks,
-                                                         sym, conv, dens,
-                                                         reinterpret_cast<const long long*>(base + b_out), 0,
-                                                         vals);
+                                                         sym, conv, dens, nullptr, 0, vals);
     k_list_accumulate<<<grid_for(nt, 256, 65535), 256, 0, s>>>(
         nt, reinterpret_cast<const long long*>(base + b_tpos), reinterpret_cast<const int*>(base + b_toff),
         reinterpret_cast<const int*>(base + b_tent), vals, reinterpret_cast<const double*>(base + b_c), HM_dev,
         DM_dev);
     e = hipGetLastError();
   }
-  // the host tables must outlive their copy
-  const hipError_t e2 = hipStreamSynchronize(s);
-  if (e == hipSuccess) e = e2;
+  // queued, not waited for: the pinned tables belong to the context
   if (e != hipSuccess) return fail(DBS_ERR_HIP, std::string("Rk_evaluate: ") + hipGetErrorString(e));
   return DBS_OK;
 }

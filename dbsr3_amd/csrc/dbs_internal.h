@@ -99,6 +99,14 @@ struct dbs_ctx {
   // staging area of the host-pointer entry points
   double* stage = nullptr;
   size_t stage_n = 0;
+  // pinned host tables of the list calls (Rk_evaluate): two buffers used in
+  // turn, each reusable once the event recorded after its upload has fired,
+  // so a call returns without waiting and the host builds the next list's
+  // tables while the device runs this one
+  char* list_host[2] = {nullptr, nullptr};
+  size_t list_host_n[2] = {0, 0};
+  hipEvent_t list_ev[2] = {nullptr, nullptr};
+  int list_cur = 0;
 };
 
 namespace dbs {

@@ -78,6 +78,10 @@ class DBS:
         post-processing (default 256 MiB)"""
         check(lib().dbs_ctx_set_workspace_limit(self.ctx, int(nbytes)))
 
+    def sync(self):
+        """wait for the work queued on the context's stream"""
+        check(lib().dbs_ctx_sync(self.ctx))
+
     # -- plumbing --------------------------------------------------------
     def _pre(self):
         _torch().cuda.synchronize()

@@ -171,7 +171,7 @@
       Integer, intent(in) :: k, itype
       Integer(c_size_t) :: bh, bd
       if (nc_cur <= 0) Return
-      bh = int(8 * ncj * nzero, c_size_t); bd = int(8 * ncj, c_size_t)
+      bh = 8_c_size_t * ncj * nzero; bd = 8_c_size_t * ncj
       Call dbs_gpu_check(dbs_memcpy_h2d(ctx, hm_dev, hm_host, bh))
       Call dbs_gpu_check(dbs_memcpy_h2d(ctx, dm_dev, dm_host, bd))
       Call dbs_gpu_check(dbs_rk_evaluate_dev(ctx, itype, k, nc_cur, c_cur, k1_cur, k2_cur, k3_cur, k4_cur, &

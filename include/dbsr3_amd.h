@@ -196,7 +196,11 @@ int dbs_decode_int(int n, const int *code, int *met, int *k, int *i1, int *i2, i
  * host arrays) of type itype (1..6) and multipole k on the device and add
  * S*c to HM(ic,jc) (jc <= nzero) or DM(jc) (ic == jc > nzero).  Entries
  * with |c| < eps_c are skipped.  Contributions to one element are added in
- * list order, as the reference does.  pq_dev(ns,2,norb); HM_dev(ldhm,*). */
+ * list order, as the reference does.  pq_dev(ns,2,norb); HM_dev(ldhm,*).
+ * The work is queued on the context stream and the call returns without
+ * waiting (the host arrays may be reused at once: the tables go through
+ * the context's pinned buffers); read HM/DM after dbs_ctx_sync or with
+ * the context's own copies (dbs_memcpy_d2h). */
 int dbs_rk_evaluate_dev(dbs_ctx *ctx, int itype, int k, int nc, const double *c, const int *K1,
                         const int *K2, const int *K3, const int *K4, double eps_c, int ibi,
                         const double *pq_dev, int norb, double *HM_dev, int ldhm, int nzero,

@@ -65,6 +65,19 @@ def test_rk_evaluate(setup):
     assert s > 0
     assert np.max(np.abs(HM - HM_ref)) <= TOL * s
     assert np.max(np.abs(DM - DM_ref)) <= TOL * s
+    # every list queued without waiting (the tables of consecutive lists in
+    # the context's two pinned buffers in turn), one wait at the end: the
+    # same sums in the same order
+    HM_q, DM_q = fortran_to_device(HM0), fortran_to_device(DM0)
+    nq = 0
+    for it, k, lst in lists:
+        if len(lst[0]) == 0:
+            continue
+        rk_evaluate(d, it, k, lst, pq_dev, norb, HM_q, ncj, nzero, DM_q, sync=False)
+        nq += 1
+    assert nq >= 3
+    d.sync()
+    assert np.array_equal(device_to_fortran(HM_q), HM) and np.array_equal(DM_q.cpu().numpy(), DM)
     torch.cuda.synchronize()
 
 
