@@ -744,8 +744,22 @@ def run_config6(args):
     workload = {"workload": "CI3 Add_integral (with Breit) of 2e5 seeded terms, 30 orbitals, 400 CSFs, ns=200, "
                             "k=0..4; every merged list by Rk_evaluate/Sk_evaluate into HM/DM; one step = every list",
                 "terms": nterm, "merged_entries": int(nl), "lists": len(lists), "ns": ns, "csfs": ncsf}
+    # algorithmic work of a sweep: per list the case-2 products of its
+    # distinct left densities, 2 K^2 flops each (K = ns*kb: kb = ks for R^k,
+    # 2ks-1 for S^k), and SUM_AmB, 2 K flops per entry
+    flops = 0.0
+    for it, k, lst in lists:
+        cl = np.asarray(lst[0])
+        keep = np.abs(cl) >= 1e-10
+        K = ns * (ks if it <= 4 else 2 * ks - 1)
+        flops += 2.0 * K * K * len(np.unique(np.asarray(lst[1])[keep])) + 2.0 * K * int(keep.sum())
+    ach = flops / (ms * 1e-3) / 1e12
+    roof = {"bound": "mfma", "achieved": ach, "peak": 78.6, "unit": "TFLOP/s", "frac": ach / 78.6,
+            "traffic": None, "phase": "sweep", "flops_per_sweep": flops,
+            "note": "whole sweep over the host-timed step (lists queued without waiting; the host's table "
+                    "building is the longer side): the case-2 products (rocBLAS dgemm) and SUM_AmB"}
     return config_line(6, value, "list entries/s", ms, args.steps, 1, workload,
-                       {"cpu_baseline": cpu, "max_rel_err_sampled_lists": err,
+                       {"roofline": roof, "cpu_baseline": cpu, "max_rel_err_sampled_lists": err,
                         "hm_norm": float(np.abs(hm).sum())})
 
 
