@@ -485,9 +485,12 @@ def run_config2(args):
     nl = sum(len(l[2][0]) for l in lists)
 
     def sweep():
+        # calls queued without waiting where the path allows it (itype 4,
+        # itype 3 without the asymptotic coefficients); one wait at the end
         for it, kk, lst in lists:
             for atype in range(4):
-                m.R_data(it, kk, atype, lst, pq_dev, nb, ibo=ibo, quadr_pq=qpq)
+                m.R_data(it, kk, atype, lst, pq_dev, nb, ibo=ibo, quadr_pq=qpq, sync=False)
+        d.sync()
 
     sweep()  # warm: builds and caches the R^k arrays; blocks checked below
     torch.cuda.synchronize()

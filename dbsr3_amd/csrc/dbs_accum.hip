@@ -986,9 +986,11 @@ int dbs_rdata_34_dev(dbs_ctx* ctx, int itype, int atype, int k, int nc, const do
   int* d_j1 = d_goff + goff.size();
   int* d_j2 = d_j1 + ej1.size();
   int4* d_gi = reinterpret_cast<int4*>(ctx->scratch + ((off_g + 1) & ~(size_t)1));
-  // the tables in the scratch layout from off_c on, one upload
-  std::vector<double> blob(need - off_c, 0.0);
-  char* hb = reinterpret_cast<char*>(blob.data());
+  // the tables in the scratch layout from off_c on, one upload from the
+  // context's pinned list buffers (the call does not wait for it)
+  const size_t nblob = (need - off_c) * sizeof(double);
+  char* hb = nullptr;
+  if ((rc = list_stage_begin(ctx, nblob, &hb))) return rc;
   auto stage = [&](const void* dev, const void* src, size_t bytes) {
     if (bytes) std::memcpy(hb + (reinterpret_cast<const char*>(dev) - reinterpret_cast<const char*>(d_c)), src, bytes);
   };
@@ -997,10 +999,8 @@ int dbs_rdata_34_dev(dbs_ctx* ctx, int itype, int atype, int k, int nc, const do
   stage(d_j1, ej1.data(), ej1.size() * sizeof(int));
   stage(d_j2, ej2.data(), ej2.size() * sizeof(int));
   stage(d_gi, ginfo.data(), ginfo.size() * sizeof(int4));
-  if (hipMemcpyAsync(d_c, blob.data(), blob.size() * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess) {
-    (void)hipStreamSynchronize(s);  // blob is a pageable host buffer that ends with this call
-    return fail(DBS_ERR_HIP, "R_data: upload");
-  }
+  if (hipMemcpyAsync(d_c, hb, nblob, hipMemcpyHostToDevice, s) != hipSuccess) return fail(DBS_ERR_HIP, "R_data: upload");
+  if ((rc = list_stage_end(ctx))) return rc;
   if (sym == 0) {
     k_group_density<<<grid_of(per * ng, 256), 256, 0, s>>>(ng, d_goff, d_j1, d_j2, d_c, pq_dev, ns, ks, jp1, jp2,
                                                             sym, d_xx);
@@ -1010,10 +1010,7 @@ int dbs_rdata_34_dev(dbs_ctx* ctx, int itype, int atype, int k, int nc, const do
                                                                           ns, jp1, jp2, d_xx);
   }
   if (sym == 0) {
-    if ((rc = conv_gemm(ctx, 1, ns * ks, ng, rkb, d_xx, d_a))) {
-      (void)hipStreamSynchronize(s);
-      return rc;
-    }
+    if ((rc = conv_gemm(ctx, 1, ns * ks, ng, rkb, d_xx, d_a))) return rc;
     k_band_zero<<<grid_of(per * ng, 256), 256, 0, s>>>(ng, ns, ks, d_a);
   } else if (ks <= CV_KS) {
     constexpr int GB = 4;  // 37 KB of LDS, four workgroups per CU (measured best of 2, 4, 8, 16)
@@ -1056,9 +1053,8 @@ int dbs_rdata_34_dev(dbs_ctx* ctx, int itype, int atype, int k, int nc, const do
                                                                          d_a + per * g, hch_dev);
     }
   }
-  // the host tables must outlive their copies
+  // queued, not waited for: the pinned tables belong to the context
   hipError_t e = hipGetLastError();
-  if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) rc = fail(DBS_ERR_HIP, std::string("R_data: ") + hipGetErrorString(e));
   return rc;
 }

@@ -11,6 +11,8 @@
 #include <mutex>
 
 #include "dbs_device.h"
+#include <algorithm>
+
 #include "dbs_internal.h"
 
 namespace dbs {
@@ -35,6 +37,29 @@ int ensure_scratch(dbs_ctx* c, size_t n) {
   c->scratch_n = 0;
   DBS_HIP(hipMalloc(&c->scratch, n * sizeof(double)));
   c->scratch_n = n;
+  return DBS_OK;
+}
+
+int list_stage_begin(dbs_ctx* c, size_t bytes, char** host) {
+  const int hb = c->list_cur;
+  if (!c->list_ev[hb]) DBS_HIP(hipEventCreateWithFlags(&c->list_ev[hb], hipEventDisableTiming));
+  DBS_HIP(hipEventSynchronize(c->list_ev[hb]));
+  if (c->list_host_n[hb] < bytes) {
+    if (c->list_host[hb]) DBS_HIP(hipHostFree(c->list_host[hb]));
+    c->list_host[hb] = nullptr;
+    c->list_host_n[hb] = 0;
+    const size_t cap = std::max(bytes, (size_t)1 << 20);
+    DBS_HIP(hipHostMalloc(reinterpret_cast<void**>(&c->list_host[hb]), cap, hipHostMallocDefault));
+    c->list_host_n[hb] = cap;
+  }
+  *host = c->list_host[hb];
+  return DBS_OK;
+}
+
+int list_stage_end(dbs_ctx* c) {
+  const int hb = c->list_cur;
+  c->list_cur ^= 1;
+  DBS_HIP(hipEventRecord(c->list_ev[hb], c->stream));
   return DBS_OK;
 }
 

@@ -3541,19 +3541,8 @@ int dbs_rk_evaluate_dev(dbs_ctx* ctx, int itype, int k, int nc, const double* c,
   const size_t b_tent = slot(tflat.size() * sizeof(int));
   nblob = (nblob + 15) & ~(size_t)15;
   if ((rc = ensure_scratch(ctx, o_tab + nblob / 8 + 2))) return rc;
-  const int hb = ctx->list_cur;
-  if (!ctx->list_ev[hb]) DBS_HIP(hipEventCreateWithFlags(&ctx->list_ev[hb], hipEventDisableTiming));
-  // the upload that last read this buffer has finished
-  DBS_HIP(hipEventSynchronize(ctx->list_ev[hb]));
-  if (ctx->list_host_n[hb] < nblob) {
-    if (ctx->list_host[hb]) DBS_HIP(hipHostFree(ctx->list_host[hb]));
-    ctx->list_host[hb] = nullptr;
-    ctx->list_host_n[hb] = 0;
-    const size_t cap = std::max(nblob, (size_t)1 << 20);
-    DBS_HIP(hipHostMalloc(reinterpret_cast<void**>(&ctx->list_host[hb]), cap, hipHostMallocDefault));
-    ctx->list_host_n[hb] = cap;
-  }
-  char* blob = ctx->list_host[hb];
+  char* blob = nullptr;
+  if ((rc = list_stage_begin(ctx, nblob, &blob))) return rc;
   auto put = [&](size_t off, const void* src, size_t bytes) {
     if (bytes) std::memcpy(blob + off, src, bytes);
   };
@@ -3570,8 +3559,7 @@ int dbs_rk_evaluate_dev(dbs_ctx* ctx, int itype, int k, int nc, const double* c,
   double* conv = ctx->scratch + o_conv;
   double* vals = ctx->scratch + o_vals;
   hipError_t e = hipMemcpyAsync(base, blob, nblob, hipMemcpyHostToDevice, s);
-  if (e == hipSuccess) e = hipEventRecord(ctx->list_ev[hb], s);
-  ctx->list_cur ^= 1;
+  if (e == hipSuccess && (rc = list_stage_end(ctx))) return rc;
   if (e == hipSuccess) {
     const int nsp = (int)spec.size();
     k_plain_dens<<<grid_for((long long)nsp * per, 256, 8192), 256, 0, s>>>(

@@ -117,6 +117,12 @@ int build_rkb(dbs_ctx* c, int type, int k, double** out);
 // rkb permuted to the K x K matrix rkm(i + ns*ip, j + ns*jp), K = ns*kb,
 // cached with the arrays (negative keys in rkb_cache, one LRU and limit)
 int build_rkm(dbs_ctx* c, int type, int k, double** out);
+// pinned host tables of the queued list calls (Rk_evaluate, R_data): the
+// context's next buffer, grown to at least `bytes`, once the upload that
+// last read it has finished; after queueing the upload from it on the
+// context stream, list_stage_end records that and moves to the other one
+int list_stage_begin(dbs_ctx* c, size_t bytes, char** host);
+int list_stage_end(dbs_ctx* c);
 // convol of n densities (K x n, one after another) against one array on the
 // context stream (rocBLAS dgemm): case 1 conv = rkm dens, case 2
 // conv = rkm^T dens (every output of the K x n result written; the entries

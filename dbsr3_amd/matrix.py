@@ -187,27 +187,32 @@ class MatBlocks:
                         htarg.ctypes.data_as(_dp) if htarg is not None else None, qb, quser)
 
     def _rs(self, name, tkey, k, lst, pq, norb, ibo, ipbs, kbort, vch, fppqq, iitar, iptar, htarg,
-            quadr_pq=None):
+            quadr_pq=None, sync=True):
         c, K1, K2, K3, K4 = lst
         c = np.ascontiguousarray(c, dtype=np.float64)
         K = [_i32(x) for x in (K1, K2, K3, K4)]
         aux = self._aux(ibo, ipbs, kbort, vch, fppqq, iitar, iptar, htarg, quadr_pq)
-        self._pre()
+        if sync:
+            self._pre()
         check(getattr(lib(), name)(self.dbs.ctx, C.byref(self._desc), C.byref(aux), *tkey, k, len(c),
                                    c.ctypes.data_as(_dp), *[x.ctypes.data_as(_ip) for x in K], _vp(pq), norb))
-        self._post()
+        if sync:
+            self._post()
 
     def R_data(self, itype, kpol, btype, lst, pq, norb, ibo=2 ** 15, ipbs=None, kbort=None, vch=None,
-               fppqq=None, iitar=0, iptar=None, htarg=None, quadr_pq=None):
+               fppqq=None, iitar=0, iptar=None, htarg=None, quadr_pq=None, sync=True):
         """R_data(itype, kpol, btype) of one merged list lst = (c, K1, K2, K3, K4)
         in IPT order; pq: device tensor holding pq(ns,2,norb); vch: device
         tensor (nvch, ms) holding V_ch(ms,nvch); ipbs/iptar 1-based tables
         (element i-1 for orbital/channel i); quadr_pq: the host program's
         QUADR_PQ (needed by itype 3, btype 0/1 for the asymptotic
         coefficients acf), a callable quadr_pq(i1, i2, m) or a table
-        q(norb,norb,0:mmax) looked up by the library."""
+        q(norb,norb,0:mmax) looked up by the library.  sync=False leaves
+        the call's device work queued (itype 3/4 return without waiting):
+        nothing may touch the matrices from another stream before
+        dbs.sync()."""
         self._rs("dbs_rdata_dev", (itype, btype), kpol, lst, pq, norb, ibo, ipbs, kbort, vch, fppqq, iitar,
-                 iptar, htarg, quadr_pq)
+                 iptar, htarg, quadr_pq, sync)
 
     def S_data(self, ktype, kpol, lst, pq, norb, ibo=2 ** 15, ipbs=None, kbort=None, vch=None,
                fppqq=None, iitar=0, iptar=None, htarg=None):

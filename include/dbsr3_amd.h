@@ -210,7 +210,10 @@ int dbs_rk_evaluate_dev(dbs_ctx *ctx, int itype, int k, int nc, const double *c,
  * c-weighted density of orbitals (K2,K3) is convolved (case 1 's' or case
  * 3 'x') and added by UPDATE_HS (DBSR_MAT3/updates.f90:110-179) to block
  * icc(ich,jch) of hch_dev(ms,ms,nblocks); icc is (nch,nch) host, 1-based
- * block numbers, 0 = not stored.  UPDATE_ACF terms are not produced. */
+ * block numbers, 0 = not stored.  UPDATE_ACF terms are not produced.
+ * Queued on the context stream, returns without waiting (tables through
+ * the context's pinned buffers): read hch after dbs_ctx_sync or through
+ * the context's copies. */
 int dbs_rdata_34_dev(dbs_ctx *ctx, int itype, int atype, int k, int nc, const double *c,
                      const int *K1, const int *K2, const int *K3, int ibo, const double *pq_dev,
                      int norb, const int *icc, int nch, double *hch_dev, int nblocks);

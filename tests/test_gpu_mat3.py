@@ -109,6 +109,19 @@ def _check(stype, itype, atype, spec, seed, k=2, n=120):
         assert np.max(np.abs(acf_ref)) > 0
     acf_gpu = m.acf.cpu().numpy().T
     assert np.array_equal(acf_gpu, acf_ref), "acf"
+    if stype == 0 and itype >= 3:
+        # the same list queued three times without waiting (the tables in
+        # the context's two pinned buffers in turn), one wait at the end:
+        # three times the waited-for call's contribution, bit for bit
+        m2 = MatBlocks(d, NCH, NPERT)
+        ref2 = MatBlocks(d, NCH, NPERT)
+        for _ in range(3):
+            m2.R_data(itype, k, atype, lst, dev["pq"], NORB, quadr_pq=lambda i1, i2, mm: qpq[i1 - 1, i2 - 1],
+                      sync=False, **kw)
+            ref2.R_data(itype, k, atype, lst, dev["pq"], NORB, quadr_pq=lambda i1, i2, mm: qpq[i1 - 1, i2 - 1],
+                        **kw)
+        d.sync()
+        assert np.array_equal(_hch(m2), _hch(ref2)) and np.array_equal(m2.acf.cpu().numpy(), ref2.acf.cpu().numpy())
 
 
 @pytest.mark.parametrize("itype", [1, 2, 3, 4])
