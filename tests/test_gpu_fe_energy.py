@@ -7,8 +7,7 @@ average energy, -1271.392047659758646 au) and :418-422 (the configuration
 energies); the device matrices and integrals are checked against the oracle,
 which reproduces those energies to 2e-16 (tests/test_oracle_fe.py).
 
-Tolerance: 1e-12 relative on every energy (the north_star's per-result
-bound).  The engine sums each integral's quadrature in another order than
+Tolerance: 1e-12 relative on every energy (the path's per-result bound).  The engine sums each integral's quadrature in another order than
 hf_fk / hf_gk (G^k's P/Q cross term as qpqp + pqpq rather than 2 qpqp),
 and <i|h_D|i> is a device matrix product; each differs from the oracle by a
 few 1e-16 of its magnitude.
