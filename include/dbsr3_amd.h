@@ -51,7 +51,8 @@ int dbs_version(void);
  * alloc_DBS_gauss + Def_pbsp/Def_qbsp (DBSR_HF/dbsr_lib_dbs.f90:480-596,
  * 911-1074): knots t(1:ns+ks), orders ks/ksp/ksq (ksp<=0 -> ks-1,
  * ksq<=0 -> ks, as :565-566).  Gauss tables and B-spline values are built
- * on the device and stay resident.
+ * on the device and stay resident.  A context has one stream, one scratch
+ * and two pinned list buffers: one host thread at a time may call into it.
  * ------------------------------------------------------------------- */
 int dbs_ctx_create(dbs_ctx **ctx, int ns, int ks, int ksp, int ksq, const double *t, int device);
 int dbs_ctx_destroy(dbs_ctx *ctx);
