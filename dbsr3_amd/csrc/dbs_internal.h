@@ -123,6 +123,10 @@ int build_rkm(dbs_ctx* c, int type, int k, double** out);
 // context stream, list_stage_end records that and moves to the other one
 int list_stage_begin(dbs_ctx* c, size_t bytes, char** host);
 int list_stage_end(dbs_ctx* c);
+// UPDATE_ACF (updates.f90) of a list, queued on the context stream without
+// waiting (dbs_update_acf_dev waits)
+int update_acf_queued(dbs_ctx* ctx, const dbs_mat_blocks* m, int n, const int* k, const int* ich, const int* jch,
+                      const double* C);
 // convol of n densities (K x n, one after another) against one array on the
 // context stream (rocBLAS dgemm): case 1 conv = rkm dens, case 2
 // conv = rkm^T dens (every output of the K x n result written; the entries

@@ -1179,7 +1179,7 @@ int rs_data(dbs_ctx* ctx, const dbs_mat_blocks* m, const dbs_mat3_aux* aux, int 
     rc = dbs_rdata_34_dev(ctx, itype, atype, k, nc, c, K1, K2, K3, aux->ibo, pq_dev, norb, m->icc, m->nch,
                           m->hch_dev, m->nblocks);
     if (rc || !acf) return rc;
-    return dbs_update_acf_dev(ctx, m, (int)aC.size(), ak.data(), aich.data(), ajch.data(), aC.data());
+    return update_acf_queued(ctx, m, (int)aC.size(), ak.data(), aich.data(), ajch.data(), aC.data());
   }
   if (itype == 1) return structure1(ctx, m, X, P, stype, k, nc, c, K1, K2, K3, K4, pq_dev, norb);
   if (itype <= 5) return structure2(ctx, m, X, P, stype, k, nc, c, K1, K2, K3, K4, pq_dev, norb);

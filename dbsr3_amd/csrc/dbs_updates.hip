@@ -16,6 +16,7 @@
 // so every result is bit-identical to it.  The updates of one matrix element
 // are applied by one thread in list order: no atomics, no reassociation.
 #include <algorithm>
+#include <cstring>
 #include <cmath>
 #include <string>
 #include <vector>
@@ -321,6 +322,50 @@ int check_blocks(const dbs_mat_blocks* m) {
   DBS_CHECK(m, DBS_ERR_ARG, "matrix storage: null descriptor");
   DBS_CHECK(m->nch >= 0 && m->npert >= 0 && m->nblocks >= 0 && m->ncb >= 0 && m->nbb >= 0, DBS_ERR_ARG,
             "matrix storage: negative dimension");
+  return DBS_OK;
+}
+
+// UPDATE_ACF of a list queued on the context stream without waiting: the
+// tables (targets ascending, their entries in list order, values) go
+// through the context's pinned list buffers into the scratch, which the
+// stream order keeps behind the kernels that used the scratch before
+int update_acf_queued(dbs_ctx* ctx, const dbs_mat_blocks* m, int n, const int* k, const int* ich, const int* jch,
+                      const double* C) {
+  DBS_CHECK(ctx && (n <= 0 || (k && ich && jch && C)), DBS_ERR_ARG, "Update_cf: null argument");
+  int rc = check_blocks(m);
+  if (rc || n <= 0) return rc;
+  std::vector<long long> t(n);
+  for (int e = 0; e < n; ++e) {
+    if (k[e] > m->mk) return fail(DBS_ERR_ARG, "Update_cf: k > kmk ");
+    if (k[e] < 0) return fail(DBS_ERR_ARG, "Update_cf: k < 0");
+    int b = 0;
+    if ((rc = chan_block(m, ich[e], jch[e], "Update_cf", &b))) return rc;
+    t[e] = b == 0 ? -1 : (long long)(b - 1) + (long long)m->nblocks * k[e];
+  }
+  std::vector<int> order, toff;
+  std::vector<long long> tpos;
+  group_by(t, order, tpos, toff);
+  if (tpos.empty()) return DBS_OK;
+  const int nt = (int)tpos.size();
+  const size_t nv = order.size();
+  const size_t b_v = 8 * (size_t)nt, b_toff = b_v + 8 * nv, nbytes = (b_toff + 4 * ((size_t)nt + 1) + 15) & ~(size_t)15;
+  if ((rc = ensure_scratch(ctx, nbytes / 8 + 2))) return rc;
+  char* hb = nullptr;
+  if ((rc = list_stage_begin(ctx, nbytes, &hb))) return rc;
+  std::memcpy(hb, tpos.data(), 8 * (size_t)nt);
+  double* hv = reinterpret_cast<double*>(hb + b_v);
+  for (size_t q = 0; q < nv; ++q) hv[q] = C[order[q]];
+  std::memcpy(hb + b_toff, toff.data(), 4 * ((size_t)nt + 1));
+  char* base = reinterpret_cast<char*>(ctx->scratch);
+  if (hipMemcpyAsync(base, hb, nbytes, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+    return fail(DBS_ERR_HIP, "UPDATE_ACF: upload");
+  if ((rc = list_stage_end(ctx))) return rc;
+  k_accumulate_list<<<grid_n(nt, 256), 256, 0, ctx->stream>>>(nt, reinterpret_cast<const long long*>(base),
+                                                               reinterpret_cast<const int*>(base + b_toff),
+                                                               reinterpret_cast<const double*>(base + b_v),
+                                                               m->acf_dev);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(DBS_ERR_HIP, std::string("UPDATE_ACF: ") + hipGetErrorString(e));
   return DBS_OK;
 }
 
